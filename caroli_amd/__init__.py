@@ -1,0 +1,300 @@
+"""MI355X nested-dissection transmission: the Python face of the engine's C ABI.
+
+Mirrors the API of `NestedTwoTerminal` (crates/caroli-green/src/nested.rs:467-713), its
+`LeadSelfEnergies` (nested.rs:439-459), `Precision`/`PrecisionOptions`/`PrecisionReport`
+(crates/caroli-green/src/precision.rs:24-134) and `GreenError` (crates/caroli-green/src/lib.rs:35-47),
+so that code written against the reference reads the same. `TwoTerminal` (two_terminal.rs:64-193)
+is the same engine under the name the reference's README uses.
+
+All numerical work runs in `libcaroli_amd.so`: the fronts, root and trace on the GPU, the symbolic
+phase and the FP64 lead self-energies on the host.
+"""
+import ctypes as C
+import enum
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _capi as capi
+
+__all__ = [
+    "GreenError", "Precision", "PrecisionOptions", "PrecisionReport", "LeadSelfEnergies", "NestedTwoTerminal",
+    "TwoTerminal", "Model", "device_available", "batch_transmission",
+]
+
+
+class GreenError(RuntimeError):
+    """GreenError of crates/caroli-green/src/lib.rs:35-47, by status code."""
+
+    KINDS = {1: "Singular", 2: "Eigen", 3: "ModeCount", 4: "NoConvergence", 6: "Invalid", 7: "Device"}
+
+    def __init__(self, code, what=""):
+        self.code = int(code)
+        self.kind = self.KINDS.get(self.code, f"status {code}")
+        super().__init__(f"{self.kind}{': ' + what if what else ''}")
+
+
+def _check(rc, what=""):
+    if rc != 0:
+        raise GreenError(rc, what)
+
+
+class Precision(enum.IntEnum):
+    """Front precision (precision.rs:24-32), plus the FP64-refined mixed mode."""
+
+    Double = 0
+    Single = 1
+    Mixed = 2
+
+
+@dataclass
+class PrecisionOptions:
+    """precision.rs:36-61. The engine reports |dT| for every point (`every` = 1)."""
+
+    precision: Precision = Precision.Mixed
+    tolerance: float = 1e-8
+
+
+@dataclass
+class PrecisionReport:
+    """precision.rs:65-78, with the per-point differences."""
+
+    precision: Precision
+    checked: int
+    median: float
+    max: float
+    fell_back: bool
+    differences: np.ndarray
+
+
+def device_available() -> bool:
+    return bool(capi.cg_device_available())
+
+
+def _dptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class Model(enum.IntEnum):
+    """Preset models of the engine's host builders (crates/caroli-model/src/presets.rs,
+    crates/caroli-bench/src/main.rs:77-92)."""
+
+    BHZ = 0
+    Square = 1
+    Graphene = 2
+    Cubic = 3
+
+
+class LeadSelfEnergies:
+    """Lead self-energies of one energy, block by block (nested.rs:439-459)."""
+
+    def __init__(self, handle, owner):
+        self._h = C.c_void_p(handle)
+        self._owner = owner
+
+    def __del__(self):
+        if getattr(self, "_h", None) and self._h.value:
+            capi.cg_sigma_destroy(self._h)
+            self._h = C.c_void_p(None)
+
+    def energy(self) -> float:
+        return capi.cg_sigma_energy(self._h)
+
+    def block(self, b):
+        """(Sigma_L, Sigma_R, channels_L, channels_R) of block b."""
+        nl, nr = self._owner.block_sizes(b)[:2]
+        sl = np.zeros((nl, nl), dtype=np.complex128, order="F")
+        sr = np.zeros((nr, nr), dtype=np.complex128, order="F")
+        cl, cr = C.c_int32(), C.c_int32()
+        _check(capi.cg_sigma_block(self._h, b, _dptr(sl), _dptr(sr), C.byref(cl), C.byref(cr)))
+        return sl, sr, cl.value, cr.value
+
+
+class NestedTwoTerminal:
+    """A two-terminal device solved by nested dissection (nested.rs:467-471)."""
+
+    def __init__(self, handle):
+        self._h = C.c_void_p(handle)
+
+    def __del__(self):
+        if getattr(self, "_h", None) and self._h.value:
+            capi.cg_system_destroy(self._h)
+            self._h = C.c_void_p(None)
+
+    @classmethod
+    def preset(cls, model, extent, axis=0, disorder=0, strength=0.0, concentration=0.0, seed=1, site_potential=None):
+        """A preset model on `Region::cuboid(extent)` (see caroli_amd.h, cg_preset_create)."""
+        ext = (C.c_uint64 * 3)(*[int(x) for x in (list(extent) + [0, 0, 0])[:3]])
+        h = C.c_void_p()
+        pot = None
+        if site_potential is not None:
+            site_potential = np.ascontiguousarray(site_potential, dtype=np.float64)
+            pot = _dptr(site_potential)
+        _check(capi.cg_preset_create(int(model), ext, axis, disorder, strength, concentration, seed, pot, C.byref(h)),
+               "preset")
+        return cls(h.value)
+
+    @classmethod
+    def from_arrays(cls, n, row_ptr, cols, vals, cells, site_orbitals, site_offsets, dim, hop_range, axis,
+                    left_h00, left_h01, right_h00, right_h01):
+        """`from_model` from the arrays the models produce (cg_system_create)."""
+        rp = np.ascontiguousarray(row_ptr, dtype=np.int64)
+        cl = np.ascontiguousarray(cols, dtype=np.uint32)
+        vv = np.ascontiguousarray(vals, dtype=np.complex128)
+        ce = np.ascontiguousarray(cells, dtype=np.int64).reshape(-1)
+        so = np.ascontiguousarray(site_orbitals, dtype=np.int32)
+        sf = np.ascontiguousarray(site_offsets, dtype=np.uint64)
+        rg = np.ascontiguousarray(hop_range, dtype=np.int64)
+        mats = [np.asfortranarray(m, dtype=np.complex128) for m in (left_h00, left_h01, right_h00, right_h01)]
+        h = C.c_void_p()
+        _check(capi.cg_system_create(
+            n, rp.ctypes.data_as(C.POINTER(C.c_int64)), cl.ctypes.data_as(C.POINTER(C.c_uint32)), _dptr(vv),
+            len(so), ce.ctypes.data_as(C.POINTER(C.c_int64)), so.ctypes.data_as(C.POINTER(C.c_int32)),
+            sf.ctypes.data_as(C.POINTER(C.c_uint64)), dim, rg.ctypes.data_as(C.POINTER(C.c_int64)), axis,
+            _dptr(mats[0]), _dptr(mats[1]), mats[0].shape[0], _dptr(mats[2]), _dptr(mats[3]), mats[2].shape[0],
+            C.byref(h)), "from_arrays")
+        return cls(h.value)
+
+    # -- structure -------------------------------------------------------------------------
+    def block_count(self) -> int:
+        return capi.cg_block_count(self._h)
+
+    def sizes(self):
+        n, nnz, ns = C.c_uint64(), C.c_uint64(), C.c_uint64()
+        _check(capi.cg_system_sizes(self._h, C.byref(n), C.byref(nnz), C.byref(ns)))
+        return n.value, nnz.value, ns.value
+
+    def block_sizes(self, b):
+        """(nl, nr, tree nodes, tree levels) of block b."""
+        v = [C.c_uint64() for _ in range(4)]
+        _check(capi.cg_block_sizes(self._h, b, *[C.byref(x) for x in v]))
+        return tuple(x.value for x in v)
+
+    def arrays(self):
+        """The system arrays: dict of CSR, geometry and lead cells."""
+        n, nnz, ns = self.sizes()
+        rp = np.zeros(n + 1, np.int64)
+        cl = np.zeros(nnz, np.uint32)
+        vv = np.zeros(nnz, np.complex128)
+        ce = np.zeros((ns, 3), np.int64)
+        so = np.zeros(ns, np.int32)
+        sf = np.zeros(ns, np.uint64)
+        dim, axis = C.c_int32(), C.c_int32()
+        rg = np.zeros(3, np.int64)
+        nl = self._lead_size(0)
+        nr = self._lead_size(1)
+        lh = [np.zeros((nl, nl), np.complex128, order="F") for _ in range(2)]
+        rh = [np.zeros((nr, nr), np.complex128, order="F") for _ in range(2)]
+        _check(capi.cg_system_arrays(
+            self._h, rp.ctypes.data_as(C.POINTER(C.c_int64)), cl.ctypes.data_as(C.POINTER(C.c_uint32)), _dptr(vv),
+            ce.ctypes.data_as(C.POINTER(C.c_int64)), so.ctypes.data_as(C.POINTER(C.c_int32)),
+            sf.ctypes.data_as(C.POINTER(C.c_uint64)), C.byref(dim), rg.ctypes.data_as(C.POINTER(C.c_int64)),
+            C.byref(axis), _dptr(lh[0]), _dptr(lh[1]), _dptr(rh[0]), _dptr(rh[1])))
+        return dict(n=n, row_ptr=rp, cols=cl, vals=vv, cells=ce, site_orbitals=so, site_offsets=sf, dim=dim.value,
+                    range=rg, axis=axis.value, left_h00=lh[0], left_h01=lh[1], right_h00=rh[0], right_h01=rh[1])
+
+    def _lead_size(self, side):
+        # The lead cells cover the whole end slices (all blocks).
+        n, nnz, ns = self.sizes()
+        ce = np.zeros((ns, 3), np.int64)
+        so = np.zeros(ns, np.int32)
+        axis = C.c_int32()
+        _check(capi.cg_system_arrays(self._h, None, None, None, ce.ctypes.data_as(C.POINTER(C.c_int64)),
+                                     so.ctypes.data_as(C.POINTER(C.c_int32)), None, None, None, C.byref(axis),
+                                     None, None, None, None))
+        c = ce[:, axis.value]
+        end = c.min() if side == 0 else c.max()
+        return int(so[c == end].sum())
+
+    def tree_node(self, b, k):
+        ne, nb = C.c_uint64(), C.c_uint64()
+        _check(capi.cg_tree_node(self._h, b, k, C.byref(ne), C.byref(nb), None, None))
+        e = np.zeros(ne.value, np.uint64)
+        d = np.zeros(nb.value, np.uint64)
+        _check(capi.cg_tree_node(self._h, b, k, C.byref(ne), C.byref(nb), e.ctypes.data_as(C.POINTER(C.c_uint64)),
+                                 d.ctypes.data_as(C.POINTER(C.c_uint64))))
+        return e, d
+
+    def work(self):
+        """(flops, bytes) of one transmission point (8 flops per complex MAC, 16-byte words)."""
+        f, b = C.c_double(), C.c_double()
+        _check(capi.cg_work(self._h, C.byref(f), C.byref(b)))
+        return f.value, b.value
+
+    # -- self-energies -------------------------------------------------------------------------
+    def self_energies(self, energy) -> LeadSelfEnergies:
+        """nested.rs:652-676 (FP64 mode matching on the host)."""
+        g = C.c_void_p()
+        _check(capi.cg_self_energies(self._h, float(energy), C.byref(g)), f"E = {energy}")
+        return LeadSelfEnergies(g.value, self)
+
+    def sigma_from_arrays(self, energy, blocks) -> LeadSelfEnergies:
+        """Self-energies given per block as (Sigma_L, Sigma_R)."""
+        flat = np.concatenate([np.asfortranarray(m, dtype=np.complex128).reshape(-1, order="F")
+                               for pair in blocks for m in pair])
+        g = C.c_void_p()
+        _check(capi.cg_sigma_from_arrays(self._h, float(energy), _dptr(flat), C.byref(g)))
+        return LeadSelfEnergies(g.value, self)
+
+    # -- transmission -------------------------------------------------------------------------
+    def transmission(self, energy, precision=Precision.Double) -> float:
+        """nested.rs:618-620."""
+        t = C.c_double()
+        _check(capi.cg_transmission(self._h, float(energy), int(precision), C.byref(t)), f"E = {energy}")
+        return t.value
+
+    def transmissions(self, energies, precision=Precision.Double):
+        """nested.rs:623-626: one batched launch; NaN where a point failed."""
+        e = np.ascontiguousarray(energies, dtype=np.float64)
+        t = np.zeros_like(e)
+        st = np.zeros(len(e), np.int32)
+        _check(capi.cg_transmissions(self._h, _dptr(e), len(e), int(precision), _dptr(t),
+                                     st.ctypes.data_as(C.POINTER(C.c_int32))))
+        return t, st
+
+    def transmission_with(self, sigma: LeadSelfEnergies) -> float:
+        """nested.rs:682-684."""
+        return self.transmission_with_precision(sigma, Precision.Double)
+
+    def transmission_with_precision(self, sigma: LeadSelfEnergies, precision) -> float:
+        """nested.rs:688-713."""
+        t = C.c_double()
+        _check(capi.cg_transmission_with(self._h, sigma._h, int(precision), C.byref(t)))
+        return t.value
+
+    def transmissions_checked(self, energies, options: PrecisionOptions):
+        """nested.rs:630-641 with every point checked against complex f64 (precision.rs:91-134):
+        returns (values, report); above the tolerance everything is recomputed in Double."""
+        sig = [self.self_energies(e) for e in energies]
+        fast, _ = batch_transmission([self] * len(sig), sig, options.precision)
+        exact, _ = batch_transmission([self] * len(sig), sig, Precision.Double)
+        d = np.abs(fast - exact)
+        worst = float(np.nanmax(d)) if len(d) else 0.0
+        fell = not (worst <= options.tolerance)
+        rep = PrecisionReport(options.precision, len(d), float(np.nanmedian(d)) if len(d) else 0.0, worst, fell, d)
+        return (exact if fell else fast), rep
+
+    def backward(self, sigma: LeadSelfEnergies):
+        """T and dT/d(eps_a) for every orbital a (adjoint sweep over the stored fronts)."""
+        n = self.sizes()[0]
+        g = np.zeros(n)
+        t = C.c_double()
+        _check(capi.cg_backward(self._h, sigma._h, C.byref(t), _dptr(g)))
+        return t.value, g
+
+
+TwoTerminal = NestedTwoTerminal
+
+
+def batch_transmission(systems, sigmas, precision=Precision.Double, stats=None):
+    """One batched launch over points (systems[p], sigmas[p]) (cg_batch_transmission).
+    Returns (T, status); `stats` (a capi.BatchStats) receives the timing and work counts."""
+    n = len(systems)
+    sh = (C.c_void_p * n)(*[s._h.value for s in systems])
+    gh = (C.c_void_p * n)(*[g._h.value for g in sigmas])
+    t = np.zeros(n)
+    st = np.zeros(n, np.int32)
+    bs = stats if stats is not None else capi.BatchStats()
+    _check(capi.cg_batch_transmission(sh, gh, n, int(precision), _dptr(t), st.ctypes.data_as(C.POINTER(C.c_int32)),
+                                      C.byref(bs)))
+    return t, st

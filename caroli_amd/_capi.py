@@ -1,0 +1,77 @@
+"""ctypes binding of include/caroli_amd.h (the engine's C ABI).
+
+Loads `caroli_amd/libcaroli_amd.so`, built in-tree by `caroli_amd/build.py`. There is no fallback:
+a missing or unloadable library raises at import.
+"""
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libcaroli_amd.so")
+
+if not os.path.exists(LIB_PATH):
+    raise ImportError(f"{LIB_PATH} is missing: build it with `python caroli_amd/build.py`")
+lib = C.CDLL(LIB_PATH)
+
+c_sys = C.c_void_p
+c_sig = C.c_void_p
+P = C.POINTER
+
+
+class BatchStats(C.Structure):
+    _fields_ = [
+        ("seconds", C.c_double),
+        ("front_flops", C.c_double),
+        ("front_bytes", C.c_double),
+        ("delayed_points", C.c_int64),
+        ("refinement_iters", C.c_int64),
+        ("max_refine_resid", C.c_double),
+    ]
+
+
+def _sig(name, res, *args):
+    f = getattr(lib, name)
+    f.restype = res
+    f.argtypes = list(args)
+    return f
+
+
+cg_system_create = _sig("cg_system_create", C.c_int32, C.c_uint64, P(C.c_int64), P(C.c_uint32), P(C.c_double),
+                        C.c_uint64, P(C.c_int64), P(C.c_int32), P(C.c_uint64), C.c_int32, P(C.c_int64), C.c_int32,
+                        P(C.c_double), P(C.c_double), C.c_uint64, P(C.c_double), P(C.c_double), C.c_uint64,
+                        P(c_sys))
+cg_preset_create = _sig("cg_preset_create", C.c_int32, C.c_int32, P(C.c_uint64), C.c_int32, C.c_int32, C.c_double,
+                        C.c_double, C.c_uint64, P(C.c_double), P(c_sys))
+cg_system_destroy = _sig("cg_system_destroy", None, c_sys)
+cg_block_count = _sig("cg_block_count", C.c_int32, c_sys)
+cg_system_sizes = _sig("cg_system_sizes", C.c_int32, c_sys, P(C.c_uint64), P(C.c_uint64), P(C.c_uint64))
+cg_block_sizes = _sig("cg_block_sizes", C.c_int32, c_sys, C.c_int32, P(C.c_uint64), P(C.c_uint64), P(C.c_uint64),
+                      P(C.c_uint64))
+cg_system_arrays = _sig("cg_system_arrays", C.c_int32, c_sys, P(C.c_int64), P(C.c_uint32), P(C.c_double),
+                        P(C.c_int64), P(C.c_int32), P(C.c_uint64), P(C.c_int32), P(C.c_int64), P(C.c_int32),
+                        P(C.c_double), P(C.c_double), P(C.c_double), P(C.c_double))
+cg_tree_node = _sig("cg_tree_node", C.c_int32, c_sys, C.c_int32, C.c_int32, P(C.c_uint64), P(C.c_uint64),
+                    P(C.c_uint64), P(C.c_uint64))
+cg_work = _sig("cg_work", C.c_int32, c_sys, P(C.c_double), P(C.c_double))
+cg_self_energies = _sig("cg_self_energies", C.c_int32, c_sys, C.c_double, P(c_sig))
+cg_sigma_from_arrays = _sig("cg_sigma_from_arrays", C.c_int32, c_sys, C.c_double, P(C.c_double), P(c_sig))
+cg_sigma_block = _sig("cg_sigma_block", C.c_int32, c_sig, C.c_int32, P(C.c_double), P(C.c_double), P(C.c_int32),
+                      P(C.c_int32))
+cg_sigma_energy = _sig("cg_sigma_energy", C.c_double, c_sig)
+cg_sigma_destroy = _sig("cg_sigma_destroy", None, c_sig)
+cg_transmission = _sig("cg_transmission", C.c_int32, c_sys, C.c_double, C.c_int32, P(C.c_double))
+cg_transmission_with = _sig("cg_transmission_with", C.c_int32, c_sys, c_sig, C.c_int32, P(C.c_double))
+cg_transmissions = _sig("cg_transmissions", C.c_int32, c_sys, P(C.c_double), C.c_int64, C.c_int32, P(C.c_double),
+                        P(C.c_int32))
+cg_batch_transmission = _sig("cg_batch_transmission", C.c_int32, P(c_sys), P(c_sig), C.c_int64, C.c_int32,
+                             P(C.c_double), P(C.c_int32), P(BatchStats))
+cg_backward = _sig("cg_backward", C.c_int32, c_sys, c_sig, P(C.c_double), P(C.c_double))
+cg_device_available = _sig("cg_device_available", C.c_int32)
+
+# Every function the header declares, for the export check.
+EXPORTS = [
+    "cg_system_create", "cg_preset_create", "cg_system_destroy", "cg_block_count", "cg_system_sizes",
+    "cg_block_sizes", "cg_system_arrays", "cg_tree_node", "cg_work", "cg_self_energies", "cg_sigma_from_arrays",
+    "cg_sigma_block", "cg_sigma_energy", "cg_sigma_destroy", "cg_transmission", "cg_transmission_with",
+    "cg_transmissions", "cg_batch_transmission", "cg_backward", "cg_device_available",
+]

@@ -1,0 +1,418 @@
+// The C ABI (include/caroli_amd.h): system construction, host self-energies, and the calls into
+// the GPU engine.
+#include <cmath>
+#include <cstring>
+#include <exception>
+
+#include "../../include/caroli_amd.h"
+#include "engine.hpp"
+#include "system.hpp"
+
+using namespace caroli;
+
+namespace caroli {
+
+CMat broadening(const CMat& s) {
+    CMat g(s.rows, s.cols);
+    for (int j = 0; j < s.cols; ++j)
+        for (int i = 0; i < s.rows; ++i) g(i, j) = cplx(0, 1) * (s(i, j) - std::conj(s(j, i)));
+    return g;
+}
+
+bool sigma_matches(const cg_system& s, const cg_sigma& g) {
+    if (g.blocks.size() != s.sym.blocks.size()) return false;
+    for (size_t b = 0; b < g.blocks.size(); ++b) {
+        auto& x = s.sym.blocks[b];
+        auto& y = g.blocks[b];
+        if (!(x.left_h00 == y.left_h00 && x.left_h01 == y.left_h01 && x.right_h00 == y.right_h00 &&
+              x.right_h01 == y.right_h01 && x.left_orbitals == y.left_orbitals && x.right_orbitals == y.right_orbitals))
+            return false;
+    }
+    return true;
+}
+
+Work algorithmic_work(const Symbolic& sym, size_t nnz) {
+    Work w;
+    const double es = 16.0;
+    for (auto& b : sym.blocks) {
+        auto& t = b.tree;
+        for (size_t k = 0; k < t.nodes.size(); ++k) {
+            double ne = double(t.plans[k].ne), n = double(t.plans[k].vars.size()), nb = n - ne;
+            w.flops += 8.0 / 3.0 * ne * ne * ne + 8.0 * ne * ne * nb + 8.0 * ne * nb * nb;
+            double kids = 0.0;
+            for (int c : t.nodes[k].children) {
+                double m = double(t.nodes[c].boundary.size());
+                kids += m * m;
+            }
+            w.bytes += es * (n * n + kids + nb * nb);
+        }
+        double k = double(b.kept.size()), nl = double(b.left_orbitals.size()), nr = double(b.right_orbitals.size());
+        w.flops += 8.0 / 3.0 * k * k * k + 8.0 * k * k * nr + 8.0 * nl * nr * (nl + nr);
+        w.bytes += es * k * k;
+    }
+    w.bytes += double(nnz) * (es + 4.0);
+    return w;
+}
+
+}  // namespace caroli
+
+namespace {
+
+uint64_t fnv(uint64_t h, const void* p, size_t n) {
+    auto* c = static_cast<const unsigned char*>(p);
+    for (size_t i = 0; i < n; ++i) h = (h ^ c[i]) * 1099511628211ULL;
+    return h;
+}
+
+template <class V>
+uint64_t fnv_vec(uint64_t h, const V& v) {
+    return fnv(h, v.data(), v.size() * sizeof(v[0]));
+}
+
+uint64_t pattern_hash(uint64_t h, const CMat& m) {
+    for (auto& z : m.a) {
+        char nz = z != cplx(0.0);
+        h = fnv(h, &nz, 1);
+    }
+    return h;
+}
+
+CMat read_mat(const double* p, size_t n) {
+    CMat m(static_cast<int>(n), static_cast<int>(n));
+    for (size_t k = 0; k < n * n; ++k) m.a[k] = cplx(p[2 * k], p[2 * k + 1]);
+    return m;
+}
+
+void write_mat(const CMat& m, double* p) {
+    for (size_t k = 0; k < m.a.size(); ++k) {
+        p[2 * k] = m.a[k].real();
+        p[2 * k + 1] = m.a[k].imag();
+    }
+}
+
+int32_t finish(cg_system* s, cg_system** out) {
+    s->sym = symbolic_phase(s->spec);
+    uint64_t h = 1469598103934665603ULL;
+    h = fnv(h, &s->spec.h.n, sizeof(s->spec.h.n));
+    h = fnv_vec(h, s->spec.h.row_ptr);
+    h = fnv_vec(h, s->spec.h.cols);
+    h = fnv_vec(h, s->spec.cells);
+    h = fnv_vec(h, s->spec.site_orbitals);
+    h = fnv(h, &s->spec.dim, sizeof(int));
+    h = fnv(h, s->spec.range.data(), sizeof(int64_t) * 3);
+    h = fnv(h, &s->spec.axis, sizeof(int));
+    for (auto* m : {&s->spec.left_h00, &s->spec.left_h01, &s->spec.right_h00, &s->spec.right_h01}) h = pattern_hash(h, *m);
+    s->key = h;
+    for (auto& b : s->sym.blocks) {
+        s->left_modes.emplace_back(b.left_h00, b.left_h01);
+        s->right_modes.emplace_back(b.right_h00, b.right_h01);
+    }
+    *out = s;
+    return CG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t cg_system_create(uint64_t n, const int64_t* row_ptr, const uint32_t* cols, const double* vals, uint64_t nsites,
+                         const int64_t* cells, const int32_t* site_orbitals, const uint64_t* site_offsets, int32_t dim,
+                         const int64_t* range, int32_t axis, const double* left_h00, const double* left_h01, uint64_t nl,
+                         const double* right_h00, const double* right_h01, uint64_t nr, cg_system** out) {
+    if (!out || !row_ptr || !cols || !vals || !cells || !site_orbitals || !site_offsets || !range || !left_h00 ||
+        !left_h01 || !right_h00 || !right_h01 || dim < 1 || dim > 3 || axis < 0 || axis >= dim || nsites == 0)
+        return CG_ERR_INVALID;
+    try {
+        auto* s = new cg_system;
+        auto& sp = s->spec;
+        sp.h.n = n;
+        sp.h.row_ptr.assign(row_ptr, row_ptr + n + 1);
+        size_t nnz = size_t(row_ptr[n]);
+        for (size_t r = 0; r < n; ++r)
+            if (row_ptr[r] > row_ptr[r + 1]) {
+                delete s;
+                return CG_ERR_INVALID;
+            }
+        sp.h.cols.assign(cols, cols + nnz);
+        for (auto c : sp.h.cols)
+            if (c >= n) {
+                delete s;
+                return CG_ERR_INVALID;
+            }
+        sp.h.vals.resize(nnz);
+        for (size_t e = 0; e < nnz; ++e) sp.h.vals[e] = cplx(vals[2 * e], vals[2 * e + 1]);
+        size_t total = 0;
+        for (size_t i = 0; i < nsites; ++i) {
+            sp.cells.push_back({cells[3 * i], cells[3 * i + 1], cells[3 * i + 2]});
+            sp.site_orbitals.push_back(site_orbitals[i]);
+            sp.site_offsets.push_back(site_offsets[i]);
+            total += size_t(site_orbitals[i]);
+            if (site_offsets[i] + size_t(site_orbitals[i]) > n) {
+                delete s;
+                return CG_ERR_INVALID;
+            }
+        }
+        if (total != n) {
+            delete s;
+            return CG_ERR_INVALID;
+        }
+        sp.dim = dim;
+        sp.range = {range[0], range[1], range[2]};
+        sp.axis = axis;
+        sp.left_h00 = read_mat(left_h00, nl);
+        sp.left_h01 = read_mat(left_h01, nl);
+        sp.right_h00 = read_mat(right_h00, nr);
+        sp.right_h01 = read_mat(right_h01, nr);
+        return finish(s, out);
+    } catch (const std::exception&) {
+        return CG_ERR_INVALID;
+    }
+}
+
+int32_t cg_preset_create(int32_t model, const uint64_t* extent, int32_t axis, int32_t disorder, double strength,
+                         double concentration, uint64_t seed, const double* site_potential, cg_system** out) {
+    if (!out || !extent) return CG_ERR_INVALID;
+    try {
+        PresetArgs a;
+        a.model = model;
+        for (int d = 0; d < 3; ++d) a.extent[d] = extent[d];
+        a.axis = axis;
+        a.disorder = disorder;
+        a.strength = strength;
+        a.concentration = concentration;
+        a.seed = seed;
+        a.site_potential = site_potential;
+        auto* s = new cg_system;
+        s->spec = preset_system(a);
+        return finish(s, out);
+    } catch (const std::exception&) {
+        return CG_ERR_INVALID;
+    }
+}
+
+void cg_system_destroy(cg_system* s) {
+    if (!s) return;
+    gpu::release(s);
+    delete s;
+}
+
+int32_t cg_block_count(const cg_system* s) { return s ? int32_t(s->sym.blocks.size()) : -1; }
+
+int32_t cg_system_sizes(const cg_system* s, uint64_t* n, uint64_t* nnz, uint64_t* nsites) {
+    if (!s) return CG_ERR_INVALID;
+    if (n) *n = s->spec.h.n;
+    if (nnz) *nnz = s->spec.h.vals.size();
+    if (nsites) *nsites = s->spec.cells.size();
+    return CG_OK;
+}
+
+int32_t cg_block_sizes(const cg_system* s, int32_t b, uint64_t* nl, uint64_t* nr, uint64_t* nodes, uint64_t* levels) {
+    if (!s || b < 0 || b >= int32_t(s->sym.blocks.size())) return CG_ERR_INVALID;
+    auto& blk = s->sym.blocks[b];
+    if (nl) *nl = blk.left_orbitals.size();
+    if (nr) *nr = blk.right_orbitals.size();
+    if (nodes) *nodes = blk.tree.nodes.size();
+    if (levels) *levels = blk.tree.levels;
+    return CG_OK;
+}
+
+int32_t cg_system_arrays(const cg_system* s, int64_t* row_ptr, uint32_t* cols, double* vals, int64_t* cells,
+                         int32_t* site_orbitals, uint64_t* site_offsets, int32_t* dim, int64_t* range, int32_t* axis,
+                         double* left_h00, double* left_h01, double* right_h00, double* right_h01) {
+    if (!s) return CG_ERR_INVALID;
+    auto& sp = s->spec;
+    if (row_ptr) std::memcpy(row_ptr, sp.h.row_ptr.data(), sp.h.row_ptr.size() * sizeof(int64_t));
+    if (cols) std::memcpy(cols, sp.h.cols.data(), sp.h.cols.size() * sizeof(uint32_t));
+    if (vals) std::memcpy(vals, sp.h.vals.data(), sp.h.vals.size() * sizeof(cplx));
+    if (cells)
+        for (size_t i = 0; i < sp.cells.size(); ++i)
+            for (int d = 0; d < 3; ++d) cells[3 * i + d] = sp.cells[i][d];
+    if (site_orbitals) std::memcpy(site_orbitals, sp.site_orbitals.data(), sp.site_orbitals.size() * sizeof(int32_t));
+    if (site_offsets)
+        for (size_t i = 0; i < sp.site_offsets.size(); ++i) site_offsets[i] = sp.site_offsets[i];
+    if (dim) *dim = sp.dim;
+    if (range)
+        for (int d = 0; d < 3; ++d) range[d] = sp.range[d];
+    if (axis) *axis = sp.axis;
+    if (left_h00) write_mat(sp.left_h00, left_h00);
+    if (left_h01) write_mat(sp.left_h01, left_h01);
+    if (right_h00) write_mat(sp.right_h00, right_h00);
+    if (right_h01) write_mat(sp.right_h01, right_h01);
+    return CG_OK;
+}
+
+int32_t cg_tree_node(const cg_system* s, int32_t b, int32_t k, uint64_t* ne, uint64_t* nb, uint64_t* eliminate,
+                     uint64_t* boundary) {
+    if (!s || b < 0 || b >= int32_t(s->sym.blocks.size())) return CG_ERR_INVALID;
+    auto& t = s->sym.blocks[b].tree;
+    if (k < 0 || k >= int32_t(t.nodes.size())) return CG_ERR_INVALID;
+    auto& nd = t.nodes[k];
+    if (ne) *ne = nd.eliminate.size();
+    if (nb) *nb = nd.boundary.size();
+    if (eliminate) std::copy(nd.eliminate.begin(), nd.eliminate.end(), eliminate);
+    if (boundary) std::copy(nd.boundary.begin(), nd.boundary.end(), boundary);
+    return CG_OK;
+}
+
+int32_t cg_work(const cg_system* s, double* flops, double* bytes) {
+    if (!s) return CG_ERR_INVALID;
+    Work w = algorithmic_work(s->sym, s->spec.h.vals.size());
+    if (flops) *flops = w.flops;
+    if (bytes) *bytes = w.bytes;
+    return CG_OK;
+}
+
+// NestedTwoTerminal::self_energies (nested.rs:652-676): per block, the left lead's Sigma_L and
+// the right lead's Sigma_R (the left modes serve both when the leads are equal).
+int32_t cg_self_energies(const cg_system* s, double energy, cg_sigma** out) {
+    if (!s || !out) return CG_ERR_INVALID;
+    try {
+        auto* g = new cg_sigma;
+        g->energy = energy;
+        for (size_t b = 0; b < s->sym.blocks.size(); ++b) {
+            auto& blk = s->sym.blocks[b];
+            cg_sigma::Block sb;
+            LeadModes ml, mr;
+            Status st = s->left_modes[b].modes_at(energy, ml);
+            if (st == kOk) {
+                if (blk.same_lead)
+                    mr = ml;
+                else
+                    st = s->right_modes[b].modes_at(energy, mr);
+            }
+            if (st != kOk) {
+                delete g;
+                return st;
+            }
+            sb.sl = ml.sigma_left();
+            sb.sr = mr.sigma_right();
+            sb.gl = broadening(sb.sl);
+            sb.gr = broadening(sb.sr);
+            sb.channels_l = ml.channels;
+            sb.channels_r = mr.channels;
+            sb.left_h00 = blk.left_h00;
+            sb.left_h01 = blk.left_h01;
+            sb.right_h00 = blk.right_h00;
+            sb.right_h01 = blk.right_h01;
+            sb.left_orbitals = blk.left_orbitals;
+            sb.right_orbitals = blk.right_orbitals;
+            g->blocks.push_back(std::move(sb));
+        }
+        *out = g;
+        return CG_OK;
+    } catch (const std::exception&) {
+        return CG_ERR_EIGEN;
+    }
+}
+
+int32_t cg_sigma_from_arrays(const cg_system* s, double energy, const double* sigmas, cg_sigma** out) {
+    if (!s || !sigmas || !out) return CG_ERR_INVALID;
+    auto* g = new cg_sigma;
+    g->energy = energy;
+    const double* p = sigmas;
+    for (auto& blk : s->sym.blocks) {
+        cg_sigma::Block sb;
+        size_t nl = blk.left_orbitals.size(), nr = blk.right_orbitals.size();
+        sb.sl = read_mat(p, nl);
+        p += 2 * nl * nl;
+        sb.sr = read_mat(p, nr);
+        p += 2 * nr * nr;
+        sb.gl = broadening(sb.sl);
+        sb.gr = broadening(sb.sr);
+        sb.left_h00 = blk.left_h00;
+        sb.left_h01 = blk.left_h01;
+        sb.right_h00 = blk.right_h00;
+        sb.right_h01 = blk.right_h01;
+        sb.left_orbitals = blk.left_orbitals;
+        sb.right_orbitals = blk.right_orbitals;
+        g->blocks.push_back(std::move(sb));
+    }
+    *out = g;
+    return CG_OK;
+}
+
+int32_t cg_sigma_block(const cg_sigma* g, int32_t b, double* sigma_l, double* sigma_r, int32_t* channels_l,
+                       int32_t* channels_r) {
+    if (!g || b < 0 || b >= int32_t(g->blocks.size())) return CG_ERR_INVALID;
+    auto& sb = g->blocks[b];
+    if (sigma_l) write_mat(sb.sl, sigma_l);
+    if (sigma_r) write_mat(sb.sr, sigma_r);
+    if (channels_l) *channels_l = sb.channels_l;
+    if (channels_r) *channels_r = sb.channels_r;
+    return CG_OK;
+}
+
+double cg_sigma_energy(const cg_sigma* g) { return g ? g->energy : NAN; }
+
+void cg_sigma_destroy(cg_sigma* g) { delete g; }
+
+int32_t cg_batch_transmission(const cg_system* const* systems, const cg_sigma* const* sigmas, int64_t count,
+                              int32_t precision, double* out_t, int32_t* out_status, cg_batch_stats* stats) {
+    if (count < 0 || (count > 0 && (!systems || !sigmas || !out_t))) return CG_ERR_INVALID;
+    if (precision < CG_DOUBLE || precision > CG_MIXED) return CG_ERR_INVALID;
+    for (int64_t p = 0; p < count; ++p)
+        if (!systems[p] || !sigmas[p] || !sigma_matches(*systems[p], *sigmas[p])) return CG_ERR_INVALID;
+    try {
+        return gpu::batch_transmission(systems, sigmas, count, precision, out_t, out_status, stats);
+    } catch (const std::exception&) {
+        return CG_ERR_DEVICE;
+    }
+}
+
+int32_t cg_transmission_with(const cg_system* s, const cg_sigma* g, int32_t precision, double* out_t) {
+    if (!s || !g || !out_t) return CG_ERR_INVALID;
+    int32_t status = CG_OK;
+    int32_t rc = cg_batch_transmission(&s, &g, 1, precision, out_t, &status, nullptr);
+    return rc != CG_OK ? rc : status;
+}
+
+int32_t cg_transmission(const cg_system* s, double energy, int32_t precision, double* out_t) {
+    if (!s || !out_t) return CG_ERR_INVALID;
+    cg_sigma* g = nullptr;
+    int32_t rc = cg_self_energies(s, energy, &g);
+    if (rc != CG_OK) {
+        *out_t = NAN;
+        return rc;
+    }
+    rc = cg_transmission_with(s, g, precision, out_t);
+    cg_sigma_destroy(g);
+    return rc;
+}
+
+int32_t cg_transmissions(const cg_system* s, const double* energies, int64_t count, int32_t precision, double* out_t,
+                         int32_t* out_status) {
+    if (!s || (count > 0 && (!energies || !out_t))) return CG_ERR_INVALID;
+    std::vector<cg_sigma*> sig(count, nullptr);
+    std::vector<int64_t> ok;
+    for (int64_t p = 0; p < count; ++p) {
+        int32_t rc = cg_self_energies(s, energies[p], &sig[p]);
+        out_t[p] = NAN;
+        if (out_status) out_status[p] = rc;
+        if (rc == CG_OK) ok.push_back(p);
+    }
+    std::vector<const cg_system*> ss(ok.size(), s);
+    std::vector<const cg_sigma*> gg;
+    for (auto p : ok) gg.push_back(sig[p]);
+    std::vector<double> t(ok.size());
+    std::vector<int32_t> st(ok.size());
+    int32_t rc = ok.empty() ? CG_OK : cg_batch_transmission(ss.data(), gg.data(), int64_t(ok.size()), precision, t.data(), st.data(), nullptr);
+    for (size_t q = 0; q < ok.size(); ++q) {
+        out_t[ok[q]] = t[q];
+        if (out_status) out_status[ok[q]] = st[q];
+    }
+    for (auto* g : sig) cg_sigma_destroy(g);
+    return rc;
+}
+
+int32_t cg_backward(const cg_system* s, const cg_sigma* g, double* out_t, double* out_grad) {
+    if (!s || !g || !out_t || !out_grad || !sigma_matches(*s, *g)) return CG_ERR_INVALID;
+    try {
+        return gpu::backward(s, g, out_t, out_grad);
+    } catch (const std::exception&) {
+        return CG_ERR_DEVICE;
+    }
+}
+
+int32_t cg_device_available(void) { return gpu::device_available() ? 1 : 0; }
+
+}  // extern "C"

@@ -1,0 +1,636 @@
+// Host side of the GPU engine: device plans of the elimination trees, the batched level sweep,
+// the re-planned sweep of points whose separators delayed, and the device bookkeeping.
+//
+// The numeric phase of crates/caroli-green/src/nested.rs (Tree::root/fronts/front :246-374,
+// root_with_leads :806-845, lead_green :783-802, caroli_trace lib.rs:108-113) runs on the device;
+// the host only plans (integers) and moves inputs and results.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <map>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+
+#include "engine.hpp"
+#include "fronts.hpp"
+
+namespace caroli::gpu {
+
+namespace {
+
+struct DeviceError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+#define HIP_CHECK(x)                                                                                   \
+    do {                                                                                               \
+        hipError_t e_ = (x);                                                                           \
+        if (e_ != hipSuccess) throw DeviceError(std::string(#x) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+// Pivot tests of crates/caroli-dense/src/field.rs:61 (c64) and :111 (c32).
+constexpr double kTolDouble = 1e-11;
+constexpr double kTolSingle = 1e-5;
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    DevBuf(DevBuf&& o) noexcept : p(o.p), bytes(o.bytes) {
+        o.p = nullptr;
+        o.bytes = 0;
+    }
+    ~DevBuf() {
+        if (p) (void)hipFree(p);
+    }
+    void ensure(size_t b) {
+        if (b <= bytes) return;
+        if (p) HIP_CHECK(hipFree(p));
+        p = nullptr;
+        bytes = 0;
+        HIP_CHECK(hipMalloc(&p, std::max<size_t>(b, 256)));
+        bytes = std::max<size_t>(b, 256);
+    }
+    template <class T>
+    T* as() const {
+        return static_cast<T*>(p);
+    }
+};
+
+template <class T>
+void upload(DevBuf& d, const std::vector<T>& v, hipStream_t st) {
+    d.ensure(v.size() * sizeof(T));
+    if (!v.empty()) HIP_CHECK(hipMemcpyAsync(d.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, st));
+}
+
+/// Descriptors of one block's fronts (tree nodes, then the root), in host memory.
+struct HostPlan {
+    std::vector<NodeDesc> nodes;
+    std::vector<Entry> ent;
+    std::vector<int32_t> diag;
+    std::vector<KidDesc> kids;
+    std::vector<int32_t> maps;
+    /// Node indices per level: all, and split into LDS-sized and larger fronts per precision.
+    std::vector<std::vector<int32_t>> level_all, small128, large128, small64, large64;
+    int64_t arena_stride = 0, root_stride = 0, piv_stride = 0;
+    int32_t slots = 0;
+    int32_t root = 0;  // index of the root descriptor
+};
+
+struct DevPlan {
+    DevBuf nodes, ent, diag, kids, maps, levels;
+    // Ranges into `levels` for each list of each level.
+    std::vector<std::pair<int32_t, int32_t>> all, s128, l128, s64, l64;
+    int32_t root_at = 0;  // position of the root index in `levels`
+};
+
+}  // namespace
+
+struct Structure {
+    std::vector<HostPlan> host;
+    std::vector<DevPlan> dev;
+};
+
+namespace {
+
+/// Root front of a block with `delayed` variables before the kept ones (nested.rs:262-280,
+/// :806-838), bordered by unit columns on the right slice and unit rows on the left slice:
+/// eliminating the first nd + k variables leaves -G_LR in the trailing block.
+void root_desc(const Block& blk, const SparseMatrix& h, const std::vector<size_t>& delayed, NodeDesc& d,
+               std::vector<Entry>& ent, std::vector<int32_t>& diag) {
+    const int nd = int(delayed.size()), k = int(blk.kept.size());
+    const int nl = int(blk.left_orbitals.size()), nr = int(blk.right_orbitals.size());
+    std::vector<size_t> vars = delayed;
+    vars.insert(vars.end(), blk.kept.begin(), blk.kept.end());
+    Positions kept(blk.kept);
+    d.ent_beg = int32_t(ent.size());
+    for (int a = 0; a < k; ++a) {
+        size_t i = blk.kept[a];
+        for (int64_t e = h.row_ptr[i]; e < h.row_ptr[i + 1]; ++e) {
+            int q = kept.get(h.cols[e]);
+            if (q >= 0) ent.push_back({nd + a, nd + q, int32_t(e), 0});
+        }
+    }
+    d.ent_end = int32_t(ent.size());
+    d.diag_beg = int32_t(diag.size());
+    for (int a = 0; a < k; ++a) diag.push_back(nd + a);
+    d.diag_end = int32_t(diag.size());
+    d.m = std::max(nl, nr);
+    d.n = nd + k + d.m;
+    d.ne = nd + k;
+    d.ne1 = nd;
+    d.nd = nd;
+    d.root = 1;
+    d.nl = nl;
+    d.nr = nr;
+    d.right_at = blk.right;
+}
+
+HostPlan build_plan(const Block& blk, const SparseMatrix& h) {
+    const Tree& t = blk.tree;
+    HostPlan hp;
+    const int nn = int(t.nodes.size());
+    int64_t off = 0, piv = 0;
+    for (int k = 0; k < nn; ++k) {
+        const Plan& pl = t.plans[k];
+        NodeDesc d{};
+        d.off = off;
+        d.n = int32_t(pl.vars.size());
+        d.ne = pl.ne;
+        d.ne1 = pl.ne;
+        d.ent_beg = int32_t(hp.ent.size());
+        for (auto& e : pl.entries) hp.ent.push_back({e.a, e.b, int32_t(e.k), e.conj ? 1 : 0});
+        d.ent_end = int32_t(hp.ent.size());
+        d.diag_beg = int32_t(hp.diag.size());
+        hp.diag.insert(hp.diag.end(), pl.diagonal.begin(), pl.diagonal.end());
+        d.diag_end = int32_t(hp.diag.size());
+        d.kid_beg = int32_t(hp.kids.size());
+        for (size_t q = 0; q < t.nodes[k].children.size(); ++q) {
+            const NodeDesc& c = hp.nodes[t.nodes[k].children[q]];
+            KidDesc kd{};
+            kd.src = c.off + c.ne + int64_t(c.ne) * c.n;
+            kd.ld = c.n;
+            kd.size = c.n - c.ne;
+            kd.map_beg = int32_t(hp.maps.size());
+            hp.maps.insert(hp.maps.end(), pl.child_pos[q].begin(), pl.child_pos[q].end());
+            hp.kids.push_back(kd);
+        }
+        d.kid_end = int32_t(hp.kids.size());
+        d.slot = k;
+        d.piv_off = piv;
+        off += int64_t(d.n) * d.n;
+        piv += d.ne;
+        hp.nodes.push_back(d);
+    }
+    hp.arena_stride = std::max<int64_t>(off, 1);
+    NodeDesc r{};
+    root_desc(blk, h, {}, r, hp.ent, hp.diag);
+    r.off = 0;
+    r.kid_beg = int32_t(hp.kids.size());
+    Positions kept(blk.kept);
+    for (int top : t.top) {
+        const NodeDesc& c = hp.nodes[top];
+        KidDesc kd{};
+        kd.src = c.off + c.ne + int64_t(c.ne) * c.n;
+        kd.ld = c.n;
+        kd.size = c.n - c.ne;
+        kd.map_beg = int32_t(hp.maps.size());
+        for (size_t o : t.nodes[top].boundary) {
+            int q = kept.get(o);
+            if (q < 0) throw std::logic_error("top boundary outside the kept orbitals");
+            hp.maps.push_back(q);
+        }
+        hp.kids.push_back(kd);
+    }
+    r.kid_end = int32_t(hp.kids.size());
+    r.slot = nn;
+    r.piv_off = piv;
+    piv += r.ne;
+    hp.root = nn;
+    hp.nodes.push_back(r);
+    hp.root_stride = int64_t(r.n) * r.n;
+    hp.piv_stride = std::max<int64_t>(piv, 1);
+    hp.slots = nn + 1;
+    hp.level_all.assign(t.levels, {});
+    hp.small128 = hp.large128 = hp.small64 = hp.large64 = hp.level_all;
+    for (int k = 0; k < nn; ++k) {
+        int lv = t.height[k], n = hp.nodes[k].n;
+        hp.level_all[lv].push_back(k);
+        (n <= kLdsFrontC128 ? hp.small128 : hp.large128)[lv].push_back(k);
+        (n <= kLdsFrontC64 ? hp.small64 : hp.large64)[lv].push_back(k);
+    }
+    return hp;
+}
+
+void upload_plan(const HostPlan& hp, DevPlan& dp, hipStream_t st) {
+    upload(dp.nodes, hp.nodes, st);
+    upload(dp.ent, hp.ent, st);
+    upload(dp.diag, hp.diag, st);
+    upload(dp.kids, hp.kids, st);
+    upload(dp.maps, hp.maps, st);
+    std::vector<int32_t> lv;
+    auto add = [&](const std::vector<std::vector<int32_t>>& lists, std::vector<std::pair<int32_t, int32_t>>& rng) {
+        for (auto& l : lists) {
+            rng.push_back({int32_t(lv.size()), int32_t(l.size())});
+            lv.insert(lv.end(), l.begin(), l.end());
+        }
+    };
+    add(hp.level_all, dp.all);
+    add(hp.small128, dp.s128);
+    add(hp.large128, dp.l128);
+    add(hp.small64, dp.s64);
+    add(hp.large64, dp.l64);
+    dp.root_at = int32_t(lv.size());
+    lv.push_back(hp.root);
+    upload(dp.levels, lv, st);
+}
+
+struct Engine {
+    std::mutex mu;
+    hipStream_t stream = nullptr;
+    bool ok = false;
+    std::map<uint64_t, std::weak_ptr<Structure>> registry;
+    DevBuf arena, root_arena, piv, status, tout, vals_ptrs, z, sigma, gamma;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+
+    Engine() {
+        int count = 0;
+        if (hipGetDeviceCount(&count) != hipSuccess || count == 0) return;
+        if (hipStreamCreateWithFlags(&stream, hipStreamNonBlocking) != hipSuccess) return;
+        if (hipEventCreate(&ev0) != hipSuccess || hipEventCreate(&ev1) != hipSuccess) return;
+        ok = true;
+    }
+    static Engine& get() {
+        static Engine e;
+        return e;
+    }
+};
+
+std::shared_ptr<Structure> structure_of(Engine& eng, const cg_system* s) {
+    std::lock_guard<std::mutex> g(s->mu);
+    if (s->structure) return s->structure;
+    auto it = eng.registry.find(s->key);
+    if (it != eng.registry.end())
+        if (auto sp = it->second.lock()) {
+            s->structure = sp;
+            return sp;
+        }
+    auto sp = std::make_shared<Structure>();
+    for (auto& blk : s->sym.blocks) sp->host.push_back(build_plan(blk, s->spec.h));
+    sp->dev.resize(sp->host.size());
+    for (size_t b = 0; b < sp->host.size(); ++b) upload_plan(sp->host[b], sp->dev[b], eng.stream);
+    HIP_CHECK(hipStreamSynchronize(eng.stream));
+    eng.registry[s->key] = sp;
+    s->structure = sp;
+    return sp;
+}
+
+const double2* vals_of(Engine& eng, const cg_system* s) {
+    std::lock_guard<std::mutex> g(s->mu);
+    if (!s->d_vals) {
+        size_t bytes = std::max<size_t>(s->spec.h.vals.size(), 1) * sizeof(double2);
+        HIP_CHECK(hipMalloc(&s->d_vals, bytes));
+        if (!s->spec.h.vals.empty())
+            HIP_CHECK(hipMemcpy(s->d_vals, s->spec.h.vals.data(), s->spec.h.vals.size() * sizeof(double2), hipMemcpyHostToDevice));
+    }
+    return static_cast<const double2*>(s->d_vals);
+}
+
+/// Per-point inputs of one block: energies, self-energies and broadenings, Hamiltonian values.
+void stage_points(Engine& eng, const std::vector<const cg_system*>& sys, const std::vector<const cg_sigma*>& sig, int b,
+                  int nl, int nr, PointsView& pv) {
+    const size_t P = sys.size();
+    const int64_t stride = int64_t(nl) * nl + int64_t(nr) * nr;
+    std::vector<const double2*> vp(P);
+    std::vector<double2> z(P), sg(P * stride), gm(P * stride);
+    for (size_t p = 0; p < P; ++p) {
+        vp[p] = vals_of(eng, sys[p]);
+        z[p] = double2{sig[p]->energy, 0.0};
+        auto& blk = sig[p]->blocks[b];
+        auto put = [&](std::vector<double2>& dst, const CMat& a, const CMat& c) {
+            double2* o = dst.data() + p * stride;
+            for (size_t e = 0; e < a.a.size(); ++e) o[e] = double2{a.a[e].real(), a.a[e].imag()};
+            o += a.a.size();
+            for (size_t e = 0; e < c.a.size(); ++e) o[e] = double2{c.a[e].real(), c.a[e].imag()};
+        };
+        put(sg, blk.sl, blk.sr);
+        put(gm, blk.gl, blk.gr);
+    }
+    upload(eng.vals_ptrs, vp, eng.stream);
+    upload(eng.z, z, eng.stream);
+    upload(eng.sigma, sg, eng.stream);
+    upload(eng.gamma, gm, eng.stream);
+    pv.vals = eng.vals_ptrs.as<const double2* const>();
+    pv.z = eng.z.as<const double2>();
+    pv.sigma = eng.sigma.as<const double2>();
+    pv.gamma = eng.gamma.as<const double2>();
+    pv.sigma_stride = stride;
+}
+
+/// One sweep over the static plan of block b for the points of `pv`.
+void sweep(Engine& eng, const HostPlan& hp, const DevPlan& dp, const PointsView& pv, int32_t P, bool single) {
+    const NodeDesc* nodes = dp.nodes.as<NodeDesc>();
+    const int32_t* lv = dp.levels.as<int32_t>();
+    const Entry* ent = dp.ent.as<Entry>();
+    const int32_t* diag = dp.diag.as<int32_t>();
+    const KidDesc* kids = dp.kids.as<KidDesc>();
+    const int32_t* maps = dp.maps.as<int32_t>();
+    for (size_t l = 0; l < dp.all.size(); ++l) {
+        auto [a0, ac] = dp.all[l];
+        if (single) {
+            launch_assemble_c64(nodes, lv + a0, ac, ent, diag, kids, maps, pv, P, eng.stream);
+            launch_factor_c64(nodes, lv + dp.s64[l].first, dp.s64[l].second, pv, P, kTolSingle, true, eng.stream);
+            launch_factor_c64(nodes, lv + dp.l64[l].first, dp.l64[l].second, pv, P, kTolSingle, false, eng.stream);
+        } else {
+            launch_assemble_c128(nodes, lv + a0, ac, ent, diag, kids, maps, pv, P, eng.stream);
+            launch_factor_c128(nodes, lv + dp.s128[l].first, dp.s128[l].second, pv, P, kTolDouble, true, false, eng.stream);
+            launch_factor_c128(nodes, lv + dp.l128[l].first, dp.l128[l].second, pv, P, kTolDouble, false, false, eng.stream);
+        }
+    }
+    launch_assemble_root(nodes, lv + dp.root_at, 1, ent, diag, kids, maps, pv, P, single ? 1 : 0, eng.stream);
+    launch_factor_c128(nodes, lv + dp.root_at, 1, pv, P, kTolDouble, false, true, eng.stream);
+    launch_trace(nodes + hp.root, pv, P, eng.stream);
+    HIP_CHECK(hipGetLastError());
+}
+
+/// What a front handed to its parent in the re-planned sweep.
+struct Handed {
+    std::vector<size_t> vars;
+    int64_t src;
+    int32_t ld, size;
+};
+
+/// Re-planned sweep of one point of one block whose separators delayed: level by level, the
+/// fronts above delayed children are assembled from the variables actually handed over
+/// (nested.rs:331-344), exactly as the reference's unplanned path. Returns the block's T or NaN
+/// and the status.
+int32_t replanned_point(Engine& eng, const cg_system* s, const cg_sigma* g, int b, bool single, double& t_out) {
+    const Block& blk = s->sym.blocks[b];
+    const Tree& t = blk.tree;
+    const HostPlan& stat = s->structure->host[b];
+    const int nn = int(t.nodes.size());
+    const size_t es = single ? sizeof(float2) : sizeof(double2);
+    std::vector<const cg_system*> one_s{s};
+    std::vector<const cg_sigma*> one_g{g};
+    PointsView pv{};
+    stage_points(eng, one_s, one_g, b, int(blk.left_orbitals.size()), int(blk.right_orbitals.size()), pv);
+    DevBuf arena, piv, status, tout, root, dn, dent, ddiag, dkids, dmaps, dlv;
+    int64_t used = 0, piv_used = 0;
+    auto grow = [&](DevBuf& buf, int64_t need_bytes, int64_t keep_bytes) {
+        if (size_t(need_bytes) <= buf.bytes) return;
+        DevBuf nb;
+        nb.ensure(size_t(std::max<int64_t>(need_bytes, 2 * int64_t(buf.bytes))));
+        if (keep_bytes > 0) HIP_CHECK(hipMemcpyAsync(nb.p, buf.p, size_t(keep_bytes), hipMemcpyDeviceToDevice, eng.stream));
+        std::swap(buf.p, nb.p);
+        std::swap(buf.bytes, nb.bytes);
+    };
+    status.ensure(sizeof(int32_t) * (nn + 1));
+    tout.ensure(sizeof(double));
+    std::vector<Handed> handed(nn);
+    std::vector<int32_t> st_host(nn + 1, 0);
+    pv.status = status.as<int32_t>();
+    pv.status_stride = nn + 1;
+    pv.t_out = tout.as<double>();
+    pv.arena_stride = 0;
+    pv.piv_stride = 0;
+    pv.root_stride = 0;
+    for (int l = 0; l < t.levels + 1; ++l) {
+        const bool root_level = l == t.levels;
+        const int64_t prev_used = used, prev_piv = piv_used;
+        std::vector<NodeDesc> nodes;
+        std::vector<Entry> ent;
+        std::vector<int32_t> diag, maps, list;
+        std::vector<KidDesc> kids;
+        std::vector<std::vector<size_t>> fvars;
+        std::vector<int> which;
+        auto add_kids = [&](NodeDesc& d, const std::vector<int>& children, const Positions& pos) {
+            d.kid_beg = int32_t(kids.size());
+            for (int c : children) {
+                KidDesc kd{};
+                kd.src = handed[c].src;
+                kd.ld = handed[c].ld;
+                kd.size = handed[c].size;
+                kd.map_beg = int32_t(maps.size());
+                for (size_t o : handed[c].vars) {
+                    int q = pos.get(o);
+                    if (q < 0) throw std::logic_error("handed variable missing from the parent front");
+                    maps.push_back(q);
+                }
+                kids.push_back(kd);
+            }
+            d.kid_end = int32_t(kids.size());
+        };
+        if (!root_level) {
+            for (int k = 0; k < nn; ++k) {
+                if (t.height[k] != l) continue;
+                const Node& node = t.nodes[k];
+                std::vector<const std::vector<size_t>*> hv;
+                for (int c : node.children) hv.push_back(&handed[c].vars);
+                auto [vars, ne] = front_vars(node, hv);
+                Positions pos(vars);
+                NodeDesc d{};
+                d.n = int32_t(vars.size());
+                d.ne = ne;
+                d.ne1 = ne;
+                d.off = used;
+                d.slot = k;
+                d.piv_off = piv_used;
+                used += int64_t(d.n) * d.n;
+                piv_used += ne;
+                d.ent_beg = int32_t(ent.size());
+                for (auto& e : separator_entries(s->spec.h, node, pos)) ent.push_back({e.a, e.b, int32_t(e.k), e.conj ? 1 : 0});
+                d.ent_end = int32_t(ent.size());
+                d.diag_beg = int32_t(diag.size());
+                for (size_t o : node.eliminate) diag.push_back(pos.get(o));
+                d.diag_end = int32_t(diag.size());
+                add_kids(d, node.children, pos);
+                list.push_back(int32_t(nodes.size()));
+                nodes.push_back(d);
+                fvars.push_back(vars);
+                which.push_back(k);
+            }
+        } else {
+            Positions in_kept(blk.kept);
+            std::vector<size_t> delayed;
+            for (int top : t.top)
+                for (size_t o : handed[top].vars)
+                    if (in_kept.get(o) < 0) delayed.push_back(o);
+            std::sort(delayed.begin(), delayed.end());
+            delayed.erase(std::unique(delayed.begin(), delayed.end()), delayed.end());
+            NodeDesc d{};
+            root_desc(blk, s->spec.h, delayed, d, ent, diag);
+            std::vector<size_t> vars = delayed;
+            vars.insert(vars.end(), blk.kept.begin(), blk.kept.end());
+            Positions pos(vars);
+            add_kids(d, t.top, pos);
+            d.off = 0;
+            d.slot = nn;
+            d.piv_off = piv_used;
+            piv_used += d.ne;
+            list.push_back(0);
+            nodes.push_back(d);
+        }
+        grow(arena, used * int64_t(es), prev_used * int64_t(es));
+        grow(piv, piv_used * 4, prev_piv * 4);
+        pv.arena = arena.p;
+        pv.piv = piv.as<int32_t>();
+        upload(dn, nodes, eng.stream);
+        upload(dent, ent, eng.stream);
+        upload(ddiag, diag, eng.stream);
+        upload(dkids, kids, eng.stream);
+        upload(dmaps, maps, eng.stream);
+        upload(dlv, list, eng.stream);
+        const NodeDesc* dd = dn.as<NodeDesc>();
+        if (!root_level) {
+            std::vector<int32_t> small, large;
+            for (size_t q = 0; q < nodes.size(); ++q)
+                (nodes[q].n <= (single ? kLdsFrontC64 : kLdsFrontC128) ? small : large).push_back(int32_t(q));
+            DevBuf ds, dl;
+            upload(ds, small, eng.stream);
+            upload(dl, large, eng.stream);
+            if (single) {
+                launch_assemble_c64(dd, dlv.as<int32_t>(), int32_t(list.size()), dent.as<Entry>(), ddiag.as<int32_t>(),
+                                    dkids.as<KidDesc>(), dmaps.as<int32_t>(), pv, 1, eng.stream);
+                launch_factor_c64(dd, ds.as<int32_t>(), int32_t(small.size()), pv, 1, kTolSingle, true, eng.stream);
+                launch_factor_c64(dd, dl.as<int32_t>(), int32_t(large.size()), pv, 1, kTolSingle, false, eng.stream);
+            } else {
+                launch_assemble_c128(dd, dlv.as<int32_t>(), int32_t(list.size()), dent.as<Entry>(), ddiag.as<int32_t>(),
+                                     dkids.as<KidDesc>(), dmaps.as<int32_t>(), pv, 1, eng.stream);
+                launch_factor_c128(dd, ds.as<int32_t>(), int32_t(small.size()), pv, 1, kTolDouble, true, false, eng.stream);
+                launch_factor_c128(dd, dl.as<int32_t>(), int32_t(large.size()), pv, 1, kTolDouble, false, false, eng.stream);
+            }
+            HIP_CHECK(hipGetLastError());
+            HIP_CHECK(hipMemcpyAsync(st_host.data(), status.p, sizeof(int32_t) * (nn + 1), hipMemcpyDeviceToHost, eng.stream));
+            HIP_CHECK(hipStreamSynchronize(eng.stream));
+            for (size_t q = 0; q < nodes.size(); ++q) {
+                int k = which[q];
+                const NodeDesc& d = nodes[q];
+                if (st_host[k] == kFrontOk) {
+                    handed[k] = {t.nodes[k].boundary, d.off + d.ne + int64_t(d.ne) * d.n, d.n, d.n - d.ne};
+                } else {  // delayed (or nothing eliminated): the whole front moves up
+                    handed[k] = {fvars[q], d.off, d.n, d.n};
+                }
+            }
+        } else {
+            const NodeDesc& d = nodes[0];
+            root.ensure(size_t(int64_t(d.n) * d.n) * sizeof(double2));
+            pv.root_arena = root.p;
+            launch_assemble_root(dd, dlv.as<int32_t>(), 1, dent.as<Entry>(), ddiag.as<int32_t>(), dkids.as<KidDesc>(),
+                                 dmaps.as<int32_t>(), pv, 1, single ? 1 : 0, eng.stream);
+            launch_factor_c128(dd, dlv.as<int32_t>(), 1, pv, 1, kTolDouble, false, true, eng.stream);
+            launch_trace(dd, pv, 1, eng.stream);
+            HIP_CHECK(hipGetLastError());
+            int32_t rs = 0;
+            HIP_CHECK(hipMemcpyAsync(&rs, static_cast<int32_t*>(status.p) + nn, sizeof(int32_t), hipMemcpyDeviceToHost, eng.stream));
+            HIP_CHECK(hipMemcpyAsync(&t_out, tout.p, sizeof(double), hipMemcpyDeviceToHost, eng.stream));
+            HIP_CHECK(hipStreamSynchronize(eng.stream));
+            if (rs != kFrontOk) {
+                t_out = NAN;
+                return CG_ERR_SINGULAR;
+            }
+            if (!std::isfinite(t_out)) return CG_ERR_SINGULAR;
+        }
+    }
+    return CG_OK;
+}
+
+}  // namespace
+
+bool device_available() { return Engine::get().ok; }
+
+void release(const cg_system* s) {
+    if (s->d_vals) {
+        (void)hipFree(s->d_vals);
+        s->d_vals = nullptr;
+    }
+}
+
+int32_t batch_transmission(const cg_system* const* systems, const cg_sigma* const* sigmas, int64_t count,
+                           int32_t precision, double* out_t, int32_t* out_status, cg_batch_stats* stats) {
+    Engine& eng = Engine::get();
+    if (!eng.ok) return CG_ERR_DEVICE;
+    if (precision == CG_MIXED) return CG_ERR_INVALID;  // not available in this engine yet
+    std::lock_guard<std::mutex> lock(eng.mu);
+    const bool single = precision == CG_SINGLE;
+    const size_t es = single ? sizeof(float2) : sizeof(double2);
+    cg_batch_stats local{};
+    for (int64_t p = 0; p < count; ++p) {
+        out_t[p] = 0.0;
+        if (out_status) out_status[p] = CG_OK;
+    }
+    // Group the points by structure.
+    std::map<uint64_t, std::vector<int64_t>> groups;
+    for (int64_t p = 0; p < count; ++p) groups[systems[p]->key].push_back(p);
+    std::vector<int32_t> status(count, CG_OK);
+    for (auto& [key, idx] : groups) {
+        auto sp = structure_of(eng, systems[idx[0]]);
+        for (auto p : idx) structure_of(eng, systems[p]);
+        Work w = algorithmic_work(systems[idx[0]]->sym, systems[idx[0]]->spec.h.vals.size());
+        local.front_flops += w.flops * double(idx.size());
+        local.front_bytes += w.bytes * double(idx.size()) * (single ? 0.5 : 1.0);
+        for (size_t b = 0; b < sp->host.size(); ++b) {
+            const HostPlan& hp = sp->host[b];
+            const DevPlan& dp = sp->dev[b];
+            const Block& blk = systems[idx[0]]->sym.blocks[b];
+            // Points per launch: bounded by the arena budget and the grid's y dimension.
+            const size_t per_point = size_t(hp.arena_stride) * es + size_t(hp.root_stride) * sizeof(double2) +
+                                     size_t(hp.piv_stride) * 4 + size_t(hp.slots) * 4 + 64;
+            const size_t budget = size_t(96) << 30;
+            size_t chunk = std::max<size_t>(1, std::min<size_t>({budget / per_point, 65535, idx.size()}));
+            for (size_t c0 = 0; c0 < idx.size(); c0 += chunk) {
+                size_t P = std::min(chunk, idx.size() - c0);
+                std::vector<const cg_system*> ss;
+                std::vector<const cg_sigma*> gg;
+                for (size_t q = 0; q < P; ++q) {
+                    ss.push_back(systems[idx[c0 + q]]);
+                    gg.push_back(sigmas[idx[c0 + q]]);
+                }
+                PointsView pv{};
+                stage_points(eng, ss, gg, int(b), int(blk.left_orbitals.size()), int(blk.right_orbitals.size()), pv);
+                eng.arena.ensure(P * size_t(hp.arena_stride) * es);
+                eng.root_arena.ensure(P * size_t(hp.root_stride) * sizeof(double2));
+                eng.piv.ensure(P * size_t(hp.piv_stride) * 4);
+                eng.status.ensure(P * size_t(hp.slots) * 4);
+                eng.tout.ensure(P * sizeof(double));
+                pv.arena = eng.arena.p;
+                pv.arena_stride = hp.arena_stride;
+                pv.root_arena = eng.root_arena.p;
+                pv.root_stride = hp.root_stride;
+                pv.piv = eng.piv.as<int32_t>();
+                pv.piv_stride = hp.piv_stride;
+                pv.status = eng.status.as<int32_t>();
+                pv.status_stride = hp.slots;
+                pv.t_out = eng.tout.as<double>();
+                HIP_CHECK(hipEventRecord(eng.ev0, eng.stream));
+                sweep(eng, hp, dp, pv, int32_t(P), single);
+                HIP_CHECK(hipEventRecord(eng.ev1, eng.stream));
+                std::vector<double> t(P);
+                std::vector<int32_t> st(P * size_t(hp.slots));
+                HIP_CHECK(hipMemcpyAsync(t.data(), eng.tout.p, P * sizeof(double), hipMemcpyDeviceToHost, eng.stream));
+                HIP_CHECK(hipMemcpyAsync(st.data(), eng.status.p, st.size() * 4, hipMemcpyDeviceToHost, eng.stream));
+                HIP_CHECK(hipStreamSynchronize(eng.stream));
+                float ms = 0.f;
+                HIP_CHECK(hipEventElapsedTime(&ms, eng.ev0, eng.ev1));
+                local.seconds += ms * 1e-3;
+                for (size_t q = 0; q < P; ++q) {
+                    int64_t p = idx[c0 + q];
+                    bool delayed = false;
+                    int32_t root_status = st[q * hp.slots + hp.root];
+                    for (int32_t k = 0; k < hp.root; ++k) delayed |= st[q * hp.slots + k] != kFrontOk;
+                    if (delayed) {
+                        double tb = NAN;
+                        auto t0 = std::chrono::steady_clock::now();
+                        int32_t rc = replanned_point(eng, systems[p], sigmas[p], int(b), single, tb);
+                        local.seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+                        ++local.delayed_points;
+                        if (rc != CG_OK) status[p] = rc;
+                        out_t[p] += tb;
+                    } else if (root_status != kFrontOk || !std::isfinite(t[q])) {
+                        status[p] = CG_ERR_SINGULAR;
+                        out_t[p] = NAN;
+                    } else {
+                        out_t[p] += t[q];
+                    }
+                }
+            }
+        }
+    }
+    for (int64_t p = 0; p < count; ++p) {
+        if (status[p] != CG_OK) out_t[p] = NAN;
+        if (out_status) out_status[p] = status[p];
+    }
+    if (stats) *stats = local;
+    return CG_OK;
+}
+
+int32_t backward(const cg_system*, const cg_sigma*, double* out_t, double*) {
+    *out_t = NAN;
+    return CG_ERR_INVALID;  // provided by the adjoint engine
+}
+
+}  // namespace caroli::gpu

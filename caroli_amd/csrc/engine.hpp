@@ -1,0 +1,23 @@
+// The GPU engine: batched multifrontal elimination of many transmission points on one device.
+#pragma once
+#include <cstdint>
+
+#include "../../include/caroli_amd.h"
+#include "system.hpp"
+
+namespace caroli::gpu {
+
+/// Transmission of `count` points (systems[p], sigmas[p]) in `precision`, level by level over
+/// the elimination trees, all points of one structure per launch.
+int32_t batch_transmission(const cg_system* const* systems, const cg_sigma* const* sigmas, int64_t count,
+                           int32_t precision, double* out_t, int32_t* out_status, cg_batch_stats* stats);
+
+/// T and dT/d(eps_a) for every orbital, from the stored fronts of one point.
+int32_t backward(const cg_system* s, const cg_sigma* g, double* out_t, double* out_grad);
+
+bool device_available();
+
+/// Frees the device copies cached for a system.
+void release(const cg_system* s);
+
+}  // namespace caroli::gpu

@@ -1,0 +1,82 @@
+// Device-side descriptors of fronts and the launchers of the front kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace caroli::gpu {
+
+/// One front of an elimination tree, as the kernels see it. The front F (n x n, column-major,
+/// leading dimension n) sits at `off` complex words into a point's arena. Columns [0, ne) are
+/// eliminated with row pivoting; the pivots of columns [0, ne1) are tested against the largest
+/// entry of F[0:ne1, 0:ne1] (a failure delays the front), those of [ne1, ne) only for zero.
+struct NodeDesc {
+    int64_t off;
+    int32_t n, ne, ne1;
+    int32_t ent_beg, ent_end;    // H entries of the separator
+    int32_t diag_beg, diag_end;  // positions receiving z
+    int32_t kid_beg, kid_end;    // children
+    int32_t root;                // 1: subtract the self-energies and add the borders
+    int32_t nl, nr, right_at, m; // root: lead slices (from position nd), border size m
+    int32_t nd;                  // root: variables delayed up to it
+    int32_t slot;                // status / pivot slot of the front within a point
+    int64_t piv_off;             // first pivot index of the front within a point
+};
+
+/// An H entry: F[a, b] -= conj ? conj(vals[k]) : vals[k].
+struct Entry {
+    int32_t a, b, k, conj;
+};
+
+/// A child's Schur complement (size x size, leading dimension ld) at `src` in the kid arena,
+/// added at the front positions maps[map_beg + i].
+struct KidDesc {
+    int64_t src;
+    int32_t ld, size, map_beg, pad;
+};
+
+/// Per-point inputs and outputs of one launch.
+struct PointsView {
+    const double2* const* vals;  // device CSR values of each point's system
+    const double2* z;            // energy of each point
+    const double2* sigma;        // Sigma_L (nl^2) then Sigma_R (nr^2) of each point
+    const double2* gamma;        // Gamma_L then Gamma_R of each point
+    int64_t sigma_stride;
+    void* arena;                 // fronts of each point
+    int64_t arena_stride;        // in complex words of the front type
+    void* root_arena;            // root fronts (complex f64)
+    int64_t root_stride;
+    int32_t* piv;
+    int64_t piv_stride;
+    int32_t* status;  // per point: one slot per front
+    int32_t status_stride;
+    double* t_out;    // per point
+};
+
+/// Status codes written by the kernels.
+enum : int32_t { kFrontOk = 0, kFrontDelayed = 5, kFrontSingular = 1 };
+
+/// Largest front eliminated in LDS by the small-front kernels.
+constexpr int kLdsFrontC128 = 64;
+constexpr int kLdsFrontC64 = 90;
+
+// Launchers (fronts.hip). `level` lists node indices into `nodes`; grid = count x points.
+void launch_assemble_c128(const NodeDesc* nodes, const int32_t* level, int32_t count, const Entry* ent,
+                          const int32_t* diag, const KidDesc* kids, const int32_t* maps, const PointsView& pv,
+                          int32_t points, hipStream_t st);
+void launch_assemble_c64(const NodeDesc* nodes, const int32_t* level, int32_t count, const Entry* ent,
+                         const int32_t* diag, const KidDesc* kids, const int32_t* maps, const PointsView& pv,
+                         int32_t points, hipStream_t st);
+/// Root assembly (complex f64 root, children in the front precision: c64 when single != 0).
+void launch_assemble_root(const NodeDesc* nodes, const int32_t* level, int32_t count, const Entry* ent,
+                          const int32_t* diag, const KidDesc* kids, const int32_t* maps, const PointsView& pv,
+                          int32_t points, int single, hipStream_t st);
+/// Elimination of the fronts listed (small: in LDS, n <= kLdsFront*; large: in global memory).
+void launch_factor_c128(const NodeDesc* nodes, const int32_t* level, int32_t count, const PointsView& pv,
+                        int32_t points, double tol, bool small, bool root, hipStream_t st);
+void launch_factor_c64(const NodeDesc* nodes, const int32_t* level, int32_t count, const PointsView& pv,
+                       int32_t points, double tol, bool small, hipStream_t st);
+/// Caroli trace of the root border block: t_out[p] = Tr[Gamma_L G Gamma_R G^H].
+void launch_trace(const NodeDesc* root, const PointsView& pv, int32_t points, hipStream_t st);
+
+}  // namespace caroli::gpu

@@ -1,0 +1,168 @@
+// Host-side input builders: lattices, regions, tight-binding models, the sparse Hamiltonian of a
+// region, the lead cells, reproducible random streams and disorder.
+//
+// Behaviour follows caroli-lattice (crates/caroli-lattice/src/lib.rs), caroli-model
+// (crates/caroli-model/src/{lib.rs,sparse.rs,presets.rs,gamma.rs}) and caroli-disorder
+// (crates/caroli-disorder/src/{lib.rs,rng.rs}): same site order, same orbital layout, same
+// sparse entries and the same random draws, so a seed gives the same Hamiltonian.
+#pragma once
+#include <array>
+#include <functional>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "cmat.hpp"
+
+namespace caroli {
+
+using Cell = std::array<int64_t, 3>;
+
+struct Sublattice {
+    std::array<double, 3> offset;
+    int orbitals;
+};
+
+/// Bravais lattice with a basis (crates/caroli-lattice/src/lib.rs:27-62).
+struct Lattice {
+    std::vector<std::array<double, 3>> vectors;
+    std::vector<Sublattice> subs;
+    int dim() const { return int(vectors.size()); }
+    static Lattice square(double a, int orbitals);
+    static Lattice cubic(double a, int orbitals);
+    static Lattice honeycomb(double a, int orbitals);
+};
+
+struct Site {
+    Cell cell;
+    int sub;
+    bool operator<(const Site& o) const {
+        return cell != o.cell ? cell < o.cell : sub < o.sub;
+    }
+    bool operator==(const Site& o) const { return cell == o.cell && sub == o.sub; }
+};
+
+/// Ordered set of sites; site i holds orbitals offset(i) .. offset(i) + orbitals(i)
+/// (crates/caroli-lattice/src/lib.rs:98-194).
+struct Region {
+    Lattice lattice;
+    std::vector<Site> sites;
+    std::vector<size_t> offsets;
+    size_t orbitals = 0;
+    std::map<Site, size_t> index;
+
+    static Region from_sites(const Lattice& lat, std::vector<Site> sites);
+    /// First axis slowest, sublattice fastest (lib.rs:126-140).
+    static Region cuboid(const Lattice& lat, std::array<size_t, 3> extent);
+    int site_orbitals(size_t i) const { return lattice.subs[sites[i].sub].orbitals; }
+    long find(const Site& s) const {
+        auto it = index.find(s);
+        return it == index.end() ? -1 : long(it->second);
+    }
+    /// Slices along `axis`, by increasing coordinate, each in region order (lib.rs:188-194).
+    std::vector<Region> slices(int axis) const;
+    Region translated(Cell d) const;
+};
+
+struct Hopping {
+    int from, to;
+    Cell offset;
+    CMat matrix;
+};
+
+/// Square complex CSR matrix with sorted columns (crates/caroli-model/src/sparse.rs:13-44).
+struct SparseMatrix {
+    size_t n = 0;
+    std::vector<int64_t> row_ptr;
+    std::vector<uint32_t> cols;
+    std::vector<cplx> vals;
+    /// Duplicates summed, columns sorted in each row (sparse.rs:23-44).
+    static SparseMatrix from_triplets(size_t n, std::vector<std::tuple<size_t, size_t, cplx>> t);
+};
+
+/// Tight-binding model: on-site matrix per sublattice, hoppings, extra site-dependent on-site
+/// terms (crates/caroli-model/src/lib.rs:188-470).
+struct Model {
+    Lattice lattice;
+    std::vector<CMat> onsite;
+    std::vector<Hopping> hoppings;
+    std::vector<std::function<bool(const Site&, CMat&)>> extra;
+
+    explicit Model(Lattice lat);
+    Model& set_onsite(int sub, const CMat& m);
+    Model& hopping(int from, int to, Cell offset, const CMat& m);
+    CMat onsite_matrix(const Site& s) const;
+    SparseMatrix sparse_hamiltonian(const Region& r) const;
+    CMat hamiltonian(const Region& r) const;
+    CMat coupling(const Region& a, const Region& b) const;
+
+   private:
+    template <class Sink>
+    void build(Sink& sink, const Region& r) const;
+    template <class Sink>
+    void couplings_one_way(Sink& sink, const Region& a, const Region& b) const;
+};
+
+/// BHZ parameters (crates/caroli-model/src/presets.rs:11-27).
+struct BhzParams {
+    double a = 1.0, b = -0.376406, c = 0.0, d = -0.280933, mass = -0.137, lambda = 0.0;
+};
+Model bhz(const BhzParams& p);
+Model square_model();
+Model cubic_model();
+Model graphene_model();
+/// 4x4 Gamma matrices of the BHZ basis (crates/caroli-model/src/gamma.rs).
+CMat gamma_matrix(const std::string& name);
+
+/// xoshiro256** with splitmix64 and gfortran seeding (crates/caroli-disorder/src/rng.rs:10-90).
+struct Xoshiro256 {
+    uint64_t s[4];
+    static Xoshiro256 seed_from_u64(uint64_t seed);
+    static Xoshiro256 gfortran_put(const int32_t put[8]);
+    static Xoshiro256 gfortran(int32_t seed);
+    uint64_t next_u64();
+    double uniform();
+    size_t integer(size_t lo, size_t hi);
+};
+
+/// Partial Fisher-Yates (crates/caroli-disorder/src/lib.rs:18-26).
+std::vector<Site> choose_sites(Xoshiro256& rng, const std::vector<Site>& cand, size_t n);
+
+/// A system as the transport engine receives it: the sparse Hamiltonian of the region, the
+/// geometry the elimination tree is cut from, and the two lead cells.
+struct SystemSpec {
+    SparseMatrix h;
+    std::vector<Cell> cells;      // per site
+    std::vector<int> site_orbitals;
+    std::vector<size_t> site_offsets;
+    int dim = 2;
+    std::array<int64_t, 3> range{1, 1, 1};  // hopping range per axis
+    int axis = 0;
+    CMat left_h00, left_h01, right_h00, right_h01;
+};
+
+/// What `NestedTwoTerminal::from_model(device, lead, region, axis)` reads from its models
+/// (crates/caroli-green/src/nested.rs:530-557, gauge.rs:81-98 without a field).
+SystemSpec system_from_models(const Model& device, const Model& lead, const Region& region, int axis);
+
+/// Named workloads: the model, region and seeded disorder of a preset system.
+///  model 0: BHZ (BhzParams::default());  1: square lattice, t = 1;
+///        2: zigzag graphene, t = 1 (crates/caroli-bench/src/main.rs:86-91);  3: cubic, t = 1.
+///  disorder 0: none;  1: Anderson `w (u - 0.5)` on every orbital of every site;
+///           2: magnetic impurities with polarized moments (m_z = 1), opposite g, exchange J.
+/// Disorder follows the `caroli run` seeding (crates/caroli-cli/src/build.rs:251-306):
+/// `Xoshiro256::seed_from_u64(seed)`, draws over all region sites in region order.
+struct PresetArgs {
+    int model = 0;
+    size_t extent[3] = {16, 10, 0};
+    int axis = 0;
+    int disorder = 0;
+    double strength = 0.0;       // Anderson width w, or exchange J
+    double concentration = 0.0;  // magnetic impurities per site
+    uint64_t seed = 1;
+    /// Optional extra potential per site (added to every orbital of the site), or null.
+    const double* site_potential = nullptr;
+};
+SystemSpec preset_system(const PresetArgs& a);
+
+}  // namespace caroli

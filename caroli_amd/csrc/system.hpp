@@ -1,0 +1,53 @@
+// The objects behind the C ABI handles: a system (NestedTwoTerminal) and its lead
+// self-energies (LeadSelfEnergies).
+#pragma once
+#include <memory>
+#include <mutex>
+
+#include "modes.hpp"
+#include "symbolic.hpp"
+
+namespace caroli {
+
+namespace gpu {
+struct Structure;  // device copy of the plans, shared by systems of one structure
+}
+
+/// Algorithmic work of one transmission point: 8 flops per complex multiply-add; 16-byte complex
+/// words for one assembly of each front plus one write and one read of its Schur complement.
+struct Work {
+    double flops = 0, bytes = 0;
+};
+Work algorithmic_work(const Symbolic& sym, size_t nnz);
+
+}  // namespace caroli
+
+struct cg_system {
+    caroli::SystemSpec spec;
+    caroli::Symbolic sym;
+    std::vector<caroli::ModeSolver> left_modes, right_modes;
+    /// Key of the symbolic structure: systems with equal keys share device plans.
+    uint64_t key = 0;
+    mutable std::mutex mu;
+    mutable std::shared_ptr<caroli::gpu::Structure> structure;
+    /// Device copy of the Hamiltonian values (complex f64), uploaded on first use.
+    mutable void* d_vals = nullptr;
+};
+
+struct cg_sigma {
+    struct Block {
+        caroli::CMat sl, sr, gl, gr;  // Sigma_L, Sigma_R and their broadenings
+        int channels_l = 0, channels_r = 0;
+        caroli::CMat left_h00, left_h01, right_h00, right_h01;
+        std::vector<size_t> left_orbitals, right_orbitals;
+    };
+    double energy = 0;
+    std::vector<Block> blocks;
+};
+
+namespace caroli {
+/// Gamma = i (Sigma - Sigma^H) (crates/caroli-green/src/lib.rs:68-71).
+CMat broadening(const CMat& s);
+/// True when `g` holds self-energies of the leads and blocks of `s` (nested.rs:693-707).
+bool sigma_matches(const cg_system& s, const cg_sigma& g);
+}  // namespace caroli

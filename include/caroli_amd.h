@@ -1,0 +1,137 @@
+/* C ABI of the MI355X nested-dissection transmission engine.
+ *
+ * It replaces, for a caller in another language (the reference's Rust crates would bind it with
+ * `extern "C"`), the nested-dissection path of crates/caroli-green/src/nested.rs:
+ *   NestedTwoTerminal::from_model          nested.rs:530   -> cg_system_create / cg_preset_create
+ *   NestedTwoTerminal::block_count         nested.rs:613   -> cg_block_count
+ *   NestedTwoTerminal::transmission        nested.rs:618   -> cg_transmission
+ *   NestedTwoTerminal::transmissions       nested.rs:623   -> cg_transmissions
+ *   NestedTwoTerminal::self_energies       nested.rs:652   -> cg_self_energies
+ *   NestedTwoTerminal::transmission_with   nested.rs:682   -> cg_transmission_with
+ *   ::transmission_with_precision          nested.rs:688   -> cg_transmission_with (precision)
+ *   ::transmissions_checked                nested.rs:630   -> cg_batch_transmission (per-point dT)
+ *   TwoTerminal::transmission              two_terminal.rs:186 -> cg_transmission (same engine)
+ * and adds the batched launch over systems x energies (the callers' rayon fan-out,
+ * crates/caroli-cli/src/run.rs:113-127) and the adjoint dT/dV (no counterpart in the reference).
+ *
+ * Conventions: complex numbers are interleaved (re, im) doubles; matrices are column-major;
+ * sizes are 64-bit; every function returns a status code (CG_OK or a CG_ERR_* value), whose
+ * meaning follows GreenError (crates/caroli-green/src/lib.rs:35-47). Handles are thread-safe for
+ * concurrent calls (device work is serialised internally). */
+#ifndef CAROLI_AMD_H
+#define CAROLI_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  CG_OK = 0,
+  CG_ERR_SINGULAR = 1,       /* GreenError::Singular(E): singular root or lead surface state */
+  CG_ERR_EIGEN = 2,          /* GreenError::Eigen */
+  CG_ERR_MODE_COUNT = 3,     /* GreenError::ModeCount */
+  CG_ERR_NO_CONVERGENCE = 4, /* refinement did not converge (mixed precision) */
+  CG_ERR_INVALID = 6,        /* malformed arguments (the reference panics, nested.rs:531-534, :694) */
+  CG_ERR_DEVICE = 7          /* HIP error, or no device */
+};
+
+/* Front precision (crates/caroli-green/src/precision.rs:24-32, plus the refined mixed mode). */
+enum {
+  CG_DOUBLE = 0, /* complex f64 fronts (Precision::Double) */
+  CG_SINGLE = 1, /* complex f32 fronts, unrefined (Precision::Single) */
+  CG_MIXED = 2   /* complex f32 fronts with FP64-residual iterative refinement */
+};
+
+typedef struct cg_system cg_system; /* NestedTwoTerminal */
+typedef struct cg_sigma cg_sigma;   /* LeadSelfEnergies (nested.rs:439-459) */
+
+/* NestedTwoTerminal::from_model (nested.rs:530-610) from the arrays the models produce:
+ * the CSR Hamiltonian of the region (crates/caroli-model/src/sparse.rs:13-19; vals interleaved),
+ * the sites (cells: 3 int64 per site; orbital count and first orbital per site), the lattice
+ * dimension, the hopping range per axis, the transport axis, and the lead cells
+ * (h00 = <cell|H|cell>, h01 = <cell|H|cell+axis>) of the left (first slice) and right (last
+ * slice) leads, nl x nl and nr x nr. */
+int32_t cg_system_create(uint64_t n, const int64_t* row_ptr, const uint32_t* cols, const double* vals,
+                         uint64_t nsites, const int64_t* cells, const int32_t* site_orbitals,
+                         const uint64_t* site_offsets, int32_t dim, const int64_t* range, int32_t axis,
+                         const double* left_h00, const double* left_h01, uint64_t nl,
+                         const double* right_h00, const double* right_h01, uint64_t nr, cg_system** out);
+
+/* Host builders of named workloads (models of crates/caroli-model/src/presets.rs and
+ * crates/caroli-bench/src/main.rs:77-92, disorder seeded as crates/caroli-cli/src/build.rs:251-306).
+ * model: 0 BHZ, 1 square, 2 zigzag graphene, 3 cubic.  disorder: 0 none, 1 Anderson (strength =
+ * width w), 2 polarized magnetic impurities, opposite g (strength = J, concentration).
+ * site_potential: optional extra on-site potential per site (null for none). */
+int32_t cg_preset_create(int32_t model, const uint64_t* extent, int32_t axis, int32_t disorder, double strength,
+                         double concentration, uint64_t seed, const double* site_potential, cg_system** out);
+
+void cg_system_destroy(cg_system* s);
+
+/* NestedTwoTerminal::block_count (nested.rs:613). */
+int32_t cg_block_count(const cg_system* s);
+
+/* Sizes: n orbitals, nnz, number of sites; per block b: nl, nr, tree nodes, tree levels. */
+int32_t cg_system_sizes(const cg_system* s, uint64_t* n, uint64_t* nnz, uint64_t* nsites);
+int32_t cg_block_sizes(const cg_system* s, int32_t b, uint64_t* nl, uint64_t* nr, uint64_t* nodes, uint64_t* levels);
+
+/* Copies of the system arrays (any pointer may be null): CSR, geometry, lead cells. */
+int32_t cg_system_arrays(const cg_system* s, int64_t* row_ptr, uint32_t* cols, double* vals, int64_t* cells,
+                         int32_t* site_orbitals, uint64_t* site_offsets, int32_t* dim, int64_t* range,
+                         int32_t* axis, double* left_h00, double* left_h01, double* right_h00, double* right_h01);
+
+/* Node k of block b's elimination tree: counts, and the sorted orbitals when arrays are given. */
+int32_t cg_tree_node(const cg_system* s, int32_t b, int32_t k, uint64_t* ne, uint64_t* nb, uint64_t* eliminate,
+                     uint64_t* boundary);
+
+/* Algorithmic work of one transmission point (complex MAC = 8 flops, 16-byte complex):
+ * flops of the fronts and root, and bytes of one assembly + one Schur write/read per front. */
+int32_t cg_work(const cg_system* s, double* flops, double* bytes);
+
+/* NestedTwoTerminal::self_energies (nested.rs:652-676): FP64 mode matching on the host. */
+int32_t cg_self_energies(const cg_system* s, double energy, cg_sigma** out);
+/* Self-energies given by the caller (per block: Sigma_L nl x nl then Sigma_R nr x nr). */
+int32_t cg_sigma_from_arrays(const cg_system* s, double energy, const double* sigmas, cg_sigma** out);
+/* Copies Sigma_L and Sigma_R of block b (either may be null); channels = propagating modes. */
+int32_t cg_sigma_block(const cg_sigma* g, int32_t b, double* sigma_l, double* sigma_r, int32_t* channels_l,
+                       int32_t* channels_r);
+double cg_sigma_energy(const cg_sigma* g);
+void cg_sigma_destroy(cg_sigma* g);
+
+/* NestedTwoTerminal::transmission (nested.rs:618) on the GPU (self-energies on the host). */
+int32_t cg_transmission(const cg_system* s, double energy, int32_t precision, double* out_t);
+/* transmission_with / transmission_with_precision (nested.rs:682-713); panics of the reference
+ * on foreign self-energies (nested.rs:694, :701-707) become CG_ERR_INVALID. */
+int32_t cg_transmission_with(const cg_system* s, const cg_sigma* g, int32_t precision, double* out_t);
+/* NestedTwoTerminal::transmissions (nested.rs:623-626): one batched launch. */
+int32_t cg_transmissions(const cg_system* s, const double* energies, int64_t count, int32_t precision,
+                         double* out_t, int32_t* out_status);
+
+/* Statistics of the last batched call on this thread. */
+typedef struct cg_batch_stats {
+  double seconds;           /* wall time of the device work (inputs resident) */
+  double front_flops;       /* algorithmic flops of the batch */
+  double front_bytes;       /* algorithmic bytes of the batch */
+  int64_t delayed_points;   /* points redone with delayed pivots */
+  int64_t refinement_iters; /* mixed precision: total refinement iterations */
+  double max_refine_resid;  /* mixed precision: largest final relative correction */
+} cg_batch_stats;
+
+/* The batched launch: point p is (systems[p], sigmas[p]); all energies x configurations (x
+ * systems of several shapes) run in one call, level by level over the elimination trees.
+ * out_t[p] = T or NaN; out_status[p] = status code. */
+int32_t cg_batch_transmission(const cg_system* const* systems, const cg_sigma* const* sigmas, int64_t count,
+                              int32_t precision, double* out_t, int32_t* out_status, cg_batch_stats* stats);
+
+/* Adjoint: T and dT/d(eps_a) for every orbital a of the device (n values, on-site energies),
+ * from one backward sweep over the stored fronts (no counterpart in the reference). */
+int32_t cg_backward(const cg_system* s, const cg_sigma* g, double* out_t, double* out_grad);
+
+/* Device present and usable (1) or not (0). */
+int32_t cg_device_available(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
