@@ -298,3 +298,40 @@ def batch_transmission(systems, sigmas, precision=Precision.Double, stats=None):
     _check(capi.cg_batch_transmission(sh, gh, n, int(precision), _dptr(t), st.ctypes.data_as(C.POINTER(C.c_int32)),
                                       C.byref(bs)))
     return t, st
+
+
+class ResidentBatch:
+    """Points whose inputs stay in HBM between launches (cg_batch_create / cg_batch_run)."""
+
+    def __init__(self, systems, sigmas, precision=Precision.Double):
+        self._keep = (list(systems), list(sigmas))
+        n = len(systems)
+        self.n = n
+        sh = (C.c_void_p * n)(*[s._h.value for s in systems])
+        gh = (C.c_void_p * n)(*[g._h.value for g in sigmas])
+        h = C.c_void_p()
+        _check(capi.cg_batch_create(sh, gh, n, int(precision), C.byref(h)), "batch")
+        self._h = h
+
+    def __del__(self):
+        if getattr(self, "_h", None) and self._h.value:
+            capi.cg_batch_destroy(self._h)
+            self._h = C.c_void_p(None)
+
+    def run(self, stats=None):
+        t = np.zeros(self.n)
+        st = np.zeros(self.n, np.int32)
+        bs = stats if stats is not None else capi.BatchStats()
+        _check(capi.cg_batch_run(self._h, _dptr(t), st.ctypes.data_as(C.POINTER(C.c_int32)), C.byref(bs)))
+        return t, st
+
+
+def kernel_timing(enable=True):
+    _check(capi.cg_kernel_timing(1 if enable else 0))
+
+
+def kernel_stats():
+    """Per kernel family: dict name -> (launches, seconds, algorithmic flops, algorithmic bytes)."""
+    buf = (capi.KernelStat * 16)()
+    n = capi.cg_kernel_stats(buf, 16)
+    return {buf[i].name.decode(): (buf[i].launches, buf[i].seconds, buf[i].flops, buf[i].bytes) for i in range(n)}

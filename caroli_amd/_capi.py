@@ -75,3 +75,23 @@ EXPORTS = [
     "cg_sigma_block", "cg_sigma_energy", "cg_sigma_destroy", "cg_transmission", "cg_transmission_with",
     "cg_transmissions", "cg_batch_transmission", "cg_backward", "cg_device_available",
 ]
+
+
+class KernelStat(C.Structure):
+    _fields_ = [
+        ("name", C.c_char * 48),
+        ("launches", C.c_int64),
+        ("seconds", C.c_double),
+        ("flops", C.c_double),
+        ("bytes", C.c_double),
+    ]
+
+
+cg_batch_create = _sig("cg_batch_create", C.c_int32, P(c_sys), P(c_sig), C.c_int64, C.c_int32, P(C.c_void_p))
+cg_batch_run = _sig("cg_batch_run", C.c_int32, C.c_void_p, P(C.c_double), P(C.c_int32), P(BatchStats))
+cg_batch_destroy = _sig("cg_batch_destroy", None, C.c_void_p)
+cg_kernel_timing = _sig("cg_kernel_timing", C.c_int32, C.c_int32)
+cg_kernel_stats = _sig("cg_kernel_stats", C.c_int32, P(KernelStat), C.c_int32)
+cg_uniforms = _sig("cg_uniforms", C.c_int32, C.c_int32, C.c_uint64, P(C.c_int32), C.c_int64, P(C.c_double))
+EXPORTS += ["cg_batch_create", "cg_batch_run", "cg_batch_destroy", "cg_kernel_timing", "cg_kernel_stats",
+            "cg_uniforms"]

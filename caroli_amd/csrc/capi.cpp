@@ -196,6 +196,15 @@ void cg_system_destroy(cg_system* s) {
     delete s;
 }
 
+int32_t cg_uniforms(int32_t mode, uint64_t seed, const int32_t* put, int64_t count, double* out) {
+    if (count < 0 || (count > 0 && !out) || mode < 0 || mode > 2 || (mode == 2 && !put)) return CG_ERR_INVALID;
+    Xoshiro256 r = mode == 0 ? Xoshiro256::seed_from_u64(seed)
+                   : mode == 1 ? Xoshiro256::gfortran(int32_t(seed))
+                               : Xoshiro256::gfortran_put(put);
+    for (int64_t i = 0; i < count; ++i) out[i] = r.uniform();
+    return CG_OK;
+}
+
 int32_t cg_block_count(const cg_system* s) { return s ? int32_t(s->sym.blocks.size()) : -1; }
 
 int32_t cg_system_sizes(const cg_system* s, uint64_t* n, uint64_t* nnz, uint64_t* nsites) {
@@ -357,6 +366,43 @@ int32_t cg_batch_transmission(const cg_system* const* systems, const cg_sigma* c
     } catch (const std::exception&) {
         return CG_ERR_DEVICE;
     }
+}
+
+int32_t cg_batch_create(const cg_system* const* systems, const cg_sigma* const* sigmas, int64_t count, int32_t precision,
+                        cg_batch** out) {
+    if (!out || count < 0 || (count > 0 && (!systems || !sigmas))) return CG_ERR_INVALID;
+    if (precision < CG_DOUBLE || precision > CG_MIXED) return CG_ERR_INVALID;
+    for (int64_t p = 0; p < count; ++p)
+        if (!systems[p] || !sigmas[p] || !sigma_matches(*systems[p], *sigmas[p])) return CG_ERR_INVALID;
+    try {
+        return gpu::create_batch(systems, sigmas, count, precision, out);
+    } catch (const std::exception&) {
+        return CG_ERR_DEVICE;
+    }
+}
+
+int32_t cg_batch_run(cg_batch* batch, double* out_t, int32_t* out_status, cg_batch_stats* stats) {
+    if (!batch || !out_t) return CG_ERR_INVALID;
+    try {
+        return gpu::run_batch(batch, out_t, out_status, stats);
+    } catch (const std::exception&) {
+        return CG_ERR_DEVICE;
+    }
+}
+
+void cg_batch_destroy(cg_batch* batch) { gpu::destroy_batch(batch); }
+
+int32_t cg_kernel_timing(int32_t enable) {
+    try {
+        return gpu::kernel_timing(enable);
+    } catch (const std::exception&) {
+        return CG_ERR_DEVICE;
+    }
+}
+
+int32_t cg_kernel_stats(cg_kernel_stat* out, int32_t max_records) {
+    if (!out || max_records < 0) return CG_ERR_INVALID;
+    return gpu::kernel_stats(out, max_records);
 }
 
 int32_t cg_transmission_with(const cg_system* s, const cg_sigma* g, int32_t precision, double* out_t) {

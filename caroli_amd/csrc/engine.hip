@@ -8,6 +8,8 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cstdio>
+#include <memory>
 #include <cmath>
 #include <map>
 #include <mutex>
@@ -65,7 +67,10 @@ struct DevBuf {
 template <class T>
 void upload(DevBuf& d, const std::vector<T>& v, hipStream_t st) {
     d.ensure(v.size() * sizeof(T));
-    if (!v.empty()) HIP_CHECK(hipMemcpyAsync(d.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, st));
+    if (v.empty()) return;
+    // The host vector may die right after: wait for the copy.
+    HIP_CHECK(hipMemcpyAsync(d.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, st));
+    HIP_CHECK(hipStreamSynchronize(st));
 }
 
 /// Descriptors of one block's fronts (tree nodes, then the root), in host memory.
@@ -80,10 +85,15 @@ struct HostPlan {
     int64_t arena_stride = 0, root_stride = 0, piv_stride = 0;
     int32_t slots = 0;
     int32_t root = 0;  // index of the root descriptor
+    /// Orbital of every front variable (NodeDesc::var_off), for the solves.
+    std::vector<int32_t> vars;
+    /// The block's orbitals, and each orbital's position in the left / right slice (-1 if none).
+    std::vector<int32_t> rows, left_pos, right_pos;
+    int32_t max_ne = 0;  // largest ne of a front or the root
 };
 
 struct DevPlan {
-    DevBuf nodes, ent, diag, kids, maps, levels;
+    DevBuf nodes, ent, diag, kids, maps, levels, vars, rows, left_pos, right_pos, row_ptr, cols;
     // Ranges into `levels` for each list of each level.
     std::vector<std::pair<int32_t, int32_t>> all, s128, l128, s64, l64;
     int32_t root_at = 0;  // position of the root index in `levels`
@@ -163,6 +173,9 @@ HostPlan build_plan(const Block& blk, const SparseMatrix& h) {
         d.kid_end = int32_t(hp.kids.size());
         d.slot = k;
         d.piv_off = piv;
+        d.var_off = int64_t(hp.vars.size());
+        for (size_t o : pl.vars) hp.vars.push_back(int32_t(o));
+        hp.max_ne = std::max(hp.max_ne, d.ne);
         off += int64_t(d.n) * d.n;
         piv += d.ne;
         hp.nodes.push_back(d);
@@ -190,8 +203,20 @@ HostPlan build_plan(const Block& blk, const SparseMatrix& h) {
     r.kid_end = int32_t(hp.kids.size());
     r.slot = nn;
     r.piv_off = piv;
+    r.var_off = int64_t(hp.vars.size());
+    for (size_t o : blk.kept) hp.vars.push_back(int32_t(o));
+    hp.max_ne = std::max(hp.max_ne, r.ne);
     piv += r.ne;
     hp.root = nn;
+    // Orbitals of the block and their slice positions (residuals of the refinement).
+    for (auto& node : t.nodes) hp.rows.insert(hp.rows.end(), node.eliminate.begin(), node.eliminate.end());
+    hp.rows.insert(hp.rows.end(), blk.kept.begin(), blk.kept.end());
+    std::sort(hp.rows.begin(), hp.rows.end());
+    hp.rows.erase(std::unique(hp.rows.begin(), hp.rows.end()), hp.rows.end());
+    hp.left_pos.assign(h.n, -1);
+    hp.right_pos.assign(h.n, -1);
+    for (size_t a = 0; a < blk.left_orbitals.size(); ++a) hp.left_pos[blk.left_orbitals[a]] = int32_t(a);
+    for (size_t a = 0; a < blk.right_orbitals.size(); ++a) hp.right_pos[blk.right_orbitals[a]] = int32_t(a);
     hp.nodes.push_back(r);
     hp.root_stride = int64_t(r.n) * r.n;
     hp.piv_stride = std::max<int64_t>(piv, 1);
@@ -207,7 +232,13 @@ HostPlan build_plan(const Block& blk, const SparseMatrix& h) {
     return hp;
 }
 
-void upload_plan(const HostPlan& hp, DevPlan& dp, hipStream_t st) {
+void upload_plan(const HostPlan& hp, DevPlan& dp, const SparseMatrix& h, hipStream_t st) {
+    upload(dp.vars, hp.vars, st);
+    upload(dp.rows, hp.rows, st);
+    upload(dp.left_pos, hp.left_pos, st);
+    upload(dp.right_pos, hp.right_pos, st);
+    upload(dp.row_ptr, h.row_ptr, st);
+    upload(dp.cols, h.cols, st);
     upload(dp.nodes, hp.nodes, st);
     upload(dp.ent, hp.ent, st);
     upload(dp.diag, hp.diag, st);
@@ -230,13 +261,38 @@ void upload_plan(const HostPlan& hp, DevPlan& dp, hipStream_t st) {
     upload(dp.levels, lv, st);
 }
 
+/// Device copies of the per-point inputs of one launch unit.
+struct Inputs {
+    DevBuf vals_ptrs, z, sigma, gamma;
+    int64_t stride = 0;
+};
+
+/// Kernel families timed by cg_kernel_timing.
+enum Family { kAssemble, kFactorLds, kFactorGlobal, kRootAssemble, kRootFactor, kTrace, kFamilies };
+const char* const kFamilyNames[kFamilies] = {"assemble_kernel", "factor_small_kernel", "factor_global_kernel",
+                                             "assemble_kernel<root>", "factor_global_kernel<root>", "trace_kernel"};
+
 struct Engine {
     std::mutex mu;
     hipStream_t stream = nullptr;
     bool ok = false;
     std::map<uint64_t, std::weak_ptr<Structure>> registry;
-    DevBuf arena, root_arena, piv, status, tout, vals_ptrs, z, sigma, gamma;
+    DevBuf arena, root_arena, piv, status, tout;
+    Inputs scratch_in;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // Per-kernel timing.
+    bool timing = false;
+    struct Fam {
+        int64_t launches = 0;
+        double seconds = 0, flops = 0, bytes = 0;
+    } fams[kFamilies];
+    struct Pending {
+        int fam;
+        hipEvent_t a, b;
+        double flops, bytes;
+    };
+    std::vector<Pending> pending;
+    std::vector<hipEvent_t> pool;
 
     Engine() {
         int count = 0;
@@ -248,6 +304,44 @@ struct Engine {
     static Engine& get() {
         static Engine e;
         return e;
+    }
+    hipEvent_t take() {
+        if (pool.empty()) {
+            hipEvent_t e;
+            HIP_CHECK(hipEventCreate(&e));
+            return e;
+        }
+        hipEvent_t e = pool.back();
+        pool.pop_back();
+        return e;
+    }
+    /// Runs `launch`, bracketed by events when timing is on.
+    template <class F>
+    void timed(int fam, double flops, double bytes, F&& launch) {
+        if (!timing) {
+            launch();
+            return;
+        }
+        hipEvent_t a = take(), b = take();
+        HIP_CHECK(hipEventRecord(a, stream));
+        launch();
+        HIP_CHECK(hipEventRecord(b, stream));
+        pending.push_back({fam, a, b, flops, bytes});
+    }
+    /// Folds the finished timed launches into the counters (after a stream synchronisation).
+    void collect() {
+        for (auto& p : pending) {
+            float ms = 0.f;
+            HIP_CHECK(hipEventElapsedTime(&ms, p.a, p.b));
+            auto& f = fams[p.fam];
+            f.launches += 1;
+            f.seconds += ms * 1e-3;
+            f.flops += p.flops;
+            f.bytes += p.bytes;
+            pool.push_back(p.a);
+            pool.push_back(p.b);
+        }
+        pending.clear();
     }
 };
 
@@ -263,7 +357,7 @@ std::shared_ptr<Structure> structure_of(Engine& eng, const cg_system* s) {
     auto sp = std::make_shared<Structure>();
     for (auto& blk : s->sym.blocks) sp->host.push_back(build_plan(blk, s->spec.h));
     sp->dev.resize(sp->host.size());
-    for (size_t b = 0; b < sp->host.size(); ++b) upload_plan(sp->host[b], sp->dev[b], eng.stream);
+    for (size_t b = 0; b < sp->host.size(); ++b) upload_plan(sp->host[b], sp->dev[b], s->spec.h, eng.stream);
     HIP_CHECK(hipStreamSynchronize(eng.stream));
     eng.registry[s->key] = sp;
     s->structure = sp;
@@ -281,9 +375,17 @@ const double2* vals_of(Engine& eng, const cg_system* s) {
     return static_cast<const double2*>(s->d_vals);
 }
 
+void bind_inputs(const Inputs& in, PointsView& pv) {
+    pv.vals = in.vals_ptrs.as<const double2* const>();
+    pv.z = in.z.as<const double2>();
+    pv.sigma = in.sigma.as<const double2>();
+    pv.gamma = in.gamma.as<const double2>();
+    pv.sigma_stride = in.stride;
+}
+
 /// Per-point inputs of one block: energies, self-energies and broadenings, Hamiltonian values.
 void stage_points(Engine& eng, const std::vector<const cg_system*>& sys, const std::vector<const cg_sigma*>& sig, int b,
-                  int nl, int nr, PointsView& pv) {
+                  int nl, int nr, Inputs& in, PointsView& pv) {
     const size_t P = sys.size();
     const int64_t stride = int64_t(nl) * nl + int64_t(nr) * nr;
     std::vector<const double2*> vp(P);
@@ -301,15 +403,37 @@ void stage_points(Engine& eng, const std::vector<const cg_system*>& sys, const s
         put(sg, blk.sl, blk.sr);
         put(gm, blk.gl, blk.gr);
     }
-    upload(eng.vals_ptrs, vp, eng.stream);
-    upload(eng.z, z, eng.stream);
-    upload(eng.sigma, sg, eng.stream);
-    upload(eng.gamma, gm, eng.stream);
-    pv.vals = eng.vals_ptrs.as<const double2* const>();
-    pv.z = eng.z.as<const double2>();
-    pv.sigma = eng.sigma.as<const double2>();
-    pv.gamma = eng.gamma.as<const double2>();
-    pv.sigma_stride = stride;
+    upload(in.vals_ptrs, vp, eng.stream);
+    upload(in.z, z, eng.stream);
+    upload(in.sigma, sg, eng.stream);
+    upload(in.gamma, gm, eng.stream);
+    in.stride = stride;
+    bind_inputs(in, pv);
+}
+
+/// Algorithmic flops (8 per complex MAC) and bytes (one read and one write of the front) of
+/// eliminating the fronts `list`; bytes of assembling them (front written, children read).
+double list_flops(const std::vector<NodeDesc>& nodes, const std::vector<int32_t>& list) {
+    double f = 0;
+    for (int32_t k : list) {
+        double ne = nodes[k].ne, nb = nodes[k].n - nodes[k].ne;
+        f += 8.0 / 3.0 * ne * ne * ne + 8.0 * ne * ne * nb + 8.0 * ne * nb * nb;
+    }
+    return f;
+}
+double list_factor_bytes(const std::vector<NodeDesc>& nodes, const std::vector<int32_t>& list, double es) {
+    double b = 0;
+    for (int32_t k : list) b += 2.0 * es * double(nodes[k].n) * nodes[k].n;
+    return b;
+}
+double list_assemble_bytes(const HostPlan& hp, const std::vector<int32_t>& list, double es, double kid_es) {
+    double b = 0;
+    for (int32_t k : list) {
+        const NodeDesc& d = hp.nodes[k];
+        b += es * double(d.n) * d.n + 32.0 * (d.ent_end - d.ent_beg);
+        for (int c = d.kid_beg; c < d.kid_end; ++c) b += kid_es * double(hp.kids[c].size) * hp.kids[c].size;
+    }
+    return b;
 }
 
 /// One sweep over the static plan of block b for the points of `pv`.
@@ -320,21 +444,47 @@ void sweep(Engine& eng, const HostPlan& hp, const DevPlan& dp, const PointsView&
     const int32_t* diag = dp.diag.as<int32_t>();
     const KidDesc* kids = dp.kids.as<KidDesc>();
     const int32_t* maps = dp.maps.as<int32_t>();
+    const double es = single ? 8.0 : 16.0;
+    const bool t = eng.timing;
     for (size_t l = 0; l < dp.all.size(); ++l) {
         auto [a0, ac] = dp.all[l];
-        if (single) {
-            launch_assemble_c64(nodes, lv + a0, ac, ent, diag, kids, maps, pv, P, eng.stream);
-            launch_factor_c64(nodes, lv + dp.s64[l].first, dp.s64[l].second, pv, P, kTolSingle, true, eng.stream);
-            launch_factor_c64(nodes, lv + dp.l64[l].first, dp.l64[l].second, pv, P, kTolSingle, false, eng.stream);
-        } else {
-            launch_assemble_c128(nodes, lv + a0, ac, ent, diag, kids, maps, pv, P, eng.stream);
-            launch_factor_c128(nodes, lv + dp.s128[l].first, dp.s128[l].second, pv, P, kTolDouble, true, false, eng.stream);
-            launch_factor_c128(nodes, lv + dp.l128[l].first, dp.l128[l].second, pv, P, kTolDouble, false, false, eng.stream);
-        }
+        auto& sm = single ? hp.small64[l] : hp.small128[l];
+        auto& lg = single ? hp.large64[l] : hp.large128[l];
+        auto& rs = single ? dp.s64[l] : dp.s128[l];
+        auto& rl = single ? dp.l64[l] : dp.l128[l];
+        eng.timed(kAssemble, 0.0, t ? P * list_assemble_bytes(hp, hp.level_all[l], es, es) : 0.0, [&] {
+            if (single)
+                launch_assemble_c64(nodes, lv + a0, ac, ent, diag, kids, maps, pv, P, eng.stream);
+            else
+                launch_assemble_c128(nodes, lv + a0, ac, ent, diag, kids, maps, pv, P, eng.stream);
+        });
+        if (rs.second > 0)
+            eng.timed(kFactorLds, t ? P * list_flops(hp.nodes, sm) : 0.0, t ? P * list_factor_bytes(hp.nodes, sm, es) : 0.0, [&] {
+                if (single)
+                    launch_factor_c64(nodes, lv + rs.first, rs.second, pv, P, kTolSingle, true, eng.stream);
+                else
+                    launch_factor_c128(nodes, lv + rs.first, rs.second, pv, P, kTolDouble, true, false, eng.stream);
+            });
+        if (rl.second > 0)
+            eng.timed(kFactorGlobal, t ? P * list_flops(hp.nodes, lg) : 0.0, t ? P * list_factor_bytes(hp.nodes, lg, es) : 0.0, [&] {
+                if (single)
+                    launch_factor_c64(nodes, lv + rl.first, rl.second, pv, P, kTolSingle, false, eng.stream);
+                else
+                    launch_factor_c128(nodes, lv + rl.first, rl.second, pv, P, kTolDouble, false, false, eng.stream);
+            });
     }
-    launch_assemble_root(nodes, lv + dp.root_at, 1, ent, diag, kids, maps, pv, P, single ? 1 : 0, eng.stream);
-    launch_factor_c128(nodes, lv + dp.root_at, 1, pv, P, kTolDouble, false, true, eng.stream);
-    launch_trace(nodes + hp.root, pv, P, eng.stream);
+    const NodeDesc& r = hp.nodes[hp.root];
+    const double k = r.ne, nl = r.nl, nr = r.nr;
+    std::vector<int32_t> rootl{hp.root};
+    eng.timed(kRootAssemble, 0.0, t ? P * list_assemble_bytes(hp, rootl, 16.0, es) : 0.0, [&] {
+        launch_assemble_root(nodes, lv + dp.root_at, 1, ent, diag, kids, maps, pv, P, single ? 1 : 0, eng.stream);
+    });
+    eng.timed(kRootFactor, P * (8.0 / 3.0 * k * k * k + 8.0 * k * k * nr), P * 32.0 * double(r.n) * r.n, [&] {
+        launch_factor_c128(nodes, lv + dp.root_at, 1, pv, P, kTolDouble, false, true, eng.stream);
+    });
+    eng.timed(kTrace, P * 8.0 * nl * nr * (nl + nr), P * 16.0 * (nl * nl + nr * nr + nl * nr), [&] {
+        launch_trace(nodes + hp.root, pv, P, eng.stream);
+    });
     HIP_CHECK(hipGetLastError());
 }
 
@@ -352,13 +502,12 @@ struct Handed {
 int32_t replanned_point(Engine& eng, const cg_system* s, const cg_sigma* g, int b, bool single, double& t_out) {
     const Block& blk = s->sym.blocks[b];
     const Tree& t = blk.tree;
-    const HostPlan& stat = s->structure->host[b];
     const int nn = int(t.nodes.size());
     const size_t es = single ? sizeof(float2) : sizeof(double2);
     std::vector<const cg_system*> one_s{s};
     std::vector<const cg_sigma*> one_g{g};
     PointsView pv{};
-    stage_points(eng, one_s, one_g, b, int(blk.left_orbitals.size()), int(blk.right_orbitals.size()), pv);
+    stage_points(eng, one_s, one_g, b, int(blk.left_orbitals.size()), int(blk.right_orbitals.size()), eng.scratch_in, pv);
     DevBuf arena, piv, status, tout, root, dn, dent, ddiag, dkids, dmaps, dlv;
     int64_t used = 0, piv_used = 0;
     auto grow = [&](DevBuf& buf, int64_t need_bytes, int64_t keep_bytes) {
@@ -487,6 +636,23 @@ int32_t replanned_point(Engine& eng, const cg_system* s, const cg_sigma* g, int 
             HIP_CHECK(hipGetLastError());
             HIP_CHECK(hipMemcpyAsync(st_host.data(), status.p, sizeof(int32_t) * (nn + 1), hipMemcpyDeviceToHost, eng.stream));
             HIP_CHECK(hipStreamSynchronize(eng.stream));
+            // A delayed front moves up as assembled (nested.rs:371-372): the fronts whose pivot test
+            // failed part-way are assembled again over their partial elimination.
+            std::vector<int32_t> failed;
+            for (size_t q = 0; q < nodes.size(); ++q)
+                if (st_host[which[q]] != kFrontOk) failed.push_back(int32_t(q));
+            if (!failed.empty()) {
+                DevBuf df;
+                upload(df, failed, eng.stream);
+                if (single)
+                    launch_assemble_c64(dd, df.as<int32_t>(), int32_t(failed.size()), dent.as<Entry>(), ddiag.as<int32_t>(),
+                                        dkids.as<KidDesc>(), dmaps.as<int32_t>(), pv, 1, eng.stream);
+                else
+                    launch_assemble_c128(dd, df.as<int32_t>(), int32_t(failed.size()), dent.as<Entry>(), ddiag.as<int32_t>(),
+                                         dkids.as<KidDesc>(), dmaps.as<int32_t>(), pv, 1, eng.stream);
+                HIP_CHECK(hipGetLastError());
+                HIP_CHECK(hipStreamSynchronize(eng.stream));
+            }
             for (size_t q = 0; q < nodes.size(); ++q) {
                 int k = which[q];
                 const NodeDesc& d = nodes[q];
@@ -530,93 +696,135 @@ void release(const cg_system* s) {
     }
 }
 
-int32_t batch_transmission(const cg_system* const* systems, const cg_sigma* const* sigmas, int64_t count,
-                           int32_t precision, double* out_t, int32_t* out_status, cg_batch_stats* stats) {
+}  // namespace caroli::gpu
+
+/// A resident batch: launch units (structure, block, chunk of points) with their inputs in HBM.
+struct cg_batch {
+    struct Unit {
+        std::shared_ptr<caroli::gpu::Structure> structure;
+        int block;
+        std::vector<int64_t> points;
+        caroli::gpu::Inputs in;
+    };
+    std::vector<const cg_system*> systems;
+    std::vector<const cg_sigma*> sigmas;
+    int32_t precision = CG_DOUBLE;
+    std::vector<Unit> units;
+    double flops = 0, bytes = 0;
+};
+
+namespace caroli::gpu {
+
+int32_t create_batch(const cg_system* const* systems, const cg_sigma* const* sigmas, int64_t count, int32_t precision,
+                     cg_batch** out) {
     Engine& eng = Engine::get();
     if (!eng.ok) return CG_ERR_DEVICE;
     if (precision == CG_MIXED) return CG_ERR_INVALID;  // not available in this engine yet
     std::lock_guard<std::mutex> lock(eng.mu);
+    auto bt = std::make_unique<cg_batch>();
+    bt->systems.assign(systems, systems + count);
+    bt->sigmas.assign(sigmas, sigmas + count);
+    bt->precision = precision;
     const bool single = precision == CG_SINGLE;
     const size_t es = single ? sizeof(float2) : sizeof(double2);
-    cg_batch_stats local{};
-    for (int64_t p = 0; p < count; ++p) {
-        out_t[p] = 0.0;
-        if (out_status) out_status[p] = CG_OK;
-    }
-    // Group the points by structure.
     std::map<uint64_t, std::vector<int64_t>> groups;
     for (int64_t p = 0; p < count; ++p) groups[systems[p]->key].push_back(p);
-    std::vector<int32_t> status(count, CG_OK);
     for (auto& [key, idx] : groups) {
         auto sp = structure_of(eng, systems[idx[0]]);
         for (auto p : idx) structure_of(eng, systems[p]);
         Work w = algorithmic_work(systems[idx[0]]->sym, systems[idx[0]]->spec.h.vals.size());
-        local.front_flops += w.flops * double(idx.size());
-        local.front_bytes += w.bytes * double(idx.size()) * (single ? 0.5 : 1.0);
+        bt->flops += w.flops * double(idx.size());
+        bt->bytes += w.bytes * double(idx.size()) * (single ? 0.5 : 1.0);
         for (size_t b = 0; b < sp->host.size(); ++b) {
             const HostPlan& hp = sp->host[b];
-            const DevPlan& dp = sp->dev[b];
             const Block& blk = systems[idx[0]]->sym.blocks[b];
-            // Points per launch: bounded by the arena budget and the grid's y dimension.
+            // Points per launch: bounded by the scratch budget and the grid's y dimension.
             const size_t per_point = size_t(hp.arena_stride) * es + size_t(hp.root_stride) * sizeof(double2) +
                                      size_t(hp.piv_stride) * 4 + size_t(hp.slots) * 4 + 64;
             const size_t budget = size_t(96) << 30;
             size_t chunk = std::max<size_t>(1, std::min<size_t>({budget / per_point, 65535, idx.size()}));
             for (size_t c0 = 0; c0 < idx.size(); c0 += chunk) {
                 size_t P = std::min(chunk, idx.size() - c0);
+                cg_batch::Unit u;
+                u.structure = sp;
+                u.block = int(b);
                 std::vector<const cg_system*> ss;
                 std::vector<const cg_sigma*> gg;
                 for (size_t q = 0; q < P; ++q) {
+                    u.points.push_back(idx[c0 + q]);
                     ss.push_back(systems[idx[c0 + q]]);
                     gg.push_back(sigmas[idx[c0 + q]]);
                 }
                 PointsView pv{};
-                stage_points(eng, ss, gg, int(b), int(blk.left_orbitals.size()), int(blk.right_orbitals.size()), pv);
+                stage_points(eng, ss, gg, int(b), int(blk.left_orbitals.size()), int(blk.right_orbitals.size()), u.in, pv);
                 eng.arena.ensure(P * size_t(hp.arena_stride) * es);
                 eng.root_arena.ensure(P * size_t(hp.root_stride) * sizeof(double2));
                 eng.piv.ensure(P * size_t(hp.piv_stride) * 4);
                 eng.status.ensure(P * size_t(hp.slots) * 4);
                 eng.tout.ensure(P * sizeof(double));
-                pv.arena = eng.arena.p;
-                pv.arena_stride = hp.arena_stride;
-                pv.root_arena = eng.root_arena.p;
-                pv.root_stride = hp.root_stride;
-                pv.piv = eng.piv.as<int32_t>();
-                pv.piv_stride = hp.piv_stride;
-                pv.status = eng.status.as<int32_t>();
-                pv.status_stride = hp.slots;
-                pv.t_out = eng.tout.as<double>();
-                HIP_CHECK(hipEventRecord(eng.ev0, eng.stream));
-                sweep(eng, hp, dp, pv, int32_t(P), single);
-                HIP_CHECK(hipEventRecord(eng.ev1, eng.stream));
-                std::vector<double> t(P);
-                std::vector<int32_t> st(P * size_t(hp.slots));
-                HIP_CHECK(hipMemcpyAsync(t.data(), eng.tout.p, P * sizeof(double), hipMemcpyDeviceToHost, eng.stream));
-                HIP_CHECK(hipMemcpyAsync(st.data(), eng.status.p, st.size() * 4, hipMemcpyDeviceToHost, eng.stream));
-                HIP_CHECK(hipStreamSynchronize(eng.stream));
-                float ms = 0.f;
-                HIP_CHECK(hipEventElapsedTime(&ms, eng.ev0, eng.ev1));
-                local.seconds += ms * 1e-3;
-                for (size_t q = 0; q < P; ++q) {
-                    int64_t p = idx[c0 + q];
-                    bool delayed = false;
-                    int32_t root_status = st[q * hp.slots + hp.root];
-                    for (int32_t k = 0; k < hp.root; ++k) delayed |= st[q * hp.slots + k] != kFrontOk;
-                    if (delayed) {
-                        double tb = NAN;
-                        auto t0 = std::chrono::steady_clock::now();
-                        int32_t rc = replanned_point(eng, systems[p], sigmas[p], int(b), single, tb);
-                        local.seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-                        ++local.delayed_points;
-                        if (rc != CG_OK) status[p] = rc;
-                        out_t[p] += tb;
-                    } else if (root_status != kFrontOk || !std::isfinite(t[q])) {
-                        status[p] = CG_ERR_SINGULAR;
-                        out_t[p] = NAN;
-                    } else {
-                        out_t[p] += t[q];
-                    }
-                }
+                bt->units.push_back(std::move(u));
+            }
+        }
+    }
+    *out = bt.release();
+    return CG_OK;
+}
+
+int32_t run_batch(cg_batch* bt, double* out_t, int32_t* out_status, cg_batch_stats* stats) {
+    Engine& eng = Engine::get();
+    if (!eng.ok) return CG_ERR_DEVICE;
+    std::lock_guard<std::mutex> lock(eng.mu);
+    const bool single = bt->precision == CG_SINGLE;
+    const int64_t count = int64_t(bt->systems.size());
+    cg_batch_stats local{};
+    local.front_flops = bt->flops;
+    local.front_bytes = bt->bytes;
+    std::vector<int32_t> status(count, CG_OK);
+    for (int64_t p = 0; p < count; ++p) out_t[p] = 0.0;
+    for (auto& u : bt->units) {
+        const HostPlan& hp = u.structure->host[u.block];
+        const DevPlan& dp = u.structure->dev[u.block];
+        const size_t P = u.points.size();
+        PointsView pv{};
+        bind_inputs(u.in, pv);
+        pv.arena = eng.arena.p;
+        pv.arena_stride = hp.arena_stride;
+        pv.root_arena = eng.root_arena.p;
+        pv.root_stride = hp.root_stride;
+        pv.piv = eng.piv.as<int32_t>();
+        pv.piv_stride = hp.piv_stride;
+        pv.status = eng.status.as<int32_t>();
+        pv.status_stride = hp.slots;
+        pv.t_out = eng.tout.as<double>();
+        HIP_CHECK(hipEventRecord(eng.ev0, eng.stream));
+        sweep(eng, hp, dp, pv, int32_t(P), single);
+        HIP_CHECK(hipEventRecord(eng.ev1, eng.stream));
+        std::vector<double> t(P);
+        std::vector<int32_t> st(P * size_t(hp.slots));
+        HIP_CHECK(hipMemcpyAsync(t.data(), eng.tout.p, P * sizeof(double), hipMemcpyDeviceToHost, eng.stream));
+        HIP_CHECK(hipMemcpyAsync(st.data(), eng.status.p, st.size() * 4, hipMemcpyDeviceToHost, eng.stream));
+        HIP_CHECK(hipStreamSynchronize(eng.stream));
+        eng.collect();
+        float ms = 0.f;
+        HIP_CHECK(hipEventElapsedTime(&ms, eng.ev0, eng.ev1));
+        local.seconds += ms * 1e-3;
+        for (size_t q = 0; q < P; ++q) {
+            int64_t p = u.points[q];
+            bool delayed = false;
+            int32_t root_status = st[q * hp.slots + hp.root];
+            for (int32_t k = 0; k < hp.root; ++k) delayed |= st[q * hp.slots + k] != kFrontOk;
+            if (delayed) {
+                double tb = NAN;
+                auto t0 = std::chrono::steady_clock::now();
+                int32_t rc = replanned_point(eng, bt->systems[p], bt->sigmas[p], u.block, single, tb);
+                local.seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+                ++local.delayed_points;
+                if (rc != CG_OK) status[p] = rc;
+                out_t[p] += tb;
+            } else if (root_status != kFrontOk || !std::isfinite(t[q])) {
+                status[p] = CG_ERR_SINGULAR;
+            } else {
+                out_t[p] += t[q];
             }
         }
     }
@@ -626,6 +834,42 @@ int32_t batch_transmission(const cg_system* const* systems, const cg_sigma* cons
     }
     if (stats) *stats = local;
     return CG_OK;
+}
+
+int32_t batch_transmission(const cg_system* const* systems, const cg_sigma* const* sigmas, int64_t count,
+                           int32_t precision, double* out_t, int32_t* out_status, cg_batch_stats* stats) {
+    cg_batch* bt = nullptr;
+    int32_t rc = create_batch(systems, sigmas, count, precision, &bt);
+    if (rc != CG_OK) return rc;
+    std::unique_ptr<cg_batch> hold(bt);
+    return run_batch(bt, out_t, out_status, stats);
+}
+
+void destroy_batch(cg_batch* b) { delete b; }
+
+int32_t kernel_timing(int32_t enable) {
+    Engine& eng = Engine::get();
+    if (!eng.ok) return CG_ERR_DEVICE;
+    std::lock_guard<std::mutex> lock(eng.mu);
+    eng.timing = enable != 0;
+    if (enable)
+        for (auto& f : eng.fams) f = Engine::Fam{};
+    return CG_OK;
+}
+
+int32_t kernel_stats(cg_kernel_stat* out, int32_t max_records) {
+    Engine& eng = Engine::get();
+    std::lock_guard<std::mutex> lock(eng.mu);
+    int32_t n = 0;
+    for (int f = 0; f < kFamilies && n < max_records; ++f) {
+        cg_kernel_stat& r = out[n++];
+        std::snprintf(r.name, sizeof(r.name), "%s", kFamilyNames[f]);
+        r.launches = eng.fams[f].launches;
+        r.seconds = eng.fams[f].seconds;
+        r.flops = eng.fams[f].flops;
+        r.bytes = eng.fams[f].bytes;
+    }
+    return n;
 }
 
 int32_t backward(const cg_system*, const cg_sigma*, double* out_t, double*) {

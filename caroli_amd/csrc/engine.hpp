@@ -12,6 +12,13 @@ namespace caroli::gpu {
 int32_t batch_transmission(const cg_system* const* systems, const cg_sigma* const* sigmas, int64_t count,
                            int32_t precision, double* out_t, int32_t* out_status, cg_batch_stats* stats);
 
+int32_t create_batch(const cg_system* const* systems, const cg_sigma* const* sigmas, int64_t count, int32_t precision,
+                     cg_batch** out);
+int32_t run_batch(cg_batch* b, double* out_t, int32_t* out_status, cg_batch_stats* stats);
+void destroy_batch(cg_batch* b);
+int32_t kernel_timing(int32_t enable);
+int32_t kernel_stats(cg_kernel_stat* out, int32_t max_records);
+
 /// T and dT/d(eps_a) for every orbital, from the stored fronts of one point.
 int32_t backward(const cg_system* s, const cg_sigma* g, double* out_t, double* out_grad);
 

@@ -11,6 +11,10 @@
 //  - trace: the Caroli trace Tr[Gamma_L G Gamma_R G^H] as one fused reduction.
 #include <hip/hip_runtime.h>
 
+#include <cstdlib>
+#include <cstring>
+
+#include "cx.hpp"
 #include "fronts.hpp"
 
 namespace caroli::gpu {
@@ -19,68 +23,6 @@ namespace {
 
 constexpr int kThreads = 256;
 
-template <class T>
-struct Real;
-template <>
-struct Real<double2> {
-    using type = double;
-};
-template <>
-struct Real<float2> {
-    using type = float;
-};
-
-template <class T>
-__device__ __forceinline__ T cx(double re, double im) {
-    using R = typename Real<T>::type;
-    T t;
-    t.x = R(re);
-    t.y = R(im);
-    return t;
-}
-template <class T>
-__device__ __forceinline__ T cmul(T a, T b) {
-    T c;
-    c.x = a.x * b.x - a.y * b.y;
-    c.y = a.x * b.y + a.y * b.x;
-    return c;
-}
-template <class T>
-__device__ __forceinline__ T cadd(T a, T b) {
-    a.x += b.x;
-    a.y += b.y;
-    return a;
-}
-template <class T>
-__device__ __forceinline__ T csub(T a, T b) {
-    a.x -= b.x;
-    a.y -= b.y;
-    return a;
-}
-template <class T>
-__device__ __forceinline__ double cabs(T a) {
-    return hypot(double(a.x), double(a.y));
-}
-template <class T>
-__device__ __forceinline__ T crecip(T a) {
-    // 1 / a with one division of the squared modulus, scaled against overflow.
-    using R = typename Real<T>::type;
-    R s = fmax(fabs(a.x), fabs(a.y));
-    R ar = a.x / s, ai = a.y / s;
-    R d = (ar * ar + ai * ai) * s;
-    T r;
-    r.x = ar / d;
-    r.y = -ai / d;
-    return r;
-}
-template <class TO, class TI>
-__device__ __forceinline__ TO convert(TI a) {
-    using R = typename Real<TO>::type;
-    TO t;
-    t.x = R(a.x);
-    t.y = R(a.y);
-    return t;
-}
 
 // Block-wide max of (value, -index): the largest value, the smallest index on ties.
 __device__ void block_argmax(double& v, int& idx, double* sv, int* si) {
@@ -247,6 +189,165 @@ __device__ void eliminate(T* W, const NodeDesc& nd, int32_t* piv, int32_t* statu
     if (tid == 0) *status = kFrontOk;
 }
 
+// MFMA 16x16x4 on one wavefront: lane l holds A[l % 16][l / 16] and B[l / 16][l % 16]. The
+// accumulator element j of lane l holds D[row(l, j)][l % 16], with row = 4 (l / 16) + j for
+// f32 and row = l / 16 + 4 j for f64 (measured on gfx950).
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+template <class T>
+struct Mfma;
+template <>
+struct Mfma<double2> {
+    using acc = f64x4;
+    static __device__ __forceinline__ acc mac(double a, double b, acc c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int lane, int j) { return lane / 16 + 4 * j; }
+};
+template <>
+struct Mfma<float2> {
+    using acc = f32x4;
+    static __device__ __forceinline__ acc mac(float a, float b, acc c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int lane, int j) { return 4 * (lane / 16) + j; }
+};
+
+constexpr int kPanel = 16;  // panel width of the blocked elimination
+
+// Blocked right-looking partial LU of a front in global memory (the same factorisation as
+// `eliminate`, panel by panel): unblocked pivoting inside a panel of kPanel columns, the
+// triangular solve of the panel's row block, then the trailing update on MFMA, computed as
+// C^T -= U12^T L21^T so that the front's columns are read and written contiguously.
+template <class T>
+__device__ void eliminate_blocked(T* F, const NodeDesc& nd, int32_t* piv, int32_t* status, double tol, bool root) {
+    using R = typename Real<T>::type;
+    using M = Mfma<T>;
+    __shared__ double sv[kThreads / 64];
+    __shared__ int si[kThreads / 64];
+    __shared__ T l11[kPanel * kPanel];
+    const int n = nd.n, ne = nd.ne, ne1 = nd.ne1, tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    auto at = [&](int i, int j) -> T& { return F[i + int64_t(j) * n]; };
+    double m = 0.0;
+    for (int idx = tid; idx < ne1 * ne1; idx += kThreads) m = fmax(m, cabs(at(idx % ne1, idx / ne1)));
+    const double scale = block_max(m, sv);
+    for (int k0 = 0; k0 < ne; k0 += kPanel) {
+        const int w = min(kPanel, ne - k0);
+        // 1. Panel: columns [k0, k0 + w), rows [k0, n).
+        for (int k = k0; k < k0 + w; ++k) {
+            const int lim = k < ne1 ? ne1 : ne;
+            double big = -1.0;
+            int p = k;
+            for (int i = k + tid; i < lim; i += kThreads) {
+                double v = cabs(at(i, k));
+                if (v > big) {
+                    big = v;
+                    p = i;
+                }
+            }
+            block_argmax(big, p, sv, si);
+            const bool ok = k < ne1 ? (big > tol * scale) : (big > 0.0 && big < INFINITY);
+            if (!ok) {
+                if (tid == 0) *status = (k < ne1 && !root) ? kFrontDelayed : kFrontSingular;
+                return;
+            }
+            if (p != k)
+                for (int j = tid; j < n; j += kThreads) {
+                    T x = at(k, j);
+                    at(k, j) = at(p, j);
+                    at(p, j) = x;
+                }
+            if (tid == 0) piv[k] = p;
+            __syncthreads();
+            const T inv = crecip(at(k, k));
+            for (int i = k + 1 + tid; i < n; i += kThreads) at(i, k) = cmul(at(i, k), inv);
+            __syncthreads();
+            const int cols = k0 + w - k - 1, rows = n - k - 1;
+            for (int idx = tid; idx < cols * rows; idx += kThreads) {
+                int i = k + 1 + idx % rows, j = k + 1 + idx / rows;
+                at(i, j) = csub(at(i, j), cmul(at(i, k), at(k, j)));
+            }
+            __syncthreads();
+        }
+        const int j1 = k0 + w;  // first trailing row and column
+        if (j1 >= n) break;
+        // 2. U12 = L11^-1 A12 on the panel's rows, one column per lane.
+        for (int idx = tid; idx < kPanel * kPanel; idx += kThreads) {
+            int r = idx % kPanel, c = idx / kPanel;
+            l11[idx] = (r < w && c < w) ? at(k0 + r, k0 + c) : cx<T>(0.0, 0.0);
+        }
+        __syncthreads();
+        for (int j = j1 + tid; j < n; j += kThreads) {
+            T u[kPanel];
+#pragma unroll
+            for (int r = 0; r < kPanel; ++r) u[r] = r < w ? at(k0 + r, j) : cx<T>(0.0, 0.0);
+#pragma unroll
+            for (int r = 1; r < kPanel; ++r)
+#pragma unroll
+                for (int c = 0; c < r; ++c) u[r] = csub(u[r], cmul(l11[r + c * kPanel], u[c]));
+#pragma unroll
+            for (int r = 0; r < kPanel; ++r)
+                if (r < w) at(k0 + r, j) = u[r];
+        }
+        __syncthreads();
+        // 3. Trailing update on MFMA: each wave owns columns of 16x16 tiles.
+        const int rem = n - j1, tiles = (rem + 15) / 16;
+        for (int tj = wave; tj < tiles; tj += kThreads / 64) {
+            const int j0 = j1 + 16 * tj;
+            // A' = U12^T: lane holds U12[k0 + 4 s + lane / 16][j0 + lane % 16] for the 4 slices.
+            R ar[4], ai[4];
+#pragma unroll
+            for (int s = 0; s < 4; ++s) {
+                int r = 4 * s + lane / 16, j = j0 + lane % 16;
+                T v = (r < w && j < n) ? at(k0 + r, j) : cx<T>(0.0, 0.0);
+                ar[s] = v.x;
+                ai[s] = v.y;
+            }
+            for (int ti = 0; ti < tiles; ++ti) {
+                const int i0 = j1 + 16 * ti;
+                typename M::acc cr = {0, 0, 0, 0}, ci = {0, 0, 0, 0};
+#pragma unroll
+                for (int s = 0; s < 4; ++s) {
+                    // B' = L21^T: lane holds L21[i0 + lane % 16][k0 + 4 s + lane / 16].
+                    int c = 4 * s + lane / 16, i = i0 + lane % 16;
+                    T b = (c < w && i < n) ? at(i, k0 + c) : cx<T>(0.0, 0.0);
+                    cr = M::mac(ar[s], b.x, cr);
+                    cr = M::mac(-ai[s], b.y, cr);
+                    ci = M::mac(ar[s], b.y, ci);
+                    ci = M::mac(ai[s], b.x, ci);
+                }
+                // D'[row(lane, jj)][lane % 16] = C[i0 + lane % 16][j0 + row(lane, jj)].
+                const int i = i0 + lane % 16;
+                if (i < n)
+#pragma unroll
+                    for (int jj = 0; jj < 4; ++jj) {
+                        int j = j0 + M::row(lane, jj);
+                        if (j < n) {
+                            T& d = at(i, j);
+                            d.x -= cr[jj];
+                            d.y -= ci[jj];
+                        }
+                    }
+            }
+        }
+        __syncthreads();
+    }
+    if (tid == 0) *status = kFrontOk;
+}
+
+template <class T, bool kRoot>
+__global__ void __launch_bounds__(kThreads)
+    factor_blocked_kernel(const NodeDesc* __restrict__ nodes, const int32_t* __restrict__ level, PointsView pv, double tol) {
+    const NodeDesc nd = nodes[level[blockIdx.x]];
+    const int64_t p = blockIdx.y;
+    T* F = (kRoot ? reinterpret_cast<T*>(pv.root_arena) + p * pv.root_stride
+                  : reinterpret_cast<T*>(pv.arena) + p * pv.arena_stride) +
+           nd.off;
+    eliminate_blocked(F, nd, pv.piv + p * pv.piv_stride + nd.piv_off, pv.status + p * pv.status_stride + nd.slot, tol,
+                      kRoot);
+}
+
 template <class T, int NMAX>
 __global__ void __launch_bounds__(kThreads)
     factor_small_kernel(const NodeDesc* __restrict__ nodes, const int32_t* __restrict__ level, PointsView pv, double tol) {
@@ -330,15 +431,27 @@ void launch_assemble_root(const NodeDesc* nodes, const int32_t* level, int32_t c
                            level, ent, diag, kids, maps, pv);
 }
 
+bool blocked_fronts() {
+    const char* v = std::getenv("CAROLI_FRONT_KERNEL");
+    return !(v && std::strcmp(v, "unblocked") == 0);
+}
+
 void launch_factor_c128(const NodeDesc* nodes, const int32_t* level, int32_t count, const PointsView& pv,
                         int32_t points, double tol, bool small, bool root, hipStream_t st) {
     if (count <= 0 || points <= 0) return;
-    if (root)
+    const bool blk = blocked_fronts();
+    if (root && blk)
+        hipLaunchKernelGGL((factor_blocked_kernel<double2, true>), grid2(count, points), dim3(kThreads), 0, st, nodes,
+                           level, pv, tol);
+    else if (root)
         hipLaunchKernelGGL((factor_global_kernel<double2, true>), grid2(count, points), dim3(kThreads), 0, st, nodes,
                            level, pv, tol);
     else if (small)
         hipLaunchKernelGGL((factor_small_kernel<double2, kLdsFrontC128>), grid2(count, points), dim3(kThreads), 0, st,
                            nodes, level, pv, tol);
+    else if (blk)
+        hipLaunchKernelGGL((factor_blocked_kernel<double2, false>), grid2(count, points), dim3(kThreads), 0, st, nodes,
+                           level, pv, tol);
     else
         hipLaunchKernelGGL((factor_global_kernel<double2, false>), grid2(count, points), dim3(kThreads), 0, st, nodes,
                            level, pv, tol);
@@ -350,6 +463,9 @@ void launch_factor_c64(const NodeDesc* nodes, const int32_t* level, int32_t coun
     if (small)
         hipLaunchKernelGGL((factor_small_kernel<float2, kLdsFrontC64>), grid2(count, points), dim3(kThreads), 0, st,
                            nodes, level, pv, tol);
+    else if (blocked_fronts())
+        hipLaunchKernelGGL((factor_blocked_kernel<float2, false>), grid2(count, points), dim3(kThreads), 0, st, nodes,
+                           level, pv, tol);
     else
         hipLaunchKernelGGL((factor_global_kernel<float2, false>), grid2(count, points), dim3(kThreads), 0, st, nodes,
                            level, pv, tol);

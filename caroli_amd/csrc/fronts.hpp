@@ -21,6 +21,16 @@ struct NodeDesc {
     int32_t nd;                  // root: variables delayed up to it
     int32_t slot;                // status / pivot slot of the front within a point
     int64_t piv_off;             // first pivot index of the front within a point
+    int64_t var_off;             // first of its n orbital indices in the plan's variable list
+};
+
+/// Right-hand sides of the multifrontal solves: per point, c columns of n_orb complex f64
+/// (column-major, leading dimension ldx), overwritten by the solution.
+struct SolveView {
+    double2* x;
+    int64_t x_stride;
+    int32_t ldx, c;
+    const int32_t* vars;  // orbital index of every front variable (NodeDesc::var_off)
 };
 
 /// An H entry: F[a, b] -= conj ? conj(vals[k]) : vals[k].
@@ -78,5 +88,18 @@ void launch_factor_c64(const NodeDesc* nodes, const int32_t* level, int32_t coun
                        int32_t points, double tol, bool small, hipStream_t st);
 /// Caroli trace of the root border block: t_out[p] = Tr[Gamma_L G Gamma_R G^H].
 void launch_trace(const NodeDesc* root, const PointsView& pv, int32_t points, hipStream_t st);
+
+/// Largest ne * c of a front handled by the solve kernels (LDS words of complex f64).
+constexpr int kSolveLds = 4096;
+
+/// Multifrontal solves with the stored factors, one tree level per launch (grid = fronts x points).
+/// forward: children first; A x = b (transpose = 0) or A^T x = b (transpose = 1).
+/// root: the dense root; backward: parents first. `single`: the fronts hold complex f32 factors.
+void launch_solve_forward(const NodeDesc* nodes, const int32_t* level, int32_t count, const PointsView& pv,
+                          const SolveView& sv, int32_t points, bool single, bool transpose, hipStream_t st);
+void launch_solve_root(const NodeDesc* nodes, const int32_t* level, const PointsView& pv, const SolveView& sv,
+                       int32_t points, bool transpose, hipStream_t st);
+void launch_solve_backward(const NodeDesc* nodes, const int32_t* level, int32_t count, const PointsView& pv,
+                           const SolveView& sv, int32_t points, bool single, bool transpose, hipStream_t st);
 
 }  // namespace caroli::gpu

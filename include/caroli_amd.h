@@ -69,6 +69,11 @@ int32_t cg_preset_create(int32_t model, const uint64_t* extent, int32_t axis, in
 
 void cg_system_destroy(cg_system* s);
 
+/* The random stream of the builders (crates/caroli-disorder/src/rng.rs): `count` uniforms in
+ * [0, 1) of xoshiro256** seeded by mode 0 `Xoshiro256::seed_from_u64(seed)`, mode 1
+ * `Xoshiro256::gfortran(seed)`, or mode 2 `gfortran_put(put)` with put = 8 int32 values. */
+int32_t cg_uniforms(int32_t mode, uint64_t seed, const int32_t* put, int64_t count, double* out);
+
 /* NestedTwoTerminal::block_count (nested.rs:613). */
 int32_t cg_block_count(const cg_system* s);
 
@@ -123,6 +128,27 @@ typedef struct cg_batch_stats {
  * out_t[p] = T or NaN; out_status[p] = status code. */
 int32_t cg_batch_transmission(const cg_system* const* systems, const cg_sigma* const* sigmas, int64_t count,
                               int32_t precision, double* out_t, int32_t* out_status, cg_batch_stats* stats);
+
+/* A resident batch: the inputs of the points uploaded once to HBM, for repeated launches
+ * (serving loops, benchmarks). cg_batch_run computes T for every point from the resident
+ * inputs; the systems and self-energies must outlive the batch. */
+typedef struct cg_batch cg_batch;
+int32_t cg_batch_create(const cg_system* const* systems, const cg_sigma* const* sigmas, int64_t count,
+                        int32_t precision, cg_batch** out);
+int32_t cg_batch_run(cg_batch* batch, double* out_t, int32_t* out_status, cg_batch_stats* stats);
+void cg_batch_destroy(cg_batch* batch);
+
+/* Per-kernel timing with HIP events on the engine's stream (enable = 1 resets the counters).
+ * Each record: kernel family name, launches, total seconds, algorithmic flops and bytes. */
+typedef struct cg_kernel_stat {
+  char name[48];
+  int64_t launches;
+  double seconds;
+  double flops;
+  double bytes;
+} cg_kernel_stat;
+int32_t cg_kernel_timing(int32_t enable);
+int32_t cg_kernel_stats(cg_kernel_stat* out, int32_t max_records);
 
 /* Adjoint: T and dT/d(eps_a) for every orbital a of the device (n values, on-site energies),
  * from one backward sweep over the stored fronts (no counterpart in the reference). */

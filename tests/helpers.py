@@ -1,0 +1,45 @@
+"""Shared helpers of the tests: systems, self-energies and the oracle's view of them."""
+import numpy as np
+
+import caroli_amd as ca
+from _oracle import OracleSystem
+
+
+def sigmas_of(sys_, energy):
+    """The engine's host self-energies at `energy`, as the LeadSelfEnergies handle and per-block arrays."""
+    sig = sys_.self_energies(energy)
+    return sig, [sig.block(b)[:2] for b in range(sys_.block_count())]
+
+
+def oracle_of(sys_):
+    return OracleSystem(sys_.arrays())
+
+
+def hash_disorder(region_cells, subs, w):
+    """The reproducible on-site disorder of crates/caroli-green/tests/nested.rs:30-39, per site."""
+    out = []
+    for (x, y, _), s in zip(region_cells, subs):
+        k = (int(x) * 7919 + int(y) * 104_729 + int(s) * 31) & 0xFFFFFFFFFFFFFFFF
+        u = (((k * 6_364_136_223_846_793_005 + 1_442_695_040_888_963_407) & 0xFFFFFFFFFFFFFFFF) >> 11) / float(1 << 53)
+        out.append(w * (u - 0.5))
+    return np.array(out)
+
+
+def cuboid_cells(extent, subs_per_cell, dim):
+    """Cells and sublattice of each site of Region::cuboid (first axis slowest, sublattice fastest)."""
+    n = [extent[a] if a < dim else 1 for a in range(3)]
+    cells, subs = [], []
+    for i in range(n[0]):
+        for j in range(n[1]):
+            for k in range(n[2]):
+                for s in range(subs_per_cell):
+                    cells.append((i, j, k))
+                    subs.append(s)
+    return cells, subs
+
+
+def hashed(model, extent, w, subs_per_cell=1, dim=2, axis=0):
+    """`disordered(model, w, ...)` of the reference's nested tests, on a cuboid."""
+    cells, subs = cuboid_cells(extent, subs_per_cell, dim)
+    pot = hash_disorder(cells, subs, w)
+    return ca.NestedTwoTerminal.preset(model, extent, axis=axis, site_potential=pot)
