@@ -269,8 +269,10 @@ struct Inputs {
 
 /// Kernel families timed by cg_kernel_timing.
 enum Family { kAssemble, kFactorLds, kFactorGlobal, kRootAssemble, kRootFactor, kTrace, kFamilies };
-const char* const kFamilyNames[kFamilies] = {"assemble_kernel", "factor_small_kernel", "factor_global_kernel",
-                                             "assemble_kernel<root>", "factor_global_kernel<root>", "trace_kernel"};
+// Large fronts and the root run factor_blocked_kernel (or factor_global_kernel when
+// CAROLI_FRONT_KERNEL=unblocked, and for complex f32 fronts).
+const char* const kFamilyNames[kFamilies] = {"assemble_kernel", "factor_small_kernel", "factor_large_kernel",
+                                             "assemble_kernel<root>", "factor_large_kernel<root>", "trace_kernel"};
 
 struct Engine {
     std::mutex mu;
@@ -846,6 +848,73 @@ int32_t batch_transmission(const cg_system* const* systems, const cg_sigma* cons
 }
 
 void destroy_batch(cg_batch* b) { delete b; }
+
+int32_t front_schur(const double* f, int64_t n, int64_t ne, int64_t ne1, int32_t precision, double* out_s,
+                    int32_t* out_status) {
+    Engine& eng = Engine::get();
+    if (!eng.ok) return CG_ERR_DEVICE;
+    std::lock_guard<std::mutex> lock(eng.mu);
+    const bool single = precision == CG_SINGLE;
+    NodeDesc d{};
+    d.off = 0;
+    d.n = int32_t(n);
+    d.ne = int32_t(ne);
+    d.ne1 = int32_t(ne1);
+    d.slot = 0;
+    d.piv_off = 0;
+    DevBuf fb, nodes, lv, pivb, stb;
+    const size_t nn = size_t(n) * size_t(n);
+    if (single) {
+        std::vector<float2> h(nn);
+        for (size_t e = 0; e < nn; ++e) h[e] = float2{float(f[2 * e]), float(f[2 * e + 1])};
+        upload(fb, h, eng.stream);
+    } else {
+        std::vector<double2> h(nn);
+        for (size_t e = 0; e < nn; ++e) h[e] = double2{f[2 * e], f[2 * e + 1]};
+        upload(fb, h, eng.stream);
+    }
+    upload(nodes, std::vector<NodeDesc>{d}, eng.stream);
+    upload(lv, std::vector<int32_t>{0}, eng.stream);
+    pivb.ensure(size_t(std::max<int64_t>(ne, 1)) * 4);
+    stb.ensure(4);
+    PointsView pv{};
+    pv.arena = fb.p;
+    pv.piv = pivb.as<int32_t>();
+    pv.status = stb.as<int32_t>();
+    pv.status_stride = 1;
+    const bool small = n <= (single ? kLdsFrontC64 : kLdsFrontC128);
+    if (single)
+        launch_factor_c64(nodes.as<NodeDesc>(), lv.as<int32_t>(), 1, pv, 1, kTolSingle, small, eng.stream);
+    else
+        launch_factor_c128(nodes.as<NodeDesc>(), lv.as<int32_t>(), 1, pv, 1, kTolDouble, small, false, eng.stream);
+    HIP_CHECK(hipGetLastError());
+    int32_t st = 0;
+    HIP_CHECK(hipMemcpyAsync(&st, stb.p, 4, hipMemcpyDeviceToHost, eng.stream));
+    const int64_t nb = n - ne;
+    if (single) {
+        std::vector<float2> h(nn);
+        HIP_CHECK(hipMemcpyAsync(h.data(), fb.p, nn * sizeof(float2), hipMemcpyDeviceToHost, eng.stream));
+        HIP_CHECK(hipStreamSynchronize(eng.stream));
+        for (int64_t j = 0; j < nb; ++j)
+            for (int64_t i = 0; i < nb; ++i) {
+                float2 v = h[size_t(ne + i) + size_t(ne + j) * size_t(n)];
+                out_s[2 * (i + j * nb)] = v.x;
+                out_s[2 * (i + j * nb) + 1] = v.y;
+            }
+    } else {
+        std::vector<double2> h(nn);
+        HIP_CHECK(hipMemcpyAsync(h.data(), fb.p, nn * sizeof(double2), hipMemcpyDeviceToHost, eng.stream));
+        HIP_CHECK(hipStreamSynchronize(eng.stream));
+        for (int64_t j = 0; j < nb; ++j)
+            for (int64_t i = 0; i < nb; ++i) {
+                double2 v = h[size_t(ne + i) + size_t(ne + j) * size_t(n)];
+                out_s[2 * (i + j * nb)] = v.x;
+                out_s[2 * (i + j * nb) + 1] = v.y;
+            }
+    }
+    *out_status = st;
+    return CG_OK;
+}
 
 int32_t kernel_timing(int32_t enable) {
     Engine& eng = Engine::get();

@@ -17,6 +17,8 @@ int32_t create_batch(const cg_system* const* systems, const cg_sigma* const* sig
 int32_t run_batch(cg_batch* b, double* out_t, int32_t* out_status, cg_batch_stats* stats);
 void destroy_batch(cg_batch* b);
 int32_t kernel_timing(int32_t enable);
+int32_t front_schur(const double* f, int64_t n, int64_t ne, int64_t ne1, int32_t precision, double* out_s,
+                    int32_t* out_status);
 int32_t kernel_stats(cg_kernel_stat* out, int32_t max_records);
 
 /// T and dT/d(eps_a) for every orbital, from the stored fronts of one point.

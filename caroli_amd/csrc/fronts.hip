@@ -41,7 +41,7 @@ __device__ void block_argmax(double& v, int& idx, double* sv, int* si) {
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        for (int w = 1; w < kThreads / 64; ++w)
+        for (int w = 1; w < int(blockDim.x) / 64; ++w)
             if (sv[w] > sv[0] || (sv[w] == sv[0] && si[w] < si[0])) {
                 sv[0] = sv[w];
                 si[0] = si[w];
@@ -58,7 +58,7 @@ __device__ double block_max(double v, double* sv) {
     if ((threadIdx.x & 63) == 0) sv[threadIdx.x >> 6] = v;
     __syncthreads();
     double r = sv[0];
-    for (int w = 1; w < kThreads / 64; ++w) r = fmax(r, sv[w]);
+    for (int w = 1; w < int(blockDim.x) / 64; ++w) r = fmax(r, sv[w]);
     __syncthreads();
     return r;
 }
@@ -68,7 +68,7 @@ __device__ double block_sum(double v, double* sv) {
     if ((threadIdx.x & 63) == 0) sv[threadIdx.x >> 6] = v;
     __syncthreads();
     double r = 0.0;
-    for (int w = 0; w < kThreads / 64; ++w) r += sv[w];
+    for (int w = 0; w < int(blockDim.x) / 64; ++w) r += sv[w];
     __syncthreads();
     return r;
 }
@@ -142,8 +142,8 @@ __global__ void __launch_bounds__(kThreads)
 // Partial LU of one front (see NodeDesc). W points at the working copy (LDS or the front itself).
 template <class T>
 __device__ void eliminate(T* W, const NodeDesc& nd, int32_t* piv, int32_t* status, double tol, bool root) {
-    __shared__ double sv[kThreads / 64];
-    __shared__ int si[kThreads / 64];
+    __shared__ double sv[16];
+    __shared__ int si[16];
     const int n = nd.n, ne = nd.ne, ne1 = nd.ne1, tid = threadIdx.x;
     double m = 0.0;
     for (int idx = tid; idx < ne1 * ne1; idx += kThreads) m = fmax(m, cabs(W[(idx % ne1) + int64_t(idx / ne1) * n]));
@@ -223,8 +223,8 @@ template <class T>
 __device__ void eliminate_blocked(T* F, const NodeDesc& nd, int32_t* piv, int32_t* status, double tol, bool root) {
     using R = typename Real<T>::type;
     using M = Mfma<T>;
-    __shared__ double sv[kThreads / 64];
-    __shared__ int si[kThreads / 64];
+    __shared__ double sv[16];
+    __shared__ int si[16];
     __shared__ T l11[kPanel * kPanel];
     const int n = nd.n, ne = nd.ne, ne1 = nd.ne1, tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
@@ -375,7 +375,7 @@ __global__ void __launch_bounds__(kThreads)
 }
 
 __global__ void __launch_bounds__(kThreads) trace_kernel(const NodeDesc* __restrict__ rootp, PointsView pv) {
-    __shared__ double sv[kThreads / 64];
+    __shared__ double sv[16];
     const NodeDesc nd = *rootp;
     const int64_t p = blockIdx.x;
     const double2* F = reinterpret_cast<const double2*>(pv.root_arena) + p * pv.root_stride + nd.off;
@@ -463,9 +463,6 @@ void launch_factor_c64(const NodeDesc* nodes, const int32_t* level, int32_t coun
     if (small)
         hipLaunchKernelGGL((factor_small_kernel<float2, kLdsFrontC64>), grid2(count, points), dim3(kThreads), 0, st,
                            nodes, level, pv, tol);
-    else if (blocked_fronts())
-        hipLaunchKernelGGL((factor_blocked_kernel<float2, false>), grid2(count, points), dim3(kThreads), 0, st, nodes,
-                           level, pv, tol);
     else
         hipLaunchKernelGGL((factor_global_kernel<float2, false>), grid2(count, points), dim3(kThreads), 0, st, nodes,
                            level, pv, tol);

@@ -129,6 +129,14 @@ typedef struct cg_batch_stats {
 int32_t cg_batch_transmission(const cg_system* const* systems, const cg_sigma* const* sigmas, int64_t count,
                               int32_t precision, double* out_t, int32_t* out_status, cg_batch_stats* stats);
 
+/* The front update on the GPU, as caroli_dense::Field::schur (crates/caroli-dense/src/field.rs:44):
+ * S = F22 - F21 F11^-1 F12 of the leading ne x ne block of the n x n front f (column-major), with
+ * the pivot test of field.rs:61 / :111 on the pivots of columns [0, ne1) (ne1 = ne for a front).
+ * precision CG_DOUBLE or CG_SINGLE. out_s receives (n - ne)^2 values; *out_status is 0, or 5 when
+ * the block is refused (numerically singular: the reference returns None). */
+int32_t cg_front_schur(const double* f, int64_t n, int64_t ne, int64_t ne1, int32_t precision, double* out_s,
+                       int32_t* out_status);
+
 /* A resident batch: the inputs of the points uploaded once to HBM, for repeated launches
  * (serving loops, benchmarks). cg_batch_run computes T for every point from the resident
  * inputs; the systems and self-energies must outlive the batch. */

@@ -1,0 +1,74 @@
+"""GPU front update (Field::schur, crates/caroli-dense/src/field.rs:44-105) against numpy.
+
+S = F22 - F21 F11^-1 F12 for random fronts of the sizes the trees produce: LDS-resident fronts
+(n <= 64) and the blocked MFMA kernel (larger), both kernels of large fronts, complex f64 to a
+relative 1e-11 and complex f32 to 1e-4; a singular leading block is refused (field.rs:199-208).
+"""
+import ctypes as C
+import os
+
+import numpy as np
+import pytest
+
+from caroli_amd import _capi as capi
+
+pytestmark = pytest.mark.gpu
+
+SIZES = [(12, 5, 5), (60, 20, 20), (64, 64, 64), (90, 40, 40), (100, 37, 37), (200, 64, 64), (384, 128, 128),
+         (60, 40, 0), (300, 256, 0)]
+
+
+def front(n, seed):
+    """A front like field.rs:183-191: smooth entries plus a dominant diagonal."""
+    rng = np.random.default_rng(seed)
+    f = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
+    return np.asfortranarray(f + 3.0 * np.sqrt(n) * np.eye(n))
+
+
+def gpu_schur(f, ne, ne1, precision):
+    n = f.shape[0]
+    s = np.zeros((n - ne, n - ne), np.complex128, order="F")
+    st = C.c_int32(-1)
+    fd = np.asfortranarray(f, np.complex128)
+    rc = capi.cg_front_schur(fd.ctypes.data_as(C.POINTER(C.c_double)), n, ne, ne1, precision,
+                             s.ctypes.data_as(C.POINTER(C.c_double)), C.byref(st))
+    assert rc == 0
+    return s, st.value
+
+
+def numpy_schur(f, ne):
+    return f[ne:, ne:] - f[ne:, :ne] @ np.linalg.solve(f[:ne, :ne], f[:ne, ne:])
+
+
+@pytest.mark.parametrize("kernel", ["blocked", "unblocked"])
+@pytest.mark.parametrize("n,ne,ne1", SIZES)
+def test_front_schur_double(n, ne, ne1, kernel, monkeypatch):
+    monkeypatch.setenv("CAROLI_FRONT_KERNEL", kernel)
+    f = front(n, n + ne)
+    print(f"[gpu] front n={n} ne={ne} ne1={ne1} {kernel}", flush=True)
+    s, st = gpu_schur(f, ne, ne1, 0)
+    assert st == 0
+    if ne == n:
+        return  # nothing left: only the pivot test ran
+    ref = numpy_schur(f, ne)
+    assert np.abs(s - ref).max() <= 1e-11 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("n,ne", [(12, 5), (90, 40), (200, 64), (384, 128)])
+def test_front_schur_single(n, ne):
+    f = front(n, 7 * n + ne)
+    print(f"[gpu] front c64 n={n} ne={ne}", flush=True)
+    s, st = gpu_schur(f, ne, ne, 1)
+    assert st == 0
+    ref = numpy_schur(f, ne)
+    assert np.abs(s - ref).max() <= 1e-4 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("n,ne", [(10, 4), (120, 70)])
+@pytest.mark.parametrize("precision", [0, 1])
+def test_singular_block_is_refused(n, ne, precision):
+    f = front(n, 3)
+    f[1, :] = f[0, :]  # two equal rows in the leading block
+    print(f"[gpu] singular front n={n} ne={ne} precision {precision}", flush=True)
+    _, st = gpu_schur(f, ne, ne, precision)
+    assert st == 5
