@@ -19,6 +19,38 @@ CMat broadening(const CMat& s) {
     return g;
 }
 
+CMat gamma_factor(const CMat& g) {
+    const int n = g.rows;
+    std::vector<double> d(n);
+    double top = 0.0;
+    for (int i = 0; i < n; ++i) {
+        d[i] = g(i, i).real();
+        top = std::max(top, d[i]);
+    }
+    std::vector<std::vector<cplx>> cols;
+    std::vector<char> used(n, 0);
+    while (int(cols.size()) < n) {
+        int p = -1;
+        for (int i = 0; i < n; ++i)
+            if (!used[i] && (p < 0 || d[i] > d[p])) p = i;
+        if (p < 0 || !(d[p] > 1e-12 * top)) break;
+        used[p] = 1;
+        std::vector<cplx> w(n);
+        double r = std::sqrt(d[p]);
+        for (int i = 0; i < n; ++i) {
+            cplx v = g(i, p);
+            for (auto& c : cols) v -= c[i] * std::conj(c[p]);
+            w[i] = v / r;
+        }
+        for (int i = 0; i < n; ++i) d[i] -= std::norm(w[i]);
+        cols.push_back(std::move(w));
+    }
+    CMat w(n, int(cols.size()));
+    for (int c = 0; c < w.cols; ++c)
+        for (int i = 0; i < n; ++i) w(i, c) = cols[c][i];
+    return w;
+}
+
 bool sigma_matches(const cg_system& s, const cg_sigma& g) {
     if (g.blocks.size() != s.sym.blocks.size()) return false;
     for (size_t b = 0; b < g.blocks.size(); ++b) {
@@ -297,6 +329,8 @@ int32_t cg_self_energies(const cg_system* s, double energy, cg_sigma** out) {
             sb.sr = mr.sigma_right();
             sb.gl = broadening(sb.sl);
             sb.gr = broadening(sb.sr);
+            sb.wl = gamma_factor(sb.gl);
+            sb.wr = gamma_factor(sb.gr);
             sb.channels_l = ml.channels;
             sb.channels_r = mr.channels;
             sb.left_h00 = blk.left_h00;
@@ -328,6 +362,8 @@ int32_t cg_sigma_from_arrays(const cg_system* s, double energy, const double* si
         p += 2 * nr * nr;
         sb.gl = broadening(sb.sl);
         sb.gr = broadening(sb.sr);
+        sb.wl = gamma_factor(sb.gl);
+        sb.wr = gamma_factor(sb.gr);
         sb.left_h00 = blk.left_h00;
         sb.left_h01 = blk.left_h01;
         sb.right_h00 = blk.right_h00;

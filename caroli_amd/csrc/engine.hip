@@ -88,12 +88,12 @@ struct HostPlan {
     /// Orbital of every front variable (NodeDesc::var_off), for the solves.
     std::vector<int32_t> vars;
     /// The block's orbitals, and each orbital's position in the left / right slice (-1 if none).
-    std::vector<int32_t> rows, left_pos, right_pos;
+    std::vector<int32_t> rows, left_pos, right_pos, left_orbs, right_orbs;
     int32_t max_ne = 0;  // largest ne of a front or the root
 };
 
 struct DevPlan {
-    DevBuf nodes, ent, diag, kids, maps, levels, vars, rows, left_pos, right_pos, row_ptr, cols;
+    DevBuf nodes, ent, diag, kids, maps, levels, vars, rows, left_pos, right_pos, row_ptr, cols, left_orbs, right_orbs;
     // Ranges into `levels` for each list of each level.
     std::vector<std::pair<int32_t, int32_t>> all, s128, l128, s64, l64;
     int32_t root_at = 0;  // position of the root index in `levels`
@@ -217,6 +217,8 @@ HostPlan build_plan(const Block& blk, const SparseMatrix& h) {
     hp.right_pos.assign(h.n, -1);
     for (size_t a = 0; a < blk.left_orbitals.size(); ++a) hp.left_pos[blk.left_orbitals[a]] = int32_t(a);
     for (size_t a = 0; a < blk.right_orbitals.size(); ++a) hp.right_pos[blk.right_orbitals[a]] = int32_t(a);
+    for (size_t o : blk.left_orbitals) hp.left_orbs.push_back(int32_t(o));
+    for (size_t o : blk.right_orbitals) hp.right_orbs.push_back(int32_t(o));
     hp.nodes.push_back(r);
     hp.root_stride = int64_t(r.n) * r.n;
     hp.piv_stride = std::max<int64_t>(piv, 1);
@@ -239,6 +241,8 @@ void upload_plan(const HostPlan& hp, DevPlan& dp, const SparseMatrix& h, hipStre
     upload(dp.right_pos, hp.right_pos, st);
     upload(dp.row_ptr, h.row_ptr, st);
     upload(dp.cols, h.cols, st);
+    upload(dp.left_orbs, hp.left_orbs, st);
+    upload(dp.right_orbs, hp.right_orbs, st);
     upload(dp.nodes, hp.nodes, st);
     upload(dp.ent, hp.ent, st);
     upload(dp.diag, hp.diag, st);
@@ -687,6 +691,112 @@ int32_t replanned_point(Engine& eng, const cg_system* s, const cg_sigma* g, int 
     return CG_OK;
 }
 
+/// Multifrontal solve of the columns of `sv` with the factors the last sweep left in the arenas:
+/// forward elimination (skipped when the right-hand sides live on the kept orbitals only, where
+/// it does nothing), the dense root, the backward substitution.
+void solve(Engine& eng, const DevPlan& dp, const PointsView& pv, const SolveView& sv, int32_t P, bool single,
+           bool transpose, bool kept_only) {
+    const NodeDesc* nodes = dp.nodes.as<NodeDesc>();
+    const int32_t* lv = dp.levels.as<int32_t>();
+    if (!kept_only)
+        for (size_t l = 0; l < dp.all.size(); ++l)
+            launch_solve_forward(nodes, lv + dp.all[l].first, dp.all[l].second, pv, sv, P, single, transpose, eng.stream);
+    launch_solve_root(nodes, lv + dp.root_at, pv, sv, P, transpose, eng.stream);
+    for (size_t l = dp.all.size(); l-- > 0;)
+        launch_solve_backward(nodes, lv + dp.all[l].first, dp.all[l].second, pv, sv, P, single, transpose, eng.stream);
+    HIP_CHECK(hipGetLastError());
+}
+
+/// Refinement: iterate until the relative correction max|d| / max|x| is below this.
+constexpr double kRefineTol = 1e-12;
+constexpr int kRefineMaxIter = 12;
+
+struct Refined {
+    std::vector<double> t;
+    std::vector<char> ok;
+    int64_t iters = 0;
+    double worst = 0.0;
+};
+
+/// Transmission of block b for the P points of `pv` from the factors of the last sweep (complex
+/// f32 fronts when `single`): x = A^-1 E_R W_R, refined with FP64 residuals b - A x of the full
+/// operator z - H - Sigma until the correction is below kRefineTol; then T_b = |W_L^H x_L|^2.
+Refined refine(Engine& eng, const HostPlan& hp, const DevPlan& dp, const PointsView& pv,
+               const std::vector<const cg_sigma*>& sig, int b, int64_t n_orb, bool single) {
+    const int32_t P = int32_t(sig.size());
+    Refined out;
+    out.t.assign(P, 0.0);
+    out.ok.assign(P, 1);
+    int c = 0;
+    for (auto* g : sig) c = std::max(c, g->blocks[b].wr.cols);
+    if (c == 0) return out;
+    if (int64_t(hp.max_ne) * c > kSolveLds) {
+        out.ok.assign(P, 0);
+        return out;
+    }
+    const int nl = int(hp.left_orbs.size()), nr = int(hp.right_orbs.size()), nrows = int(hp.rows.size());
+    std::vector<double2> w(size_t(P) * nr * c, double2{0.0, 0.0});
+    for (int32_t p = 0; p < P; ++p) {
+        const CMat& wr = sig[p]->blocks[b].wr;
+        for (int col = 0; col < wr.cols; ++col)
+            for (int a = 0; a < nr; ++a)
+                w[size_t(p) * nr * c + a + size_t(col) * nr] = double2{wr(a, col).real(), wr(a, col).imag()};
+    }
+    DevBuf dw, xb, bb, db, rb, gb;
+    upload(dw, w, eng.stream);
+    const size_t bytes = size_t(P) * size_t(n_orb) * c * sizeof(double2);
+    xb.ensure(bytes);
+    bb.ensure(bytes);
+    db.ensure(bytes);
+    rb.ensure(size_t(P) * sizeof(double));
+    gb.ensure(size_t(P) * nl * c * sizeof(double2));
+    HIP_CHECK(hipMemsetAsync(bb.p, 0, bytes, eng.stream));
+    SolveView xv{xb.as<double2>(), n_orb * c, int32_t(n_orb), c, dp.vars.as<int32_t>()};
+    SolveView bv = xv, dv = xv;
+    bv.x = bb.as<double2>();
+    dv.x = db.as<double2>();
+    launch_scatter(bv, dp.right_orbs.as<int32_t>(), nr, dw.as<double2>(), int64_t(nr) * c, false, P, eng.stream);
+    HIP_CHECK(hipMemcpyAsync(xb.p, bb.p, bytes, hipMemcpyDeviceToDevice, eng.stream));
+    solve(eng, dp, pv, xv, P, single, false, true);
+    std::vector<double> ratio(P, INFINITY);
+    for (int it = 0; it < kRefineMaxIter; ++it) {
+        launch_residual(pv, xv, bb.as<double2>(), db.as<double2>(), dp.rows.as<int32_t>(), nrows,
+                        dp.row_ptr.as<int64_t>(), dp.cols.as<uint32_t>(), dp.left_pos.as<int32_t>(),
+                        dp.right_pos.as<int32_t>(), dp.left_orbs.as<int32_t>(), nl, dp.right_orbs.as<int32_t>(), nr, P,
+                        eng.stream);
+        solve(eng, dp, pv, dv, P, single, false, false);
+        launch_update(xv, db.as<double2>(), dp.rows.as<int32_t>(), nrows, rb.as<double>(), P, eng.stream);
+        HIP_CHECK(hipGetLastError());
+        HIP_CHECK(hipMemcpyAsync(ratio.data(), rb.p, P * sizeof(double), hipMemcpyDeviceToHost, eng.stream));
+        HIP_CHECK(hipStreamSynchronize(eng.stream));
+        ++out.iters;
+        bool done = true;
+        for (double r : ratio) done &= r <= kRefineTol;
+        if (done) break;
+    }
+    launch_gather(xv, dp.left_orbs.as<int32_t>(), nl, gb.as<double2>(), P, eng.stream);
+    std::vector<double2> xl(size_t(P) * nl * c);
+    HIP_CHECK(hipMemcpyAsync(xl.data(), gb.p, xl.size() * sizeof(double2), hipMemcpyDeviceToHost, eng.stream));
+    HIP_CHECK(hipStreamSynchronize(eng.stream));
+    for (int32_t p = 0; p < P; ++p) {
+        out.ok[p] = ratio[p] <= kRefineTol;
+        out.worst = std::max(out.worst, ratio[p]);
+        const CMat& wl = sig[p]->blocks[b].wl;
+        double t = 0.0;
+        for (int col = 0; col < c; ++col)
+            for (int d = 0; d < wl.cols; ++d) {
+                cplx acc = 0.0;
+                for (int a = 0; a < nl; ++a) {
+                    double2 x = xl[size_t(p) * nl * c + a + size_t(col) * nl];
+                    acc += std::conj(wl(a, d)) * cplx(x.x, x.y);
+                }
+                t += std::norm(acc);
+            }
+        out.t[p] = t;
+    }
+    return out;
+}
+
 }  // namespace
 
 bool device_available() { return Engine::get().ok; }
@@ -721,13 +831,12 @@ int32_t create_batch(const cg_system* const* systems, const cg_sigma* const* sig
                      cg_batch** out) {
     Engine& eng = Engine::get();
     if (!eng.ok) return CG_ERR_DEVICE;
-    if (precision == CG_MIXED) return CG_ERR_INVALID;  // not available in this engine yet
     std::lock_guard<std::mutex> lock(eng.mu);
     auto bt = std::make_unique<cg_batch>();
     bt->systems.assign(systems, systems + count);
     bt->sigmas.assign(sigmas, sigmas + count);
     bt->precision = precision;
-    const bool single = precision == CG_SINGLE;
+    const bool single = precision != CG_DOUBLE;
     const size_t es = single ? sizeof(float2) : sizeof(double2);
     std::map<uint64_t, std::vector<int64_t>> groups;
     for (int64_t p = 0; p < count; ++p) groups[systems[p]->key].push_back(p);
@@ -775,59 +884,107 @@ int32_t create_batch(const cg_system* const* systems, const cg_sigma* const* sig
 int32_t run_batch(cg_batch* bt, double* out_t, int32_t* out_status, cg_batch_stats* stats) {
     Engine& eng = Engine::get();
     if (!eng.ok) return CG_ERR_DEVICE;
-    std::lock_guard<std::mutex> lock(eng.mu);
-    const bool single = bt->precision == CG_SINGLE;
+    const bool mixed = bt->precision == CG_MIXED;
+    const bool single = bt->precision == CG_SINGLE || mixed;
     const int64_t count = int64_t(bt->systems.size());
     cg_batch_stats local{};
     local.front_flops = bt->flops;
     local.front_bytes = bt->bytes;
     std::vector<int32_t> status(count, CG_OK);
+    std::vector<char> redo(count, 0);  // mixed: recompute in complex f64
     for (int64_t p = 0; p < count; ++p) out_t[p] = 0.0;
-    for (auto& u : bt->units) {
-        const HostPlan& hp = u.structure->host[u.block];
-        const DevPlan& dp = u.structure->dev[u.block];
-        const size_t P = u.points.size();
-        PointsView pv{};
-        bind_inputs(u.in, pv);
-        pv.arena = eng.arena.p;
-        pv.arena_stride = hp.arena_stride;
-        pv.root_arena = eng.root_arena.p;
-        pv.root_stride = hp.root_stride;
-        pv.piv = eng.piv.as<int32_t>();
-        pv.piv_stride = hp.piv_stride;
-        pv.status = eng.status.as<int32_t>();
-        pv.status_stride = hp.slots;
-        pv.t_out = eng.tout.as<double>();
-        HIP_CHECK(hipEventRecord(eng.ev0, eng.stream));
-        sweep(eng, hp, dp, pv, int32_t(P), single);
-        HIP_CHECK(hipEventRecord(eng.ev1, eng.stream));
-        std::vector<double> t(P);
-        std::vector<int32_t> st(P * size_t(hp.slots));
-        HIP_CHECK(hipMemcpyAsync(t.data(), eng.tout.p, P * sizeof(double), hipMemcpyDeviceToHost, eng.stream));
-        HIP_CHECK(hipMemcpyAsync(st.data(), eng.status.p, st.size() * 4, hipMemcpyDeviceToHost, eng.stream));
-        HIP_CHECK(hipStreamSynchronize(eng.stream));
-        eng.collect();
-        float ms = 0.f;
-        HIP_CHECK(hipEventElapsedTime(&ms, eng.ev0, eng.ev1));
-        local.seconds += ms * 1e-3;
-        for (size_t q = 0; q < P; ++q) {
-            int64_t p = u.points[q];
-            bool delayed = false;
-            int32_t root_status = st[q * hp.slots + hp.root];
-            for (int32_t k = 0; k < hp.root; ++k) delayed |= st[q * hp.slots + k] != kFrontOk;
-            if (delayed) {
-                double tb = NAN;
+    {
+        std::lock_guard<std::mutex> lock(eng.mu);
+        for (auto& u : bt->units) {
+            const HostPlan& hp = u.structure->host[u.block];
+            const DevPlan& dp = u.structure->dev[u.block];
+            const size_t P = u.points.size();
+            PointsView pv{};
+            bind_inputs(u.in, pv);
+            pv.arena = eng.arena.p;
+            pv.arena_stride = hp.arena_stride;
+            pv.root_arena = eng.root_arena.p;
+            pv.root_stride = hp.root_stride;
+            pv.piv = eng.piv.as<int32_t>();
+            pv.piv_stride = hp.piv_stride;
+            pv.status = eng.status.as<int32_t>();
+            pv.status_stride = hp.slots;
+            pv.t_out = eng.tout.as<double>();
+            HIP_CHECK(hipEventRecord(eng.ev0, eng.stream));
+            sweep(eng, hp, dp, pv, int32_t(P), single);
+            HIP_CHECK(hipEventRecord(eng.ev1, eng.stream));
+            std::vector<double> t(P);
+            std::vector<int32_t> st(P * size_t(hp.slots));
+            HIP_CHECK(hipMemcpyAsync(t.data(), eng.tout.p, P * sizeof(double), hipMemcpyDeviceToHost, eng.stream));
+            HIP_CHECK(hipMemcpyAsync(st.data(), eng.status.p, st.size() * 4, hipMemcpyDeviceToHost, eng.stream));
+            HIP_CHECK(hipStreamSynchronize(eng.stream));
+            eng.collect();
+            float ms = 0.f;
+            HIP_CHECK(hipEventElapsedTime(&ms, eng.ev0, eng.ev1));
+            local.seconds += ms * 1e-3;
+            if (mixed) {
                 auto t0 = std::chrono::steady_clock::now();
-                int32_t rc = replanned_point(eng, bt->systems[p], bt->sigmas[p], u.block, single, tb);
+                std::vector<const cg_sigma*> sig;
+                for (auto p : u.points) sig.push_back(bt->sigmas[p]);
+                Refined rf = refine(eng, hp, dp, pv, sig, u.block, int64_t(bt->systems[u.points[0]]->spec.h.n), true);
                 local.seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-                ++local.delayed_points;
-                if (rc != CG_OK) status[p] = rc;
-                out_t[p] += tb;
-            } else if (root_status != kFrontOk || !std::isfinite(t[q])) {
-                status[p] = CG_ERR_SINGULAR;
-            } else {
-                out_t[p] += t[q];
+                local.refinement_iters += rf.iters;
+                for (size_t q = 0; q < P; ++q) {
+                    int64_t p = u.points[q];
+                    bool delayed = false;
+                    for (int32_t k = 0; k < hp.root; ++k) delayed |= st[q * hp.slots + k] != kFrontOk;
+                    if (delayed || st[q * hp.slots + hp.root] != kFrontOk || !rf.ok[q] || !std::isfinite(rf.t[q])) {
+                        redo[p] = 1;
+                        local.delayed_points += delayed;
+                    } else {
+                        local.max_refine_resid = std::max(local.max_refine_resid, rf.worst);
+                        out_t[p] += rf.t[q];
+                    }
+                }
+                continue;
             }
+            for (size_t q = 0; q < P; ++q) {
+                int64_t p = u.points[q];
+                bool delayed = false;
+                int32_t root_status = st[q * hp.slots + hp.root];
+                for (int32_t k = 0; k < hp.root; ++k) delayed |= st[q * hp.slots + k] != kFrontOk;
+                if (delayed) {
+                    double tb = NAN;
+                    auto t0 = std::chrono::steady_clock::now();
+                    int32_t rc = replanned_point(eng, bt->systems[p], bt->sigmas[p], u.block, single, tb);
+                    local.seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+                    ++local.delayed_points;
+                    if (rc != CG_OK) status[p] = rc;
+                    out_t[p] += tb;
+                } else if (root_status != kFrontOk || !std::isfinite(t[q])) {
+                    status[p] = CG_ERR_SINGULAR;
+                } else {
+                    out_t[p] += t[q];
+                }
+            }
+        }
+    }
+    // Mixed precision: points whose c32 fronts delayed or whose refinement did not converge are
+    // recomputed in complex f64 on the device.
+    std::vector<int64_t> again;
+    for (int64_t p = 0; p < count; ++p)
+        if (redo[p]) again.push_back(p);
+    if (!again.empty()) {
+        std::vector<const cg_system*> ss;
+        std::vector<const cg_sigma*> gg;
+        for (auto p : again) {
+            ss.push_back(bt->systems[p]);
+            gg.push_back(bt->sigmas[p]);
+        }
+        std::vector<double> t(again.size());
+        std::vector<int32_t> st(again.size());
+        cg_batch_stats s2{};
+        int32_t rc = batch_transmission(ss.data(), gg.data(), int64_t(again.size()), CG_DOUBLE, t.data(), st.data(), &s2);
+        if (rc != CG_OK) return rc;
+        local.seconds += s2.seconds;
+        for (size_t q = 0; q < again.size(); ++q) {
+            out_t[again[q]] = t[q];
+            status[again[q]] = st[q];
         }
     }
     for (int64_t p = 0; p < count; ++p) {
@@ -941,9 +1098,93 @@ int32_t kernel_stats(cg_kernel_stat* out, int32_t max_records) {
     return n;
 }
 
-int32_t backward(const cg_system*, const cg_sigma*, double* out_t, double*) {
-    *out_t = NAN;
-    return CG_ERR_INVALID;  // provided by the adjoint engine
+// Adjoint of T = Tr[Gamma_L G Gamma_R G^H], G = (z - H - Sigma)^-1: for an on-site energy eps_a,
+// dG/deps_a = G e_a e_a^T G, so dT/deps_a = 2 Re (G Gamma_R G^H Gamma_L G)_aa. With Gamma = W W^H,
+// psi = G E_R W_R (the columns on the right slice) and phi = G^T E_L conj(W_L) (rows on the left
+// slice, by a solve with A^T), t = W_L^H psi_L:  dT/deps_a = 2 Re sum_cd psi_ac conj(t_dc) phi_ad.
+// Both solves reuse the complex f64 factors of one forward sweep: the root solve, then the
+// backward substitution through every front (the right-hand sides live on the lead slices).
+int32_t backward(const cg_system* s, const cg_sigma* g, double* out_t, double* out_grad) {
+    Engine& eng = Engine::get();
+    if (!eng.ok) return CG_ERR_DEVICE;
+    std::lock_guard<std::mutex> lock(eng.mu);
+    auto sp = structure_of(eng, s);
+    const int64_t n_orb = int64_t(s->spec.h.n);
+    std::fill(out_grad, out_grad + n_orb, 0.0);
+    double total = 0.0;
+    for (size_t b = 0; b < sp->host.size(); ++b) {
+        const HostPlan& hp = sp->host[b];
+        const DevPlan& dp = sp->dev[b];
+        const Block& blk = s->sym.blocks[b];
+        const auto& sb = g->blocks[b];
+        PointsView pv{};
+        stage_points(eng, {s}, {g}, int(b), int(blk.left_orbitals.size()), int(blk.right_orbitals.size()), eng.scratch_in, pv);
+        eng.arena.ensure(size_t(hp.arena_stride) * sizeof(double2));
+        eng.root_arena.ensure(size_t(hp.root_stride) * sizeof(double2));
+        eng.piv.ensure(size_t(hp.piv_stride) * 4);
+        eng.status.ensure(size_t(hp.slots) * 4);
+        eng.tout.ensure(sizeof(double));
+        pv.arena = eng.arena.p;
+        pv.arena_stride = hp.arena_stride;
+        pv.root_arena = eng.root_arena.p;
+        pv.root_stride = hp.root_stride;
+        pv.piv = eng.piv.as<int32_t>();
+        pv.piv_stride = hp.piv_stride;
+        pv.status = eng.status.as<int32_t>();
+        pv.status_stride = hp.slots;
+        pv.t_out = eng.tout.as<double>();
+        sweep(eng, hp, dp, pv, 1, false);
+        std::vector<int32_t> st(hp.slots);
+        HIP_CHECK(hipMemcpyAsync(st.data(), eng.status.p, st.size() * 4, hipMemcpyDeviceToHost, eng.stream));
+        HIP_CHECK(hipStreamSynchronize(eng.stream));
+        for (int32_t k = 0; k < hp.root; ++k)
+            if (st[k] != kFrontOk) return CG_ERR_INVALID;  // delayed separators: not supported by the adjoint
+        if (st[hp.root] != kFrontOk) return CG_ERR_SINGULAR;
+        const int cr = sb.wr.cols, cl = sb.wl.cols;
+        const int nl = int(hp.left_orbs.size()), nr = int(hp.right_orbs.size());
+        if (cr == 0 || cl == 0) continue;
+        if (int64_t(hp.max_ne) * std::max(cr, cl) > kSolveLds) return CG_ERR_INVALID;
+        auto columns = [&](const CMat& w, int m, const DevBuf& orbs, bool transpose, std::vector<double2>& host) {
+            const int c = w.cols;
+            std::vector<double2> wh(size_t(m) * c);
+            for (int col = 0; col < c; ++col)
+                for (int a = 0; a < m; ++a) wh[a + size_t(col) * m] = double2{w(a, col).real(), w(a, col).imag()};
+            DevBuf dw, xb;
+            upload(dw, wh, eng.stream);
+            const size_t bytes = size_t(n_orb) * c * sizeof(double2);
+            xb.ensure(bytes);
+            HIP_CHECK(hipMemsetAsync(xb.p, 0, bytes, eng.stream));
+            SolveView xv{xb.as<double2>(), n_orb * c, int32_t(n_orb), c, dp.vars.as<int32_t>()};
+            launch_scatter(xv, orbs.as<int32_t>(), m, dw.as<double2>(), int64_t(m) * c, transpose, 1, eng.stream);
+            solve(eng, dp, pv, xv, 1, false, transpose, true);
+            host.resize(size_t(n_orb) * c);
+            HIP_CHECK(hipMemcpyAsync(host.data(), xb.p, bytes, hipMemcpyDeviceToHost, eng.stream));
+            HIP_CHECK(hipStreamSynchronize(eng.stream));
+        };
+        std::vector<double2> psi, phi;
+        columns(sb.wr, nr, dp.right_orbs, false, psi);  // psi = G E_R W_R
+        columns(sb.wl, nl, dp.left_orbs, true, phi);    // phi = G^T E_L conj(W_L)
+        auto at = [&](const std::vector<double2>& m, int64_t row, int col) {
+            double2 v = m[size_t(row) + size_t(col) * size_t(n_orb)];
+            return cplx(v.x, v.y);
+        };
+        CMat t(cl, cr);  // t = W_L^H psi_L
+        for (int c = 0; c < cr; ++c)
+            for (int d = 0; d < cl; ++d) {
+                cplx acc = 0.0;
+                for (int a = 0; a < nl; ++a) acc += std::conj(sb.wl(a, d)) * at(psi, hp.left_orbs[a], c);
+                t(d, c) = acc;
+                total += std::norm(acc);
+            }
+        for (int32_t row : hp.rows) {
+            cplx acc = 0.0;
+            for (int c = 0; c < cr; ++c)
+                for (int d = 0; d < cl; ++d) acc += at(psi, row, c) * std::conj(t(d, c)) * at(phi, row, d);
+            out_grad[row] += 2.0 * acc.real();
+        }
+    }
+    *out_t = total;
+    return CG_OK;
 }
 
 }  // namespace caroli::gpu

@@ -348,6 +348,127 @@ __global__ void __launch_bounds__(kThreads)
                       kRoot);
 }
 
+// One 16x16 tile of C -= A B on MFMA, computed as C^T -= B^T A^T so that the front's columns are
+// read and written contiguously: A = F[i0:i0+16, k0:k1] (rows of L), B = F[k0:k1, j0:j0+16]
+// (rows of U), K = k1 - k0 in slices of 4. Every lane runs every MFMA; entries outside [0, n)
+// and those `skip` selects are not stored.
+template <class T, class Skip>
+__device__ __forceinline__ void tile_update(T* F, int n, int i0, int j0, int k0, int k1, int lane, Skip skip) {
+    using M = Mfma<T>;
+    auto at = [&](int i, int j) -> T& { return F[i + int64_t(j) * n]; };
+    typename M::acc cr = {0, 0, 0, 0}, ci = {0, 0, 0, 0};
+    const int li = lane % 16, lk = lane / 16;
+    for (int kk = k0; kk < k1; kk += 4) {
+        const int k = kk + lk;
+        const int j = j0 + li, i = i0 + li;
+        T a = (k < k1 && j < n) ? at(k, j) : cx<T>(0.0, 0.0);
+        T b = (k < k1 && i < n) ? at(i, k) : cx<T>(0.0, 0.0);
+        cr = M::mac(a.x, b.x, cr);
+        cr = M::mac(-a.y, b.y, cr);
+        ci = M::mac(a.x, b.y, ci);
+        ci = M::mac(a.y, b.x, ci);
+    }
+    const int i = i0 + li;
+    for (int jj = 0; jj < 4; ++jj) {
+        const int j = j0 + M::row(lane, jj);
+        const bool store = i < n && j < n && !skip(i, j);
+        if (store) {
+            T& d = at(i, j);
+            d.x -= cr[jj];
+            d.y -= ci[jj];
+        }
+    }
+}
+// Blocked partial LU with a deferred Schur update: pivoting inside panels of kPanel columns,
+// the triangular solve of each panel's row block, and MFMA updates of the separator rows and
+// columns only; S = F22 - L21 U12 is formed once at the end with K = ne, so the Schur block is
+// read and written once instead of once per panel.
+template <class T>
+__device__ void eliminate_deferred(T* F, const NodeDesc& nd, int32_t* piv, int32_t* status, double tol, bool root) {
+    __shared__ double sv[16];
+    __shared__ int si[16];
+    __shared__ T l11[kPanel * kPanel];
+    const int n = nd.n, ne = nd.ne, ne1 = nd.ne1, tid = threadIdx.x, nt = blockDim.x;
+    const int lane = tid & 63, wave = tid >> 6, waves = nt >> 6;
+    auto at = [&](int i, int j) -> T& { return F[i + int64_t(j) * n]; };
+    double m = 0.0;
+    for (int idx = tid; idx < ne1 * ne1; idx += nt) m = fmax(m, cabs(at(idx % ne1, idx / ne1)));
+    const double scale = block_max(m, sv);
+    for (int k0 = 0; k0 < ne; k0 += kPanel) {
+        const int w = min(kPanel, ne - k0);
+        for (int k = k0; k < k0 + w; ++k) {
+            const int lim = k < ne1 ? ne1 : ne;
+            double big = -1.0;
+            int p = k;
+            for (int i = k + tid; i < lim; i += nt) {
+                double v = cabs(at(i, k));
+                if (v > big) { big = v; p = i; }
+            }
+            block_argmax(big, p, sv, si);
+            const bool ok = k < ne1 ? (big > tol * scale) : (big > 0.0 && big < INFINITY);
+            if (!ok) { if (tid == 0) *status = (k < ne1 && !root) ? kFrontDelayed : kFrontSingular; return; }
+            if (p != k)
+                for (int j = tid; j < n; j += nt) { T x = at(k, j); at(k, j) = at(p, j); at(p, j) = x; }
+            if (tid == 0) piv[k] = p;
+            __syncthreads();
+            const T inv = crecip(at(k, k));
+            for (int i = k + 1 + tid; i < n; i += nt) at(i, k) = cmul(at(i, k), inv);
+            __syncthreads();
+            const int cols = k0 + w - k - 1, rows = n - k - 1;
+            for (int idx = tid; idx < cols * rows; idx += nt) {
+                int i = k + 1 + idx % rows, j = k + 1 + idx / rows;
+                at(i, j) = csub(at(i, j), cmul(at(i, k), at(k, j)));
+            }
+            __syncthreads();
+        }
+        const int j1 = k0 + w;
+        if (j1 >= n) break;
+        for (int idx = tid; idx < kPanel * kPanel; idx += nt) {
+            int r = idx % kPanel, c = idx / kPanel;
+            l11[idx] = (r < w && c < w) ? at(k0 + r, k0 + c) : cx<T>(0.0, 0.0);
+        }
+        __syncthreads();
+        for (int j = j1 + tid; j < n; j += nt)
+            for (int r = 1; r < w; ++r) {
+                T acc = at(k0 + r, j);
+                for (int c = 0; c < r; ++c) acc = csub(acc, cmul(l11[r + c * kPanel], at(k0 + c, j)));
+                at(k0 + r, j) = acc;
+            }
+        __syncthreads();
+        if (j1 < ne) {
+            const int tiles_c = (n - j1 + 15) / 16;
+            for (int tj = wave; tj < tiles_c; tj += waves) {
+                const int j0 = j1 + 16 * tj;
+                const int rows_end = j0 < ne ? n : ne;
+                for (int i0 = j1; i0 < rows_end; i0 += 16)
+                    tile_update(F, n, i0, j0, k0, j1, lane, [&](int i, int j) { return i >= ne && j >= ne; });
+            }
+            __syncthreads();
+        }
+    }
+    if (ne < n) {
+        const int tiles = (n - ne + 15) / 16;
+        for (int t = wave; t < tiles * tiles; t += waves) {
+            const int tj = t / tiles, ti = t % tiles;
+            tile_update(F, n, ne + 16 * ti, ne + 16 * tj, 0, ne, lane, [](int, int) { return false; });
+        }
+    }
+    __syncthreads();
+    if (tid == 0) *status = kFrontOk;
+}
+
+template <class T, bool kRoot>
+__global__ void __launch_bounds__(kThreads)
+    factor_deferred_kernel(const NodeDesc* __restrict__ nodes, const int32_t* __restrict__ level, PointsView pv, double tol) {
+    const NodeDesc nd = nodes[level[blockIdx.x]];
+    const int64_t p = blockIdx.y;
+    T* F = (kRoot ? reinterpret_cast<T*>(pv.root_arena) + p * pv.root_stride
+                  : reinterpret_cast<T*>(pv.arena) + p * pv.arena_stride) +
+           nd.off;
+    eliminate_deferred(F, nd, pv.piv + p * pv.piv_stride + nd.piv_off, pv.status + p * pv.status_stride + nd.slot, tol,
+                       kRoot);
+}
+
 template <class T, int NMAX>
 __global__ void __launch_bounds__(kThreads)
     factor_small_kernel(const NodeDesc* __restrict__ nodes, const int32_t* __restrict__ level, PointsView pv, double tol) {
@@ -436,10 +557,24 @@ bool blocked_fronts() {
     return !(v && std::strcmp(v, "unblocked") == 0);
 }
 
+bool deferred_fronts() {
+    const char* v = std::getenv("CAROLI_FRONT_KERNEL");
+    return v && std::strcmp(v, "deferred") == 0;
+}
+
 void launch_factor_c128(const NodeDesc* nodes, const int32_t* level, int32_t count, const PointsView& pv,
                         int32_t points, double tol, bool small, bool root, hipStream_t st) {
     if (count <= 0 || points <= 0) return;
     const bool blk = blocked_fronts();
+    if (deferred_fronts() && !small) {
+        if (root)
+            hipLaunchKernelGGL((factor_deferred_kernel<double2, true>), grid2(count, points), dim3(kThreads), 0, st,
+                               nodes, level, pv, tol);
+        else
+            hipLaunchKernelGGL((factor_deferred_kernel<double2, false>), grid2(count, points), dim3(kThreads), 0, st,
+                               nodes, level, pv, tol);
+        return;
+    }
     if (root && blk)
         hipLaunchKernelGGL((factor_blocked_kernel<double2, true>), grid2(count, points), dim3(kThreads), 0, st, nodes,
                            level, pv, tol);

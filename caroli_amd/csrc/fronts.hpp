@@ -102,4 +102,15 @@ void launch_solve_root(const NodeDesc* nodes, const int32_t* level, const Points
 void launch_solve_backward(const NodeDesc* nodes, const int32_t* level, int32_t count, const PointsView& pv,
                            const SolveView& sv, int32_t points, bool single, bool transpose, hipStream_t st);
 
+/// Refinement helpers (all per point, on c columns of n_orb rows).
+void launch_scatter(const SolveView& sv, const int32_t* orbs, int32_t m, const double2* w, int64_t w_stride, bool conj,
+                    int32_t points, hipStream_t st);
+void launch_residual(const PointsView& pv, const SolveView& xv, const double2* b, double2* d, const int32_t* rows,
+                     int32_t nrows, const int64_t* row_ptr, const uint32_t* cols, const int32_t* left_pos,
+                     const int32_t* right_pos, const int32_t* left_orbs, int32_t nl, const int32_t* right_orbs,
+                     int32_t nr, int32_t points, hipStream_t st);
+void launch_update(const SolveView& xv, const double2* d, const int32_t* rows, int32_t nrows, double* ratio,
+                   int32_t points, hipStream_t st);
+void launch_gather(const SolveView& xv, const int32_t* orbs, int32_t m, double2* out, int32_t points, hipStream_t st);
+
 }  // namespace caroli::gpu

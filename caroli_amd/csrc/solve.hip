@@ -11,6 +11,8 @@
 // the parent's variables, so they are accumulated with FP64 atomics.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "cx.hpp"
 #include "fronts.hpp"
 
@@ -210,7 +212,130 @@ __global__ void __launch_bounds__(kThreads)
     for (int idx = tid; idx < ne * c; idx += kThreads) X[vars[idx % ne] + int64_t(idx / ne) * sv.ldx] = y[idx];
 }
 
+// B[orbs[a], col] = W[a, col] (or conj) for a < m, per point; B must be zero elsewhere.
+__global__ void __launch_bounds__(kThreads)
+    scatter_kernel(SolveView sv, const int32_t* __restrict__ orbs, int32_t m, const double2* __restrict__ w,
+                   int64_t w_stride, int32_t conj) {
+    const int64_t p = blockIdx.y;
+    double2* X = sv.x + p * sv.x_stride;
+    const double2* W = w + p * w_stride;
+    for (int idx = blockIdx.x * kThreads + threadIdx.x; idx < m * sv.c; idx += gridDim.x * kThreads) {
+        int a = idx % m, col = idx / m;
+        double2 v = W[a + int64_t(col) * m];
+        if (conj) v.y = -v.y;
+        X[orbs[a] + int64_t(col) * sv.ldx] = v;
+    }
+}
+
+// D = B - (z - H - Sigma_L - Sigma_R) X on the rows of a block, in FP64 (the residual of the
+// refinement). Sigma_L acts on the left-slice rows (positions left_pos), Sigma_R on the right.
+__global__ void __launch_bounds__(kThreads)
+    residual_kernel(PointsView pv, SolveView xv, const double2* __restrict__ b, double2* __restrict__ d,
+                    const int32_t* __restrict__ rows, int32_t nrows, const int64_t* __restrict__ row_ptr,
+                    const uint32_t* __restrict__ cols, const int32_t* __restrict__ left_pos,
+                    const int32_t* __restrict__ right_pos, const int32_t* __restrict__ left_orbs, int32_t nl,
+                    const int32_t* __restrict__ right_orbs, int32_t nr) {
+    const int64_t p = blockIdx.y;
+    const double2* X = xv.x + p * xv.x_stride;
+    const double2* B = b + p * xv.x_stride;
+    double2* D = d + p * xv.x_stride;
+    const double2* vals = pv.vals[p];
+    const double2 z = pv.z[p];
+    const double2* sl = pv.sigma + p * pv.sigma_stride;
+    const double2* sr = sl + int64_t(nl) * nl;
+    for (int idx = blockIdx.x * kThreads + threadIdx.x; idx < nrows * xv.c; idx += gridDim.x * kThreads) {
+        const int r = rows[idx % nrows], col = idx / nrows;
+        const double2* x = X + int64_t(col) * xv.ldx;
+        double2 acc = cmul(z, x[r]);
+        for (int64_t e = row_ptr[r]; e < row_ptr[r + 1]; ++e) acc = csub(acc, cmul(vals[e], x[cols[e]]));
+        const int lp = left_pos[r], rp = right_pos[r];
+        if (lp >= 0)
+            for (int a = 0; a < nl; ++a) acc = csub(acc, cmul(sl[lp + int64_t(a) * nl], x[left_orbs[a]]));
+        if (rp >= 0)
+            for (int a = 0; a < nr; ++a) acc = csub(acc, cmul(sr[rp + int64_t(a) * nr], x[right_orbs[a]]));
+        const int64_t at = r + int64_t(col) * xv.ldx;
+        D[at] = csub(B[at], acc);
+    }
+}
+
+// X += D on the block rows; ratio[p] = max |D| / max |X| (the relative correction).
+__global__ void __launch_bounds__(kThreads)
+    update_kernel(SolveView xv, const double2* __restrict__ d, const int32_t* __restrict__ rows, int32_t nrows,
+                  double* __restrict__ ratio) {
+    __shared__ double sd[kThreads / 64], sx[kThreads / 64];
+    const int64_t p = blockIdx.x;
+    double2* X = xv.x + p * xv.x_stride;
+    const double2* D = d + p * xv.x_stride;
+    double md = 0.0, mx = 0.0;
+    for (int idx = threadIdx.x; idx < nrows * xv.c; idx += kThreads) {
+        const int64_t at = rows[idx % nrows] + int64_t(idx / nrows) * xv.ldx;
+        double2 v = cadd(X[at], D[at]);
+        X[at] = v;
+        md = fmax(md, cabs(D[at]));
+        mx = fmax(mx, cabs(v));
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        md = fmax(md, __shfl_xor(md, off, 64));
+        mx = fmax(mx, __shfl_xor(mx, off, 64));
+    }
+    if ((threadIdx.x & 63) == 0) {
+        sd[threadIdx.x >> 6] = md;
+        sx[threadIdx.x >> 6] = mx;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < kThreads / 64; ++w) {
+            md = fmax(md, sd[w]);
+            mx = fmax(mx, sx[w]);
+        }
+        md = fmax(md, sd[0]);
+        mx = fmax(mx, sx[0]);
+        ratio[p] = mx > 0.0 ? md / mx : (md > 0.0 ? INFINITY : 0.0);
+    }
+}
+
+// out[a, col] = X[orbs[a], col] per point (m x c, column-major).
+__global__ void __launch_bounds__(kThreads)
+    gather_kernel(SolveView xv, const int32_t* __restrict__ orbs, int32_t m, double2* __restrict__ out) {
+    const int64_t p = blockIdx.y;
+    const double2* X = xv.x + p * xv.x_stride;
+    double2* O = out + p * int64_t(m) * xv.c;
+    for (int idx = blockIdx.x * kThreads + threadIdx.x; idx < m * xv.c; idx += gridDim.x * kThreads)
+        O[idx] = X[orbs[idx % m] + int64_t(idx / m) * xv.ldx];
+}
+
 }  // namespace
+
+inline dim3 rows_grid(int64_t work, int32_t points) {
+    return dim3(unsigned(std::min<int64_t>((work + kThreads - 1) / kThreads, 4096)), unsigned(points));
+}
+
+void launch_scatter(const SolveView& sv, const int32_t* orbs, int32_t m, const double2* w, int64_t w_stride, bool conj,
+                    int32_t points, hipStream_t st) {
+    if (points <= 0 || m <= 0 || sv.c <= 0) return;
+    hipLaunchKernelGGL(scatter_kernel, rows_grid(int64_t(m) * sv.c, points), dim3(kThreads), 0, st, sv, orbs, m, w,
+                       w_stride, conj ? 1 : 0);
+}
+
+void launch_residual(const PointsView& pv, const SolveView& xv, const double2* b, double2* d, const int32_t* rows,
+                     int32_t nrows, const int64_t* row_ptr, const uint32_t* cols, const int32_t* left_pos,
+                     const int32_t* right_pos, const int32_t* left_orbs, int32_t nl, const int32_t* right_orbs,
+                     int32_t nr, int32_t points, hipStream_t st) {
+    if (points <= 0 || xv.c <= 0) return;
+    hipLaunchKernelGGL(residual_kernel, rows_grid(int64_t(nrows) * xv.c, points), dim3(kThreads), 0, st, pv, xv, b, d,
+                       rows, nrows, row_ptr, cols, left_pos, right_pos, left_orbs, nl, right_orbs, nr);
+}
+
+void launch_update(const SolveView& xv, const double2* d, const int32_t* rows, int32_t nrows, double* ratio,
+                   int32_t points, hipStream_t st) {
+    if (points <= 0 || xv.c <= 0) return;
+    hipLaunchKernelGGL(update_kernel, dim3(unsigned(points)), dim3(kThreads), 0, st, xv, d, rows, nrows, ratio);
+}
+
+void launch_gather(const SolveView& xv, const int32_t* orbs, int32_t m, double2* out, int32_t points, hipStream_t st) {
+    if (points <= 0 || m <= 0 || xv.c <= 0) return;
+    hipLaunchKernelGGL(gather_kernel, rows_grid(int64_t(m) * xv.c, points), dim3(kThreads), 0, st, xv, orbs, m, out);
+}
 
 using SolveKernel = void (*)(const NodeDesc*, const int32_t*, PointsView, SolveView);
 

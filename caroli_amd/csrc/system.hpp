@@ -37,6 +37,7 @@ struct cg_system {
 struct cg_sigma {
     struct Block {
         caroli::CMat sl, sr, gl, gr;  // Sigma_L, Sigma_R and their broadenings
+        caroli::CMat wl, wr;          // Gamma = W W^H (one column per channel)
         int channels_l = 0, channels_r = 0;
         caroli::CMat left_h00, left_h01, right_h00, right_h01;
         std::vector<size_t> left_orbitals, right_orbitals;
@@ -48,6 +49,11 @@ struct cg_sigma {
 namespace caroli {
 /// Gamma = i (Sigma - Sigma^H) (crates/caroli-green/src/lib.rs:68-71).
 CMat broadening(const CMat& s);
+/// W with W W^H = Gamma for a positive semidefinite Gamma: diagonally pivoted Cholesky, stopped
+/// when the largest remaining diagonal falls below 1e-12 of the largest diagonal (the role of
+/// gamma_root, crates/caroli-green/src/nested.rs:1038-1047; W differs from it by a unitary,
+/// which leaves |W_L^H G W_R|^2 = Tr[Gamma_L G Gamma_R G^H] unchanged).
+CMat gamma_factor(const CMat& g);
 /// True when `g` holds self-energies of the leads and blocks of `s` (nested.rs:693-707).
 bool sigma_matches(const cg_system& s, const cg_sigma& g);
 }  // namespace caroli

@@ -315,3 +315,24 @@ def test_invalid_arguments_are_refused():
     ext = (C.c_uint64 * 3)(4, 4, 0)
     assert capi.cg_preset_create(9, ext, 0, 0, 0.0, 0.0, 1, None, C.byref(h)) == 6
     assert capi.cg_transmission_with(None, None, 0, None) == 6
+
+
+def test_oracle_gradient_matches_finite_differences():
+    # The FP64 gradient built from G columns (2 Re (G Gamma_R G^H Gamma_L G)_aa, the checker of the
+    # GPU adjoint) against central differences of the oracle's own transmission.
+    from test_gpu_mixed_adjoint import oracle_gradient
+
+    ext, w, e, h = [8, 5, 0], 40 / 72.9, 0.01, 1e-6
+    s = ca.NestedTwoTerminal.preset(ca.Model.BHZ, ext, disorder=1, strength=w, seed=1)
+    sig = s.self_energies(e)
+    sig_arr = [sig.block(b)[:2] for b in range(s.block_count())]
+    grad = oracle_gradient(s, e)
+    for i in (3, 17, 26):
+        pot = np.zeros(ext[0] * ext[1])
+        pot[i] = h
+        tp = OracleSystem(ca.NestedTwoTerminal.preset(ca.Model.BHZ, ext, disorder=1, strength=w, seed=1,
+                                                      site_potential=pot).arrays()).transmission_with(e, sig_arr)[0]
+        tm = OracleSystem(ca.NestedTwoTerminal.preset(ca.Model.BHZ, ext, disorder=1, strength=w, seed=1,
+                                                      site_potential=-pot).arrays()).transmission_with(e, sig_arr)[0]
+        fd = (tp - tm) / (2 * h)
+        assert abs(fd - grad[4 * i:4 * i + 4].sum()) < 1e-6 * max(1.0, abs(fd)), (i, fd)
