@@ -80,11 +80,13 @@ def main():
     flops_pt, bytes_pt = systems[0].work()
 
     # Dominant kernel family by time, and its roofline.
+    # The bound is the roof the kernel's algorithmic intensity meets first: flops/byte against
+    # the ridge point peak_flops / peak_bandwidth.
     dom = max(ks.items(), key=lambda kv: kv[1][1])
     name, (launches, secs, fl, by) = dom
-    bound = "mfma" if fl > 0 and fl / max(by, 1) > 10 else "hbm"
+    peak = PEAK_FP32_MFMA if prec != ca.Precision.Double else PEAK_FP64_MFMA
+    bound = "mfma" if fl > 0 and fl / max(by, 1) >= peak / PEAK_HBM else "hbm"
     if bound == "mfma":
-        peak = PEAK_FP32_MFMA if prec != ca.Precision.Double else PEAK_FP64_MFMA
         achieved = fl / secs
         roof = {"bound": "mfma", "kernel": name, "achieved": achieved / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
                 "frac": achieved / peak, "traffic": None, "launches": launches,
@@ -143,6 +145,8 @@ def main():
         "max_abs_dT_vs_cpu": max_dt,
         "status_ok": int((st == 0).sum()),
         "delayed_points": int(stats.delayed_points),
+        "refinement_iters": int(stats.refinement_iters),
+        "max_refine_correction": float(stats.max_refine_resid),
         "device_seconds_per_step": stats.seconds,
         "algorithmic_flops_per_point": flops_pt,
         "algorithmic_bytes_per_point": bytes_pt,

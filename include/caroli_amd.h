@@ -50,9 +50,10 @@ typedef struct cg_sigma cg_sigma;   /* LeadSelfEnergies (nested.rs:439-459) */
 /* NestedTwoTerminal::from_model (nested.rs:530-610) from the arrays the models produce:
  * the CSR Hamiltonian of the region (crates/caroli-model/src/sparse.rs:13-19; vals interleaved),
  * the sites (cells: 3 int64 per site; orbital count and first orbital per site), the lattice
- * dimension, the hopping range per axis, the transport axis, and the lead cells
- * (h00 = <cell|H|cell>, h01 = <cell|H|cell+axis>) of the left (first slice) and right (last
- * slice) leads, nl x nl and nr x nr. */
+ * dimension, the hopping range per axis, the transport axis, and the lead cells of the left
+ * (first slice) and right (last slice) leads, nl x nl and nr x nr: h00 = <cell|H|cell> and
+ * h01 = <cell|H|next cell outward>, i.e. cell - axis for the left lead and cell + axis for the
+ * right one (the Lead pairs of nested.rs:543-546). */
 int32_t cg_system_create(uint64_t n, const int64_t* row_ptr, const uint32_t* cols, const double* vals,
                          uint64_t nsites, const int64_t* cells, const int32_t* site_orbitals,
                          const uint64_t* site_offsets, int32_t dim, const int64_t* range, int32_t axis,
