@@ -73,6 +73,92 @@ void upload(DevBuf& d, const std::vector<T>& v, hipStream_t st) {
     HIP_CHECK(hipStreamSynchronize(st));
 }
 
+/// The fronts of one launch group (a tree level in one precision, or the root) split by kernel:
+/// small fronts in LDS size classes, large fronts with the tiles of their Schur blocks.
+struct LevelSet {
+    static constexpr int kClasses = 3;
+    std::vector<int32_t> lds[kClasses];
+    int32_t lds_n[kClasses] = {0, 0, 0};  // largest n of each class
+    std::vector<int32_t> large;
+    int32_t large_ne = 0;  // largest ne of the large fronts
+    std::vector<Tile> tiles;
+    // Algorithmic work (complex words, flops) for the kernel timers.
+    double lds_flops[kClasses] = {0, 0, 0}, lds_words[kClasses] = {0, 0, 0}, lds_s_words[kClasses] = {0, 0, 0},
+           lds_f_words[kClasses] = {0, 0, 0};
+    double asm_words = 0, border_flops = 0, border_words = 0, schur_flops = 0, schur_words = 0;
+};
+
+/// Device pointers of a LevelSet's lists.
+struct DevLevel {
+    const int32_t* lds[LevelSet::kClasses] = {nullptr, nullptr, nullptr};
+    const int32_t* large = nullptr;
+    const Tile* tiles = nullptr;
+};
+
+/// Splits the fronts `list` of `nodes` by kernel (root: the bordered root front, always large).
+LevelSet classify(const std::vector<NodeDesc>& nodes, const std::vector<KidDesc>& kids, const std::vector<int32_t>& list,
+                  bool single, bool root) {
+    LevelSet L;
+    const int limit = single ? kLdsFrontC64 : kLdsFrontC128;
+    const bool unblocked = unblocked_fronts();
+    for (int32_t q : list) {
+        const NodeDesc& d = nodes[q];
+        const double ne = d.ne, nb = d.n - d.ne, n2 = double(d.n) * d.n;
+        double kid_words = 0;
+        for (int c = d.kid_beg; c < d.kid_end; ++c) kid_words += double(kids[c].size) * kids[c].size;
+        const double fl = 8.0 / 3.0 * ne * ne * ne + 8.0 * ne * ne * nb + 8.0 * ne * nb * nb;
+        if (!root && d.n <= limit) {
+            const int c = d.n <= 16 ? 0 : d.n <= 32 ? 1 : 2;
+            L.lds[c].push_back(q);
+            L.lds_n[c] = std::max(L.lds_n[c], d.n);
+            L.lds_flops[c] += fl;
+            L.lds_words[c] += kid_words + 2.0 * (d.ent_end - d.ent_beg);
+            L.lds_s_words[c] += nb * nb;
+            L.lds_f_words[c] += n2;
+            continue;
+        }
+        L.large.push_back(q);
+        L.large_ne = std::max(L.large_ne, d.ne);
+        L.asm_words += n2 + kid_words + 2.0 * (d.ent_end - d.ent_beg);
+        if (unblocked) {
+            L.border_flops += fl;
+            L.border_words += 2.0 * n2;
+            continue;
+        }
+        L.border_flops += 8.0 / 3.0 * ne * ne * ne + 8.0 * ne * ne * nb;
+        L.border_words += 2.0 * (n2 - nb * nb);
+        L.schur_flops += 8.0 * ne * nb * nb;
+        L.schur_words += 2.0 * nb * nb + 2.0 * ne * nb;
+        for (int i0 = 0; i0 < d.n - d.ne; i0 += kTile)
+            for (int j0 = 0; j0 < d.n - d.ne; j0 += kTile) L.tiles.push_back({q, i0, j0, 0});
+    }
+    return L;
+}
+
+/// Appends a LevelSet's lists to the device list / tile arrays; `at` records the offsets.
+struct SetOffsets {
+    int32_t lds[LevelSet::kClasses], large, tiles;
+};
+SetOffsets append_set(const LevelSet& L, std::vector<int32_t>& lv, std::vector<Tile>& tl) {
+    SetOffsets o{};
+    for (int c = 0; c < LevelSet::kClasses; ++c) {
+        o.lds[c] = int32_t(lv.size());
+        lv.insert(lv.end(), L.lds[c].begin(), L.lds[c].end());
+    }
+    o.large = int32_t(lv.size());
+    lv.insert(lv.end(), L.large.begin(), L.large.end());
+    o.tiles = int32_t(tl.size());
+    tl.insert(tl.end(), L.tiles.begin(), L.tiles.end());
+    return o;
+}
+DevLevel resolve(const SetOffsets& o, const DevBuf& lv, const DevBuf& tl) {
+    DevLevel d;
+    for (int c = 0; c < LevelSet::kClasses; ++c) d.lds[c] = lv.as<int32_t>() + o.lds[c];
+    d.large = lv.as<int32_t>() + o.large;
+    d.tiles = tl.as<Tile>() + o.tiles;
+    return d;
+}
+
 /// Descriptors of one block's fronts (tree nodes, then the root), in host memory.
 struct HostPlan {
     std::vector<NodeDesc> nodes;
@@ -80,8 +166,11 @@ struct HostPlan {
     std::vector<int32_t> diag;
     std::vector<KidDesc> kids;
     std::vector<int32_t> maps;
-    /// Node indices per level: all, and split into LDS-sized and larger fronts per precision.
-    std::vector<std::vector<int32_t>> level_all, small128, large128, small64, large64;
+    /// Node indices per level, and the same split by kernel per front precision (0: complex
+    /// f64, 1: complex f32); the root's own split.
+    std::vector<std::vector<int32_t>> level_all;
+    std::vector<LevelSet> sets[2];
+    LevelSet root_set;
     int64_t arena_stride = 0, root_stride = 0, piv_stride = 0;
     int32_t slots = 0;
     int32_t root = 0;  // index of the root descriptor
@@ -93,9 +182,13 @@ struct HostPlan {
 };
 
 struct DevPlan {
-    DevBuf nodes, ent, diag, kids, maps, levels, vars, rows, left_pos, right_pos, row_ptr, cols, left_orbs, right_orbs;
-    // Ranges into `levels` for each list of each level.
-    std::vector<std::pair<int32_t, int32_t>> all, s128, l128, s64, l64;
+    DevBuf nodes, ent, diag, kids, maps, levels, tiles, vars, rows, left_pos, right_pos, row_ptr, cols, left_orbs,
+        right_orbs;
+    // Ranges into `levels` of each level's node list (for the solves); the kernel split of every
+    // level per front precision, and of the root.
+    std::vector<std::pair<int32_t, int32_t>> all;
+    std::vector<DevLevel> sets[2];
+    DevLevel root_set;
     int32_t root_at = 0;  // position of the root index in `levels`
 };
 
@@ -224,13 +317,10 @@ HostPlan build_plan(const Block& blk, const SparseMatrix& h) {
     hp.piv_stride = std::max<int64_t>(piv, 1);
     hp.slots = nn + 1;
     hp.level_all.assign(t.levels, {});
-    hp.small128 = hp.large128 = hp.small64 = hp.large64 = hp.level_all;
-    for (int k = 0; k < nn; ++k) {
-        int lv = t.height[k], n = hp.nodes[k].n;
-        hp.level_all[lv].push_back(k);
-        (n <= kLdsFrontC128 ? hp.small128 : hp.large128)[lv].push_back(k);
-        (n <= kLdsFrontC64 ? hp.small64 : hp.large64)[lv].push_back(k);
-    }
+    for (int k = 0; k < nn; ++k) hp.level_all[t.height[k]].push_back(k);
+    for (int prec = 0; prec < 2; ++prec)
+        for (auto& l : hp.level_all) hp.sets[prec].push_back(classify(hp.nodes, hp.kids, l, prec == 1, false));
+    hp.root_set = classify(hp.nodes, hp.kids, {hp.root}, false, true);
     return hp;
 }
 
@@ -249,20 +339,22 @@ void upload_plan(const HostPlan& hp, DevPlan& dp, const SparseMatrix& h, hipStre
     upload(dp.kids, hp.kids, st);
     upload(dp.maps, hp.maps, st);
     std::vector<int32_t> lv;
-    auto add = [&](const std::vector<std::vector<int32_t>>& lists, std::vector<std::pair<int32_t, int32_t>>& rng) {
-        for (auto& l : lists) {
-            rng.push_back({int32_t(lv.size()), int32_t(l.size())});
-            lv.insert(lv.end(), l.begin(), l.end());
-        }
-    };
-    add(hp.level_all, dp.all);
-    add(hp.small128, dp.s128);
-    add(hp.large128, dp.l128);
-    add(hp.small64, dp.s64);
-    add(hp.large64, dp.l64);
+    std::vector<Tile> tl;
+    for (auto& l : hp.level_all) {
+        dp.all.push_back({int32_t(lv.size()), int32_t(l.size())});
+        lv.insert(lv.end(), l.begin(), l.end());
+    }
+    std::vector<SetOffsets> offs[2];
+    for (int prec = 0; prec < 2; ++prec)
+        for (auto& L : hp.sets[prec]) offs[prec].push_back(append_set(L, lv, tl));
+    SetOffsets root_off = append_set(hp.root_set, lv, tl);
     dp.root_at = int32_t(lv.size());
     lv.push_back(hp.root);
     upload(dp.levels, lv, st);
+    upload(dp.tiles, tl, st);
+    for (int prec = 0; prec < 2; ++prec)
+        for (auto& o : offs[prec]) dp.sets[prec].push_back(resolve(o, dp.levels, dp.tiles));
+    dp.root_set = resolve(root_off, dp.levels, dp.tiles);
 }
 
 /// Device copies of the per-point inputs of one launch unit.
@@ -271,12 +363,22 @@ struct Inputs {
     int64_t stride = 0;
 };
 
-/// Kernel families timed by cg_kernel_timing.
-enum Family { kAssemble, kFactorLds, kFactorGlobal, kRootAssemble, kRootFactor, kTrace, kFamilies };
-// Large fronts and the root run factor_blocked_kernel (or factor_global_kernel when
-// CAROLI_FRONT_KERNEL=unblocked, and for complex f32 fronts).
-const char* const kFamilyNames[kFamilies] = {"assemble_kernel", "factor_small_kernel", "factor_large_kernel",
-                                             "assemble_kernel<root>", "factor_large_kernel<root>", "trace_kernel"};
+/// Kernel families timed by cg_kernel_timing (border_kernel stands for factor_global_kernel when
+/// CAROLI_FRONT_KERNEL=unblocked).
+enum Family {
+    kFrontLds,
+    kAssemble,
+    kBorder,
+    kSchur,
+    kRootAssemble,
+    kRootBorder,
+    kRootSchur,
+    kTrace,
+    kFamilies
+};
+const char* const kFamilyNames[kFamilies] = {"front_lds_kernel", "assemble_kernel",        "border_kernel",
+                                             "schur_kernel",     "assemble_kernel<root>", "border_kernel<root>",
+                                             "schur_kernel<root>", "trace_kernel"};
 
 struct Engine {
     std::mutex mu;
@@ -417,77 +519,61 @@ void stage_points(Engine& eng, const std::vector<const cg_system*>& sys, const s
     bind_inputs(in, pv);
 }
 
-/// Algorithmic flops (8 per complex MAC) and bytes (one read and one write of the front) of
-/// eliminating the fronts `list`; bytes of assembling them (front written, children read).
-double list_flops(const std::vector<NodeDesc>& nodes, const std::vector<int32_t>& list) {
-    double f = 0;
-    for (int32_t k : list) {
-        double ne = nodes[k].ne, nb = nodes[k].n - nodes[k].ne;
-        f += 8.0 / 3.0 * ne * ne * ne + 8.0 * ne * ne * nb + 8.0 * ne * nb * nb;
+/// Launches the fronts of one LevelSet for the P points of `pv`: the LDS classes, then the large
+/// fronts (assembly, border, Schur tiles). `root`: the bordered root front (complex f64 whatever
+/// the precision of the fronts below it).
+void run_level(Engine& eng, const LevelSet& L, const DevLevel& D, const NodeDesc* nodes, const Entry* ent,
+               const int32_t* diag, const KidDesc* kids, const int32_t* maps, const PointsView& pv, int32_t P,
+               bool single, bool root) {
+    const double es = single ? 8.0 : 16.0, fes = root ? 16.0 : es;
+    const double tol = single && !root ? kTolSingle : kTolDouble;
+    for (int c = 0; c < LevelSet::kClasses; ++c) {
+        const int32_t count = int32_t(L.lds[c].size());
+        if (count == 0) continue;
+        const double words = L.lds_words[c] + (pv.keep ? L.lds_f_words[c] : L.lds_s_words[c]);
+        eng.timed(kFrontLds, P * L.lds_flops[c], P * es * words, [&] {
+            launch_front_lds(nodes, D.lds[c], count, L.lds_n[c], ent, diag, kids, maps, pv, P, tol, single, true,
+                             eng.stream);
+        });
     }
-    return f;
-}
-double list_factor_bytes(const std::vector<NodeDesc>& nodes, const std::vector<int32_t>& list, double es) {
-    double b = 0;
-    for (int32_t k : list) b += 2.0 * es * double(nodes[k].n) * nodes[k].n;
-    return b;
-}
-double list_assemble_bytes(const HostPlan& hp, const std::vector<int32_t>& list, double es, double kid_es) {
-    double b = 0;
-    for (int32_t k : list) {
-        const NodeDesc& d = hp.nodes[k];
-        b += es * double(d.n) * d.n + 32.0 * (d.ent_end - d.ent_beg);
-        for (int c = d.kid_beg; c < d.kid_end; ++c) b += kid_es * double(hp.kids[c].size) * hp.kids[c].size;
+    const int32_t count = int32_t(L.large.size());
+    if (count == 0) return;
+    eng.timed(root ? kRootAssemble : kAssemble, 0.0, P * fes * L.asm_words, [&] {
+        if (root)
+            launch_assemble_root(nodes, D.large, count, ent, diag, kids, maps, pv, P, single ? 1 : 0, eng.stream);
+        else if (single)
+            launch_assemble_c64(nodes, D.large, count, ent, diag, kids, maps, pv, P, eng.stream);
+        else
+            launch_assemble_c128(nodes, D.large, count, ent, diag, kids, maps, pv, P, eng.stream);
+    });
+    const bool fsingle = single && !root;
+    if (unblocked_fronts()) {
+        eng.timed(root ? kRootBorder : kBorder, P * L.border_flops, P * fes * L.border_words, [&] {
+            launch_factor_unblocked(nodes, D.large, count, pv, P, tol, fsingle, root, eng.stream);
+        });
+        return;
     }
-    return b;
+    eng.timed(root ? kRootBorder : kBorder, P * L.border_flops, P * fes * L.border_words, [&] {
+        launch_border(nodes, D.large, count, L.large_ne, pv, P, tol, fsingle, root, eng.stream);
+    });
+    eng.timed(root ? kRootSchur : kSchur, P * L.schur_flops, P * fes * L.schur_words, [&] {
+        launch_schur(nodes, D.tiles, int32_t(L.tiles.size()), pv, P, fsingle, root, eng.stream);
+    });
 }
 
 /// One sweep over the static plan of block b for the points of `pv`.
 void sweep(Engine& eng, const HostPlan& hp, const DevPlan& dp, const PointsView& pv, int32_t P, bool single) {
     const NodeDesc* nodes = dp.nodes.as<NodeDesc>();
-    const int32_t* lv = dp.levels.as<int32_t>();
     const Entry* ent = dp.ent.as<Entry>();
     const int32_t* diag = dp.diag.as<int32_t>();
     const KidDesc* kids = dp.kids.as<KidDesc>();
     const int32_t* maps = dp.maps.as<int32_t>();
-    const double es = single ? 8.0 : 16.0;
-    const bool t = eng.timing;
-    for (size_t l = 0; l < dp.all.size(); ++l) {
-        auto [a0, ac] = dp.all[l];
-        auto& sm = single ? hp.small64[l] : hp.small128[l];
-        auto& lg = single ? hp.large64[l] : hp.large128[l];
-        auto& rs = single ? dp.s64[l] : dp.s128[l];
-        auto& rl = single ? dp.l64[l] : dp.l128[l];
-        eng.timed(kAssemble, 0.0, t ? P * list_assemble_bytes(hp, hp.level_all[l], es, es) : 0.0, [&] {
-            if (single)
-                launch_assemble_c64(nodes, lv + a0, ac, ent, diag, kids, maps, pv, P, eng.stream);
-            else
-                launch_assemble_c128(nodes, lv + a0, ac, ent, diag, kids, maps, pv, P, eng.stream);
-        });
-        if (rs.second > 0)
-            eng.timed(kFactorLds, t ? P * list_flops(hp.nodes, sm) : 0.0, t ? P * list_factor_bytes(hp.nodes, sm, es) : 0.0, [&] {
-                if (single)
-                    launch_factor_c64(nodes, lv + rs.first, rs.second, pv, P, kTolSingle, true, eng.stream);
-                else
-                    launch_factor_c128(nodes, lv + rs.first, rs.second, pv, P, kTolDouble, true, false, eng.stream);
-            });
-        if (rl.second > 0)
-            eng.timed(kFactorGlobal, t ? P * list_flops(hp.nodes, lg) : 0.0, t ? P * list_factor_bytes(hp.nodes, lg, es) : 0.0, [&] {
-                if (single)
-                    launch_factor_c64(nodes, lv + rl.first, rl.second, pv, P, kTolSingle, false, eng.stream);
-                else
-                    launch_factor_c128(nodes, lv + rl.first, rl.second, pv, P, kTolDouble, false, false, eng.stream);
-            });
-    }
+    const int prec = single ? 1 : 0;
+    for (size_t l = 0; l < hp.level_all.size(); ++l)
+        run_level(eng, hp.sets[prec][l], dp.sets[prec][l], nodes, ent, diag, kids, maps, pv, P, single, false);
+    run_level(eng, hp.root_set, dp.root_set, nodes, ent, diag, kids, maps, pv, P, single, true);
     const NodeDesc& r = hp.nodes[hp.root];
-    const double k = r.ne, nl = r.nl, nr = r.nr;
-    std::vector<int32_t> rootl{hp.root};
-    eng.timed(kRootAssemble, 0.0, t ? P * list_assemble_bytes(hp, rootl, 16.0, es) : 0.0, [&] {
-        launch_assemble_root(nodes, lv + dp.root_at, 1, ent, diag, kids, maps, pv, P, single ? 1 : 0, eng.stream);
-    });
-    eng.timed(kRootFactor, P * (8.0 / 3.0 * k * k * k + 8.0 * k * k * nr), P * 32.0 * double(r.n) * r.n, [&] {
-        launch_factor_c128(nodes, lv + dp.root_at, 1, pv, P, kTolDouble, false, true, eng.stream);
-    });
+    const double nl = r.nl, nr = r.nr;
     eng.timed(kTrace, P * 8.0 * nl * nr * (nl + nr), P * 16.0 * (nl * nl + nr * nr + nl * nr), [&] {
         launch_trace(nodes + hp.root, pv, P, eng.stream);
     });
@@ -514,7 +600,7 @@ int32_t replanned_point(Engine& eng, const cg_system* s, const cg_sigma* g, int 
     std::vector<const cg_sigma*> one_g{g};
     PointsView pv{};
     stage_points(eng, one_s, one_g, b, int(blk.left_orbitals.size()), int(blk.right_orbitals.size()), eng.scratch_in, pv);
-    DevBuf arena, piv, status, tout, root, dn, dent, ddiag, dkids, dmaps, dlv;
+    DevBuf arena, piv, status, tout, root, dn, dent, ddiag, dkids, dmaps;
     int64_t used = 0, piv_used = 0;
     auto grow = [&](DevBuf& buf, int64_t need_bytes, int64_t keep_bytes) {
         if (size_t(need_bytes) <= buf.bytes) return;
@@ -619,26 +705,18 @@ int32_t replanned_point(Engine& eng, const cg_system* s, const cg_sigma* g, int 
         upload(ddiag, diag, eng.stream);
         upload(dkids, kids, eng.stream);
         upload(dmaps, maps, eng.stream);
-        upload(dlv, list, eng.stream);
         const NodeDesc* dd = dn.as<NodeDesc>();
         if (!root_level) {
-            std::vector<int32_t> small, large;
-            for (size_t q = 0; q < nodes.size(); ++q)
-                (nodes[q].n <= (single ? kLdsFrontC64 : kLdsFrontC128) ? small : large).push_back(int32_t(q));
-            DevBuf ds, dl;
-            upload(ds, small, eng.stream);
-            upload(dl, large, eng.stream);
-            if (single) {
-                launch_assemble_c64(dd, dlv.as<int32_t>(), int32_t(list.size()), dent.as<Entry>(), ddiag.as<int32_t>(),
-                                    dkids.as<KidDesc>(), dmaps.as<int32_t>(), pv, 1, eng.stream);
-                launch_factor_c64(dd, ds.as<int32_t>(), int32_t(small.size()), pv, 1, kTolSingle, true, eng.stream);
-                launch_factor_c64(dd, dl.as<int32_t>(), int32_t(large.size()), pv, 1, kTolSingle, false, eng.stream);
-            } else {
-                launch_assemble_c128(dd, dlv.as<int32_t>(), int32_t(list.size()), dent.as<Entry>(), ddiag.as<int32_t>(),
-                                     dkids.as<KidDesc>(), dmaps.as<int32_t>(), pv, 1, eng.stream);
-                launch_factor_c128(dd, ds.as<int32_t>(), int32_t(small.size()), pv, 1, kTolDouble, true, false, eng.stream);
-                launch_factor_c128(dd, dl.as<int32_t>(), int32_t(large.size()), pv, 1, kTolDouble, false, false, eng.stream);
-            }
+            // The same kernels as the static sweep, on this level's re-planned fronts.
+            LevelSet L = classify(nodes, kids, list, single, false);
+            std::vector<int32_t> lv;
+            std::vector<Tile> tl;
+            SetOffsets o = append_set(L, lv, tl);
+            DevBuf dl, dt;
+            upload(dl, lv, eng.stream);
+            upload(dt, tl, eng.stream);
+            run_level(eng, L, resolve(o, dl, dt), dd, dent.as<Entry>(), ddiag.as<int32_t>(), dkids.as<KidDesc>(),
+                      dmaps.as<int32_t>(), pv, 1, single, false);
             HIP_CHECK(hipGetLastError());
             HIP_CHECK(hipMemcpyAsync(st_host.data(), status.p, sizeof(int32_t) * (nn + 1), hipMemcpyDeviceToHost, eng.stream));
             HIP_CHECK(hipStreamSynchronize(eng.stream));
@@ -672,9 +750,15 @@ int32_t replanned_point(Engine& eng, const cg_system* s, const cg_sigma* g, int 
             const NodeDesc& d = nodes[0];
             root.ensure(size_t(int64_t(d.n) * d.n) * sizeof(double2));
             pv.root_arena = root.p;
-            launch_assemble_root(dd, dlv.as<int32_t>(), 1, dent.as<Entry>(), ddiag.as<int32_t>(), dkids.as<KidDesc>(),
-                                 dmaps.as<int32_t>(), pv, 1, single ? 1 : 0, eng.stream);
-            launch_factor_c128(dd, dlv.as<int32_t>(), 1, pv, 1, kTolDouble, false, true, eng.stream);
+            LevelSet L = classify(nodes, kids, list, single, true);
+            std::vector<int32_t> lv;
+            std::vector<Tile> tl;
+            SetOffsets o = append_set(L, lv, tl);
+            DevBuf dl, dt;
+            upload(dl, lv, eng.stream);
+            upload(dt, tl, eng.stream);
+            run_level(eng, L, resolve(o, dl, dt), dd, dent.as<Entry>(), ddiag.as<int32_t>(), dkids.as<KidDesc>(),
+                      dmaps.as<int32_t>(), pv, 1, single, true);
             launch_trace(dd, pv, 1, eng.stream);
             HIP_CHECK(hipGetLastError());
             int32_t rs = 0;
@@ -910,6 +994,7 @@ int32_t run_batch(cg_batch* bt, double* out_t, int32_t* out_status, cg_batch_sta
             pv.status = eng.status.as<int32_t>();
             pv.status_stride = hp.slots;
             pv.t_out = eng.tout.as<double>();
+            pv.keep = mixed ? 1 : 0;  // the refinement solves with the factors
             HIP_CHECK(hipEventRecord(eng.ev0, eng.stream));
             sweep(eng, hp, dp, pv, int32_t(P), single);
             HIP_CHECK(hipEventRecord(eng.ev1, eng.stream));
@@ -1019,7 +1104,7 @@ int32_t front_schur(const double* f, int64_t n, int64_t ne, int64_t ne1, int32_t
     d.ne1 = int32_t(ne1);
     d.slot = 0;
     d.piv_off = 0;
-    DevBuf fb, nodes, lv, pivb, stb;
+    DevBuf fb, nodes, lv, pivb, stb, tiles;
     const size_t nn = size_t(n) * size_t(n);
     if (single) {
         std::vector<float2> h(nn);
@@ -1039,11 +1124,22 @@ int32_t front_schur(const double* f, int64_t n, int64_t ne, int64_t ne1, int32_t
     pv.piv = pivb.as<int32_t>();
     pv.status = stb.as<int32_t>();
     pv.status_stride = 1;
+    pv.keep = 1;
     const bool small = n <= (single ? kLdsFrontC64 : kLdsFrontC128);
-    if (single)
-        launch_factor_c64(nodes.as<NodeDesc>(), lv.as<int32_t>(), 1, pv, 1, kTolSingle, small, eng.stream);
-    else
-        launch_factor_c128(nodes.as<NodeDesc>(), lv.as<int32_t>(), 1, pv, 1, kTolDouble, small, false, eng.stream);
+    const double tol = single ? kTolSingle : kTolDouble;
+    if (small) {
+        launch_front_lds(nodes.as<NodeDesc>(), lv.as<int32_t>(), 1, int32_t(n), nullptr, nullptr, nullptr, nullptr, pv, 1,
+                         tol, single, false, eng.stream);
+    } else if (unblocked_fronts()) {
+        launch_factor_unblocked(nodes.as<NodeDesc>(), lv.as<int32_t>(), 1, pv, 1, tol, single, false, eng.stream);
+    } else {
+        std::vector<Tile> tl;
+        for (int i0 = 0; i0 < n - ne; i0 += kTile)
+            for (int j0 = 0; j0 < n - ne; j0 += kTile) tl.push_back({0, i0, j0, 0});
+        upload(tiles, tl, eng.stream);
+        launch_border(nodes.as<NodeDesc>(), lv.as<int32_t>(), 1, int32_t(ne), pv, 1, tol, single, false, eng.stream);
+        launch_schur(nodes.as<NodeDesc>(), tiles.as<Tile>(), int32_t(tl.size()), pv, 1, single, false, eng.stream);
+    }
     HIP_CHECK(hipGetLastError());
     int32_t st = 0;
     HIP_CHECK(hipMemcpyAsync(&st, stb.p, 4, hipMemcpyDeviceToHost, eng.stream));
@@ -1133,6 +1229,7 @@ int32_t backward(const cg_system* s, const cg_sigma* g, double* out_t, double* o
         pv.status = eng.status.as<int32_t>();
         pv.status_stride = hp.slots;
         pv.t_out = eng.tout.as<double>();
+        pv.keep = 1;  // the adjoint solves with the factors
         sweep(eng, hp, dp, pv, 1, false);
         std::vector<int32_t> st(hp.slots);
         HIP_CHECK(hipMemcpyAsync(st.data(), eng.status.p, st.size() * 4, hipMemcpyDeviceToHost, eng.stream));

@@ -1,18 +1,23 @@
 // Front kernels of the multifrontal elimination on CDNA4 (gfx950).
 //
-// One workgroup (256 lanes = 4 wavefronts of 64) per (front, point): the launch grid is the fronts
-// of one tree level times the points of the batch, so thousands of small fronts fill the 256 CUs.
-//  - assemble: F = z - H on the separator, the children's Schur complements added at their
-//    positions (extend-add, coalesced along the columns), and at the root the lead self-energies
-//    and the unit borders that make the Green's function block the Schur complement of the front.
-//  - factor: partial LU with row pivoting restricted to the separator rows, the pivot test of
-//    crates/caroli-dense/src/field.rs:61 and nested.rs:1167-1223, and the rank-1 updates that leave
-//    S = F22 - F21 F11^-1 F12 in the trailing block. Small fronts run in LDS.
-//  - trace: the Caroli trace Tr[Gamma_L G Gamma_R G^H] as one fused reduction.
+// Every launch covers the fronts of one tree level (or one size class of them) times the points
+// of the batch, so thousands of fronts fill the 256 CUs.
+//  - front_lds_kernel: small fronts (n <= 64 in complex f64, <= 90 in f32) are assembled straight
+//    into LDS (F = z - H on the separator plus the children's Schur complements) and eliminated
+//    there, one wave per front: partial LU with row pivoting restricted to the separator rows and
+//    the pivot test of crates/caroli-dense/src/field.rs:61 / nested.rs:1167-1223.
+//  - large fronts: assemble_kernel writes F to HBM; border_kernel factors the separator block and
+//    the border strips (F11 = P^T L11 U11, L21, U12) panel by panel; schur_kernel forms
+//    S = F22 - L21 U12 on MFMA over 64 x 64 tiles, many workgroups per front.
+//  - the root is a large front bordered by the lead slices (assemble_kernel<root>), whose Schur
+//    block is -G_LR; trace_kernel evaluates Tr[Gamma_L G Gamma_R G^H] from it.
 #include <hip/hip_runtime.h>
 
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
+#include <set>
+#include <stdexcept>
 
 #include "cx.hpp"
 #include "fronts.hpp"
@@ -213,275 +218,433 @@ struct Mfma<float2> {
     static __device__ __forceinline__ int row(int lane, int j) { return 4 * (lane / 16) + j; }
 };
 
-constexpr int kPanel = 16;  // panel width of the blocked elimination
+// Largest value and smallest index on ties over one wavefront, with a payload; every lane gets
+// the result.
+__device__ __forceinline__ void wave_argmax(double& v, int& idx, int& pay) {
+    for (int off = 32; off > 0; off >>= 1) {
+        const double ov = __shfl_xor(v, off, 64);
+        const int oi = __shfl_xor(idx, off, 64), op = __shfl_xor(pay, off, 64);
+        if (ov > v || (ov == v && oi < idx)) {
+            v = ov;
+            idx = oi;
+            pay = op;
+        }
+    }
+}
+__device__ __forceinline__ void wave_argmax(double& v, int& idx) {
+    int pay = 0;
+    wave_argmax(v, idx, pay);
+}
 
-// Blocked right-looking partial LU of a front in global memory (the same factorisation as
-// `eliminate`, panel by panel): unblocked pivoting inside a panel of kPanel columns, the
-// triangular solve of the panel's row block, then the trailing update on MFMA, computed as
-// C^T -= U12^T L21^T so that the front's columns are read and written contiguously.
-template <class T>
-__device__ void eliminate_blocked(T* F, const NodeDesc& nd, int32_t* piv, int32_t* status, double tol, bool root) {
-    using R = typename Real<T>::type;
-    using M = Mfma<T>;
-    __shared__ double sv[16];
-    __shared__ int si[16];
-    __shared__ T l11[kPanel * kPanel];
-    const int n = nd.n, ne = nd.ne, ne1 = nd.ne1, tid = threadIdx.x;
-    const int lane = tid & 63, wave = tid >> 6;
-    auto at = [&](int i, int j) -> T& { return F[i + int64_t(j) * n]; };
+// ---------------------------------------------------------------------------------------------
+// Small fronts: assembly and elimination in LDS, one workgroup of NT lanes per (front, point).
+//
+// The front is assembled straight into LDS (no HBM copy of the assembled front), eliminated with
+// one lane per row, and written back once: the whole factored front when the factors are kept
+// for the solves (pv.keep), else only the Schur block the parent reads. Lane t owns physical row
+// t of the LDS copy for the whole elimination; a row interchange only swaps the logical positions
+// two lanes track, and the rows are written back in logical order, which leaves the factors of
+// interchanging whole rows (LAPACK's convention, the one the solves use).
+template <class T, int NT, bool kAssemble>
+__global__ void __launch_bounds__(NT)
+    front_lds_kernel(const NodeDesc* __restrict__ nodes, const int32_t* __restrict__ level,
+                     const Entry* __restrict__ ent, const int32_t* __restrict__ diag,
+                     const KidDesc* __restrict__ kids, const int32_t* __restrict__ maps, PointsView pv, double tol) {
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    __shared__ double sv[NT / 64];
+    __shared__ int32_t perm[kLdsMaxN];
+    __shared__ double rv[2][NT / 64];
+    __shared__ int32_t ri[2][NT / 64], rp[2][NT / 64];
+    T* W = reinterpret_cast<T*>(lds_raw);
+    const NodeDesc nd = nodes[level[blockIdx.x]];
+    const int64_t p = blockIdx.y;
+    T* F = reinterpret_cast<T*>(pv.arena) + p * pv.arena_stride + nd.off;
+    int32_t* piv = pv.piv + p * pv.piv_stride + nd.piv_off;
+    int32_t* status = pv.status + p * pv.status_stride + nd.slot;
+    const int n = nd.n, ne = nd.ne, ne1 = nd.ne1, tid = threadIdx.x, lane = tid & 63;
+    const int n2 = n * n;
+    if (kAssemble) {
+        const T zero = cx<T>(0.0, 0.0);
+        for (int i = tid; i < n2; i += NT) W[i] = zero;
+        __syncthreads();
+        const double2* vals = pv.vals[p];
+        for (int e = nd.ent_beg + tid; e < nd.ent_end; e += NT) {
+            Entry en = ent[e];
+            double2 v = vals[en.k];
+            if (en.conj) v.y = -v.y;
+            const int at = en.a + en.b * n;
+            W[at] = csub(W[at], convert<T>(v));
+        }
+        __syncthreads();
+        const T z = convert<T>(pv.z[p]);
+        for (int d = nd.diag_beg + tid; d < nd.diag_end; d += NT) {
+            const int q = diag[d];
+            W[q + q * n] = cadd(W[q + q * n], z);
+        }
+        __syncthreads();
+        const T* kidbase = reinterpret_cast<const T*>(pv.arena) + p * pv.arena_stride;
+        for (int c = nd.kid_beg; c < nd.kid_end; ++c) {
+            const KidDesc kd = kids[c];
+            const T* S = kidbase + kd.src;
+            const int32_t* map = maps + kd.map_beg;
+            const int m = kd.size, m2 = m * m;
+            for (int idx = tid; idx < m2; idx += NT) {
+                const int a = idx % m, b = idx / m;
+                const int at = map[a] + map[b] * n;
+                W[at] = cadd(W[at], S[a + int64_t(b) * kd.ld]);
+            }
+            __syncthreads();
+        }
+    } else {
+        for (int i = tid; i < n2; i += NT) W[i] = F[i];
+        __syncthreads();
+    }
     double m = 0.0;
-    for (int idx = tid; idx < ne1 * ne1; idx += kThreads) m = fmax(m, cabs(at(idx % ne1, idx / ne1)));
+    for (int idx = tid; idx < ne1 * ne1; idx += NT) m = fmax(m, cabs(W[(idx % ne1) + (idx / ne1) * n]));
     const double scale = block_max(m, sv);
-    for (int k0 = 0; k0 < ne; k0 += kPanel) {
-        const int w = min(kPanel, ne - k0);
-        // 1. Panel: columns [k0, k0 + w), rows [k0, n).
-        for (int k = k0; k < k0 + w; ++k) {
-            const int lim = k < ne1 ? ne1 : ne;
-            double big = -1.0;
-            int p = k;
-            for (int i = k + tid; i < lim; i += kThreads) {
-                double v = cabs(at(i, k));
-                if (v > big) {
-                    big = v;
-                    p = i;
-                }
-            }
-            block_argmax(big, p, sv, si);
-            const bool ok = k < ne1 ? (big > tol * scale) : (big > 0.0 && big < INFINITY);
-            if (!ok) {
-                if (tid == 0) *status = (k < ne1 && !root) ? kFrontDelayed : kFrontSingular;
-                return;
-            }
-            if (p != k)
-                for (int j = tid; j < n; j += kThreads) {
-                    T x = at(k, j);
-                    at(k, j) = at(p, j);
-                    at(p, j) = x;
-                }
-            if (tid == 0) piv[k] = p;
-            __syncthreads();
-            const T inv = crecip(at(k, k));
-            for (int i = k + 1 + tid; i < n; i += kThreads) at(i, k) = cmul(at(i, k), inv);
-            __syncthreads();
-            const int cols = k0 + w - k - 1, rows = n - k - 1;
-            for (int idx = tid; idx < cols * rows; idx += kThreads) {
-                int i = k + 1 + idx % rows, j = k + 1 + idx / rows;
-                at(i, j) = csub(at(i, j), cmul(at(i, k), at(k, j)));
+    // Lane t owns physical row t and tracks its logical position lg.
+    int lg = tid;
+    for (int k = 0; k < ne; ++k) {
+        const int lim = k < ne1 ? ne1 : ne;
+        double big = -1.0;
+        int at = kLdsMaxN, ph = 0;  // pivot: value, logical row, physical row
+        if (tid < n && lg >= k && lg < lim) {
+            big = cabs(W[tid + k * n]);
+            at = lg;
+            ph = tid;
+        }
+        wave_argmax(big, at, ph);
+        if constexpr (NT > 64) {  // combine the waves (double-buffered by step parity)
+            const int par = k & 1;
+            if (lane == 0) {
+                rv[par][tid >> 6] = big;
+                ri[par][tid >> 6] = at;
+                rp[par][tid >> 6] = ph;
             }
             __syncthreads();
-        }
-        const int j1 = k0 + w;  // first trailing row and column
-        if (j1 >= n) break;
-        // 2. U12 = L11^-1 A12 on the panel's rows, one column per lane.
-        for (int idx = tid; idx < kPanel * kPanel; idx += kThreads) {
-            int r = idx % kPanel, c = idx / kPanel;
-            l11[idx] = (r < w && c < w) ? at(k0 + r, k0 + c) : cx<T>(0.0, 0.0);
-        }
-        __syncthreads();
-        for (int j = j1 + tid; j < n; j += kThreads) {
-            T u[kPanel];
-#pragma unroll
-            for (int r = 0; r < kPanel; ++r) u[r] = r < w ? at(k0 + r, j) : cx<T>(0.0, 0.0);
-#pragma unroll
-            for (int r = 1; r < kPanel; ++r)
-#pragma unroll
-                for (int c = 0; c < r; ++c) u[r] = csub(u[r], cmul(l11[r + c * kPanel], u[c]));
-#pragma unroll
-            for (int r = 0; r < kPanel; ++r)
-                if (r < w) at(k0 + r, j) = u[r];
-        }
-        __syncthreads();
-        // 3. Trailing update on MFMA: each wave owns columns of 16x16 tiles.
-        const int rem = n - j1, tiles = (rem + 15) / 16;
-        for (int tj = wave; tj < tiles; tj += kThreads / 64) {
-            const int j0 = j1 + 16 * tj;
-            // A' = U12^T: lane holds U12[k0 + 4 s + lane / 16][j0 + lane % 16] for the 4 slices.
-            R ar[4], ai[4];
-#pragma unroll
-            for (int s = 0; s < 4; ++s) {
-                int r = 4 * s + lane / 16, j = j0 + lane % 16;
-                T v = (r < w && j < n) ? at(k0 + r, j) : cx<T>(0.0, 0.0);
-                ar[s] = v.x;
-                ai[s] = v.y;
-            }
-            for (int ti = 0; ti < tiles; ++ti) {
-                const int i0 = j1 + 16 * ti;
-                typename M::acc cr = {0, 0, 0, 0}, ci = {0, 0, 0, 0};
-#pragma unroll
-                for (int s = 0; s < 4; ++s) {
-                    // B' = L21^T: lane holds L21[i0 + lane % 16][k0 + 4 s + lane / 16].
-                    int c = 4 * s + lane / 16, i = i0 + lane % 16;
-                    T b = (c < w && i < n) ? at(i, k0 + c) : cx<T>(0.0, 0.0);
-                    cr = M::mac(ar[s], b.x, cr);
-                    cr = M::mac(-ai[s], b.y, cr);
-                    ci = M::mac(ar[s], b.y, ci);
-                    ci = M::mac(ai[s], b.x, ci);
+            for (int w = 0; w < NT / 64; ++w) {
+                const double ov = rv[par][w];
+                const int oi = ri[par][w];
+                if (ov > big || (ov == big && oi < at)) {
+                    big = ov;
+                    at = oi;
+                    ph = rp[par][w];
                 }
-                // D'[row(lane, jj)][lane % 16] = C[i0 + lane % 16][j0 + row(lane, jj)].
-                const int i = i0 + lane % 16;
-                if (i < n)
-#pragma unroll
-                    for (int jj = 0; jj < 4; ++jj) {
-                        int j = j0 + M::row(lane, jj);
-                        if (j < n) {
-                            T& d = at(i, j);
-                            d.x -= cr[jj];
-                            d.y -= ci[jj];
-                        }
-                    }
             }
+        }
+        const bool ok = k < ne1 ? (big > tol * scale) : (big > 0.0 && big < INFINITY);
+        if (!ok) {
+            if (tid == 0) *status = k < ne1 ? kFrontDelayed : kFrontSingular;
+            return;
+        }
+        if (tid == 0) piv[k] = at;
+        if (tid == ph)
+            lg = k;
+        else if (lg == k)
+            lg = at;
+        if (tid < n && lg > k) {
+            const T l = cmul(W[tid + k * n], crecip(W[ph + k * n]));
+            W[tid + k * n] = l;
+            for (int j = k + 1; j < n; ++j) W[tid + j * n] = csub(W[tid + j * n], cmul(l, W[ph + j * n]));
         }
         __syncthreads();
     }
     if (tid == 0) *status = kFrontOk;
+    if (tid < n) perm[lg] = tid;
+    __syncthreads();
+    if (pv.keep) {
+        for (int idx = tid; idx < n2; idx += NT) {
+            const int i = idx % n, j = idx / n;
+            F[idx] = W[perm[i] + j * n];
+        }
+    } else {
+        const int nb = n - ne, nb2 = nb * nb;
+        for (int idx = tid; idx < nb2; idx += NT) {
+            const int i = ne + idx % nb, j = ne + idx / nb;
+            F[i + j * n] = W[perm[i] + j * n];
+        }
+    }
 }
 
-template <class T, bool kRoot>
-__global__ void __launch_bounds__(kThreads)
-    factor_blocked_kernel(const NodeDesc* __restrict__ nodes, const int32_t* __restrict__ level, PointsView pv, double tol) {
-    const NodeDesc nd = nodes[level[blockIdx.x]];
-    const int64_t p = blockIdx.y;
-    T* F = (kRoot ? reinterpret_cast<T*>(pv.root_arena) + p * pv.root_stride
-                  : reinterpret_cast<T*>(pv.arena) + p * pv.arena_stride) +
-           nd.off;
-    eliminate_blocked(F, nd, pv.piv + p * pv.piv_stride + nd.piv_off, pv.status + p * pv.status_stride + nd.slot, tol,
-                      kRoot);
-}
+// ---------------------------------------------------------------------------------------------
+// Large fronts, step 1: the border. Partial LU of F11 (row pivoting within the separator rows,
+// the pivot test of field.rs:61), L21 = F21 U11^-1 and U12 = L11^-1 P F12, in place, panel by
+// panel (kPanel columns), without touching F22: the Schur update S = F22 - L21 U12 is left to
+// schur_kernel, which spreads it over many workgroups. Per panel:
+//   1. the panel's separator rows [k0, ne) are factored in LDS (when they fit, else in place);
+//   2. its interchanges are applied to the other columns, the L21 rows are solved against the
+//      panel's U11 block (one lane per row), and the panel goes back to the front;
+//   3. the panel's U rows are solved with its unit-lower L11 block (one lane per column);
+//   4. the rest of the border (rows or columns below ne) gets the rank-kPanel update on MFMA.
+constexpr int kBorderThreads = 256;
 
-// One 16x16 tile of C -= A B on MFMA, computed as C^T -= B^T A^T so that the front's columns are
-// read and written contiguously: A = F[i0:i0+16, k0:k1] (rows of L), B = F[k0:k1, j0:j0+16]
-// (rows of U), K = k1 - k0 in slices of 4. Every lane runs every MFMA; entries outside [0, n)
-// and those `skip` selects are not stored.
-template <class T, class Skip>
-__device__ __forceinline__ void tile_update(T* F, int n, int i0, int j0, int k0, int k1, int lane, Skip skip) {
+template <class T>
+__device__ __forceinline__ void mfma_tile_update(T* F, int n, int ne, int i0, int j0, int k0, int w, int lane) {
+    // F[i, j] -= sum_c L[i, k0 + c] U[k0 + c, j] over c < w for the 16 x 16 tile at (i0, j0), as
+    // C^T -= U^T L^T so that lanes read and write the front's columns contiguously. Entries in
+    // F22 (i, j >= ne) are not stored.
     using M = Mfma<T>;
-    auto at = [&](int i, int j) -> T& { return F[i + int64_t(j) * n]; };
     typename M::acc cr = {0, 0, 0, 0}, ci = {0, 0, 0, 0};
     const int li = lane % 16, lk = lane / 16;
-    for (int kk = k0; kk < k1; kk += 4) {
-        const int k = kk + lk;
+#pragma unroll
+    for (int kk = 0; kk < kPanel; kk += 4) {
+        const int c = kk + lk;
         const int j = j0 + li, i = i0 + li;
-        T a = (k < k1 && j < n) ? at(k, j) : cx<T>(0.0, 0.0);
-        T b = (k < k1 && i < n) ? at(i, k) : cx<T>(0.0, 0.0);
+        // Clamped loads and a select of the value (a select of addresses would go through flat).
+        T a = F[(k0 + min(c, w - 1)) + int64_t(min(j, n - 1)) * n];
+        T b = F[min(i, n - 1) + int64_t(k0 + min(c, w - 1)) * n];
+        if (!(c < w && j < n)) a = cx<T>(0.0, 0.0);
+        if (!(c < w && i < n)) b = cx<T>(0.0, 0.0);
         cr = M::mac(a.x, b.x, cr);
         cr = M::mac(-a.y, b.y, cr);
         ci = M::mac(a.x, b.y, ci);
         ci = M::mac(a.y, b.x, ci);
     }
     const int i = i0 + li;
+#pragma unroll
     for (int jj = 0; jj < 4; ++jj) {
         const int j = j0 + M::row(lane, jj);
-        const bool store = i < n && j < n && !skip(i, j);
-        if (store) {
-            T& d = at(i, j);
+        if (i < n && j < n && (i < ne || j < ne)) {
+            T& d = F[i + int64_t(j) * n];
             d.x -= cr[jj];
             d.y -= ci[jj];
         }
     }
 }
-// Blocked partial LU with a deferred Schur update: pivoting inside panels of kPanel columns,
-// the triangular solve of each panel's row block, and MFMA updates of the separator rows and
-// columns only; S = F22 - L21 U12 is formed once at the end with K = ne, so the Schur block is
-// read and written once instead of once per panel.
-template <class T>
-__device__ void eliminate_deferred(T* F, const NodeDesc& nd, int32_t* piv, int32_t* status, double tol, bool root) {
-    __shared__ double sv[16];
-    __shared__ int si[16];
-    __shared__ T l11[kPanel * kPanel];
-    const int n = nd.n, ne = nd.ne, ne1 = nd.ne1, tid = threadIdx.x, nt = blockDim.x;
-    const int lane = tid & 63, wave = tid >> 6, waves = nt >> 6;
+
+// The border factorisation of one front; kLds: the panel's separator rows are held in LDS (P).
+template <class T, bool kRoot, bool kLds>
+__device__ __forceinline__ void border_factor(T* F, T* P, int32_t* spiv, int n, int ne, int ne1, int32_t* piv,
+                                              int32_t* status, double scale, double tol) {
+    constexpr int NT = kBorderThreads;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     auto at = [&](int i, int j) -> T& { return F[i + int64_t(j) * n]; };
-    double m = 0.0;
-    for (int idx = tid; idx < ne1 * ne1; idx += nt) m = fmax(m, cabs(at(idx % ne1, idx / ne1)));
-    const double scale = block_max(m, sv);
     for (int k0 = 0; k0 < ne; k0 += kPanel) {
-        const int w = min(kPanel, ne - k0);
-        for (int k = k0; k < k0 + w; ++k) {
-            const int lim = k < ne1 ? ne1 : ne;
+        const int w = min(kPanel, ne - k0), R = ne - k0, k1 = k0 + w;
+        // Panel element (row i in [k0, ne), panel column c).
+        auto pp = [&](int i, int c) -> T& { return kLds ? P[(i - k0) + c * R] : at(i, k0 + c); };
+        if (kLds)
+            for (int e = tid; e < R * w; e += NT) P[e] = at(k0 + e % R, k0 + e / R);
+        __syncthreads();
+        // 1. Panel factorisation on the separator rows.
+        for (int c = 0; c < w; ++c) {
+            const int k = k0 + c, lim = k < ne1 ? ne1 : ne;
             double big = -1.0;
-            int p = k;
-            for (int i = k + tid; i < lim; i += nt) {
-                double v = cabs(at(i, k));
-                if (v > big) { big = v; p = i; }
+            int q = k;
+            for (int i = k + lane; i < lim; i += 64) {
+                const double v = cabs(pp(i, c));
+                if (v > big) {
+                    big = v;
+                    q = i;
+                }
             }
-            block_argmax(big, p, sv, si);
+            wave_argmax(big, q);  // every wave finds the same pivot
             const bool ok = k < ne1 ? (big > tol * scale) : (big > 0.0 && big < INFINITY);
-            if (!ok) { if (tid == 0) *status = (k < ne1 && !root) ? kFrontDelayed : kFrontSingular; return; }
-            if (p != k)
-                for (int j = tid; j < n; j += nt) { T x = at(k, j); at(k, j) = at(p, j); at(p, j) = x; }
-            if (tid == 0) piv[k] = p;
+            if (!ok) {
+                if (tid == 0) *status = (k < ne1 && !kRoot) ? kFrontDelayed : kFrontSingular;
+                return;
+            }
+            if (tid == 0) {
+                piv[k] = q;
+                spiv[c] = q;
+            }
+            if (q != k && tid < w) {
+                T x = pp(k, tid);
+                pp(k, tid) = pp(q, tid);
+                pp(q, tid) = x;
+            }
             __syncthreads();
-            const T inv = crecip(at(k, k));
-            for (int i = k + 1 + tid; i < n; i += nt) at(i, k) = cmul(at(i, k), inv);
-            __syncthreads();
-            const int cols = k0 + w - k - 1, rows = n - k - 1;
-            for (int idx = tid; idx < cols * rows; idx += nt) {
-                int i = k + 1 + idx % rows, j = k + 1 + idx / rows;
-                at(i, j) = csub(at(i, j), cmul(at(i, k), at(k, j)));
+            const T inv = crecip(pp(k, c));
+            for (int i = k + 1 + tid; i < ne; i += NT) {
+                const T l = cmul(pp(i, c), inv);
+                pp(i, c) = l;
+                for (int c2 = c + 1; c2 < w; ++c2) pp(i, c2) = csub(pp(i, c2), cmul(l, pp(k, c2)));
             }
             __syncthreads();
         }
-        const int j1 = k0 + w;
-        if (j1 >= n) break;
-        for (int idx = tid; idx < kPanel * kPanel; idx += nt) {
-            int r = idx % kPanel, c = idx / kPanel;
-            l11[idx] = (r < w && c < w) ? at(k0 + r, k0 + c) : cx<T>(0.0, 0.0);
+        // 2. Interchanges on the other columns (separator rows only), the panel back in place, and
+        //    the L21 rows: l U11 = f with U11 the panel's upper triangle.
+        for (int j = tid; j < n; j += NT) {
+            if (j >= k0 && j < k1) continue;
+            for (int c = 0; c < w; ++c) {
+                const int q = spiv[c], k = k0 + c;
+                if (q != k) {
+                    T x = at(k, j);
+                    at(k, j) = at(q, j);
+                    at(q, j) = x;
+                }
+            }
+        }
+        if (kLds)
+            for (int e = tid; e < R * w; e += NT) at(k0 + e % R, k0 + e / R) = P[e];
+        for (int i = ne + tid; i < n; i += NT) {
+            T l[kPanel];
+#pragma unroll
+            for (int c = 0; c < kPanel; ++c) {
+                l[c] = at(i, k0 + min(c, w - 1));
+                if (c >= w) l[c] = cx<T>(0.0, 0.0);
+            }
+#pragma unroll
+            for (int c = 0; c < kPanel; ++c) {
+                if (c < w) {
+                    T acc = l[c];
+#pragma unroll
+                    for (int c2 = 0; c2 < c; ++c2) acc = csub(acc, cmul(l[c2], pp(k0 + c2, c)));
+                    l[c] = cmul(acc, crecip(pp(k0 + c, c)));
+                }
+            }
+#pragma unroll
+            for (int c = 0; c < kPanel; ++c)
+                if (c < w) at(i, k0 + c) = l[c];
         }
         __syncthreads();
-        for (int j = j1 + tid; j < n; j += nt)
-            for (int r = 1; r < w; ++r) {
-                T acc = at(k0 + r, j);
-                for (int c = 0; c < r; ++c) acc = csub(acc, cmul(l11[r + c * kPanel], at(k0 + c, j)));
-                at(k0 + r, j) = acc;
+        if (k1 >= n) break;
+        // 3. U rows of the panel: unit-lower solve, one lane per column.
+        for (int j = k1 + tid; j < n; j += NT) {
+            T u[kPanel];
+#pragma unroll
+            for (int c = 0; c < kPanel; ++c) {
+                u[c] = at(k0 + min(c, w - 1), j);
+                if (c >= w) u[c] = cx<T>(0.0, 0.0);
             }
+#pragma unroll
+            for (int c = 1; c < kPanel; ++c) {
+                if (c < w) {
+                    T acc = u[c];
+#pragma unroll
+                    for (int c2 = 0; c2 < c; ++c2) acc = csub(acc, cmul(pp(k0 + c, c2), u[c2]));
+                    u[c] = acc;
+                }
+            }
+#pragma unroll
+            for (int c = 1; c < kPanel; ++c)
+                if (c < w) at(k0 + c, j) = u[c];
+        }
         __syncthreads();
-        if (j1 < ne) {
-            const int tiles_c = (n - j1 + 15) / 16;
-            for (int tj = wave; tj < tiles_c; tj += waves) {
-                const int j0 = j1 + 16 * tj;
-                const int rows_end = j0 < ne ? n : ne;
-                for (int i0 = j1; i0 < rows_end; i0 += 16)
-                    tile_update(F, n, i0, j0, k0, j1, lane, [&](int i, int j) { return i >= ne && j >= ne; });
+        // 4. Rank-w update of the border below and right of the panel, 16 x 16 tiles on MFMA; the
+        //    tiles inside F22 are skipped.
+        if (k1 < ne) {
+            const int tiles = (n - k1 + 15) / 16;
+            for (int t = wave; t < tiles * tiles; t += NT / 64) {
+                const int i0 = k1 + 16 * (t % tiles), j0 = k1 + 16 * (t / tiles);
+                if (i0 >= ne && j0 >= ne) continue;
+                mfma_tile_update(F, n, ne, i0, j0, k0, w, lane);
             }
-            __syncthreads();
         }
+        __syncthreads();
     }
-    if (ne < n) {
-        const int tiles = (n - ne + 15) / 16;
-        for (int t = wave; t < tiles * tiles; t += waves) {
-            const int tj = t / tiles, ti = t % tiles;
-            tile_update(F, n, ne + 16 * ti, ne + 16 * tj, 0, ne, lane, [](int, int) { return false; });
-        }
-    }
-    __syncthreads();
     if (tid == 0) *status = kFrontOk;
 }
 
 template <class T, bool kRoot>
-__global__ void __launch_bounds__(kThreads)
-    factor_deferred_kernel(const NodeDesc* __restrict__ nodes, const int32_t* __restrict__ level, PointsView pv, double tol) {
+__global__ void __launch_bounds__(kBorderThreads)
+    border_kernel(const NodeDesc* __restrict__ nodes, const int32_t* __restrict__ level, PointsView pv, double tol,
+                  int lds_rows) {
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    __shared__ double sv[kBorderThreads / 64];
+    __shared__ int32_t spiv[kPanel];
     const NodeDesc nd = nodes[level[blockIdx.x]];
     const int64_t p = blockIdx.y;
     T* F = (kRoot ? reinterpret_cast<T*>(pv.root_arena) + p * pv.root_stride
                   : reinterpret_cast<T*>(pv.arena) + p * pv.arena_stride) +
            nd.off;
-    eliminate_deferred(F, nd, pv.piv + p * pv.piv_stride + nd.piv_off, pv.status + p * pv.status_stride + nd.slot, tol,
-                       kRoot);
+    int32_t* piv = pv.piv + p * pv.piv_stride + nd.piv_off;
+    int32_t* status = pv.status + p * pv.status_stride + nd.slot;
+    const int n = nd.n, ne = nd.ne, ne1 = nd.ne1;
+    double m = 0.0;
+    for (int idx = threadIdx.x; idx < ne1 * ne1; idx += kBorderThreads)
+        m = fmax(m, cabs(F[(idx % ne1) + int64_t(idx / ne1) * n]));
+    const double scale = block_max(m, sv);
+    if (ne <= lds_rows)
+        border_factor<T, kRoot, true>(F, reinterpret_cast<T*>(lds_raw), spiv, n, ne, ne1, piv, status, scale, tol);
+    else
+        border_factor<T, kRoot, false>(F, nullptr, spiv, n, ne, ne1, piv, status, scale, tol);
 }
 
-template <class T, int NMAX>
-__global__ void __launch_bounds__(kThreads)
-    factor_small_kernel(const NodeDesc* __restrict__ nodes, const int32_t* __restrict__ level, PointsView pv, double tol) {
-    __shared__ T w[NMAX * NMAX];
-    const NodeDesc nd = nodes[level[blockIdx.x]];
+// ---------------------------------------------------------------------------------------------
+// Large fronts, step 2: S = F22 - L21 U12 (K = ne) over 64 x 64 tiles of the Schur block, one
+// workgroup per (tile, point), on MFMA. Each wave owns a 32 x 32 quarter (2 x 2 blocks of 16 x 16,
+// complex = 4 real MFMAs per block and K-step of 4); L21 and U12 are staged through LDS in
+// chunks of kChunk columns / rows. Fronts whose border failed are skipped.
+constexpr int kChunk = 16, kTileLd = kTile + 1;
+
+template <class T, bool kRoot>
+__global__ void __launch_bounds__(256)
+    schur_kernel(const NodeDesc* __restrict__ nodes, const Tile* __restrict__ tiles, PointsView pv) {
+    using M = Mfma<T>;
+    __shared__ T As[kChunk * kTileLd];  // As[k * kTileLd + i] = L21[i0 + i, kc + k]
+    __shared__ T Bs[kChunk * kTileLd];  // Bs[k * kTileLd + j] = U12[kc + k, j0 + j]
+    const Tile tl = tiles[blockIdx.x];
+    const NodeDesc nd = nodes[tl.node];
     const int64_t p = blockIdx.y;
-    T* F = reinterpret_cast<T*>(pv.arena) + p * pv.arena_stride + nd.off;
-    const int n2 = nd.n * nd.n;
-    for (int i = threadIdx.x; i < n2; i += kThreads) w[i] = F[i];
-    __syncthreads();
-    eliminate(w, nd, pv.piv + p * pv.piv_stride + nd.piv_off, pv.status + p * pv.status_stride + nd.slot, tol, false);
-    __syncthreads();
-    for (int i = threadIdx.x; i < n2; i += kThreads) F[i] = w[i];
+    if (pv.status[p * pv.status_stride + nd.slot] != kFrontOk) return;
+    T* F = (kRoot ? reinterpret_cast<T*>(pv.root_arena) + p * pv.root_stride
+                  : reinterpret_cast<T*>(pv.arena) + p * pv.arena_stride) +
+           nd.off;
+    const int n = nd.n, ne = nd.ne, nb = n - ne, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wi = (wave & 1) * 32, wj = (wave >> 1) * 32;
+    const T zero = cx<T>(0.0, 0.0);
+    typename M::acc cr[2][2], ci[2][2];
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y) {
+            cr[x][y] = typename M::acc{0, 0, 0, 0};
+            ci[x][y] = typename M::acc{0, 0, 0, 0};
+        }
+    for (int kc = 0; kc < ne; kc += kChunk) {
+        // Clamped loads and a select of the value (a select of addresses would go through flat).
+        for (int e = tid; e < kTile * kChunk; e += 256) {
+            const int r = e % kTile, c = e / kTile, i = tl.i0 + r, k = kc + c;
+            T v = F[(ne + min(i, nb - 1)) + int64_t(min(k, ne - 1)) * n];
+            if (!(i < nb && k < ne)) v = zero;
+            As[c * kTileLd + r] = v;
+        }
+        for (int e = tid; e < kTile * kChunk; e += 256) {
+            const int c = e % kChunk, r = e / kChunk, j = tl.j0 + r, k = kc + c;
+            T v = F[min(k, ne - 1) + int64_t(ne + min(j, nb - 1)) * n];
+            if (!(j < nb && k < ne)) v = zero;
+            Bs[c * kTileLd + r] = v;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < kChunk; kk += 4) {
+            const int k = kk + lane / 16;
+            T a[2], b[2];
+#pragma unroll
+            for (int s = 0; s < 2; ++s) {
+                a[s] = Bs[k * kTileLd + wj + 16 * s + lane % 16];  // U12^T operand
+                b[s] = As[k * kTileLd + wi + 16 * s + lane % 16];  // L21^T operand
+            }
+#pragma unroll
+            for (int x = 0; x < 2; ++x)
+#pragma unroll
+                for (int y = 0; y < 2; ++y) {
+                    cr[x][y] = M::mac(a[x].x, b[y].x, cr[x][y]);
+                    cr[x][y] = M::mac(-a[x].y, b[y].y, cr[x][y]);
+                    ci[x][y] = M::mac(a[x].x, b[y].y, ci[x][y]);
+                    ci[x][y] = M::mac(a[x].y, b[y].x, ci[x][y]);
+                }
+        }
+        __syncthreads();
+    }
+    // Block (x, y) of the wave: lane l, element jj holds (L21 U12)[i, j] with
+    // i = wi + 16 y + l % 16 and j = wj + 16 x + row(l, jj).
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y) {
+            const int i = tl.i0 + wi + 16 * y + lane % 16;
+#pragma unroll
+            for (int jj = 0; jj < 4; ++jj) {
+                const int j = tl.j0 + wj + 16 * x + M::row(lane, jj);
+                if (i < nb && j < nb) {
+                    T& d = F[(ne + i) + int64_t(ne + j) * n];
+                    d.x -= cr[x][y][jj];
+                    d.y -= ci[x][y][jj];
+                }
+            }
+        }
 }
 
 template <class T, bool kRoot>
@@ -522,6 +685,15 @@ __global__ void __launch_bounds__(kThreads) trace_kernel(const NodeDesc* __restr
 
 inline dim3 grid2(int32_t count, int32_t points) { return dim3(unsigned(count), unsigned(points)); }
 
+// Dynamic LDS above the 64 KB default (gfx950 has 160 KB per workgroup), once per kernel.
+void allow_lds(const void* kernel) {
+    static std::mutex mu;
+    static std::set<const void*> done;
+    std::lock_guard<std::mutex> g(mu);
+    if (done.insert(kernel).second)
+        (void)hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, 96 * 1024);
+}
+
 }  // namespace
 
 void launch_assemble_c128(const NodeDesc* nodes, const int32_t* level, int32_t count, const Entry* ent,
@@ -552,54 +724,81 @@ void launch_assemble_root(const NodeDesc* nodes, const int32_t* level, int32_t c
                            level, ent, diag, kids, maps, pv);
 }
 
-bool blocked_fronts() {
+bool unblocked_fronts() {
     const char* v = std::getenv("CAROLI_FRONT_KERNEL");
-    return !(v && std::strcmp(v, "unblocked") == 0);
+    return v && std::strcmp(v, "unblocked") == 0;
 }
 
-bool deferred_fronts() {
-    const char* v = std::getenv("CAROLI_FRONT_KERNEL");
-    return v && std::strcmp(v, "deferred") == 0;
-}
-
-void launch_factor_c128(const NodeDesc* nodes, const int32_t* level, int32_t count, const PointsView& pv,
-                        int32_t points, double tol, bool small, bool root, hipStream_t st) {
+void launch_front_lds(const NodeDesc* nodes, const int32_t* level, int32_t count, int32_t nmax, const Entry* ent,
+                      const int32_t* diag, const KidDesc* kids, const int32_t* maps, const PointsView& pv,
+                      int32_t points, double tol, bool single, bool assemble, hipStream_t st) {
     if (count <= 0 || points <= 0) return;
-    const bool blk = blocked_fronts();
-    if (deferred_fronts() && !small) {
-        if (root)
-            hipLaunchKernelGGL((factor_deferred_kernel<double2, true>), grid2(count, points), dim3(kThreads), 0, st,
-                               nodes, level, pv, tol);
+    if (nmax > kLdsMaxN) throw std::invalid_argument("front too large for the LDS kernel");
+    const size_t lds = size_t(nmax) * size_t(nmax) * (single ? sizeof(float2) : sizeof(double2));
+    const bool wide = nmax > 64;
+    auto go = [&](auto kernel, int nt) {
+        allow_lds(reinterpret_cast<const void*>(kernel));
+        hipLaunchKernelGGL(kernel, grid2(count, points), dim3(nt), lds, st, nodes, level, ent, diag, kids, maps, pv, tol);
+    };
+    if (single) {
+        if (assemble)
+            wide ? go(front_lds_kernel<float2, 128, true>, 128) : go(front_lds_kernel<float2, 64, true>, 64);
         else
-            hipLaunchKernelGGL((factor_deferred_kernel<double2, false>), grid2(count, points), dim3(kThreads), 0, st,
-                               nodes, level, pv, tol);
-        return;
+            wide ? go(front_lds_kernel<float2, 128, false>, 128) : go(front_lds_kernel<float2, 64, false>, 64);
+    } else {
+        if (assemble)
+            wide ? go(front_lds_kernel<double2, 128, true>, 128) : go(front_lds_kernel<double2, 64, true>, 64);
+        else
+            wide ? go(front_lds_kernel<double2, 128, false>, 128) : go(front_lds_kernel<double2, 64, false>, 64);
     }
-    if (root && blk)
-        hipLaunchKernelGGL((factor_blocked_kernel<double2, true>), grid2(count, points), dim3(kThreads), 0, st, nodes,
+}
+
+int32_t border_lds_rows(int32_t max_ne, bool single) {
+    const int32_t cap = kBorderLdsBytes / (kPanel * int32_t(single ? sizeof(float2) : sizeof(double2)));
+    return max_ne <= cap ? max_ne : 0;
+}
+
+void launch_border(const NodeDesc* nodes, const int32_t* level, int32_t count, int32_t max_ne, const PointsView& pv,
+                   int32_t points, double tol, bool single, bool root, hipStream_t st) {
+    if (count <= 0 || points <= 0) return;
+    const int32_t rows = border_lds_rows(max_ne, single);
+    const size_t lds = size_t(rows) * kPanel * (single ? sizeof(float2) : sizeof(double2));
+    const dim3 g = grid2(count, points), b(kBorderThreads);
+    auto go = [&](auto kernel) {
+        allow_lds(reinterpret_cast<const void*>(kernel));
+        hipLaunchKernelGGL(kernel, g, b, lds, st, nodes, level, pv, tol, rows);
+    };
+    if (single)
+        go(border_kernel<float2, false>);
+    else if (root)
+        go(border_kernel<double2, true>);
+    else
+        go(border_kernel<double2, false>);
+}
+
+void launch_schur(const NodeDesc* nodes, const Tile* tiles, int32_t count, const PointsView& pv, int32_t points,
+                  bool single, bool root, hipStream_t st) {
+    if (count <= 0 || points <= 0) return;
+    const dim3 g = grid2(count, points), b(256);
+    if (single)
+        hipLaunchKernelGGL((schur_kernel<float2, false>), g, b, 0, st, nodes, tiles, pv);
+    else if (root)
+        hipLaunchKernelGGL((schur_kernel<double2, true>), g, b, 0, st, nodes, tiles, pv);
+    else
+        hipLaunchKernelGGL((schur_kernel<double2, false>), g, b, 0, st, nodes, tiles, pv);
+}
+
+void launch_factor_unblocked(const NodeDesc* nodes, const int32_t* level, int32_t count, const PointsView& pv,
+                             int32_t points, double tol, bool single, bool root, hipStream_t st) {
+    if (count <= 0 || points <= 0) return;
+    if (single)
+        hipLaunchKernelGGL((factor_global_kernel<float2, false>), grid2(count, points), dim3(kThreads), 0, st, nodes,
                            level, pv, tol);
     else if (root)
         hipLaunchKernelGGL((factor_global_kernel<double2, true>), grid2(count, points), dim3(kThreads), 0, st, nodes,
                            level, pv, tol);
-    else if (small)
-        hipLaunchKernelGGL((factor_small_kernel<double2, kLdsFrontC128>), grid2(count, points), dim3(kThreads), 0, st,
-                           nodes, level, pv, tol);
-    else if (blk)
-        hipLaunchKernelGGL((factor_blocked_kernel<double2, false>), grid2(count, points), dim3(kThreads), 0, st, nodes,
-                           level, pv, tol);
     else
         hipLaunchKernelGGL((factor_global_kernel<double2, false>), grid2(count, points), dim3(kThreads), 0, st, nodes,
-                           level, pv, tol);
-}
-
-void launch_factor_c64(const NodeDesc* nodes, const int32_t* level, int32_t count, const PointsView& pv,
-                       int32_t points, double tol, bool small, hipStream_t st) {
-    if (count <= 0 || points <= 0) return;
-    if (small)
-        hipLaunchKernelGGL((factor_small_kernel<float2, kLdsFrontC64>), grid2(count, points), dim3(kThreads), 0, st,
-                           nodes, level, pv, tol);
-    else
-        hipLaunchKernelGGL((factor_global_kernel<float2, false>), grid2(count, points), dim3(kThreads), 0, st, nodes,
                            level, pv, tol);
 }
 

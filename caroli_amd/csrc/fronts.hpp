@@ -61,14 +61,26 @@ struct PointsView {
     int32_t* status;  // per point: one slot per front
     int32_t status_stride;
     double* t_out;    // per point
+    int32_t keep;     // 1: small fronts write their factors back (for the solves), 0: only S
+};
+
+/// A kTile x kTile tile of the Schur block of front `node` (rows i0.., columns j0.. of S).
+struct Tile {
+    int32_t node, i0, j0, pad;
 };
 
 /// Status codes written by the kernels.
 enum : int32_t { kFrontOk = 0, kFrontDelayed = 5, kFrontSingular = 1 };
 
-/// Largest front eliminated in LDS by the small-front kernels.
+/// Largest front eliminated in LDS, per precision (complex f64: 64 KB of LDS, f32: 63 KB), and
+/// the largest the LDS kernel accepts at all (its row permutation).
 constexpr int kLdsFrontC128 = 64;
 constexpr int kLdsFrontC64 = 90;
+constexpr int kLdsMaxN = 128;
+/// Panel width of the border factorisation, LDS budget of its panel, Schur tile size.
+constexpr int kPanel = 16;
+constexpr int kBorderLdsBytes = 64 * 1024;
+constexpr int kTile = 64;
 
 // Launchers (fronts.hip). `level` lists node indices into `nodes`; grid = count x points.
 void launch_assemble_c128(const NodeDesc* nodes, const int32_t* level, int32_t count, const Entry* ent,
@@ -81,11 +93,20 @@ void launch_assemble_c64(const NodeDesc* nodes, const int32_t* level, int32_t co
 void launch_assemble_root(const NodeDesc* nodes, const int32_t* level, int32_t count, const Entry* ent,
                           const int32_t* diag, const KidDesc* kids, const int32_t* maps, const PointsView& pv,
                           int32_t points, int single, hipStream_t st);
-/// Elimination of the fronts listed (small: in LDS, n <= kLdsFront*; large: in global memory).
-void launch_factor_c128(const NodeDesc* nodes, const int32_t* level, int32_t count, const PointsView& pv,
-                        int32_t points, double tol, bool small, bool root, hipStream_t st);
-void launch_factor_c64(const NodeDesc* nodes, const int32_t* level, int32_t count, const PointsView& pv,
-                       int32_t points, double tol, bool small, hipStream_t st);
+/// Small fronts (n <= nmax <= kLdsMaxN): assembly (or a copy of the given front when !assemble)
+/// and elimination in LDS.
+void launch_front_lds(const NodeDesc* nodes, const int32_t* level, int32_t count, int32_t nmax, const Entry* ent,
+                      const int32_t* diag, const KidDesc* kids, const int32_t* maps, const PointsView& pv,
+                      int32_t points, double tol, bool single, bool assemble, hipStream_t st);
+/// Large fronts, assembled: the border LU (F11, L21, U12), then the Schur update over `tiles`.
+void launch_border(const NodeDesc* nodes, const int32_t* level, int32_t count, int32_t max_ne, const PointsView& pv,
+                   int32_t points, double tol, bool single, bool root, hipStream_t st);
+void launch_schur(const NodeDesc* nodes, const Tile* tiles, int32_t count, const PointsView& pv, int32_t points,
+                  bool single, bool root, hipStream_t st);
+/// The whole elimination by rank-1 updates in global memory (CAROLI_FRONT_KERNEL=unblocked).
+void launch_factor_unblocked(const NodeDesc* nodes, const int32_t* level, int32_t count, const PointsView& pv,
+                             int32_t points, double tol, bool single, bool root, hipStream_t st);
+bool unblocked_fronts();
 /// Caroli trace of the root border block: t_out[p] = Tr[Gamma_L G Gamma_R G^H].
 void launch_trace(const NodeDesc* root, const PointsView& pv, int32_t points, hipStream_t st);
 
