@@ -81,11 +81,11 @@ struct LevelSet {
     int32_t lds_n[kClasses] = {0, 0, 0};  // largest n of each class
     std::vector<int32_t> large;
     int32_t large_ne = 0;  // largest ne of the large fronts
-    std::vector<Tile> tiles;
+    std::vector<Tile> tiles, gtiles;  // Schur tiles, border tiles (gather)
     // Algorithmic work (complex words, flops) for the kernel timers.
     double lds_flops[kClasses] = {0, 0, 0}, lds_words[kClasses] = {0, 0, 0}, lds_s_words[kClasses] = {0, 0, 0},
            lds_f_words[kClasses] = {0, 0, 0};
-    double asm_words = 0, border_flops = 0, border_words = 0, schur_flops = 0, schur_words = 0;
+    double asm_words = 0, gather_words = 0, border_flops = 0, border_words = 0, schur_flops = 0, schur_words = 0;
 };
 
 /// Device pointers of a LevelSet's lists.
@@ -93,6 +93,7 @@ struct DevLevel {
     const int32_t* lds[LevelSet::kClasses] = {nullptr, nullptr, nullptr};
     const int32_t* large = nullptr;
     const Tile* tiles = nullptr;
+    const Tile* gtiles = nullptr;
 };
 
 /// Splits the fronts `list` of `nodes` by kernel (root: the bordered root front, always large).
@@ -125,19 +126,27 @@ LevelSet classify(const std::vector<NodeDesc>& nodes, const std::vector<KidDesc>
             L.border_words += 2.0 * n2;
             continue;
         }
+        // The children's words split between the border (gathered first) and F22 (by the Schur
+        // tiles) in proportion to the areas.
+        const double border_share = (n2 - nb * nb) / std::max(n2, 1.0);
+        L.gather_words += (n2 - nb * nb) + border_share * kid_words;
         L.border_flops += 8.0 / 3.0 * ne * ne * ne + 8.0 * ne * ne * nb;
         L.border_words += 2.0 * (n2 - nb * nb);
         L.schur_flops += 8.0 * ne * nb * nb;
-        L.schur_words += 2.0 * nb * nb + 2.0 * ne * nb;
+        L.schur_words += nb * nb + 2.0 * ne * nb + (root ? nb * nb : (1.0 - border_share) * kid_words);
         for (int i0 = 0; i0 < d.n - d.ne; i0 += kTile)
             for (int j0 = 0; j0 < d.n - d.ne; j0 += kTile) L.tiles.push_back({q, i0, j0, 0});
+        if (!root)
+            for (int j0 = 0; j0 < d.n; j0 += kTile)
+                for (int i0 = 0; i0 < d.n; i0 += kTile)
+                    if (i0 < d.ne || j0 < d.ne) L.gtiles.push_back({q, i0, j0, 0});
     }
     return L;
 }
 
 /// Appends a LevelSet's lists to the device list / tile arrays; `at` records the offsets.
 struct SetOffsets {
-    int32_t lds[LevelSet::kClasses], large, tiles;
+    int32_t lds[LevelSet::kClasses], large, tiles, gtiles;
 };
 SetOffsets append_set(const LevelSet& L, std::vector<int32_t>& lv, std::vector<Tile>& tl) {
     SetOffsets o{};
@@ -149,6 +158,8 @@ SetOffsets append_set(const LevelSet& L, std::vector<int32_t>& lv, std::vector<T
     lv.insert(lv.end(), L.large.begin(), L.large.end());
     o.tiles = int32_t(tl.size());
     tl.insert(tl.end(), L.tiles.begin(), L.tiles.end());
+    o.gtiles = int32_t(tl.size());
+    tl.insert(tl.end(), L.gtiles.begin(), L.gtiles.end());
     return o;
 }
 DevLevel resolve(const SetOffsets& o, const DevBuf& lv, const DevBuf& tl) {
@@ -156,6 +167,7 @@ DevLevel resolve(const SetOffsets& o, const DevBuf& lv, const DevBuf& tl) {
     for (int c = 0; c < LevelSet::kClasses; ++c) d.lds[c] = lv.as<int32_t>() + o.lds[c];
     d.large = lv.as<int32_t>() + o.large;
     d.tiles = tl.as<Tile>() + o.tiles;
+    d.gtiles = tl.as<Tile>() + o.gtiles;
     return d;
 }
 
@@ -234,6 +246,17 @@ void root_desc(const Block& blk, const SparseMatrix& h, const std::vector<size_t
     d.right_at = blk.right;
 }
 
+/// Appends the inverse maps of kids [beg, end) of a front of n variables to `maps`: the child's
+/// index of every front position, -1 where the child contributes nothing.
+void add_inverse_maps(std::vector<KidDesc>& kids, int32_t beg, int32_t end, std::vector<int32_t>& maps, int n) {
+    for (int32_t c = beg; c < end; ++c) {
+        KidDesc& kd = kids[c];
+        kd.inv_beg = int32_t(maps.size());
+        maps.resize(maps.size() + size_t(n), -1);
+        for (int a = 0; a < kd.size; ++a) maps[size_t(kd.inv_beg) + size_t(maps[size_t(kd.map_beg) + a])] = a;
+    }
+}
+
 HostPlan build_plan(const Block& blk, const SparseMatrix& h) {
     const Tree& t = blk.tree;
     HostPlan hp;
@@ -264,6 +287,7 @@ HostPlan build_plan(const Block& blk, const SparseMatrix& h) {
             hp.kids.push_back(kd);
         }
         d.kid_end = int32_t(hp.kids.size());
+        add_inverse_maps(hp.kids, d.kid_beg, d.kid_end, hp.maps, d.n);
         d.slot = k;
         d.piv_off = piv;
         d.var_off = int64_t(hp.vars.size());
@@ -359,7 +383,7 @@ void upload_plan(const HostPlan& hp, DevPlan& dp, const SparseMatrix& h, hipStre
 
 /// Device copies of the per-point inputs of one launch unit.
 struct Inputs {
-    DevBuf vals_ptrs, z, sigma, gamma;
+    DevBuf vals_ptrs, z, sigma, gamma, w, wcols;
     int64_t stride = 0;
 };
 
@@ -367,6 +391,7 @@ struct Inputs {
 /// CAROLI_FRONT_KERNEL=unblocked).
 enum Family {
     kFrontLds,
+    kGather,
     kAssemble,
     kBorder,
     kSchur,
@@ -376,9 +401,9 @@ enum Family {
     kTrace,
     kFamilies
 };
-const char* const kFamilyNames[kFamilies] = {"front_lds_kernel", "assemble_kernel",        "border_kernel",
-                                             "schur_kernel",     "assemble_kernel<root>", "border_kernel<root>",
-                                             "schur_kernel<root>", "trace_kernel"};
+const char* const kFamilyNames[kFamilies] = {"front_lds_kernel",      "gather_kernel",       "assemble_kernel",
+                                             "border_kernel",         "schur_kernel",        "assemble_kernel<root>",
+                                             "border_kernel<root>",   "schur_kernel<root>",  "trace_kernel"};
 
 struct Engine {
     std::mutex mu;
@@ -488,6 +513,8 @@ void bind_inputs(const Inputs& in, PointsView& pv) {
     pv.z = in.z.as<const double2>();
     pv.sigma = in.sigma.as<const double2>();
     pv.gamma = in.gamma.as<const double2>();
+    pv.w = in.w.as<const double2>();
+    pv.wcols = in.wcols.as<const int32_t>();
     pv.sigma_stride = in.stride;
 }
 
@@ -497,7 +524,8 @@ void stage_points(Engine& eng, const std::vector<const cg_system*>& sys, const s
     const size_t P = sys.size();
     const int64_t stride = int64_t(nl) * nl + int64_t(nr) * nr;
     std::vector<const double2*> vp(P);
-    std::vector<double2> z(P), sg(P * stride), gm(P * stride);
+    std::vector<double2> z(P), sg(P * stride), gm(P * stride), wv(P * stride, double2{0.0, 0.0});
+    std::vector<int32_t> wc(2 * P);
     for (size_t p = 0; p < P; ++p) {
         vp[p] = vals_of(eng, sys[p]);
         z[p] = double2{sig[p]->energy, 0.0};
@@ -510,11 +538,20 @@ void stage_points(Engine& eng, const std::vector<const cg_system*>& sys, const s
         };
         put(sg, blk.sl, blk.sr);
         put(gm, blk.gl, blk.gr);
+        // W_L (nl x cl) at 0 and W_R (nr x cr) at nl^2, column-major.
+        double2* o = wv.data() + p * stride;
+        for (size_t e = 0; e < blk.wl.a.size(); ++e) o[e] = double2{blk.wl.a[e].real(), blk.wl.a[e].imag()};
+        o += int64_t(nl) * nl;
+        for (size_t e = 0; e < blk.wr.a.size(); ++e) o[e] = double2{blk.wr.a[e].real(), blk.wr.a[e].imag()};
+        wc[2 * p] = blk.wl.cols;
+        wc[2 * p + 1] = blk.wr.cols;
     }
     upload(in.vals_ptrs, vp, eng.stream);
     upload(in.z, z, eng.stream);
     upload(in.sigma, sg, eng.stream);
     upload(in.gamma, gm, eng.stream);
+    upload(in.w, wv, eng.stream);
+    upload(in.wcols, wc, eng.stream);
     in.stride = stride;
     bind_inputs(in, pv);
 }
@@ -538,26 +575,32 @@ void run_level(Engine& eng, const LevelSet& L, const DevLevel& D, const NodeDesc
     }
     const int32_t count = int32_t(L.large.size());
     if (count == 0) return;
-    eng.timed(root ? kRootAssemble : kAssemble, 0.0, P * fes * L.asm_words, [&] {
-        if (root)
-            launch_assemble_root(nodes, D.large, count, ent, diag, kids, maps, pv, P, single ? 1 : 0, eng.stream);
-        else if (single)
-            launch_assemble_c64(nodes, D.large, count, ent, diag, kids, maps, pv, P, eng.stream);
-        else
-            launch_assemble_c128(nodes, D.large, count, ent, diag, kids, maps, pv, P, eng.stream);
-    });
-    const bool fsingle = single && !root;
-    if (unblocked_fronts()) {
+    const bool fsingle = single && !root, unblocked = unblocked_fronts();
+    if (root || unblocked) {  // the whole front assembled in HBM
+        eng.timed(root ? kRootAssemble : kAssemble, 0.0, P * fes * L.asm_words, [&] {
+            if (root)
+                launch_assemble_root(nodes, D.large, count, ent, diag, kids, maps, pv, P, single ? 1 : 0, eng.stream);
+            else if (single)
+                launch_assemble_c64(nodes, D.large, count, ent, diag, kids, maps, pv, P, eng.stream);
+            else
+                launch_assemble_c128(nodes, D.large, count, ent, diag, kids, maps, pv, P, eng.stream);
+        });
+    } else {  // only the border; F22 is gathered by the Schur tiles
+        eng.timed(kGather, 0.0, P * fes * L.gather_words, [&] {
+            launch_gather(nodes, D.gtiles, int32_t(L.gtiles.size()), kids, maps, pv, P, fsingle, eng.stream);
+        });
+    }
+    if (unblocked) {
         eng.timed(root ? kRootBorder : kBorder, P * L.border_flops, P * fes * L.border_words, [&] {
             launch_factor_unblocked(nodes, D.large, count, pv, P, tol, fsingle, root, eng.stream);
         });
         return;
     }
     eng.timed(root ? kRootBorder : kBorder, P * L.border_flops, P * fes * L.border_words, [&] {
-        launch_border(nodes, D.large, count, L.large_ne, pv, P, tol, fsingle, root, eng.stream);
+        launch_border(nodes, D.large, count, L.large_ne, ent, diag, pv, P, tol, fsingle, root, eng.stream);
     });
     eng.timed(root ? kRootSchur : kSchur, P * L.schur_flops, P * fes * L.schur_words, [&] {
-        launch_schur(nodes, D.tiles, int32_t(L.tiles.size()), pv, P, fsingle, root, eng.stream);
+        launch_schur(nodes, D.tiles, int32_t(L.tiles.size()), kids, maps, pv, P, fsingle, root, !root, eng.stream);
     });
 }
 
@@ -574,8 +617,8 @@ void sweep(Engine& eng, const HostPlan& hp, const DevPlan& dp, const PointsView&
     run_level(eng, hp.root_set, dp.root_set, nodes, ent, diag, kids, maps, pv, P, single, true);
     const NodeDesc& r = hp.nodes[hp.root];
     const double nl = r.nl, nr = r.nr;
-    eng.timed(kTrace, P * 8.0 * nl * nr * (nl + nr), P * 16.0 * (nl * nl + nr * nr + nl * nr), [&] {
-        launch_trace(nodes + hp.root, pv, P, eng.stream);
+    eng.timed(kTrace, 0.0, P * 16.0 * (nl * nl + nr * nr + nl * nr), [&] {
+        launch_trace(nodes + hp.root, r.nl, pv, P, eng.stream);
     });
     HIP_CHECK(hipGetLastError());
 }
@@ -645,6 +688,7 @@ int32_t replanned_point(Engine& eng, const cg_system* s, const cg_sigma* g, int 
                 kids.push_back(kd);
             }
             d.kid_end = int32_t(kids.size());
+            add_inverse_maps(kids, d.kid_beg, d.kid_end, maps, int(pos.v.size()));
         };
         if (!root_level) {
             for (int k = 0; k < nn; ++k) {
@@ -759,7 +803,7 @@ int32_t replanned_point(Engine& eng, const cg_system* s, const cg_sigma* g, int 
             upload(dt, tl, eng.stream);
             run_level(eng, L, resolve(o, dl, dt), dd, dent.as<Entry>(), ddiag.as<int32_t>(), dkids.as<KidDesc>(),
                       dmaps.as<int32_t>(), pv, 1, single, true);
-            launch_trace(dd, pv, 1, eng.stream);
+            launch_trace(dd, nodes[0].nl, pv, 1, eng.stream);
             HIP_CHECK(hipGetLastError());
             int32_t rs = 0;
             HIP_CHECK(hipMemcpyAsync(&rs, static_cast<int32_t*>(status.p) + nn, sizeof(int32_t), hipMemcpyDeviceToHost, eng.stream));
@@ -1137,8 +1181,10 @@ int32_t front_schur(const double* f, int64_t n, int64_t ne, int64_t ne1, int32_t
         for (int i0 = 0; i0 < n - ne; i0 += kTile)
             for (int j0 = 0; j0 < n - ne; j0 += kTile) tl.push_back({0, i0, j0, 0});
         upload(tiles, tl, eng.stream);
-        launch_border(nodes.as<NodeDesc>(), lv.as<int32_t>(), 1, int32_t(ne), pv, 1, tol, single, false, eng.stream);
-        launch_schur(nodes.as<NodeDesc>(), tiles.as<Tile>(), int32_t(tl.size()), pv, 1, single, false, eng.stream);
+        launch_border(nodes.as<NodeDesc>(), lv.as<int32_t>(), 1, int32_t(ne), nullptr, nullptr, pv, 1, tol, single, false,
+                      eng.stream);
+        launch_schur(nodes.as<NodeDesc>(), tiles.as<Tile>(), int32_t(tl.size()), nullptr, nullptr, pv, 1, single, false,
+                     false, eng.stream);
     }
     HIP_CHECK(hipGetLastError());
     int32_t st = 0;

@@ -18,6 +18,7 @@
 #include <mutex>
 #include <set>
 #include <stdexcept>
+#include <type_traits>
 
 #include "cx.hpp"
 #include "fronts.hpp"
@@ -78,6 +79,93 @@ __device__ double block_sum(double v, double* sv) {
     return r;
 }
 
+// Largest value and smallest index on ties over one wavefront, with a payload; every lane gets
+// the result.
+__device__ __forceinline__ void wave_argmax(double& v, int& idx, int& pay) {
+    for (int off = 32; off > 0; off >>= 1) {
+        const double ov = __shfl_xor(v, off, 64);
+        const int oi = __shfl_xor(idx, off, 64), op = __shfl_xor(pay, off, 64);
+        if (ov > v || (ov == v && oi < idx)) {
+            v = ov;
+            idx = oi;
+            pay = op;
+        }
+    }
+}
+__device__ __forceinline__ void wave_argmax(double& v, int& idx) {
+    int pay = 0;
+    wave_argmax(v, idx, pay);
+}
+
+// Assembly loops issue their global loads in batches of kBatch per lane, so that a lane has
+// kBatch loads in flight instead of one (the loops are latency-bound otherwise). Indexes past the
+// end are clamped for the loads; only the updates are guarded.
+constexpr int kBatch = 8;
+
+// The H entries of a front: add(at, v) with at = a + b n and v = -H (conjugated as listed).
+template <int NT, class Add>
+__device__ __forceinline__ void add_entries(const Entry* ent, int beg, int end, const double2* vals, int n, Add add) {
+    for (int base = beg + int(threadIdx.x); base < end; base += NT * kBatch) {
+        Entry en[kBatch];
+#pragma unroll
+        for (int u = 0; u < kBatch; ++u) en[u] = ent[min(base + u * NT, end - 1)];
+        double2 v[kBatch];
+#pragma unroll
+        for (int u = 0; u < kBatch; ++u) v[u] = vals[en[u].k];
+#pragma unroll
+        for (int u = 0; u < kBatch; ++u)
+            if (base + u * NT < end) {
+                double2 x = v[u];
+                x.x = -x.x;
+                x.y = en[u].conj ? x.y : -x.y;
+                add(en[u].a + en[u].b * n, x);
+            }
+    }
+}
+
+// A child's Schur complement S (m x m, leading dimension ld) added at map[a] + map[b] n:
+// add(at, value).
+template <int NT, class TK, class Add>
+__device__ __forceinline__ void extend_add(const TK* S, int ld, const int32_t* map, int m, int n, Add add) {
+    const int m2 = m * m;
+    for (int base = int(threadIdx.x); base < m2; base += NT * kBatch) {
+        TK v[kBatch];
+        int at[kBatch];
+#pragma unroll
+        for (int u = 0; u < kBatch; ++u) {
+            const int idx = min(base + u * NT, m2 - 1), a = idx % m, b = idx / m;
+            v[u] = S[a + int64_t(b) * ld];
+            at[u] = map[a] + map[b] * n;
+        }
+#pragma unroll
+        for (int u = 0; u < kBatch; ++u)
+            if (base + u * NT < m2) add(at[u], v[u]);
+    }
+}
+
+// extend_add for a front in global memory: the front's old values are loaded for the whole batch
+// before any is stored (the positions of one child are distinct).
+template <int NT, class TF, class TK>
+__device__ __forceinline__ void extend_add_global(TF* F, const TK* S, int ld, const int32_t* map, int m, int n) {
+    const int m2 = m * m;
+    for (int base = int(threadIdx.x); base < m2; base += NT * kBatch) {
+        TK v[kBatch];
+        int64_t at[kBatch];
+#pragma unroll
+        for (int u = 0; u < kBatch; ++u) {
+            const int idx = min(base + u * NT, m2 - 1), a = idx % m, b = idx / m;
+            v[u] = S[a + int64_t(b) * ld];
+            at[u] = map[a] + int64_t(map[b]) * n;
+        }
+        TF f[kBatch];
+#pragma unroll
+        for (int u = 0; u < kBatch; ++u) f[u] = F[at[u]];
+#pragma unroll
+        for (int u = 0; u < kBatch; ++u)
+            if (base + u * NT < m2) F[at[u]] = cadd(f[u], convert<TF>(v[u]));
+    }
+}
+
 // Assembly of one front (see the file comment). TF: type of the front, TK: type of the children.
 template <class TF, class TK, bool kRoot>
 __global__ void __launch_bounds__(kThreads)
@@ -131,15 +219,7 @@ __global__ void __launch_bounds__(kThreads)
     }
     for (int c = nd.kid_beg; c < nd.kid_end; ++c) {
         const KidDesc kd = kids[c];
-        const TK* S = kidbase + kd.src;
-        const int32_t* map = maps + kd.map_beg;
-        const int m = kd.size;
-        const int64_t m2 = int64_t(m) * m;
-        for (int64_t idx = tid; idx < m2; idx += kThreads) {
-            int a = int(idx % m), b = int(idx / m);
-            int64_t at = map[a] + int64_t(map[b]) * n;
-            F[at] = cadd(F[at], convert<TF>(S[a + int64_t(b) * kd.ld]));
-        }
+        extend_add_global<kThreads>(F, kidbase + kd.src, kd.ld, maps + kd.map_beg, kd.size, n);
         __syncthreads();
     }
 }
@@ -218,41 +298,26 @@ struct Mfma<float2> {
     static __device__ __forceinline__ int row(int lane, int j) { return 4 * (lane / 16) + j; }
 };
 
-// Largest value and smallest index on ties over one wavefront, with a payload; every lane gets
-// the result.
-__device__ __forceinline__ void wave_argmax(double& v, int& idx, int& pay) {
-    for (int off = 32; off > 0; off >>= 1) {
-        const double ov = __shfl_xor(v, off, 64);
-        const int oi = __shfl_xor(idx, off, 64), op = __shfl_xor(pay, off, 64);
-        if (ov > v || (ov == v && oi < idx)) {
-            v = ov;
-            idx = oi;
-            pay = op;
-        }
-    }
-}
-__device__ __forceinline__ void wave_argmax(double& v, int& idx) {
-    int pay = 0;
-    wave_argmax(v, idx, pay);
-}
+
 
 // ---------------------------------------------------------------------------------------------
 // Small fronts: assembly and elimination in LDS, one workgroup of NT lanes per (front, point).
 //
-// The front is assembled straight into LDS (no HBM copy of the assembled front), eliminated with
-// one lane per row, and written back once: the whole factored front when the factors are kept
-// for the solves (pv.keep), else only the Schur block the parent reads. Lane t owns physical row
-// t of the LDS copy for the whole elimination; a row interchange only swaps the logical positions
-// two lanes track, and the rows are written back in logical order, which leaves the factors of
+// The front is assembled straight into LDS (no HBM copy of the assembled front; leading
+// dimension n + 1 so that lanes on neighbouring columns hit different banks), eliminated there,
+// and written back once: the whole factored front when the factors are kept for the solves
+// (pv.keep), else only the Schur block the parent reads. G lanes share a row: lane t owns
+// physical row t / G and the columns j = t (mod G) of it for the whole elimination (the G lanes
+// of a row sit in one wavefront). A row interchange only swaps the logical positions two rows
+// track, and the rows are written back in logical order, which leaves the factors of
 // interchanging whole rows (LAPACK's convention, the one the solves use).
-template <class T, int NT, bool kAssemble>
+template <class T, int NT, int G, bool kAssemble>
 __global__ void __launch_bounds__(NT)
     front_lds_kernel(const NodeDesc* __restrict__ nodes, const int32_t* __restrict__ level,
                      const Entry* __restrict__ ent, const int32_t* __restrict__ diag,
                      const KidDesc* __restrict__ kids, const int32_t* __restrict__ maps, PointsView pv, double tol) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     __shared__ double sv[NT / 64];
-    __shared__ int32_t perm[kLdsMaxN];
     __shared__ double rv[2][NT / 64];
     __shared__ int32_t ri[2][NT / 64], rp[2][NT / 64];
     T* W = reinterpret_cast<T*>(lds_raw);
@@ -261,57 +326,45 @@ __global__ void __launch_bounds__(NT)
     T* F = reinterpret_cast<T*>(pv.arena) + p * pv.arena_stride + nd.off;
     int32_t* piv = pv.piv + p * pv.piv_stride + nd.piv_off;
     int32_t* status = pv.status + p * pv.status_stride + nd.slot;
-    const int n = nd.n, ne = nd.ne, ne1 = nd.ne1, tid = threadIdx.x, lane = tid & 63;
-    const int n2 = n * n;
+    const int n = nd.n, ne = nd.ne, ne1 = nd.ne1, tid = threadIdx.x, lane = tid & 63, ld = n + 1;
+    const int nw = n * ld;
     if (kAssemble) {
         const T zero = cx<T>(0.0, 0.0);
-        for (int i = tid; i < n2; i += NT) W[i] = zero;
+        for (int i = tid; i < nw; i += NT) W[i] = zero;
         __syncthreads();
-        const double2* vals = pv.vals[p];
-        for (int e = nd.ent_beg + tid; e < nd.ent_end; e += NT) {
-            Entry en = ent[e];
-            double2 v = vals[en.k];
-            if (en.conj) v.y = -v.y;
-            const int at = en.a + en.b * n;
-            W[at] = csub(W[at], convert<T>(v));
-        }
+        add_entries<NT>(ent, nd.ent_beg, nd.ent_end, pv.vals[p], ld,
+                        [&](int at, double2 v) { W[at] = cadd(W[at], convert<T>(v)); });
         __syncthreads();
         const T z = convert<T>(pv.z[p]);
         for (int d = nd.diag_beg + tid; d < nd.diag_end; d += NT) {
             const int q = diag[d];
-            W[q + q * n] = cadd(W[q + q * n], z);
+            W[q + q * ld] = cadd(W[q + q * ld], z);
         }
         __syncthreads();
         const T* kidbase = reinterpret_cast<const T*>(pv.arena) + p * pv.arena_stride;
         for (int c = nd.kid_beg; c < nd.kid_end; ++c) {
             const KidDesc kd = kids[c];
-            const T* S = kidbase + kd.src;
-            const int32_t* map = maps + kd.map_beg;
-            const int m = kd.size, m2 = m * m;
-            for (int idx = tid; idx < m2; idx += NT) {
-                const int a = idx % m, b = idx / m;
-                const int at = map[a] + map[b] * n;
-                W[at] = cadd(W[at], S[a + int64_t(b) * kd.ld]);
-            }
+            extend_add<NT>(kidbase + kd.src, kd.ld, maps + kd.map_beg, kd.size, ld,
+                           [&](int at, T v) { W[at] = cadd(W[at], v); });
             __syncthreads();
         }
     } else {
-        for (int i = tid; i < n2; i += NT) W[i] = F[i];
+        for (int idx = tid; idx < n * n; idx += NT) W[(idx % n) + (idx / n) * ld] = F[idx];
         __syncthreads();
     }
     double m = 0.0;
-    for (int idx = tid; idx < ne1 * ne1; idx += NT) m = fmax(m, cabs(W[(idx % ne1) + (idx / ne1) * n]));
+    for (int idx = tid; idx < ne1 * ne1; idx += NT) m = fmax(m, cabs(W[(idx % ne1) + (idx / ne1) * ld]));
     const double scale = block_max(m, sv);
-    // Lane t owns physical row t and tracks its logical position lg.
-    int lg = tid;
+    const int g = tid % G, r = tid / G;  // this lane's column class and physical row
+    int lg = r;                           // the row's logical position
     for (int k = 0; k < ne; ++k) {
         const int lim = k < ne1 ? ne1 : ne;
         double big = -1.0;
         int at = kLdsMaxN, ph = 0;  // pivot: value, logical row, physical row
-        if (tid < n && lg >= k && lg < lim) {
-            big = cabs(W[tid + k * n]);
+        if (g == 0 && r < n && lg >= k && lg < lim) {
+            big = cabs(W[r + k * ld]);
             at = lg;
-            ph = tid;
+            ph = r;
         }
         wave_argmax(big, at, ph);
         if constexpr (NT > 64) {  // combine the waves (double-buffered by step parity)
@@ -338,30 +391,26 @@ __global__ void __launch_bounds__(NT)
             return;
         }
         if (tid == 0) piv[k] = at;
-        if (tid == ph)
+        if (r == ph)
             lg = k;
         else if (lg == k)
             lg = at;
-        if (tid < n && lg > k) {
-            const T l = cmul(W[tid + k * n], crecip(W[ph + k * n]));
-            W[tid + k * n] = l;
-            for (int j = k + 1; j < n; ++j) W[tid + j * n] = csub(W[tid + j * n], cmul(l, W[ph + j * n]));
+        if (r < n && lg > k) {
+            // The G lanes of this row read W[r, k] before lane g = 0 overwrites it (one wavefront).
+            const T l = cmul(W[r + k * ld], crecip(W[ph + k * ld]));
+            const int j0 = k + 1 + ((g - (k + 1)) % G + G) % G;
+            for (int j = j0; j < n; j += G) W[r + j * ld] = csub(W[r + j * ld], cmul(l, W[ph + j * ld]));
+            if (g == 0) W[r + k * ld] = l;
         }
         __syncthreads();
     }
     if (tid == 0) *status = kFrontOk;
-    if (tid < n) perm[lg] = tid;
-    __syncthreads();
-    if (pv.keep) {
-        for (int idx = tid; idx < n2; idx += NT) {
-            const int i = idx % n, j = idx / n;
-            F[idx] = W[perm[i] + j * n];
-        }
-    } else {
-        const int nb = n - ne, nb2 = nb * nb;
-        for (int idx = tid; idx < nb2; idx += NT) {
-            const int i = ne + idx % nb, j = ne + idx / nb;
-            F[i + j * n] = W[perm[i] + j * n];
+    // Row r goes to logical row lg: lanes of one wave cover G columns of 64 / G rows.
+    if (r < n) {
+        if (pv.keep) {
+            for (int j = g; j < n; j += G) F[lg + j * n] = W[r + j * ld];
+        } else if (lg >= ne) {
+            for (int j = ne + ((g - ne) % G + G) % G; j < n; j += G) F[lg + j * n] = W[r + j * ld];
         }
     }
 }
@@ -370,63 +419,83 @@ __global__ void __launch_bounds__(NT)
 // Large fronts, step 1: the border. Partial LU of F11 (row pivoting within the separator rows,
 // the pivot test of field.rs:61), L21 = F21 U11^-1 and U12 = L11^-1 P F12, in place, panel by
 // panel (kPanel columns), without touching F22: the Schur update S = F22 - L21 U12 is left to
-// schur_kernel, which spreads it over many workgroups. Per panel:
-//   1. the panel's separator rows [k0, ne) are factored in LDS (when they fit, else in place);
-//   2. its interchanges are applied to the other columns, the L21 rows are solved against the
-//      panel's U11 block (one lane per row), and the panel goes back to the front;
-//   3. the panel's U rows are solved with its unit-lower L11 block (one lane per column);
-//   4. the rest of the border (rows or columns below ne) gets the rank-kPanel update on MFMA.
+// schur_kernel, which spreads it over many workgroups. The border is factored left-looking
+// (Crout): a panel's columns and rows receive the updates of all earlier panels at once, as
+// MFMA products with K = k0, when the panel comes up. Per panel [k0, k1):
+//   1. panel columns, rows [k0, n):   F -= L[:, 0:k0] U[0:k0, panel];
+//   2. the panel's separator rows [k0, ne) are factored in LDS (when they fit, else in place);
+//   3. its interchanges (as one permutation of at most 2 kPanel rows) reach the other columns,
+//      the L21 rows are solved against the panel's U11 block, the panel goes back to the front;
+//   4. panel rows, columns [k1, n):   F -= L[panel, 0:k0] U[0:k0, k1:n];
+//   5. the panel's U rows are solved with its unit-lower L11 block (one lane per column).
 constexpr int kBorderThreads = 256;
 
+// C -= A B for one 16 x 16 tile of the front F: C = F[i0:i0+16, j0:j0+16], A = F[i0:, 0:kend]
+// (L columns), B = F[0:kend, j0:] (U rows), computed as C^T -= B^T A^T so that lanes read and
+// write the front's columns contiguously; entries outside rows < rmax, columns < cmax are not
+// stored. Loads are clamped inside the front and their values selected (a select of addresses
+// would go through flat), 16 values of K per batch of loads.
 template <class T>
-__device__ __forceinline__ void mfma_tile_update(T* F, int n, int ne, int i0, int j0, int k0, int w, int lane) {
-    // F[i, j] -= sum_c L[i, k0 + c] U[k0 + c, j] over c < w for the 16 x 16 tile at (i0, j0), as
-    // C^T -= U^T L^T so that lanes read and write the front's columns contiguously. Entries in
-    // F22 (i, j >= ne) are not stored.
+__device__ __forceinline__ void mfma_gemm_tile(T* F, int n, int i0, int j0, int kend, int rmax, int cmax, int lane) {
     using M = Mfma<T>;
     typename M::acc cr = {0, 0, 0, 0}, ci = {0, 0, 0, 0};
     const int li = lane % 16, lk = lane / 16;
+    const int64_t ii = min(i0 + li, n - 1), jj0 = int64_t(min(j0 + li, n - 1)) * n;
+    for (int k = 0; k < kend; k += 16) {
+        T a[4], b[4];
 #pragma unroll
-    for (int kk = 0; kk < kPanel; kk += 4) {
-        const int c = kk + lk;
-        const int j = j0 + li, i = i0 + li;
-        // Clamped loads and a select of the value (a select of addresses would go through flat).
-        T a = F[(k0 + min(c, w - 1)) + int64_t(min(j, n - 1)) * n];
-        T b = F[min(i, n - 1) + int64_t(k0 + min(c, w - 1)) * n];
-        if (!(c < w && j < n)) a = cx<T>(0.0, 0.0);
-        if (!(c < w && i < n)) b = cx<T>(0.0, 0.0);
-        cr = M::mac(a.x, b.x, cr);
-        cr = M::mac(-a.y, b.y, cr);
-        ci = M::mac(a.x, b.y, ci);
-        ci = M::mac(a.y, b.x, ci);
+        for (int s = 0; s < 4; ++s) {
+            const int kk = k + 4 * s + lk, kc = min(kk, kend - 1);
+            a[s] = F[kc + jj0];
+            b[s] = F[ii + int64_t(kc) * n];
+            if (kk >= kend) {
+                a[s] = cx<T>(0.0, 0.0);
+                b[s] = cx<T>(0.0, 0.0);
+            }
+        }
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            cr = M::mac(a[s].x, b[s].x, cr);
+            cr = M::mac(-a[s].y, b[s].y, cr);
+            ci = M::mac(a[s].x, b[s].y, ci);
+            ci = M::mac(a[s].y, b[s].x, ci);
+        }
     }
     const int i = i0 + li;
 #pragma unroll
-    for (int jj = 0; jj < 4; ++jj) {
-        const int j = j0 + M::row(lane, jj);
-        if (i < n && j < n && (i < ne || j < ne)) {
+    for (int q = 0; q < 4; ++q) {
+        const int j = j0 + M::row(lane, q);
+        if (i < rmax && j < cmax) {
             T& d = F[i + int64_t(j) * n];
-            d.x -= cr[jj];
-            d.y -= ci[jj];
+            d.x -= cr[q];
+            d.y -= ci[q];
         }
     }
 }
 
 // The border factorisation of one front; kLds: the panel's separator rows are held in LDS (P).
+// mv: LDS table of the panel's row moves (count, then destination and source rows); buf: LDS
+// staging of the moved rows (2 kPanel x kMoveCols).
 template <class T, bool kRoot, bool kLds>
-__device__ __forceinline__ void border_factor(T* F, T* P, int32_t* spiv, int n, int ne, int ne1, int32_t* piv,
-                                              int32_t* status, double scale, double tol) {
+__device__ __forceinline__ void border_factor(T* F, T* P, T* buf, int32_t* spiv, int32_t* mv, int n, int ne, int ne1,
+                                              int32_t* piv, int32_t* status, double scale, double tol) {
     constexpr int NT = kBorderThreads;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     auto at = [&](int i, int j) -> T& { return F[i + int64_t(j) * n]; };
     for (int k0 = 0; k0 < ne; k0 += kPanel) {
         const int w = min(kPanel, ne - k0), R = ne - k0, k1 = k0 + w;
+        // 1. Earlier panels' updates of this panel's columns.
+        if (k0 > 0) {
+            for (int t = wave; t * 16 < n - k0; t += NT / 64) mfma_gemm_tile(F, n, k0 + 16 * t, k0, k0, n, k1, lane);
+            __syncthreads();
+        }
         // Panel element (row i in [k0, ne), panel column c).
         auto pp = [&](int i, int c) -> T& { return kLds ? P[(i - k0) + c * R] : at(i, k0 + c); };
-        if (kLds)
+        if (kLds) {
             for (int e = tid; e < R * w; e += NT) P[e] = at(k0 + e % R, k0 + e / R);
-        __syncthreads();
-        // 1. Panel factorisation on the separator rows.
+            __syncthreads();
+        }
+        // 2. Panel factorisation on the separator rows.
         for (int c = 0; c < w; ++c) {
             const int k = k0 + c, lim = k < ne1 ? ne1 : ne;
             double big = -1.0;
@@ -462,18 +531,63 @@ __device__ __forceinline__ void border_factor(T* F, T* P, int32_t* spiv, int n, 
             }
             __syncthreads();
         }
-        // 2. Interchanges on the other columns (separator rows only), the panel back in place, and
-        //    the L21 rows: l U11 = f with U11 the panel's upper triangle.
-        for (int j = tid; j < n; j += NT) {
-            if (j >= k0 && j < k1) continue;
+        // The interchanges k0 + c <-> spiv[c], in order, as one permutation: row mv[1 + m] of the
+        // front receives the old row mv[1 + 2 kPanel + m].
+        if (tid == 0) {  // rows touched in mv[1 + 2 kPanel + s] (positions) and mv[1 + s] (sources)
+            int32_t* pos = mv + 1 + 2 * kPanel;
+            int32_t* src = mv + 1;
+            int cnt = 0;
+            auto slot = [&](int r) {
+                for (int s = 0; s < cnt; ++s)
+                    if (pos[s] == r) return s;
+                pos[cnt] = r;
+                src[cnt] = r;
+                return cnt++;
+            };
             for (int c = 0; c < w; ++c) {
-                const int q = spiv[c], k = k0 + c;
-                if (q != k) {
-                    T x = at(k, j);
-                    at(k, j) = at(q, j);
-                    at(q, j) = x;
+                const int a = slot(k0 + c), b = slot(spiv[c]);
+                const int x = src[a];
+                src[a] = src[b];
+                src[b] = x;
+            }
+            int m = 0;  // compact to the rows that change: mv[1 + m] <- mv[1 + 2 kPanel + m]
+            for (int s = 0; s < cnt; ++s)
+                if (pos[s] != src[s]) {
+                    const int ps = pos[s], sr = src[s];
+                    mv[1 + m] = ps;
+                    mv[1 + 2 * kPanel + m] = sr;
+                    ++m;
+                }
+            mv[0] = m;
+        }
+        __syncthreads();
+        // 3. The moves on the columns outside the panel (separator rows only), the panel back in
+        //    place, and the L21 rows: l U11 = f with U11 the panel's upper triangle.
+        //    The moved rows are staged through LDS (buf) kMoveCols columns at a time: all loads of a
+        //    chunk land before any store.
+        const int moves = mv[0];
+        for (int j0 = 0; moves > 0 && j0 < n - w; j0 += kMoveCols) {
+            const int cols = min(kMoveCols, n - w - j0), pairs = moves * cols;
+            auto col = [&](int jc) { return j0 + jc < k0 ? j0 + jc : j0 + jc + w; };
+            for (int e0 = tid; e0 < pairs; e0 += NT * 4) {
+                T v[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int e = min(e0 + u * NT, pairs - 1), m = e / cols, jc = e % cols;
+                    v[u] = at(mv[1 + 2 * kPanel + m], col(jc));
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int e = e0 + u * NT;
+                    if (e < pairs) buf[(e / cols) * kMoveCols + e % cols] = v[u];
                 }
             }
+            __syncthreads();
+            for (int e = tid; e < pairs; e += NT) {
+                const int m = e / cols, jc = e % cols;
+                at(mv[1 + m], col(jc)) = buf[m * kMoveCols + jc];
+            }
+            __syncthreads();
         }
         if (kLds)
             for (int e = tid; e < R * w; e += NT) at(k0 + e % R, k0 + e / R) = P[e];
@@ -499,7 +613,12 @@ __device__ __forceinline__ void border_factor(T* F, T* P, int32_t* spiv, int n, 
         }
         __syncthreads();
         if (k1 >= n) break;
-        // 3. U rows of the panel: unit-lower solve, one lane per column.
+        // 4. Earlier panels' updates of this panel's rows right of it.
+        if (k0 > 0) {
+            for (int t = wave; t * 16 < n - k1; t += NT / 64) mfma_gemm_tile(F, n, k0, k1 + 16 * t, k0, k1, n, lane);
+            __syncthreads();
+        }
+        // 5. U rows of the panel: unit-lower solve, one lane per column.
         for (int j = k1 + tid; j < n; j += NT) {
             T u[kPanel];
 #pragma unroll
@@ -521,28 +640,18 @@ __device__ __forceinline__ void border_factor(T* F, T* P, int32_t* spiv, int n, 
                 if (c < w) at(k0 + c, j) = u[c];
         }
         __syncthreads();
-        // 4. Rank-w update of the border below and right of the panel, 16 x 16 tiles on MFMA; the
-        //    tiles inside F22 are skipped.
-        if (k1 < ne) {
-            const int tiles = (n - k1 + 15) / 16;
-            for (int t = wave; t < tiles * tiles; t += NT / 64) {
-                const int i0 = k1 + 16 * (t % tiles), j0 = k1 + 16 * (t / tiles);
-                if (i0 >= ne && j0 >= ne) continue;
-                mfma_tile_update(F, n, ne, i0, j0, k0, w, lane);
-            }
-        }
-        __syncthreads();
     }
     if (tid == 0) *status = kFrontOk;
 }
 
 template <class T, bool kRoot>
 __global__ void __launch_bounds__(kBorderThreads)
-    border_kernel(const NodeDesc* __restrict__ nodes, const int32_t* __restrict__ level, PointsView pv, double tol,
-                  int lds_rows) {
+    border_kernel(const NodeDesc* __restrict__ nodes, const int32_t* __restrict__ level, const Entry* __restrict__ ent,
+                  const int32_t* __restrict__ diag, PointsView pv, double tol, int lds_rows) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     __shared__ double sv[kBorderThreads / 64];
     __shared__ int32_t spiv[kPanel];
+    __shared__ int32_t mv[1 + 4 * kPanel];
     const NodeDesc nd = nodes[level[blockIdx.x]];
     const int64_t p = blockIdx.y;
     T* F = (kRoot ? reinterpret_cast<T*>(pv.root_arena) + p * pv.root_stride
@@ -551,14 +660,79 @@ __global__ void __launch_bounds__(kBorderThreads)
     int32_t* piv = pv.piv + p * pv.piv_stride + nd.piv_off;
     int32_t* status = pv.status + p * pv.status_stride + nd.slot;
     const int n = nd.n, ne = nd.ne, ne1 = nd.ne1;
+    if (!kRoot && nd.ent_end > nd.ent_beg) {  // -H on the separator (the children were gathered before)
+        add_entries<kBorderThreads>(ent, nd.ent_beg, nd.ent_end, pv.vals[p], n,
+                                    [&](int at, double2 v) { F[at] = cadd(F[at], convert<T>(v)); });
+        __syncthreads();
+    }
+    if (!kRoot && nd.diag_end > nd.diag_beg) {  // z on the separator's diagonal
+        const T z = convert<T>(pv.z[p]);
+        for (int d = nd.diag_beg + int(threadIdx.x); d < nd.diag_end; d += kBorderThreads) {
+            const int64_t q = diag[d];
+            F[q + q * n] = cadd(F[q + q * n], z);
+        }
+        __syncthreads();
+    }
     double m = 0.0;
     for (int idx = threadIdx.x; idx < ne1 * ne1; idx += kBorderThreads)
         m = fmax(m, cabs(F[(idx % ne1) + int64_t(idx / ne1) * n]));
     const double scale = block_max(m, sv);
+    T* buf = reinterpret_cast<T*>(lds_raw);  // the move buffer first, then the panel
     if (ne <= lds_rows)
-        border_factor<T, kRoot, true>(F, reinterpret_cast<T*>(lds_raw), spiv, n, ne, ne1, piv, status, scale, tol);
+        border_factor<T, kRoot, true>(F, buf + 2 * kPanel * kMoveCols, buf, spiv, mv, n, ne, ne1, piv, status, scale,
+                                      tol);
     else
-        border_factor<T, kRoot, false>(F, nullptr, spiv, n, ne, ne1, piv, status, scale, tol);
+        border_factor<T, kRoot, false>(F, nullptr, buf, spiv, mv, n, ne, ne1, piv, status, scale, tol);
+}
+
+// ---------------------------------------------------------------------------------------------
+// Large fronts, step 0: the children's Schur complements gathered into the border of the front
+// (rows or columns below ne; F22 is gathered by schur_kernel), one workgroup per 64 x 64 tile:
+// F[i, j] = sum over children c of S_c[inv_c(i), inv_c(j)], written once (no zero fill, no
+// read-modify-write).
+template <class T>
+__global__ void __launch_bounds__(256)
+    gather_kernel(const NodeDesc* __restrict__ nodes, const Tile* __restrict__ tiles, const KidDesc* __restrict__ kids,
+                  const int32_t* __restrict__ maps, PointsView pv) {
+    const Tile tl = tiles[blockIdx.x];
+    const NodeDesc nd = nodes[tl.node];
+    const int64_t p = blockIdx.y;
+    const T* kidbase = reinterpret_cast<const T*>(pv.arena) + p * pv.arena_stride;
+    T* F = reinterpret_cast<T*>(pv.arena) + p * pv.arena_stride + nd.off;
+    const int n = nd.n, ne = nd.ne;
+    const int rows = min(kTile, n - tl.i0), cols = min(kTile, n - tl.j0), cnt = rows * cols;
+    for (int base = int(threadIdx.x); base < cnt; base += 256 * kBatch) {
+        int ii[kBatch], jj[kBatch];
+#pragma unroll
+        for (int u = 0; u < kBatch; ++u) {
+            const int idx = min(base + u * 256, cnt - 1);
+            ii[u] = tl.i0 + idx % rows;
+            jj[u] = tl.j0 + idx / rows;
+        }
+        T acc[kBatch];
+#pragma unroll
+        for (int u = 0; u < kBatch; ++u) acc[u] = cx<T>(0.0, 0.0);
+        for (int c = nd.kid_beg; c < nd.kid_end; ++c) {  // per child: all map loads, then all values
+            const KidDesc kd = kids[c];
+            const int32_t* inv = maps + kd.inv_beg;
+            const T* S = kidbase + kd.src;
+            int a[kBatch], b[kBatch];
+#pragma unroll
+            for (int u = 0; u < kBatch; ++u) {
+                a[u] = inv[ii[u]];
+                b[u] = inv[jj[u]];
+            }
+#pragma unroll
+            for (int u = 0; u < kBatch; ++u) {
+                T v = S[max(a[u], 0) + int64_t(max(b[u], 0)) * kd.ld];
+                if (a[u] < 0 || b[u] < 0) v = cx<T>(0.0, 0.0);
+                acc[u] = cadd(acc[u], v);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kBatch; ++u)
+            if (base + u * 256 < cnt && (ii[u] < ne || jj[u] < ne)) F[ii[u] + int64_t(jj[u]) * n] = acc[u];
+    }
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -568,9 +742,10 @@ __global__ void __launch_bounds__(kBorderThreads)
 // chunks of kChunk columns / rows. Fronts whose border failed are skipped.
 constexpr int kChunk = 16, kTileLd = kTile + 1;
 
-template <class T, bool kRoot>
+template <class T, bool kRoot, bool kGather>
 __global__ void __launch_bounds__(256)
-    schur_kernel(const NodeDesc* __restrict__ nodes, const Tile* __restrict__ tiles, PointsView pv) {
+    schur_kernel(const NodeDesc* __restrict__ nodes, const Tile* __restrict__ tiles, const KidDesc* __restrict__ kids,
+                 const int32_t* __restrict__ maps, PointsView pv) {
     using M = Mfma<T>;
     __shared__ T As[kChunk * kTileLd];  // As[k * kTileLd + i] = L21[i0 + i, kc + k]
     __shared__ T Bs[kChunk * kTileLd];  // Bs[k * kTileLd + j] = U12[kc + k, j0 + j]
@@ -629,9 +804,49 @@ __global__ void __launch_bounds__(256)
         __syncthreads();
     }
     // Block (x, y) of the wave: lane l, element jj holds (L21 U12)[i, j] with
-    // i = wi + 16 y + l % 16 and j = wj + 16 x + row(l, jj).
+    // i = wi + 16 y + l % 16 and j = wj + 16 x + row(l, jj). F22 is read (all 16 values of a lane
+    // first) or gathered from the children, then S = F22 - L21 U12 is stored.
+    // Two halves (x) to bound the registers: F22 values (read, or gathered from the children with
+    // all map loads of a child before its 8 values), then S = F22 - L21 U12 stored.
+    const T* kidbase = reinterpret_cast<const T*>(pv.arena) + p * pv.arena_stride;
+    int ir[2];  // this lane's rows of S (clamped)
 #pragma unroll
-    for (int x = 0; x < 2; ++x)
+    for (int y = 0; y < 2; ++y) ir[y] = min(tl.i0 + wi + 16 * y + lane % 16, nb - 1);
+#pragma unroll
+    for (int x = 0; x < 2; ++x) {
+        int jc[4];
+#pragma unroll
+        for (int jj = 0; jj < 4; ++jj) jc[jj] = min(tl.j0 + wj + 16 * x + M::row(lane, jj), nb - 1);
+        T f22[2][4];
+        if (kGather) {
+#pragma unroll
+            for (int y = 0; y < 2; ++y)
+#pragma unroll
+                for (int jj = 0; jj < 4; ++jj) f22[y][jj] = zero;
+            for (int c = nd.kid_beg; c < nd.kid_end; ++c) {
+                const KidDesc kd = kids[c];
+                const int32_t* inv = maps + kd.inv_beg + ne;
+                const T* S = kidbase + kd.src;
+                int a[2], b[4];
+#pragma unroll
+                for (int y = 0; y < 2; ++y) a[y] = inv[ir[y]];
+#pragma unroll
+                for (int jj = 0; jj < 4; ++jj) b[jj] = inv[jc[jj]];
+#pragma unroll
+                for (int y = 0; y < 2; ++y)
+#pragma unroll
+                    for (int jj = 0; jj < 4; ++jj) {
+                        T v = S[max(a[y], 0) + int64_t(max(b[jj], 0)) * kd.ld];
+                        if (a[y] < 0 || b[jj] < 0) v = zero;
+                        f22[y][jj] = cadd(f22[y][jj], v);
+                    }
+            }
+        } else {
+#pragma unroll
+            for (int y = 0; y < 2; ++y)
+#pragma unroll
+                for (int jj = 0; jj < 4; ++jj) f22[y][jj] = F[(ne + ir[y]) + int64_t(ne + jc[jj]) * n];
+        }
 #pragma unroll
         for (int y = 0; y < 2; ++y) {
             const int i = tl.i0 + wi + 16 * y + lane % 16;
@@ -639,12 +854,14 @@ __global__ void __launch_bounds__(256)
             for (int jj = 0; jj < 4; ++jj) {
                 const int j = tl.j0 + wj + 16 * x + M::row(lane, jj);
                 if (i < nb && j < nb) {
-                    T& d = F[(ne + i) + int64_t(ne + j) * n];
+                    T d = f22[y][jj];
                     d.x -= cr[x][y][jj];
                     d.y -= ci[x][y][jj];
+                    F[(ne + i) + int64_t(ne + j) * n] = d;
                 }
             }
         }
+    }
 }
 
 template <class T, bool kRoot>
@@ -683,6 +900,52 @@ __global__ void __launch_bounds__(kThreads) trace_kernel(const NodeDesc* __restr
     }
 }
 
+// T = |W_L^H G_LR W_R|_F^2 = Tr[Gamma_L G Gamma_R G^H] with Gamma = W W^H: columns of X = G_LR W_R
+// in chunks that fit the LDS, then the squared moduli of W_L^H X. Work 8 nl nr cr + 8 cl nl cr
+// flops instead of 8 nl nr (nl + nr).
+constexpr int kTraceLds = 64 * 1024;
+__global__ void __launch_bounds__(kThreads) trace_w_kernel(const NodeDesc* __restrict__ rootp, PointsView pv) {
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    __shared__ double sv[kThreads / 64];
+    double2* X = reinterpret_cast<double2*>(lds_raw);
+    const NodeDesc nd = *rootp;
+    const int64_t p = blockIdx.x;
+    const double2* F = reinterpret_cast<const double2*>(pv.root_arena) + p * pv.root_stride + nd.off;
+    const int n = nd.n, k0 = nd.ne, nl = nd.nl, nr = nd.nr, tid = threadIdx.x;
+    const double2* wl = pv.w + p * pv.sigma_stride;
+    const double2* wr = wl + int64_t(nl) * nl;
+    const int cl = pv.wcols[2 * p], cr = pv.wcols[2 * p + 1];
+    const int chunk = max(1, min(cr, kTraceLds / int(sizeof(double2)) / max(nl, 1)));
+    // The border block holds -G_LR; the sign cancels in |.|^2.
+    double acc = 0.0;
+    for (int c0 = 0; c0 < cr; c0 += chunk) {
+        const int cc = min(chunk, cr - c0);
+        for (int idx = tid; idx < nl * cc; idx += kThreads) {
+            const int a = idx % nl, c = c0 + idx / nl;
+            double2 x = cx<double2>(0, 0);
+            for (int b = 0; b < nr; ++b) x = cadd(x, cmul(F[(k0 + a) + int64_t(k0 + b) * n], wr[b + int64_t(c) * nr]));
+            X[idx] = x;
+        }
+        __syncthreads();
+        for (int idx = tid; idx < cl * cc; idx += kThreads) {
+            const int d = idx % cl, c = idx / cl;
+            double2 y = cx<double2>(0, 0);
+            for (int a = 0; a < nl; ++a) {
+                double2 w = wl[a + int64_t(d) * nl];
+                w.y = -w.y;
+                y = cadd(y, cmul(w, X[a + c * nl]));
+            }
+            acc += y.x * y.x + y.y * y.y;
+        }
+        __syncthreads();
+    }
+    const double t = block_sum(acc, sv);
+    if (tid == 0) {
+        const int32_t st = pv.status[p * pv.status_stride + nd.slot];
+        pv.t_out[p] = st == kFrontOk ? t : NAN;
+    }
+}
+
 inline dim3 grid2(int32_t count, int32_t points) { return dim3(unsigned(count), unsigned(points)); }
 
 // Dynamic LDS above the 64 KB default (gfx950 has 160 KB per workgroup), once per kernel.
@@ -691,7 +954,7 @@ void allow_lds(const void* kernel) {
     static std::set<const void*> done;
     std::lock_guard<std::mutex> g(mu);
     if (done.insert(kernel).second)
-        (void)hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, 96 * 1024);
+        (void)hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, 128 * 1024);
 }
 
 }  // namespace
@@ -734,23 +997,29 @@ void launch_front_lds(const NodeDesc* nodes, const int32_t* level, int32_t count
                       int32_t points, double tol, bool single, bool assemble, hipStream_t st) {
     if (count <= 0 || points <= 0) return;
     if (nmax > kLdsMaxN) throw std::invalid_argument("front too large for the LDS kernel");
-    const size_t lds = size_t(nmax) * size_t(nmax) * (single ? sizeof(float2) : sizeof(double2));
-    const bool wide = nmax > 64;
+    const size_t lds = size_t(nmax) * size_t(nmax + 1) * (single ? sizeof(float2) : sizeof(double2));
+    // Lanes per row and workgroup size by the largest front: 16 rows x 4, 32 x 2 (one wave),
+    // 64 x 4, 128 x 2 (four waves).
     auto go = [&](auto kernel, int nt) {
         allow_lds(reinterpret_cast<const void*>(kernel));
         hipLaunchKernelGGL(kernel, grid2(count, points), dim3(nt), lds, st, nodes, level, ent, diag, kids, maps, pv, tol);
     };
-    if (single) {
-        if (assemble)
-            wide ? go(front_lds_kernel<float2, 128, true>, 128) : go(front_lds_kernel<float2, 64, true>, 64);
+    auto pick = [&](auto t, auto a) {
+        using T = decltype(t);
+        constexpr bool A = decltype(a)::value;
+        if (nmax <= 16)
+            go(front_lds_kernel<T, 64, 4, A>, 64);
+        else if (nmax <= 32)
+            go(front_lds_kernel<T, 64, 2, A>, 64);
+        else if (nmax <= 64)
+            go(front_lds_kernel<T, 256, 4, A>, 256);
         else
-            wide ? go(front_lds_kernel<float2, 128, false>, 128) : go(front_lds_kernel<float2, 64, false>, 64);
-    } else {
-        if (assemble)
-            wide ? go(front_lds_kernel<double2, 128, true>, 128) : go(front_lds_kernel<double2, 64, true>, 64);
-        else
-            wide ? go(front_lds_kernel<double2, 128, false>, 128) : go(front_lds_kernel<double2, 64, false>, 64);
-    }
+            go(front_lds_kernel<T, 256, 2, A>, 256);
+    };
+    if (single)
+        assemble ? pick(float2{}, std::true_type{}) : pick(float2{}, std::false_type{});
+    else
+        assemble ? pick(double2{}, std::true_type{}) : pick(double2{}, std::false_type{});
 }
 
 int32_t border_lds_rows(int32_t max_ne, bool single) {
@@ -758,15 +1027,25 @@ int32_t border_lds_rows(int32_t max_ne, bool single) {
     return max_ne <= cap ? max_ne : 0;
 }
 
-void launch_border(const NodeDesc* nodes, const int32_t* level, int32_t count, int32_t max_ne, const PointsView& pv,
-                   int32_t points, double tol, bool single, bool root, hipStream_t st) {
+void launch_gather(const NodeDesc* nodes, const Tile* tiles, int32_t count, const KidDesc* kids, const int32_t* maps,
+                   const PointsView& pv, int32_t points, bool single, hipStream_t st) {
+    if (count <= 0 || points <= 0) return;
+    if (single)
+        hipLaunchKernelGGL(gather_kernel<float2>, grid2(count, points), dim3(256), 0, st, nodes, tiles, kids, maps, pv);
+    else
+        hipLaunchKernelGGL(gather_kernel<double2>, grid2(count, points), dim3(256), 0, st, nodes, tiles, kids, maps, pv);
+}
+
+void launch_border(const NodeDesc* nodes, const int32_t* level, int32_t count, int32_t max_ne, const Entry* ent,
+                   const int32_t* diag, const PointsView& pv, int32_t points, double tol, bool single, bool root,
+                   hipStream_t st) {
     if (count <= 0 || points <= 0) return;
     const int32_t rows = border_lds_rows(max_ne, single);
-    const size_t lds = size_t(rows) * kPanel * (single ? sizeof(float2) : sizeof(double2));
+    const size_t lds = size_t(rows * kPanel + 2 * kPanel * kMoveCols) * (single ? sizeof(float2) : sizeof(double2));
     const dim3 g = grid2(count, points), b(kBorderThreads);
     auto go = [&](auto kernel) {
         allow_lds(reinterpret_cast<const void*>(kernel));
-        hipLaunchKernelGGL(kernel, g, b, lds, st, nodes, level, pv, tol, rows);
+        hipLaunchKernelGGL(kernel, g, b, lds, st, nodes, level, ent, diag, pv, tol, rows);
     };
     if (single)
         go(border_kernel<float2, false>);
@@ -776,16 +1055,17 @@ void launch_border(const NodeDesc* nodes, const int32_t* level, int32_t count, i
         go(border_kernel<double2, false>);
 }
 
-void launch_schur(const NodeDesc* nodes, const Tile* tiles, int32_t count, const PointsView& pv, int32_t points,
-                  bool single, bool root, hipStream_t st) {
+void launch_schur(const NodeDesc* nodes, const Tile* tiles, int32_t count, const KidDesc* kids, const int32_t* maps,
+                  const PointsView& pv, int32_t points, bool single, bool root, bool gather, hipStream_t st) {
     if (count <= 0 || points <= 0) return;
     const dim3 g = grid2(count, points), b(256);
-    if (single)
-        hipLaunchKernelGGL((schur_kernel<float2, false>), g, b, 0, st, nodes, tiles, pv);
-    else if (root)
-        hipLaunchKernelGGL((schur_kernel<double2, true>), g, b, 0, st, nodes, tiles, pv);
+    auto go = [&](auto kernel) { hipLaunchKernelGGL(kernel, g, b, 0, st, nodes, tiles, kids, maps, pv); };
+    if (root)
+        go(schur_kernel<double2, true, false>);
+    else if (single)
+        gather ? go(schur_kernel<float2, false, true>) : go(schur_kernel<float2, false, false>);
     else
-        hipLaunchKernelGGL((schur_kernel<double2, false>), g, b, 0, st, nodes, tiles, pv);
+        gather ? go(schur_kernel<double2, false, true>) : go(schur_kernel<double2, false, false>);
 }
 
 void launch_factor_unblocked(const NodeDesc* nodes, const int32_t* level, int32_t count, const PointsView& pv,
@@ -802,9 +1082,14 @@ void launch_factor_unblocked(const NodeDesc* nodes, const int32_t* level, int32_
                            level, pv, tol);
 }
 
-void launch_trace(const NodeDesc* root, const PointsView& pv, int32_t points, hipStream_t st) {
+void launch_trace(const NodeDesc* root, int32_t nl, const PointsView& pv, int32_t points, hipStream_t st) {
     if (points <= 0) return;
-    hipLaunchKernelGGL(trace_kernel, dim3(unsigned(points)), dim3(kThreads), 0, st, root, pv);
+    if (nl <= kTraceRows) {
+        allow_lds(reinterpret_cast<const void*>(trace_w_kernel));
+        hipLaunchKernelGGL(trace_w_kernel, dim3(unsigned(points)), dim3(kThreads), kTraceLds, st, root, pv);
+    } else {
+        hipLaunchKernelGGL(trace_kernel, dim3(unsigned(points)), dim3(kThreads), 0, st, root, pv);
+    }
 }
 
 }  // namespace caroli::gpu

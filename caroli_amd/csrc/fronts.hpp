@@ -39,10 +39,11 @@ struct Entry {
 };
 
 /// A child's Schur complement (size x size, leading dimension ld) at `src` in the kid arena,
-/// added at the front positions maps[map_beg + i].
+/// added at the front positions maps[map_beg + i]; maps[inv_beg + q] is the child's index of
+/// front position q (-1 when the child does not touch it), n entries.
 struct KidDesc {
     int64_t src;
-    int32_t ld, size, map_beg, pad;
+    int32_t ld, size, map_beg, inv_beg;
 };
 
 /// Per-point inputs and outputs of one launch.
@@ -51,6 +52,8 @@ struct PointsView {
     const double2* z;            // energy of each point
     const double2* sigma;        // Sigma_L (nl^2) then Sigma_R (nr^2) of each point
     const double2* gamma;        // Gamma_L then Gamma_R of each point
+    const double2* w;            // W_L (nl x cl, at 0) and W_R (nr x cr, at nl^2): Gamma = W W^H
+    const int32_t* wcols;        // cl, cr of each point
     int64_t sigma_stride;
     void* arena;                 // fronts of each point
     int64_t arena_stride;        // in complex words of the front type
@@ -80,6 +83,7 @@ constexpr int kLdsMaxN = 128;
 /// Panel width of the border factorisation, LDS budget of its panel, Schur tile size.
 constexpr int kPanel = 16;
 constexpr int kBorderLdsBytes = 64 * 1024;
+constexpr int kMoveCols = 64;  // columns per chunk of the border's row moves
 constexpr int kTile = 64;
 
 // Launchers (fronts.hip). `level` lists node indices into `nodes`; grid = count x points.
@@ -98,17 +102,25 @@ void launch_assemble_root(const NodeDesc* nodes, const int32_t* level, int32_t c
 void launch_front_lds(const NodeDesc* nodes, const int32_t* level, int32_t count, int32_t nmax, const Entry* ent,
                       const int32_t* diag, const KidDesc* kids, const int32_t* maps, const PointsView& pv,
                       int32_t points, double tol, bool single, bool assemble, hipStream_t st);
-/// Large fronts, assembled: the border LU (F11, L21, U12), then the Schur update over `tiles`.
-void launch_border(const NodeDesc* nodes, const int32_t* level, int32_t count, int32_t max_ne, const PointsView& pv,
-                   int32_t points, double tol, bool single, bool root, hipStream_t st);
-void launch_schur(const NodeDesc* nodes, const Tile* tiles, int32_t count, const PointsView& pv, int32_t points,
-                  bool single, bool root, hipStream_t st);
+/// Large fronts: the children's contributions to the border (rows or columns below ne) gathered
+/// over `tiles`; the border LU (F11, L21, U12; non-root fronts first add -H and z); the Schur
+/// update S = F22 - L21 U12 over `tiles`, F22 gathered from the children (gather) or read from the
+/// front.
+void launch_gather(const NodeDesc* nodes, const Tile* tiles, int32_t count, const KidDesc* kids, const int32_t* maps,
+                   const PointsView& pv, int32_t points, bool single, hipStream_t st);
+void launch_border(const NodeDesc* nodes, const int32_t* level, int32_t count, int32_t max_ne, const Entry* ent,
+                   const int32_t* diag, const PointsView& pv, int32_t points, double tol, bool single, bool root,
+                   hipStream_t st);
+void launch_schur(const NodeDesc* nodes, const Tile* tiles, int32_t count, const KidDesc* kids, const int32_t* maps,
+                  const PointsView& pv, int32_t points, bool single, bool root, bool gather, hipStream_t st);
 /// The whole elimination by rank-1 updates in global memory (CAROLI_FRONT_KERNEL=unblocked).
 void launch_factor_unblocked(const NodeDesc* nodes, const int32_t* level, int32_t count, const PointsView& pv,
                              int32_t points, double tol, bool single, bool root, hipStream_t st);
 bool unblocked_fronts();
-/// Caroli trace of the root border block: t_out[p] = Tr[Gamma_L G Gamma_R G^H].
-void launch_trace(const NodeDesc* root, const PointsView& pv, int32_t points, hipStream_t st);
+/// Caroli trace of the root border block: t_out[p] = Tr[Gamma_L G Gamma_R G^H], evaluated as
+/// |W_L^H G_LR W_R|_F^2 (nl <= kTraceRows), else from Gamma directly.
+constexpr int kTraceRows = 4096;
+void launch_trace(const NodeDesc* root, int32_t nl, const PointsView& pv, int32_t points, hipStream_t st);
 
 /// Largest ne * c of a front handled by the solve kernels (LDS words of complex f64).
 constexpr int kSolveLds = 4096;
