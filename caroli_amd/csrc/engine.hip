@@ -82,10 +82,15 @@ struct LevelSet {
     std::vector<int32_t> large;
     int32_t large_ne = 0;  // largest ne of the large fronts
     std::vector<Tile> tiles, gtiles;  // Schur tiles, border tiles (gather)
+    // Large fronts split by the border method: small separators (f11 + trsm), others (border).
+    std::vector<int32_t> f11, big;
+    int32_t f11_ne = 0, big_ne = 0;
+    std::vector<Tile> ttiles;  // trsm tiles of the f11 fronts
     // Algorithmic work (complex words, flops) for the kernel timers.
     double lds_flops[kClasses] = {0, 0, 0}, lds_words[kClasses] = {0, 0, 0}, lds_s_words[kClasses] = {0, 0, 0},
            lds_f_words[kClasses] = {0, 0, 0};
     double asm_words = 0, gather_words = 0, border_flops = 0, border_words = 0, schur_flops = 0, schur_words = 0;
+    double f11_flops = 0, f11_words = 0, trsm_flops = 0, trsm_words = 0;
 };
 
 /// Device pointers of a LevelSet's lists.
@@ -94,6 +99,9 @@ struct DevLevel {
     const int32_t* large = nullptr;
     const Tile* tiles = nullptr;
     const Tile* gtiles = nullptr;
+    const int32_t* f11 = nullptr;
+    const int32_t* big = nullptr;
+    const Tile* ttiles = nullptr;
 };
 
 /// Splits the fronts `list` of `nodes` by kernel (root: the bordered root front, always large).
@@ -101,7 +109,7 @@ LevelSet classify(const std::vector<NodeDesc>& nodes, const std::vector<KidDesc>
                   bool single, bool root) {
     LevelSet L;
     const int limit = single ? kLdsFrontC64 : kLdsFrontC128;
-    const bool unblocked = unblocked_fronts();
+    const bool unblocked = unblocked_fronts(), split = !border_only();
     for (int32_t q : list) {
         const NodeDesc& d = nodes[q];
         const double ne = d.ne, nb = d.n - d.ne, n2 = double(d.n) * d.n;
@@ -130,8 +138,23 @@ LevelSet classify(const std::vector<NodeDesc>& nodes, const std::vector<KidDesc>
         // tiles) in proportion to the areas.
         const double border_share = (n2 - nb * nb) / std::max(n2, 1.0);
         L.gather_words += (n2 - nb * nb) + border_share * kid_words;
-        L.border_flops += 8.0 / 3.0 * ne * ne * ne + 8.0 * ne * ne * nb;
-        L.border_words += 2.0 * (n2 - nb * nb);
+        if (!root && split && d.ne <= kF11Lds) {
+            L.f11.push_back(q);
+            L.f11_ne = std::max(L.f11_ne, d.ne);
+            L.f11_flops += 8.0 / 3.0 * ne * ne * ne;
+            L.f11_words += 2.0 * ne * ne;
+            L.trsm_flops += 8.0 * ne * ne * nb;
+            L.trsm_words += 4.0 * ne * nb + 2.0 * ne * ne * std::ceil(nb / kTrsmSpan);
+            for (int o = 0; o < d.n - d.ne; o += kTrsmSpan) {
+                L.ttiles.push_back({q, 0, o, 0});
+                L.ttiles.push_back({q, o, 0, 1});
+            }
+        } else {
+            L.big.push_back(q);
+            L.big_ne = std::max(L.big_ne, d.ne);
+            L.border_flops += 8.0 / 3.0 * ne * ne * ne + 8.0 * ne * ne * nb;
+            L.border_words += 2.0 * (n2 - nb * nb);
+        }
         L.schur_flops += 8.0 * ne * nb * nb;
         L.schur_words += nb * nb + 2.0 * ne * nb + (root ? nb * nb : (1.0 - border_share) * kid_words);
         for (int i0 = 0; i0 < d.n - d.ne; i0 += kTile)
@@ -146,7 +169,7 @@ LevelSet classify(const std::vector<NodeDesc>& nodes, const std::vector<KidDesc>
 
 /// Appends a LevelSet's lists to the device list / tile arrays; `at` records the offsets.
 struct SetOffsets {
-    int32_t lds[LevelSet::kClasses], large, tiles, gtiles;
+    int32_t lds[LevelSet::kClasses], large, tiles, gtiles, f11, big, ttiles;
 };
 SetOffsets append_set(const LevelSet& L, std::vector<int32_t>& lv, std::vector<Tile>& tl) {
     SetOffsets o{};
@@ -160,6 +183,12 @@ SetOffsets append_set(const LevelSet& L, std::vector<int32_t>& lv, std::vector<T
     tl.insert(tl.end(), L.tiles.begin(), L.tiles.end());
     o.gtiles = int32_t(tl.size());
     tl.insert(tl.end(), L.gtiles.begin(), L.gtiles.end());
+    o.f11 = int32_t(lv.size());
+    lv.insert(lv.end(), L.f11.begin(), L.f11.end());
+    o.big = int32_t(lv.size());
+    lv.insert(lv.end(), L.big.begin(), L.big.end());
+    o.ttiles = int32_t(tl.size());
+    tl.insert(tl.end(), L.ttiles.begin(), L.ttiles.end());
     return o;
 }
 DevLevel resolve(const SetOffsets& o, const DevBuf& lv, const DevBuf& tl) {
@@ -168,6 +197,9 @@ DevLevel resolve(const SetOffsets& o, const DevBuf& lv, const DevBuf& tl) {
     d.large = lv.as<int32_t>() + o.large;
     d.tiles = tl.as<Tile>() + o.tiles;
     d.gtiles = tl.as<Tile>() + o.gtiles;
+    d.f11 = lv.as<int32_t>() + o.f11;
+    d.big = lv.as<int32_t>() + o.big;
+    d.ttiles = tl.as<Tile>() + o.ttiles;
     return d;
 }
 
@@ -393,6 +425,8 @@ enum Family {
     kFrontLds,
     kGather,
     kAssemble,
+    kF11,
+    kTrsm,
     kBorder,
     kSchur,
     kRootAssemble,
@@ -401,9 +435,10 @@ enum Family {
     kTrace,
     kFamilies
 };
-const char* const kFamilyNames[kFamilies] = {"front_lds_kernel",      "gather_kernel",       "assemble_kernel",
-                                             "border_kernel",         "schur_kernel",        "assemble_kernel<root>",
-                                             "border_kernel<root>",   "schur_kernel<root>",  "trace_kernel"};
+const char* const kFamilyNames[kFamilies] = {"front_lds_kernel",    "gather_kernel",         "assemble_kernel",
+                                             "f11_kernel",          "trsm_kernel",           "border_kernel",
+                                             "schur_kernel",        "assemble_kernel<root>", "border_kernel<root>",
+                                             "schur_kernel<root>",  "trace_kernel"};
 
 struct Engine {
     std::mutex mu;
@@ -596,9 +631,19 @@ void run_level(Engine& eng, const LevelSet& L, const DevLevel& D, const NodeDesc
         });
         return;
     }
-    eng.timed(root ? kRootBorder : kBorder, P * L.border_flops, P * fes * L.border_words, [&] {
-        launch_border(nodes, D.large, count, L.large_ne, ent, diag, pv, P, tol, fsingle, root, eng.stream);
-    });
+    if (!L.f11.empty()) {
+        eng.timed(kF11, P * L.f11_flops, P * fes * L.f11_words, [&] {
+            launch_f11(nodes, D.f11, int32_t(L.f11.size()), L.f11_ne, ent, diag, pv, P, tol, fsingle, eng.stream);
+        });
+        eng.timed(kTrsm, P * L.trsm_flops, P * fes * L.trsm_words, [&] {
+            launch_trsm(nodes, D.ttiles, int32_t(L.ttiles.size()), L.f11_ne, pv, P, fsingle, eng.stream);
+        });
+    }
+    if (!L.big.empty())
+        eng.timed(root ? kRootBorder : kBorder, P * L.border_flops, P * fes * L.border_words, [&] {
+            launch_border(nodes, D.big, int32_t(L.big.size()), L.big_ne, ent, diag, pv, P, tol, fsingle, root,
+                          eng.stream);
+        });
     eng.timed(root ? kRootSchur : kSchur, P * L.schur_flops, P * fes * L.schur_words, [&] {
         launch_schur(nodes, D.tiles, int32_t(L.tiles.size()), kids, maps, pv, P, fsingle, root, !root, eng.stream);
     });
@@ -1148,7 +1193,7 @@ int32_t front_schur(const double* f, int64_t n, int64_t ne, int64_t ne1, int32_t
     d.ne1 = int32_t(ne1);
     d.slot = 0;
     d.piv_off = 0;
-    DevBuf fb, nodes, lv, pivb, stb, tiles;
+    DevBuf fb, nodes, lv, pivb, stb, tiles, ttiles;
     const size_t nn = size_t(n) * size_t(n);
     if (single) {
         std::vector<float2> h(nn);
@@ -1177,12 +1222,24 @@ int32_t front_schur(const double* f, int64_t n, int64_t ne, int64_t ne1, int32_t
     } else if (unblocked_fronts()) {
         launch_factor_unblocked(nodes.as<NodeDesc>(), lv.as<int32_t>(), 1, pv, 1, tol, single, false, eng.stream);
     } else {
-        std::vector<Tile> tl;
+        std::vector<Tile> tl, tt;
         for (int i0 = 0; i0 < n - ne; i0 += kTile)
             for (int j0 = 0; j0 < n - ne; j0 += kTile) tl.push_back({0, i0, j0, 0});
         upload(tiles, tl, eng.stream);
-        launch_border(nodes.as<NodeDesc>(), lv.as<int32_t>(), 1, int32_t(ne), nullptr, nullptr, pv, 1, tol, single, false,
-                      eng.stream);
+        if (ne <= kF11Lds && !border_only()) {
+            for (int o = 0; o < n - ne; o += kTrsmSpan) {
+                tt.push_back({0, 0, o, 0});
+                tt.push_back({0, o, 0, 1});
+            }
+            upload(ttiles, tt, eng.stream);
+            launch_f11(nodes.as<NodeDesc>(), lv.as<int32_t>(), 1, int32_t(ne), nullptr, nullptr, pv, 1, tol, single,
+                       eng.stream);
+            launch_trsm(nodes.as<NodeDesc>(), ttiles.as<Tile>(), int32_t(tt.size()), int32_t(ne), pv, 1, single,
+                        eng.stream);
+        } else {
+            launch_border(nodes.as<NodeDesc>(), lv.as<int32_t>(), 1, int32_t(ne), nullptr, nullptr, pv, 1, tol, single,
+                          false, eng.stream);
+        }
         launch_schur(nodes.as<NodeDesc>(), tiles.as<Tile>(), int32_t(tl.size()), nullptr, nullptr, pv, 1, single, false,
                      false, eng.stream);
     }

@@ -300,63 +300,22 @@ struct Mfma<float2> {
 
 
 
-// ---------------------------------------------------------------------------------------------
-// Small fronts: assembly and elimination in LDS, one workgroup of NT lanes per (front, point).
-//
-// The front is assembled straight into LDS (no HBM copy of the assembled front; leading
-// dimension n + 1 so that lanes on neighbouring columns hit different banks), eliminated there,
-// and written back once: the whole factored front when the factors are kept for the solves
-// (pv.keep), else only the Schur block the parent reads. G lanes share a row: lane t owns
-// physical row t / G and the columns j = t (mod G) of it for the whole elimination (the G lanes
-// of a row sit in one wavefront). A row interchange only swaps the logical positions two rows
-// track, and the rows are written back in logical order, which leaves the factors of
-// interchanging whole rows (LAPACK's convention, the one the solves use).
-template <class T, int NT, int G, bool kAssemble>
-__global__ void __launch_bounds__(NT)
-    front_lds_kernel(const NodeDesc* __restrict__ nodes, const int32_t* __restrict__ level,
-                     const Entry* __restrict__ ent, const int32_t* __restrict__ diag,
-                     const KidDesc* __restrict__ kids, const int32_t* __restrict__ maps, PointsView pv, double tol) {
-    extern __shared__ __align__(16) unsigned char lds_raw[];
+// Partial LU in LDS of the n x n matrix W (leading dimension ld) over its first ne columns, row
+// pivoting restricted to the separator rows and the pivot test of field.rs:61 on columns
+// [0, ne1) (later columns only for zero). G lanes share a row: lane t owns physical row t / G and
+// the columns j = t (mod G); lg is the row's logical position on entry (its physical row) and on
+// return. Interchanges only swap logical positions; LAPACK's pivots go to piv. Returns false (and
+// sets the status) when a pivot fails; the whole workgroup returns alike.
+template <class T, int NT, int G>
+__device__ __forceinline__ bool lds_lu(T* W, int ld, int n, int ne, int ne1, double tol, int32_t* piv,
+                                       int32_t* status, int& lg) {
     __shared__ double sv[NT / 64];
     __shared__ double rv[2][NT / 64];
     __shared__ int32_t ri[2][NT / 64], rp[2][NT / 64];
-    T* W = reinterpret_cast<T*>(lds_raw);
-    const NodeDesc nd = nodes[level[blockIdx.x]];
-    const int64_t p = blockIdx.y;
-    T* F = reinterpret_cast<T*>(pv.arena) + p * pv.arena_stride + nd.off;
-    int32_t* piv = pv.piv + p * pv.piv_stride + nd.piv_off;
-    int32_t* status = pv.status + p * pv.status_stride + nd.slot;
-    const int n = nd.n, ne = nd.ne, ne1 = nd.ne1, tid = threadIdx.x, lane = tid & 63, ld = n + 1;
-    const int nw = n * ld;
-    if (kAssemble) {
-        const T zero = cx<T>(0.0, 0.0);
-        for (int i = tid; i < nw; i += NT) W[i] = zero;
-        __syncthreads();
-        add_entries<NT>(ent, nd.ent_beg, nd.ent_end, pv.vals[p], ld,
-                        [&](int at, double2 v) { W[at] = cadd(W[at], convert<T>(v)); });
-        __syncthreads();
-        const T z = convert<T>(pv.z[p]);
-        for (int d = nd.diag_beg + tid; d < nd.diag_end; d += NT) {
-            const int q = diag[d];
-            W[q + q * ld] = cadd(W[q + q * ld], z);
-        }
-        __syncthreads();
-        const T* kidbase = reinterpret_cast<const T*>(pv.arena) + p * pv.arena_stride;
-        for (int c = nd.kid_beg; c < nd.kid_end; ++c) {
-            const KidDesc kd = kids[c];
-            extend_add<NT>(kidbase + kd.src, kd.ld, maps + kd.map_beg, kd.size, ld,
-                           [&](int at, T v) { W[at] = cadd(W[at], v); });
-            __syncthreads();
-        }
-    } else {
-        for (int idx = tid; idx < n * n; idx += NT) W[(idx % n) + (idx / n) * ld] = F[idx];
-        __syncthreads();
-    }
+    const int tid = threadIdx.x, lane = tid & 63, g = tid % G, r = tid / G;
     double m = 0.0;
     for (int idx = tid; idx < ne1 * ne1; idx += NT) m = fmax(m, cabs(W[(idx % ne1) + (idx / ne1) * ld]));
     const double scale = block_max(m, sv);
-    const int g = tid % G, r = tid / G;  // this lane's column class and physical row
-    int lg = r;                           // the row's logical position
     for (int k = 0; k < ne; ++k) {
         const int lim = k < ne1 ? ne1 : ne;
         double big = -1.0;
@@ -388,7 +347,7 @@ __global__ void __launch_bounds__(NT)
         const bool ok = k < ne1 ? (big > tol * scale) : (big > 0.0 && big < INFINITY);
         if (!ok) {
             if (tid == 0) *status = k < ne1 ? kFrontDelayed : kFrontSingular;
-            return;
+            return false;
         }
         if (tid == 0) piv[k] = at;
         if (r == ph)
@@ -405,6 +364,61 @@ __global__ void __launch_bounds__(NT)
         __syncthreads();
     }
     if (tid == 0) *status = kFrontOk;
+    return true;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Small fronts: assembly and elimination in LDS, one workgroup of NT lanes per (front, point).
+//
+// The front is assembled straight into LDS (no HBM copy of the assembled front; leading
+// dimension n + 1 so that lanes on neighbouring columns hit different banks), eliminated there,
+// and written back once: the whole factored front when the factors are kept for the solves
+// (pv.keep), else only the Schur block the parent reads. G lanes share a row: lane t owns
+// physical row t / G and the columns j = t (mod G) of it for the whole elimination (the G lanes
+// of a row sit in one wavefront). A row interchange only swaps the logical positions two rows
+// track, and the rows are written back in logical order, which leaves the factors of
+// interchanging whole rows (LAPACK's convention, the one the solves use).
+template <class T, int NT, int G, bool kAssemble>
+__global__ void __launch_bounds__(NT)
+    front_lds_kernel(const NodeDesc* __restrict__ nodes, const int32_t* __restrict__ level,
+                     const Entry* __restrict__ ent, const int32_t* __restrict__ diag,
+                     const KidDesc* __restrict__ kids, const int32_t* __restrict__ maps, PointsView pv, double tol) {
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    T* W = reinterpret_cast<T*>(lds_raw);
+    const NodeDesc nd = nodes[level[blockIdx.x]];
+    const int64_t p = blockIdx.y;
+    T* F = reinterpret_cast<T*>(pv.arena) + p * pv.arena_stride + nd.off;
+    int32_t* piv = pv.piv + p * pv.piv_stride + nd.piv_off;
+    int32_t* status = pv.status + p * pv.status_stride + nd.slot;
+    const int n = nd.n, ne = nd.ne, ne1 = nd.ne1, tid = threadIdx.x, ld = n + 1;
+    const int nw = n * ld;
+    if (kAssemble) {
+        const T zero = cx<T>(0.0, 0.0);
+        for (int i = tid; i < nw; i += NT) W[i] = zero;
+        __syncthreads();
+        add_entries<NT>(ent, nd.ent_beg, nd.ent_end, pv.vals[p], ld,
+                        [&](int at, double2 v) { W[at] = cadd(W[at], convert<T>(v)); });
+        __syncthreads();
+        const T z = convert<T>(pv.z[p]);
+        for (int d = nd.diag_beg + tid; d < nd.diag_end; d += NT) {
+            const int q = diag[d];
+            W[q + q * ld] = cadd(W[q + q * ld], z);
+        }
+        __syncthreads();
+        const T* kidbase = reinterpret_cast<const T*>(pv.arena) + p * pv.arena_stride;
+        for (int c = nd.kid_beg; c < nd.kid_end; ++c) {
+            const KidDesc kd = kids[c];
+            extend_add<NT>(kidbase + kd.src, kd.ld, maps + kd.map_beg, kd.size, ld,
+                           [&](int at, T v) { W[at] = cadd(W[at], v); });
+            __syncthreads();
+        }
+    } else {
+        for (int idx = tid; idx < n * n; idx += NT) W[(idx % n) + (idx / n) * ld] = F[idx];
+        __syncthreads();
+    }
+    const int g = tid % G, r = tid / G;  // this lane's column class and physical row
+    int lg = r;                           // the row's logical position
+    if (!lds_lu<T, NT, G>(W, ld, n, ne, ne1, tol, piv, status, lg)) return;
     // Row r goes to logical row lg: lanes of one wave cover G columns of 64 / G rows.
     if (r < n) {
         if (pv.keep) {
@@ -736,6 +750,136 @@ __global__ void __launch_bounds__(256)
 }
 
 // ---------------------------------------------------------------------------------------------
+// Large fronts with a small separator (ne <= kF11Lds): the border in two launches instead of
+// border_kernel's panel loop.
+//  - f11_kernel, one workgroup per front: -H and z added to the gathered border, then the LU of
+//    F11 in LDS (lds_lu) written back with its pivots;
+//  - trsm_kernel, one workgroup per kTrsmSpan columns of F12 (U12 = L11^-1 P F12) or rows of F21
+//    (L21 = F21 U11^-1): the triangle in LDS, each column / row in the registers of 8 lanes.
+template <class T, int G>
+__global__ void __launch_bounds__(256)
+    f11_kernel(const NodeDesc* __restrict__ nodes, const int32_t* __restrict__ level, const Entry* __restrict__ ent,
+               const int32_t* __restrict__ diag, PointsView pv, double tol) {
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    T* W = reinterpret_cast<T*>(lds_raw);
+    const NodeDesc nd = nodes[level[blockIdx.x]];
+    const int64_t p = blockIdx.y;
+    T* F = reinterpret_cast<T*>(pv.arena) + p * pv.arena_stride + nd.off;
+    int32_t* piv = pv.piv + p * pv.piv_stride + nd.piv_off;
+    int32_t* status = pv.status + p * pv.status_stride + nd.slot;
+    const int n = nd.n, ne = nd.ne, ne1 = nd.ne1, tid = threadIdx.x, ld = ne + 1;
+    if (nd.ent_end > nd.ent_beg) {
+        add_entries<256>(ent, nd.ent_beg, nd.ent_end, pv.vals[p], n,
+                         [&](int at, double2 v) { F[at] = cadd(F[at], convert<T>(v)); });
+        __syncthreads();
+    }
+    if (nd.diag_end > nd.diag_beg) {
+        const T z = convert<T>(pv.z[p]);
+        for (int d = nd.diag_beg + tid; d < nd.diag_end; d += 256) {
+            const int64_t q = diag[d];
+            F[q + q * n] = cadd(F[q + q * n], z);
+        }
+        __syncthreads();
+    }
+    for (int idx = tid; idx < ne * ne; idx += 256) W[(idx % ne) + (idx / ne) * ld] = F[(idx % ne) + int64_t(idx / ne) * n];
+    __syncthreads();
+    const int g = tid % G, r = tid / G;
+    int lg = r;
+    if (!lds_lu<T, 256, G>(W, ld, ne, ne, ne1, tol, piv, status, lg)) return;
+    if (r < ne)
+        for (int j = g; j < ne; j += G) F[lg + int64_t(j) * n] = W[r + j * ld];
+}
+
+
+
+template <class T>
+__global__ void __launch_bounds__(256)
+    trsm_kernel(const NodeDesc* __restrict__ nodes, const Tile* __restrict__ tiles, PointsView pv) {
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    __shared__ int32_t src[kLdsMaxN];
+    const Tile tl = tiles[blockIdx.x];
+    const NodeDesc nd = nodes[tl.node];
+    const int64_t p = blockIdx.y;
+    if (pv.status[p * pv.status_stride + nd.slot] != kFrontOk) return;
+    T* F = reinterpret_cast<T*>(pv.arena) + p * pv.arena_stride + nd.off;
+    const int32_t* piv = pv.piv + p * pv.piv_stride + nd.piv_off;
+    const int n = nd.n, ne = nd.ne, nb = n - ne, tid = threadIdx.x, la = ne + 1;
+    T* A = reinterpret_cast<T*>(lds_raw);  // L11 (kind 0) or U11 (kind 1), leading dimension la
+    const bool upper = tl.pad != 0;
+    if (!upper && tid == 0) {  // the rows of P F12: row i is the front's row src[i]
+        for (int i = 0; i < ne; ++i) src[i] = i;
+        for (int k = 0; k < ne; ++k) {
+            const int t = src[k];
+            src[k] = src[piv[k]];
+            src[piv[k]] = t;
+        }
+    }
+    for (int idx = tid; idx < ne * ne; idx += 256) A[(idx % ne) + (idx / ne) * la] = F[(idx % ne) + int64_t(idx / ne) * n];
+    __syncthreads();
+    // Eight lanes per column of U12 (kind 0) or row of L21 (kind 1): lane g holds entries
+    // g + 8 s (s < 8) of it in registers; the solve runs with shuffles inside the 8 lanes.
+    const int grp = tid / 8, g = tid % 8;
+    const int start = upper ? tl.i0 : tl.j0, end = min(start + kTrsmSpan, nb);
+    for (int c0 = start; c0 < end; c0 += 32) {
+        const int col = min(c0 + grp, end - 1);
+        T x[8];
+#pragma unroll
+        for (int s = 0; s < 8; ++s) {
+            const int e = min(g + 8 * s, ne - 1);
+            x[s] = upper ? F[(ne + col) + int64_t(e) * n] : F[src[e] + int64_t(ne + col) * n];
+        }
+        if (!upper) {  // forward substitution with the unit-lower L11
+#pragma unroll
+            for (int kb = 0; kb < 8; ++kb)
+#pragma unroll
+                for (int kk = 0; kk < 8; ++kk) {
+                    const int k = 8 * kb + kk;
+                    if (k < ne - 1) {
+                        T xk;
+                        xk.x = __shfl(x[kb].x, kk, 8);
+                        xk.y = __shfl(x[kb].y, kk, 8);
+#pragma unroll
+                        for (int s = 0; s < 8; ++s) {
+                            const int i = g + 8 * s;
+                            if (i > k && i < ne) x[s] = csub(x[s], cmul(A[i + k * la], xk));
+                        }
+                    }
+                }
+        } else {  // x U11 = f, column by column
+#pragma unroll
+            for (int kb = 0; kb < 8; ++kb)
+#pragma unroll
+                for (int kk = 0; kk < 8; ++kk) {
+                    const int k = 8 * kb + kk;
+                    if (k < ne) {
+                        const T cand = cmul(x[kb], crecip(A[k + k * la]));
+                        T xk;
+                        xk.x = __shfl(cand.x, kk, 8);
+                        xk.y = __shfl(cand.y, kk, 8);
+                        if (g == kk) x[kb] = xk;
+#pragma unroll
+                        for (int s = 0; s < 8; ++s) {
+                            const int j = g + 8 * s;
+                            if (j > k && j < ne) x[s] = csub(x[s], cmul(xk, A[k + j * la]));
+                        }
+                    }
+                }
+        }
+        if (c0 + grp < end)
+#pragma unroll
+            for (int s = 0; s < 8; ++s) {
+                const int e = g + 8 * s;
+                if (e < ne) {
+                    if (upper)
+                        F[(ne + col) + int64_t(e) * n] = x[s];
+                    else
+                        F[e + int64_t(ne + col) * n] = x[s];
+                }
+            }
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
 // Large fronts, step 2: S = F22 - L21 U12 (K = ne) over 64 x 64 tiles of the Schur block, one
 // workgroup per (tile, point), on MFMA. Each wave owns a 32 x 32 quarter (2 x 2 blocks of 16 x 16,
 // complex = 4 real MFMAs per block and K-step of 4); L21 and U12 are staged through LDS in
@@ -992,6 +1136,11 @@ bool unblocked_fronts() {
     return v && std::strcmp(v, "unblocked") == 0;
 }
 
+bool border_only() {
+    const char* v = std::getenv("CAROLI_FRONT_KERNEL");
+    return v && std::strcmp(v, "border") == 0;
+}
+
 void launch_front_lds(const NodeDesc* nodes, const int32_t* level, int32_t count, int32_t nmax, const Entry* ent,
                       const int32_t* diag, const KidDesc* kids, const int32_t* maps, const PointsView& pv,
                       int32_t points, double tol, bool single, bool assemble, hipStream_t st) {
@@ -1053,6 +1202,33 @@ void launch_border(const NodeDesc* nodes, const int32_t* level, int32_t count, i
         go(border_kernel<double2, true>);
     else
         go(border_kernel<double2, false>);
+}
+
+void launch_f11(const NodeDesc* nodes, const int32_t* level, int32_t count, int32_t max_ne, const Entry* ent,
+                const int32_t* diag, const PointsView& pv, int32_t points, double tol, bool single, hipStream_t st) {
+    if (count <= 0 || points <= 0) return;
+    if (max_ne > kLdsMaxN) throw std::invalid_argument("separator too large for f11_kernel");
+    const size_t lds = size_t(max_ne) * size_t(max_ne + 1) * (single ? sizeof(float2) : sizeof(double2));
+    auto go = [&](auto kernel) {
+        allow_lds(reinterpret_cast<const void*>(kernel));
+        hipLaunchKernelGGL(kernel, grid2(count, points), dim3(256), lds, st, nodes, level, ent, diag, pv, tol);
+    };
+    if (single)
+        max_ne <= 64 ? go(f11_kernel<float2, 4>) : go(f11_kernel<float2, 2>);
+    else
+        max_ne <= 64 ? go(f11_kernel<double2, 4>) : go(f11_kernel<double2, 2>);
+}
+
+void launch_trsm(const NodeDesc* nodes, const Tile* tiles, int32_t count, int32_t max_ne, const PointsView& pv,
+                 int32_t points, bool single, hipStream_t st) {
+    if (count <= 0 || points <= 0) return;
+    if (max_ne > kF11Lds) throw std::invalid_argument("separator too large for trsm_kernel");
+    const size_t lds = size_t(max_ne) * size_t(max_ne + 1) * (single ? sizeof(float2) : sizeof(double2));
+    auto go = [&](auto kernel) {
+        allow_lds(reinterpret_cast<const void*>(kernel));
+        hipLaunchKernelGGL(kernel, grid2(count, points), dim3(256), lds, st, nodes, tiles, pv);
+    };
+    single ? go(trsm_kernel<float2>) : go(trsm_kernel<double2>);
 }
 
 void launch_schur(const NodeDesc* nodes, const Tile* tiles, int32_t count, const KidDesc* kids, const int32_t* maps,

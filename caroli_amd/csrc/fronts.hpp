@@ -84,6 +84,10 @@ constexpr int kLdsMaxN = 128;
 constexpr int kPanel = 16;
 constexpr int kBorderLdsBytes = 64 * 1024;
 constexpr int kMoveCols = 64;  // columns per chunk of the border's row moves
+/// Separators up to kF11Lds variables are factored in LDS (f11_kernel) and their border strips
+/// solved kTrsmSpan columns / rows per workgroup (trsm_kernel).
+constexpr int kF11Lds = 64;
+constexpr int kTrsmSpan = 64;
 constexpr int kTile = 64;
 
 // Launchers (fronts.hip). `level` lists node indices into `nodes`; grid = count x points.
@@ -113,6 +117,15 @@ void launch_border(const NodeDesc* nodes, const int32_t* level, int32_t count, i
                    hipStream_t st);
 void launch_schur(const NodeDesc* nodes, const Tile* tiles, int32_t count, const KidDesc* kids, const int32_t* maps,
                   const PointsView& pv, int32_t points, bool single, bool root, bool gather, hipStream_t st);
+/// Large fronts with ne <= kF11Lds: the LU of F11 in LDS (after -H and z), then the triangular
+/// solves over `tiles` (pad 0: kTrsmSpan columns of U12 from j0; pad 1: kTrsmSpan rows of L21
+/// from i0).
+void launch_f11(const NodeDesc* nodes, const int32_t* level, int32_t count, int32_t max_ne, const Entry* ent,
+                const int32_t* diag, const PointsView& pv, int32_t points, double tol, bool single, hipStream_t st);
+void launch_trsm(const NodeDesc* nodes, const Tile* tiles, int32_t count, int32_t max_ne, const PointsView& pv,
+                 int32_t points, bool single, hipStream_t st);
+/// CAROLI_FRONT_KERNEL=border: every large front through border_kernel (no f11/trsm split).
+bool border_only();
 /// The whole elimination by rank-1 updates in global memory (CAROLI_FRONT_KERNEL=unblocked).
 void launch_factor_unblocked(const NodeDesc* nodes, const int32_t* level, int32_t count, const PointsView& pv,
                              int32_t points, double tol, bool single, bool root, hipStream_t st);

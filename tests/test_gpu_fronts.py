@@ -1,8 +1,10 @@
 """GPU front update (Field::schur, crates/caroli-dense/src/field.rs:44-105) against numpy.
 
-S = F22 - F21 F11^-1 F12 for random fronts of the sizes the trees produce: LDS-resident fronts
-(n <= 64) and the blocked MFMA kernel (larger), both kernels of large fronts, complex f64 to a
-relative 1e-11 and complex f32 to 1e-4; a singular leading block is refused (field.rs:199-208).
+S = F22 - F21 F11^-1 F12 for random fronts of the sizes the trees produce, through every kernel
+path: LDS-resident fronts (n <= 64), the separator LU in LDS with the triangular-solve tiles
+(ne <= 64), the panel-by-panel border kernel, and the unblocked fallback; then the MFMA Schur
+tiles. Complex f64 to a relative 1e-11, complex f32 to 1e-4; a singular leading block is refused
+(field.rs:199-208).
 """
 import ctypes as C
 import os
@@ -40,7 +42,7 @@ def numpy_schur(f, ne):
     return f[ne:, ne:] - f[ne:, :ne] @ np.linalg.solve(f[:ne, :ne], f[:ne, ne:])
 
 
-@pytest.mark.parametrize("kernel", ["blocked", "unblocked"])
+@pytest.mark.parametrize("kernel", ["default", "border", "unblocked"])
 @pytest.mark.parametrize("n,ne,ne1", SIZES)
 def test_front_schur_double(n, ne, ne1, kernel, monkeypatch):
     monkeypatch.setenv("CAROLI_FRONT_KERNEL", kernel)
@@ -64,7 +66,7 @@ def test_front_schur_single(n, ne):
     assert np.abs(s - ref).max() <= 1e-4 * np.abs(ref).max()
 
 
-@pytest.mark.parametrize("n,ne", [(10, 4), (120, 70)])
+@pytest.mark.parametrize("n,ne", [(10, 4), (120, 70), (150, 40)])
 @pytest.mark.parametrize("precision", [0, 1])
 def test_singular_block_is_refused(n, ne, precision):
     f = front(n, 3)
