@@ -102,6 +102,22 @@ __device__ __forceinline__ void wave_argmax(double& v, int& idx) {
 // end are clamped for the loads; only the updates are guarded.
 constexpr int kBatch = 8;
 
+// Workgroups of a launch go round-robin to the 8 XCDs (each with its own L2): linear id l runs
+// on XCD l % 8. Tile kernels renumber their (tile, point) pairs so that each XCD receives one
+// contiguous range, and neighbouring tiles of one front and point share an L2.
+constexpr int kXcds = 8;
+struct TilePoint {
+    int tile;
+    int64_t point;
+};
+__device__ __forceinline__ TilePoint xcd_tile() {
+    const uint64_t T = gridDim.x, total = T * gridDim.y;
+    const uint64_t lin = blockIdx.x + uint64_t(blockIdx.y) * T;
+    const uint64_t q = total / kXcds, r = total % kXcds, x = lin % kXcds, idx = lin / kXcds;
+    const uint64_t logical = (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + idx;
+    return {int(logical % T), int64_t(logical / T)};
+}
+
 // The H entries of a front: add(at, v) with at = a + b n and v = -H (conjugated as listed).
 template <int NT, class Add>
 __device__ __forceinline__ void add_entries(const Entry* ent, int beg, int end, const double2* vals, int n, Add add) {
@@ -708,9 +724,10 @@ template <class T>
 __global__ void __launch_bounds__(256)
     gather_kernel(const NodeDesc* __restrict__ nodes, const Tile* __restrict__ tiles, const KidDesc* __restrict__ kids,
                   const int32_t* __restrict__ maps, PointsView pv) {
-    const Tile tl = tiles[blockIdx.x];
+    const TilePoint tp = xcd_tile();
+    const Tile tl = tiles[tp.tile];
     const NodeDesc nd = nodes[tl.node];
-    const int64_t p = blockIdx.y;
+    const int64_t p = tp.point;
     const T* kidbase = reinterpret_cast<const T*>(pv.arena) + p * pv.arena_stride;
     T* F = reinterpret_cast<T*>(pv.arena) + p * pv.arena_stride + nd.off;
     const int n = nd.n, ne = nd.ne;
@@ -792,14 +809,15 @@ __global__ void __launch_bounds__(256)
 
 
 
-template <class T>
+template <class T, int NS>
 __global__ void __launch_bounds__(256)
     trsm_kernel(const NodeDesc* __restrict__ nodes, const Tile* __restrict__ tiles, PointsView pv) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     __shared__ int32_t src[kLdsMaxN];
-    const Tile tl = tiles[blockIdx.x];
+    const TilePoint tp = xcd_tile();
+    const Tile tl = tiles[tp.tile];
     const NodeDesc nd = nodes[tl.node];
-    const int64_t p = blockIdx.y;
+    const int64_t p = tp.point;
     if (pv.status[p * pv.status_stride + nd.slot] != kFrontOk) return;
     T* F = reinterpret_cast<T*>(pv.arena) + p * pv.arena_stride + nd.off;
     const int32_t* piv = pv.piv + p * pv.piv_stride + nd.piv_off;
@@ -817,20 +835,21 @@ __global__ void __launch_bounds__(256)
     for (int idx = tid; idx < ne * ne; idx += 256) A[(idx % ne) + (idx / ne) * la] = F[(idx % ne) + int64_t(idx / ne) * n];
     __syncthreads();
     // Eight lanes per column of U12 (kind 0) or row of L21 (kind 1): lane g holds entries
-    // g + 8 s (s < 8) of it in registers; the solve runs with shuffles inside the 8 lanes.
+    // g + 8 s (s < NS, ne <= 8 NS) of it in registers; the solve runs with shuffles inside the 8
+    // lanes.
     const int grp = tid / 8, g = tid % 8;
     const int start = upper ? tl.i0 : tl.j0, end = min(start + kTrsmSpan, nb);
     for (int c0 = start; c0 < end; c0 += 32) {
         const int col = min(c0 + grp, end - 1);
-        T x[8];
+        T x[NS];
 #pragma unroll
-        for (int s = 0; s < 8; ++s) {
+        for (int s = 0; s < NS; ++s) {
             const int e = min(g + 8 * s, ne - 1);
             x[s] = upper ? F[(ne + col) + int64_t(e) * n] : F[src[e] + int64_t(ne + col) * n];
         }
         if (!upper) {  // forward substitution with the unit-lower L11
 #pragma unroll
-            for (int kb = 0; kb < 8; ++kb)
+            for (int kb = 0; kb < NS; ++kb)
 #pragma unroll
                 for (int kk = 0; kk < 8; ++kk) {
                     const int k = 8 * kb + kk;
@@ -839,7 +858,7 @@ __global__ void __launch_bounds__(256)
                         xk.x = __shfl(x[kb].x, kk, 8);
                         xk.y = __shfl(x[kb].y, kk, 8);
 #pragma unroll
-                        for (int s = 0; s < 8; ++s) {
+                        for (int s = 0; s < NS; ++s) {
                             const int i = g + 8 * s;
                             if (i > k && i < ne) x[s] = csub(x[s], cmul(A[i + k * la], xk));
                         }
@@ -847,7 +866,7 @@ __global__ void __launch_bounds__(256)
                 }
         } else {  // x U11 = f, column by column
 #pragma unroll
-            for (int kb = 0; kb < 8; ++kb)
+            for (int kb = 0; kb < NS; ++kb)
 #pragma unroll
                 for (int kk = 0; kk < 8; ++kk) {
                     const int k = 8 * kb + kk;
@@ -858,7 +877,7 @@ __global__ void __launch_bounds__(256)
                         xk.y = __shfl(cand.y, kk, 8);
                         if (g == kk) x[kb] = xk;
 #pragma unroll
-                        for (int s = 0; s < 8; ++s) {
+                        for (int s = 0; s < NS; ++s) {
                             const int j = g + 8 * s;
                             if (j > k && j < ne) x[s] = csub(x[s], cmul(xk, A[k + j * la]));
                         }
@@ -867,7 +886,7 @@ __global__ void __launch_bounds__(256)
         }
         if (c0 + grp < end)
 #pragma unroll
-            for (int s = 0; s < 8; ++s) {
+            for (int s = 0; s < NS; ++s) {
                 const int e = g + 8 * s;
                 if (e < ne) {
                     if (upper)
@@ -893,9 +912,10 @@ __global__ void __launch_bounds__(256)
     using M = Mfma<T>;
     __shared__ T As[kChunk * kTileLd];  // As[k * kTileLd + i] = L21[i0 + i, kc + k]
     __shared__ T Bs[kChunk * kTileLd];  // Bs[k * kTileLd + j] = U12[kc + k, j0 + j]
-    const Tile tl = tiles[blockIdx.x];
+    const TilePoint tp = xcd_tile();
+    const Tile tl = tiles[tp.tile];
     const NodeDesc nd = nodes[tl.node];
-    const int64_t p = blockIdx.y;
+    const int64_t p = tp.point;
     if (pv.status[p * pv.status_stride + nd.slot] != kFrontOk) return;
     T* F = (kRoot ? reinterpret_cast<T*>(pv.root_arena) + p * pv.root_stride
                   : reinterpret_cast<T*>(pv.arena) + p * pv.arena_stride) +
@@ -1228,7 +1248,18 @@ void launch_trsm(const NodeDesc* nodes, const Tile* tiles, int32_t count, int32_
         allow_lds(reinterpret_cast<const void*>(kernel));
         hipLaunchKernelGGL(kernel, grid2(count, points), dim3(256), lds, st, nodes, tiles, pv);
     };
-    single ? go(trsm_kernel<float2>) : go(trsm_kernel<double2>);
+    auto pick = [&](auto t) {  // register slots per lane: ne <= 8 NS
+        using T = decltype(t);
+        if (max_ne <= 16)
+            go(trsm_kernel<T, 2>);
+        else if (max_ne <= 32)
+            go(trsm_kernel<T, 4>);
+        else if (max_ne <= 48)
+            go(trsm_kernel<T, 6>);
+        else
+            go(trsm_kernel<T, 8>);
+    };
+    single ? pick(float2{}) : pick(double2{});
 }
 
 void launch_schur(const NodeDesc* nodes, const Tile* tiles, int32_t count, const KidDesc* kids, const int32_t* maps,
