@@ -33,9 +33,13 @@ struct DeviceError : std::runtime_error {
         if (e_ != hipSuccess) throw DeviceError(std::string(#x) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
-// Pivot tests of crates/caroli-dense/src/field.rs:61 (c64) and :111 (c32).
+// Pivot tests of crates/caroli-dense/src/field.rs:61 (c64) and :111 (c32). The refined mixed
+// mode uses complex f32 fronts only as the preconditioner of an FP64 refinement: it accepts
+// pivots down to kTolMixed (still 16 f32 ulps above rounding), so that fewer points fall back to
+// complex f64; the refinement decides whether the factors are good enough.
 constexpr double kTolDouble = 1e-11;
 constexpr double kTolSingle = 1e-5;
+constexpr double kTolMixed = 1e-6;
 
 struct DevBuf {
     void* p = nullptr;
@@ -76,9 +80,11 @@ void upload(DevBuf& d, const std::vector<T>& v, hipStream_t st) {
 /// The fronts of one launch group (a tree level in one precision, or the root) split by kernel:
 /// small fronts in LDS size classes, large fronts with the tiles of their Schur blocks.
 struct LevelSet {
-    static constexpr int kClasses = 3;
+    // LDS classes by front size: n <= 16, 32, 48, then up to the precision's limit (the LDS a
+    // workgroup takes, and so the workgroups a CU holds, follow the class's largest front).
+    static constexpr int kClasses = 4;
     std::vector<int32_t> lds[kClasses];
-    int32_t lds_n[kClasses] = {0, 0, 0};  // largest n of each class
+    int32_t lds_n[kClasses] = {0, 0, 0, 0};  // largest n of each class
     std::vector<int32_t> large;
     int32_t large_ne = 0;  // largest ne of the large fronts
     std::vector<Tile> tiles, gtiles;  // Schur tiles, border tiles (gather)
@@ -87,15 +93,14 @@ struct LevelSet {
     int32_t f11_ne = 0, big_ne = 0;
     std::vector<Tile> ttiles;  // trsm tiles of the f11 fronts
     // Algorithmic work (complex words, flops) for the kernel timers.
-    double lds_flops[kClasses] = {0, 0, 0}, lds_words[kClasses] = {0, 0, 0}, lds_s_words[kClasses] = {0, 0, 0},
-           lds_f_words[kClasses] = {0, 0, 0};
+    double lds_flops[kClasses] = {}, lds_words[kClasses] = {}, lds_s_words[kClasses] = {}, lds_f_words[kClasses] = {};
     double asm_words = 0, gather_words = 0, border_flops = 0, border_words = 0, schur_flops = 0, schur_words = 0;
     double f11_flops = 0, f11_words = 0, trsm_flops = 0, trsm_words = 0;
 };
 
 /// Device pointers of a LevelSet's lists.
 struct DevLevel {
-    const int32_t* lds[LevelSet::kClasses] = {nullptr, nullptr, nullptr};
+    const int32_t* lds[LevelSet::kClasses] = {};
     const int32_t* large = nullptr;
     const Tile* tiles = nullptr;
     const Tile* gtiles = nullptr;
@@ -117,7 +122,7 @@ LevelSet classify(const std::vector<NodeDesc>& nodes, const std::vector<KidDesc>
         for (int c = d.kid_beg; c < d.kid_end; ++c) kid_words += double(kids[c].size) * kids[c].size;
         const double fl = 8.0 / 3.0 * ne * ne * ne + 8.0 * ne * ne * nb + 8.0 * ne * nb * nb;
         if (!root && d.n <= limit) {
-            const int c = d.n <= 16 ? 0 : d.n <= 32 ? 1 : 2;
+            const int c = d.n <= 16 ? 0 : d.n <= 32 ? 1 : d.n <= 48 ? 2 : 3;
             L.lds[c].push_back(q);
             L.lds_n[c] = std::max(L.lds_n[c], d.n);
             L.lds_flops[c] += fl;
@@ -443,6 +448,7 @@ const char* const kFamilyNames[kFamilies] = {"front_lds_kernel",    "gather_kern
 struct Engine {
     std::mutex mu;
     hipStream_t stream = nullptr;
+    double tol_single = kTolSingle;  // pivot test of complex f32 fronts in the current run
     bool ok = false;
     std::map<uint64_t, std::weak_ptr<Structure>> registry;
     DevBuf arena, root_arena, piv, status, tout;
@@ -598,7 +604,7 @@ void run_level(Engine& eng, const LevelSet& L, const DevLevel& D, const NodeDesc
                const int32_t* diag, const KidDesc* kids, const int32_t* maps, const PointsView& pv, int32_t P,
                bool single, bool root) {
     const double es = single ? 8.0 : 16.0, fes = root ? 16.0 : es;
-    const double tol = single && !root ? kTolSingle : kTolDouble;
+    const double tol = single && !root ? eng.tol_single : kTolDouble;
     for (int c = 0; c < LevelSet::kClasses; ++c) {
         const int32_t count = int32_t(L.lds[c].size());
         if (count == 0) continue;
@@ -881,7 +887,7 @@ void solve(Engine& eng, const DevPlan& dp, const PointsView& pv, const SolveView
 }
 
 /// Refinement: iterate until the relative correction max|d| / max|x| is below this.
-constexpr double kRefineTol = 1e-12;
+constexpr double kRefineTol = 1e-10;
 constexpr int kRefineMaxIter = 12;
 
 struct Refined {
@@ -895,7 +901,8 @@ struct Refined {
 /// f32 fronts when `single`): x = A^-1 E_R W_R, refined with FP64 residuals b - A x of the full
 /// operator z - H - Sigma until the correction is below kRefineTol; then T_b = |W_L^H x_L|^2.
 Refined refine(Engine& eng, const HostPlan& hp, const DevPlan& dp, const PointsView& pv,
-               const std::vector<const cg_sigma*>& sig, int b, int64_t n_orb, bool single) {
+               const std::vector<const cg_sigma*>& sig, int b, int64_t n_orb, bool single,
+               const std::vector<char>& skip) {
     const int32_t P = int32_t(sig.size());
     Refined out;
     out.t.assign(P, 0.0);
@@ -931,7 +938,8 @@ Refined refine(Engine& eng, const HostPlan& hp, const DevPlan& dp, const PointsV
     launch_scatter(bv, dp.right_orbs.as<int32_t>(), nr, dw.as<double2>(), int64_t(nr) * c, false, P, eng.stream);
     HIP_CHECK(hipMemcpyAsync(xb.p, bb.p, bytes, hipMemcpyDeviceToDevice, eng.stream));
     solve(eng, dp, pv, xv, P, single, false, true);
-    std::vector<double> ratio(P, INFINITY);
+    std::vector<double> ratio(P, INFINITY), last(P, INFINITY);
+    std::vector<char> gave_up(P, 0);
     for (int it = 0; it < kRefineMaxIter; ++it) {
         launch_residual(pv, xv, bb.as<double2>(), db.as<double2>(), dp.rows.as<int32_t>(), nrows,
                         dp.row_ptr.as<int64_t>(), dp.cols.as<uint32_t>(), dp.left_pos.as<int32_t>(),
@@ -943,8 +951,19 @@ Refined refine(Engine& eng, const HostPlan& hp, const DevPlan& dp, const PointsV
         HIP_CHECK(hipMemcpyAsync(ratio.data(), rb.p, P * sizeof(double), hipMemcpyDeviceToHost, eng.stream));
         HIP_CHECK(hipStreamSynchronize(eng.stream));
         ++out.iters;
+        // Points whose factors are unusable (skip), or whose correction stopped shrinking, are left
+        // to the complex f64 recomputation and no longer hold the loop.
         bool done = true;
-        for (double r : ratio) done &= r <= kRefineTol;
+        for (int32_t p = 0; p < P; ++p) {
+            const double r = ratio[p];
+            if (skip[p] || gave_up[p]) continue;
+            if (!(r < INFINITY) || (it >= 2 && r > 0.5 * last[p])) {
+                gave_up[p] = 1;
+                continue;
+            }
+            last[p] = r;
+            done &= r <= kRefineTol;
+        }
         if (done) break;
     }
     launch_gather(xv, dp.left_orbs.as<int32_t>(), nl, gb.as<double2>(), P, eng.stream);
@@ -952,8 +971,8 @@ Refined refine(Engine& eng, const HostPlan& hp, const DevPlan& dp, const PointsV
     HIP_CHECK(hipMemcpyAsync(xl.data(), gb.p, xl.size() * sizeof(double2), hipMemcpyDeviceToHost, eng.stream));
     HIP_CHECK(hipStreamSynchronize(eng.stream));
     for (int32_t p = 0; p < P; ++p) {
-        out.ok[p] = ratio[p] <= kRefineTol;
-        out.worst = std::max(out.worst, ratio[p]);
+        out.ok[p] = !skip[p] && !gave_up[p] && ratio[p] <= kRefineTol;
+        if (out.ok[p]) out.worst = std::max(out.worst, ratio[p]);
         const CMat& wl = sig[p]->blocks[b].wl;
         double t = 0.0;
         for (int col = 0; col < c; ++col)
@@ -1068,6 +1087,7 @@ int32_t run_batch(cg_batch* bt, double* out_t, int32_t* out_status, cg_batch_sta
     for (int64_t p = 0; p < count; ++p) out_t[p] = 0.0;
     {
         std::lock_guard<std::mutex> lock(eng.mu);
+        eng.tol_single = mixed ? kTolMixed : kTolSingle;
         for (auto& u : bt->units) {
             const HostPlan& hp = u.structure->host[u.block];
             const DevPlan& dp = u.structure->dev[u.block];
@@ -1100,7 +1120,11 @@ int32_t run_batch(cg_batch* bt, double* out_t, int32_t* out_status, cg_batch_sta
                 auto t0 = std::chrono::steady_clock::now();
                 std::vector<const cg_sigma*> sig;
                 for (auto p : u.points) sig.push_back(bt->sigmas[p]);
-                Refined rf = refine(eng, hp, dp, pv, sig, u.block, int64_t(bt->systems[u.points[0]]->spec.h.n), true);
+                std::vector<char> skip(P, 0);
+                for (size_t q = 0; q < P; ++q)
+                    for (int32_t k = 0; k <= hp.root; ++k) skip[q] |= st[q * hp.slots + k] != kFrontOk;
+                Refined rf = refine(eng, hp, dp, pv, sig, u.block, int64_t(bt->systems[u.points[0]]->spec.h.n), true,
+                                    skip);
                 local.seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
                 local.refinement_iters += rf.iters;
                 for (size_t q = 0; q < P; ++q) {

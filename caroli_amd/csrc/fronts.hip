@@ -458,7 +458,9 @@ __global__ void __launch_bounds__(NT)
 //      the L21 rows are solved against the panel's U11 block, the panel goes back to the front;
 //   4. panel rows, columns [k1, n):   F -= L[panel, 0:k0] U[0:k0, k1:n];
 //   5. the panel's U rows are solved with its unit-lower L11 block (one lane per column).
-constexpr int kBorderThreads = 256;
+// Workgroup size of border_kernel: 4 wavefronts for the separators past kF11Lds, 8 for the root
+// (one front per point: its panel loop is the longest chain of a sweep).
+constexpr int kBorderThreads = 256, kRootThreads = 512;
 
 // C -= A B for one 16 x 16 tile of the front F: C = F[i0:i0+16, j0:j0+16], A = F[i0:, 0:kend]
 // (L columns), B = F[0:kend, j0:] (U rows), computed as C^T -= B^T A^T so that lanes read and
@@ -506,10 +508,9 @@ __device__ __forceinline__ void mfma_gemm_tile(T* F, int n, int i0, int j0, int 
 // The border factorisation of one front; kLds: the panel's separator rows are held in LDS (P).
 // mv: LDS table of the panel's row moves (count, then destination and source rows); buf: LDS
 // staging of the moved rows (2 kPanel x kMoveCols).
-template <class T, bool kRoot, bool kLds>
+template <class T, bool kRoot, bool kLds, int NT>
 __device__ __forceinline__ void border_factor(T* F, T* P, T* buf, int32_t* spiv, int32_t* mv, int n, int ne, int ne1,
                                               int32_t* piv, int32_t* status, double scale, double tol) {
-    constexpr int NT = kBorderThreads;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     auto at = [&](int i, int j) -> T& { return F[i + int64_t(j) * n]; };
     for (int k0 = 0; k0 < ne; k0 += kPanel) {
@@ -674,12 +675,12 @@ __device__ __forceinline__ void border_factor(T* F, T* P, T* buf, int32_t* spiv,
     if (tid == 0) *status = kFrontOk;
 }
 
-template <class T, bool kRoot>
-__global__ void __launch_bounds__(kBorderThreads)
+template <class T, bool kRoot, int NT>
+__global__ void __launch_bounds__(NT)
     border_kernel(const NodeDesc* __restrict__ nodes, const int32_t* __restrict__ level, const Entry* __restrict__ ent,
                   const int32_t* __restrict__ diag, PointsView pv, double tol, int lds_rows) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
-    __shared__ double sv[kBorderThreads / 64];
+    __shared__ double sv[NT / 64];
     __shared__ int32_t spiv[kPanel];
     __shared__ int32_t mv[1 + 4 * kPanel];
     const NodeDesc nd = nodes[level[blockIdx.x]];
@@ -691,28 +692,28 @@ __global__ void __launch_bounds__(kBorderThreads)
     int32_t* status = pv.status + p * pv.status_stride + nd.slot;
     const int n = nd.n, ne = nd.ne, ne1 = nd.ne1;
     if (!kRoot && nd.ent_end > nd.ent_beg) {  // -H on the separator (the children were gathered before)
-        add_entries<kBorderThreads>(ent, nd.ent_beg, nd.ent_end, pv.vals[p], n,
+        add_entries<NT>(ent, nd.ent_beg, nd.ent_end, pv.vals[p], n,
                                     [&](int at, double2 v) { F[at] = cadd(F[at], convert<T>(v)); });
         __syncthreads();
     }
     if (!kRoot && nd.diag_end > nd.diag_beg) {  // z on the separator's diagonal
         const T z = convert<T>(pv.z[p]);
-        for (int d = nd.diag_beg + int(threadIdx.x); d < nd.diag_end; d += kBorderThreads) {
+        for (int d = nd.diag_beg + int(threadIdx.x); d < nd.diag_end; d += NT) {
             const int64_t q = diag[d];
             F[q + q * n] = cadd(F[q + q * n], z);
         }
         __syncthreads();
     }
     double m = 0.0;
-    for (int idx = threadIdx.x; idx < ne1 * ne1; idx += kBorderThreads)
+    for (int idx = threadIdx.x; idx < ne1 * ne1; idx += NT)
         m = fmax(m, cabs(F[(idx % ne1) + int64_t(idx / ne1) * n]));
     const double scale = block_max(m, sv);
     T* buf = reinterpret_cast<T*>(lds_raw);  // the move buffer first, then the panel
     if (ne <= lds_rows)
-        border_factor<T, kRoot, true>(F, buf + 2 * kPanel * kMoveCols, buf, spiv, mv, n, ne, ne1, piv, status, scale,
+        border_factor<T, kRoot, true, NT>(F, buf + 2 * kPanel * kMoveCols, buf, spiv, mv, n, ne, ne1, piv, status, scale,
                                       tol);
     else
-        border_factor<T, kRoot, false>(F, nullptr, buf, spiv, mv, n, ne, ne1, piv, status, scale, tol);
+        border_factor<T, kRoot, false, NT>(F, nullptr, buf, spiv, mv, n, ne, ne1, piv, status, scale, tol);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -968,10 +969,9 @@ __global__ void __launch_bounds__(256)
         __syncthreads();
     }
     // Block (x, y) of the wave: lane l, element jj holds (L21 U12)[i, j] with
-    // i = wi + 16 y + l % 16 and j = wj + 16 x + row(l, jj). F22 is read (all 16 values of a lane
-    // first) or gathered from the children, then S = F22 - L21 U12 is stored.
-    // Two halves (x) to bound the registers: F22 values (read, or gathered from the children with
-    // all map loads of a child before its 8 values), then S = F22 - L21 U12 stored.
+    // i = wi + 16 y + l % 16 and j = wj + 16 x + row(l, jj). In two halves (x) to bound the
+    // registers: F22 (read, or gathered from the children with all map loads before the values;
+    // two children, the usual case, in one batch), then S = F22 - L21 U12 stored.
     const T* kidbase = reinterpret_cast<const T*>(pv.arena) + p * pv.arena_stride;
     int ir[2];  // this lane's rows of S (clamped)
 #pragma unroll
@@ -982,7 +982,39 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
         for (int jj = 0; jj < 4; ++jj) jc[jj] = min(tl.j0 + wj + 16 * x + M::row(lane, jj), nb - 1);
         T f22[2][4];
-        if (kGather) {
+        if (kGather && nd.kid_end - nd.kid_beg == 2) {
+            const KidDesc k0 = kids[nd.kid_beg], k1 = kids[nd.kid_beg + 1];
+            const int32_t *inv0 = maps + k0.inv_beg + ne, *inv1 = maps + k1.inv_beg + ne;
+            int a0[2], a1[2], b0[4], b1[4];
+#pragma unroll
+            for (int y = 0; y < 2; ++y) {
+                a0[y] = inv0[ir[y]];
+                a1[y] = inv1[ir[y]];
+            }
+#pragma unroll
+            for (int jj = 0; jj < 4; ++jj) {
+                b0[jj] = inv0[jc[jj]];
+                b1[jj] = inv1[jc[jj]];
+            }
+            const T *S0 = kidbase + k0.src, *S1 = kidbase + k1.src;
+            T v0[2][4], v1[2][4];
+#pragma unroll
+            for (int y = 0; y < 2; ++y)
+#pragma unroll
+                for (int jj = 0; jj < 4; ++jj) {
+                    v0[y][jj] = S0[max(a0[y], 0) + int64_t(max(b0[jj], 0)) * k0.ld];
+                    v1[y][jj] = S1[max(a1[y], 0) + int64_t(max(b1[jj], 0)) * k1.ld];
+                }
+#pragma unroll
+            for (int y = 0; y < 2; ++y)
+#pragma unroll
+                for (int jj = 0; jj < 4; ++jj) {
+                    T f = zero;
+                    if (a0[y] >= 0 && b0[jj] >= 0) f = cadd(f, v0[y][jj]);
+                    if (a1[y] >= 0 && b1[jj] >= 0) f = cadd(f, v1[y][jj]);
+                    f22[y][jj] = f;
+                }
+        } else if (kGather) {
 #pragma unroll
             for (int y = 0; y < 2; ++y)
 #pragma unroll
@@ -1211,17 +1243,17 @@ void launch_border(const NodeDesc* nodes, const int32_t* level, int32_t count, i
     if (count <= 0 || points <= 0) return;
     const int32_t rows = border_lds_rows(max_ne, single);
     const size_t lds = size_t(rows * kPanel + 2 * kPanel * kMoveCols) * (single ? sizeof(float2) : sizeof(double2));
-    const dim3 g = grid2(count, points), b(kBorderThreads);
-    auto go = [&](auto kernel) {
+    const dim3 g = grid2(count, points);
+    auto go = [&](auto kernel, int nt) {
         allow_lds(reinterpret_cast<const void*>(kernel));
-        hipLaunchKernelGGL(kernel, g, b, lds, st, nodes, level, ent, diag, pv, tol, rows);
+        hipLaunchKernelGGL(kernel, g, dim3(nt), lds, st, nodes, level, ent, diag, pv, tol, rows);
     };
     if (single)
-        go(border_kernel<float2, false>);
+        go(border_kernel<float2, false, kBorderThreads>, kBorderThreads);
     else if (root)
-        go(border_kernel<double2, true>);
+        go(border_kernel<double2, true, kRootThreads>, kRootThreads);
     else
-        go(border_kernel<double2, false>);
+        go(border_kernel<double2, false, kBorderThreads>, kBorderThreads);
 }
 
 void launch_f11(const NodeDesc* nodes, const int32_t* level, int32_t count, int32_t max_ne, const Entry* ent,
