@@ -135,10 +135,6 @@ int32_t finish(cg_system* s, cg_system** out) {
     h = fnv(h, &s->spec.axis, sizeof(int));
     for (auto* m : {&s->spec.left_h00, &s->spec.left_h01, &s->spec.right_h00, &s->spec.right_h01}) h = pattern_hash(h, *m);
     s->key = h;
-    for (auto& b : s->sym.blocks) {
-        s->left_modes.emplace_back(b.left_h00, b.left_h01);
-        s->right_modes.emplace_back(b.right_h00, b.right_h01);
-    }
     *out = s;
     return CG_OK;
 }
@@ -305,21 +301,34 @@ int32_t cg_work(const cg_system* s, double* flops, double* bytes) {
 
 // NestedTwoTerminal::self_energies (nested.rs:652-676): per block, the left lead's Sigma_L and
 // the right lead's Sigma_R (the left modes serve both when the leads are equal).
+void ensure_modes(const cg_system* s) {
+    std::lock_guard<std::mutex> g(s->mu);
+    if (!s->left_modes.empty() || s->sym.blocks.empty()) return;
+    s->left_modes.resize(s->sym.blocks.size());
+    s->right_modes.resize(s->sym.blocks.size());
+    for (size_t b = 0; b < s->sym.blocks.size(); ++b) {
+        const auto& blk = s->sym.blocks[b];
+        s->left_modes[b].emplace(blk.left_h00, blk.left_h01);
+        if (!blk.same_lead) s->right_modes[b].emplace(blk.right_h00, blk.right_h01);
+    }
+}
+
 int32_t cg_self_energies(const cg_system* s, double energy, cg_sigma** out) {
     if (!s || !out) return CG_ERR_INVALID;
     try {
+        ensure_modes(s);
         auto* g = new cg_sigma;
         g->energy = energy;
         for (size_t b = 0; b < s->sym.blocks.size(); ++b) {
             auto& blk = s->sym.blocks[b];
             cg_sigma::Block sb;
             LeadModes ml, mr;
-            Status st = s->left_modes[b].modes_at(energy, ml);
+            Status st = s->left_modes[b]->modes_at(energy, ml);
             if (st == kOk) {
                 if (blk.same_lead)
                     mr = ml;
                 else
-                    st = s->right_modes[b].modes_at(energy, mr);
+                    st = s->right_modes[b]->modes_at(energy, mr);
             }
             if (st != kOk) {
                 delete g;

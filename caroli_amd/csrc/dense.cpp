@@ -347,60 +347,139 @@ bool svd(const CMat& a, CMat& u, std::vector<double>& s, CMat& v) {
 }
 
 bool hermitian_eigen(const CMat& a0, std::vector<double>& w, CMat& v) {
+    // Householder reduction to Hermitian tridiagonal form A = Q T Q^H (reflectors kept), the
+    // subdiagonal made real by a diagonal unitary D (T = D T' D^H), the implicit QL iteration with
+    // Wilkinson shifts on the real tridiagonal T' = Z diag(w) Z^T, and V = Q D Z.
     const int n = a0.rows;
     CMat a(n, n);
     for (int j = 0; j < n; ++j)
         for (int i = 0; i < n; ++i) a(i, j) = (a0(i, j) + std::conj(a0(j, i))) * 0.5;
-    v = CMat::identity(n);
-    double scale = a.norm_max();
-    for (int sweep = 0; sweep < 100; ++sweep) {
-        double off = 0.0;
-        for (int j = 0; j < n; ++j)
-            for (int i = 0; i < j; ++i) off = std::max(off, std::abs(a(i, j)));
-        if (off <= 1e-16 * scale || scale == 0.0) break;
-        if (sweep == 99) return false;
-        for (int p = 0; p < n - 1; ++p)
-            for (int q = p + 1; q < n; ++q) {
-                cplx b = a(p, q);
-                double ab = std::abs(b);
-                if (ab <= 1e-300) continue;
-                cplx wph = std::conj(b) / ab;
-                double ap = a(p, p).real(), aq = a(q, q).real();
-                double theta = (aq - ap) / (2.0 * ab);
-                double tt = (theta >= 0 ? 1.0 : -1.0) / (std::abs(theta) + std::sqrt(theta * theta + 1.0));
-                double c = 1.0 / std::sqrt(tt * tt + 1.0), s = tt * c;
-                // U = [[c, s], [-s w, c w]] on (p, q); A <- U^H A U, V <- V U.
-                for (int i = 0; i < n; ++i) {
-                    cplx x = a(i, p), y = a(i, q);
-                    a(i, p) = c * x - s * wph * y;
-                    a(i, q) = s * x + c * wph * y;
+    w.assign(n, 0.0);
+    if (n == 0) {
+        v = CMat(0, 0);
+        return true;
+    }
+    std::vector<std::vector<cplx>> refl;  // reflector k acts on rows/columns k+1..n-1
+    std::vector<cplx> p(n), wv(n);
+    for (int k = 0; k + 2 < n; ++k) {
+        const int m = n - k - 1;
+        std::vector<cplx> x(a.col(k) + k + 1, a.col(k) + n);
+        double nx = 0.0;
+        for (auto& z : x) nx += std::norm(z);
+        nx = std::sqrt(nx);
+        if (nx == 0.0) {
+            refl.emplace_back();
+            continue;
+        }
+        const cplx ph = std::abs(x[0]) > 0.0 ? x[0] / std::abs(x[0]) : cplx(1.0);
+        const cplx alpha = -ph * nx;
+        x[0] -= alpha;
+        double nv = 0.0;
+        for (auto& z : x) nv += std::norm(z);
+        nv = std::sqrt(nv);
+        for (auto& z : x) z /= nv;
+        // A22 <- H A22 H with H = I - 2 x x^H: p = A22 x, g = x^H p, w = 2 p - 2 g x,
+        // A22 <- A22 - x w^H - w x^H.
+        for (int i = 0; i < m; ++i) p[i] = 0.0;
+        for (int j = 0; j < m; ++j) {
+            const cplx xj = x[j];
+            const cplx* col = a.col(k + 1 + j) + k + 1;
+            for (int i = 0; i < m; ++i) p[i] += col[i] * xj;
+        }
+        cplx g = 0.0;
+        for (int i = 0; i < m; ++i) g += std::conj(x[i]) * p[i];
+        for (int i = 0; i < m; ++i) wv[i] = 2.0 * p[i] - 2.0 * g.real() * x[i];
+        for (int j = 0; j < m; ++j) {
+            const cplx cxj = std::conj(x[j]), cwj = std::conj(wv[j]);
+            cplx* col = a.col(k + 1 + j) + k + 1;
+            for (int i = 0; i < m; ++i) col[i] -= x[i] * cwj + wv[i] * cxj;
+        }
+        a(k + 1, k) = alpha;
+        a(k, k + 1) = std::conj(alpha);
+        for (int i = k + 2; i < n; ++i) a(i, k) = a(k, i) = 0.0;
+        refl.push_back(std::move(x));
+    }
+    // Real tridiagonal: d, e (e[i] couples i and i + 1) and the phases of D.
+    std::vector<double> d(n), e(n, 0.0);
+    std::vector<cplx> phase(n, 1.0);
+    for (int i = 0; i < n; ++i) d[i] = a(i, i).real();
+    for (int i = 0; i + 1 < n; ++i) {
+        const cplx s = a(i + 1, i);
+        const double m = std::abs(s);
+        e[i] = m;
+        phase[i + 1] = m > 0.0 ? phase[i] * s / m : phase[i];
+    }
+    std::vector<double> z(size_t(n) * n, 0.0);  // column-major Z
+    for (int i = 0; i < n; ++i) z[i + size_t(i) * n] = 1.0;
+    const double eps = std::numeric_limits<double>::epsilon();
+    for (int l = 0; l < n; ++l) {
+        int iter = 0, m;
+        do {
+            for (m = l; m + 1 < n; ++m) {
+                const double dd = std::abs(d[m]) + std::abs(d[m + 1]);
+                if (std::abs(e[m]) <= eps * dd) break;
+            }
+            if (m == l) break;
+            if (++iter > 60) return false;
+            double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
+            double r = std::hypot(g, 1.0);
+            g = d[m] - d[l] + e[l] / (g + (g >= 0.0 ? r : -r));
+            double s = 1.0, c = 1.0, pp = 0.0;
+            int i;
+            bool deflated = false;
+            for (i = m - 1; i >= l; --i) {
+                const double f = s * e[i], b = c * e[i];
+                r = std::hypot(f, g);
+                e[i + 1] = r;
+                if (r == 0.0) {
+                    d[i + 1] -= pp;
+                    e[m] = 0.0;
+                    deflated = true;
+                    break;
                 }
-                for (int j = 0; j < n; ++j) {
-                    cplx x = a(p, j), y = a(q, j);
-                    a(p, j) = c * x - s * std::conj(wph) * y;
-                    a(q, j) = s * x + c * std::conj(wph) * y;
-                }
-                a(p, q) = 0.0;
-                a(q, p) = 0.0;
-                a(p, p) = a(p, p).real();
-                a(q, q) = a(q, q).real();
-                for (int i = 0; i < n; ++i) {
-                    cplx x = v(i, p), y = v(i, q);
-                    v(i, p) = c * x - s * wph * y;
-                    v(i, q) = s * x + c * wph * y;
+                s = f / r;
+                c = g / r;
+                g = d[i + 1] - pp;
+                r = (d[i] - g) * s + 2.0 * c * b;
+                pp = s * r;
+                d[i + 1] = g + pp;
+                g = c * r - b;
+                double* zi = z.data() + size_t(i) * n;
+                double* zj = z.data() + size_t(i + 1) * n;
+                for (int k = 0; k < n; ++k) {
+                    const double t = zj[k];
+                    zj[k] = s * zi[k] + c * t;
+                    zi[k] = c * zi[k] - s * t;
                 }
             }
+            if (deflated) continue;
+            d[l] -= pp;
+            e[l] = g;
+            e[m] = 0.0;
+        } while (true);
     }
+    // V = Q D Z, eigenvalues ascending.
     std::vector<int> order(n);
     for (int j = 0; j < n; ++j) order[j] = j;
-    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return a(x, x).real() < a(y, y).real(); });
-    w.resize(n);
-    CMat v2(n, n);
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return d[x] < d[y]; });
+    v = CMat(n, n);
     for (int k = 0; k < n; ++k) {
-        w[k] = a(order[k], order[k]).real();
-        for (int i = 0; i < n; ++i) v2(i, k) = v(i, order[k]);
+        w[k] = d[order[k]];
+        const double* zc = z.data() + size_t(order[k]) * n;
+        for (int i = 0; i < n; ++i) v(i, k) = phase[i] * zc[i];
     }
-    v = v2;
+    for (int k = int(refl.size()) - 1; k >= 0; --k) {
+        const auto& x = refl[k];
+        if (x.empty()) continue;
+        const int m = int(x.size());
+        for (int j = 0; j < n; ++j) {
+            cplx* col = v.col(j) + k + 1;
+            cplx t = 0.0;
+            for (int i = 0; i < m; ++i) t += std::conj(x[i]) * col[i];
+            t *= 2.0;
+            for (int i = 0; i < m; ++i) col[i] -= x[i] * t;
+        }
+    }
     return true;
 }
 

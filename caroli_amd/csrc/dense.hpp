@@ -6,7 +6,8 @@
 // reorder in schur.rs:199-251, svd :347-355, hermitian_eigen :359-367, orthonormal_columns) gives
 // the mode solver of crates/caroli-green/src/modes.rs, written here from the textbook algorithms
 // (Golub & Van Loan: Householder Hessenberg reduction and the implicit single-shift QR iteration,
-// Givens swaps of adjacent eigenvalues; one-sided Jacobi SVD; cyclic Jacobi for Hermitian matrices).
+// Givens swaps of adjacent eigenvalues; one-sided Jacobi SVD; Householder tridiagonalisation and
+// the implicit QL iteration with Wilkinson shifts for Hermitian matrices).
 #pragma once
 #include <vector>
 

@@ -2,6 +2,7 @@
 // self-energies (LeadSelfEnergies).
 #pragma once
 #include <memory>
+#include <optional>
 #include <mutex>
 
 #include "modes.hpp"
@@ -25,7 +26,10 @@ Work algorithmic_work(const Symbolic& sym, size_t nnz);
 struct cg_system {
     caroli::SystemSpec spec;
     caroli::Symbolic sym;
-    std::vector<caroli::ModeSolver> left_modes, right_modes;
+    /// Lead mode solvers per block, built on the first self-energy request (a system used only
+    /// with self-energies computed elsewhere never builds them); the right one only when the
+    /// block's leads differ.
+    mutable std::vector<std::optional<caroli::ModeSolver>> left_modes, right_modes;
     /// Key of the symbolic structure: systems with equal keys share device plans.
     uint64_t key = 0;
     mutable std::mutex mu;

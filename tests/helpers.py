@@ -43,3 +43,23 @@ def hashed(model, extent, w, subs_per_cell=1, dim=2, axis=0):
     cells, subs = cuboid_cells(extent, subs_per_cell, dim)
     pot = hash_disorder(cells, subs, w)
     return ca.NestedTwoTerminal.preset(model, extent, axis=axis, site_potential=pot)
+
+
+def oracle_gradient(s, energy):
+    """2 Re (G Gamma_R G^H Gamma_L G)_aa from the oracle's G columns (right slice) and the columns of
+    G^T (left slice, from the transposed system)."""
+    arr = s.arrays()
+    o, ot = OracleSystem(arr), OracleSystem(arr, transpose=True)
+    sig = s.self_energies(energy)
+    grad = np.zeros(arr["n"])
+    for b in range(s.block_count()):
+        sl, sr, _, _ = sig.block(b)
+        left, right, _ = o.block(b)
+        g_r = o.green_columns(b, energy, sl, sr, 1)              # G[:, R]
+        gt_l = ot.green_columns(b, energy, sl.T, sr.T, 0)        # G^T[:, L] = G[L, :]^T
+        gl = 1j * (sl - sl.conj().T)
+        gr = 1j * (sr - sr.conj().T)
+        g_lr = g_r[left.astype(int), :]
+        m = gr @ g_lr.conj().T @ gl                               # Gamma_R G_LR^H Gamma_L
+        grad += 2 * np.real(np.einsum("ac,cd,ad->a", g_r, m, gt_l))
+    return grad

@@ -12,8 +12,7 @@ import numpy as np
 import pytest
 
 import caroli_amd as ca
-from _oracle import OracleSystem
-from helpers import cuboid_cells, hash_disorder, hashed, oracle_of, sigmas_of
+from helpers import oracle_gradient, oracle_of
 
 pytestmark = pytest.mark.gpu
 
@@ -59,26 +58,6 @@ def test_mixed_square_with_delays():
     # The clean lattice at E = 0 has singular interior blocks: those points are recomputed in f64.
     s = ca.NestedTwoTerminal.preset(ca.Model.Square, [14, 9, 0])
     mixed_vs_oracle([s], [-1.3, 0.0, 0.4], "square 14x9 clean")
-
-
-def oracle_gradient(s, energy):
-    """2 Re (G Gamma_R G^H Gamma_L G)_aa from the oracle's G columns (right slice) and the columns of
-    G^T (left slice, from the transposed system)."""
-    arr = s.arrays()
-    o, ot = OracleSystem(arr), OracleSystem(arr, transpose=True)
-    sig = s.self_energies(energy)
-    grad = np.zeros(arr["n"])
-    for b in range(s.block_count()):
-        sl, sr, _, _ = sig.block(b)
-        left, right, _ = o.block(b)
-        g_r = o.green_columns(b, energy, sl, sr, 1)              # G[:, R]
-        gt_l = ot.green_columns(b, energy, sl.T, sr.T, 0)        # G^T[:, L] = G[L, :]^T
-        gl = 1j * (sl - sl.conj().T)
-        gr = 1j * (sr - sr.conj().T)
-        g_lr = g_r[left.astype(int), :]
-        m = gr @ g_lr.conj().T @ gl                               # Gamma_R G_LR^H Gamma_L
-        grad += 2 * np.real(np.einsum("ac,cd,ad->a", g_r, m, gt_l))
-    return grad
 
 
 def test_adjoint_matches_fp64_gradient_and_finite_differences():

@@ -320,7 +320,7 @@ def test_invalid_arguments_are_refused():
 def test_oracle_gradient_matches_finite_differences():
     # The FP64 gradient built from G columns (2 Re (G Gamma_R G^H Gamma_L G)_aa, the checker of the
     # GPU adjoint) against central differences of the oracle's own transmission.
-    from test_gpu_mixed_adjoint import oracle_gradient
+    from helpers import oracle_gradient
 
     ext, w, e, h = [8, 5, 0], 40 / 72.9, 0.01, 1e-6
     s = ca.NestedTwoTerminal.preset(ca.Model.BHZ, ext, disorder=1, strength=w, seed=1)
