@@ -1,18 +1,30 @@
 """Benchmark: transmission points per second of the nested-dissection engine on one MI355X.
 
-Workload (the equal-size batch): BHZ ribbon 200 x 64 sites (51,200 orbitals, two spin blocks),
-polarized magnetic impurities (concentration 0.2, exchange J = 0.4, opposite g), configurations
-seeded 1..8 as `caroli run` does; 33 energies in [-0.16, 0.16]; 264 points per step. The lead
-self-energies are computed once per energy on the host (FP64) and shared by the configurations
-(crates/caroli-cli/src/run.rs:113-127); they are inputs, resident in HBM with the Hamiltonians.
+Default workload (the equal-size batch, `--config c2`): BHZ ribbon 200 x 64 sites (51,200 orbitals,
+two spin blocks), polarized magnetic impurities (concentration 0.2, exchange J = 0.4, opposite g),
+configurations seeded 1..8 as `caroli run` does; 33 energies in [-0.16, 0.16]; 264 points per
+step, complex128. The lead self-energies are computed once per energy on the host (FP64) and shared
+by the configurations (crates/caroli-cli/src/run.rs:113-127); they are inputs, resident in HBM with
+the Hamiltonians before the timed region.
 
-A step is one batched launch over all 264 points. The JSON line carries the per-kernel roofline
-(HIP events on the engine's stream during the timed steps) and the CPU baseline: the oracle (FP64
-C++ restatement of the reference path) on the host cores for a bounded sample of the same points.
+Other workloads: `--config c3` zigzag graphene 512 x 64 two-site cells, Anderson w = 1, 256
+energies x 32 configurations (8,192 points; refined mixed precision by default); `--config c4`
+the adjoint of BHZ 100 x 32 (Anderson w = 40/72.9, E = 0.01; a step is one backward sweep giving
+dT/d(eps) for all 12,800 orbitals); `--config c5` the uneven cubic batch (W = 16.5, lengths 2L,
+cross-sections L drawn from {16, 24, 32, 40, 48} with xoshiro256** seeded 5, 4 energies in
+[-1, 1]) on the first `--systems` systems whose L is at most `--max-l`.
+
+A step is one batched launch over all points. The JSON line carries the roofline of the dominant
+kernel family (HIP events on the engine's stream during the timed steps; the HBM traffic per
+launch from the rocprofv3 counter summary in profiles/ when one matches) and the CPU baseline: the
+oracle (FP64 C++ restatement of the reference path) on the host cores for a bounded sample of the
+same points.
 """
 import argparse
+import glob
 import json
 import os
+import re
 import sys
 import time
 from concurrent.futures import ThreadPoolExecutor
@@ -27,42 +39,137 @@ PEAK_FP32_MFMA = 157.3e12  # MI355X dense FP32 matrix
 PEAK_HBM = 8.0e12
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--precision", default="double", choices=["double", "single", "mixed"])
-    ap.add_argument("--configs", type=int, default=8)
-    ap.add_argument("--energies", type=int, default=33)
-    ap.add_argument("--width", type=int, default=64)
-    ap.add_argument("--length", type=int, default=200)
-    ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU time of the baseline sample")
-    ap.add_argument("--no-cpu", action="store_true")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for {args.gpus}): the engine has no multi-GPU path; "
-              "independent points would shard by energy across GPUs.", file=sys.stderr)
-        sys.exit(2)
+def workload_tag(args):
+    return {"c2": "bhz200x64", "c3": "graphene512x64", "c4": "bhz100x32_adjoint", "c5": "cubic_uneven"}[args.config]
 
-    import caroli_amd as ca
 
-    if not ca.device_available():
-        raise SystemExit("no HIP device: the engine has no CPU path")
+def traffic_from_profiles(family, tag, precision):
+    """HBM bytes per launch of `family` on this workload from the committed counter summaries.
+
+    The summaries (profiles/*.md) carry lines `traffic_bytes_per_launch: <family> <workload>
+    <precision> <bytes>`, from two rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE) scaled by the
+    same counters read on a plain device copy of known size."""
+    pat = re.compile(r"^traffic_bytes_per_launch:\s+(\S+)\s+(\S+)\s+(\S+)\s+([0-9.eE+]+)\s*$")
+    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*.md"))):
+        with open(path) as f:
+            for line in f:
+                m = pat.match(line.strip())
+                if m and m.group(1) == family and m.group(2) == tag and m.group(3) == precision:
+                    return float(m.group(4)), os.path.relpath(path, ROOT)
+    return None, None
+
+
+def roofline_of(ks, prec_name, tag):
+    """Roofline object of the dominant kernel family (by time), from the engine's HIP-event timers.
+
+    The bound is the roof the family's algorithmic intensity meets first: flops per byte against
+    the ridge point peak_flops / peak_bandwidth."""
+    name, (launches, secs, fl, by) = max(ks.items(), key=lambda kv: kv[1][1])
+    peak = PEAK_FP64_MFMA if prec_name == "double" else PEAK_FP32_MFMA
+    bound = "mfma" if fl > 0 and fl / max(by, 1) >= peak / PEAK_HBM else "hbm"
+    traffic, source = traffic_from_profiles(name, tag, prec_name)
+    if bound == "mfma":
+        achieved = fl / secs
+        roof = {"bound": "mfma", "achieved": achieved / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
+                "frac": achieved / peak}
+    else:
+        achieved = by / secs
+        roof = {"bound": "hbm", "achieved": achieved / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s",
+                "frac": achieved / PEAK_HBM}
+    roof.update({"traffic": traffic, "traffic_source": source, "kernel": name, "launches": launches,
+                 "avg_launch_ms": 1e3 * secs / max(launches, 1),
+                 "algorithmic_bytes_per_launch": by / max(launches, 1),
+                 "algorithmic_flops_per_launch": fl / max(launches, 1)})
+    return roof
+
+
+def kernels_of(ks, steps):
+    return {k: {"launches": v[0], "ms": 1e3 * v[1] / max(steps, 1),
+                "tflops": (v[2] / v[1] / 1e12) if v[1] > 0 else 0.0,
+                "gbs": (v[3] / v[1] / 1e9) if v[1] > 0 else 0.0} for k, v in ks.items()}
+
+
+def build_points(args, ca, threads):
+    """Systems, per-point (system, self-energies, energy) lists and the description of a T workload."""
+    if args.config == "c2":
+        systems = [ca.NestedTwoTerminal.preset(ca.Model.BHZ, [args.length, args.width, 0], disorder=2, strength=0.4,
+                                               concentration=0.2, seed=1 + c) for c in range(args.configs)]
+        energies = np.linspace(-0.16, 0.16, args.energies)
+        desc = (f"BHZ {args.length}x{args.width} magnetic c=0.2 J=0.4 opposite-g, "
+                f"{args.energies} E x {args.configs} configs")
+        groups = [(systems, energies)]
+    elif args.config == "c3":
+        systems = [ca.NestedTwoTerminal.preset(ca.Model.Graphene, [512, 64, 0], disorder=1, strength=1.0, seed=1 + c)
+                   for c in range(args.configs)]
+        energies = np.linspace(-0.5, 0.5, args.energies)
+        desc = f"zigzag graphene 512x64 cells (128 atoms wide), Anderson w=1, {args.energies} E x {args.configs} configs"
+        groups = [(systems, energies)]
+    else:  # c5
+        import ctypes as C
+        u = np.zeros(512)
+        ca.capi.cg_uniforms(0, 5, None, 512, u.ctypes.data_as(C.POINTER(C.c_double)))
+        sizes = [[16, 24, 32, 40, 48][int(5 * x)] for x in u]
+        chosen = [(s, L) for s, L in enumerate(sizes) if L <= args.max_l][:args.systems]
+        energies = np.linspace(-1.0, 1.0, 4)
+        by_l = {}
+        for s, L in chosen:
+            by_l.setdefault(L, []).append(ca.NestedTwoTerminal.preset(ca.Model.Cubic, [2 * L, L, L], disorder=1,
+                                                                      strength=16.5, seed=1000 + s))
+        groups = [(by_l[L], energies) for L in sorted(by_l)]
+        desc = (f"cubic Anderson W=16.5, 2L x L x L, systems {[s for s, _ in chosen]} of the 512 (L <= {args.max_l}: "
+                f"L = {[L for _, L in chosen]}), 4 E")
+    pts_sys, pts_sig, pts_e = [], [], []
+    t0 = time.time()
+    for systems, energies in groups:
+        with ThreadPoolExecutor(threads) as ex:
+            sigmas = list(ex.map(systems[0].self_energies, energies))
+        for s in systems:
+            for e, g in zip(energies, sigmas):
+                pts_sys.append(s)
+                pts_sig.append(g)
+                pts_e.append(e)
+    return pts_sys, pts_sig, pts_e, desc, time.time() - t0
+
+
+def cpu_baseline(pts_sys, pts_sig, pts_e, t_gpu, args, threads):
+    """The oracle on the host cores, one point per thread, on a bounded sample of the points."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from _oracle import OracleSystem, transmissions_parallel
+
+    npts = len(pts_sys)
+    cache = {}
+
+    def osys(s):
+        if id(s) not in cache:
+            cache[id(s)] = OracleSystem(s.arrays())
+        return cache[id(s)]
+
+    def flat(p):
+        s, g = pts_sys[p], pts_sig[p]
+        return OracleSystem.pack([g.block(b)[:2] for b in range(s.block_count())])
+
+    t0 = time.perf_counter()
+    transmissions_parallel([osys(pts_sys[0])], [pts_e[0]], [flat(0)], 1)
+    lat = time.perf_counter() - t0
+    per_thread = max(1, int(args.cpu_seconds / max(lat, 1e-6)))
+    sample = min(npts, threads * per_thread)
+    idx = np.linspace(0, npts - 1, sample).astype(int)
+    t0 = time.perf_counter()
+    tc, _ = transmissions_parallel([osys(pts_sys[i]) for i in idx], [pts_e[i] for i in idx], [flat(i) for i in idx],
+                                   threads)
+    cpu_s = time.perf_counter() - t0
+    cpu = {"value": sample / cpu_s, "unit": "points/s", "cores": threads, "kind": "port",
+           "sample": f"{sample} of the {npts} points (evenly spaced), FP64 oracle, one point per thread",
+           "single_thread_s_per_point": lat}
+    return cpu, float(np.max(np.abs(tc - t_gpu[idx])))
+
+
+def run_transmission(args, ca):
     prec = {"double": ca.Precision.Double, "single": ca.Precision.Single, "mixed": ca.Precision.Mixed}[args.precision]
     threads = max(1, min(os.cpu_count() or 1, 16))
-
     t0 = time.time()
-    systems = [ca.NestedTwoTerminal.preset(ca.Model.BHZ, [args.length, args.width, 0], disorder=2, strength=0.4,
-                                           concentration=0.2, seed=1 + c) for c in range(args.configs)]
-    build_s = time.time() - t0
-    energies = np.linspace(-0.16, 0.16, args.energies)
-    t0 = time.time()
-    with ThreadPoolExecutor(threads) as ex:
-        sigmas = list(ex.map(systems[0].self_energies, energies))
-    sigma_s = time.time() - t0
-    pts_sys = [s for s in systems for _ in energies]
-    pts_sig = [g for _ in systems for g in sigmas]
+    pts_sys, pts_sig, pts_e, desc, sigma_s = build_points(args, ca, threads)
+    build_s = time.time() - t0 - sigma_s
     npts = len(pts_sys)
     batch = ca.ResidentBatch(pts_sys, pts_sig, prec)
     for _ in range(args.warmup):
@@ -76,71 +183,19 @@ def main():
     ks = ca.kernel_stats()
     ca.kernel_timing(False)
     step = float(np.mean(times))
-    value = npts / step
-    flops_pt, bytes_pt = systems[0].work()
-
-    # Dominant kernel family by time, and its roofline.
-    # The bound is the roof the kernel's algorithmic intensity meets first: flops/byte against
-    # the ridge point peak_flops / peak_bandwidth.
-    dom = max(ks.items(), key=lambda kv: kv[1][1])
-    name, (launches, secs, fl, by) = dom
-    peak = PEAK_FP32_MFMA if prec != ca.Precision.Double else PEAK_FP64_MFMA
-    bound = "mfma" if fl > 0 and fl / max(by, 1) >= peak / PEAK_HBM else "hbm"
-    if bound == "mfma":
-        achieved = fl / secs
-        roof = {"bound": "mfma", "kernel": name, "achieved": achieved / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
-                "frac": achieved / peak, "traffic": None, "launches": launches,
-                "avg_launch_ms": 1e3 * secs / max(launches, 1)}
-    else:
-        achieved = by / secs
-        roof = {"bound": "hbm", "kernel": name, "achieved": achieved / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s",
-                "frac": achieved / PEAK_HBM, "traffic": None, "launches": launches,
-                "avg_launch_ms": 1e3 * secs / max(launches, 1)}
-    kernels = {k: {"launches": v[0], "ms": 1e3 * v[1] / max(args.steps, 1),
-                   "tflops": (v[2] / v[1] / 1e12) if v[1] > 0 else 0.0,
-                   "gbs": (v[3] / v[1] / 1e9) if v[1] > 0 else 0.0} for k, v in ks.items()}
-
-    cpu = None
-    max_dt = None
-    if not args.no_cpu:
-        sys.path.insert(0, os.path.join(ROOT, "tests"))
-        from _oracle import OracleSystem, transmissions_parallel
-
-        osys = [OracleSystem(s.arrays()) for s in systems]
-        flats = [OracleSystem.pack([g.block(b)[:2] for b in range(systems[0].block_count())]) for g in sigmas]
-        # Single-thread latency of one point, then a bounded sample on all threads.
-        t0 = time.perf_counter()
-        one, _ = transmissions_parallel([osys[0]], [energies[0]], [flats[0]], 1)
-        lat = time.perf_counter() - t0
-        per_thread = max(1, int(args.cpu_seconds / max(lat, 1e-6)))
-        sample = min(npts, threads * per_thread)
-        idx = np.linspace(0, npts - 1, sample).astype(int)
-        t0 = time.perf_counter()
-        tc, failed = transmissions_parallel([osys[i // len(energies)] for i in idx], [energies[i % len(energies)] for i in idx],
-                                            [flats[i % len(energies)] for i in idx], threads)
-        cpu_s = time.perf_counter() - t0
-        cpu = {"value": sample / cpu_s, "unit": "points/s", "cores": threads, "kind": "port",
-               "sample": f"{sample} of the {npts} points (evenly spaced), FP64 oracle, one point per thread",
-               "single_thread_s_per_point": lat}
-        max_dt = float(np.max(np.abs(tc - t[idx])))
-
-    line = {
+    cpu, max_dt = (None, None) if args.no_cpu else cpu_baseline(pts_sys, pts_sig, pts_e, t, args, threads)
+    flops_pt = np.mean([s.work()[0] for s in pts_sys])
+    bytes_pt = np.mean([s.work()[1] for s in pts_sys])
+    return {
         "metric": "T(E) points/sec at |dT|<=1e-8 vs FP64 CPU",
-        "value": value,
+        "value": npts / step,
         "unit": "points/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
         "ms_per_step": 1e3 * step,
         "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
         "dtype": {"double": "complex128", "single": "complex64", "mixed": "complex64+f64 refinement"}[args.precision],
-        "data": "synthetic (seeded disorder, host-built Hamiltonians)",
-        "config": {"workload": f"BHZ {args.length}x{args.width} magnetic c=0.2 J=0.4 opposite-g, "
-                               f"{args.energies} E x {args.configs} configs", "points_per_step": npts,
-                   "precision": args.precision, "orbitals": systems[0].sizes()[0], "blocks": systems[0].block_count()},
-        "roofline": roof,
+        "config": {"workload": desc, "points_per_step": npts, "precision": args.precision,
+                   "orbitals": int(max(s.sizes()[0] for s in pts_sys))},
+        "roofline": roofline_of(ks, args.precision, workload_tag(args)),
         "cpu_baseline": cpu,
         "max_abs_dT_vs_cpu": max_dt,
         "status_ok": int((st == 0).sum()),
@@ -151,11 +206,107 @@ def main():
         "algorithmic_flops_per_point": flops_pt,
         "algorithmic_bytes_per_point": bytes_pt,
         "whole_step_tflops": flops_pt * npts / step / 1e12,
+        "whole_step_algorithmic_gbs": bytes_pt * npts / step / 1e9,
         "sigma_host_seconds": sigma_s,
         "build_seconds": build_s,
-        "kernels": kernels,
+        "kernels": kernels_of(ks, args.steps),
     }
-    print(json.dumps(line), flush=True)
+
+
+def run_adjoint(args, ca):
+    """One backward sweep (T and dT/d(eps) of every orbital) per step, BHZ 100 x 32."""
+    s = ca.NestedTwoTerminal.preset(ca.Model.BHZ, [100, 32, 0], disorder=1, strength=40 / 72.9, seed=1)
+    sig = s.self_energies(0.01)
+    for _ in range(args.warmup):
+        s.backward(sig)
+    ca.kernel_timing(True)
+    times = []
+    for _ in range(args.steps):
+        t0 = time.perf_counter()
+        t, grad = s.backward(sig)
+        times.append(time.perf_counter() - t0)
+    ks = ca.kernel_stats()
+    ca.kernel_timing(False)
+    step = float(np.mean(times))
+    cpu, err = None, None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        from helpers import oracle_gradient
+
+        t0 = time.perf_counter()
+        ref = oracle_gradient(s, 0.01)
+        cpu_s = time.perf_counter() - t0
+        cpu = {"value": 1.0 / cpu_s, "unit": "gradients/s", "cores": 1, "kind": "port",
+               "sample": "one FP64 gradient from the oracle's G columns (G E_R and G^T E_L, all orbitals)"}
+        err = float(np.abs(grad - ref).max() / np.abs(ref).max())
+    return {
+        "metric": "adjoint sweeps/sec (T and dT/d(eps) for all orbitals)",
+        "value": 1.0 / step,
+        "unit": "gradients/s",
+        "ms_per_step": 1e3 * step,
+        "higher_is_better": True,
+        "dtype": "complex128",
+        "config": {"workload": "BHZ 100x32 Anderson w=40/72.9, E=0.01, adjoint over 12,800 orbitals", "T": t},
+        "roofline": roofline_of(ks, "double", workload_tag(args)),
+        "cpu_baseline": cpu,
+        "max_rel_grad_vs_cpu": err,
+        "kernels": kernels_of(ks, args.steps),
+    }
+
+
+def copy_calibration():
+    """A plain device copy of known size, for scaling the FETCH_SIZE / WRITE_SIZE counters."""
+    import torch
+
+    x = torch.empty(1 << 27, dtype=torch.complex128, device="cuda")  # 2 GiB
+    x.fill_(1.0)
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": x.numel() * 16}), flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", default="c2", choices=["c2", "c3", "c4", "c5"])
+    ap.add_argument("--precision", default=None, choices=["double", "single", "mixed"])
+    ap.add_argument("--configs", type=int, default=None, help="disorder configurations (c2: 8, c3: 32)")
+    ap.add_argument("--energies", type=int, default=None, help="energies (c2: 33, c3: 256)")
+    ap.add_argument("--width", type=int, default=64)
+    ap.add_argument("--length", type=int, default=200)
+    ap.add_argument("--systems", type=int, default=8, help="c5: systems taken from the 512")
+    ap.add_argument("--max-l", type=int, default=24, help="c5: largest cross-section taken")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU time of the baseline sample")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true", help="run a 2 GiB device copy and exit")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}): the engine has no multi-GPU path; "
+              "independent points would shard by energy across GPUs.", file=sys.stderr)
+        sys.exit(2)
+    if args.copy_calibration:
+        copy_calibration()
+        return
+    defaults = {"c2": ("double", 8, 33), "c3": ("mixed", 32, 256), "c4": ("double", 1, 1), "c5": ("double", 1, 4)}
+    p, c, e = defaults[args.config]
+    args.precision = args.precision or p
+    args.configs = args.configs or c
+    args.energies = args.energies or e
+
+    import caroli_amd as ca
+
+    if not ca.device_available():
+        raise SystemExit("no HIP device: the engine has no CPU path")
+    line = run_adjoint(args, ca) if args.config == "c4" else run_transmission(args, ca)
+    line.update({"n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "scaling": "weak", "vs_baseline": None,
+                 "data": "synthetic (seeded disorder, host-built Hamiltonians)"})
+    keys = ["metric", "value", "unit", "n_gpus", "steps", "warmup", "ms_per_step", "higher_is_better", "scaling",
+            "vs_baseline", "dtype", "data", "config"]
+    print(json.dumps({**{k: line[k] for k in keys}, **{k: v for k, v in line.items() if k not in keys}}), flush=True)
 
 
 if __name__ == "__main__":

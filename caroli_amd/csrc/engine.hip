@@ -422,6 +422,7 @@ void upload_plan(const HostPlan& hp, DevPlan& dp, const SparseMatrix& h, hipStre
 struct Inputs {
     DevBuf vals_ptrs, z, sigma, gamma, w, wcols;
     int64_t stride = 0;
+    int32_t cl_max = 0, cr_max = 0;  // largest channel counts of the points
 };
 
 /// Kernel families timed by cg_kernel_timing (border_kernel stands for factor_global_kernel when
@@ -438,12 +439,13 @@ enum Family {
     kRootBorder,
     kRootSchur,
     kTrace,
+    kSolve,
     kFamilies
 };
 const char* const kFamilyNames[kFamilies] = {"front_lds_kernel",    "gather_kernel",         "assemble_kernel",
                                              "f11_kernel",          "trsm_kernel",           "border_kernel",
                                              "schur_kernel",        "assemble_kernel<root>", "border_kernel<root>",
-                                             "schur_kernel<root>",  "trace_kernel"};
+                                             "schur_kernel<root>",  "trace_kernel",          "solve_kernels"};
 
 struct Engine {
     std::mutex mu;
@@ -451,7 +453,7 @@ struct Engine {
     double tol_single = kTolSingle;  // pivot test of complex f32 fronts in the current run
     bool ok = false;
     std::map<uint64_t, std::weak_ptr<Structure>> registry;
-    DevBuf arena, root_arena, piv, status, tout;
+    DevBuf arena, root_arena, piv, status, tout, trace_x, trace_p;
     Inputs scratch_in;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     // Per-kernel timing.
@@ -564,6 +566,7 @@ void stage_points(Engine& eng, const std::vector<const cg_system*>& sys, const s
                   int nl, int nr, Inputs& in, PointsView& pv) {
     const size_t P = sys.size();
     const int64_t stride = int64_t(nl) * nl + int64_t(nr) * nr;
+    in.cl_max = in.cr_max = 0;
     std::vector<const double2*> vp(P);
     std::vector<double2> z(P), sg(P * stride), gm(P * stride), wv(P * stride, double2{0.0, 0.0});
     std::vector<int32_t> wc(2 * P);
@@ -586,6 +589,8 @@ void stage_points(Engine& eng, const std::vector<const cg_system*>& sys, const s
         for (size_t e = 0; e < blk.wr.a.size(); ++e) o[e] = double2{blk.wr.a[e].real(), blk.wr.a[e].imag()};
         wc[2 * p] = blk.wl.cols;
         wc[2 * p + 1] = blk.wr.cols;
+        in.cl_max = std::max(in.cl_max, blk.wl.cols);
+        in.cr_max = std::max(in.cr_max, blk.wr.cols);
     }
     upload(in.vals_ptrs, vp, eng.stream);
     upload(in.z, z, eng.stream);
@@ -655,8 +660,24 @@ void run_level(Engine& eng, const LevelSet& L, const DevLevel& D, const NodeDesc
     });
 }
 
+/// The trace of the root's border block for the P points of `pv` (scratch of the GEMM path from
+/// the engine).
+void launch_trace_for(Engine& eng, const NodeDesc* root, int32_t nl, const Inputs& in, const PointsView& pv,
+                      int32_t P) {
+    double2* xbuf = nullptr;
+    double* partial = nullptr;
+    if (std::max(in.cl_max, in.cr_max) > kTraceFewChannels) {
+        eng.trace_x.ensure(size_t(P) * trace_x_words(nl, in.cr_max) * sizeof(double2));
+        eng.trace_p.ensure(size_t(P) * trace_partials(in.cl_max, in.cr_max) * sizeof(double));
+        xbuf = eng.trace_x.as<double2>();
+        partial = eng.trace_p.as<double>();
+    }
+    launch_trace(root, nl, in.cl_max, in.cr_max, xbuf, partial, pv, P, eng.stream);
+}
+
 /// One sweep over the static plan of block b for the points of `pv`.
-void sweep(Engine& eng, const HostPlan& hp, const DevPlan& dp, const PointsView& pv, int32_t P, bool single) {
+void sweep(Engine& eng, const HostPlan& hp, const DevPlan& dp, const PointsView& pv, int32_t P, bool single,
+           const Inputs& in) {
     const NodeDesc* nodes = dp.nodes.as<NodeDesc>();
     const Entry* ent = dp.ent.as<Entry>();
     const int32_t* diag = dp.diag.as<int32_t>();
@@ -669,7 +690,7 @@ void sweep(Engine& eng, const HostPlan& hp, const DevPlan& dp, const PointsView&
     const NodeDesc& r = hp.nodes[hp.root];
     const double nl = r.nl, nr = r.nr;
     eng.timed(kTrace, 0.0, P * 16.0 * (nl * nl + nr * nr + nl * nr), [&] {
-        launch_trace(nodes + hp.root, r.nl, pv, P, eng.stream);
+        launch_trace_for(eng, nodes + hp.root, r.nl, in, pv, P);
     });
     HIP_CHECK(hipGetLastError());
 }
@@ -854,7 +875,7 @@ int32_t replanned_point(Engine& eng, const cg_system* s, const cg_sigma* g, int 
             upload(dt, tl, eng.stream);
             run_level(eng, L, resolve(o, dl, dt), dd, dent.as<Entry>(), ddiag.as<int32_t>(), dkids.as<KidDesc>(),
                       dmaps.as<int32_t>(), pv, 1, single, true);
-            launch_trace(dd, nodes[0].nl, pv, 1, eng.stream);
+            launch_trace_for(eng, dd, nodes[0].nl, eng.scratch_in, pv, 1);
             HIP_CHECK(hipGetLastError());
             int32_t rs = 0;
             HIP_CHECK(hipMemcpyAsync(&rs, static_cast<int32_t*>(status.p) + nn, sizeof(int32_t), hipMemcpyDeviceToHost, eng.stream));
@@ -877,12 +898,16 @@ void solve(Engine& eng, const DevPlan& dp, const PointsView& pv, const SolveView
            bool transpose, bool kept_only) {
     const NodeDesc* nodes = dp.nodes.as<NodeDesc>();
     const int32_t* lv = dp.levels.as<int32_t>();
-    if (!kept_only)
-        for (size_t l = 0; l < dp.all.size(); ++l)
-            launch_solve_forward(nodes, lv + dp.all[l].first, dp.all[l].second, pv, sv, P, single, transpose, eng.stream);
-    launch_solve_root(nodes, lv + dp.root_at, pv, sv, P, transpose, eng.stream);
-    for (size_t l = dp.all.size(); l-- > 0;)
-        launch_solve_backward(nodes, lv + dp.all[l].first, dp.all[l].second, pv, sv, P, single, transpose, eng.stream);
+    eng.timed(kSolve, 0.0, 0.0, [&] {
+        if (!kept_only)
+            for (size_t l = 0; l < dp.all.size(); ++l)
+                launch_solve_forward(nodes, lv + dp.all[l].first, dp.all[l].second, pv, sv, P, single, transpose,
+                                     eng.stream);
+        launch_solve_root(nodes, lv + dp.root_at, pv, sv, P, transpose, eng.stream);
+        for (size_t l = dp.all.size(); l-- > 0;)
+            launch_solve_backward(nodes, lv + dp.all[l].first, dp.all[l].second, pv, sv, P, single, transpose,
+                                  eng.stream);
+    });
     HIP_CHECK(hipGetLastError());
 }
 
@@ -1105,7 +1130,7 @@ int32_t run_batch(cg_batch* bt, double* out_t, int32_t* out_status, cg_batch_sta
             pv.t_out = eng.tout.as<double>();
             pv.keep = mixed ? 1 : 0;  // the refinement solves with the factors
             HIP_CHECK(hipEventRecord(eng.ev0, eng.stream));
-            sweep(eng, hp, dp, pv, int32_t(P), single);
+            sweep(eng, hp, dp, pv, int32_t(P), single, u.in);
             HIP_CHECK(hipEventRecord(eng.ev1, eng.stream));
             std::vector<double> t(P);
             std::vector<int32_t> st(P * size_t(hp.slots));
@@ -1184,6 +1209,11 @@ int32_t run_batch(cg_batch* bt, double* out_t, int32_t* out_status, cg_batch_sta
             out_t[again[q]] = t[q];
             status[again[q]] = st[q];
         }
+    }
+    {
+        std::lock_guard<std::mutex> lock(eng.mu);
+        HIP_CHECK(hipStreamSynchronize(eng.stream));
+        eng.collect();  // timers of the refinement and of re-planned points
     }
     for (int64_t p = 0; p < count; ++p) {
         if (status[p] != CG_OK) out_t[p] = NAN;
@@ -1357,10 +1387,11 @@ int32_t backward(const cg_system* s, const cg_sigma* g, double* out_t, double* o
         pv.status_stride = hp.slots;
         pv.t_out = eng.tout.as<double>();
         pv.keep = 1;  // the adjoint solves with the factors
-        sweep(eng, hp, dp, pv, 1, false);
+        sweep(eng, hp, dp, pv, 1, false, eng.scratch_in);
         std::vector<int32_t> st(hp.slots);
         HIP_CHECK(hipMemcpyAsync(st.data(), eng.status.p, st.size() * 4, hipMemcpyDeviceToHost, eng.stream));
         HIP_CHECK(hipStreamSynchronize(eng.stream));
+        eng.collect();
         for (int32_t k = 0; k < hp.root; ++k)
             if (st[k] != kFrontOk) return CG_ERR_INVALID;  // delayed separators: not supported by the adjoint
         if (st[hp.root] != kFrontOk) return CG_ERR_SINGULAR;
@@ -1384,6 +1415,7 @@ int32_t backward(const cg_system* s, const cg_sigma* g, double* out_t, double* o
             host.resize(size_t(n_orb) * c);
             HIP_CHECK(hipMemcpyAsync(host.data(), xb.p, bytes, hipMemcpyDeviceToHost, eng.stream));
             HIP_CHECK(hipStreamSynchronize(eng.stream));
+            eng.collect();
         };
         std::vector<double2> psi, phi;
         columns(sb.wr, nr, dp.right_orbs, false, psi);  // psi = G E_R W_R

@@ -13,6 +13,7 @@
 //    block is -G_LR; trace_kernel evaluates Tr[Gamma_L G Gamma_R G^H] from it.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdlib>
 #include <cstring>
 #include <mutex>
@@ -1142,6 +1143,147 @@ __global__ void __launch_bounds__(kThreads) trace_w_kernel(const NodeDesc* __res
     }
 }
 
+// Trace as two batched complex GEMMs on MFMA when the leads carry many channels:
+//   X = G_LR W_R (nl x cr)   then   T = |W_L^H X|_F^2,
+// 64 x 64 tiles per workgroup as in schur_kernel (four 32 x 32 wave quarters, K staged through LDS
+// in chunks of kChunk). W columns past a point's channel count are zero, so the tiles run to the
+// largest count of the batch.
+template <bool kConjA>
+__device__ __forceinline__ void tile_zgemm(const double2* __restrict__ A, int lda, const double2* __restrict__ B,
+                                           int ldb, int M, int N, int K, int i0, int j0, double2* As, double2* Bs,
+                                           f64x4 (&cr)[2][2], f64x4 (&ci)[2][2]) {
+    using M4 = Mfma<double2>;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wi = (wave & 1) * 32, wj = (wave >> 1) * 32;
+    const double2 zero = cx<double2>(0.0, 0.0);
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y) {
+            cr[x][y] = f64x4{0, 0, 0, 0};
+            ci[x][y] = f64x4{0, 0, 0, 0};
+        }
+    for (int kc = 0; kc < K; kc += kChunk) {
+        for (int e = tid; e < kTile * kChunk; e += 256) {
+            // As[k][i] = op(A)(i0 + i, kc + k); clamped loads, values selected.
+            int r, c;
+            if (kConjA) {
+                c = e % kChunk;
+                r = e / kChunk;
+            } else {
+                r = e % kTile;
+                c = e / kTile;
+            }
+            const int i = min(i0 + r, M - 1), k = min(kc + c, K - 1);
+            double2 v = kConjA ? A[k + int64_t(i) * lda] : A[i + int64_t(k) * lda];
+            if (kConjA) v.y = -v.y;
+            if (!(i0 + r < M && kc + c < K)) v = zero;
+            As[c * kTileLd + r] = v;
+        }
+        for (int e = tid; e < kTile * kChunk; e += 256) {
+            const int c = e % kChunk, r = e / kChunk;
+            const int j = min(j0 + r, N - 1), k = min(kc + c, K - 1);
+            double2 v = B[k + int64_t(j) * ldb];
+            if (!(j0 + r < N && kc + c < K)) v = zero;
+            Bs[c * kTileLd + r] = v;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < kChunk; kk += 4) {
+            const int k = kk + lane / 16;
+            double2 a[2], b[2];
+#pragma unroll
+            for (int s = 0; s < 2; ++s) {
+                a[s] = Bs[k * kTileLd + wj + 16 * s + lane % 16];
+                b[s] = As[k * kTileLd + wi + 16 * s + lane % 16];
+            }
+#pragma unroll
+            for (int x = 0; x < 2; ++x)
+#pragma unroll
+                for (int y = 0; y < 2; ++y) {
+                    cr[x][y] = M4::mac(a[x].x, b[y].x, cr[x][y]);
+                    cr[x][y] = M4::mac(-a[x].y, b[y].y, cr[x][y]);
+                    ci[x][y] = M4::mac(a[x].x, b[y].y, ci[x][y]);
+                    ci[x][y] = M4::mac(a[x].y, b[y].x, ci[x][y]);
+                }
+        }
+        __syncthreads();
+    }
+}
+
+// X[p] = G_LR W_R; grid (tiles of nl x cr, points).
+__global__ void __launch_bounds__(256)
+    trace_x_kernel(const NodeDesc* __restrict__ rootp, PointsView pv, double2* __restrict__ X, int cr_max, int tiles_j) {
+    __shared__ double2 As[kChunk * kTileLd];
+    __shared__ double2 Bs[kChunk * kTileLd];
+    const NodeDesc nd = *rootp;
+    const int64_t p = blockIdx.y;
+    const int nl = nd.nl, nr = nd.nr, n = nd.n, k0 = nd.ne;
+    const int i0 = kTile * (blockIdx.x / tiles_j), j0 = kTile * (blockIdx.x % tiles_j);
+    const double2* G = reinterpret_cast<const double2*>(pv.root_arena) + p * pv.root_stride + nd.off + k0 +
+                       int64_t(k0) * n;  // the border block: -G_LR
+    const double2* WR = pv.w + p * pv.sigma_stride + int64_t(nl) * nl;
+    f64x4 cr[2][2], ci[2][2];
+    tile_zgemm<false>(G, n, WR, nr, nl, cr_max, nr, i0, j0, As, Bs, cr, ci);
+    double2* Xp = X + p * int64_t(nl) * cr_max;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, wi = (wave & 1) * 32, wj = (wave >> 1) * 32;
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y) {
+            const int i = i0 + wi + 16 * y + lane % 16;
+#pragma unroll
+            for (int jj = 0; jj < 4; ++jj) {
+                const int j = j0 + wj + 16 * x + Mfma<double2>::row(lane, jj);
+                if (i < nl && j < cr_max) Xp[i + int64_t(j) * nl] = double2{cr[x][y][jj], ci[x][y][jj]};
+            }
+        }
+}
+
+// partial[p][tile] = sum of |W_L^H X|^2 over the tile; grid (tiles of cl x cr, points).
+__global__ void __launch_bounds__(256)
+    trace_y_kernel(const NodeDesc* __restrict__ rootp, PointsView pv, const double2* __restrict__ X, int cl_max,
+                   int cr_max, int tiles_j, double* __restrict__ partial) {
+    __shared__ double2 As[kChunk * kTileLd];
+    __shared__ double2 Bs[kChunk * kTileLd];
+    __shared__ double sv[4];
+    const NodeDesc nd = *rootp;
+    const int64_t p = blockIdx.y;
+    const int nl = nd.nl;
+    const int i0 = kTile * (blockIdx.x / tiles_j), j0 = kTile * (blockIdx.x % tiles_j);
+    const double2* WL = pv.w + p * pv.sigma_stride;
+    f64x4 cr[2][2], ci[2][2];
+    tile_zgemm<true>(WL, nl, X + p * int64_t(nl) * cr_max, nl, cl_max, cr_max, nl, i0, j0, As, Bs, cr, ci);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, wi = (wave & 1) * 32, wj = (wave >> 1) * 32;
+    double acc = 0.0;
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y) {
+            const int i = i0 + wi + 16 * y + lane % 16;
+#pragma unroll
+            for (int jj = 0; jj < 4; ++jj) {
+                const int j = j0 + wj + 16 * x + Mfma<double2>::row(lane, jj);
+                if (i < cl_max && j < cr_max) acc += cr[x][y][jj] * cr[x][y][jj] + ci[x][y][jj] * ci[x][y][jj];
+            }
+        }
+    const double t = block_sum(acc, sv);
+    if (threadIdx.x == 0) partial[p * gridDim.x + blockIdx.x] = t;
+}
+
+__global__ void __launch_bounds__(64)
+    trace_sum_kernel(const NodeDesc* __restrict__ rootp, PointsView pv, const double* __restrict__ partial, int tiles) {
+    const NodeDesc nd = *rootp;
+    const int64_t p = blockIdx.x;
+    double acc = 0.0;
+    for (int t = threadIdx.x; t < tiles; t += 64) acc += partial[p * tiles + t];
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+    if (threadIdx.x == 0) {
+        const int32_t st = pv.status[p * pv.status_stride + nd.slot];
+        pv.t_out[p] = st == kFrontOk ? acc : NAN;
+    }
+}
+
 inline dim3 grid2(int32_t count, int32_t points) { return dim3(unsigned(count), unsigned(points)); }
 
 // Dynamic LDS above the 64 KB default (gfx950 has 160 KB per workgroup), once per kernel.
@@ -1321,8 +1463,19 @@ void launch_factor_unblocked(const NodeDesc* nodes, const int32_t* level, int32_
                            level, pv, tol);
 }
 
-void launch_trace(const NodeDesc* root, int32_t nl, const PointsView& pv, int32_t points, hipStream_t st) {
+void launch_trace(const NodeDesc* root, int32_t nl, int32_t cl_max, int32_t cr_max, double2* xbuf, double* partial,
+                  const PointsView& pv, int32_t points, hipStream_t st) {
     if (points <= 0) return;
+    if (std::max(cl_max, cr_max) > kTraceFewChannels && xbuf && partial) {
+        const int ti = (nl + kTile - 1) / kTile, tj = (cr_max + kTile - 1) / kTile;
+        hipLaunchKernelGGL(trace_x_kernel, dim3(unsigned(ti * tj), unsigned(points)), dim3(256), 0, st, root, pv, xbuf,
+                           cr_max, tj);
+        const int yi = (cl_max + kTile - 1) / kTile;
+        hipLaunchKernelGGL(trace_y_kernel, dim3(unsigned(yi * tj), unsigned(points)), dim3(256), 0, st, root, pv, xbuf,
+                           cl_max, cr_max, tj, partial);
+        hipLaunchKernelGGL(trace_sum_kernel, dim3(unsigned(points)), dim3(64), 0, st, root, pv, partial, yi * tj);
+        return;
+    }
     if (nl <= kTraceRows) {
         allow_lds(reinterpret_cast<const void*>(trace_w_kernel));
         hipLaunchKernelGGL(trace_w_kernel, dim3(unsigned(points)), dim3(kThreads), kTraceLds, st, root, pv);

@@ -131,9 +131,18 @@ void launch_factor_unblocked(const NodeDesc* nodes, const int32_t* level, int32_
                              int32_t points, double tol, bool single, bool root, hipStream_t st);
 bool unblocked_fronts();
 /// Caroli trace of the root border block: t_out[p] = Tr[Gamma_L G Gamma_R G^H], evaluated as
-/// |W_L^H G_LR W_R|_F^2 (nl <= kTraceRows), else from Gamma directly.
+/// |W_L^H G_LR W_R|_F^2 with cl_max / cr_max the largest channel counts of the launch: two batched
+/// MFMA GEMMs (scratch xbuf of points x nl x cr_max, partial of points x tiles) past
+/// kTraceFewChannels channels, else one workgroup per point (nl <= kTraceRows) or from Gamma.
 constexpr int kTraceRows = 4096;
-void launch_trace(const NodeDesc* root, int32_t nl, const PointsView& pv, int32_t points, hipStream_t st);
+constexpr int kTraceFewChannels = 16;
+void launch_trace(const NodeDesc* root, int32_t nl, int32_t cl_max, int32_t cr_max, double2* xbuf, double* partial,
+                  const PointsView& pv, int32_t points, hipStream_t st);
+/// Sizes of the scratch of launch_trace's GEMM path.
+inline size_t trace_x_words(int32_t nl, int32_t cr_max) { return size_t(nl) * size_t(cr_max); }
+inline size_t trace_partials(int32_t cl_max, int32_t cr_max) {
+    return size_t((cl_max + kTile - 1) / kTile) * size_t((cr_max + kTile - 1) / kTile);
+}
 
 /// Largest ne * c of a front handled by the solve kernels (LDS words of complex f64).
 constexpr int kSolveLds = 4096;
