@@ -161,7 +161,8 @@ def cpu_baseline(pts_sys, pts_sig, pts_e, t_gpu, args, threads):
     cpu = {"value": sample / cpu_s, "unit": "points/s", "cores": threads, "kind": "port",
            "sample": f"{sample} of the {npts} points (evenly spaced), FP64 oracle, one point per thread",
            "single_thread_s_per_point": lat}
-    return cpu, float(np.max(np.abs(tc - t_gpu[idx])))
+    d = np.abs(tc - t_gpu[idx])
+    return cpu, float(np.max(d)), float(np.max(d / (1.0 + np.abs(tc))))
 
 
 def run_transmission(args, ca):
@@ -183,7 +184,7 @@ def run_transmission(args, ca):
     ks = ca.kernel_stats()
     ca.kernel_timing(False)
     step = float(np.mean(times))
-    cpu, max_dt = (None, None) if args.no_cpu else cpu_baseline(pts_sys, pts_sig, pts_e, t, args, threads)
+    cpu, max_dt, max_rel = (None, None, None) if args.no_cpu else cpu_baseline(pts_sys, pts_sig, pts_e, t, args, threads)
     flops_pt = np.mean([s.work()[0] for s in pts_sys])
     bytes_pt = np.mean([s.work()[1] for s in pts_sys])
     return {
@@ -198,6 +199,7 @@ def run_transmission(args, ca):
         "roofline": roofline_of(ks, args.precision, workload_tag(args)),
         "cpu_baseline": cpu,
         "max_abs_dT_vs_cpu": max_dt,
+        "max_dT_over_1_plus_T_vs_cpu": max_rel,
         "status_ok": int((st == 0).sum()),
         "delayed_points": int(stats.delayed_points),
         "refinement_iters": int(stats.refinement_iters),
