@@ -447,13 +447,22 @@ const char* const kFamilyNames[kFamilies] = {"front_lds_kernel",    "gather_kern
                                              "schur_kernel",        "assemble_kernel<root>", "border_kernel<root>",
                                              "schur_kernel<root>",  "trace_kernel",          "solve_kernels"};
 
+/// A stream with the device buffers of one sweep in flight: the arenas, pivots, statuses, results
+/// and the trace scratch. Two lanes let the sweeps of two launch units (spin blocks, systems of
+/// other sizes) run at once: their top tree levels hold few fronts each and leave CUs idle alone.
+struct Lane {
+    hipStream_t stream = nullptr;
+    DevBuf arena, root_arena, piv, status, tout, trace_x, trace_p;
+};
+constexpr int kLanes = 2;
+
 struct Engine {
     std::mutex mu;
-    hipStream_t stream = nullptr;
     double tol_single = kTolSingle;  // pivot test of complex f32 fronts in the current run
     bool ok = false;
     std::map<uint64_t, std::weak_ptr<Structure>> registry;
-    DevBuf arena, root_arena, piv, status, tout, trace_x, trace_p;
+    Lane lanes[kLanes];
+    Lane* lane = &lanes[0];  // the lane work is issued to (lane 0 outside the concurrent sweeps)
     Inputs scratch_in;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     // Per-kernel timing.
@@ -473,7 +482,8 @@ struct Engine {
     Engine() {
         int count = 0;
         if (hipGetDeviceCount(&count) != hipSuccess || count == 0) return;
-        if (hipStreamCreateWithFlags(&stream, hipStreamNonBlocking) != hipSuccess) return;
+        for (auto& l : lanes)
+            if (hipStreamCreateWithFlags(&l.stream, hipStreamNonBlocking) != hipSuccess) return;
         if (hipEventCreate(&ev0) != hipSuccess || hipEventCreate(&ev1) != hipSuccess) return;
         ok = true;
     }
@@ -499,9 +509,9 @@ struct Engine {
             return;
         }
         hipEvent_t a = take(), b = take();
-        HIP_CHECK(hipEventRecord(a, stream));
+        HIP_CHECK(hipEventRecord(a, lane->stream));
         launch();
-        HIP_CHECK(hipEventRecord(b, stream));
+        HIP_CHECK(hipEventRecord(b, lane->stream));
         pending.push_back({fam, a, b, flops, bytes});
     }
     /// Folds the finished timed launches into the counters (after a stream synchronisation).
@@ -533,8 +543,8 @@ std::shared_ptr<Structure> structure_of(Engine& eng, const cg_system* s) {
     auto sp = std::make_shared<Structure>();
     for (auto& blk : s->sym.blocks) sp->host.push_back(build_plan(blk, s->spec.h));
     sp->dev.resize(sp->host.size());
-    for (size_t b = 0; b < sp->host.size(); ++b) upload_plan(sp->host[b], sp->dev[b], s->spec.h, eng.stream);
-    HIP_CHECK(hipStreamSynchronize(eng.stream));
+    for (size_t b = 0; b < sp->host.size(); ++b) upload_plan(sp->host[b], sp->dev[b], s->spec.h, eng.lane->stream);
+    HIP_CHECK(hipStreamSynchronize(eng.lane->stream));
     eng.registry[s->key] = sp;
     s->structure = sp;
     return sp;
@@ -592,12 +602,12 @@ void stage_points(Engine& eng, const std::vector<const cg_system*>& sys, const s
         in.cl_max = std::max(in.cl_max, blk.wl.cols);
         in.cr_max = std::max(in.cr_max, blk.wr.cols);
     }
-    upload(in.vals_ptrs, vp, eng.stream);
-    upload(in.z, z, eng.stream);
-    upload(in.sigma, sg, eng.stream);
-    upload(in.gamma, gm, eng.stream);
-    upload(in.w, wv, eng.stream);
-    upload(in.wcols, wc, eng.stream);
+    upload(in.vals_ptrs, vp, eng.lane->stream);
+    upload(in.z, z, eng.lane->stream);
+    upload(in.sigma, sg, eng.lane->stream);
+    upload(in.gamma, gm, eng.lane->stream);
+    upload(in.w, wv, eng.lane->stream);
+    upload(in.wcols, wc, eng.lane->stream);
     in.stride = stride;
     bind_inputs(in, pv);
 }
@@ -616,7 +626,7 @@ void run_level(Engine& eng, const LevelSet& L, const DevLevel& D, const NodeDesc
         const double words = L.lds_words[c] + (pv.keep ? L.lds_f_words[c] : L.lds_s_words[c]);
         eng.timed(kFrontLds, P * L.lds_flops[c], P * es * words, [&] {
             launch_front_lds(nodes, D.lds[c], count, L.lds_n[c], ent, diag, kids, maps, pv, P, tol, single, true,
-                             eng.stream);
+                             eng.lane->stream);
         });
     }
     const int32_t count = int32_t(L.large.size());
@@ -625,38 +635,38 @@ void run_level(Engine& eng, const LevelSet& L, const DevLevel& D, const NodeDesc
     if (root || unblocked) {  // the whole front assembled in HBM
         eng.timed(root ? kRootAssemble : kAssemble, 0.0, P * fes * L.asm_words, [&] {
             if (root)
-                launch_assemble_root(nodes, D.large, count, ent, diag, kids, maps, pv, P, single ? 1 : 0, eng.stream);
+                launch_assemble_root(nodes, D.large, count, ent, diag, kids, maps, pv, P, single ? 1 : 0, eng.lane->stream);
             else if (single)
-                launch_assemble_c64(nodes, D.large, count, ent, diag, kids, maps, pv, P, eng.stream);
+                launch_assemble_c64(nodes, D.large, count, ent, diag, kids, maps, pv, P, eng.lane->stream);
             else
-                launch_assemble_c128(nodes, D.large, count, ent, diag, kids, maps, pv, P, eng.stream);
+                launch_assemble_c128(nodes, D.large, count, ent, diag, kids, maps, pv, P, eng.lane->stream);
         });
     } else {  // only the border; F22 is gathered by the Schur tiles
         eng.timed(kGather, 0.0, P * fes * L.gather_words, [&] {
-            launch_gather(nodes, D.gtiles, int32_t(L.gtiles.size()), kids, maps, pv, P, fsingle, eng.stream);
+            launch_gather(nodes, D.gtiles, int32_t(L.gtiles.size()), kids, maps, pv, P, fsingle, eng.lane->stream);
         });
     }
     if (unblocked) {
         eng.timed(root ? kRootBorder : kBorder, P * L.border_flops, P * fes * L.border_words, [&] {
-            launch_factor_unblocked(nodes, D.large, count, pv, P, tol, fsingle, root, eng.stream);
+            launch_factor_unblocked(nodes, D.large, count, pv, P, tol, fsingle, root, eng.lane->stream);
         });
         return;
     }
     if (!L.f11.empty()) {
         eng.timed(kF11, P * L.f11_flops, P * fes * L.f11_words, [&] {
-            launch_f11(nodes, D.f11, int32_t(L.f11.size()), L.f11_ne, ent, diag, pv, P, tol, fsingle, eng.stream);
+            launch_f11(nodes, D.f11, int32_t(L.f11.size()), L.f11_ne, ent, diag, pv, P, tol, fsingle, eng.lane->stream);
         });
         eng.timed(kTrsm, P * L.trsm_flops, P * fes * L.trsm_words, [&] {
-            launch_trsm(nodes, D.ttiles, int32_t(L.ttiles.size()), L.f11_ne, pv, P, fsingle, eng.stream);
+            launch_trsm(nodes, D.ttiles, int32_t(L.ttiles.size()), L.f11_ne, pv, P, fsingle, eng.lane->stream);
         });
     }
     if (!L.big.empty())
         eng.timed(root ? kRootBorder : kBorder, P * L.border_flops, P * fes * L.border_words, [&] {
             launch_border(nodes, D.big, int32_t(L.big.size()), L.big_ne, ent, diag, pv, P, tol, fsingle, root,
-                          eng.stream);
+                          eng.lane->stream);
         });
     eng.timed(root ? kRootSchur : kSchur, P * L.schur_flops, P * fes * L.schur_words, [&] {
-        launch_schur(nodes, D.tiles, int32_t(L.tiles.size()), kids, maps, pv, P, fsingle, root, !root, eng.stream);
+        launch_schur(nodes, D.tiles, int32_t(L.tiles.size()), kids, maps, pv, P, fsingle, root, !root, eng.lane->stream);
     });
 }
 
@@ -667,12 +677,12 @@ void launch_trace_for(Engine& eng, const NodeDesc* root, int32_t nl, const Input
     double2* xbuf = nullptr;
     double* partial = nullptr;
     if (std::max(in.cl_max, in.cr_max) > kTraceFewChannels) {
-        eng.trace_x.ensure(size_t(P) * trace_x_words(nl, in.cr_max) * sizeof(double2));
-        eng.trace_p.ensure(size_t(P) * trace_partials(in.cl_max, in.cr_max) * sizeof(double));
-        xbuf = eng.trace_x.as<double2>();
-        partial = eng.trace_p.as<double>();
+        eng.lane->trace_x.ensure(size_t(P) * trace_x_words(nl, in.cr_max) * sizeof(double2));
+        eng.lane->trace_p.ensure(size_t(P) * trace_partials(in.cl_max, in.cr_max) * sizeof(double));
+        xbuf = eng.lane->trace_x.as<double2>();
+        partial = eng.lane->trace_p.as<double>();
     }
-    launch_trace(root, nl, in.cl_max, in.cr_max, xbuf, partial, pv, P, eng.stream);
+    launch_trace(root, nl, in.cl_max, in.cr_max, xbuf, partial, pv, P, eng.lane->stream);
 }
 
 /// One sweep over the static plan of block b for the points of `pv`.
@@ -721,7 +731,7 @@ int32_t replanned_point(Engine& eng, const cg_system* s, const cg_sigma* g, int 
         if (size_t(need_bytes) <= buf.bytes) return;
         DevBuf nb;
         nb.ensure(size_t(std::max<int64_t>(need_bytes, 2 * int64_t(buf.bytes))));
-        if (keep_bytes > 0) HIP_CHECK(hipMemcpyAsync(nb.p, buf.p, size_t(keep_bytes), hipMemcpyDeviceToDevice, eng.stream));
+        if (keep_bytes > 0) HIP_CHECK(hipMemcpyAsync(nb.p, buf.p, size_t(keep_bytes), hipMemcpyDeviceToDevice, eng.lane->stream));
         std::swap(buf.p, nb.p);
         std::swap(buf.bytes, nb.bytes);
     };
@@ -816,11 +826,11 @@ int32_t replanned_point(Engine& eng, const cg_system* s, const cg_sigma* g, int 
         grow(piv, piv_used * 4, prev_piv * 4);
         pv.arena = arena.p;
         pv.piv = piv.as<int32_t>();
-        upload(dn, nodes, eng.stream);
-        upload(dent, ent, eng.stream);
-        upload(ddiag, diag, eng.stream);
-        upload(dkids, kids, eng.stream);
-        upload(dmaps, maps, eng.stream);
+        upload(dn, nodes, eng.lane->stream);
+        upload(dent, ent, eng.lane->stream);
+        upload(ddiag, diag, eng.lane->stream);
+        upload(dkids, kids, eng.lane->stream);
+        upload(dmaps, maps, eng.lane->stream);
         const NodeDesc* dd = dn.as<NodeDesc>();
         if (!root_level) {
             // The same kernels as the static sweep, on this level's re-planned fronts.
@@ -829,13 +839,13 @@ int32_t replanned_point(Engine& eng, const cg_system* s, const cg_sigma* g, int 
             std::vector<Tile> tl;
             SetOffsets o = append_set(L, lv, tl);
             DevBuf dl, dt;
-            upload(dl, lv, eng.stream);
-            upload(dt, tl, eng.stream);
+            upload(dl, lv, eng.lane->stream);
+            upload(dt, tl, eng.lane->stream);
             run_level(eng, L, resolve(o, dl, dt), dd, dent.as<Entry>(), ddiag.as<int32_t>(), dkids.as<KidDesc>(),
                       dmaps.as<int32_t>(), pv, 1, single, false);
             HIP_CHECK(hipGetLastError());
-            HIP_CHECK(hipMemcpyAsync(st_host.data(), status.p, sizeof(int32_t) * (nn + 1), hipMemcpyDeviceToHost, eng.stream));
-            HIP_CHECK(hipStreamSynchronize(eng.stream));
+            HIP_CHECK(hipMemcpyAsync(st_host.data(), status.p, sizeof(int32_t) * (nn + 1), hipMemcpyDeviceToHost, eng.lane->stream));
+            HIP_CHECK(hipStreamSynchronize(eng.lane->stream));
             // A delayed front moves up as assembled (nested.rs:371-372): the fronts whose pivot test
             // failed part-way are assembled again over their partial elimination.
             std::vector<int32_t> failed;
@@ -843,15 +853,15 @@ int32_t replanned_point(Engine& eng, const cg_system* s, const cg_sigma* g, int 
                 if (st_host[which[q]] != kFrontOk) failed.push_back(int32_t(q));
             if (!failed.empty()) {
                 DevBuf df;
-                upload(df, failed, eng.stream);
+                upload(df, failed, eng.lane->stream);
                 if (single)
                     launch_assemble_c64(dd, df.as<int32_t>(), int32_t(failed.size()), dent.as<Entry>(), ddiag.as<int32_t>(),
-                                        dkids.as<KidDesc>(), dmaps.as<int32_t>(), pv, 1, eng.stream);
+                                        dkids.as<KidDesc>(), dmaps.as<int32_t>(), pv, 1, eng.lane->stream);
                 else
                     launch_assemble_c128(dd, df.as<int32_t>(), int32_t(failed.size()), dent.as<Entry>(), ddiag.as<int32_t>(),
-                                         dkids.as<KidDesc>(), dmaps.as<int32_t>(), pv, 1, eng.stream);
+                                         dkids.as<KidDesc>(), dmaps.as<int32_t>(), pv, 1, eng.lane->stream);
                 HIP_CHECK(hipGetLastError());
-                HIP_CHECK(hipStreamSynchronize(eng.stream));
+                HIP_CHECK(hipStreamSynchronize(eng.lane->stream));
             }
             for (size_t q = 0; q < nodes.size(); ++q) {
                 int k = which[q];
@@ -871,16 +881,16 @@ int32_t replanned_point(Engine& eng, const cg_system* s, const cg_sigma* g, int 
             std::vector<Tile> tl;
             SetOffsets o = append_set(L, lv, tl);
             DevBuf dl, dt;
-            upload(dl, lv, eng.stream);
-            upload(dt, tl, eng.stream);
+            upload(dl, lv, eng.lane->stream);
+            upload(dt, tl, eng.lane->stream);
             run_level(eng, L, resolve(o, dl, dt), dd, dent.as<Entry>(), ddiag.as<int32_t>(), dkids.as<KidDesc>(),
                       dmaps.as<int32_t>(), pv, 1, single, true);
             launch_trace_for(eng, dd, nodes[0].nl, eng.scratch_in, pv, 1);
             HIP_CHECK(hipGetLastError());
             int32_t rs = 0;
-            HIP_CHECK(hipMemcpyAsync(&rs, static_cast<int32_t*>(status.p) + nn, sizeof(int32_t), hipMemcpyDeviceToHost, eng.stream));
-            HIP_CHECK(hipMemcpyAsync(&t_out, tout.p, sizeof(double), hipMemcpyDeviceToHost, eng.stream));
-            HIP_CHECK(hipStreamSynchronize(eng.stream));
+            HIP_CHECK(hipMemcpyAsync(&rs, static_cast<int32_t*>(status.p) + nn, sizeof(int32_t), hipMemcpyDeviceToHost, eng.lane->stream));
+            HIP_CHECK(hipMemcpyAsync(&t_out, tout.p, sizeof(double), hipMemcpyDeviceToHost, eng.lane->stream));
+            HIP_CHECK(hipStreamSynchronize(eng.lane->stream));
             if (rs != kFrontOk) {
                 t_out = NAN;
                 return CG_ERR_SINGULAR;
@@ -902,11 +912,11 @@ void solve(Engine& eng, const DevPlan& dp, const PointsView& pv, const SolveView
         if (!kept_only)
             for (size_t l = 0; l < dp.all.size(); ++l)
                 launch_solve_forward(nodes, lv + dp.all[l].first, dp.all[l].second, pv, sv, P, single, transpose,
-                                     eng.stream);
-        launch_solve_root(nodes, lv + dp.root_at, pv, sv, P, transpose, eng.stream);
+                                     eng.lane->stream);
+        launch_solve_root(nodes, lv + dp.root_at, pv, sv, P, transpose, eng.lane->stream);
         for (size_t l = dp.all.size(); l-- > 0;)
             launch_solve_backward(nodes, lv + dp.all[l].first, dp.all[l].second, pv, sv, P, single, transpose,
-                                  eng.stream);
+                                  eng.lane->stream);
     });
     HIP_CHECK(hipGetLastError());
 }
@@ -948,20 +958,20 @@ Refined refine(Engine& eng, const HostPlan& hp, const DevPlan& dp, const PointsV
                 w[size_t(p) * nr * c + a + size_t(col) * nr] = double2{wr(a, col).real(), wr(a, col).imag()};
     }
     DevBuf dw, xb, bb, db, rb, gb;
-    upload(dw, w, eng.stream);
+    upload(dw, w, eng.lane->stream);
     const size_t bytes = size_t(P) * size_t(n_orb) * c * sizeof(double2);
     xb.ensure(bytes);
     bb.ensure(bytes);
     db.ensure(bytes);
     rb.ensure(size_t(P) * sizeof(double));
     gb.ensure(size_t(P) * nl * c * sizeof(double2));
-    HIP_CHECK(hipMemsetAsync(bb.p, 0, bytes, eng.stream));
+    HIP_CHECK(hipMemsetAsync(bb.p, 0, bytes, eng.lane->stream));
     SolveView xv{xb.as<double2>(), n_orb * c, int32_t(n_orb), c, dp.vars.as<int32_t>()};
     SolveView bv = xv, dv = xv;
     bv.x = bb.as<double2>();
     dv.x = db.as<double2>();
-    launch_scatter(bv, dp.right_orbs.as<int32_t>(), nr, dw.as<double2>(), int64_t(nr) * c, false, P, eng.stream);
-    HIP_CHECK(hipMemcpyAsync(xb.p, bb.p, bytes, hipMemcpyDeviceToDevice, eng.stream));
+    launch_scatter(bv, dp.right_orbs.as<int32_t>(), nr, dw.as<double2>(), int64_t(nr) * c, false, P, eng.lane->stream);
+    HIP_CHECK(hipMemcpyAsync(xb.p, bb.p, bytes, hipMemcpyDeviceToDevice, eng.lane->stream));
     solve(eng, dp, pv, xv, P, single, false, true);
     std::vector<double> ratio(P, INFINITY), last(P, INFINITY);
     std::vector<char> gave_up(P, 0);
@@ -969,12 +979,12 @@ Refined refine(Engine& eng, const HostPlan& hp, const DevPlan& dp, const PointsV
         launch_residual(pv, xv, bb.as<double2>(), db.as<double2>(), dp.rows.as<int32_t>(), nrows,
                         dp.row_ptr.as<int64_t>(), dp.cols.as<uint32_t>(), dp.left_pos.as<int32_t>(),
                         dp.right_pos.as<int32_t>(), dp.left_orbs.as<int32_t>(), nl, dp.right_orbs.as<int32_t>(), nr, P,
-                        eng.stream);
+                        eng.lane->stream);
         solve(eng, dp, pv, dv, P, single, false, false);
-        launch_update(xv, db.as<double2>(), dp.rows.as<int32_t>(), nrows, rb.as<double>(), P, eng.stream);
+        launch_update(xv, db.as<double2>(), dp.rows.as<int32_t>(), nrows, rb.as<double>(), P, eng.lane->stream);
         HIP_CHECK(hipGetLastError());
-        HIP_CHECK(hipMemcpyAsync(ratio.data(), rb.p, P * sizeof(double), hipMemcpyDeviceToHost, eng.stream));
-        HIP_CHECK(hipStreamSynchronize(eng.stream));
+        HIP_CHECK(hipMemcpyAsync(ratio.data(), rb.p, P * sizeof(double), hipMemcpyDeviceToHost, eng.lane->stream));
+        HIP_CHECK(hipStreamSynchronize(eng.lane->stream));
         ++out.iters;
         // Points whose factors are unusable (skip), or whose correction stopped shrinking, are left
         // to the complex f64 recomputation and no longer hold the loop.
@@ -991,10 +1001,10 @@ Refined refine(Engine& eng, const HostPlan& hp, const DevPlan& dp, const PointsV
         }
         if (done) break;
     }
-    launch_gather(xv, dp.left_orbs.as<int32_t>(), nl, gb.as<double2>(), P, eng.stream);
+    launch_gather(xv, dp.left_orbs.as<int32_t>(), nl, gb.as<double2>(), P, eng.lane->stream);
     std::vector<double2> xl(size_t(P) * nl * c);
-    HIP_CHECK(hipMemcpyAsync(xl.data(), gb.p, xl.size() * sizeof(double2), hipMemcpyDeviceToHost, eng.stream));
-    HIP_CHECK(hipStreamSynchronize(eng.stream));
+    HIP_CHECK(hipMemcpyAsync(xl.data(), gb.p, xl.size() * sizeof(double2), hipMemcpyDeviceToHost, eng.lane->stream));
+    HIP_CHECK(hipStreamSynchronize(eng.lane->stream));
     for (int32_t p = 0; p < P; ++p) {
         out.ok[p] = !skip[p] && !gave_up[p] && ratio[p] <= kRefineTol;
         if (out.ok[p]) out.worst = std::max(out.worst, ratio[p]);
@@ -1040,6 +1050,7 @@ struct cg_batch {
     int32_t precision = CG_DOUBLE;
     std::vector<Unit> units;
     double flops = 0, bytes = 0;
+    int lanes = 1;  // units in flight at once (caroli::gpu::kLanes when their buffers fit)
 };
 
 namespace caroli::gpu {
@@ -1057,6 +1068,10 @@ int32_t create_batch(const cg_system* const* systems, const cg_sigma* const* sig
     const size_t es = single ? sizeof(float2) : sizeof(double2);
     std::map<uint64_t, std::vector<int64_t>> groups;
     for (int64_t p = 0; p < count; ++p) groups[systems[p]->key].push_back(p);
+    struct Need {
+        size_t arena, root, piv, status, tout;
+    };
+    std::vector<Need> need;  // device bytes of each unit
     for (auto& [key, idx] : groups) {
         auto sp = structure_of(eng, systems[idx[0]]);
         for (auto p : idx) structure_of(eng, systems[p]);
@@ -1085,14 +1100,33 @@ int32_t create_batch(const cg_system* const* systems, const cg_sigma* const* sig
                 }
                 PointsView pv{};
                 stage_points(eng, ss, gg, int(b), int(blk.left_orbitals.size()), int(blk.right_orbitals.size()), u.in, pv);
-                eng.arena.ensure(P * size_t(hp.arena_stride) * es);
-                eng.root_arena.ensure(P * size_t(hp.root_stride) * sizeof(double2));
-                eng.piv.ensure(P * size_t(hp.piv_stride) * 4);
-                eng.status.ensure(P * size_t(hp.slots) * 4);
-                eng.tout.ensure(P * sizeof(double));
+                need.push_back({P * size_t(hp.arena_stride) * es, P * size_t(hp.root_stride) * sizeof(double2),
+                                P * size_t(hp.piv_stride) * 4, P * size_t(hp.slots) * 4, P * sizeof(double)});
                 bt->units.push_back(std::move(u));
             }
         }
+    }
+    // Lanes: two when the two largest units fit the concurrent budget together (and the mode runs
+    // no refinement, which works unit by unit).
+    size_t big1 = 0, big2 = 0;
+    for (auto& nd : need) {
+        const size_t t = nd.arena + nd.root + nd.piv + nd.status + nd.tout;
+        if (t > big1) {
+            big2 = big1;
+            big1 = t;
+        } else if (t > big2) {
+            big2 = t;
+        }
+    }
+    const size_t concurrent_budget = size_t(160) << 30;
+    bt->lanes = (precision != CG_MIXED && need.size() > 1 && big1 + big2 <= concurrent_budget) ? kLanes : 1;
+    for (size_t ui = 0; ui < need.size(); ++ui) {
+        Lane& ln = eng.lanes[ui % size_t(bt->lanes)];
+        ln.arena.ensure(need[ui].arena);
+        ln.root_arena.ensure(need[ui].root);
+        ln.piv.ensure(need[ui].piv);
+        ln.status.ensure(need[ui].status);
+        ln.tout.ensure(need[ui].tout);
     }
     *out = bt.release();
     return CG_OK;
@@ -1113,76 +1147,102 @@ int32_t run_batch(cg_batch* bt, double* out_t, int32_t* out_status, cg_batch_sta
     {
         std::lock_guard<std::mutex> lock(eng.mu);
         eng.tol_single = mixed ? kTolMixed : kTolSingle;
-        for (auto& u : bt->units) {
-            const HostPlan& hp = u.structure->host[u.block];
-            const DevPlan& dp = u.structure->dev[u.block];
-            const size_t P = u.points.size();
-            PointsView pv{};
-            bind_inputs(u.in, pv);
-            pv.arena = eng.arena.p;
-            pv.arena_stride = hp.arena_stride;
-            pv.root_arena = eng.root_arena.p;
-            pv.root_stride = hp.root_stride;
-            pv.piv = eng.piv.as<int32_t>();
-            pv.piv_stride = hp.piv_stride;
-            pv.status = eng.status.as<int32_t>();
-            pv.status_stride = hp.slots;
-            pv.t_out = eng.tout.as<double>();
-            pv.keep = mixed ? 1 : 0;  // the refinement solves with the factors
-            HIP_CHECK(hipEventRecord(eng.ev0, eng.stream));
-            sweep(eng, hp, dp, pv, int32_t(P), single, u.in);
-            HIP_CHECK(hipEventRecord(eng.ev1, eng.stream));
-            std::vector<double> t(P);
-            std::vector<int32_t> st(P * size_t(hp.slots));
-            HIP_CHECK(hipMemcpyAsync(t.data(), eng.tout.p, P * sizeof(double), hipMemcpyDeviceToHost, eng.stream));
-            HIP_CHECK(hipMemcpyAsync(st.data(), eng.status.p, st.size() * 4, hipMemcpyDeviceToHost, eng.stream));
-            HIP_CHECK(hipStreamSynchronize(eng.stream));
+        // Units go out in groups of bt->lanes, one lane (stream and buffers) each, so that their
+        // sweeps overlap; results are read back once the group is done.
+        // Per-kernel timing runs the units one at a time, so that each timed launch has the GPU to
+        // itself and its duration is its own.
+        const size_t L = (mixed || eng.timing) ? 1 : size_t(bt->lanes);
+        for (size_t u0 = 0; u0 < bt->units.size(); u0 += L) {
+            const size_t u1 = std::min(u0 + L, bt->units.size());
+            std::vector<PointsView> pvs(u1 - u0);
+            std::vector<std::vector<double>> ts(u1 - u0);
+            std::vector<std::vector<int32_t>> sts(u1 - u0);
+            auto w0 = std::chrono::steady_clock::now();
+            for (size_t ui = u0; ui < u1; ++ui) {
+                auto& u = bt->units[ui];
+                const HostPlan& hp = u.structure->host[u.block];
+                const DevPlan& dp = u.structure->dev[u.block];
+                eng.lane = &eng.lanes[ui - u0];
+                PointsView& pv = pvs[ui - u0];
+                bind_inputs(u.in, pv);
+                pv.arena = eng.lane->arena.p;
+                pv.arena_stride = hp.arena_stride;
+                pv.root_arena = eng.lane->root_arena.p;
+                pv.root_stride = hp.root_stride;
+                pv.piv = eng.lane->piv.as<int32_t>();
+                pv.piv_stride = hp.piv_stride;
+                pv.status = eng.lane->status.as<int32_t>();
+                pv.status_stride = hp.slots;
+                pv.t_out = eng.lane->tout.as<double>();
+                pv.keep = mixed ? 1 : 0;  // the refinement solves with the factors
+                sweep(eng, hp, dp, pv, int32_t(u.points.size()), single, u.in);
+            }
+            for (size_t ui = u0; ui < u1; ++ui) {
+                auto& u = bt->units[ui];
+                const HostPlan& hp = u.structure->host[u.block];
+                Lane& ln = eng.lanes[ui - u0];
+                const size_t P = u.points.size();
+                ts[ui - u0].resize(P);
+                sts[ui - u0].resize(P * size_t(hp.slots));
+                HIP_CHECK(hipMemcpyAsync(ts[ui - u0].data(), ln.tout.p, P * sizeof(double), hipMemcpyDeviceToHost,
+                                         ln.stream));
+                HIP_CHECK(hipMemcpyAsync(sts[ui - u0].data(), ln.status.p, sts[ui - u0].size() * 4,
+                                         hipMemcpyDeviceToHost, ln.stream));
+            }
+            for (size_t ui = u0; ui < u1; ++ui) HIP_CHECK(hipStreamSynchronize(eng.lanes[ui - u0].stream));
+            eng.lane = &eng.lanes[0];
             eng.collect();
-            float ms = 0.f;
-            HIP_CHECK(hipEventElapsedTime(&ms, eng.ev0, eng.ev1));
-            local.seconds += ms * 1e-3;
-            if (mixed) {
-                auto t0 = std::chrono::steady_clock::now();
-                std::vector<const cg_sigma*> sig;
-                for (auto p : u.points) sig.push_back(bt->sigmas[p]);
-                std::vector<char> skip(P, 0);
-                for (size_t q = 0; q < P; ++q)
-                    for (int32_t k = 0; k <= hp.root; ++k) skip[q] |= st[q * hp.slots + k] != kFrontOk;
-                Refined rf = refine(eng, hp, dp, pv, sig, u.block, int64_t(bt->systems[u.points[0]]->spec.h.n), true,
-                                    skip);
-                local.seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-                local.refinement_iters += rf.iters;
+            local.seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - w0).count();
+            for (size_t ui = u0; ui < u1; ++ui) {
+                auto& u = bt->units[ui];
+                const HostPlan& hp = u.structure->host[u.block];
+                const DevPlan& dp = u.structure->dev[u.block];
+                const size_t P = u.points.size();
+                const std::vector<double>& t = ts[ui - u0];
+                const std::vector<int32_t>& st = sts[ui - u0];
+                if (mixed) {
+                    auto t0 = std::chrono::steady_clock::now();
+                    std::vector<const cg_sigma*> sig;
+                    for (auto p : u.points) sig.push_back(bt->sigmas[p]);
+                    std::vector<char> skip(P, 0);
+                    for (size_t q = 0; q < P; ++q)
+                        for (int32_t k = 0; k <= hp.root; ++k) skip[q] |= st[q * hp.slots + k] != kFrontOk;
+                    Refined rf = refine(eng, hp, dp, pvs[ui - u0], sig, u.block,
+                                        int64_t(bt->systems[u.points[0]]->spec.h.n), true, skip);
+                    local.seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+                    local.refinement_iters += rf.iters;
+                    for (size_t q = 0; q < P; ++q) {
+                        int64_t p = u.points[q];
+                        bool delayed = false;
+                        for (int32_t k = 0; k < hp.root; ++k) delayed |= st[q * hp.slots + k] != kFrontOk;
+                        if (delayed || st[q * hp.slots + hp.root] != kFrontOk || !rf.ok[q] || !std::isfinite(rf.t[q])) {
+                            redo[p] = 1;
+                            local.delayed_points += delayed;
+                        } else {
+                            local.max_refine_resid = std::max(local.max_refine_resid, rf.worst);
+                            out_t[p] += rf.t[q];
+                        }
+                    }
+                    continue;
+                }
                 for (size_t q = 0; q < P; ++q) {
                     int64_t p = u.points[q];
                     bool delayed = false;
+                    int32_t root_status = st[q * hp.slots + hp.root];
                     for (int32_t k = 0; k < hp.root; ++k) delayed |= st[q * hp.slots + k] != kFrontOk;
-                    if (delayed || st[q * hp.slots + hp.root] != kFrontOk || !rf.ok[q] || !std::isfinite(rf.t[q])) {
-                        redo[p] = 1;
-                        local.delayed_points += delayed;
+                    if (delayed) {
+                        double tb = NAN;
+                        auto t0 = std::chrono::steady_clock::now();
+                        int32_t rc = replanned_point(eng, bt->systems[p], bt->sigmas[p], u.block, single, tb);
+                        local.seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+                        ++local.delayed_points;
+                        if (rc != CG_OK) status[p] = rc;
+                        out_t[p] += tb;
+                    } else if (root_status != kFrontOk || !std::isfinite(t[q])) {
+                        status[p] = CG_ERR_SINGULAR;
                     } else {
-                        local.max_refine_resid = std::max(local.max_refine_resid, rf.worst);
-                        out_t[p] += rf.t[q];
+                        out_t[p] += t[q];
                     }
-                }
-                continue;
-            }
-            for (size_t q = 0; q < P; ++q) {
-                int64_t p = u.points[q];
-                bool delayed = false;
-                int32_t root_status = st[q * hp.slots + hp.root];
-                for (int32_t k = 0; k < hp.root; ++k) delayed |= st[q * hp.slots + k] != kFrontOk;
-                if (delayed) {
-                    double tb = NAN;
-                    auto t0 = std::chrono::steady_clock::now();
-                    int32_t rc = replanned_point(eng, bt->systems[p], bt->sigmas[p], u.block, single, tb);
-                    local.seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-                    ++local.delayed_points;
-                    if (rc != CG_OK) status[p] = rc;
-                    out_t[p] += tb;
-                } else if (root_status != kFrontOk || !std::isfinite(t[q])) {
-                    status[p] = CG_ERR_SINGULAR;
-                } else {
-                    out_t[p] += t[q];
                 }
             }
         }
@@ -1212,7 +1272,7 @@ int32_t run_batch(cg_batch* bt, double* out_t, int32_t* out_status, cg_batch_sta
     }
     {
         std::lock_guard<std::mutex> lock(eng.mu);
-        HIP_CHECK(hipStreamSynchronize(eng.stream));
+        HIP_CHECK(hipStreamSynchronize(eng.lane->stream));
         eng.collect();  // timers of the refinement and of re-planned points
     }
     for (int64_t p = 0; p < count; ++p) {
@@ -1252,14 +1312,14 @@ int32_t front_schur(const double* f, int64_t n, int64_t ne, int64_t ne1, int32_t
     if (single) {
         std::vector<float2> h(nn);
         for (size_t e = 0; e < nn; ++e) h[e] = float2{float(f[2 * e]), float(f[2 * e + 1])};
-        upload(fb, h, eng.stream);
+        upload(fb, h, eng.lane->stream);
     } else {
         std::vector<double2> h(nn);
         for (size_t e = 0; e < nn; ++e) h[e] = double2{f[2 * e], f[2 * e + 1]};
-        upload(fb, h, eng.stream);
+        upload(fb, h, eng.lane->stream);
     }
-    upload(nodes, std::vector<NodeDesc>{d}, eng.stream);
-    upload(lv, std::vector<int32_t>{0}, eng.stream);
+    upload(nodes, std::vector<NodeDesc>{d}, eng.lane->stream);
+    upload(lv, std::vector<int32_t>{0}, eng.lane->stream);
     pivb.ensure(size_t(std::max<int64_t>(ne, 1)) * 4);
     stb.ensure(4);
     PointsView pv{};
@@ -1272,39 +1332,39 @@ int32_t front_schur(const double* f, int64_t n, int64_t ne, int64_t ne1, int32_t
     const double tol = single ? kTolSingle : kTolDouble;
     if (small) {
         launch_front_lds(nodes.as<NodeDesc>(), lv.as<int32_t>(), 1, int32_t(n), nullptr, nullptr, nullptr, nullptr, pv, 1,
-                         tol, single, false, eng.stream);
+                         tol, single, false, eng.lane->stream);
     } else if (unblocked_fronts()) {
-        launch_factor_unblocked(nodes.as<NodeDesc>(), lv.as<int32_t>(), 1, pv, 1, tol, single, false, eng.stream);
+        launch_factor_unblocked(nodes.as<NodeDesc>(), lv.as<int32_t>(), 1, pv, 1, tol, single, false, eng.lane->stream);
     } else {
         std::vector<Tile> tl, tt;
         for (int i0 = 0; i0 < n - ne; i0 += kTile)
             for (int j0 = 0; j0 < n - ne; j0 += kTile) tl.push_back({0, i0, j0, 0});
-        upload(tiles, tl, eng.stream);
+        upload(tiles, tl, eng.lane->stream);
         if (ne <= kF11Lds && !border_only()) {
             for (int o = 0; o < n - ne; o += kTrsmSpan) {
                 tt.push_back({0, 0, o, 0});
                 tt.push_back({0, o, 0, 1});
             }
-            upload(ttiles, tt, eng.stream);
+            upload(ttiles, tt, eng.lane->stream);
             launch_f11(nodes.as<NodeDesc>(), lv.as<int32_t>(), 1, int32_t(ne), nullptr, nullptr, pv, 1, tol, single,
-                       eng.stream);
+                       eng.lane->stream);
             launch_trsm(nodes.as<NodeDesc>(), ttiles.as<Tile>(), int32_t(tt.size()), int32_t(ne), pv, 1, single,
-                        eng.stream);
+                        eng.lane->stream);
         } else {
             launch_border(nodes.as<NodeDesc>(), lv.as<int32_t>(), 1, int32_t(ne), nullptr, nullptr, pv, 1, tol, single,
-                          false, eng.stream);
+                          false, eng.lane->stream);
         }
         launch_schur(nodes.as<NodeDesc>(), tiles.as<Tile>(), int32_t(tl.size()), nullptr, nullptr, pv, 1, single, false,
-                     false, eng.stream);
+                     false, eng.lane->stream);
     }
     HIP_CHECK(hipGetLastError());
     int32_t st = 0;
-    HIP_CHECK(hipMemcpyAsync(&st, stb.p, 4, hipMemcpyDeviceToHost, eng.stream));
+    HIP_CHECK(hipMemcpyAsync(&st, stb.p, 4, hipMemcpyDeviceToHost, eng.lane->stream));
     const int64_t nb = n - ne;
     if (single) {
         std::vector<float2> h(nn);
-        HIP_CHECK(hipMemcpyAsync(h.data(), fb.p, nn * sizeof(float2), hipMemcpyDeviceToHost, eng.stream));
-        HIP_CHECK(hipStreamSynchronize(eng.stream));
+        HIP_CHECK(hipMemcpyAsync(h.data(), fb.p, nn * sizeof(float2), hipMemcpyDeviceToHost, eng.lane->stream));
+        HIP_CHECK(hipStreamSynchronize(eng.lane->stream));
         for (int64_t j = 0; j < nb; ++j)
             for (int64_t i = 0; i < nb; ++i) {
                 float2 v = h[size_t(ne + i) + size_t(ne + j) * size_t(n)];
@@ -1313,8 +1373,8 @@ int32_t front_schur(const double* f, int64_t n, int64_t ne, int64_t ne1, int32_t
             }
     } else {
         std::vector<double2> h(nn);
-        HIP_CHECK(hipMemcpyAsync(h.data(), fb.p, nn * sizeof(double2), hipMemcpyDeviceToHost, eng.stream));
-        HIP_CHECK(hipStreamSynchronize(eng.stream));
+        HIP_CHECK(hipMemcpyAsync(h.data(), fb.p, nn * sizeof(double2), hipMemcpyDeviceToHost, eng.lane->stream));
+        HIP_CHECK(hipStreamSynchronize(eng.lane->stream));
         for (int64_t j = 0; j < nb; ++j)
             for (int64_t i = 0; i < nb; ++i) {
                 double2 v = h[size_t(ne + i) + size_t(ne + j) * size_t(n)];
@@ -1372,25 +1432,25 @@ int32_t backward(const cg_system* s, const cg_sigma* g, double* out_t, double* o
         const auto& sb = g->blocks[b];
         PointsView pv{};
         stage_points(eng, {s}, {g}, int(b), int(blk.left_orbitals.size()), int(blk.right_orbitals.size()), eng.scratch_in, pv);
-        eng.arena.ensure(size_t(hp.arena_stride) * sizeof(double2));
-        eng.root_arena.ensure(size_t(hp.root_stride) * sizeof(double2));
-        eng.piv.ensure(size_t(hp.piv_stride) * 4);
-        eng.status.ensure(size_t(hp.slots) * 4);
-        eng.tout.ensure(sizeof(double));
-        pv.arena = eng.arena.p;
+        eng.lane->arena.ensure(size_t(hp.arena_stride) * sizeof(double2));
+        eng.lane->root_arena.ensure(size_t(hp.root_stride) * sizeof(double2));
+        eng.lane->piv.ensure(size_t(hp.piv_stride) * 4);
+        eng.lane->status.ensure(size_t(hp.slots) * 4);
+        eng.lane->tout.ensure(sizeof(double));
+        pv.arena = eng.lane->arena.p;
         pv.arena_stride = hp.arena_stride;
-        pv.root_arena = eng.root_arena.p;
+        pv.root_arena = eng.lane->root_arena.p;
         pv.root_stride = hp.root_stride;
-        pv.piv = eng.piv.as<int32_t>();
+        pv.piv = eng.lane->piv.as<int32_t>();
         pv.piv_stride = hp.piv_stride;
-        pv.status = eng.status.as<int32_t>();
+        pv.status = eng.lane->status.as<int32_t>();
         pv.status_stride = hp.slots;
-        pv.t_out = eng.tout.as<double>();
+        pv.t_out = eng.lane->tout.as<double>();
         pv.keep = 1;  // the adjoint solves with the factors
         sweep(eng, hp, dp, pv, 1, false, eng.scratch_in);
         std::vector<int32_t> st(hp.slots);
-        HIP_CHECK(hipMemcpyAsync(st.data(), eng.status.p, st.size() * 4, hipMemcpyDeviceToHost, eng.stream));
-        HIP_CHECK(hipStreamSynchronize(eng.stream));
+        HIP_CHECK(hipMemcpyAsync(st.data(), eng.lane->status.p, st.size() * 4, hipMemcpyDeviceToHost, eng.lane->stream));
+        HIP_CHECK(hipStreamSynchronize(eng.lane->stream));
         eng.collect();
         for (int32_t k = 0; k < hp.root; ++k)
             if (st[k] != kFrontOk) return CG_ERR_INVALID;  // delayed separators: not supported by the adjoint
@@ -1405,16 +1465,16 @@ int32_t backward(const cg_system* s, const cg_sigma* g, double* out_t, double* o
             for (int col = 0; col < c; ++col)
                 for (int a = 0; a < m; ++a) wh[a + size_t(col) * m] = double2{w(a, col).real(), w(a, col).imag()};
             DevBuf dw, xb;
-            upload(dw, wh, eng.stream);
+            upload(dw, wh, eng.lane->stream);
             const size_t bytes = size_t(n_orb) * c * sizeof(double2);
             xb.ensure(bytes);
-            HIP_CHECK(hipMemsetAsync(xb.p, 0, bytes, eng.stream));
+            HIP_CHECK(hipMemsetAsync(xb.p, 0, bytes, eng.lane->stream));
             SolveView xv{xb.as<double2>(), n_orb * c, int32_t(n_orb), c, dp.vars.as<int32_t>()};
-            launch_scatter(xv, orbs.as<int32_t>(), m, dw.as<double2>(), int64_t(m) * c, transpose, 1, eng.stream);
+            launch_scatter(xv, orbs.as<int32_t>(), m, dw.as<double2>(), int64_t(m) * c, transpose, 1, eng.lane->stream);
             solve(eng, dp, pv, xv, 1, false, transpose, true);
             host.resize(size_t(n_orb) * c);
-            HIP_CHECK(hipMemcpyAsync(host.data(), xb.p, bytes, hipMemcpyDeviceToHost, eng.stream));
-            HIP_CHECK(hipStreamSynchronize(eng.stream));
+            HIP_CHECK(hipMemcpyAsync(host.data(), xb.p, bytes, hipMemcpyDeviceToHost, eng.lane->stream));
+            HIP_CHECK(hipStreamSynchronize(eng.lane->stream));
             eng.collect();
         };
         std::vector<double2> psi, phi;

@@ -147,8 +147,10 @@ int32_t cg_batch_create(const cg_system* const* systems, const cg_sigma* const* 
 int32_t cg_batch_run(cg_batch* batch, double* out_t, int32_t* out_status, cg_batch_stats* stats);
 void cg_batch_destroy(cg_batch* batch);
 
-/* Per-kernel timing with HIP events on the engine's stream (enable = 1 resets the counters).
- * Each record: kernel family name, launches, total seconds, algorithmic flops and bytes. */
+/* Per-kernel timing with HIP events on the engine's streams (enable = 1 resets the counters).
+ * Each record: kernel family name, launches, total seconds, algorithmic flops and bytes. While
+ * timing is on, the launch units of a batch run one after another instead of two at a time, so
+ * that every timed launch has the GPU to itself. */
 typedef struct cg_kernel_stat {
   char name[48];
   int64_t launches;
