@@ -92,11 +92,58 @@ struct LevelSet {
     std::vector<int32_t> f11, big;
     int32_t f11_ne = 0, big_ne = 0;
     std::vector<Tile> ttiles;  // trsm tiles of the f11 fronts
+    // Blocked fronts (largest separator first): per step s of kBlk columns, the first blk_count[s]
+    // of them take part; their row and update tiles are rtiles / utiles [off[s], off[s + 1]).
+    std::vector<int32_t> blk, blk_count, rt_off, ut_off;
+    std::vector<Tile> rtiles, utiles;
+    std::vector<double> panel_flops, panel_words, row_flops, row_words, upd_flops, upd_words;
     // Algorithmic work (complex words, flops) for the kernel timers.
     double lds_flops[kClasses] = {}, lds_words[kClasses] = {}, lds_s_words[kClasses] = {}, lds_f_words[kClasses] = {};
     double asm_words = 0, gather_words = 0, border_flops = 0, border_words = 0, schur_flops = 0, schur_words = 0;
     double f11_flops = 0, f11_words = 0, trsm_flops = 0, trsm_words = 0;
+    /// Lists the blocked steps of `blk` (sorted here by separator size, largest first).
+    void plan_blocked(const std::vector<NodeDesc>& nodes);
 };
+
+void LevelSet::plan_blocked(const std::vector<NodeDesc>& nodes) {
+    std::stable_sort(blk.begin(), blk.end(), [&](int32_t a, int32_t b) { return nodes[a].ne > nodes[b].ne; });
+    const int steps = blk.empty() ? 0 : (nodes[blk[0]].ne + kBlk - 1) / kBlk;
+    rt_off.assign(1, 0);
+    ut_off.assign(1, 0);
+    for (int s = 0; s < steps; ++s) {
+        const int k0 = s * kBlk;
+        int32_t count = 0;
+        double pf = 0, pw = 0, rf = 0, rw = 0, uf = 0, uw = 0;
+        for (int32_t q : blk) {
+            const NodeDesc& d = nodes[q];
+            if (d.ne <= k0) break;
+            ++count;
+            const int w = std::min(kBlk, d.ne - k0), k1 = k0 + w, n = d.n, ne = d.ne;
+            const double R = n - k0;
+            pf += 8.0 * (R * w * w / 2.0);
+            pw += 2.0 * R * w;
+            for (int j0 = 0; j0 < k0; j0 += kRowCols) rtiles.push_back({q, k0, j0, 0});
+            for (int j0 = k1; j0 < n; j0 += kRowCols) rtiles.push_back({q, k0, j0, 1});
+            rf += 8.0 * w * w / 2.0 * (n - k1);
+            rw += 2.0 * w * (n - k1) + 4.0 * w * k0;
+            for (int j0 = k1; j0 < n; j0 += kTile)
+                for (int i0 = k1; i0 < n; i0 += kTile)
+                    if (i0 < ne || j0 < ne) utiles.push_back({q, i0, j0, 0});
+            const double area = double(n - k1) * (ne - k1) + double(ne - k1) * (n - ne);
+            uf += 8.0 * w * area;
+            uw += 2.0 * area + w * (2.0 * n - k1 - k1);
+        }
+        blk_count.push_back(count);
+        rt_off.push_back(int32_t(rtiles.size()));
+        ut_off.push_back(int32_t(utiles.size()));
+        panel_flops.push_back(pf);
+        panel_words.push_back(pw);
+        row_flops.push_back(rf);
+        row_words.push_back(rw);
+        upd_flops.push_back(uf);
+        upd_words.push_back(uw);
+    }
+}
 
 /// Device pointers of a LevelSet's lists.
 struct DevLevel {
@@ -107,6 +154,9 @@ struct DevLevel {
     const int32_t* f11 = nullptr;
     const int32_t* big = nullptr;
     const Tile* ttiles = nullptr;
+    const int32_t* blk = nullptr;
+    const Tile* rtiles = nullptr;
+    const Tile* utiles = nullptr;
 };
 
 /// Splits the fronts `list` of `nodes` by kernel (root: the bordered root front, always large).
@@ -115,6 +165,7 @@ LevelSet classify(const std::vector<NodeDesc>& nodes, const std::vector<KidDesc>
     LevelSet L;
     const int limit = single ? kLdsFrontC64 : kLdsFrontC128;
     const bool unblocked = unblocked_fronts(), split = !border_only();
+    const int blocked_above = blocked_min_ne();
     for (int32_t q : list) {
         const NodeDesc& d = nodes[q];
         const double ne = d.ne, nb = d.n - d.ne, n2 = double(d.n) * d.n;
@@ -143,7 +194,9 @@ LevelSet classify(const std::vector<NodeDesc>& nodes, const std::vector<KidDesc>
         // tiles) in proportion to the areas.
         const double border_share = (n2 - nb * nb) / std::max(n2, 1.0);
         L.gather_words += (n2 - nb * nb) + border_share * kid_words;
-        if (!root && split && d.ne <= kF11Lds) {
+        if (d.ne > blocked_above) {
+            L.blk.push_back(q);
+        } else if (!root && split && d.ne <= kF11Lds) {
             L.f11.push_back(q);
             L.f11_ne = std::max(L.f11_ne, d.ne);
             L.f11_flops += 8.0 / 3.0 * ne * ne * ne;
@@ -169,12 +222,13 @@ LevelSet classify(const std::vector<NodeDesc>& nodes, const std::vector<KidDesc>
                 for (int i0 = 0; i0 < d.n; i0 += kTile)
                     if (i0 < d.ne || j0 < d.ne) L.gtiles.push_back({q, i0, j0, 0});
     }
+    L.plan_blocked(nodes);
     return L;
 }
 
 /// Appends a LevelSet's lists to the device list / tile arrays; `at` records the offsets.
 struct SetOffsets {
-    int32_t lds[LevelSet::kClasses], large, tiles, gtiles, f11, big, ttiles;
+    int32_t lds[LevelSet::kClasses], large, tiles, gtiles, f11, big, ttiles, blk, rtiles, utiles;
 };
 SetOffsets append_set(const LevelSet& L, std::vector<int32_t>& lv, std::vector<Tile>& tl) {
     SetOffsets o{};
@@ -194,6 +248,12 @@ SetOffsets append_set(const LevelSet& L, std::vector<int32_t>& lv, std::vector<T
     lv.insert(lv.end(), L.big.begin(), L.big.end());
     o.ttiles = int32_t(tl.size());
     tl.insert(tl.end(), L.ttiles.begin(), L.ttiles.end());
+    o.blk = int32_t(lv.size());
+    lv.insert(lv.end(), L.blk.begin(), L.blk.end());
+    o.rtiles = int32_t(tl.size());
+    tl.insert(tl.end(), L.rtiles.begin(), L.rtiles.end());
+    o.utiles = int32_t(tl.size());
+    tl.insert(tl.end(), L.utiles.begin(), L.utiles.end());
     return o;
 }
 DevLevel resolve(const SetOffsets& o, const DevBuf& lv, const DevBuf& tl) {
@@ -205,6 +265,9 @@ DevLevel resolve(const SetOffsets& o, const DevBuf& lv, const DevBuf& tl) {
     d.f11 = lv.as<int32_t>() + o.f11;
     d.big = lv.as<int32_t>() + o.big;
     d.ttiles = tl.as<Tile>() + o.ttiles;
+    d.blk = lv.as<int32_t>() + o.blk;
+    d.rtiles = tl.as<Tile>() + o.rtiles;
+    d.utiles = tl.as<Tile>() + o.utiles;
     return d;
 }
 
@@ -331,7 +394,7 @@ HostPlan build_plan(const Block& blk, const SparseMatrix& h) {
         for (size_t o : pl.vars) hp.vars.push_back(int32_t(o));
         hp.max_ne = std::max(hp.max_ne, d.ne);
         off += int64_t(d.n) * d.n;
-        piv += d.ne;
+        piv += d.ne + kPivExtra;
         hp.nodes.push_back(d);
     }
     hp.arena_stride = std::max<int64_t>(off, 1);
@@ -360,7 +423,7 @@ HostPlan build_plan(const Block& blk, const SparseMatrix& h) {
     r.var_off = int64_t(hp.vars.size());
     for (size_t o : blk.kept) hp.vars.push_back(int32_t(o));
     hp.max_ne = std::max(hp.max_ne, r.ne);
-    piv += r.ne;
+    piv += r.ne + kPivExtra;
     hp.root = nn;
     // Orbitals of the block and their slice positions (residuals of the refinement).
     for (auto& node : t.nodes) hp.rows.insert(hp.rows.end(), node.eliminate.begin(), node.eliminate.end());
@@ -440,12 +503,16 @@ enum Family {
     kRootSchur,
     kTrace,
     kSolve,
+    kBlkPanel,
+    kBlkRow,
+    kBlkUpdate,
     kFamilies
 };
 const char* const kFamilyNames[kFamilies] = {"front_lds_kernel",    "gather_kernel",         "assemble_kernel",
                                              "f11_kernel",          "trsm_kernel",           "border_kernel",
                                              "schur_kernel",        "assemble_kernel<root>", "border_kernel<root>",
-                                             "schur_kernel<root>",  "trace_kernel",          "solve_kernels"};
+                                             "schur_kernel<root>",  "trace_kernel",          "solve_kernels",
+                                             "blocked_panel_kernel", "blocked_row_kernel",   "blocked_update_kernel"};
 
 /// A stream with the device buffers of one sweep in flight: the arenas, pivots, statuses, results
 /// and the trace scratch. Two lanes let the sweeps of two launch units (spin blocks, systems of
@@ -612,6 +679,29 @@ void stage_points(Engine& eng, const std::vector<const cg_system*>& sys, const s
     bind_inputs(in, pv);
 }
 
+/// The blocked fronts of a LevelSet, step by step (their Schur blocks come with the other large
+/// fronts' tiles afterwards).
+void run_blocked(Engine& eng, const LevelSet& L, const DevLevel& D, const NodeDesc* nodes, const Entry* ent,
+                 const int32_t* diag, const PointsView& pv, int32_t P, double tol, bool fsingle, bool root) {
+    const double es = root ? 16.0 : fsingle ? 8.0 : 16.0;
+    for (size_t s = 0; s < L.blk_count.size(); ++s) {
+        const int32_t k0 = int32_t(s) * kBlk;
+        eng.timed(kBlkPanel, P * L.panel_flops[s], P * es * L.panel_words[s], [&] {
+            launch_blocked_panel(nodes, D.blk, L.blk_count[s], ent, diag, pv, P, tol, fsingle, root, k0,
+                                 eng.lane->stream);
+        });
+        const int32_t nr = L.rt_off[s + 1] - L.rt_off[s], nu = L.ut_off[s + 1] - L.ut_off[s];
+        if (nr > 0)
+            eng.timed(kBlkRow, P * L.row_flops[s], P * es * L.row_words[s], [&] {
+                launch_blocked_row(nodes, D.rtiles + L.rt_off[s], nr, pv, P, fsingle, root, k0, eng.lane->stream);
+            });
+        if (nu > 0)
+            eng.timed(kBlkUpdate, P * L.upd_flops[s], P * es * L.upd_words[s], [&] {
+                launch_blocked_update(nodes, D.utiles + L.ut_off[s], nu, pv, P, fsingle, root, k0, eng.lane->stream);
+            });
+    }
+}
+
 /// Launches the fronts of one LevelSet for the P points of `pv`: the LDS classes, then the large
 /// fronts (assembly, border, Schur tiles). `root`: the bordered root front (complex f64 whatever
 /// the precision of the fronts below it).
@@ -652,6 +742,7 @@ void run_level(Engine& eng, const LevelSet& L, const DevLevel& D, const NodeDesc
         });
         return;
     }
+    run_blocked(eng, L, D, nodes, ent, diag, pv, P, tol, fsingle, root);
     if (!L.f11.empty()) {
         eng.timed(kF11, P * L.f11_flops, P * fes * L.f11_words, [&] {
             launch_f11(nodes, D.f11, int32_t(L.f11.size()), L.f11_ne, ent, diag, pv, P, tol, fsingle, eng.lane->stream);
@@ -788,7 +879,7 @@ int32_t replanned_point(Engine& eng, const cg_system* s, const cg_sigma* g, int 
                 d.slot = k;
                 d.piv_off = piv_used;
                 used += int64_t(d.n) * d.n;
-                piv_used += ne;
+                piv_used += ne + kPivExtra;
                 d.ent_beg = int32_t(ent.size());
                 for (auto& e : separator_entries(s->spec.h, node, pos)) ent.push_back({e.a, e.b, int32_t(e.k), e.conj ? 1 : 0});
                 d.ent_end = int32_t(ent.size());
@@ -818,7 +909,7 @@ int32_t replanned_point(Engine& eng, const cg_system* s, const cg_sigma* g, int 
             d.off = 0;
             d.slot = nn;
             d.piv_off = piv_used;
-            piv_used += d.ne;
+            piv_used += d.ne + kPivExtra;
             list.push_back(0);
             nodes.push_back(d);
         }
@@ -1307,7 +1398,7 @@ int32_t front_schur(const double* f, int64_t n, int64_t ne, int64_t ne1, int32_t
     d.ne1 = int32_t(ne1);
     d.slot = 0;
     d.piv_off = 0;
-    DevBuf fb, nodes, lv, pivb, stb, tiles, ttiles;
+    DevBuf fb, nodes, lv, pivb, stb, tiles, ttiles, blists, btiles;
     const size_t nn = size_t(n) * size_t(n);
     if (single) {
         std::vector<float2> h(nn);
@@ -1320,7 +1411,7 @@ int32_t front_schur(const double* f, int64_t n, int64_t ne, int64_t ne1, int32_t
     }
     upload(nodes, std::vector<NodeDesc>{d}, eng.lane->stream);
     upload(lv, std::vector<int32_t>{0}, eng.lane->stream);
-    pivb.ensure(size_t(std::max<int64_t>(ne, 1)) * 4);
+    pivb.ensure(size_t(ne + kPivExtra) * 4);
     stb.ensure(4);
     PointsView pv{};
     pv.arena = fb.p;
@@ -1340,7 +1431,18 @@ int32_t front_schur(const double* f, int64_t n, int64_t ne, int64_t ne1, int32_t
         for (int i0 = 0; i0 < n - ne; i0 += kTile)
             for (int j0 = 0; j0 < n - ne; j0 += kTile) tl.push_back({0, i0, j0, 0});
         upload(tiles, tl, eng.lane->stream);
-        if (ne <= kF11Lds && !border_only()) {
+        if (ne > blocked_min_ne()) {
+            LevelSet L;
+            L.blk = {0};
+            L.plan_blocked({d});
+            std::vector<int32_t> lvb;
+            std::vector<Tile> tlb;
+            const SetOffsets o = append_set(L, lvb, tlb);
+            upload(blists, lvb, eng.lane->stream);
+            upload(btiles, tlb, eng.lane->stream);
+            run_blocked(eng, L, resolve(o, blists, btiles), nodes.as<NodeDesc>(), nullptr, nullptr, pv, 1, tol, single,
+                        false);
+        } else if (ne <= kF11Lds && !border_only()) {
             for (int o = 0; o < n - ne; o += kTrsmSpan) {
                 tt.push_back({0, 0, o, 0});
                 tt.push_back({0, o, 0, 1});

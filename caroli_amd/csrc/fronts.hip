@@ -1061,6 +1061,323 @@ __global__ void __launch_bounds__(256)
     }
 }
 
+// ---------------------------------------------------------------------------------------------
+// Blocked fronts (large separators): the LU of F11 and the border strips L21, U12 in steps of kBlk
+// columns, three launches per step over the fronts of a level that still have columns left, with
+// many workgroups per front for the two wide parts:
+//  - blocked_panel_kernel, one workgroup per front: the panel F[k0:n, k0:k1] (k1 = k0 + kBlk)
+//    factored with row pivoting over the separator rows and the pivot test of field.rs:61, in
+//    sub-panels of kSub columns (rank-1 updates inside a sub-panel, one rank-kSub update of the
+//    rest of the panel after it). Interchanges swap whole rows of the panel only. At k0 = 0 it
+//    first adds -H and z to the gathered border and keeps the pivot scale max|F11| after the
+//    front's pivots (two words, read by the later steps);
+//  - blocked_row_kernel, one workgroup per kRowCols columns outside the panel: the step's row
+//    interchanges as one permutation of the separator rows (all loads staged through LDS before
+//    any store), and right of the panel the U rows U[k0:k1, j] = L_kk^-1 F[k0:k1, j] solved in
+//    the registers of 8 lanes per column;
+//  - blocked_update_kernel, one workgroup per 64 x 64 tile: F -= L[:, k0:k1] U[k0:k1, :] on MFMA
+//    over the rows and columns beyond k1, except F22 (rows and columns >= ne): schur_kernel forms
+//    S = F22 - L21 U12 from the children once, with K = ne, after the last step.
+// The factors end as border_kernel leaves them (LAPACK's getrf layout with the interchanges
+// applied to whole separator rows), so the solves and the root's trace read them alike.
+constexpr int kPanelThreads = 512, kSub = 16;
+// Separators above this many variables take the blocked path by default.
+constexpr int kBlockedMinNe = 128;
+
+__device__ __forceinline__ void put_scale(int32_t* at, double s) {
+    at[0] = __double2loint(s);
+    at[1] = __double2hiint(s);
+}
+__device__ __forceinline__ double get_scale(const int32_t* at) { return __hiloint2double(at[1], at[0]); }
+
+template <class T, bool kRoot>
+__global__ void __launch_bounds__(kPanelThreads)
+    blocked_panel_kernel(const NodeDesc* __restrict__ nodes, const int32_t* __restrict__ list,
+                         const Entry* __restrict__ ent, const int32_t* __restrict__ diag, PointsView pv, double tol,
+                         int k0) {
+    constexpr int NT = kPanelThreads;
+    __shared__ double sv[NT / 64];
+    __shared__ int si[NT / 64];
+    __shared__ T su[kSub];         // the pivot row inside the sub-panel
+    __shared__ T us[kSub * kBlk];  // U rows of a sub-panel right of it: us[t * kBlk + j]
+    const NodeDesc nd = nodes[list[blockIdx.x]];
+    const int64_t p = blockIdx.y;
+    T* F = (kRoot ? reinterpret_cast<T*>(pv.root_arena) + p * pv.root_stride
+                  : reinterpret_cast<T*>(pv.arena) + p * pv.arena_stride) +
+           nd.off;
+    int32_t* piv = pv.piv + p * pv.piv_stride + nd.piv_off;
+    int32_t* status = pv.status + p * pv.status_stride + nd.slot;
+    const int n = nd.n, ne = nd.ne, ne1 = nd.ne1, tid = threadIdx.x;
+    auto at = [&](int i, int j) -> T& { return F[i + int64_t(j) * n]; };
+    double scale;
+    if (k0 == 0) {
+        if (!kRoot && nd.ent_end > nd.ent_beg) {  // -H on the separator (the children were gathered before)
+            add_entries<NT>(ent, nd.ent_beg, nd.ent_end, pv.vals[p], n,
+                            [&](int a, double2 v) { F[a] = cadd(F[a], convert<T>(v)); });
+            __syncthreads();
+        }
+        if (!kRoot && nd.diag_end > nd.diag_beg) {  // z on the separator's diagonal
+            const T z = convert<T>(pv.z[p]);
+            for (int d = nd.diag_beg + tid; d < nd.diag_end; d += NT) {
+                const int64_t q = diag[d];
+                F[q + q * n] = cadd(F[q + q * n], z);
+            }
+            __syncthreads();
+        }
+        double m = 0.0;
+        for (int idx = tid; idx < ne1 * ne1; idx += NT) m = fmax(m, cabs(at(idx % ne1, idx / ne1)));
+        scale = block_max(m, sv);
+        if (tid == 0) {
+            put_scale(piv + ne, scale);
+            *status = kFrontOk;
+        }
+    } else {
+        if (*status != kFrontOk) return;
+        scale = get_scale(piv + ne);
+    }
+    const int w = min(kBlk, ne - k0), k1 = k0 + w;
+    for (int s0 = 0; s0 < w; s0 += kSub) {
+        const int ws = min(kSub, w - s0), c0 = k0 + s0;  // sub-panel columns [c0, c0 + ws)
+        for (int c = 0; c < ws; ++c) {
+            const int k = c0 + c, lim = k < ne1 ? ne1 : ne;
+            double big = -1.0;
+            int q = k;
+            for (int i = k + tid; i < lim; i += NT) {
+                const double v = cabs(at(i, k));
+                if (v > big) {
+                    big = v;
+                    q = i;
+                }
+            }
+            block_argmax(big, q, sv, si);
+            const bool ok = k < ne1 ? (big > tol * scale) : (big > 0.0 && big < INFINITY);
+            if (!ok) {
+                if (tid == 0) *status = (k < ne1 && !kRoot) ? kFrontDelayed : kFrontSingular;
+                return;
+            }
+            if (tid == 0) piv[k] = q;
+            if (q != k && tid < w) {
+                const T x = at(k, k0 + tid);
+                at(k, k0 + tid) = at(q, k0 + tid);
+                at(q, k0 + tid) = x;
+            }
+            __syncthreads();
+            if (tid < ws - c) su[tid] = at(k, k + tid);
+            __syncthreads();
+            const T inv = crecip(su[0]);
+            for (int i = k + 1 + tid; i < n; i += NT) {
+                const T l = cmul(at(i, k), inv);
+                at(i, k) = l;
+                for (int t = 1; t < ws - c; ++t) at(i, k + t) = csub(at(i, k + t), cmul(l, su[t]));
+            }
+            __syncthreads();
+        }
+        const int r0 = c0 + ws, wr = k1 - r0;  // panel columns right of the sub-panel
+        if (wr <= 0) break;
+        if (tid < wr) {  // their sub-panel rows: unit-lower solve with L[c0:r0, c0:r0], one lane per column
+            const int j = r0 + tid;
+            T u[kSub];
+#pragma unroll
+            for (int t = 0; t < kSub; ++t) u[t] = t < ws ? at(c0 + t, j) : cx<T>(0.0, 0.0);
+#pragma unroll
+            for (int t = 1; t < kSub; ++t)
+                if (t < ws) {
+                    T acc = u[t];
+#pragma unroll
+                    for (int t2 = 0; t2 < t; ++t2) acc = csub(acc, cmul(at(c0 + t, c0 + t2), u[t2]));
+                    u[t] = acc;
+                }
+#pragma unroll
+            for (int t = 0; t < kSub; ++t) {
+                if (t < ws) at(c0 + t, j) = u[t];
+                us[t * kBlk + tid] = u[t];
+            }
+        }
+        __syncthreads();
+        for (int i = r0 + tid; i < n; i += NT) {  // rank-ws update of the rows below
+            T l[kSub];
+#pragma unroll
+            for (int t = 0; t < kSub; ++t) l[t] = t < ws ? at(i, c0 + t) : cx<T>(0.0, 0.0);
+            for (int j = 0; j < wr; ++j) {
+                T acc = at(i, r0 + j);
+#pragma unroll
+                for (int t = 0; t < kSub; ++t) acc = csub(acc, cmul(l[t], us[t * kBlk + j]));
+                at(i, r0 + j) = acc;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// The interchanges k0 + c <-> sp[c] (c < w, in order) as one permutation: new row r holds the old
+// row traced back through them.
+__device__ __forceinline__ int traced_row(int r, const int32_t* sp, int k0, int w) {
+    for (int c = w - 1; c >= 0; --c) {
+        const int a = k0 + c, b = sp[c];
+        if (r == a)
+            r = b;
+        else if (r == b)
+            r = a;
+    }
+    return r;
+}
+
+template <class T, bool kRoot>
+__global__ void __launch_bounds__(256)
+    blocked_row_kernel(const NodeDesc* __restrict__ nodes, const Tile* __restrict__ tiles, PointsView pv, int k0) {
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    __shared__ int32_t sp[kBlk], psrc[kBlk], opos[kBlk], osrc[kBlk];
+    T* X = reinterpret_cast<T*>(lds_raw);  // new panel rows, w x kRowCols; then the triangle L_kk
+    T* O = X + kBlk * kRowCols;            // new rows below the panel, w x kRowCols
+    const TilePoint tp = xcd_tile();
+    const Tile tl = tiles[tp.tile];
+    const NodeDesc nd = nodes[tl.node];
+    const int64_t p = tp.point;
+    if (pv.status[p * pv.status_stride + nd.slot] != kFrontOk) return;
+    T* F = (kRoot ? reinterpret_cast<T*>(pv.root_arena) + p * pv.root_stride
+                  : reinterpret_cast<T*>(pv.arena) + p * pv.arena_stride) +
+           nd.off;
+    const int32_t* piv = pv.piv + p * pv.piv_stride + nd.piv_off;
+    const int n = nd.n, ne = nd.ne, tid = threadIdx.x, w = min(kBlk, ne - k0), k1 = k0 + w;
+    const bool right = tl.pad != 0;
+    const int j0 = tl.j0, cols = min(kRowCols, (right ? n : k0) - j0);
+    auto at = [&](int i, int j) -> T& { return F[i + int64_t(j) * n]; };
+    if (tid < w) sp[tid] = piv[k0 + tid];
+    __syncthreads();
+    if (tid < w) {
+        psrc[tid] = traced_row(k0 + tid, sp, k0, w);
+    } else if (tid >= 64 && tid < 64 + w) {  // a row below the panel that an interchange reached
+        const int t = tid - 64, pos = sp[t];
+        bool first = pos >= k1;
+        for (int c = 0; c < t; ++c) first = first && sp[c] != pos;
+        opos[t] = first ? pos : -1;
+        osrc[t] = first ? traced_row(pos, sp, k0, w) : pos;
+    }
+    __syncthreads();
+    for (int e = tid; e < w * cols; e += 256) {
+        const int r = e % w, jc = e / w;
+        X[r + jc * kBlk] = at(psrc[r], j0 + jc);
+        if (opos[r] >= 0) O[r + jc * kBlk] = at(osrc[r], j0 + jc);
+    }
+    __syncthreads();
+    for (int e = tid; e < w * cols; e += 256) {
+        const int r = e % w, jc = e / w;
+        if (opos[r] >= 0) at(opos[r], j0 + jc) = O[r + jc * kBlk];
+        if (!right && psrc[r] != k0 + r) at(k0 + r, j0 + jc) = X[r + jc * kBlk];
+    }
+    if (!right) return;
+    // U rows: eight lanes per column, lane g holding rows g + 8 s of it.
+    const int grp = tid / 8, g = tid % 8, col = min(grp, cols - 1);
+    T x[8];
+#pragma unroll
+    for (int s = 0; s < 8; ++s) x[s] = X[min(g + 8 * s, w - 1) + col * kBlk];
+    __syncthreads();
+    constexpr int la = kBlk + 1;
+    T* A = X;  // L_kk, leading dimension la
+    for (int e = tid; e < w * w; e += 256) A[(e % w) + (e / w) * la] = at(k0 + e % w, k0 + e / w);
+    __syncthreads();
+#pragma unroll
+    for (int kb = 0; kb < 8; ++kb)
+#pragma unroll
+        for (int kk = 0; kk < 8; ++kk) {
+            const int k = 8 * kb + kk;
+            if (k < w - 1) {
+                T xk;
+                xk.x = __shfl(x[kb].x, kk, 8);
+                xk.y = __shfl(x[kb].y, kk, 8);
+#pragma unroll
+                for (int s = 0; s < 8; ++s) {
+                    const int i = g + 8 * s;
+                    if (i > k && i < w) x[s] = csub(x[s], cmul(A[i + k * la], xk));
+                }
+            }
+        }
+    if (grp < cols)
+#pragma unroll
+        for (int s = 0; s < 8; ++s)
+            if (g + 8 * s < w) at(k0 + g + 8 * s, j0 + grp) = x[s];
+}
+
+template <class T, bool kRoot>
+__global__ void __launch_bounds__(256)
+    blocked_update_kernel(const NodeDesc* __restrict__ nodes, const Tile* __restrict__ tiles, PointsView pv, int k0) {
+    using M = Mfma<T>;
+    __shared__ T As[kChunk * kTileLd];  // As[k * kTileLd + i] = L[i0 + i, k0 + kc + k]
+    __shared__ T Bs[kChunk * kTileLd];  // Bs[k * kTileLd + j] = U[k0 + kc + k, j0 + j]
+    const TilePoint tp = xcd_tile();
+    const Tile tl = tiles[tp.tile];
+    const NodeDesc nd = nodes[tl.node];
+    const int64_t p = tp.point;
+    if (pv.status[p * pv.status_stride + nd.slot] != kFrontOk) return;
+    T* F = (kRoot ? reinterpret_cast<T*>(pv.root_arena) + p * pv.root_stride
+                  : reinterpret_cast<T*>(pv.arena) + p * pv.arena_stride) +
+           nd.off;
+    const int n = nd.n, ne = nd.ne, w = min(kBlk, ne - k0), tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wi = (wave & 1) * 32, wj = (wave >> 1) * 32;
+    const T zero = cx<T>(0.0, 0.0);
+    typename M::acc cr[2][2], ci[2][2];
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y) {
+            cr[x][y] = typename M::acc{0, 0, 0, 0};
+            ci[x][y] = typename M::acc{0, 0, 0, 0};
+        }
+    for (int kc = 0; kc < w; kc += kChunk) {
+        for (int e = tid; e < kTile * kChunk; e += 256) {
+            const int r = e % kTile, c = e / kTile, i = tl.i0 + r, k = kc + c;
+            T v = F[min(i, n - 1) + int64_t(k0 + min(k, w - 1)) * n];
+            if (!(i < n && k < w)) v = zero;
+            As[c * kTileLd + r] = v;
+        }
+        for (int e = tid; e < kTile * kChunk; e += 256) {
+            const int c = e % kChunk, r = e / kChunk, j = tl.j0 + r, k = kc + c;
+            T v = F[(k0 + min(k, w - 1)) + int64_t(min(j, n - 1)) * n];
+            if (!(j < n && k < w)) v = zero;
+            Bs[c * kTileLd + r] = v;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < kChunk; kk += 4) {
+            const int k = kk + lane / 16;
+            T a[2], b[2];
+#pragma unroll
+            for (int s = 0; s < 2; ++s) {
+                a[s] = Bs[k * kTileLd + wj + 16 * s + lane % 16];  // U^T operand
+                b[s] = As[k * kTileLd + wi + 16 * s + lane % 16];  // L^T operand
+            }
+#pragma unroll
+            for (int x = 0; x < 2; ++x)
+#pragma unroll
+                for (int y = 0; y < 2; ++y) {
+                    cr[x][y] = M::mac(a[x].x, b[y].x, cr[x][y]);
+                    cr[x][y] = M::mac(-a[x].y, b[y].y, cr[x][y]);
+                    ci[x][y] = M::mac(a[x].x, b[y].y, ci[x][y]);
+                    ci[x][y] = M::mac(a[x].y, b[y].x, ci[x][y]);
+                }
+        }
+        __syncthreads();
+    }
+    // Block (x, y) of the wave: lane l, element jj holds (L U)[i, j] with i = wi + 16 y + l % 16 and
+    // j = wj + 16 x + row(l, jj) (tile-relative); F22 entries are left to schur_kernel.
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y) {
+            const int i = tl.i0 + wi + 16 * y + lane % 16;
+#pragma unroll
+            for (int jj = 0; jj < 4; ++jj) {
+                const int j = tl.j0 + wj + 16 * x + M::row(lane, jj);
+                if (i < n && j < n && (i < ne || j < ne)) {
+                    T d = F[i + int64_t(j) * n];
+                    d.x -= cr[x][y][jj];
+                    d.y -= ci[x][y][jj];
+                    F[i + int64_t(j) * n] = d;
+                }
+            }
+        }
+}
+
 template <class T, bool kRoot>
 __global__ void __launch_bounds__(kThreads)
     factor_global_kernel(const NodeDesc* __restrict__ nodes, const int32_t* __restrict__ level, PointsView pv, double tol) {
@@ -1333,6 +1650,59 @@ bool unblocked_fronts() {
 bool border_only() {
     const char* v = std::getenv("CAROLI_FRONT_KERNEL");
     return v && std::strcmp(v, "border") == 0;
+}
+
+int blocked_min_ne() {
+    const char* v = std::getenv("CAROLI_FRONT_KERNEL");
+    if (v && std::strcmp(v, "blocked") == 0) return 0;
+    if (v && (std::strcmp(v, "border") == 0 || std::strcmp(v, "unblocked") == 0)) return 1 << 30;
+    return kBlockedMinNe;
+}
+
+void launch_blocked_panel(const NodeDesc* nodes, const int32_t* list, int32_t count, const Entry* ent,
+                          const int32_t* diag, const PointsView& pv, int32_t points, double tol, bool single, bool root,
+                          int32_t k0, hipStream_t st) {
+    if (count <= 0 || points <= 0) return;
+    const dim3 g = grid2(count, points), b(kPanelThreads);
+    auto go = [&](auto kernel) { hipLaunchKernelGGL(kernel, g, b, 0, st, nodes, list, ent, diag, pv, tol, k0); };
+    if (root)
+        go(blocked_panel_kernel<double2, true>);
+    else if (single)
+        go(blocked_panel_kernel<float2, false>);
+    else
+        go(blocked_panel_kernel<double2, false>);
+}
+
+void launch_blocked_row(const NodeDesc* nodes, const Tile* tiles, int32_t count, const PointsView& pv, int32_t points,
+                        bool single, bool root, int32_t k0, hipStream_t st) {
+    if (count <= 0 || points <= 0) return;
+    // The moved rows (2 kBlk x kRowCols), then the triangle (kBlk x (kBlk + 1)).
+    const size_t words = size_t(std::max(2 * kBlk * kRowCols, kBlk * (kBlk + 1)));
+    const size_t lds = words * ((single && !root) ? sizeof(float2) : sizeof(double2));
+    const dim3 g = grid2(count, points), b(256);
+    auto go = [&](auto kernel) {
+        allow_lds(reinterpret_cast<const void*>(kernel));
+        hipLaunchKernelGGL(kernel, g, b, lds, st, nodes, tiles, pv, k0);
+    };
+    if (root)
+        go(blocked_row_kernel<double2, true>);
+    else if (single)
+        go(blocked_row_kernel<float2, false>);
+    else
+        go(blocked_row_kernel<double2, false>);
+}
+
+void launch_blocked_update(const NodeDesc* nodes, const Tile* tiles, int32_t count, const PointsView& pv,
+                           int32_t points, bool single, bool root, int32_t k0, hipStream_t st) {
+    if (count <= 0 || points <= 0) return;
+    const dim3 g = grid2(count, points), b(256);
+    auto go = [&](auto kernel) { hipLaunchKernelGGL(kernel, g, b, 0, st, nodes, tiles, pv, k0); };
+    if (root)
+        go(blocked_update_kernel<double2, true>);
+    else if (single)
+        go(blocked_update_kernel<float2, false>);
+    else
+        go(blocked_update_kernel<double2, false>);
 }
 
 void launch_front_lds(const NodeDesc* nodes, const int32_t* level, int32_t count, int32_t nmax, const Entry* ent,

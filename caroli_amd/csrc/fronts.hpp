@@ -89,6 +89,13 @@ constexpr int kMoveCols = 64;  // columns per chunk of the border's row moves
 constexpr int kF11Lds = 64;
 constexpr int kTrsmSpan = 64;
 constexpr int kTile = 64;
+/// Blocked fronts: separators above blocked_min_ne() variables are factored kBlk columns per step
+/// (panel, row interchanges and U rows, trailing update; many workgroups per front), kRowCols
+/// columns per workgroup of the row step. Each front's pivots are followed by kPivExtra words
+/// (the pivot scale of the blocked steps).
+constexpr int kBlk = 64;
+constexpr int kRowCols = 32;
+constexpr int kPivExtra = 2;
 
 // Launchers (fronts.hip). `level` lists node indices into `nodes`; grid = count x points.
 void launch_assemble_c128(const NodeDesc* nodes, const int32_t* level, int32_t count, const Entry* ent,
@@ -126,6 +133,16 @@ void launch_trsm(const NodeDesc* nodes, const Tile* tiles, int32_t count, int32_
                  int32_t points, bool single, hipStream_t st);
 /// CAROLI_FRONT_KERNEL=border: every large front through border_kernel (no f11/trsm split).
 bool border_only();
+/// Smallest separator (exclusive) of the blocked path: CAROLI_FRONT_KERNEL=blocked takes every
+/// large front and the root there, =border / =unblocked none.
+int blocked_min_ne();
+void launch_blocked_panel(const NodeDesc* nodes, const int32_t* list, int32_t count, const Entry* ent,
+                          const int32_t* diag, const PointsView& pv, int32_t points, double tol, bool single, bool root,
+                          int32_t k0, hipStream_t st);
+void launch_blocked_row(const NodeDesc* nodes, const Tile* tiles, int32_t count, const PointsView& pv, int32_t points,
+                        bool single, bool root, int32_t k0, hipStream_t st);
+void launch_blocked_update(const NodeDesc* nodes, const Tile* tiles, int32_t count, const PointsView& pv,
+                           int32_t points, bool single, bool root, int32_t k0, hipStream_t st);
 /// The whole elimination by rank-1 updates in global memory (CAROLI_FRONT_KERNEL=unblocked).
 void launch_factor_unblocked(const NodeDesc* nodes, const int32_t* level, int32_t count, const PointsView& pv,
                              int32_t points, double tol, bool single, bool root, hipStream_t st);

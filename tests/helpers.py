@@ -5,6 +5,13 @@ import caroli_amd as ca
 from _oracle import OracleSystem
 
 
+TOL = 1e-10  # plain complex128 against the oracle: |dT| <= TOL (1 + |T|)
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
 def sigmas_of(sys_, energy):
     """The engine's host self-energies at `energy`, as the LeadSelfEnergies handle and per-block arrays."""
     sig = sys_.self_energies(energy)
@@ -13,6 +20,24 @@ def sigmas_of(sys_, energy):
 
 def oracle_of(sys_):
     return OracleSystem(sys_.arrays())
+
+
+def compare(sys_, energies, label, precision=ca.Precision.Double, tol=TOL):
+    """One batched launch over `energies` against the oracle's T, to tol (1 + max |T|)."""
+    o = oracle_of(sys_)
+    sig, tref = [], []
+    for e in energies:
+        g, arr = sigmas_of(sys_, e)
+        t, rc = o.transmission_with(e, arr)
+        assert rc == 0, f"{label} E={e}: oracle singular"
+        sig.append(g)
+        tref.append(t)
+    say(f"[gpu] {label}: {len(energies)} points, precision {int(precision)}")
+    t, st = ca.batch_transmission([sys_] * len(sig), sig, precision)
+    assert (st == 0).all(), f"{label}: status {st}"
+    d = np.abs(t - np.array(tref))
+    assert d.max() <= tol * (1 + np.abs(tref).max()), f"{label}: max |dT| {d.max():.3e}\n{t}\n{tref}"
+    return t, np.array(tref)
 
 
 def hash_disorder(region_cells, subs, w):

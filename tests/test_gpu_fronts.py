@@ -2,8 +2,9 @@
 
 S = F22 - F21 F11^-1 F12 for random fronts of the sizes the trees produce, through every kernel
 path: LDS-resident fronts (n <= 64), the separator LU in LDS with the triangular-solve tiles
-(ne <= 64), the panel-by-panel border kernel, and the unblocked fallback; then the MFMA Schur
-tiles. Complex f64 to a relative 1e-11, complex f32 to 1e-4; a singular leading block is refused
+(ne <= 64), the panel-by-panel border kernel, the blocked path (panel, row and update launches per
+64 columns; the default above 128 separator variables) and the unblocked fallback; then the MFMA
+Schur tiles. Complex f64 to a relative 1e-11, complex f32 to 1e-4; a singular leading block is refused
 (field.rs:199-208).
 """
 import ctypes as C
@@ -17,7 +18,7 @@ from caroli_amd import _capi as capi
 pytestmark = pytest.mark.gpu
 
 SIZES = [(12, 5, 5), (60, 20, 20), (64, 64, 64), (90, 40, 40), (100, 37, 37), (200, 64, 64), (384, 128, 128),
-         (60, 40, 0), (300, 256, 0)]
+         (60, 40, 0), (300, 256, 0), (700, 300, 300), (530, 200, 60), (260, 260, 260)]
 
 
 def front(n, seed):
@@ -42,7 +43,7 @@ def numpy_schur(f, ne):
     return f[ne:, ne:] - f[ne:, :ne] @ np.linalg.solve(f[:ne, :ne], f[:ne, ne:])
 
 
-@pytest.mark.parametrize("kernel", ["default", "border", "unblocked"])
+@pytest.mark.parametrize("kernel", ["default", "border", "blocked", "unblocked"])
 @pytest.mark.parametrize("n,ne,ne1", SIZES)
 def test_front_schur_double(n, ne, ne1, kernel, monkeypatch):
     monkeypatch.setenv("CAROLI_FRONT_KERNEL", kernel)
@@ -56,7 +57,7 @@ def test_front_schur_double(n, ne, ne1, kernel, monkeypatch):
     assert np.abs(s - ref).max() <= 1e-11 * np.abs(ref).max()
 
 
-@pytest.mark.parametrize("n,ne", [(12, 5), (90, 40), (200, 64), (384, 128)])
+@pytest.mark.parametrize("n,ne", [(12, 5), (90, 40), (200, 64), (384, 128), (500, 200)])
 def test_front_schur_single(n, ne):
     f = front(n, 7 * n + ne)
     print(f"[gpu] front c64 n={n} ne={ne}", flush=True)
@@ -66,7 +67,7 @@ def test_front_schur_single(n, ne):
     assert np.abs(s - ref).max() <= 1e-4 * np.abs(ref).max()
 
 
-@pytest.mark.parametrize("n,ne", [(10, 4), (120, 70), (150, 40)])
+@pytest.mark.parametrize("n,ne", [(10, 4), (120, 70), (150, 40), (400, 200)])
 @pytest.mark.parametrize("precision", [0, 1])
 def test_singular_block_is_refused(n, ne, precision):
     f = front(n, 3)

@@ -8,32 +8,9 @@ import numpy as np
 import pytest
 
 import caroli_amd as ca
-from helpers import hashed, oracle_of, sigmas_of
+from helpers import TOL, compare, hashed, oracle_of, say
 
 pytestmark = pytest.mark.gpu
-
-TOL = 1e-10
-
-
-def say(*a):
-    print(*a, flush=True)
-
-
-def compare(sys_, energies, label, precision=ca.Precision.Double, tol=TOL):
-    o = oracle_of(sys_)
-    sig, tref = [], []
-    for e in energies:
-        g, arr = sigmas_of(sys_, e)
-        t, rc = o.transmission_with(e, arr)
-        assert rc == 0, f"{label} E={e}: oracle singular"
-        sig.append(g)
-        tref.append(t)
-    say(f"[gpu] {label}: {len(energies)} points, precision {int(precision)}")
-    t, st = ca.batch_transmission([sys_] * len(sig), sig, precision)
-    assert (st == 0).all(), f"{label}: status {st}"
-    d = np.abs(t - np.array(tref))
-    assert d.max() <= tol * (1 + np.abs(tref).max()), f"{label}: max |dT| {d.max():.3e}\n{t}\n{tref}"
-    return t, np.array(tref)
 
 
 def test_device_present():
