@@ -119,13 +119,14 @@ void LevelSet::plan_blocked(const std::vector<NodeDesc>& nodes) {
             if (d.ne <= k0) break;
             ++count;
             const int w = std::min(kBlk, d.ne - k0), k1 = k0 + w, n = d.n, ne = d.ne;
-            const double R = n - k0;
+            const double R = ne - k0;
             pf += 8.0 * (R * w * w / 2.0);
             pw += 2.0 * R * w;
             for (int j0 = 0; j0 < k0; j0 += kRowCols) rtiles.push_back({q, k0, j0, 0});
             for (int j0 = k1; j0 < n; j0 += kRowCols) rtiles.push_back({q, k0, j0, 1});
-            rf += 8.0 * w * w / 2.0 * (n - k1);
-            rw += 2.0 * w * (n - k1) + 4.0 * w * k0;
+            for (int i0 = ne; i0 < n; i0 += kRowCols) rtiles.push_back({q, k0, i0, 2});
+            rf += 8.0 * w * w / 2.0 * (n - k1 + n - ne);
+            rw += 2.0 * w * (n - k1 + n - ne) + 4.0 * w * k0;
             for (int j0 = k1; j0 < n; j0 += kTile)
                 for (int i0 = k1; i0 < n; i0 += kTile)
                     if (i0 < ne || j0 < ne) utiles.push_back({q, i0, j0, 0});

@@ -1065,16 +1065,18 @@ __global__ void __launch_bounds__(256)
 // Blocked fronts (large separators): the LU of F11 and the border strips L21, U12 in steps of kBlk
 // columns, three launches per step over the fronts of a level that still have columns left, with
 // many workgroups per front for the two wide parts:
-//  - blocked_panel_kernel, one workgroup per front: the panel F[k0:n, k0:k1] (k1 = k0 + kBlk)
-//    factored with row pivoting over the separator rows and the pivot test of field.rs:61, in
-//    sub-panels of kSub columns (rank-1 updates inside a sub-panel, one rank-kSub update of the
-//    rest of the panel after it). Interchanges swap whole rows of the panel only. At k0 = 0 it
-//    first adds -H and z to the gathered border and keeps the pivot scale max|F11| after the
-//    front's pivots (two words, read by the later steps);
+//  - blocked_panel_kernel, one workgroup per front: the separator rows of the panel,
+//    F[k0:ne, k0:k1] (k1 = k0 + kBlk), factored with row pivoting and the pivot test of
+//    field.rs:61, in sub-panels of kSub columns (rank-1 updates inside a sub-panel, one rank-kSub
+//    update of the rest of the panel after it; each pass also finds the next column's pivot
+//    candidates). Interchanges swap whole rows of the panel only. At k0 = 0 it first adds -H and
+//    z to the gathered border and keeps the pivot scale max|F11| after the front's pivots (two
+//    words, read by the later steps);
 //  - blocked_row_kernel, one workgroup per kRowCols columns outside the panel: the step's row
 //    interchanges as one permutation of the separator rows (all loads staged through LDS before
-//    any store), and right of the panel the U rows U[k0:k1, j] = L_kk^-1 F[k0:k1, j] solved in
-//    the registers of 8 lanes per column;
+//    any store), and right of the panel the U rows U[k0:k1, j] = L_kk^-1 F[k0:k1, j]; or per
+//    kRowCols rows of the L21 strip, l = f U_kk^-1. Both solves run in the registers of 8 lanes
+//    per column or row;
 //  - blocked_update_kernel, one workgroup per 64 x 64 tile: F -= L[:, k0:k1] U[k0:k1, :] on MFMA
 //    over the rows and columns beyond k1, except F22 (rows and columns >= ne): schur_kernel forms
 //    S = F22 - L21 U12 from the children once, with K = ne, after the last step.
@@ -1100,6 +1102,7 @@ __global__ void __launch_bounds__(kPanelThreads)
     __shared__ int si[NT / 64];
     __shared__ T su[kSub];         // the pivot row inside the sub-panel
     __shared__ T us[kSub * kBlk];  // U rows of a sub-panel right of it: us[t * kBlk + j]
+    __shared__ T ls[kSub * kSub];  // the sub-panel's unit-lower triangle: ls[t + kSub * t2] = L[t, t2]
     const NodeDesc nd = nodes[list[blockIdx.x]];
     const int64_t p = blockIdx.y;
     T* F = (kRoot ? reinterpret_cast<T*>(pv.root_arena) + p * pv.root_stride
@@ -1136,19 +1139,22 @@ __global__ void __launch_bounds__(kPanelThreads)
         scale = get_scale(piv + ne);
     }
     const int w = min(kBlk, ne - k0), k1 = k0 + w;
+    // Each pass that updates the next column to eliminate also keeps each lane's pivot candidate
+    // of it (largest |entry|, first row on ties), so that no separate pass searches it; the first
+    // column of the panel is searched here.
+    double big = -1.0;
+    int q = k0;
+    for (int i = k0 + tid; i < (k0 < ne1 ? ne1 : ne); i += NT) {
+        const double v = cabs(at(i, k0));
+        if (v > big) {
+            big = v;
+            q = i;
+        }
+    }
     for (int s0 = 0; s0 < w; s0 += kSub) {
         const int ws = min(kSub, w - s0), c0 = k0 + s0;  // sub-panel columns [c0, c0 + ws)
         for (int c = 0; c < ws; ++c) {
-            const int k = c0 + c, lim = k < ne1 ? ne1 : ne;
-            double big = -1.0;
-            int q = k;
-            for (int i = k + tid; i < lim; i += NT) {
-                const double v = cabs(at(i, k));
-                if (v > big) {
-                    big = v;
-                    q = i;
-                }
-            }
+            const int k = c0 + c;
             block_argmax(big, q, sv, si);
             const bool ok = k < ne1 ? (big > tol * scale) : (big > 0.0 && big < INFINITY);
             if (!ok) {
@@ -1156,53 +1162,94 @@ __global__ void __launch_bounds__(kPanelThreads)
                 return;
             }
             if (tid == 0) piv[k] = q;
-            if (q != k && tid < w) {
-                const T x = at(k, k0 + tid);
-                at(k, k0 + tid) = at(q, k0 + tid);
-                at(q, k0 + tid) = x;
+            if (tid < w) {  // rows k and q interchanged across the panel; the new row k from column k on to su
+                const T xk = at(k, k0 + tid), xq = at(q, k0 + tid);
+                if (q != k) {
+                    at(k, k0 + tid) = xq;
+                    at(q, k0 + tid) = xk;
+                }
+                const int t = k0 + tid - k;
+                if (t >= 0 && t < ws - c) su[t] = xq;
             }
             __syncthreads();
-            if (tid < ws - c) su[tid] = at(k, k + tid);
-            __syncthreads();
             const T inv = crecip(su[0]);
-            for (int i = k + 1 + tid; i < n; i += NT) {
-                const T l = cmul(at(i, k), inv);
+            const int kn = k + 1, limn = kn < ne1 ? ne1 : ne, last = c0 + ws - 1;
+            const bool track = c + 1 < ws;
+            big = -1.0;
+            q = kn;
+#pragma unroll 1
+            for (int i = kn + tid; i < ne; i += NT) {  // separator rows only; L21 comes from the row step
+                T v[kSub];
+#pragma unroll
+                for (int t = 0; t < kSub; ++t) v[t] = at(i, min(k + t, last));
+                const T l = cmul(v[0], inv);
+#pragma unroll
+                for (int t = 1; t < kSub; ++t) v[t] = csub(v[t], cmul(l, su[t]));
                 at(i, k) = l;
-                for (int t = 1; t < ws - c; ++t) at(i, k + t) = csub(at(i, k + t), cmul(l, su[t]));
+#pragma unroll
+                for (int t = 1; t < kSub; ++t)
+                    if (t < ws - c) at(i, k + t) = v[t];
+                if (track && i < limn) {
+                    const double a = cabs(v[1]);
+                    if (a > big) {
+                        big = a;
+                        q = i;
+                    }
+                }
             }
             __syncthreads();
         }
         const int r0 = c0 + ws, wr = k1 - r0;  // panel columns right of the sub-panel
         if (wr <= 0) break;
-        if (tid < wr) {  // their sub-panel rows: unit-lower solve with L[c0:r0, c0:r0], one lane per column
-            const int j = r0 + tid;
-            T u[kSub];
-#pragma unroll
-            for (int t = 0; t < kSub; ++t) u[t] = t < ws ? at(c0 + t, j) : cx<T>(0.0, 0.0);
-#pragma unroll
-            for (int t = 1; t < kSub; ++t)
-                if (t < ws) {
-                    T acc = u[t];
-#pragma unroll
-                    for (int t2 = 0; t2 < t; ++t2) acc = csub(acc, cmul(at(c0 + t, c0 + t2), u[t2]));
-                    u[t] = acc;
-                }
-#pragma unroll
-            for (int t = 0; t < kSub; ++t) {
-                if (t < ws) at(c0 + t, j) = u[t];
-                us[t * kBlk + tid] = u[t];
-            }
+        if (tid < kSub * kSub) {  // L[c0:r0, c0:r0] to LDS (strictly lower part; zero elsewhere)
+            const int t = tid % kSub, t2 = tid / kSub;
+            ls[tid] = (t2 < t && t < ws) ? at(c0 + t, c0 + t2) : cx<T>(0.0, 0.0);
         }
         __syncthreads();
-        for (int i = r0 + tid; i < n; i += NT) {  // rank-ws update of the rows below
+        if (tid < wr) {  // their sub-panel rows: unit-lower solve with L[c0:r0, c0:r0] in LDS, one lane per column
+            const int j = r0 + tid;
+#pragma unroll
+            for (int t = 0; t < kSub; ++t) us[t * kBlk + tid] = t < ws ? at(c0 + t, j) : cx<T>(0.0, 0.0);
+#pragma unroll 1
+            for (int t = 0; t < ws - 1; ++t) {
+                const T ut = us[t * kBlk + tid];
+#pragma unroll 1
+                for (int t3 = t + 1; t3 < ws; ++t3)
+                    us[t3 * kBlk + tid] = csub(us[t3 * kBlk + tid], cmul(ls[t3 + kSub * t], ut));
+            }
+#pragma unroll 1
+            for (int t = 1; t < ws; ++t) at(c0 + t, j) = us[t * kBlk + tid];
+        }
+        __syncthreads();
+        const int limn = r0 < ne1 ? ne1 : ne;
+        big = -1.0;
+        q = r0;
+#pragma unroll 1
+        for (int i = r0 + tid; i < ne; i += NT) {  // rank-ws update of the separator rows below
             T l[kSub];
 #pragma unroll
             for (int t = 0; t < kSub; ++t) l[t] = t < ws ? at(i, c0 + t) : cx<T>(0.0, 0.0);
-            for (int j = 0; j < wr; ++j) {
-                T acc = at(i, r0 + j);
+#pragma unroll 1
+            for (int j0 = 0; j0 < wr; j0 += 2) {
+                T a[2];
 #pragma unroll
-                for (int t = 0; t < kSub; ++t) acc = csub(acc, cmul(l[t], us[t * kBlk + j]));
-                at(i, r0 + j) = acc;
+                for (int u = 0; u < 2; ++u) a[u] = at(i, r0 + min(j0 + u, wr - 1));
+#pragma unroll
+                for (int t = 0; t < kSub; ++t) {
+#pragma unroll
+                    for (int u = 0; u < 2; ++u) a[u] = csub(a[u], cmul(l[t], us[t * kBlk + min(j0 + u, wr - 1)]));
+                    __builtin_amdgcn_sched_barrier(0);  // bounds the LDS loads in flight (registers)
+                }
+#pragma unroll
+                for (int u = 0; u < 2; ++u)
+                    if (j0 + u < wr) at(i, r0 + j0 + u) = a[u];
+                if (j0 == 0 && i < limn) {
+                    const double v = cabs(a[0]);
+                    if (v > big) {
+                        big = v;
+                        q = i;
+                    }
+                }
             }
         }
         __syncthreads();
@@ -1239,9 +1286,42 @@ __global__ void __launch_bounds__(256)
            nd.off;
     const int32_t* piv = pv.piv + p * pv.piv_stride + nd.piv_off;
     const int n = nd.n, ne = nd.ne, tid = threadIdx.x, w = min(kBlk, ne - k0), k1 = k0 + w;
+    auto at = [&](int i, int j) -> T& { return F[i + int64_t(j) * n]; };
+    constexpr int la = kBlk + 1;
+    if (tl.pad == 2) {  // rows j0.. of the L21 strip: l U_kk = f, eight lanes per row (U_kk in LDS)
+        T* A = X;
+        for (int e = tid; e < w * w; e += 256) A[(e % w) + (e / w) * la] = at(k0 + e % w, k0 + e / w);
+        __syncthreads();
+        const int rows = min(kRowCols, n - tl.j0), grp = tid / 8, g = tid % 8, row = tl.j0 + min(grp, rows - 1);
+        T x[8];
+#pragma unroll
+        for (int s = 0; s < 8; ++s) x[s] = at(row, k0 + min(g + 8 * s, w - 1));
+#pragma unroll
+        for (int kb = 0; kb < 8; ++kb)
+#pragma unroll
+            for (int kk = 0; kk < 8; ++kk) {
+                const int k = 8 * kb + kk;
+                if (k < w) {
+                    const T cand = cmul(x[kb], crecip(A[k + k * la]));
+                    T xk;
+                    xk.x = __shfl(cand.x, kk, 8);
+                    xk.y = __shfl(cand.y, kk, 8);
+                    if (g == kk) x[kb] = xk;
+#pragma unroll
+                    for (int s = 0; s < 8; ++s) {
+                        const int j = g + 8 * s;
+                        if (j > k && j < w) x[s] = csub(x[s], cmul(xk, A[k + j * la]));
+                    }
+                }
+            }
+        if (grp < rows)
+#pragma unroll
+            for (int s = 0; s < 8; ++s)
+                if (g + 8 * s < w) at(row, k0 + g + 8 * s) = x[s];
+        return;
+    }
     const bool right = tl.pad != 0;
     const int j0 = tl.j0, cols = min(kRowCols, (right ? n : k0) - j0);
-    auto at = [&](int i, int j) -> T& { return F[i + int64_t(j) * n]; };
     if (tid < w) sp[tid] = piv[k0 + tid];
     __syncthreads();
     if (tid < w) {
@@ -1272,7 +1352,6 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
     for (int s = 0; s < 8; ++s) x[s] = X[min(g + 8 * s, w - 1) + col * kBlk];
     __syncthreads();
-    constexpr int la = kBlk + 1;
     T* A = X;  // L_kk, leading dimension la
     for (int e = tid; e < w * w; e += 256) A[(e % w) + (e / w) * la] = at(k0 + e % w, k0 + e / w);
     __syncthreads();
