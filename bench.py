@@ -66,6 +66,8 @@ def roofline_of(ks, prec_name, tag):
 
     The bound is the roof the family's algorithmic intensity meets first: flops per byte against
     the ridge point peak_flops / peak_bandwidth."""
+    if not ks or max(v[1] for v in ks.values()) <= 0:
+        return None  # no timed launches (--roofline-steps 0)
     name, (launches, secs, fl, by) = max(ks.items(), key=lambda kv: kv[1][1])
     peak = PEAK_FP64_MFMA if prec_name == "double" else PEAK_FP32_MFMA
     bound = "mfma" if fl > 0 and fl / max(by, 1) >= peak / PEAK_HBM else "hbm"
@@ -83,6 +85,11 @@ def roofline_of(ks, prec_name, tag):
                  "algorithmic_bytes_per_launch": by / max(launches, 1),
                  "algorithmic_flops_per_launch": fl / max(launches, 1)})
     return roof
+
+
+def progress(*a):
+    """A progress line on stderr (the JSON line alone goes to stdout)."""
+    print("[bench]", *a, file=sys.stderr, flush=True)
 
 
 def kernels_of(ks, steps):
@@ -117,6 +124,7 @@ def build_points(args, ca, threads):
         for s, L in chosen:
             by_l.setdefault(L, []).append(ca.NestedTwoTerminal.preset(ca.Model.Cubic, [2 * L, L, L], disorder=1,
                                                                       strength=16.5, seed=1000 + s))
+            progress(f"system {s} (L = {L}) built")
         groups = [(by_l[L], energies) for L in sorted(by_l)]
         desc = (f"cubic Anderson W=16.5, 2L x L x L, systems {[s for s, _ in chosen]} of the 512 (L <= {args.max_l}: "
                 f"L = {[L for _, L in chosen]}), 4 E")
@@ -125,6 +133,7 @@ def build_points(args, ca, threads):
     for systems, energies in groups:
         with ThreadPoolExecutor(threads) as ex:
             sigmas = list(ex.map(systems[0].self_energies, energies))
+        progress(f"self-energies of {len(energies)} energies done ({time.time() - t0:.1f} s)")
         for s in systems:
             for e, g in zip(energies, sigmas):
                 pts_sys.append(s)
@@ -174,14 +183,19 @@ def run_transmission(args, ca):
     pts_sys, pts_sig, pts_e, desc, sigma_s = build_points(args, ca, threads)
     build_s = time.time() - t0 - sigma_s
     npts = len(pts_sys)
+    progress(f"{npts} points: {desc}; systems built in {build_s:.1f} s, self-energies {sigma_s:.1f} s")
     batch = ca.ResidentBatch(pts_sys, pts_sig, prec)
-    for _ in range(args.warmup):
+    progress("batch resident")
+    for w in range(args.warmup):
+        t0 = time.perf_counter()
         batch.run()
+        progress(f"warmup {w + 1}: {time.perf_counter() - t0:.3f} s")
     times, stats = [], ca.capi.BatchStats()
-    for _ in range(args.steps):
+    for k in range(args.steps):
         t0 = time.perf_counter()
         t, st = batch.run(stats)
         times.append(time.perf_counter() - t0)
+        progress(f"step {k + 1}: {times[-1]:.3f} s")
     step = float(np.mean(times))
     # Kernel roofline: the same launches with the HIP-event timers on, which run the launch units
     # (the two spin blocks) one after the other so that each launch's duration is its own.
@@ -190,7 +204,14 @@ def run_transmission(args, ca):
         batch.run()
     ks = ca.kernel_stats()
     ca.kernel_timing(False)
+    progress("roofline steps done")
+    other = None
+    if args.compare and args.precision in ("double", "mixed"):
+        other = compare_precision(ca, pts_sys, pts_sig, t, npts / step, args)
     cpu, max_dt, max_rel = (None, None, None) if args.no_cpu else cpu_baseline(pts_sys, pts_sig, pts_e, t, args, threads)
+    if cpu is not None and other is not None:
+        # the other precision against the same CPU sample, through its difference to this run
+        other["max_abs_dT_vs_cpu_bound"] = max_dt + other["max_abs_dT_vs_this"]
     flops_pt = np.mean([s.work()[0] for s in pts_sys])
     bytes_pt = np.mean([s.work()[1] for s in pts_sys])
     return {
@@ -208,6 +229,7 @@ def run_transmission(args, ca):
         "max_dT_over_1_plus_T_vs_cpu": max_rel,
         "status_ok": int((st == 0).sum()),
         "delayed_points": int(stats.delayed_points),
+        "recomputed_points": int(stats.recomputed_points),
         "refinement_iters": int(stats.refinement_iters),
         "max_refine_correction": float(stats.max_refine_resid),
         "device_seconds_per_step": stats.seconds,
@@ -219,7 +241,32 @@ def run_transmission(args, ca):
         "build_seconds": build_s,
         "kernels": kernels_of(ks, args.roofline_steps),
         "roofline_steps": args.roofline_steps,
+        ("mixed" if args.precision == "double" else "plain"): other,
     }
+
+
+def compare_precision(ca, pts_sys, pts_sig, t_this, rate_this, args):
+    """The same resident points in the other precision (complex128 <-> refined mixed): its rate, the
+    ratio mixed / plain, and its largest |dT| against this run."""
+    name = "mixed" if args.precision == "double" else "double"
+    prec = ca.Precision.Mixed if name == "mixed" else ca.Precision.Double
+    batch = ca.ResidentBatch(pts_sys, pts_sig, prec)
+    for _ in range(args.warmup):
+        batch.run()
+    times, stats = [], ca.capi.BatchStats()
+    for k in range(args.steps):
+        t0 = time.perf_counter()
+        t, st = batch.run(stats)
+        times.append(time.perf_counter() - t0)
+        progress(f"{name} step {k + 1}: {times[-1]:.3f} s")
+    rate = len(pts_sys) / float(np.mean(times))
+    mixed_rate, plain_rate = (rate, rate_this) if name == "mixed" else (rate_this, rate)
+    return {"precision": name, "value": rate, "unit": "points/s", "ms_per_step": 1e3 * float(np.mean(times)),
+            "mixed_over_plain": mixed_rate / plain_rate,
+            "max_abs_dT_vs_this": float(np.max(np.abs(t - t_this))), "status_ok": int((st == 0).sum()),
+            "recomputed_points": int(stats.recomputed_points), "delayed_points": int(stats.delayed_points),
+            "refinement_iters": int(stats.refinement_iters),
+            "max_refine_correction": float(stats.max_refine_resid)}
 
 
 def run_adjoint(args, ca):
@@ -293,6 +340,8 @@ def main():
     ap.add_argument("--max-l", type=int, default=24, help="c5: largest cross-section taken")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU time of the baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-compare", dest="compare", action="store_false",
+                    help="skip the run of the same points in the other precision (mixed vs plain)")
     ap.add_argument("--copy-calibration", action="store_true", help="run a 2 GiB device copy and exit")
     args = ap.parse_args()
     if args.gpus != 1:

@@ -26,6 +26,7 @@ class BatchStats(C.Structure):
         ("delayed_points", C.c_int64),
         ("refinement_iters", C.c_int64),
         ("max_refine_resid", C.c_double),
+        ("recomputed_points", C.c_int64),
     ]
 
 

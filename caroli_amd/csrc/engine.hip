@@ -292,6 +292,7 @@ struct HostPlan {
     /// The block's orbitals, and each orbital's position in the left / right slice (-1 if none).
     std::vector<int32_t> rows, left_pos, right_pos, left_orbs, right_orbs;
     int32_t max_ne = 0;  // largest ne of a front or the root
+    int32_t max_n = 0;   // largest n of a front (the root excluded)
 };
 
 struct DevPlan {
@@ -300,6 +301,7 @@ struct DevPlan {
     // Ranges into `levels` of each level's node list (for the solves); the kernel split of every
     // level per front precision, and of the root.
     std::vector<std::pair<int32_t, int32_t>> all;
+    std::vector<int32_t> all_max_n;  // largest front of each level
     std::vector<DevLevel> sets[2];
     DevLevel root_set;
     int32_t root_at = 0;  // position of the root index in `levels`
@@ -394,6 +396,7 @@ HostPlan build_plan(const Block& blk, const SparseMatrix& h) {
         d.var_off = int64_t(hp.vars.size());
         for (size_t o : pl.vars) hp.vars.push_back(int32_t(o));
         hp.max_ne = std::max(hp.max_ne, d.ne);
+        hp.max_n = std::max(hp.max_n, d.n);
         off += int64_t(d.n) * d.n;
         piv += d.ne + kPivExtra;
         hp.nodes.push_back(d);
@@ -467,6 +470,9 @@ void upload_plan(const HostPlan& hp, DevPlan& dp, const SparseMatrix& h, hipStre
     std::vector<Tile> tl;
     for (auto& l : hp.level_all) {
         dp.all.push_back({int32_t(lv.size()), int32_t(l.size())});
+        int32_t mn = 0;
+        for (int32_t q : l) mn = std::max(mn, hp.nodes[q].n);
+        dp.all_max_n.push_back(mn);
         lv.insert(lv.end(), l.begin(), l.end());
     }
     std::vector<SetOffsets> offs[2];
@@ -996,26 +1002,35 @@ int32_t replanned_point(Engine& eng, const cg_system* s, const cg_sigma* g, int 
 /// Multifrontal solve of the columns of `sv` with the factors the last sweep left in the arenas:
 /// forward elimination (skipped when the right-hand sides live on the kept orbitals only, where
 /// it does nothing), the dense root, the backward substitution.
-void solve(Engine& eng, const DevPlan& dp, const PointsView& pv, const SolveView& sv, int32_t P, bool single,
-           bool transpose, bool kept_only) {
+void solve(Engine& eng, const HostPlan& hp, const DevPlan& dp, const PointsView& pv, const SolveView& sv, int32_t P,
+           bool single, bool transpose, bool kept_only) {
     const NodeDesc* nodes = dp.nodes.as<NodeDesc>();
     const int32_t* lv = dp.levels.as<int32_t>();
     eng.timed(kSolve, 0.0, 0.0, [&] {
         if (!kept_only)
             for (size_t l = 0; l < dp.all.size(); ++l)
-                launch_solve_forward(nodes, lv + dp.all[l].first, dp.all[l].second, pv, sv, P, single, transpose,
-                                     eng.lane->stream);
-        launch_solve_root(nodes, lv + dp.root_at, pv, sv, P, transpose, eng.lane->stream);
+                launch_solve_forward(nodes, lv + dp.all[l].first, dp.all[l].second, dp.all_max_n[l], pv, sv, P, single,
+                                     transpose, eng.lane->stream);
+        launch_solve_root(nodes, lv + dp.root_at, hp.nodes[hp.root].ne, pv, sv, P, transpose, eng.lane->stream);
         for (size_t l = dp.all.size(); l-- > 0;)
-            launch_solve_backward(nodes, lv + dp.all[l].first, dp.all[l].second, pv, sv, P, single, transpose,
-                                  eng.lane->stream);
+            launch_solve_backward(nodes, lv + dp.all[l].first, dp.all[l].second, dp.all_max_n[l], pv, sv, P, single,
+                                  transpose, eng.lane->stream);
     });
     HIP_CHECK(hipGetLastError());
 }
 
-/// Refinement: iterate until the relative correction max|d| / max|x| is below this.
+/// Refinement: a point is done when its relative correction r_k = max|d| / max|x| is below
+/// kRefineTol, or when the error left after the update, estimated as r_k * (r_k / r_{k-1}) (the
+/// next correction at the observed contraction rate), is below kRefineEst with r_k below
+/// kRefineEstFrom.
 constexpr double kRefineTol = 1e-10;
+constexpr double kRefineEst = 1e-12;
+constexpr double kRefineEstFrom = 1e-7;
 constexpr int kRefineMaxIter = 12;
+/// From the third iteration on, a point whose correction shrinks by less than 1 / kRefineRate per
+/// iteration is left to the complex f64 recomputation (f32 factors of a well-conditioned point
+/// contract by 1e-3 or better; a slow one would hold the whole batch in the loop).
+constexpr double kRefineRate = 0.25;
 
 struct Refined {
     std::vector<double> t;
@@ -1037,7 +1052,7 @@ Refined refine(Engine& eng, const HostPlan& hp, const DevPlan& dp, const PointsV
     int c = 0;
     for (auto* g : sig) c = std::max(c, g->blocks[b].wr.cols);
     if (c == 0) return out;
-    if (int64_t(hp.max_ne) * c > kSolveLds) {
+    if (!solve_fits(hp.max_n) || !solve_fits(hp.nodes[hp.root].ne)) {  // left to the complex f64 recomputation
         out.ok.assign(P, 0);
         return out;
     }
@@ -1055,7 +1070,7 @@ Refined refine(Engine& eng, const HostPlan& hp, const DevPlan& dp, const PointsV
     xb.ensure(bytes);
     bb.ensure(bytes);
     db.ensure(bytes);
-    rb.ensure(size_t(P) * sizeof(double));
+    rb.ensure(size_t(P) * 2 * sizeof(double));
     gb.ensure(size_t(P) * nl * c * sizeof(double2));
     HIP_CHECK(hipMemsetAsync(bb.p, 0, bytes, eng.lane->stream));
     SolveView xv{xb.as<double2>(), n_orb * c, int32_t(n_orb), c, dp.vars.as<int32_t>()};
@@ -1064,42 +1079,52 @@ Refined refine(Engine& eng, const HostPlan& hp, const DevPlan& dp, const PointsV
     dv.x = db.as<double2>();
     launch_scatter(bv, dp.right_orbs.as<int32_t>(), nr, dw.as<double2>(), int64_t(nr) * c, false, P, eng.lane->stream);
     HIP_CHECK(hipMemcpyAsync(xb.p, bb.p, bytes, hipMemcpyDeviceToDevice, eng.lane->stream));
-    solve(eng, dp, pv, xv, P, single, false, true);
-    std::vector<double> ratio(P, INFINITY), last(P, INFINITY);
-    std::vector<char> gave_up(P, 0);
+    solve(eng, hp, dp, pv, xv, P, single, false, true);
+    // Per point: 0 iterating, 1 converged (later corrections only add rounding noise), 2 given up
+    // (unusable factors, a correction that is not finite or shrinks too slowly): left to the
+    // complex f64 recomputation.
+    std::vector<double> ratio(P, INFINITY), last(P, INFINITY), maxes(size_t(P) * 2);
+    std::vector<char> state(P, 0);
+    for (int32_t p = 0; p < P; ++p)
+        if (skip[p]) state[p] = 2;
     for (int it = 0; it < kRefineMaxIter; ++it) {
         launch_residual(pv, xv, bb.as<double2>(), db.as<double2>(), dp.rows.as<int32_t>(), nrows,
                         dp.row_ptr.as<int64_t>(), dp.cols.as<uint32_t>(), dp.left_pos.as<int32_t>(),
                         dp.right_pos.as<int32_t>(), dp.left_orbs.as<int32_t>(), nl, dp.right_orbs.as<int32_t>(), nr, P,
                         eng.lane->stream);
-        solve(eng, dp, pv, dv, P, single, false, false);
+        solve(eng, hp, dp, pv, dv, P, single, false, false);
         launch_update(xv, db.as<double2>(), dp.rows.as<int32_t>(), nrows, rb.as<double>(), P, eng.lane->stream);
         HIP_CHECK(hipGetLastError());
-        HIP_CHECK(hipMemcpyAsync(ratio.data(), rb.p, P * sizeof(double), hipMemcpyDeviceToHost, eng.lane->stream));
+        HIP_CHECK(hipMemcpyAsync(maxes.data(), rb.p, maxes.size() * sizeof(double), hipMemcpyDeviceToHost,
+                                 eng.lane->stream));
         HIP_CHECK(hipStreamSynchronize(eng.lane->stream));
-        ++out.iters;
-        // Points whose factors are unusable (skip), or whose correction stopped shrinking, are left
-        // to the complex f64 recomputation and no longer hold the loop.
-        bool done = true;
         for (int32_t p = 0; p < P; ++p) {
-            const double r = ratio[p];
-            if (skip[p] || gave_up[p]) continue;
-            if (!(r < INFINITY) || (it >= 2 && r > 0.5 * last[p])) {
-                gave_up[p] = 1;
-                continue;
-            }
-            last[p] = r;
-            done &= r <= kRefineTol;
+            const double md = maxes[2 * p], mx = maxes[2 * p + 1];
+            ratio[p] = mx > 0.0 ? md / mx : (md > 0.0 ? INFINITY : 0.0);
         }
-        if (done) break;
+        ++out.iters;
+        bool active = false;
+        for (int32_t p = 0; p < P; ++p) {
+            if (state[p] != 0) continue;
+            const double r = ratio[p];
+            if (r <= kRefineTol || (last[p] < INFINITY && r <= kRefineEstFrom && r * r <= kRefineEst * last[p])) {
+                state[p] = 1;
+                out.worst = std::max(out.worst, r);
+            } else if (!(r < INFINITY) || (it >= 2 && r > kRefineRate * last[p])) {
+                state[p] = 2;
+            } else {
+                last[p] = r;
+                active = true;
+            }
+        }
+        if (!active) break;
     }
     launch_gather(xv, dp.left_orbs.as<int32_t>(), nl, gb.as<double2>(), P, eng.lane->stream);
     std::vector<double2> xl(size_t(P) * nl * c);
     HIP_CHECK(hipMemcpyAsync(xl.data(), gb.p, xl.size() * sizeof(double2), hipMemcpyDeviceToHost, eng.lane->stream));
     HIP_CHECK(hipStreamSynchronize(eng.lane->stream));
     for (int32_t p = 0; p < P; ++p) {
-        out.ok[p] = !skip[p] && !gave_up[p] && ratio[p] <= kRefineTol;
-        if (out.ok[p]) out.worst = std::max(out.worst, ratio[p]);
+        out.ok[p] = state[p] == 1;
         const CMat& wl = sig[p]->blocks[b].wl;
         double t = 0.0;
         for (int col = 0; col < c; ++col)
@@ -1344,6 +1369,7 @@ int32_t run_batch(cg_batch* bt, double* out_t, int32_t* out_status, cg_batch_sta
     std::vector<int64_t> again;
     for (int64_t p = 0; p < count; ++p)
         if (redo[p]) again.push_back(p);
+    local.recomputed_points = int64_t(again.size());
     if (!again.empty()) {
         std::vector<const cg_system*> ss;
         std::vector<const cg_sigma*> gg;
@@ -1561,7 +1587,8 @@ int32_t backward(const cg_system* s, const cg_sigma* g, double* out_t, double* o
         const int cr = sb.wr.cols, cl = sb.wl.cols;
         const int nl = int(hp.left_orbs.size()), nr = int(hp.right_orbs.size());
         if (cr == 0 || cl == 0) continue;
-        if (int64_t(hp.max_ne) * std::max(cr, cl) > kSolveLds) return CG_ERR_INVALID;
+        if (int64_t(hp.max_ne) * cl > kSolveLds || !solve_fits(hp.max_n) || !solve_fits(hp.nodes[hp.root].ne))
+            return CG_ERR_INVALID;  // the transposed solves hold all cl columns of a front in LDS
         auto columns = [&](const CMat& w, int m, const DevBuf& orbs, bool transpose, std::vector<double2>& host) {
             const int c = w.cols;
             std::vector<double2> wh(size_t(m) * c);
@@ -1574,7 +1601,7 @@ int32_t backward(const cg_system* s, const cg_sigma* g, double* out_t, double* o
             HIP_CHECK(hipMemsetAsync(xb.p, 0, bytes, eng.lane->stream));
             SolveView xv{xb.as<double2>(), n_orb * c, int32_t(n_orb), c, dp.vars.as<int32_t>()};
             launch_scatter(xv, orbs.as<int32_t>(), m, dw.as<double2>(), int64_t(m) * c, transpose, 1, eng.lane->stream);
-            solve(eng, dp, pv, xv, 1, false, transpose, true);
+            solve(eng, hp, dp, pv, xv, 1, false, transpose, true);
             host.resize(size_t(n_orb) * c);
             HIP_CHECK(hipMemcpyAsync(host.data(), xb.p, bytes, hipMemcpyDeviceToHost, eng.lane->stream));
             HIP_CHECK(hipStreamSynchronize(eng.lane->stream));

@@ -161,18 +161,25 @@ inline size_t trace_partials(int32_t cl_max, int32_t cr_max) {
     return size_t((cl_max + kTile - 1) / kTile) * size_t((cr_max + kTile - 1) / kTile);
 }
 
-/// Largest ne * c of a front handled by the solve kernels (LDS words of complex f64).
+/// Largest ne * c of a front handled by the transposed solve kernels (LDS words of complex f64).
 constexpr int kSolveLds = 4096;
+/// LDS a blocked solve workgroup may take; solve_fits(rows): fronts (or a root block) of `rows`
+/// rows fit in it (one right-hand-side column at a time at least).
+constexpr int kSolveMaxLds = 128 * 1024;
+bool solve_fits(int rows);
 
-/// Multifrontal solves with the stored factors, one tree level per launch (grid = fronts x points).
-/// forward: children first; A x = b (transpose = 0) or A^T x = b (transpose = 1).
-/// root: the dense root; backward: parents first. `single`: the fronts hold complex f32 factors.
-void launch_solve_forward(const NodeDesc* nodes, const int32_t* level, int32_t count, const PointsView& pv,
-                          const SolveView& sv, int32_t points, bool single, bool transpose, hipStream_t st);
-void launch_solve_root(const NodeDesc* nodes, const int32_t* level, const PointsView& pv, const SolveView& sv,
-                       int32_t points, bool transpose, hipStream_t st);
-void launch_solve_backward(const NodeDesc* nodes, const int32_t* level, int32_t count, const PointsView& pv,
-                           const SolveView& sv, int32_t points, bool single, bool transpose, hipStream_t st);
+/// Multifrontal solves with the stored factors, one tree level per launch (grid = fronts x points
+/// x column chunks). forward: children first; A x = b (transpose = 0, blocked kernels sized by
+/// max_n, the level's largest front) or A^T x = b (transpose = 1). root: the dense root (its ne =
+/// root_ne). backward: parents first. `single`: the fronts hold complex f32 factors.
+void launch_solve_forward(const NodeDesc* nodes, const int32_t* level, int32_t count, int32_t max_n,
+                          const PointsView& pv, const SolveView& sv, int32_t points, bool single, bool transpose,
+                          hipStream_t st);
+void launch_solve_root(const NodeDesc* nodes, const int32_t* level, int32_t root_ne, const PointsView& pv,
+                       const SolveView& sv, int32_t points, bool transpose, hipStream_t st);
+void launch_solve_backward(const NodeDesc* nodes, const int32_t* level, int32_t count, int32_t max_n,
+                           const PointsView& pv, const SolveView& sv, int32_t points, bool single, bool transpose,
+                           hipStream_t st);
 
 /// Refinement helpers (all per point, on c columns of n_orb rows).
 void launch_scatter(const SolveView& sv, const int32_t* orbs, int32_t m, const double2* w, int64_t w_stride, bool conj,
@@ -181,7 +188,8 @@ void launch_residual(const PointsView& pv, const SolveView& xv, const double2* b
                      int32_t nrows, const int64_t* row_ptr, const uint32_t* cols, const int32_t* left_pos,
                      const int32_t* right_pos, const int32_t* left_orbs, int32_t nl, const int32_t* right_orbs,
                      int32_t nr, int32_t points, hipStream_t st);
-void launch_update(const SolveView& xv, const double2* d, const int32_t* rows, int32_t nrows, double* ratio,
+/// X += D; maxes[2p], maxes[2p + 1] receive max |D| and max |X| of point p (2 doubles per point).
+void launch_update(const SolveView& xv, const double2* d, const int32_t* rows, int32_t nrows, double* maxes,
                    int32_t points, hipStream_t st);
 void launch_gather(const SolveView& xv, const int32_t* orbs, int32_t m, double2* out, int32_t points, hipStream_t st);
 

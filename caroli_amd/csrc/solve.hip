@@ -12,6 +12,9 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <mutex>
+#include <set>
+#include <stdexcept>
 
 #include "cx.hpp"
 #include "fronts.hpp"
@@ -32,67 +35,44 @@ __device__ __forceinline__ void atomic_sub(double2* p, double2 v) {
     atomicAdd(&p->y, -v.y);
 }
 
-template <class TF, bool kT>
+// Transposed solves (A^T x = b, the adjoint's second set of columns), one workgroup per front and
+// point with all columns in LDS (ne * c <= kSolveLds).
+template <class TF>
 __global__ void __launch_bounds__(kThreads)
-    solve_forward_kernel(const NodeDesc* __restrict__ nodes, const int32_t* __restrict__ level, PointsView pv, SolveView sv) {
+    solve_forward_t_kernel(const NodeDesc* __restrict__ nodes, const int32_t* __restrict__ level, PointsView pv,
+                           SolveView sv) {
     extern __shared__ double2 y[];  // ne x c, column-major
     const NodeDesc nd = nodes[level[blockIdx.x]];
     const int64_t p = blockIdx.y;
     const TF* F = reinterpret_cast<const TF*>(pv.arena) + p * pv.arena_stride + nd.off;
-    const int32_t* piv = pv.piv + p * pv.piv_stride + nd.piv_off;
     const int32_t* vars = sv.vars + nd.var_off;
     double2* X = sv.x + p * sv.x_stride;
     const int n = nd.n, ne = nd.ne, nb = n - ne, c = sv.c, tid = threadIdx.x;
     auto f = [&](int i, int j) { return widen(F[i + int64_t(j) * n]); };
     for (int idx = tid; idx < ne * c; idx += kThreads) y[idx] = X[vars[idx % ne] + int64_t(idx / ne) * sv.ldx];
     __syncthreads();
-    if (!kT) {
-        for (int col = tid; col < c; col += kThreads)
-            for (int k = 0; k < ne; ++k) {
-                int q = piv[k];
-                if (q != k) {
-                    double2 t = y[k + col * ne];
-                    y[k + col * ne] = y[q + col * ne];
-                    y[q + col * ne] = t;
-                }
-            }
+    for (int k = 0; k < ne; ++k) {  // U11^T g = b_e (lower, non-unit)
+        for (int col = tid; col < c; col += kThreads) y[k + col * ne] = cmul(y[k + col * ne], crecip(f(k, k)));
         __syncthreads();
-        for (int k = 0; k < ne; ++k) {  // unit lower L11
-            for (int idx = tid; idx < (ne - k - 1) * c; idx += kThreads) {
-                int i = k + 1 + idx % (ne - k - 1), col = idx / (ne - k - 1);
-                y[i + col * ne] = csub(y[i + col * ne], cmul(f(i, k), y[k + col * ne]));
-            }
-            __syncthreads();
+        for (int idx = tid; idx < (ne - k - 1) * c; idx += kThreads) {
+            int i = k + 1 + idx % (ne - k - 1), col = idx / (ne - k - 1);
+            y[i + col * ne] = csub(y[i + col * ne], cmul(f(k, i), y[k + col * ne]));
         }
-        for (int idx = tid; idx < nb * c; idx += kThreads) {  // b_b -= L21 y
-            int b = idx % nb, col = idx / nb;
-            double2 acc{0.0, 0.0};
-            for (int k = 0; k < ne; ++k) acc = cadd(acc, cmul(f(ne + b, k), y[k + col * ne]));
-            atomic_sub(&X[vars[ne + b] + int64_t(col) * sv.ldx], acc);
-        }
-    } else {
-        for (int k = 0; k < ne; ++k) {  // U11^T g = b_e (lower, non-unit)
-            for (int col = tid; col < c; col += kThreads) y[k + col * ne] = cmul(y[k + col * ne], crecip(f(k, k)));
-            __syncthreads();
-            for (int idx = tid; idx < (ne - k - 1) * c; idx += kThreads) {
-                int i = k + 1 + idx % (ne - k - 1), col = idx / (ne - k - 1);
-                y[i + col * ne] = csub(y[i + col * ne], cmul(f(k, i), y[k + col * ne]));
-            }
-            __syncthreads();
-        }
-        for (int idx = tid; idx < nb * c; idx += kThreads) {  // b_b -= U12^T g
-            int b = idx % nb, col = idx / nb;
-            double2 acc{0.0, 0.0};
-            for (int k = 0; k < ne; ++k) acc = cadd(acc, cmul(f(k, ne + b), y[k + col * ne]));
-            atomic_sub(&X[vars[ne + b] + int64_t(col) * sv.ldx], acc);
-        }
+        __syncthreads();
+    }
+    for (int idx = tid; idx < nb * c; idx += kThreads) {  // b_b -= U12^T g
+        int b = idx % nb, col = idx / nb;
+        double2 acc{0.0, 0.0};
+        for (int k = 0; k < ne; ++k) acc = cadd(acc, cmul(f(k, ne + b), y[k + col * ne]));
+        atomic_sub(&X[vars[ne + b] + int64_t(col) * sv.ldx], acc);
     }
     for (int idx = tid; idx < ne * c; idx += kThreads) X[vars[idx % ne] + int64_t(idx / ne) * sv.ldx] = y[idx];
 }
 
-template <class TF, bool kT>
+template <class TF>
 __global__ void __launch_bounds__(kThreads)
-    solve_backward_kernel(const NodeDesc* __restrict__ nodes, const int32_t* __restrict__ level, PointsView pv, SolveView sv) {
+    solve_backward_t_kernel(const NodeDesc* __restrict__ nodes, const int32_t* __restrict__ level, PointsView pv,
+                            SolveView sv) {
     extern __shared__ double2 y[];  // ne x c
     const NodeDesc nd = nodes[level[blockIdx.x]];
     const int64_t p = blockIdx.y;
@@ -102,53 +82,327 @@ __global__ void __launch_bounds__(kThreads)
     double2* X = sv.x + p * sv.x_stride;
     const int n = nd.n, ne = nd.ne, nb = n - ne, c = sv.c, tid = threadIdx.x;
     auto f = [&](int i, int j) { return widen(F[i + int64_t(j) * n]); };
-    // y = x_e - (U12 or L21^T) x_b
+    // y = x_e - L21^T x_b
     for (int idx = tid; idx < ne * c; idx += kThreads) {
         int k = idx % ne, col = idx / ne;
         double2 acc = X[vars[k] + int64_t(col) * sv.ldx];
         for (int b = 0; b < nb; ++b) {
             double2 xb = X[vars[ne + b] + int64_t(col) * sv.ldx];
-            acc = csub(acc, cmul(kT ? f(ne + b, k) : f(k, ne + b), xb));
+            acc = csub(acc, cmul(f(ne + b, k), xb));
         }
         y[idx] = acc;
     }
     __syncthreads();
-    if (!kT) {
-        for (int k = ne - 1; k >= 0; --k) {  // U11 back substitution
-            for (int col = tid; col < c; col += kThreads) y[k + col * ne] = cmul(y[k + col * ne], crecip(f(k, k)));
-            __syncthreads();
-            for (int idx = tid; idx < k * c; idx += kThreads) {
-                int i = idx % k, col = idx / k;
-                y[i + col * ne] = csub(y[i + col * ne], cmul(f(i, k), y[k + col * ne]));
-            }
-            __syncthreads();
+    for (int k = ne - 1; k >= 0; --k) {  // L11^T (unit upper)
+        for (int idx = tid; idx < k * c; idx += kThreads) {
+            int i = idx % k, col = idx / k;
+            y[i + col * ne] = csub(y[i + col * ne], cmul(f(k, i), y[k + col * ne]));
         }
-    } else {
-        for (int k = ne - 1; k >= 0; --k) {  // L11^T (unit upper)
-            for (int idx = tid; idx < k * c; idx += kThreads) {
-                int i = idx % k, col = idx / k;
-                y[i + col * ne] = csub(y[i + col * ne], cmul(f(k, i), y[k + col * ne]));
-            }
-            __syncthreads();
-        }
-        for (int col = tid; col < c; col += kThreads)  // P^T: the swaps in reverse order
-            for (int k = ne - 1; k >= 0; --k) {
-                int q = piv[k];
-                if (q != k) {
-                    double2 t = y[k + col * ne];
-                    y[k + col * ne] = y[q + col * ne];
-                    y[q + col * ne] = t;
-                }
-            }
         __syncthreads();
     }
+    for (int col = tid; col < c; col += kThreads)  // P^T: the swaps in reverse order
+        for (int k = ne - 1; k >= 0; --k) {
+            int q = piv[k];
+            if (q != k) {
+                double2 t = y[k + col * ne];
+                y[k + col * ne] = y[q + col * ne];
+                y[q + col * ne] = t;
+            }
+        }
+    __syncthreads();
     for (int idx = tid; idx < ne * c; idx += kThreads) X[vars[idx % ne] + int64_t(idx / ne) * sv.ldx] = y[idx];
 }
 
-// Dense solve with the root's LU (its leading ne x ne block, complex f64).
-template <bool kT>
+// ---------------------------------------------------------------------------------------------
+// Blocked solves of A x = b (the refinement and the adjoint's G columns). A workgroup holds CC
+// right-hand-side columns of its front in LDS (grid.z covers the columns) and reads each factor
+// entry once, coalesced: the triangles go by diagonal blocks of kDiag rows, each solved by wave 0
+// with one row per lane and the pivot values broadcast by readlane, and the rows past a diagonal
+// block are updated by all lanes as a matrix-vector product (rows over lanes, the K range split
+// across groups of lanes when the rows are few).
+constexpr int kDiag = 64;
+constexpr int kPre = 8;  // factor columns loaded ahead in a diagonal block
+constexpr int kGemvBatch = 4;  // factor entries a lane loads at once in the matrix-vector products
+// Waves per SIMD the blocked solve kernels are compiled for (at most 128 registers a lane, which
+// holds 4 columns without spilling): the solves are latency-bound and need the occupancy more
+// than wider column chunks.
+constexpr int kSolveWaves = 4;
+constexpr int kSolveCols = 4;
+
+// Workgroup -> (front, point, column chunk) with the chunk fastest and each XCD (workgroups go
+// round-robin over the 8 XCDs) taking one contiguous range, so that the chunks of a front and
+// point, which read the same factors, share an L2. grid = (count * chunks, points).
+struct SolveItem {
+    int front, chunk;
+    int64_t point;
+};
+__device__ __forceinline__ SolveItem solve_item(int chunks) {
+    const uint64_t T = gridDim.x, total = T * gridDim.y;
+    const uint64_t lin = blockIdx.x + uint64_t(blockIdx.y) * T;
+    const uint64_t q = total / 8, r = total % 8, x = lin % 8, idx = lin / 8;
+    const uint64_t logical = (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + idx;
+    const uint64_t fc = logical % T;
+    return {int(fc / chunks), int(fc % chunks), int64_t(logical / T)};
+}
+
+__device__ __forceinline__ double lane_value(double v, int k) {
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), k);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), k);
+    return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ double2 lane_value(double2 v, int k) { return double2{lane_value(v.x, k), lane_value(v.y, k)}; }
+
+// y[r] -= sum_{k in [k0, k1)} F[r + k ld] y[k] for r in [r0, r1) (CC columns of y, leading
+// dimension ldy). red: NT x CC words of LDS for the partial sums. Ends with a barrier.
+template <class TF, int NT, int CC>
+__device__ __forceinline__ void gemv_sub(const TF* __restrict__ F, int ld, int r0, int r1, int k0, int k1, double2* y,
+                                         int ldy, double2* red) {
+    const int R = r1 - r0, K = k1 - k0, tid = threadIdx.x;
+    if (R <= 0 || K <= 0) return;  // uniform
+    const int RP = min(NT, (R + 63) / 64 * 64), G = NT / RP, rr = tid % RP, g = tid / RP;
+    const int kc = (K + G - 1) / G, ka = k0 + min(g, G - 1) * kc, kz = g < G ? min(k1, ka + kc) : ka;
+    for (int base = 0; base < R; base += RP) {
+        const int r = r0 + base + rr;
+        const int64_t rc = min(r, r1 - 1);
+        double2 acc[CC];
+#pragma unroll
+        for (int col = 0; col < CC; ++col) acc[col] = double2{0.0, 0.0};
+        int k = ka;
+        for (; k + kGemvBatch <= kz; k += kGemvBatch) {
+            double2 l[kGemvBatch];
+#pragma unroll
+            for (int u = 0; u < kGemvBatch; ++u) l[u] = widen(F[rc + int64_t(k + u) * ld]);
+#pragma unroll
+            for (int u = 0; u < kGemvBatch; ++u)
+#pragma unroll
+                for (int col = 0; col < CC; ++col) acc[col] = cadd(acc[col], cmul(l[u], y[k + u + col * ldy]));
+        }
+        for (; k < kz; ++k) {
+            const double2 l = widen(F[rc + int64_t(k) * ld]);
+#pragma unroll
+            for (int col = 0; col < CC; ++col) acc[col] = cadd(acc[col], cmul(l, y[k + col * ldy]));
+        }
+        if (G == 1) {
+            if (g == 0 && r < r1)
+#pragma unroll
+                for (int col = 0; col < CC; ++col) y[r + col * ldy] = csub(y[r + col * ldy], acc[col]);
+        } else {
+            if (g < G)
+#pragma unroll
+                for (int col = 0; col < CC; ++col) red[tid + col * NT] = acc[col];
+            __syncthreads();
+            if (g == 0 && r < r1)
+#pragma unroll
+                for (int col = 0; col < CC; ++col) {
+                    double2 s = red[tid + col * NT];
+                    for (int q = 1; q < G; ++q) s = cadd(s, red[tid + q * RP + col * NT]);
+                    y[r + col * ldy] = csub(y[r + col * ldy], s);
+                }
+        }
+        __syncthreads();
+    }
+}
+
+// Wave 0: unit-lower diagonal block [kb, kb + w) of F (rows kb + lane), forward substitution.
+template <class TF, int CC>
+__device__ __forceinline__ void diag_lower(const TF* __restrict__ F, int ld, int kb, int w, double2* y, int ldy) {
+    const int lane = threadIdx.x;
+    const bool mine = lane < w;
+    const int64_t ic = kb + min(lane, w - 1);
+    double2 v[CC];
+#pragma unroll
+    for (int col = 0; col < CC; ++col) v[col] = mine ? y[kb + lane + col * ldy] : double2{0.0, 0.0};
+    for (int k0 = 0; k0 < w; k0 += kPre) {
+        double2 l[kPre];
+#pragma unroll
+        for (int u = 0; u < kPre; ++u) l[u] = widen(F[ic + int64_t(kb + min(k0 + u, w - 1)) * ld]);
+#pragma unroll
+        for (int u = 0; u < kPre; ++u) {
+            const int k = k0 + u;
+            if (k < w) {
+#pragma unroll
+                for (int col = 0; col < CC; ++col) {
+                    const double2 x = lane_value(v[col], k);
+                    if (lane > k) v[col] = csub(v[col], cmul(l[u], x));
+                }
+            }
+        }
+    }
+    if (mine)
+#pragma unroll
+        for (int col = 0; col < CC; ++col) y[kb + lane + col * ldy] = v[col];
+}
+
+// Wave 0: upper diagonal block [kb, kb + w) of F, back substitution (non-unit diagonal).
+template <class TF, int CC>
+__device__ __forceinline__ void diag_upper(const TF* __restrict__ F, int ld, int kb, int w, double2* y, int ldy) {
+    const int lane = threadIdx.x;
+    const bool mine = lane < w;
+    const int64_t ic = kb + min(lane, w - 1);
+    double2 v[CC];
+#pragma unroll
+    for (int col = 0; col < CC; ++col) v[col] = mine ? y[kb + lane + col * ldy] : double2{0.0, 0.0};
+    for (int top = w - 1; top >= 0; top -= kPre) {
+        double2 u[kPre];
+#pragma unroll
+        for (int s = 0; s < kPre; ++s) u[s] = widen(F[ic + int64_t(kb + max(top - s, 0)) * ld]);
+#pragma unroll
+        for (int s = 0; s < kPre; ++s) {
+            const int k = top - s;
+            if (k >= 0) {
+                const double2 inv = crecip(lane_value(u[s], k));  // 1 / U[kb + k, kb + k]
+#pragma unroll
+                for (int col = 0; col < CC; ++col) {
+                    const double2 x = cmul(lane_value(v[col], k), inv);
+                    if (lane == k)
+                        v[col] = x;
+                    else if (lane < k)
+                        v[col] = csub(v[col], cmul(u[s], x));
+                }
+            }
+        }
+    }
+    if (mine)
+#pragma unroll
+        for (int col = 0; col < CC; ++col) y[kb + lane + col * ldy] = v[col];
+}
+
+// Forward substitution with the unit-lower columns [0, ne) of F over the rows [0, nrows).
+template <class TF, int NT, int CC>
+__device__ __forceinline__ void lower_blocked(const TF* F, int ld, int ne, int nrows, double2* y, int ldy, double2* red) {
+    for (int kb = 0; kb < ne; kb += kDiag) {
+        const int w = min(kDiag, ne - kb);
+        if (threadIdx.x < 64) diag_lower<TF, CC>(F, ld, kb, w, y, ldy);
+        __syncthreads();
+        gemv_sub<TF, NT, CC>(F, ld, kb + w, nrows, kb, kb + w, y, ldy, red);
+    }
+}
+
+// Back substitution with the upper triangle of F[0:ne, 0:ne].
+template <class TF, int NT, int CC>
+__device__ __forceinline__ void upper_blocked(const TF* F, int ld, int ne, double2* y, int ldy, double2* red) {
+    for (int kb = (ne - 1) / kDiag * kDiag; kb >= 0; kb -= kDiag) {
+        const int w = min(kDiag, ne - kb);
+        if (threadIdx.x < 64) diag_upper<TF, CC>(F, ld, kb, w, y, ldy);
+        __syncthreads();
+        gemv_sub<TF, NT, CC>(F, ld, 0, kb, kb, kb + w, y, ldy, red);
+    }
+}
+
+// P y: LAPACK's interchanges in order, one lane per column.
+__device__ __forceinline__ void apply_pivots(const int32_t* __restrict__ piv, int ne, double2* y, int ldy, int cw) {
+    if (int(threadIdx.x) < cw) {
+        double2* yc = y + threadIdx.x * ldy;
+        for (int k = 0; k < ne; ++k) {
+            const int q = piv[k];
+            if (q > k && q < ne) {  // pivots of a front that delayed are stale; its point is not used
+                const double2 t = yc[k];
+                yc[k] = yc[q];
+                yc[q] = t;
+            }
+        }
+    }
+    __syncthreads();
+}
+
+// Forward step of a front: y = L11^-1 P b_e on the separator, b_b -= L21 y on the boundary
+// (accumulated with FP64 atomics: sibling fronts share boundary variables).
+template <class TF, int NT, int CC>
+__global__ void __launch_bounds__(NT, kSolveWaves)
+    fwd_blocked_kernel(const NodeDesc* __restrict__ nodes, const int32_t* __restrict__ level, PointsView pv, SolveView sv) {
+    extern __shared__ double2 ys[];  // n x CC, then NT x CC partial sums
+    const SolveItem it = solve_item((sv.c + CC - 1) / CC);
+    const NodeDesc nd = nodes[level[it.front]];
+    const int64_t p = it.point;
+    const int c0 = it.chunk * CC, cw = min(CC, sv.c - c0);
+    if (cw <= 0) return;
+    const TF* F = reinterpret_cast<const TF*>(pv.arena) + p * pv.arena_stride + nd.off;
+    const int32_t* piv = pv.piv + p * pv.piv_stride + nd.piv_off;
+    const int32_t* vars = sv.vars + nd.var_off;
+    double2* X = sv.x + p * sv.x_stride + int64_t(c0) * sv.ldx;
+    const int n = nd.n, ne = nd.ne, tid = threadIdx.x;
+    double2* red = ys + n * CC;
+    for (int idx = tid; idx < n * CC; idx += NT) {
+        const int r = idx % n, col = idx / n;
+        ys[idx] = (r < ne && col < cw) ? X[vars[r] + int64_t(col) * sv.ldx] : double2{0.0, 0.0};
+    }
+    __syncthreads();
+    apply_pivots(piv, ne, ys, n, cw);
+    lower_blocked<TF, NT, CC>(F, n, ne, n, ys, n, red);
+    for (int idx = tid; idx < n * cw; idx += NT) {
+        const int r = idx % n, col = idx / n;
+        double2* at = &X[vars[r] + int64_t(col) * sv.ldx];
+        const double2 v = ys[r + col * n];
+        if (r < ne) {
+            *at = v;
+        } else {
+            atomicAdd(&at->x, v.x);
+            atomicAdd(&at->y, v.y);
+        }
+    }
+}
+
+// Backward step of a front: x_e = U11^-1 (y - U12 x_b), the boundary values final.
+template <class TF, int NT, int CC>
+__global__ void __launch_bounds__(NT, kSolveWaves)
+    bwd_blocked_kernel(const NodeDesc* __restrict__ nodes, const int32_t* __restrict__ level, PointsView pv, SolveView sv) {
+    extern __shared__ double2 ys[];
+    const SolveItem it = solve_item((sv.c + CC - 1) / CC);
+    const NodeDesc nd = nodes[level[it.front]];
+    const int64_t p = it.point;
+    const int c0 = it.chunk * CC, cw = min(CC, sv.c - c0);
+    if (cw <= 0) return;
+    const TF* F = reinterpret_cast<const TF*>(pv.arena) + p * pv.arena_stride + nd.off;
+    const int32_t* vars = sv.vars + nd.var_off;
+    double2* X = sv.x + p * sv.x_stride + int64_t(c0) * sv.ldx;
+    const int n = nd.n, ne = nd.ne, tid = threadIdx.x;
+    double2* red = ys + n * CC;
+    for (int idx = tid; idx < n * CC; idx += NT) {
+        const int r = idx % n, col = idx / n;
+        ys[idx] = col < cw ? X[vars[r] + int64_t(col) * sv.ldx] : double2{0.0, 0.0};
+    }
+    __syncthreads();
+    gemv_sub<TF, NT, CC>(F, n, 0, ne, ne, n, ys, n, red);
+    upper_blocked<TF, NT, CC>(F, n, ne, ys, n, red);
+    for (int idx = tid; idx < ne * cw; idx += NT) {
+        const int r = idx % ne, col = idx / ne;
+        X[vars[r] + int64_t(col) * sv.ldx] = ys[r + col * n];
+    }
+}
+
+// The root: P, L, U of its leading ne x ne block (complex f64).
+template <int NT, int CC>
+__global__ void __launch_bounds__(NT, kSolveWaves)
+    root_blocked_kernel(const NodeDesc* __restrict__ nodes, const int32_t* __restrict__ level, PointsView pv, SolveView sv) {
+    extern __shared__ double2 ys[];
+    const SolveItem it = solve_item((sv.c + CC - 1) / CC);
+    const NodeDesc nd = nodes[level[0]];
+    const int64_t p = it.point;
+    const int c0 = it.chunk * CC, cw = min(CC, sv.c - c0);
+    if (cw <= 0) return;
+    const double2* F = reinterpret_cast<const double2*>(pv.root_arena) + p * pv.root_stride + nd.off;
+    const int32_t* piv = pv.piv + p * pv.piv_stride + nd.piv_off;
+    const int32_t* vars = sv.vars + nd.var_off;
+    double2* X = sv.x + p * sv.x_stride + int64_t(c0) * sv.ldx;
+    const int n = nd.n, ne = nd.ne, tid = threadIdx.x;
+    double2* red = ys + ne * CC;
+    for (int idx = tid; idx < ne * CC; idx += NT) {
+        const int r = idx % ne, col = idx / ne;
+        ys[idx] = col < cw ? X[vars[r] + int64_t(col) * sv.ldx] : double2{0.0, 0.0};
+    }
+    __syncthreads();
+    apply_pivots(piv, ne, ys, ne, cw);
+    lower_blocked<double2, NT, CC>(F, n, ne, ne, ys, ne, red);
+    upper_blocked<double2, NT, CC>(F, n, ne, ys, ne, red);
+    for (int idx = tid; idx < ne * cw; idx += NT) {
+        const int r = idx % ne, col = idx / ne;
+        X[vars[r] + int64_t(col) * sv.ldx] = ys[r + col * ne];
+    }
+}
+
+// Transposed dense solve with the root's LU (its leading ne x ne block, complex f64).
 __global__ void __launch_bounds__(kThreads)
-    solve_root_kernel(const NodeDesc* __restrict__ nodes, const int32_t* __restrict__ level, PointsView pv, SolveView sv) {
+    solve_root_t_kernel(const NodeDesc* __restrict__ nodes, const int32_t* __restrict__ level, PointsView pv, SolveView sv) {
     extern __shared__ double2 y[];
     const NodeDesc nd = nodes[level[0]];
     const int64_t p = blockIdx.x;
@@ -172,25 +426,7 @@ __global__ void __launch_bounds__(kThreads)
             }
         __syncthreads();
     };
-    if (!kT) {
-        swaps(false);
-        for (int k = 0; k < ne; ++k) {
-            for (int idx = tid; idx < (ne - k - 1) * c; idx += kThreads) {
-                int i = k + 1 + idx % (ne - k - 1), col = idx / (ne - k - 1);
-                y[i + col * ne] = csub(y[i + col * ne], cmul(f(i, k), y[k + col * ne]));
-            }
-            __syncthreads();
-        }
-        for (int k = ne - 1; k >= 0; --k) {
-            for (int col = tid; col < c; col += kThreads) y[k + col * ne] = cmul(y[k + col * ne], crecip(f(k, k)));
-            __syncthreads();
-            for (int idx = tid; idx < k * c; idx += kThreads) {
-                int i = idx % k, col = idx / k;
-                y[i + col * ne] = csub(y[i + col * ne], cmul(f(i, k), y[k + col * ne]));
-            }
-            __syncthreads();
-        }
-    } else {
+    {
         for (int k = 0; k < ne; ++k) {  // U^T a = b
             for (int col = tid; col < c; col += kThreads) y[k + col * ne] = cmul(y[k + col * ne], crecip(f(k, k)));
             __syncthreads();
@@ -258,20 +494,23 @@ __global__ void __launch_bounds__(kThreads)
     }
 }
 
-// X += D on the block rows; ratio[p] = max |D| / max |X| (the relative correction).
+// X += D on the block rows; maxes[2p] = max |D|, maxes[2p + 1] = max |X| (the relative
+// correction is their ratio), as the bit patterns of non-negative doubles, which order like the
+// integers. A NaN correction counts as infinite. maxes is zero on entry.
 __global__ void __launch_bounds__(kThreads)
     update_kernel(SolveView xv, const double2* __restrict__ d, const int32_t* __restrict__ rows, int32_t nrows,
-                  double* __restrict__ ratio) {
+                  unsigned long long* __restrict__ maxes) {
     __shared__ double sd[kThreads / 64], sx[kThreads / 64];
-    const int64_t p = blockIdx.x;
+    const int64_t p = blockIdx.y;
     double2* X = xv.x + p * xv.x_stride;
     const double2* D = d + p * xv.x_stride;
     double md = 0.0, mx = 0.0;
-    for (int idx = threadIdx.x; idx < nrows * xv.c; idx += kThreads) {
+    for (int idx = blockIdx.x * kThreads + threadIdx.x; idx < nrows * xv.c; idx += gridDim.x * kThreads) {
         const int64_t at = rows[idx % nrows] + int64_t(idx / nrows) * xv.ldx;
-        double2 v = cadd(X[at], D[at]);
+        const double2 dv = D[at], v = cadd(X[at], dv);
         X[at] = v;
-        md = fmax(md, cabs(D[at]));
+        const double a = cabs(dv);
+        md = fmax(md, a == a ? a : INFINITY);
         mx = fmax(mx, cabs(v));
     }
     for (int off = 32; off > 0; off >>= 1) {
@@ -288,9 +527,8 @@ __global__ void __launch_bounds__(kThreads)
             md = fmax(md, sd[w]);
             mx = fmax(mx, sx[w]);
         }
-        md = fmax(md, sd[0]);
-        mx = fmax(mx, sx[0]);
-        ratio[p] = mx > 0.0 ? md / mx : (md > 0.0 ? INFINITY : 0.0);
+        atomicMax(&maxes[2 * p], static_cast<unsigned long long>(__double_as_longlong(md)));
+        atomicMax(&maxes[2 * p + 1], static_cast<unsigned long long>(__double_as_longlong(mx)));
     }
 }
 
@@ -326,10 +564,13 @@ void launch_residual(const PointsView& pv, const SolveView& xv, const double2* b
                        rows, nrows, row_ptr, cols, left_pos, right_pos, left_orbs, nl, right_orbs, nr);
 }
 
-void launch_update(const SolveView& xv, const double2* d, const int32_t* rows, int32_t nrows, double* ratio,
+void launch_update(const SolveView& xv, const double2* d, const int32_t* rows, int32_t nrows, double* maxes,
                    int32_t points, hipStream_t st) {
     if (points <= 0 || xv.c <= 0) return;
-    hipLaunchKernelGGL(update_kernel, dim3(unsigned(points)), dim3(kThreads), 0, st, xv, d, rows, nrows, ratio);
+    if (hipMemsetAsync(maxes, 0, size_t(points) * 2 * sizeof(double), st) != hipSuccess)
+        throw std::runtime_error("hipMemsetAsync of the refinement maxima failed");
+    hipLaunchKernelGGL(update_kernel, rows_grid(int64_t(nrows) * xv.c, points), dim3(kThreads), 0, st, xv, d, rows,
+                       nrows, reinterpret_cast<unsigned long long*>(maxes));
 }
 
 void launch_gather(const SolveView& xv, const int32_t* orbs, int32_t m, double2* out, int32_t points, hipStream_t st) {
@@ -339,32 +580,102 @@ void launch_gather(const SolveView& xv, const int32_t* orbs, int32_t m, double2*
 
 using SolveKernel = void (*)(const NodeDesc*, const int32_t*, PointsView, SolveView);
 
-void launch_solve_forward(const NodeDesc* nodes, const int32_t* level, int32_t count, const PointsView& pv,
-                          const SolveView& sv, int32_t points, bool single, bool transpose, hipStream_t st) {
-    if (count <= 0 || points <= 0) return;
-    SolveKernel k = single ? (transpose ? solve_forward_kernel<float2, true> : solve_forward_kernel<float2, false>)
-                           : (transpose ? solve_forward_kernel<double2, true> : solve_forward_kernel<double2, false>);
-    hipLaunchKernelGGL(k, dim3(unsigned(count), unsigned(points)), dim3(kThreads), size_t(kSolveLds) * sizeof(double2),
-                       st, nodes, level, pv, sv);
+// Shape of a blocked solve launch over fronts of at most `rows` rows and c columns: threads,
+// columns per workgroup (kSolveCols, or one when c is one or the LDS would pass 64 KB; one column
+// fits up to kSolveMaxLds) and LDS bytes (the columns, then the partial sums).
+struct SolveShape {
+    int nt, cc;
+    size_t lds;
+};
+SolveShape solve_shape(int rows, int c, int nt) {
+    auto bytes = [&](int cc) { return size_t(rows + nt) * cc * sizeof(double2); };
+    int cc = c <= 1 ? 1 : kSolveCols;
+    if (bytes(cc) > 64 * 1024) cc = 1;
+    return {nt, cc, bytes(cc)};
+}
+SolveShape solve_shape(int rows, int c) { return solve_shape(rows, c, rows <= 128 ? 64 : 256); }
+
+template <template <class, int, int> class K, class TF, int NT>
+SolveKernel by_cc(int cc) {
+    return cc == kSolveCols ? K<TF, NT, kSolveCols>::fn : K<TF, NT, 1>::fn;
+}
+template <class TF, int NT, int CC>
+struct Fwd {
+    static constexpr SolveKernel fn = fwd_blocked_kernel<TF, NT, CC>;
+};
+template <class TF, int NT, int CC>
+struct Bwd {
+    static constexpr SolveKernel fn = bwd_blocked_kernel<TF, NT, CC>;
+};
+template <class, int NT, int CC>
+struct Root {
+    static constexpr SolveKernel fn = root_blocked_kernel<NT, CC>;
+};
+template <template <class, int, int> class K, class TF>
+SolveKernel pick(const SolveShape& s) {
+    return s.nt == 64 ? by_cc<K, TF, 64>(s.cc) : by_cc<K, TF, 256>(s.cc);
 }
 
-void launch_solve_backward(const NodeDesc* nodes, const int32_t* level, int32_t count, const PointsView& pv,
-                           const SolveView& sv, int32_t points, bool single, bool transpose, hipStream_t st) {
-    if (count <= 0 || points <= 0) return;
-    SolveKernel k = single ? (transpose ? solve_backward_kernel<float2, true> : solve_backward_kernel<float2, false>)
-                           : (transpose ? solve_backward_kernel<double2, true> : solve_backward_kernel<double2, false>);
-    hipLaunchKernelGGL(k, dim3(unsigned(count), unsigned(points)), dim3(kThreads), size_t(kSolveLds) * sizeof(double2),
-                       st, nodes, level, pv, sv);
+// Dynamic LDS past 64 KB has to be allowed per kernel, once.
+void allow_solve_lds(SolveKernel k, size_t lds) {
+    if (lds <= 64 * 1024) return;
+    static std::mutex mu;
+    static std::set<const void*> done;
+    std::lock_guard<std::mutex> g(mu);
+    if (done.insert(reinterpret_cast<const void*>(k)).second)
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(k), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  kSolveMaxLds);
 }
 
-void launch_solve_root(const NodeDesc* nodes, const int32_t* level, const PointsView& pv, const SolveView& sv,
-                       int32_t points, bool transpose, hipStream_t st) {
-    if (points <= 0) return;
-    const size_t lds = size_t(kSolveLds) * sizeof(double2);
-    if (transpose)
-        hipLaunchKernelGGL((solve_root_kernel<true>), dim3(unsigned(points)), dim3(kThreads), lds, st, nodes, level, pv, sv);
-    else
-        hipLaunchKernelGGL((solve_root_kernel<false>), dim3(unsigned(points)), dim3(kThreads), lds, st, nodes, level, pv, sv);
+bool solve_fits(int rows) { return solve_shape(rows, 1, 256).lds <= size_t(kSolveMaxLds); }
+
+void launch_solve_forward(const NodeDesc* nodes, const int32_t* level, int32_t count, int32_t max_n,
+                          const PointsView& pv, const SolveView& sv, int32_t points, bool single, bool transpose,
+                          hipStream_t st) {
+    if (count <= 0 || points <= 0 || sv.c <= 0) return;
+    if (transpose) {
+        SolveKernel k = single ? solve_forward_t_kernel<float2> : solve_forward_t_kernel<double2>;
+        hipLaunchKernelGGL(k, dim3(unsigned(count), unsigned(points)), dim3(kThreads),
+                           size_t(kSolveLds) * sizeof(double2), st, nodes, level, pv, sv);
+        return;
+    }
+    const SolveShape s = solve_shape(max_n, sv.c);
+    SolveKernel k = single ? pick<Fwd, float2>(s) : pick<Fwd, double2>(s);
+    allow_solve_lds(k, s.lds);
+    hipLaunchKernelGGL(k, dim3(unsigned(count * ((sv.c + s.cc - 1) / s.cc)), unsigned(points)), dim3(s.nt), s.lds, st,
+                       nodes, level, pv, sv);
+}
+
+void launch_solve_backward(const NodeDesc* nodes, const int32_t* level, int32_t count, int32_t max_n,
+                           const PointsView& pv, const SolveView& sv, int32_t points, bool single, bool transpose,
+                           hipStream_t st) {
+    if (count <= 0 || points <= 0 || sv.c <= 0) return;
+    if (transpose) {
+        SolveKernel k = single ? solve_backward_t_kernel<float2> : solve_backward_t_kernel<double2>;
+        hipLaunchKernelGGL(k, dim3(unsigned(count), unsigned(points)), dim3(kThreads),
+                           size_t(kSolveLds) * sizeof(double2), st, nodes, level, pv, sv);
+        return;
+    }
+    const SolveShape s = solve_shape(max_n, sv.c);
+    SolveKernel k = single ? pick<Bwd, float2>(s) : pick<Bwd, double2>(s);
+    allow_solve_lds(k, s.lds);
+    hipLaunchKernelGGL(k, dim3(unsigned(count * ((sv.c + s.cc - 1) / s.cc)), unsigned(points)), dim3(s.nt), s.lds, st,
+                       nodes, level, pv, sv);
+}
+
+void launch_solve_root(const NodeDesc* nodes, const int32_t* level, int32_t root_ne, const PointsView& pv,
+                       const SolveView& sv, int32_t points, bool transpose, hipStream_t st) {
+    if (points <= 0 || sv.c <= 0) return;
+    if (transpose) {
+        hipLaunchKernelGGL(solve_root_t_kernel, dim3(unsigned(points)), dim3(kThreads),
+                           size_t(kSolveLds) * sizeof(double2), st, nodes, level, pv, sv);
+        return;
+    }
+    const SolveShape s = solve_shape(root_ne, sv.c, 256);
+    const dim3 grid(unsigned((sv.c + s.cc - 1) / s.cc), unsigned(points));
+    const SolveKernel k = by_cc<Root, double2, 256>(s.cc);
+    allow_solve_lds(k, s.lds);
+    hipLaunchKernelGGL(k, grid, dim3(256), s.lds, st, nodes, level, pv, sv);
 }
 
 }  // namespace caroli::gpu

@@ -122,6 +122,8 @@ typedef struct cg_batch_stats {
   int64_t delayed_points;   /* points redone with delayed pivots */
   int64_t refinement_iters; /* mixed precision: total refinement iterations */
   double max_refine_resid;  /* mixed precision: largest final relative correction */
+  int64_t recomputed_points; /* mixed precision: points recomputed in complex f64 (their f32
+                                fronts delayed or their refinement did not converge) */
 } cg_batch_stats;
 
 /* The batched launch: point p is (systems[p], sigmas[p]); all energies x configurations (x

@@ -49,8 +49,10 @@ def test_graphene_512x64_double_and_mixed():
     stats = ca.capi.BatchStats()
     tm, st = ca.batch_transmission(pts_s, pts_g, ca.Precision.Mixed, stats)
     assert (st == 0).all(), st
-    say(f"    mixed max |dT| {np.abs(tm - ref).max():.2e}, refinement iterations {stats.refinement_iters}")
+    say(f"    mixed max |dT| {np.abs(tm - ref).max():.2e}, refinement iterations {stats.refinement_iters}, "
+        f"recomputed in f64 {stats.recomputed_points}")
     assert np.abs(tm - ref).max() <= 1e-8
+    assert stats.refinement_iters > 0 and stats.recomputed_points <= 1
 
 
 def test_adjoint_bhz_100x32():
@@ -98,3 +100,11 @@ def test_cubic_uneven_batch():
     t, st = ca.batch_transmission(pts_s, pts_g, ca.Precision.Double)
     assert (st == 0).all(), st
     assert np.abs(t - ref).max() <= 1e-10 * (1 + np.abs(ref).max()), np.abs(t - ref).max()
+    # Mixed: many channels (several column chunks per solve), separators of 256 and 576 variables.
+    stats = ca.capi.BatchStats()
+    tm, st = ca.batch_transmission(pts_s, pts_g, ca.Precision.Mixed, stats)
+    assert (st == 0).all(), st
+    say(f"    mixed max |dT| {np.abs(tm - ref).max():.2e}, refinement iterations {stats.refinement_iters}, "
+        f"recomputed in f64 {stats.recomputed_points}")
+    assert np.abs(tm - ref).max() <= 1e-8
+    assert stats.refinement_iters > 0 and stats.recomputed_points <= 1

@@ -21,7 +21,7 @@ def say(*a):
     print(*a, flush=True)
 
 
-def mixed_vs_oracle(systems, energies, label, tol=1e-8):
+def mixed_vs_oracle(systems, energies, label, tol=1e-8, max_recomputed=0.25):
     sig = [systems[0].self_energies(e) for e in energies]
     pts_s, pts_g, ref = [], [], []
     for s in systems:
@@ -38,8 +38,12 @@ def mixed_vs_oracle(systems, energies, label, tol=1e-8):
     assert (st == 0).all(), st
     d = np.abs(t - np.array(ref))
     say(f"    max |dT| {d.max():.2e}, refinement iterations {stats.refinement_iters}, "
-        f"last correction {stats.max_refine_resid:.1e}, recomputed {stats.delayed_points}")
+        f"last correction {stats.max_refine_resid:.1e}, recomputed in f64 {stats.recomputed_points} "
+        f"({stats.delayed_points} delayed)")
     assert d.max() <= tol, d
+    # the f32 fronts plus refinement must carry the points, not the complex f64 recomputation
+    assert stats.refinement_iters > 0
+    assert stats.recomputed_points <= max_recomputed * len(pts_s), stats.recomputed_points
 
 
 def test_mixed_bhz_magnetic():
@@ -57,7 +61,7 @@ def test_mixed_graphene_anderson():
 def test_mixed_square_with_delays():
     # The clean lattice at E = 0 has singular interior blocks: those points are recomputed in f64.
     s = ca.NestedTwoTerminal.preset(ca.Model.Square, [14, 9, 0])
-    mixed_vs_oracle([s], [-1.3, 0.0, 0.4], "square 14x9 clean")
+    mixed_vs_oracle([s], [-1.3, 0.0, 0.4], "square 14x9 clean", max_recomputed=1.0)
 
 
 def test_adjoint_matches_fp64_gradient_and_finite_differences():
