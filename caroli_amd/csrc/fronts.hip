@@ -447,6 +447,156 @@ __global__ void __launch_bounds__(NT)
 }
 
 // ---------------------------------------------------------------------------------------------
+// Small fronts (n <= 64), register-resident elimination. The front is assembled in LDS as in
+// front_lds_kernel, then lane r of every wave takes row r, and wave w the columns j = w (mod NW),
+// S of them, into registers. Step k of the partial LU:
+//  - the wave holding column k finds the pivot among the rows of logical position [k, lim) by a
+//    wave reduction (largest modulus, smallest logical position on ties), applies the pivot test
+//    of field.rs:61, swaps the logical positions of the pivot row and of the row at k (the rows
+//    stay in their lanes), turns column k of the rows below into the multipliers l = F[r, k] /
+//    F[p, k], and publishes l and the pivot row through LDS (double-buffered by step parity:
+//    one barrier a step);
+//  - every wave then updates its columns j > k: F[r, j] -= l_r F[p, j], with F[p, j] read from
+//    the pivot row's lane of the same wave by readlane (every wave holds every row).
+// LDS carries 64 multipliers a step instead of the whole trailing block. The factors are
+// written back in logical row order as front_lds_kernel leaves them (LAPACK's interchanges of
+// whole rows, the layout the solves read).
+template <class T>
+__device__ __forceinline__ T lane_of(T v, int l);
+template <>
+__device__ __forceinline__ double2 lane_of(double2 v, int l) {
+    auto one = [&](double x) {
+        const int lo = __builtin_amdgcn_readlane(__double2loint(x), l);
+        const int hi = __builtin_amdgcn_readlane(__double2hiint(x), l);
+        return __hiloint2double(hi, lo);
+    };
+    return double2{one(v.x), one(v.y)};
+}
+template <>
+__device__ __forceinline__ float2 lane_of(float2 v, int l) {
+    return float2{__int_as_float(__builtin_amdgcn_readlane(__float_as_int(v.x), l)),
+                  __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v.y), l))};
+}
+
+template <class T, int NW, int S, bool kAssemble>
+__global__ void __launch_bounds__(64 * NW)
+    front_reg_kernel(const NodeDesc* __restrict__ nodes, const int32_t* __restrict__ level,
+                     const Entry* __restrict__ ent, const int32_t* __restrict__ diag,
+                     const KidDesc* __restrict__ kids, const int32_t* __restrict__ maps, PointsView pv, double tol) {
+    constexpr int NT = 64 * NW;
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    __shared__ T lbuf[2][64];
+    __shared__ int32_t pinfo[2][2];
+    __shared__ double smax[NW];
+    T* W = reinterpret_cast<T*>(lds_raw);
+    const NodeDesc nd = nodes[level[blockIdx.x]];
+    const int64_t p = blockIdx.y;
+    T* F = reinterpret_cast<T*>(pv.arena) + p * pv.arena_stride + nd.off;
+    int32_t* piv = pv.piv + p * pv.piv_stride + nd.piv_off;
+    int32_t* status = pv.status + p * pv.status_stride + nd.slot;
+    const int n = nd.n, ne = nd.ne, ne1 = nd.ne1, tid = threadIdx.x, ld = n + 1;
+    const T zero = cx<T>(0.0, 0.0);
+    if (kAssemble) {
+        for (int i = tid; i < n * ld; i += NT) W[i] = zero;
+        __syncthreads();
+        add_entries<NT>(ent, nd.ent_beg, nd.ent_end, pv.vals[p], ld,
+                        [&](int at, double2 v) { W[at] = cadd(W[at], convert<T>(v)); });
+        __syncthreads();
+        const T z = convert<T>(pv.z[p]);
+        for (int d = nd.diag_beg + tid; d < nd.diag_end; d += NT) {
+            const int q = diag[d];
+            W[q + q * ld] = cadd(W[q + q * ld], z);
+        }
+        __syncthreads();
+        const T* kidbase = reinterpret_cast<const T*>(pv.arena) + p * pv.arena_stride;
+        for (int c = nd.kid_beg; c < nd.kid_end; ++c) {
+            const KidDesc kd = kids[c];
+            extend_add<NT>(kidbase + kd.src, kd.ld, maps + kd.map_beg, kd.size, ld,
+                           [&](int at, T v) { W[at] = cadd(W[at], v); });
+            __syncthreads();
+        }
+    } else {
+        for (int idx = tid; idx < n * n; idx += NT) W[(idx % n) + (idx / n) * ld] = F[idx];
+        __syncthreads();
+    }
+    const int r = tid & 63, wave = tid >> 6;
+    T w[S];
+    double m = 0.0;
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        const int j = wave + s * NW;
+        w[s] = (r < n && j < n) ? W[r + j * ld] : zero;
+        if (r < ne1 && j < ne1) m = fmax(m, cabs(w[s]));
+    }
+    for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
+    if (r == 0) smax[wave] = m;
+    __syncthreads();
+    double scale = smax[0];
+#pragma unroll
+    for (int q = 1; q < NW; ++q) scale = fmax(scale, smax[q]);
+    int lg = r;  // logical position of this lane's row
+    // Column k sits in w[k / NW] of wave k % NW; it is selected (and written) by unrolled
+    // compares so that w stays in registers.
+    for (int k = 0; k < ne; ++k) {
+        const int kw = k % NW, ks = k / NW, par = k & 1;
+        if (wave == kw) {
+            T colk = zero;
+#pragma unroll
+            for (int s = 0; s < S; ++s)
+                if (s == ks) colk = w[s];
+            const int lim = k < ne1 ? ne1 : ne;
+            double big = -1.0;
+            int at = kLdsMaxN, ph = 0;
+            if (r < n && lg >= k && lg < lim) {
+                big = cabs(colk);
+                at = lg;
+                ph = r;
+            }
+            wave_argmax(big, at, ph);
+            const bool ok = k < ne1 ? (big > tol * scale) : (big > 0.0 && big < INFINITY);
+            const int phs = __builtin_amdgcn_readfirstlane(ph);
+            const int lgn = r == phs ? k : (lg == k ? at : lg);
+            T l = zero;
+            if (ok) {
+                const T inv = crecip(lane_of(colk, phs));
+                if (r < n && lgn > k) {
+                    l = cmul(colk, inv);
+#pragma unroll
+                    for (int s = 0; s < S; ++s)
+                        if (s == ks) w[s] = l;
+                }
+            }
+            lbuf[par][r] = l;
+            if (r == 0) {
+                pinfo[par][0] = ok ? phs : -1;
+                pinfo[par][1] = at;
+                if (ok)
+                    piv[k] = at;
+                else
+                    *status = k < ne1 ? kFrontDelayed : kFrontSingular;
+            }
+        }
+        __syncthreads();
+        const int ph = __builtin_amdgcn_readfirstlane(pinfo[par][0]);
+        if (ph < 0) return;  // the whole workgroup: the pivot test failed
+        const int at = pinfo[par][1];
+        lg = r == ph ? k : (lg == k ? at : lg);
+        const T l = lbuf[par][r];
+#pragma unroll
+        for (int s = 0; s < S; ++s)
+            if (wave + s * NW > k) w[s] = csub(w[s], cmul(l, lane_of(w[s], ph)));
+    }
+    if (r < n) {
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+            const int j = wave + s * NW;
+            if (j < n && (pv.keep || (lg >= ne && j >= ne))) F[lg + j * n] = w[s];
+        }
+    }
+    if (tid == 0) *status = kFrontOk;
+}
+
+// ---------------------------------------------------------------------------------------------
 // Large fronts, step 1: the border. Partial LU of F11 (row pivoting within the separator rows,
 // the pivot test of field.rs:61), L21 = F21 U11^-1 and U12 = L11^-1 P F12, in place, panel by
 // panel (kPanel columns), without touching F22: the Schur update S = F22 - L21 U12 is left to
@@ -826,13 +976,16 @@ __global__ void __launch_bounds__(256)
     const int n = nd.n, ne = nd.ne, nb = n - ne, tid = threadIdx.x, la = ne + 1;
     T* A = reinterpret_cast<T*>(lds_raw);  // L11 (kind 0) or U11 (kind 1), leading dimension la
     const bool upper = tl.pad != 0;
-    if (!upper && tid == 0) {  // the rows of P F12: row i is the front's row src[i]
-        for (int i = 0; i < ne; ++i) src[i] = i;
+    if (!upper && tid < 64) {  // the rows of P F12: row i is the front's row src[i] (ne <= 64)
+        // Lane i holds src[i]; LAPACK's interchanges applied in order, two readlanes each.
+        int v = tid;
+        const int pk = tid < ne ? piv[tid] : 0;
         for (int k = 0; k < ne; ++k) {
-            const int t = src[k];
-            src[k] = src[piv[k]];
-            src[piv[k]] = t;
+            const int q = __builtin_amdgcn_readlane(pk, k);
+            const int vk = __builtin_amdgcn_readlane(v, k), vq = __builtin_amdgcn_readlane(v, q);
+            v = tid == k ? vq : (tid == q ? vk : v);
         }
+        if (tid < ne) src[tid] = v;
     }
     for (int idx = tid; idx < ne * ne; idx += 256) A[(idx % ne) + (idx / ne) * la] = F[(idx % ne) + int64_t(idx / ne) * n];
     __syncthreads();
@@ -933,21 +1086,39 @@ __global__ void __launch_bounds__(256)
             cr[x][y] = typename M::acc{0, 0, 0, 0};
             ci[x][y] = typename M::acc{0, 0, 0, 0};
         }
-    for (int kc = 0; kc < ne; kc += kChunk) {
-        // Clamped loads and a select of the value (a select of addresses would go through flat).
-        for (int e = tid; e < kTile * kChunk; e += 256) {
-            const int r = e % kTile, c = e / kTile, i = tl.i0 + r, k = kc + c;
-            T v = F[(ne + min(i, nb - 1)) + int64_t(min(k, ne - 1)) * n];
-            if (!(i < nb && k < ne)) v = zero;
-            As[c * kTileLd + r] = v;
+    // The K loop in chunks of kChunk: the next chunk's L21 / U12 entries are loaded into registers
+    // (kStage per thread and operand) while the MFMAs run on the current one from LDS. Clamped
+    // loads and a select of the value (a select of addresses would go through flat).
+    constexpr int kStage = kTile * kChunk / 256;
+    T sa[kStage], sb[kStage];
+    auto fetch = [&](int kc) {
+#pragma unroll
+        for (int u = 0; u < kStage; ++u) {
+            const int e = tid + u * 256;
+            {
+                const int r = e % kTile, c = e / kTile, i = tl.i0 + r, k = kc + c;
+                T v = F[(ne + min(i, nb - 1)) + int64_t(min(k, ne - 1)) * n];
+                if (!(i < nb && k < ne)) v = zero;
+                sa[u] = v;
+            }
+            {
+                const int c = e % kChunk, r = e / kChunk, j = tl.j0 + r, k = kc + c;
+                T v = F[min(k, ne - 1) + int64_t(ne + min(j, nb - 1)) * n];
+                if (!(j < nb && k < ne)) v = zero;
+                sb[u] = v;
+            }
         }
-        for (int e = tid; e < kTile * kChunk; e += 256) {
-            const int c = e % kChunk, r = e / kChunk, j = tl.j0 + r, k = kc + c;
-            T v = F[min(k, ne - 1) + int64_t(ne + min(j, nb - 1)) * n];
-            if (!(j < nb && k < ne)) v = zero;
-            Bs[c * kTileLd + r] = v;
+    };
+    fetch(0);
+    for (int kc = 0; kc < ne; kc += kChunk) {
+#pragma unroll
+        for (int u = 0; u < kStage; ++u) {
+            const int e = tid + u * 256;
+            As[(e / kTile) * kTileLd + e % kTile] = sa[u];
+            Bs[(e % kChunk) * kTileLd + e / kChunk] = sb[u];
         }
         __syncthreads();
+        if (kc + kChunk < ne) fetch(kc + kChunk);
 #pragma unroll
         for (int kk = 0; kk < kChunk; kk += 4) {
             const int k = kk + lane / 16;
@@ -1790,8 +1961,10 @@ void launch_front_lds(const NodeDesc* nodes, const int32_t* level, int32_t count
     if (count <= 0 || points <= 0) return;
     if (nmax > kLdsMaxN) throw std::invalid_argument("front too large for the LDS kernel");
     const size_t lds = size_t(nmax) * size_t(nmax + 1) * (single ? sizeof(float2) : sizeof(double2));
-    // Lanes per row and workgroup size by the largest front: 16 rows x 4, 32 x 2 (one wave),
-    // 64 x 4, 128 x 2 (four waves).
+    // The LDS kernel: lanes per row and workgroup size by the largest front: 16 rows x 4, 32 x 2
+    // (one wave), 64 x 4, 128 x 2 (four waves). With CAROLI_SMALL_FRONTS=reg, fronts up to 64 go
+    // to the register-resident kernel (one lane per row, 16 columns per wave), which measured
+    // slower on the BHZ and graphene batches (the steps are latency-bound either way).
     auto go = [&](auto kernel, int nt) {
         allow_lds(reinterpret_cast<const void*>(kernel));
         hipLaunchKernelGGL(kernel, grid2(count, points), dim3(nt), lds, st, nodes, level, ent, diag, kids, maps, pv, tol);
@@ -1799,7 +1972,14 @@ void launch_front_lds(const NodeDesc* nodes, const int32_t* level, int32_t count
     auto pick = [&](auto t, auto a) {
         using T = decltype(t);
         constexpr bool A = decltype(a)::value;
-        if (nmax <= 16)
+        if (nmax <= 64 && reg_small_fronts()) {  // register-resident: 1, 2 or 4 waves of 16 columns
+            if (nmax <= 16)
+                go(front_reg_kernel<T, 1, 16, A>, 64);
+            else if (nmax <= 32)
+                go(front_reg_kernel<T, 2, 16, A>, 128);
+            else
+                go(front_reg_kernel<T, 4, 16, A>, 256);
+        } else if (nmax <= 16)
             go(front_lds_kernel<T, 64, 4, A>, 64);
         else if (nmax <= 32)
             go(front_lds_kernel<T, 64, 2, A>, 64);
@@ -1812,6 +1992,11 @@ void launch_front_lds(const NodeDesc* nodes, const int32_t* level, int32_t count
         assemble ? pick(float2{}, std::true_type{}) : pick(float2{}, std::false_type{});
     else
         assemble ? pick(double2{}, std::true_type{}) : pick(double2{}, std::false_type{});
+}
+
+bool reg_small_fronts() {
+    const char* v = std::getenv("CAROLI_SMALL_FRONTS");
+    return v && std::strcmp(v, "reg") == 0;
 }
 
 int32_t border_lds_rows(int32_t max_ne, bool single) {

@@ -131,6 +131,9 @@ void launch_f11(const NodeDesc* nodes, const int32_t* level, int32_t count, int3
                 const int32_t* diag, const PointsView& pv, int32_t points, double tol, bool single, hipStream_t st);
 void launch_trsm(const NodeDesc* nodes, const Tile* tiles, int32_t count, int32_t max_ne, const PointsView& pv,
                  int32_t points, bool single, hipStream_t st);
+/// CAROLI_SMALL_FRONTS=reg: fronts up to 64 variables eliminated in registers (front_reg_kernel)
+/// instead of in LDS (front_lds_kernel).
+bool reg_small_fronts();
 /// CAROLI_FRONT_KERNEL=border: every large front through border_kernel (no f11/trsm split).
 bool border_only();
 /// Smallest separator (exclusive) of the blocked path: CAROLI_FRONT_KERNEL=blocked takes every

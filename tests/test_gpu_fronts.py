@@ -75,3 +75,19 @@ def test_singular_block_is_refused(n, ne, precision):
     print(f"[gpu] singular front n={n} ne={ne} precision {precision}", flush=True)
     _, st = gpu_schur(f, ne, ne, precision)
     assert st == 5
+
+
+@pytest.mark.parametrize("small", ["reg", "lds"])
+def test_small_front_kernels_on_systems(small, monkeypatch):
+    """Both small-front kernels (in LDS, the default, and register-resident) on whole systems
+    against the oracle: delayed separators at E = 0 of the clean lattice, both BHZ spin blocks,
+    complex f32 fronts."""
+    import caroli_amd as ca
+    from helpers import compare, hashed
+
+    monkeypatch.setenv("CAROLI_SMALL_FRONTS", small)
+    compare(ca.NestedTwoTerminal.preset(ca.Model.Square, [18, 9, 0]), [-1.3, 0.0, 0.4], f"{small} square 18x9 clean")
+    compare(hashed(ca.Model.BHZ, [15, 9, 0], 0.6), [-0.1, 0.02], f"{small} BHZ 15x9 disordered")
+    compare(hashed(ca.Model.Graphene, [13, 7, 0], 1.0, subs_per_cell=2), [-0.6, 0.3], f"{small} graphene 13x7")
+    compare(hashed(ca.Model.Square, [41, 29, 0], 2.0), [-1.7, 0.3], f"{small} square 41x29 single",
+            ca.Precision.Single, tol=1e-4)
