@@ -327,7 +327,8 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=2,
+                    help="untimed steps (the first runs eagerly, the second captures the sweeps into HIP graphs)")
     ap.add_argument("--roofline-steps", type=int, default=1,
                     help="steps after the timed ones with the per-kernel timers on (units one at a time)")
     ap.add_argument("--config", default="c2", choices=["c2", "c3", "c4", "c5"])

@@ -7,8 +7,11 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <cstdio>
+#include <cstdlib>
+#include <cstring>
 #include <memory>
 #include <cmath>
 #include <map>
@@ -1155,12 +1158,38 @@ void release(const cg_system* s) {
 }  // namespace caroli::gpu
 
 /// A resident batch: launch units (structure, block, chunk of points) with their inputs in HBM.
+/// A sweep captured as a HIP graph, with the device buffers it was captured on (the lane's
+/// arenas, pivots, statuses, results and trace scratch) and its point count: a replay is valid
+/// while they are the same.
+struct SweepGraph {
+    hipGraphExec_t exec = nullptr;
+    std::array<const void*, 8> key{};
+    int eager_runs = 0;
+    SweepGraph() = default;
+    SweepGraph(SweepGraph&& o) noexcept : exec(o.exec), key(o.key), eager_runs(o.eager_runs) { o.exec = nullptr; }
+    SweepGraph& operator=(SweepGraph&& o) noexcept {
+        reset();
+        exec = o.exec;
+        key = o.key;
+        eager_runs = o.eager_runs;
+        o.exec = nullptr;
+        return *this;
+    }
+    SweepGraph(const SweepGraph&) = delete;
+    ~SweepGraph() { reset(); }
+    void reset() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        exec = nullptr;
+    }
+};
+
 struct cg_batch {
     struct Unit {
         std::shared_ptr<caroli::gpu::Structure> structure;
         int block;
         std::vector<int64_t> points;
         caroli::gpu::Inputs in;
+        SweepGraph graph;
     };
     std::vector<const cg_system*> systems;
     std::vector<const cg_sigma*> sigmas;
@@ -1249,6 +1278,42 @@ int32_t create_batch(const cg_system* const* systems, const cg_sigma* const* sig
     return CG_OK;
 }
 
+/// CAROLI_GRAPHS=0 runs every sweep eagerly (no HIP graphs).
+bool graphs_enabled() {
+    const char* v = std::getenv("CAROLI_GRAPHS");
+    return !(v && std::strcmp(v, "0") == 0);
+}
+
+/// The sweep of a unit on the current lane: eager on the unit's first run (it sizes the lane's
+/// scratch and sets kernel attributes), captured into a HIP graph on the next and replayed after
+/// that, as long as the lane's buffers and the point count are the ones it was captured with.
+/// The sweep issues kernel launches only (no memsets or copies), so the graph holds kernel nodes.
+void sweep_graph(Engine& eng, cg_batch::Unit& u, const HostPlan& hp, const DevPlan& dp, const PointsView& pv,
+                 int32_t P, bool single) {
+    Lane& ln = *eng.lane;
+    const std::array<const void*, 8> key{ln.arena.p,   ln.root_arena.p, ln.piv.p,     ln.status.p,
+                                         ln.tout.p,    ln.trace_x.p,    ln.trace_p.p, reinterpret_cast<const void*>(intptr_t(P))};
+    SweepGraph& g = u.graph;
+    if (g.exec && g.key == key) {
+        HIP_CHECK(hipGraphLaunch(g.exec, ln.stream));
+        return;
+    }
+    g.reset();
+    if (g.eager_runs++ == 0) {
+        sweep(eng, hp, dp, pv, P, single, u.in);
+        return;
+    }
+    hipGraph_t graph = nullptr;
+    HIP_CHECK(hipStreamBeginCapture(ln.stream, hipStreamCaptureModeThreadLocal));
+    sweep(eng, hp, dp, pv, P, single, u.in);
+    HIP_CHECK(hipStreamEndCapture(ln.stream, &graph));
+    const hipError_t rc = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    HIP_CHECK(rc);
+    g.key = key;
+    HIP_CHECK(hipGraphLaunch(g.exec, ln.stream));
+}
+
 int32_t run_batch(cg_batch* bt, double* out_t, int32_t* out_status, cg_batch_stats* stats) {
     Engine& eng = Engine::get();
     if (!eng.ok) return CG_ERR_DEVICE;
@@ -1292,7 +1357,10 @@ int32_t run_batch(cg_batch* bt, double* out_t, int32_t* out_status, cg_batch_sta
                 pv.status_stride = hp.slots;
                 pv.t_out = eng.lane->tout.as<double>();
                 pv.keep = mixed ? 1 : 0;  // the refinement solves with the factors
-                sweep(eng, hp, dp, pv, int32_t(u.points.size()), single, u.in);
+                if (!mixed && !eng.timing && graphs_enabled())
+                    sweep_graph(eng, u, hp, dp, pv, int32_t(u.points.size()), single);
+                else
+                    sweep(eng, hp, dp, pv, int32_t(u.points.size()), single, u.in);
             }
             for (size_t ui = u0; ui < u1; ++ui) {
                 auto& u = bt->units[ui];
