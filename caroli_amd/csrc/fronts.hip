@@ -924,7 +924,7 @@ __global__ void __launch_bounds__(256)
 //  - f11_kernel, one workgroup per front: -H and z added to the gathered border, then the LU of
 //    F11 in LDS (lds_lu) written back with its pivots;
 //  - trsm_kernel, one workgroup per kTrsmSpan columns of F12 (U12 = L11^-1 P F12) or rows of F21
-//    (L21 = F21 U11^-1): the triangle in LDS, each column / row in the registers of 8 lanes.
+//    (L21 = F21 U11^-1): the triangle in LDS, each column / row in the registers of 4 lanes.
 template <class T, int G>
 __global__ void __launch_bounds__(256)
     f11_kernel(const NodeDesc* __restrict__ nodes, const int32_t* __restrict__ level, const Entry* __restrict__ ent,
@@ -989,68 +989,66 @@ __global__ void __launch_bounds__(256)
     }
     for (int idx = tid; idx < ne * ne; idx += 256) A[(idx % ne) + (idx / ne) * la] = F[(idx % ne) + int64_t(idx / ne) * n];
     __syncthreads();
-    // Eight lanes per column of U12 (kind 0) or row of L21 (kind 1): lane g holds entries
-    // g + 8 s (s < NS, ne <= 8 NS) of it in registers; the solve runs with shuffles inside the 8
-    // lanes.
-    const int grp = tid / 8, g = tid % 8;
+    // Four lanes per column of U12 (kind 0) or row of L21 (kind 1), the workgroup's 64 of them in
+    // one pass: lane g holds entries g + 4 s (s < NS, ne <= 4 NS) in registers, and the solve
+    // broadcasts the pivot entry inside the four lanes (a quad permutation).
+    const int grp = tid / 4, g = tid % 4;
     const int start = upper ? tl.i0 : tl.j0, end = min(start + kTrsmSpan, nb);
-    for (int c0 = start; c0 < end; c0 += 32) {
-        const int col = min(c0 + grp, end - 1);
-        T x[NS];
+    const int col = min(start + grp, end - 1);
+    T x[NS];
 #pragma unroll
-        for (int s = 0; s < NS; ++s) {
-            const int e = min(g + 8 * s, ne - 1);
-            x[s] = upper ? F[(ne + col) + int64_t(e) * n] : F[src[e] + int64_t(ne + col) * n];
-        }
-        if (!upper) {  // forward substitution with the unit-lower L11
+    for (int s = 0; s < NS; ++s) {
+        const int e = min(g + 4 * s, ne - 1);
+        x[s] = upper ? F[(ne + col) + int64_t(e) * n] : F[src[e] + int64_t(ne + col) * n];
+    }
+    if (!upper) {  // forward substitution with the unit-lower L11
 #pragma unroll
-            for (int kb = 0; kb < NS; ++kb)
+        for (int kb = 0; kb < NS; ++kb)
 #pragma unroll
-                for (int kk = 0; kk < 8; ++kk) {
-                    const int k = 8 * kb + kk;
-                    if (k < ne - 1) {
-                        T xk;
-                        xk.x = __shfl(x[kb].x, kk, 8);
-                        xk.y = __shfl(x[kb].y, kk, 8);
+            for (int kk = 0; kk < 4; ++kk) {
+                const int k = 4 * kb + kk;
+                if (k < ne - 1) {
+                    T xk;
+                    xk.x = __shfl(x[kb].x, kk, 4);
+                    xk.y = __shfl(x[kb].y, kk, 4);
 #pragma unroll
-                        for (int s = 0; s < NS; ++s) {
-                            const int i = g + 8 * s;
-                            if (i > k && i < ne) x[s] = csub(x[s], cmul(A[i + k * la], xk));
-                        }
+                    for (int s = kb; s < NS; ++s) {
+                        const int i = g + 4 * s;
+                        if (i > k && i < ne) x[s] = csub(x[s], cmul(A[i + k * la], xk));
                     }
                 }
-        } else {  // x U11 = f, column by column
+            }
+    } else {  // x U11 = f, column by column
 #pragma unroll
-            for (int kb = 0; kb < NS; ++kb)
+        for (int kb = 0; kb < NS; ++kb)
 #pragma unroll
-                for (int kk = 0; kk < 8; ++kk) {
-                    const int k = 8 * kb + kk;
-                    if (k < ne) {
-                        const T cand = cmul(x[kb], crecip(A[k + k * la]));
-                        T xk;
-                        xk.x = __shfl(cand.x, kk, 8);
-                        xk.y = __shfl(cand.y, kk, 8);
-                        if (g == kk) x[kb] = xk;
+            for (int kk = 0; kk < 4; ++kk) {
+                const int k = 4 * kb + kk;
+                if (k < ne) {
+                    const T cand = cmul(x[kb], crecip(A[k + k * la]));
+                    T xk;
+                    xk.x = __shfl(cand.x, kk, 4);
+                    xk.y = __shfl(cand.y, kk, 4);
+                    if (g == kk) x[kb] = xk;
 #pragma unroll
-                        for (int s = 0; s < NS; ++s) {
-                            const int j = g + 8 * s;
-                            if (j > k && j < ne) x[s] = csub(x[s], cmul(xk, A[k + j * la]));
-                        }
+                    for (int s = kb; s < NS; ++s) {
+                        const int j = g + 4 * s;
+                        if (j > k && j < ne) x[s] = csub(x[s], cmul(xk, A[k + j * la]));
                     }
-                }
-        }
-        if (c0 + grp < end)
-#pragma unroll
-            for (int s = 0; s < NS; ++s) {
-                const int e = g + 8 * s;
-                if (e < ne) {
-                    if (upper)
-                        F[(ne + col) + int64_t(e) * n] = x[s];
-                    else
-                        F[e + int64_t(ne + col) * n] = x[s];
                 }
             }
     }
+    if (start + grp < end)
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+            const int e = g + 4 * s;
+            if (e < ne) {
+                if (upper)
+                    F[(ne + col) + int64_t(e) * n] = x[s];
+                else
+                    F[e + int64_t(ne + col) * n] = x[s];
+            }
+        }
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -2056,16 +2054,16 @@ void launch_trsm(const NodeDesc* nodes, const Tile* tiles, int32_t count, int32_
         allow_lds(reinterpret_cast<const void*>(kernel));
         hipLaunchKernelGGL(kernel, grid2(count, points), dim3(256), lds, st, nodes, tiles, pv);
     };
-    auto pick = [&](auto t) {  // register slots per lane: ne <= 8 NS
+    auto pick = [&](auto t) {  // register slots per lane: ne <= 4 NS
         using T = decltype(t);
         if (max_ne <= 16)
-            go(trsm_kernel<T, 2>);
-        else if (max_ne <= 32)
             go(trsm_kernel<T, 4>);
-        else if (max_ne <= 48)
-            go(trsm_kernel<T, 6>);
-        else
+        else if (max_ne <= 32)
             go(trsm_kernel<T, 8>);
+        else if (max_ne <= 48)
+            go(trsm_kernel<T, 12>);
+        else
+            go(trsm_kernel<T, 16>);
     };
     single ? pick(float2{}) : pick(double2{});
 }
