@@ -525,13 +525,15 @@ const char* const kFamilyNames[kFamilies] = {"front_lds_kernel",    "gather_kern
                                              "blocked_panel_kernel", "blocked_row_kernel",   "blocked_update_kernel"};
 
 /// A stream with the device buffers of one sweep in flight: the arenas, pivots, statuses, results
-/// and the trace scratch. Two lanes let the sweeps of two launch units (spin blocks, systems of
-/// other sizes) run at once: their top tree levels hold few fronts each and leave CUs idle alone.
+/// and the trace scratch. Several lanes let the sweeps of several launch units (spin blocks, halves
+/// of a block's points, systems of other sizes) run at once: their top tree levels hold few
+/// fronts each and leave CUs idle alone.
 struct Lane {
     hipStream_t stream = nullptr;
     DevBuf arena, root_arena, piv, status, tout, trace_x, trace_p;
 };
-constexpr int kLanes = 2;
+constexpr int kLanes = 4;
+constexpr size_t kMinUnitPoints = 32;
 
 struct Engine {
     std::mutex mu;
@@ -1218,6 +1220,12 @@ int32_t create_batch(const cg_system* const* systems, const cg_sigma* const* sig
         size_t arena, root, piv, status, tout;
     };
     std::vector<Need> need;  // device bytes of each unit
+    // The points of a (structure, block) are split so that at least kLanes units can be in flight
+    // (the top tree levels hold few fronts per point and leave CUs idle with one or two units),
+    // down to kMinUnitPoints points a unit; the refinement of the mixed mode works unit by unit.
+    size_t blocks_total = 0;
+    for (auto& [key, idx] : groups) blocks_total += structure_of(eng, systems[idx[0]])->host.size();
+    const size_t splits = precision == CG_MIXED ? 1 : (kLanes + blocks_total - 1) / std::max<size_t>(blocks_total, 1);
     for (auto& [key, idx] : groups) {
         auto sp = structure_of(eng, systems[idx[0]]);
         for (auto p : idx) structure_of(eng, systems[p]);
@@ -1231,7 +1239,8 @@ int32_t create_batch(const cg_system* const* systems, const cg_sigma* const* sig
             const size_t per_point = size_t(hp.arena_stride) * es + size_t(hp.root_stride) * sizeof(double2) +
                                      size_t(hp.piv_stride) * 4 + size_t(hp.slots) * 4 + 64;
             const size_t budget = size_t(96) << 30;
-            size_t chunk = std::max<size_t>(1, std::min<size_t>({budget / per_point, 65535, idx.size()}));
+            const size_t even = std::max<size_t>(kMinUnitPoints, (idx.size() + splits - 1) / splits);
+            size_t chunk = std::max<size_t>(1, std::min<size_t>({budget / per_point, 65535, idx.size(), even}));
             for (size_t c0 = 0; c0 < idx.size(); c0 += chunk) {
                 size_t P = std::min(chunk, idx.size() - c0);
                 cg_batch::Unit u;
@@ -1252,20 +1261,20 @@ int32_t create_batch(const cg_system* const* systems, const cg_sigma* const* sig
             }
         }
     }
-    // Lanes: two when the two largest units fit the concurrent budget together (and the mode runs
-    // no refinement, which works unit by unit).
-    size_t big1 = 0, big2 = 0;
-    for (auto& nd : need) {
-        const size_t t = nd.arena + nd.root + nd.piv + nd.status + nd.tout;
-        if (t > big1) {
-            big2 = big1;
-            big1 = t;
-        } else if (t > big2) {
-            big2 = t;
-        }
-    }
+    // Lanes: as many as there are units, up to kLanes, while the largest units fit the concurrent
+    // budget together (one when the mode runs the refinement, which works unit by unit).
+    std::vector<size_t> sizes;
+    for (auto& nd : need) sizes.push_back(nd.arena + nd.root + nd.piv + nd.status + nd.tout);
+    std::sort(sizes.rbegin(), sizes.rend());
     const size_t concurrent_budget = size_t(160) << 30;
-    bt->lanes = (precision != CG_MIXED && need.size() > 1 && big1 + big2 <= concurrent_budget) ? kLanes : 1;
+    int lanes = precision == CG_MIXED ? 1 : int(std::min<size_t>(kLanes, sizes.size()));
+    auto fits = [&](int L) {
+        size_t t = 0;
+        for (int q = 0; q < L; ++q) t += sizes[q];
+        return t <= concurrent_budget;
+    };
+    while (lanes > 1 && !fits(lanes)) --lanes;
+    bt->lanes = std::max(lanes, 1);
     for (size_t ui = 0; ui < need.size(); ++ui) {
         Lane& ln = eng.lanes[ui % size_t(bt->lanes)];
         ln.arena.ensure(need[ui].arena);
