@@ -1044,6 +1044,13 @@ struct Refined {
     double worst = 0.0;
 };
 
+/// CAROLI_REFINE_TRACE=1 prints, per refinement iteration, how many points are still iterating
+/// and the range of their corrections (stderr).
+bool refine_trace() {
+    const char* v = std::getenv("CAROLI_REFINE_TRACE");
+    return v && std::strcmp(v, "1") == 0;
+}
+
 /// Transmission of block b for the P points of `pv` from the factors of the last sweep (complex
 /// f32 fronts when `single`): x = A^-1 E_R W_R, refined with FP64 residuals b - A x of the full
 /// operator z - H - Sigma until the correction is below kRefineTol; then T_b = |W_L^H x_L|^2.
@@ -1092,13 +1099,27 @@ Refined refine(Engine& eng, const HostPlan& hp, const DevPlan& dp, const PointsV
     std::vector<char> state(P, 0);
     for (int32_t p = 0; p < P; ++p)
         if (skip[p]) state[p] = 2;
+    // Each iteration runs on the points still iterating only (SolveView::pts).
+    std::vector<int32_t> act;
+    DevBuf ab;
+    ab.ensure(size_t(P) * sizeof(int32_t));
     for (int it = 0; it < kRefineMaxIter; ++it) {
-        launch_residual(pv, xv, bb.as<double2>(), db.as<double2>(), dp.rows.as<int32_t>(), nrows,
+        act.clear();
+        for (int32_t p = 0; p < P; ++p)
+            if (state[p] == 0) act.push_back(p);
+        if (act.empty()) break;
+        const int32_t A = int32_t(act.size());
+        HIP_CHECK(hipMemcpyAsync(ab.p, act.data(), act.size() * sizeof(int32_t), hipMemcpyHostToDevice,
+                                 eng.lane->stream));
+        SolveView xa = xv, da = dv;
+        xa.pts = ab.as<int32_t>();
+        da.pts = ab.as<int32_t>();
+        launch_residual(pv, xa, bb.as<double2>(), db.as<double2>(), dp.rows.as<int32_t>(), nrows,
                         dp.row_ptr.as<int64_t>(), dp.cols.as<uint32_t>(), dp.left_pos.as<int32_t>(),
-                        dp.right_pos.as<int32_t>(), dp.left_orbs.as<int32_t>(), nl, dp.right_orbs.as<int32_t>(), nr, P,
+                        dp.right_pos.as<int32_t>(), dp.left_orbs.as<int32_t>(), nl, dp.right_orbs.as<int32_t>(), nr, A,
                         eng.lane->stream);
-        solve(eng, hp, dp, pv, dv, P, single, false, false);
-        launch_update(xv, db.as<double2>(), dp.rows.as<int32_t>(), nrows, rb.as<double>(), P, eng.lane->stream);
+        solve(eng, hp, dp, pv, da, A, single, false, false);
+        launch_update(xa, db.as<double2>(), dp.rows.as<int32_t>(), nrows, rb.as<double>(), A, P, eng.lane->stream);
         HIP_CHECK(hipGetLastError());
         HIP_CHECK(hipMemcpyAsync(maxes.data(), rb.p, maxes.size() * sizeof(double), hipMemcpyDeviceToHost,
                                  eng.lane->stream));
@@ -1108,6 +1129,21 @@ Refined refine(Engine& eng, const HostPlan& hp, const DevPlan& dp, const PointsV
             ratio[p] = mx > 0.0 ? md / mx : (md > 0.0 ? INFINITY : 0.0);
         }
         ++out.iters;
+        if (refine_trace()) {
+            int n_active = 0, n_done = 0, n_gone = 0;
+            double worst = 0.0, best = INFINITY;
+            for (int32_t p = 0; p < P; ++p) {
+                n_active += state[p] == 0;
+                n_done += state[p] == 1;
+                n_gone += state[p] == 2;
+                if (state[p] == 0) {
+                    worst = std::max(worst, ratio[p]);
+                    best = std::min(best, ratio[p]);
+                }
+            }
+            std::fprintf(stderr, "[refine] block %d iteration %d: %d iterating (correction %.1e .. %.1e), %d done, %d given up\n",
+                         b, it, n_active, best, worst, n_done, n_gone);
+        }
         bool active = false;
         for (int32_t p = 0; p < P; ++p) {
             if (state[p] != 0) continue;

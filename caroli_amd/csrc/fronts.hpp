@@ -31,6 +31,7 @@ struct SolveView {
     int64_t x_stride;
     int32_t ldx, c;
     const int32_t* vars;  // orbital index of every front variable (NodeDesc::var_off)
+    const int32_t* pts;   // points of the launch (grid index q -> point pts[q]); null: all, in order
 };
 
 /// An H entry: F[a, b] -= conj ? conj(vals[k]) : vals[k].
@@ -191,9 +192,10 @@ void launch_residual(const PointsView& pv, const SolveView& xv, const double2* b
                      int32_t nrows, const int64_t* row_ptr, const uint32_t* cols, const int32_t* left_pos,
                      const int32_t* right_pos, const int32_t* left_orbs, int32_t nl, const int32_t* right_orbs,
                      int32_t nr, int32_t points, hipStream_t st);
-/// X += D; maxes[2p], maxes[2p + 1] receive max |D| and max |X| of point p (2 doubles per point).
+/// X += D; maxes[2p], maxes[2p + 1] receive max |D| and max |X| of point p (2 doubles per point,
+/// all_points of them cleared first).
 void launch_update(const SolveView& xv, const double2* d, const int32_t* rows, int32_t nrows, double* maxes,
-                   int32_t points, hipStream_t st);
+                   int32_t points, int32_t all_points, hipStream_t st);
 void launch_gather(const SolveView& xv, const int32_t* orbs, int32_t m, double2* out, int32_t points, hipStream_t st);
 
 }  // namespace caroli::gpu

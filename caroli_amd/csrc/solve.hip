@@ -25,6 +25,9 @@ namespace {
 
 constexpr int kThreads = 256;
 
+// The point of grid index q (SolveView::pts).
+__device__ __forceinline__ int64_t point_of(const SolveView& sv, int64_t q) { return sv.pts ? sv.pts[q] : q; }
+
 template <class TF>
 __device__ __forceinline__ double2 widen(TF a) {
     return double2{double(a.x), double(a.y)};
@@ -43,7 +46,7 @@ __global__ void __launch_bounds__(kThreads)
                            SolveView sv) {
     extern __shared__ double2 y[];  // ne x c, column-major
     const NodeDesc nd = nodes[level[blockIdx.x]];
-    const int64_t p = blockIdx.y;
+    const int64_t p = point_of(sv, blockIdx.y);
     const TF* F = reinterpret_cast<const TF*>(pv.arena) + p * pv.arena_stride + nd.off;
     const int32_t* vars = sv.vars + nd.var_off;
     double2* X = sv.x + p * sv.x_stride;
@@ -75,7 +78,7 @@ __global__ void __launch_bounds__(kThreads)
                             SolveView sv) {
     extern __shared__ double2 y[];  // ne x c
     const NodeDesc nd = nodes[level[blockIdx.x]];
-    const int64_t p = blockIdx.y;
+    const int64_t p = point_of(sv, blockIdx.y);
     const TF* F = reinterpret_cast<const TF*>(pv.arena) + p * pv.arena_stride + nd.off;
     const int32_t* piv = pv.piv + p * pv.piv_stride + nd.piv_off;
     const int32_t* vars = sv.vars + nd.var_off;
@@ -313,7 +316,7 @@ __global__ void __launch_bounds__(NT, kSolveWaves)
     extern __shared__ double2 ys[];  // n x CC, then NT x CC partial sums
     const SolveItem it = solve_item((sv.c + CC - 1) / CC);
     const NodeDesc nd = nodes[level[it.front]];
-    const int64_t p = it.point;
+    const int64_t p = point_of(sv, it.point);
     const int c0 = it.chunk * CC, cw = min(CC, sv.c - c0);
     if (cw <= 0) return;
     const TF* F = reinterpret_cast<const TF*>(pv.arena) + p * pv.arena_stride + nd.off;
@@ -349,7 +352,7 @@ __global__ void __launch_bounds__(NT, kSolveWaves)
     extern __shared__ double2 ys[];
     const SolveItem it = solve_item((sv.c + CC - 1) / CC);
     const NodeDesc nd = nodes[level[it.front]];
-    const int64_t p = it.point;
+    const int64_t p = point_of(sv, it.point);
     const int c0 = it.chunk * CC, cw = min(CC, sv.c - c0);
     if (cw <= 0) return;
     const TF* F = reinterpret_cast<const TF*>(pv.arena) + p * pv.arena_stride + nd.off;
@@ -377,7 +380,7 @@ __global__ void __launch_bounds__(NT, kSolveWaves)
     extern __shared__ double2 ys[];
     const SolveItem it = solve_item((sv.c + CC - 1) / CC);
     const NodeDesc nd = nodes[level[0]];
-    const int64_t p = it.point;
+    const int64_t p = point_of(sv, it.point);
     const int c0 = it.chunk * CC, cw = min(CC, sv.c - c0);
     if (cw <= 0) return;
     const double2* F = reinterpret_cast<const double2*>(pv.root_arena) + p * pv.root_stride + nd.off;
@@ -405,7 +408,7 @@ __global__ void __launch_bounds__(kThreads)
     solve_root_t_kernel(const NodeDesc* __restrict__ nodes, const int32_t* __restrict__ level, PointsView pv, SolveView sv) {
     extern __shared__ double2 y[];
     const NodeDesc nd = nodes[level[0]];
-    const int64_t p = blockIdx.x;
+    const int64_t p = point_of(sv, blockIdx.x);
     const double2* F = reinterpret_cast<const double2*>(pv.root_arena) + p * pv.root_stride + nd.off;
     const int32_t* piv = pv.piv + p * pv.piv_stride + nd.piv_off;
     const int32_t* vars = sv.vars + nd.var_off;
@@ -452,7 +455,7 @@ __global__ void __launch_bounds__(kThreads)
 __global__ void __launch_bounds__(kThreads)
     scatter_kernel(SolveView sv, const int32_t* __restrict__ orbs, int32_t m, const double2* __restrict__ w,
                    int64_t w_stride, int32_t conj) {
-    const int64_t p = blockIdx.y;
+    const int64_t p = point_of(sv, blockIdx.y);
     double2* X = sv.x + p * sv.x_stride;
     const double2* W = w + p * w_stride;
     for (int idx = blockIdx.x * kThreads + threadIdx.x; idx < m * sv.c; idx += gridDim.x * kThreads) {
@@ -471,7 +474,7 @@ __global__ void __launch_bounds__(kThreads)
                     const uint32_t* __restrict__ cols, const int32_t* __restrict__ left_pos,
                     const int32_t* __restrict__ right_pos, const int32_t* __restrict__ left_orbs, int32_t nl,
                     const int32_t* __restrict__ right_orbs, int32_t nr) {
-    const int64_t p = blockIdx.y;
+    const int64_t p = point_of(xv, blockIdx.y);
     const double2* X = xv.x + p * xv.x_stride;
     const double2* B = b + p * xv.x_stride;
     double2* D = d + p * xv.x_stride;
@@ -501,7 +504,7 @@ __global__ void __launch_bounds__(kThreads)
     update_kernel(SolveView xv, const double2* __restrict__ d, const int32_t* __restrict__ rows, int32_t nrows,
                   unsigned long long* __restrict__ maxes) {
     __shared__ double sd[kThreads / 64], sx[kThreads / 64];
-    const int64_t p = blockIdx.y;
+    const int64_t p = point_of(xv, blockIdx.y);
     double2* X = xv.x + p * xv.x_stride;
     const double2* D = d + p * xv.x_stride;
     double md = 0.0, mx = 0.0;
@@ -535,7 +538,7 @@ __global__ void __launch_bounds__(kThreads)
 // out[a, col] = X[orbs[a], col] per point (m x c, column-major).
 __global__ void __launch_bounds__(kThreads)
     gather_kernel(SolveView xv, const int32_t* __restrict__ orbs, int32_t m, double2* __restrict__ out) {
-    const int64_t p = blockIdx.y;
+    const int64_t p = point_of(xv, blockIdx.y);
     const double2* X = xv.x + p * xv.x_stride;
     double2* O = out + p * int64_t(m) * xv.c;
     for (int idx = blockIdx.x * kThreads + threadIdx.x; idx < m * xv.c; idx += gridDim.x * kThreads)
@@ -565,9 +568,9 @@ void launch_residual(const PointsView& pv, const SolveView& xv, const double2* b
 }
 
 void launch_update(const SolveView& xv, const double2* d, const int32_t* rows, int32_t nrows, double* maxes,
-                   int32_t points, hipStream_t st) {
+                   int32_t points, int32_t all_points, hipStream_t st) {
     if (points <= 0 || xv.c <= 0) return;
-    if (hipMemsetAsync(maxes, 0, size_t(points) * 2 * sizeof(double), st) != hipSuccess)
+    if (hipMemsetAsync(maxes, 0, size_t(all_points) * 2 * sizeof(double), st) != hipSuccess)
         throw std::runtime_error("hipMemsetAsync of the refinement maxima failed");
     hipLaunchKernelGGL(update_kernel, rows_grid(int64_t(nrows) * xv.c, points), dim3(kThreads), 0, st, xv, d, rows,
                        nrows, reinterpret_cast<unsigned long long*>(maxes));
