@@ -3,6 +3,8 @@
 #include <complex>
 #include <cstddef>
 #include <cstdint>
+#include <algorithm>
+#include <thread>
 #include <vector>
 
 namespace caroli {
@@ -39,17 +41,32 @@ struct CMat {
     }
 };
 
-/// c = a * b.
+/// c = a * b; large products split their columns over up to 16 threads.
 inline CMat matmul(const CMat& a, const CMat& b) {
     CMat c(a.rows, b.cols);
-    for (int j = 0; j < b.cols; ++j)
-        for (int k = 0; k < a.cols; ++k) {
-            cplx bkj = b(k, j);
-            if (bkj == cplx(0.0)) continue;
-            const cplx* ak = a.col(k);
-            cplx* cj = c.col(j);
-            for (int i = 0; i < a.rows; ++i) cj[i] += ak[i] * bkj;
-        }
+    auto columns = [&](int j0, int j1) {
+        for (int j = j0; j < j1; ++j)
+            for (int k = 0; k < a.cols; ++k) {
+                cplx bkj = b(k, j);
+                if (bkj == cplx(0.0)) continue;
+                const cplx* ak = a.col(k);
+                cplx* cj = c.col(j);
+                for (int i = 0; i < a.rows; ++i) cj[i] += ak[i] * bkj;
+            }
+    };
+    const double work = double(a.rows) * double(a.cols) * double(b.cols);
+    const int threads = work < 6.4e7 ? 1 : int(std::min<unsigned>(16, std::max(1u, std::thread::hardware_concurrency())));
+    if (threads <= 1 || b.cols < 2 * threads) {
+        columns(0, b.cols);
+        return c;
+    }
+    std::vector<std::thread> pool;
+    const int per = (b.cols + threads - 1) / threads;
+    for (int t = 0; t < threads; ++t) {
+        const int j0 = t * per, j1 = std::min(b.cols, j0 + per);
+        if (j0 < j1) pool.emplace_back(columns, j0, j1);
+    }
+    for (auto& th : pool) th.join();
     return c;
 }
 

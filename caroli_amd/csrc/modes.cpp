@@ -360,11 +360,24 @@ Status solve_general(const CMat& h0, const CMat& h1, cplx z, double scale, LeadM
     return solve_with(h0, h1, z, scale, false, out);
 }
 
+// c Y diag(d) Y^H.
+CMat cell_sigma(const CMat& y, double c, const std::vector<cplx>& d) {
+    CMat w = y;
+    for (int k = 0; k < w.cols; ++k)
+        for (int i = 0; i < w.rows; ++i) w(i, k) *= c * d[k];
+    return matmul(w, y.adjoint());
+}
+
 }  // namespace
 
 CMat LeadModes::sigma_right() const {
     const int n = u.rows, r = u.cols;
     if (r == 0) return CMat(n, n);
+    if (cell_y) {  // X_g X_a^-1 = Y diag(lambda / s) Y^H, u = s sqrt(c) I
+        std::vector<cplx> d(cell_right.size());
+        for (size_t k = 0; k < d.size(); ++k) d[k] = cell_s * cell_right[k];
+        return cell_sigma(*cell_y, cell_c, d);
+    }
     CMat t = dense::solve_right(rows_of(right, r, 2 * r), rows_of(right, 0, r));
     return sandwich(u, t);
 }
@@ -372,6 +385,11 @@ CMat LeadModes::sigma_right() const {
 CMat LeadModes::sigma_left() const {
     const int n = v.rows, r = v.cols;
     if (r == 0) return CMat(n, n);
+    if (cell_y) {  // Y_a Y_g^-1 = Y diag(s / lambda) Y^H, v = sqrt(c) I
+        std::vector<cplx> d(cell_left.size());
+        for (size_t k = 0; k < d.size(); ++k) d[k] = cell_s / cell_left[k];
+        return cell_sigma(*cell_y, cell_c, d);
+    }
     CMat t = dense::solve_right(rows_of(left, 0, r), rows_of(left, r, 2 * r));
     return sandwich(v, t);
 }
@@ -465,8 +483,9 @@ Status definite_modes(const CMat& h0, double s, const CMat& l, bool cell, double
             leftl[k] = big;
         }
     }
-    // Velocities of the propagating modes, psi = L^-H y.
-    CMat lhinv = dense::inverse(l.adjoint());
+    // Velocities of the propagating modes, psi = L^-H y (L = sqrt(c) I in the cell case).
+    CMat lhinv;
+    if (!cell) lhinv = dense::inverse(l.adjoint());
     for (int k : propagating) {
         double nn = 0.0;
         if (cell) {
@@ -499,8 +518,16 @@ Status definite_modes(const CMat& h0, double s, const CMat& l, bool cell, double
     out.left = basis(leftl);
     out.channels = int(propagating.size());
     out.energy = e;
+    if (cell) {
+        out.cell_y = y;
+        out.cell_c = c;
+        out.cell_s = s;
+        out.cell_right = right;
+        out.cell_left = leftl;
+    }
     return kOk;
 }
+
 }  // namespace
 
 // ModeSolver::modes_at with LeadModes::shifted (modes.rs:130-145, :531-537)

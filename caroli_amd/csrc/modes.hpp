@@ -6,6 +6,7 @@
 // and `Sigma_R = u X_g X_a^-1 u^H`, `Sigma_L = v Y_a Y_g^-1 v^H` (:494-511).
 #pragma once
 #include <optional>
+#include <vector>
 
 #include "cmat.hpp"
 
@@ -27,6 +28,12 @@ struct LeadModes {
     CMat right, left;     // orthonormal bases of right- and left-going modes (2r x r)
     int channels = 0;
     cplx energy;
+    // Scalar coupling (H1 + H1^H = 2 s c I, the Hermitian cell fast path): the bases are
+    // Y diag(.) with Y the unitary eigenvectors of the cell, so Sigma_R = c Y diag(s lambda_R) Y^H
+    // and Sigma_L = c Y diag(s / lambda_L) Y^H, one product each instead of a solve and two.
+    const CMat* cell_y = nullptr;
+    double cell_c = 0.0, cell_s = 1.0;
+    std::vector<cplx> cell_right, cell_left;  // lambda of the right- and left-going modes
     CMat sigma_right() const;
     CMat sigma_left() const;
 };
