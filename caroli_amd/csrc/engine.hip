@@ -1239,6 +1239,24 @@ struct cg_batch {
 
 namespace caroli::gpu {
 
+/// Device bytes a launch unit of P points of plan hp needs in a lane (front arena in the front
+/// precision, root arena, pivots, statuses, results).
+struct LaneNeed {
+    size_t arena, root, piv, status, tout;
+};
+LaneNeed lane_need(const HostPlan& hp, size_t P, bool single) {
+    const size_t es = single ? sizeof(float2) : sizeof(double2);
+    return {P * size_t(hp.arena_stride) * es, P * size_t(hp.root_stride) * sizeof(double2), P * size_t(hp.piv_stride) * 4,
+            P * size_t(hp.slots) * 4, P * sizeof(double)};
+}
+void ensure_lane(Lane& ln, const LaneNeed& nd) {
+    ln.arena.ensure(nd.arena);
+    ln.root_arena.ensure(nd.root);
+    ln.piv.ensure(nd.piv);
+    ln.status.ensure(nd.status);
+    ln.tout.ensure(nd.tout);
+}
+
 int32_t create_batch(const cg_system* const* systems, const cg_sigma* const* sigmas, int64_t count, int32_t precision,
                      cg_batch** out) {
     Engine& eng = Engine::get();
@@ -1291,8 +1309,8 @@ int32_t create_batch(const cg_system* const* systems, const cg_sigma* const* sig
                 }
                 PointsView pv{};
                 stage_points(eng, ss, gg, int(b), int(blk.left_orbitals.size()), int(blk.right_orbitals.size()), u.in, pv);
-                need.push_back({P * size_t(hp.arena_stride) * es, P * size_t(hp.root_stride) * sizeof(double2),
-                                P * size_t(hp.piv_stride) * 4, P * size_t(hp.slots) * 4, P * sizeof(double)});
+                const LaneNeed ln = lane_need(hp, P, single);
+                need.push_back({ln.arena, ln.root, ln.piv, ln.status, ln.tout});
                 bt->units.push_back(std::move(u));
             }
         }
@@ -1311,14 +1329,9 @@ int32_t create_batch(const cg_system* const* systems, const cg_sigma* const* sig
     };
     while (lanes > 1 && !fits(lanes)) --lanes;
     bt->lanes = std::max(lanes, 1);
-    for (size_t ui = 0; ui < need.size(); ++ui) {
-        Lane& ln = eng.lanes[ui % size_t(bt->lanes)];
-        ln.arena.ensure(need[ui].arena);
-        ln.root_arena.ensure(need[ui].root);
-        ln.piv.ensure(need[ui].piv);
-        ln.status.ensure(need[ui].status);
-        ln.tout.ensure(need[ui].tout);
-    }
+    for (size_t ui = 0; ui < need.size(); ++ui)
+        ensure_lane(eng.lanes[ui % size_t(bt->lanes)],
+                    {need[ui].arena, need[ui].root, need[ui].piv, need[ui].status, need[ui].tout});
     *out = bt.release();
     return CG_OK;
 }
@@ -1390,6 +1403,10 @@ int32_t run_batch(cg_batch* bt, double* out_t, int32_t* out_status, cg_batch_sta
                 const HostPlan& hp = u.structure->host[u.block];
                 const DevPlan& dp = u.structure->dev[u.block];
                 eng.lane = &eng.lanes[ui - u0];
+                // The lane's buffers cover the units create_batch gave it; run one at a time (kernel
+                // timing, mixed precision), every unit goes to lane 0, which grows to it here (the
+                // previous group has finished: its streams were synchronised).
+                ensure_lane(*eng.lane, lane_need(hp, u.points.size(), single));
                 PointsView& pv = pvs[ui - u0];
                 bind_inputs(u.in, pv);
                 pv.arena = eng.lane->arena.p;

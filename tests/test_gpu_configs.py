@@ -108,3 +108,31 @@ def test_cubic_uneven_batch():
         f"recomputed in f64 {stats.recomputed_points}")
     assert np.abs(tm - ref).max() <= 1e-8
     assert stats.refinement_iters > 0 and stats.recomputed_points <= 1
+
+
+def test_uneven_resident_batch_with_kernel_timing():
+    """Three cubic structures of different sizes in one resident batch: three launch units on three
+    lanes, each lane's buffers sized for its own unit. With the kernel timers on every unit runs on
+    lane 0 in turn, which must grow to the largest unit first; then the concurrent run."""
+    systems = [ca.NestedTwoTerminal.preset(ca.Model.Cubic, [2 * L, L, L], disorder=1, strength=16.5, seed=1000 + k)
+               for k, L in enumerate([8, 16, 12])]
+    energies = [-1.0, 1.0 / 3.0]
+    pts_s, pts_g, ref = [], [], []
+    for s in systems:
+        ps, pg, r = oracle_points([s], energies)
+        pts_s += ps
+        pts_g += pg
+        ref = np.concatenate([ref, r])
+    batch = ca.ResidentBatch(pts_s, pts_g, ca.Precision.Double)
+    say("[gpu] cubic resident batch L = 8, 16, 12 with kernel timing (units one after the other on lane 0)")
+    ca.kernel_timing(True)
+    try:
+        t, st = batch.run()
+    finally:
+        ca.kernel_timing(False)
+    assert (st == 0).all(), st
+    assert np.abs(t - ref).max() <= 1e-10 * (1 + np.abs(ref).max()), np.abs(t - ref).max()
+    say("[gpu] the same batch, units concurrent")
+    t2, st = batch.run()
+    assert (st == 0).all(), st
+    assert np.abs(t2 - ref).max() <= 1e-10 * (1 + np.abs(ref).max()), np.abs(t2 - ref).max()
