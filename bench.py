@@ -159,19 +159,35 @@ def cpu_baseline(pts_sys, pts_sig, pts_e, t_gpu, args, threads):
         s, g = pts_sys[p], pts_sig[p]
         return OracleSystem.pack([g.block(b)[:2] for b in range(s.block_count())])
 
+    # Points differ in cost when the systems differ in size (the cubic batch): the single-thread
+    # probe takes the cheapest point, and the sample the points whose estimated time fits the
+    # budget, evenly spaced; the rate of the whole workload is extrapolated from the sample's
+    # measured flop rate.
+    flops = np.array([s.work()[0] for s in pts_sys])
+    p0 = int(np.argmin(flops))
+    progress(f"cpu baseline: single-thread probe on point {p0} ({flops[p0]:.2e} flops)")
     t0 = time.perf_counter()
-    transmissions_parallel([osys(pts_sys[0])], [pts_e[0]], [flat(0)], 1)
+    transmissions_parallel([osys(pts_sys[p0])], [pts_e[p0]], [flat(p0)], 1)
     lat = time.perf_counter() - t0
-    per_thread = max(1, int(args.cpu_seconds / max(lat, 1e-6)))
-    sample = min(npts, threads * per_thread)
-    idx = np.linspace(0, npts - 1, sample).astype(int)
+    rate = flops[p0] / max(lat, 1e-9)  # flops per second of one thread
+    fits = np.nonzero(flops / rate <= args.cpu_seconds)[0]
+    if fits.size == 0:
+        fits = np.array([p0])
+    per_thread = max(1, int(args.cpu_seconds / max(float(np.mean(flops[fits])) / rate, 1e-6)))
+    sample = min(fits.size, threads * per_thread)
+    idx = fits[np.linspace(0, fits.size - 1, sample).astype(int)]
+    progress(f"cpu baseline: {sample} points on {threads} threads")
     t0 = time.perf_counter()
     tc, _ = transmissions_parallel([osys(pts_sys[i]) for i in idx], [pts_e[i] for i in idx], [flat(i) for i in idx],
                                    threads)
     cpu_s = time.perf_counter() - t0
+    progress(f"cpu baseline: {cpu_s:.1f} s")
+    sample_rate = float(np.sum(flops[idx])) / cpu_s  # flops per second on all threads
     cpu = {"value": sample / cpu_s, "unit": "points/s", "cores": threads, "kind": "port",
-           "sample": f"{sample} of the {npts} points (evenly spaced), FP64 oracle, one point per thread",
-           "single_thread_s_per_point": lat}
+           "sample": (f"{sample} of the {npts} points ({'evenly spaced' if fits.size == npts else 'evenly spaced among the ' + str(fits.size) + ' whose estimated single-thread time fits ' + str(args.cpu_seconds) + ' s'}), "
+                      "FP64 oracle, one point per thread"),
+           "single_thread_s_per_point": lat,
+           "workload_points_per_s": sample_rate / float(np.mean(flops))}
     d = np.abs(tc - t_gpu[idx])
     return cpu, float(np.max(d)), float(np.max(d / (1.0 + np.abs(tc))))
 
