@@ -14,13 +14,15 @@ dT/d(eps) for all 12,800 orbitals); `--config c5` the uneven cubic batch (W = 16
 cross-sections L drawn from {16, 24, 32, 40, 48} with xoshiro256** seeded 5, 4 energies in
 [-1, 1]) on the first `--systems` systems whose L is at most `--max-l`.
 
-A step is one batched launch over all points; the engine runs the sweeps of two launch units (the
-two spin blocks) at once on two streams. The JSON line carries the roofline of the dominant kernel
+A step is one batched launch over all points; the engine runs up to four launch units (spin blocks,
+halves of a block's points, systems of other sizes) at once on four streams, and replays each unit's
+sweep from a HIP graph after the warmups. The JSON line carries the roofline of the dominant kernel
 family: HIP events on the engine's streams during `--roofline-steps` steps that follow the timed
-ones and run the units one after the other (with two sweeps in flight, a launch's duration would
-include the other stream's work); the HBM traffic per launch comes from the rocprofv3 counter
-summary in profiles/ when one matches. The CPU baseline is the oracle (FP64 C++ restatement of the
-reference path) on the host cores for a bounded sample of the same points.
+ones and run the units one after the other (with several sweeps in flight, a launch's duration would
+include the other streams' work); the HBM traffic per launch comes from the rocprofv3 counter
+summary in profiles/ when one matches. For C2 and C3 the same points also run in the other
+precision (mixed vs plain). The CPU baseline is the oracle (FP64 C++ restatement of the reference
+path) on the host cores for a bounded sample of the same points.
 """
 import argparse
 import glob
@@ -42,7 +44,9 @@ PEAK_HBM = 8.0e12
 
 
 def workload_tag(args):
-    return {"c2": "bhz200x64", "c3": "graphene512x64", "c4": "bhz100x32_adjoint", "c5": "cubic_uneven"}[args.config]
+    if args.config == "c5":
+        return f"cubic_uneven_l{args.max_l}_s{args.systems}"
+    return {"c2": "bhz200x64", "c3": "graphene512x64", "c4": "bhz100x32_adjoint"}[args.config]
 
 
 def traffic_from_profiles(family, tag, precision):
@@ -353,8 +357,8 @@ def main():
     ap.add_argument("--energies", type=int, default=None, help="energies (c2: 33, c3: 256)")
     ap.add_argument("--width", type=int, default=64)
     ap.add_argument("--length", type=int, default=200)
-    ap.add_argument("--systems", type=int, default=8, help="c5: systems taken from the 512")
-    ap.add_argument("--max-l", type=int, default=24, help="c5: largest cross-section taken")
+    ap.add_argument("--systems", type=int, default=6, help="c5: systems taken from the 512")
+    ap.add_argument("--max-l", type=int, default=40, help="c5: largest cross-section taken")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU time of the baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-compare", dest="compare", action="store_false",
