@@ -87,7 +87,8 @@ struct LevelSet {
     // workgroup takes, and so the workgroups a CU holds, follow the class's largest front).
     static constexpr int kClasses = 4;
     std::vector<int32_t> lds[kClasses];
-    int32_t lds_n[kClasses] = {0, 0, 0, 0};  // largest n of each class
+    int32_t lds_n[kClasses] = {0, 0, 0, 0};       // largest n of each class
+    int32_t lds_border[kClasses] = {0, 0, 0, 0};  // largest border_lds_words of each class
     std::vector<int32_t> large;
     int32_t large_ne = 0;  // largest ne of the large fronts
     std::vector<Tile> tiles, gtiles;  // Schur tiles, border tiles (gather)
@@ -180,6 +181,7 @@ LevelSet classify(const std::vector<NodeDesc>& nodes, const std::vector<KidDesc>
             const int c = d.n <= 16 ? 0 : d.n <= 32 ? 1 : d.n <= 48 ? 2 : 3;
             L.lds[c].push_back(q);
             L.lds_n[c] = std::max(L.lds_n[c], d.n);
+            L.lds_border[c] = std::max(L.lds_border[c], border_lds_words(d.n, d.ne));
             L.lds_flops[c] += fl;
             L.lds_words[c] += kid_words + 2.0 * (d.ent_end - d.ent_beg);
             L.lds_s_words[c] += nb * nb;
@@ -727,7 +729,8 @@ void run_level(Engine& eng, const LevelSet& L, const DevLevel& D, const NodeDesc
         if (count == 0) continue;
         const double words = L.lds_words[c] + (pv.keep ? L.lds_f_words[c] : L.lds_s_words[c]);
         eng.timed(kFrontLds, P * L.lds_flops[c], P * es * words, [&] {
-            launch_front_lds(nodes, D.lds[c], count, L.lds_n[c], ent, diag, kids, maps, pv, P, tol, single, true,
+            launch_front_lds(nodes, D.lds[c], count, L.lds_n[c], L.lds_border[c], ent, diag, kids, maps, pv, P, tol,
+                             single, true,
                              eng.lane->stream);
         });
     }
@@ -1579,8 +1582,8 @@ int32_t front_schur(const double* f, int64_t n, int64_t ne, int64_t ne1, int32_t
     const bool small = n <= (single ? kLdsFrontC64 : kLdsFrontC128);
     const double tol = single ? kTolSingle : kTolDouble;
     if (small) {
-        launch_front_lds(nodes.as<NodeDesc>(), lv.as<int32_t>(), 1, int32_t(n), nullptr, nullptr, nullptr, nullptr, pv, 1,
-                         tol, single, false, eng.lane->stream);
+        launch_front_lds(nodes.as<NodeDesc>(), lv.as<int32_t>(), 1, int32_t(n), border_lds_words(int32_t(n), int32_t(ne)),
+                         nullptr, nullptr, nullptr, nullptr, pv, 1, tol, single, false, eng.lane->stream);
     } else if (unblocked_fronts()) {
         launch_factor_unblocked(nodes.as<NodeDesc>(), lv.as<int32_t>(), 1, pv, 1, tol, single, false, eng.lane->stream);
     } else {

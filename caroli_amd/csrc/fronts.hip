@@ -447,6 +447,179 @@ __global__ void __launch_bounds__(NT)
 }
 
 // ---------------------------------------------------------------------------------------------
+// Small fronts, border-only LDS. F22 (boundary rows and columns) gets nothing from H or z and, in
+// the elimination, only the rank-ne update L21 U12, so LDS holds the border alone: Wl = all n rows
+// of the separator columns (leading dimension n + 1) and Wr = the separator rows of the boundary
+// columns (leading dimension ne + 1), about 2 ne n words instead of n^2 (a front of 64 with a
+// separator of 14: 27 KB instead of 66 KB, so two and a half times the workgroups a CU holds).
+//  1. assembly of the border: -H on the separator, z on its diagonal, the children's entries that
+//     land in it (their F22 entries are not read here);
+//  2. the partial LU over the separator columns with pivots among the separator rows, as lds_lu:
+//     G lanes per physical row, logical positions swapped only, the pivot test of field.rs:61;
+//     boundary rows (never pivots) update their separator columns only;
+//  3. S = F22 - L21 U12 entry by entry: F22 gathered from the children through their inverse maps
+//     (or read from the given front), the product from LDS, S written once.
+// The factors written back (when kept) are those of front_lds_kernel.
+template <class T, int NT, int G, bool kAssemble>
+__global__ void __launch_bounds__(NT)
+    front_border_kernel(const NodeDesc* __restrict__ nodes, const int32_t* __restrict__ level,
+                        const Entry* __restrict__ ent, const int32_t* __restrict__ diag,
+                        const KidDesc* __restrict__ kids, const int32_t* __restrict__ maps, PointsView pv,
+                        double tol) {
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    __shared__ double sv[NT / 64];
+    __shared__ double rv[2][NT / 64];
+    __shared__ int32_t ri[2][NT / 64], rp[2][NT / 64];
+    __shared__ int32_t phys[kLdsMaxN];  // physical row of logical separator row k
+    const NodeDesc nd = nodes[level[blockIdx.x]];
+    const int64_t p = blockIdx.y;
+    T* F = reinterpret_cast<T*>(pv.arena) + p * pv.arena_stride + nd.off;
+    int32_t* piv = pv.piv + p * pv.piv_stride + nd.piv_off;
+    int32_t* status = pv.status + p * pv.status_stride + nd.slot;
+    const int n = nd.n, ne = nd.ne, ne1 = nd.ne1, nb = n - ne, tid = threadIdx.x, lane = tid & 63;
+    const int ldl = n + 1, ldr = ne + 1;
+    T* Wl = reinterpret_cast<T*>(lds_raw);  // Wl[r + j ldl], r < n, j < ne
+    T* Wr = Wl + ldl * ne;                   // Wr[r + (j - ne) ldr], r < ne, ne <= j < n
+    const T zero = cx<T>(0.0, 0.0);
+    // v added to the border entry (a, b) of the front (F22 entries are not held).
+    auto add = [&](int a, int b, T v) {
+        if (b < ne)
+            Wl[a + b * ldl] = cadd(Wl[a + b * ldl], v);
+        else if (a < ne)
+            Wr[a + (b - ne) * ldr] = cadd(Wr[a + (b - ne) * ldr], v);
+    };
+    const int words = ldl * ne + ldr * nb;
+    if (kAssemble) {
+        for (int i = tid; i < words; i += NT) Wl[i] = zero;
+        __syncthreads();
+        add_entries<NT>(ent, nd.ent_beg, nd.ent_end, pv.vals[p], n,
+                        [&](int q, double2 v) { add(q % n, q / n, convert<T>(v)); });
+        __syncthreads();
+        const T z = convert<T>(pv.z[p]);
+        for (int d = nd.diag_beg + tid; d < nd.diag_end; d += NT) {
+            const int q = diag[d];
+            add(q, q, z);
+        }
+        __syncthreads();
+        const T* kidbase = reinterpret_cast<const T*>(pv.arena) + p * pv.arena_stride;
+        for (int c = nd.kid_beg; c < nd.kid_end; ++c) {  // the children's border entries
+            const KidDesc kd = kids[c];
+            const T* S = kidbase + kd.src;
+            const int32_t* map = maps + kd.map_beg;
+            const int m2 = kd.size * kd.size;
+            for (int base = tid; base < m2; base += NT * kBatch) {
+                int ia[kBatch], ib[kBatch];
+                T v[kBatch];
+#pragma unroll
+                for (int u = 0; u < kBatch; ++u) {
+                    const int idx = min(base + u * NT, m2 - 1), a = idx % kd.size, b = idx / kd.size;
+                    ia[u] = map[a];
+                    ib[u] = map[b];
+                    v[u] = zero;
+                    if (ia[u] < ne || ib[u] < ne) v[u] = S[a + int64_t(b) * kd.ld];
+                }
+#pragma unroll
+                for (int u = 0; u < kBatch; ++u)
+                    if (base + u * NT < m2) add(ia[u], ib[u], v[u]);
+            }
+            __syncthreads();
+        }
+    } else {
+        for (int idx = tid; idx < n * ne; idx += NT) Wl[(idx % n) + (idx / n) * ldl] = F[idx];
+        for (int idx = tid; idx < ne * nb; idx += NT)
+            Wr[(idx % ne) + (idx / ne) * ldr] = F[(idx % ne) + int64_t(ne + idx / ne) * n];
+        __syncthreads();
+    }
+    // 2. Partial LU of the border (lds_lu on the split layout).
+    const int g = tid % G, r = tid / G;
+    int lg = r;
+    {
+        double mx = 0.0;
+        for (int idx = tid; idx < ne1 * ne1; idx += NT) mx = fmax(mx, cabs(Wl[(idx % ne1) + (idx / ne1) * ldl]));
+        const double scale = block_max(mx, sv);
+        for (int k = 0; k < ne; ++k) {
+            const int lim = k < ne1 ? ne1 : ne;
+            double big = -1.0;
+            int pat = kLdsMaxN, ph = 0;  // pivot: value, logical row, physical row
+            if (g == 0 && r < n && lg >= k && lg < lim) {
+                big = cabs(Wl[r + k * ldl]);
+                pat = lg;
+                ph = r;
+            }
+            wave_argmax(big, pat, ph);
+            if constexpr (NT > 64) {
+                const int par = k & 1;
+                if (lane == 0) {
+                    rv[par][tid >> 6] = big;
+                    ri[par][tid >> 6] = pat;
+                    rp[par][tid >> 6] = ph;
+                }
+                __syncthreads();
+                for (int w = 0; w < NT / 64; ++w) {
+                    const double ov = rv[par][w];
+                    const int oi = ri[par][w];
+                    if (ov > big || (ov == big && oi < pat)) {
+                        big = ov;
+                        pat = oi;
+                        ph = rp[par][w];
+                    }
+                }
+            }
+            const bool ok = k < ne1 ? (big > tol * scale) : (big > 0.0 && big < INFINITY);
+            if (!ok) {
+                if (tid == 0) *status = k < ne1 ? kFrontDelayed : kFrontSingular;
+                return;
+            }
+            if (tid == 0) piv[k] = pat;
+            if (r == ph)
+                lg = k;
+            else if (lg == k)
+                lg = pat;
+            if (r < n && lg > k) {
+                // The G lanes of this row read W[r, k] before lane g = 0 overwrites it (one wavefront).
+                const T l = cmul(Wl[r + k * ldl], crecip(Wl[ph + k * ldl]));
+                const int j0 = k + 1 + ((g - (k + 1)) % G + G) % G;
+                int j = j0;
+                for (; j < ne; j += G) Wl[r + j * ldl] = csub(Wl[r + j * ldl], cmul(l, Wl[ph + j * ldl]));
+                if (r < ne)  // separator rows: their boundary columns too
+                    for (; j < n; j += G) Wr[r + (j - ne) * ldr] = csub(Wr[r + (j - ne) * ldr], cmul(l, Wr[ph + (j - ne) * ldr]));
+                if (g == 0) Wl[r + k * ldl] = l;
+            }
+            __syncthreads();
+        }
+    }
+    if (g == 0 && r < ne) phys[lg] = r;  // separator rows keep logical positions below ne
+    __syncthreads();
+    // 3. S = F22 - L21 U12 (boundary rows are at their logical positions), written once.
+    const T* kidbase = reinterpret_cast<const T*>(pv.arena) + p * pv.arena_stride;
+    const int nb2 = nb * nb;
+    for (int idx = tid; idx < nb2; idx += NT) {
+        const int i = ne + idx % nb, j = ne + idx / nb;
+        T f = zero;
+        if (kAssemble) {
+            for (int c = nd.kid_beg; c < nd.kid_end; ++c) {
+                const KidDesc kd = kids[c];
+                const int32_t* inv = maps + kd.inv_beg;
+                const int a = inv[i], b = inv[j];
+                if (a >= 0 && b >= 0) f = cadd(f, kidbase[kd.src + a + int64_t(b) * kd.ld]);
+            }
+        } else {
+            f = F[i + int64_t(j) * n];
+        }
+        T acc = zero;
+        for (int k = 0; k < ne; ++k) acc = cadd(acc, cmul(Wl[i + k * ldl], Wr[phys[k] + (j - ne) * ldr]));
+        F[i + int64_t(j) * n] = csub(f, acc);
+    }
+    // The factors (kept for the solves): the border in logical row order.
+    if (pv.keep && r < n) {
+        for (int j = g; j < ne; j += G) F[lg + int64_t(j) * n] = Wl[r + j * ldl];
+        if (r < ne)
+            for (int j = ne + g; j < n; j += G) F[lg + int64_t(j) * n] = Wr[r + (j - ne) * ldr];
+    }
+    if (tid == 0) *status = kFrontOk;
+}
+
+// ---------------------------------------------------------------------------------------------
 // Small fronts (n <= 64), register-resident elimination. The front is assembled in LDS as in
 // front_lds_kernel, then lane r of every wave takes row r, and wave w the columns j = w (mod NW),
 // S of them, into registers. Step k of the partial LU:
@@ -1953,16 +2126,18 @@ void launch_blocked_update(const NodeDesc* nodes, const Tile* tiles, int32_t cou
         go(blocked_update_kernel<double2, false>);
 }
 
-void launch_front_lds(const NodeDesc* nodes, const int32_t* level, int32_t count, int32_t nmax, const Entry* ent,
-                      const int32_t* diag, const KidDesc* kids, const int32_t* maps, const PointsView& pv,
-                      int32_t points, double tol, bool single, bool assemble, hipStream_t st) {
+void launch_front_lds(const NodeDesc* nodes, const int32_t* level, int32_t count, int32_t nmax, int32_t border_words,
+                      const Entry* ent, const int32_t* diag, const KidDesc* kids, const int32_t* maps,
+                      const PointsView& pv, int32_t points, double tol, bool single, bool assemble, hipStream_t st) {
     if (count <= 0 || points <= 0) return;
     if (nmax > kLdsMaxN) throw std::invalid_argument("front too large for the LDS kernel");
-    const size_t lds = size_t(nmax) * size_t(nmax + 1) * (single ? sizeof(float2) : sizeof(double2));
-    // The LDS kernel: lanes per row and workgroup size by the largest front: 16 rows x 4, 32 x 2
-    // (one wave), 64 x 4, 128 x 2 (four waves). With CAROLI_SMALL_FRONTS=reg, fronts up to 64 go
-    // to the register-resident kernel (one lane per row, 16 columns per wave), which measured
-    // slower on the BHZ and graphene batches (the steps are latency-bound either way).
+    const size_t es = single ? sizeof(float2) : sizeof(double2);
+    const int mode = small_front_mode();
+    const size_t lds = mode == 0 ? size_t(border_words) * es : size_t(nmax) * size_t(nmax + 1) * es;
+    // Lanes per row and workgroup size by the largest front: 16 rows x 4 (one wave), 32 x 2 (one
+    // wave), 64 x 4, 128 x 2 (four waves). The default holds the border only in LDS
+    // (front_border_kernel); CAROLI_SMALL_FRONTS=lds holds the whole front (front_lds_kernel), =reg
+    // eliminates fronts up to 64 in registers (front_reg_kernel).
     auto go = [&](auto kernel, int nt) {
         allow_lds(reinterpret_cast<const void*>(kernel));
         hipLaunchKernelGGL(kernel, grid2(count, points), dim3(nt), lds, st, nodes, level, ent, diag, kids, maps, pv, tol);
@@ -1970,13 +2145,22 @@ void launch_front_lds(const NodeDesc* nodes, const int32_t* level, int32_t count
     auto pick = [&](auto t, auto a) {
         using T = decltype(t);
         constexpr bool A = decltype(a)::value;
-        if (nmax <= 64 && reg_small_fronts()) {  // register-resident: 1, 2 or 4 waves of 16 columns
+        if (mode == 2 && nmax <= 64) {
             if (nmax <= 16)
                 go(front_reg_kernel<T, 1, 16, A>, 64);
             else if (nmax <= 32)
                 go(front_reg_kernel<T, 2, 16, A>, 128);
             else
                 go(front_reg_kernel<T, 4, 16, A>, 256);
+        } else if (mode == 0) {
+            if (nmax <= 16)
+                go(front_border_kernel<T, 64, 4, A>, 64);
+            else if (nmax <= 32)
+                go(front_border_kernel<T, 64, 2, A>, 64);
+            else if (nmax <= 64)
+                go(front_border_kernel<T, 256, 4, A>, 256);
+            else
+                go(front_border_kernel<T, 256, 2, A>, 256);
         } else if (nmax <= 16)
             go(front_lds_kernel<T, 64, 4, A>, 64);
         else if (nmax <= 32)
@@ -1992,10 +2176,13 @@ void launch_front_lds(const NodeDesc* nodes, const int32_t* level, int32_t count
         assemble ? pick(double2{}, std::true_type{}) : pick(double2{}, std::false_type{});
 }
 
-bool reg_small_fronts() {
+int small_front_mode() {
     const char* v = std::getenv("CAROLI_SMALL_FRONTS");
-    return v && std::strcmp(v, "reg") == 0;
+    if (v && std::strcmp(v, "lds") == 0) return 1;
+    if (v && std::strcmp(v, "reg") == 0) return 2;
+    return 0;
 }
+
 
 int32_t border_lds_rows(int32_t max_ne, bool single) {
     const int32_t cap = kBorderLdsBytes / (kPanel * int32_t(single ? sizeof(float2) : sizeof(double2)));

@@ -110,10 +110,13 @@ void launch_assemble_root(const NodeDesc* nodes, const int32_t* level, int32_t c
                           const int32_t* diag, const KidDesc* kids, const int32_t* maps, const PointsView& pv,
                           int32_t points, int single, hipStream_t st);
 /// Small fronts (n <= nmax <= kLdsMaxN): assembly (or a copy of the given front when !assemble)
-/// and elimination in LDS.
-void launch_front_lds(const NodeDesc* nodes, const int32_t* level, int32_t count, int32_t nmax, const Entry* ent,
-                      const int32_t* diag, const KidDesc* kids, const int32_t* maps, const PointsView& pv,
-                      int32_t points, double tol, bool single, bool assemble, hipStream_t st);
+/// and elimination in LDS. border_words: the largest (n + 1) ne + (ne + 1) (n - ne) of the fronts
+/// (LDS words of the border-only kernel, the default).
+void launch_front_lds(const NodeDesc* nodes, const int32_t* level, int32_t count, int32_t nmax, int32_t border_words,
+                      const Entry* ent, const int32_t* diag, const KidDesc* kids, const int32_t* maps,
+                      const PointsView& pv, int32_t points, double tol, bool single, bool assemble, hipStream_t st);
+/// LDS words of a small front in the border-only kernel.
+inline int32_t border_lds_words(int32_t n, int32_t ne) { return (n + 1) * ne + (ne + 1) * (n - ne); }
 /// Large fronts: the children's contributions to the border (rows or columns below ne) gathered
 /// over `tiles`; the border LU (F11, L21, U12; non-root fronts first add -H and z); the Schur
 /// update S = F22 - L21 U12 over `tiles`, F22 gathered from the children (gather) or read from the
@@ -132,9 +135,9 @@ void launch_f11(const NodeDesc* nodes, const int32_t* level, int32_t count, int3
                 const int32_t* diag, const PointsView& pv, int32_t points, double tol, bool single, hipStream_t st);
 void launch_trsm(const NodeDesc* nodes, const Tile* tiles, int32_t count, int32_t max_ne, const PointsView& pv,
                  int32_t points, bool single, hipStream_t st);
-/// CAROLI_SMALL_FRONTS=reg: fronts up to 64 variables eliminated in registers (front_reg_kernel)
-/// instead of in LDS (front_lds_kernel).
-bool reg_small_fronts();
+/// Small-front kernel: 0 border-only LDS (front_border_kernel, the default), 1 the whole front in
+/// LDS (CAROLI_SMALL_FRONTS=lds), 2 in registers for fronts up to 64 (CAROLI_SMALL_FRONTS=reg).
+int small_front_mode();
 /// CAROLI_FRONT_KERNEL=border: every large front through border_kernel (no f11/trsm split).
 bool border_only();
 /// Smallest separator (exclusive) of the blocked path: CAROLI_FRONT_KERNEL=blocked takes every
