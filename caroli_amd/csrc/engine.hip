@@ -165,6 +165,28 @@ struct DevLevel {
 };
 
 /// Splits the fronts `list` of `nodes` by kernel (root: the bordered root front, always large).
+/// Gather tiles covering the rectangle [r0, r1) x [c0, c1) of a front exactly: kTile along the
+/// short side, and along the long side as many multiples of kTile as keep a tile near kTile^2
+/// entries (a border strip of 20 columns: tiles of 192 x 20). The extents go in Tile::pad as
+/// (rows << 16) | cols.
+void add_gather_tiles(std::vector<Tile>& out, int32_t node, int r0, int r1, int c0, int c1) {
+    const int R = r1 - r0, C = c1 - c0;
+    if (R <= 0 || C <= 0) return;
+    int th, tw;
+    if (R >= C) {
+        tw = std::min(C, kTile);
+        th = std::min(R, std::max(kTile, kTile * kTile / tw / kTile * kTile));
+    } else {
+        th = std::min(R, kTile);
+        tw = std::min(C, std::max(kTile, kTile * kTile / th / kTile * kTile));
+    }
+    for (int j0 = c0; j0 < c1; j0 += tw)
+        for (int i0 = r0; i0 < r1; i0 += th) {
+            const int h = std::min(th, r1 - i0), w = std::min(tw, c1 - j0);
+            out.push_back({node, i0, j0, (h << 16) | w});
+        }
+}
+
 LevelSet classify(const std::vector<NodeDesc>& nodes, const std::vector<KidDesc>& kids, const std::vector<int32_t>& list,
                   bool single, bool root) {
     LevelSet L;
@@ -223,10 +245,10 @@ LevelSet classify(const std::vector<NodeDesc>& nodes, const std::vector<KidDesc>
         L.schur_words += nb * nb + 2.0 * ne * nb + (root ? nb * nb : (1.0 - border_share) * kid_words);
         for (int i0 = 0; i0 < d.n - d.ne; i0 += kTile)
             for (int j0 = 0; j0 < d.n - d.ne; j0 += kTile) L.tiles.push_back({q, i0, j0, 0});
-        if (!root)
-            for (int j0 = 0; j0 < d.n; j0 += kTile)
-                for (int i0 = 0; i0 < d.n; i0 += kTile)
-                    if (i0 < d.ne || j0 < d.ne) L.gtiles.push_back({q, i0, j0, 0});
+        if (!root) {
+            add_gather_tiles(L.gtiles, q, 0, d.n, 0, d.ne);     // separator columns, all rows
+            add_gather_tiles(L.gtiles, q, 0, d.ne, d.ne, d.n);  // separator rows, boundary columns
+        }
     }
     L.plan_blocked(nodes);
     return L;
@@ -501,7 +523,8 @@ struct Inputs {
 };
 
 /// Kernel families timed by cg_kernel_timing (border_kernel stands for factor_global_kernel when
-/// CAROLI_FRONT_KERNEL=unblocked).
+/// CAROLI_FRONT_KERNEL=unblocked; the small-front family is reported under the name of the kernel
+/// CAROLI_SMALL_FRONTS selects).
 enum Family {
     kFrontLds,
     kGather,
@@ -1664,7 +1687,9 @@ int32_t kernel_stats(cg_kernel_stat* out, int32_t max_records) {
     int32_t n = 0;
     for (int f = 0; f < kFamilies && n < max_records; ++f) {
         cg_kernel_stat& r = out[n++];
-        std::snprintf(r.name, sizeof(r.name), "%s", kFamilyNames[f]);
+        // The small-front family carries the name of the kernel that ran it.
+        const char* small[3] = {"front_border_kernel", "front_lds_kernel", "front_reg_kernel"};
+        std::snprintf(r.name, sizeof(r.name), "%s", f == kFrontLds ? small[small_front_mode()] : kFamilyNames[f]);
         r.launches = eng.fams[f].launches;
         r.seconds = eng.fams[f].seconds;
         r.flops = eng.fams[f].flops;

@@ -1042,9 +1042,10 @@ __global__ void __launch_bounds__(NT)
 
 // ---------------------------------------------------------------------------------------------
 // Large fronts, step 0: the children's Schur complements gathered into the border of the front
-// (rows or columns below ne; F22 is gathered by schur_kernel), one workgroup per 64 x 64 tile:
-// F[i, j] = sum over children c of S_c[inv_c(i), inv_c(j)], written once (no zero fill, no
-// read-modify-write).
+// (rows or columns below ne; F22 is gathered by schur_kernel), one workgroup per tile of the two
+// border strips (rows x cols in Tile::pad as (rows << 16) | cols, about kTile^2 entries, inside the
+// border): F[i, j] = sum over children c of S_c[inv_c(i), inv_c(j)], written once (no zero fill,
+// no read-modify-write).
 template <class T>
 __global__ void __launch_bounds__(256)
     gather_kernel(const NodeDesc* __restrict__ nodes, const Tile* __restrict__ tiles, const KidDesc* __restrict__ kids,
@@ -1056,7 +1057,7 @@ __global__ void __launch_bounds__(256)
     const T* kidbase = reinterpret_cast<const T*>(pv.arena) + p * pv.arena_stride;
     T* F = reinterpret_cast<T*>(pv.arena) + p * pv.arena_stride + nd.off;
     const int n = nd.n, ne = nd.ne;
-    const int rows = min(kTile, n - tl.i0), cols = min(kTile, n - tl.j0), cnt = rows * cols;
+    const int rows = min(tl.pad >> 16, n - tl.i0), cols = min(tl.pad & 0xffff, n - tl.j0), cnt = rows * cols;
     for (int base = int(threadIdx.x); base < cnt; base += 256 * kBatch) {
         int ii[kBatch], jj[kBatch];
 #pragma unroll
@@ -1110,21 +1111,28 @@ __global__ void __launch_bounds__(256)
     int32_t* piv = pv.piv + p * pv.piv_stride + nd.piv_off;
     int32_t* status = pv.status + p * pv.status_stride + nd.slot;
     const int n = nd.n, ne = nd.ne, ne1 = nd.ne1, tid = threadIdx.x, ld = ne + 1;
+    // F11 (the gathered children's part) into LDS, then -H and z added there; the entries in the
+    // border strips F12 / F21 are added in place in HBM (trsm_kernel reads them next).
+    for (int idx = tid; idx < ne * ne; idx += 256) W[(idx % ne) + (idx / ne) * ld] = F[(idx % ne) + int64_t(idx / ne) * n];
+    __syncthreads();
     if (nd.ent_end > nd.ent_beg) {
-        add_entries<256>(ent, nd.ent_beg, nd.ent_end, pv.vals[p], n,
-                         [&](int at, double2 v) { F[at] = cadd(F[at], convert<T>(v)); });
+        add_entries<256>(ent, nd.ent_beg, nd.ent_end, pv.vals[p], n, [&](int at, double2 v) {
+            const int a = at % n, b = at / n;
+            if (a < ne && b < ne)
+                W[a + b * ld] = cadd(W[a + b * ld], convert<T>(v));
+            else
+                F[at] = cadd(F[at], convert<T>(v));
+        });
         __syncthreads();
     }
-    if (nd.diag_end > nd.diag_beg) {
+    if (nd.diag_end > nd.diag_beg) {  // the eliminated orbitals: all inside F11
         const T z = convert<T>(pv.z[p]);
         for (int d = nd.diag_beg + tid; d < nd.diag_end; d += 256) {
-            const int64_t q = diag[d];
-            F[q + q * n] = cadd(F[q + q * n], z);
+            const int q = diag[d];
+            W[q + q * ld] = cadd(W[q + q * ld], z);
         }
         __syncthreads();
     }
-    for (int idx = tid; idx < ne * ne; idx += 256) W[(idx % ne) + (idx / ne) * ld] = F[(idx % ne) + int64_t(idx / ne) * n];
-    __syncthreads();
     const int g = tid % G, r = tid / G;
     int lg = r;
     if (!lds_lu<T, 256, G>(W, ld, ne, ne, ne1, tol, piv, status, lg)) return;
@@ -1160,7 +1168,11 @@ __global__ void __launch_bounds__(256)
         }
         if (tid < ne) src[tid] = v;
     }
-    for (int idx = tid; idx < ne * ne; idx += 256) A[(idx % ne) + (idx / ne) * la] = F[(idx % ne) + int64_t(idx / ne) * n];
+    // Only the triangle the solve reads: the strict lower L11 (kind 0) or U11 with its diagonal.
+    for (int idx = tid; idx < ne * ne; idx += 256) {
+        const int i = idx % ne, j = idx / ne;
+        if (upper ? i <= j : i > j) A[i + j * la] = F[i + int64_t(j) * n];
+    }
     __syncthreads();
     // Four lanes per column of U12 (kind 0) or row of L21 (kind 1), the workgroup's 64 of them in
     // one pass: lane g holds entries g + 4 s (s < NS, ne <= 4 NS) in registers, and the solve
@@ -1248,7 +1260,6 @@ __global__ void __launch_bounds__(256)
            nd.off;
     const int n = nd.n, ne = nd.ne, nb = n - ne, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wi = (wave & 1) * 32, wj = (wave >> 1) * 32;
-    const T zero = cx<T>(0.0, 0.0);
     typename M::acc cr[2][2], ci[2][2];
 #pragma unroll
     for (int x = 0; x < 2; ++x)
@@ -1269,13 +1280,13 @@ __global__ void __launch_bounds__(256)
             {
                 const int r = e % kTile, c = e / kTile, i = tl.i0 + r, k = kc + c;
                 T v = F[(ne + min(i, nb - 1)) + int64_t(min(k, ne - 1)) * n];
-                if (!(i < nb && k < ne)) v = zero;
+                if (!(i < nb && k < ne)) v = cx<T>(0.0, 0.0);
                 sa[u] = v;
             }
             {
                 const int c = e % kChunk, r = e / kChunk, j = tl.j0 + r, k = kc + c;
                 T v = F[min(k, ne - 1) + int64_t(ne + min(j, nb - 1)) * n];
-                if (!(j < nb && k < ne)) v = zero;
+                if (!(j < nb && k < ne)) v = cx<T>(0.0, 0.0);
                 sb[u] = v;
             }
         }
@@ -1352,7 +1363,7 @@ __global__ void __launch_bounds__(256)
             for (int y = 0; y < 2; ++y)
 #pragma unroll
                 for (int jj = 0; jj < 4; ++jj) {
-                    T f = zero;
+                    T f = cx<T>(0.0, 0.0);
                     if (a0[y] >= 0 && b0[jj] >= 0) f = cadd(f, v0[y][jj]);
                     if (a1[y] >= 0 && b1[jj] >= 0) f = cadd(f, v1[y][jj]);
                     f22[y][jj] = f;
@@ -1361,7 +1372,7 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
             for (int y = 0; y < 2; ++y)
 #pragma unroll
-                for (int jj = 0; jj < 4; ++jj) f22[y][jj] = zero;
+                for (int jj = 0; jj < 4; ++jj) f22[y][jj] = cx<T>(0.0, 0.0);
             for (int c = nd.kid_beg; c < nd.kid_end; ++c) {
                 const KidDesc kd = kids[c];
                 const int32_t* inv = maps + kd.inv_beg + ne;
@@ -1376,7 +1387,7 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
                     for (int jj = 0; jj < 4; ++jj) {
                         T v = S[max(a[y], 0) + int64_t(max(b[jj], 0)) * kd.ld];
-                        if (a[y] < 0 || b[jj] < 0) v = zero;
+                        if (a[y] < 0 || b[jj] < 0) v = cx<T>(0.0, 0.0);
                         f22[y][jj] = cadd(f22[y][jj], v);
                     }
             }
