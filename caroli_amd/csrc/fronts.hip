@@ -1240,7 +1240,9 @@ __global__ void __launch_bounds__(256)
 // Large fronts, step 2: S = F22 - L21 U12 (K = ne) over 64 x 64 tiles of the Schur block, one
 // workgroup per (tile, point), on MFMA. Each wave owns a 32 x 32 quarter (2 x 2 blocks of 16 x 16,
 // complex = 4 real MFMAs per block and K-step of 4); L21 and U12 are staged through LDS in
-// chunks of kChunk columns / rows. Fronts whose border failed are skipped.
+// chunks of kChunk columns / rows. Fronts whose border failed are skipped. (The compiler keeps the
+// constant zero in 16-32 bytes of scratch here; the form without it, zero written as a literal,
+// schedules the K loop worse: 10 % slower on the BHZ 200 x 64 batch.)
 constexpr int kChunk = 16, kTileLd = kTile + 1;
 
 template <class T, bool kRoot, bool kGather>
@@ -1260,6 +1262,7 @@ __global__ void __launch_bounds__(256)
            nd.off;
     const int n = nd.n, ne = nd.ne, nb = n - ne, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wi = (wave & 1) * 32, wj = (wave >> 1) * 32;
+    const T zero = cx<T>(0.0, 0.0);
     typename M::acc cr[2][2], ci[2][2];
 #pragma unroll
     for (int x = 0; x < 2; ++x)
@@ -1280,13 +1283,13 @@ __global__ void __launch_bounds__(256)
             {
                 const int r = e % kTile, c = e / kTile, i = tl.i0 + r, k = kc + c;
                 T v = F[(ne + min(i, nb - 1)) + int64_t(min(k, ne - 1)) * n];
-                if (!(i < nb && k < ne)) v = cx<T>(0.0, 0.0);
+                if (!(i < nb && k < ne)) v = zero;
                 sa[u] = v;
             }
             {
                 const int c = e % kChunk, r = e / kChunk, j = tl.j0 + r, k = kc + c;
                 T v = F[min(k, ne - 1) + int64_t(ne + min(j, nb - 1)) * n];
-                if (!(j < nb && k < ne)) v = cx<T>(0.0, 0.0);
+                if (!(j < nb && k < ne)) v = zero;
                 sb[u] = v;
             }
         }
@@ -1363,7 +1366,7 @@ __global__ void __launch_bounds__(256)
             for (int y = 0; y < 2; ++y)
 #pragma unroll
                 for (int jj = 0; jj < 4; ++jj) {
-                    T f = cx<T>(0.0, 0.0);
+                    T f = zero;
                     if (a0[y] >= 0 && b0[jj] >= 0) f = cadd(f, v0[y][jj]);
                     if (a1[y] >= 0 && b1[jj] >= 0) f = cadd(f, v1[y][jj]);
                     f22[y][jj] = f;
@@ -1372,7 +1375,7 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
             for (int y = 0; y < 2; ++y)
 #pragma unroll
-                for (int jj = 0; jj < 4; ++jj) f22[y][jj] = cx<T>(0.0, 0.0);
+                for (int jj = 0; jj < 4; ++jj) f22[y][jj] = zero;
             for (int c = nd.kid_beg; c < nd.kid_end; ++c) {
                 const KidDesc kd = kids[c];
                 const int32_t* inv = maps + kd.inv_beg + ne;
@@ -1387,7 +1390,7 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
                     for (int jj = 0; jj < 4; ++jj) {
                         T v = S[max(a[y], 0) + int64_t(max(b[jj], 0)) * kd.ld];
-                        if (a[y] < 0 || b[jj] < 0) v = cx<T>(0.0, 0.0);
+                        if (a[y] < 0 || b[jj] < 0) v = zero;
                         f22[y][jj] = cadd(f22[y][jj], v);
                     }
             }
