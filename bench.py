@@ -14,6 +14,10 @@ dT/d(eps) for all 12,800 orbitals); `--config c5` the uneven cubic batch (W = 16
 cross-sections L drawn from {16, 24, 32, 40, 48} with xoshiro256** seeded 5, 4 energies in
 [-1, 1]) on the first `--systems` systems whose L is at most `--max-l`.
 
+Before the warmups the engine's autotuner times the kernel choices per front class on the batch's
+own points (`--autotune`, on by default except for mixed precision) and keeps the fastest; the JSON
+line reports the choice under `tuning`.
+
 A step is one batched launch over all points; the engine runs up to four launch units (spin blocks,
 halves of a block's points, systems of other sizes) at once on four streams, and replays each unit's
 sweep from a HIP graph after the warmups. The JSON line carries the roofline of the dominant kernel
@@ -206,6 +210,14 @@ def run_transmission(args, ca):
     progress(f"{npts} points: {desc}; systems built in {build_s:.1f} s, self-energies {sigma_s:.1f} s")
     batch = ca.ResidentBatch(pts_sys, pts_sig, prec)
     progress("batch resident")
+    tuning = None
+    if args.autotune == "on" or (args.autotune == "auto" and args.precision != "mixed"):
+        # Kernel choices timed on these points (eager sweeps, before the warmups and outside the
+        # timed steps); the fastest stays in force for everything after it in this process.
+        t0 = time.perf_counter()
+        choice, sec = batch.autotune()
+        tuning = {"choice": choice, "seconds_per_sweep": sec, "tuning_seconds": time.perf_counter() - t0}
+        progress(f"autotune: {choice}, {sec:.3f} s per eager sweep ({tuning['tuning_seconds']:.1f} s)")
     for w in range(args.warmup):
         t0 = time.perf_counter()
         batch.run()
@@ -259,6 +271,7 @@ def run_transmission(args, ca):
         "whole_step_algorithmic_gbs": bytes_pt * npts / step / 1e9,
         "sigma_host_seconds": sigma_s,
         "build_seconds": build_s,
+        "tuning": tuning if tuning is not None else {"choice": ca.get_tuning(), "autotuned": False},
         "kernels": kernels_of(ks, args.roofline_steps),
         "roofline_steps": args.roofline_steps,
         ("mixed" if args.precision == "double" else "plain"): other,
@@ -360,6 +373,9 @@ def main():
     ap.add_argument("--systems", type=int, default=6, help="c5: systems taken from the 512")
     ap.add_argument("--max-l", type=int, default=40, help="c5: largest cross-section taken")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU time of the baseline sample")
+    ap.add_argument("--autotune", default="auto", choices=["auto", "on", "off"],
+                    help="time the kernel choices on the batch first (auto: complex128 and complex64 runs; "
+                         "a mixed run re-runs its refinement per candidate)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-compare", dest="compare", action="store_false",
                     help="skip the run of the same points in the other precision (mixed vs plain)")

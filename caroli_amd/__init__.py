@@ -325,6 +325,34 @@ class ResidentBatch:
         _check(capi.cg_batch_run(self._h, _dptr(t), st.ctypes.data_as(C.POINTER(C.c_int32)), C.byref(bs)))
         return t, st
 
+    def autotune(self):
+        """Times eager sweeps of these points under candidate kernel choices (cg_batch_autotune),
+        leaves the fastest in force and the batch planned with it. Returns (choice dict, seconds
+        per sweep)."""
+        tu = capi.Tuning()
+        sec = C.c_double()
+        _check(capi.cg_batch_autotune(self._h, C.byref(tu), C.byref(sec)), "autotune")
+        return tu.as_dict(), sec.value
+
+
+def get_tuning():
+    """The kernel choices in force (cg_get_tuning) as a dict."""
+    tu = capi.Tuning()
+    _check(capi.cg_get_tuning(C.byref(tu)))
+    return tu.as_dict()
+
+
+def set_tuning(choice=None, **fields):
+    """Sets the kernel choices (cg_set_tuning): a dict or keyword fields over the ones in force;
+    set_tuning(None) with no fields goes back to the defaults and the CAROLI_* variables."""
+    if choice is None and not fields:
+        _check(capi.cg_set_tuning(None))
+        return
+    cur = get_tuning()
+    cur.update(choice or {})
+    cur.update(fields)
+    _check(capi.cg_set_tuning(C.byref(capi.Tuning(**cur))), "tuning")
+
 
 def kernel_timing(enable=True):
     _check(capi.cg_kernel_timing(1 if enable else 0))

@@ -99,3 +99,22 @@ EXPORTS += ["cg_front_schur"]
 cg_uniforms = _sig("cg_uniforms", C.c_int32, C.c_int32, C.c_uint64, P(C.c_int32), C.c_int64, P(C.c_double))
 EXPORTS += ["cg_batch_create", "cg_batch_run", "cg_batch_destroy", "cg_kernel_timing", "cg_kernel_stats",
             "cg_uniforms"]
+
+
+class Tuning(C.Structure):
+    """cg_tuning: the kernel choices per front class."""
+    _fields_ = [
+        ("small_fronts", C.c_int32),
+        ("split", C.c_int32),
+        ("blocked_min_ne", C.c_int32),
+        ("unblocked", C.c_int32),
+    ]
+
+    def as_dict(self):
+        return {f: int(getattr(self, f)) for f, _ in self._fields_}
+
+
+cg_get_tuning = _sig("cg_get_tuning", C.c_int32, P(Tuning))
+cg_set_tuning = _sig("cg_set_tuning", C.c_int32, P(Tuning))
+cg_batch_autotune = _sig("cg_batch_autotune", C.c_int32, C.c_void_p, P(Tuning), P(C.c_double))
+EXPORTS += ["cg_get_tuning", "cg_set_tuning", "cg_batch_autotune"]

@@ -437,6 +437,22 @@ int32_t cg_batch_run(cg_batch* batch, double* out_t, int32_t* out_status, cg_bat
 
 void cg_batch_destroy(cg_batch* batch) { gpu::destroy_batch(batch); }
 
+int32_t cg_get_tuning(cg_tuning* out) {
+    if (!out) return CG_ERR_INVALID;
+    return gpu::get_tuning(out);
+}
+
+int32_t cg_set_tuning(const cg_tuning* t) { return gpu::set_tuning(t); }
+
+int32_t cg_batch_autotune(cg_batch* batch, cg_tuning* out, double* out_seconds) {
+    if (!batch) return CG_ERR_INVALID;
+    try {
+        return gpu::autotune_batch(batch, out, out_seconds);
+    } catch (const std::exception&) {
+        return CG_ERR_DEVICE;
+    }
+}
+
 int32_t cg_front_schur(const double* f, int64_t n, int64_t ne, int64_t ne1, int32_t precision, double* out_s,
                        int32_t* out_status) {
     if (!f || !out_status || n <= 0 || ne < 0 || ne > n || ne1 < 0 || ne1 > ne || (ne < n && !out_s) ||

@@ -339,6 +339,7 @@ struct DevPlan {
 struct Structure {
     std::vector<HostPlan> host;
     std::vector<DevPlan> dev;
+    Tuning tuning;  // the kernel choices its plans were classified under
 };
 
 namespace {
@@ -635,16 +636,21 @@ struct Engine {
     }
 };
 
+/// The plans of a system's structure, shared by the systems of one structure key, and re-planned
+/// when the kernel tuning in force classifies fronts differently from the one they were built under.
 std::shared_ptr<Structure> structure_of(Engine& eng, const cg_system* s) {
     std::lock_guard<std::mutex> g(s->mu);
-    if (s->structure) return s->structure;
+    const Tuning tu = current_tuning();
+    if (s->structure && s->structure->tuning.same_plan(tu)) return s->structure;
     auto it = eng.registry.find(s->key);
     if (it != eng.registry.end())
-        if (auto sp = it->second.lock()) {
-            s->structure = sp;
-            return sp;
-        }
+        if (auto sp = it->second.lock())
+            if (sp->tuning.same_plan(tu)) {
+                s->structure = sp;
+                return sp;
+            }
     auto sp = std::make_shared<Structure>();
+    sp->tuning = tu;
     for (auto& blk : s->sym.blocks) sp->host.push_back(build_plan(blk, s->spec.h));
     sp->dev.resize(sp->host.size());
     for (size_t b = 0; b < sp->host.size(); ++b) upload_plan(sp->host[b], sp->dev[b], s->spec.h, eng.lane->stream);
@@ -1260,7 +1266,8 @@ struct cg_batch {
     int32_t precision = CG_DOUBLE;
     std::vector<Unit> units;
     double flops = 0, bytes = 0;
-    int lanes = 1;  // units in flight at once (caroli::gpu::kLanes when their buffers fit)
+    int lanes = 1;       // units in flight at once (caroli::gpu::kLanes when their buffers fit)
+    bool eager = false;  // sweeps without HIP graphs (the autotuner's timed runs)
 };
 
 namespace caroli::gpu {
@@ -1445,7 +1452,7 @@ int32_t run_batch(cg_batch* bt, double* out_t, int32_t* out_status, cg_batch_sta
                 pv.status_stride = hp.slots;
                 pv.t_out = eng.lane->tout.as<double>();
                 pv.keep = mixed ? 1 : 0;  // the refinement solves with the factors
-                if (!mixed && !eng.timing && graphs_enabled())
+                if (!mixed && !eng.timing && !bt->eager && graphs_enabled())
                     sweep_graph(eng, u, hp, dp, pv, int32_t(u.points.size()), single);
                 else
                     sweep(eng, hp, dp, pv, int32_t(u.points.size()), single, u.in);
@@ -1567,6 +1574,102 @@ int32_t batch_transmission(const cg_system* const* systems, const cg_sigma* cons
 }
 
 void destroy_batch(cg_batch* b) { delete b; }
+
+namespace {
+cg_tuning to_c(const Tuning& t) { return {t.small_fronts, t.split, t.blocked_min_ne, t.unblocked}; }
+Tuning from_c(const cg_tuning& c) {
+    Tuning t;
+    t.small_fronts = c.small_fronts;
+    t.split = c.split;
+    t.blocked_min_ne = c.blocked_min_ne;
+    t.unblocked = c.unblocked;
+    return t;
+}
+}  // namespace
+
+int32_t get_tuning(cg_tuning* out) {
+    *out = to_c(current_tuning());
+    return CG_OK;
+}
+
+int32_t set_tuning(const cg_tuning* t) {
+    if (!t) {
+        set_tuning_override(nullptr);
+        return CG_OK;
+    }
+    if (t->small_fronts < 0 || t->small_fronts > 2 || t->split < 0 || t->split > 1 || t->blocked_min_ne < 0 ||
+        t->unblocked < 0 || t->unblocked > 1)
+        return CG_ERR_INVALID;
+    const Tuning tu = from_c(*t);
+    set_tuning_override(&tu);
+    return CG_OK;
+}
+
+/// Coordinate descent over the kernel choices on the batch's own points: every candidate puts the
+/// batch's units on plans classified under it (their captured graphs dropped), runs one eager
+/// sweep to settle and times the next; a candidate replaces the best so far when 2 % faster.
+int32_t autotune_batch(cg_batch* bt, cg_tuning* out, double* out_seconds) {
+    Engine& eng = Engine::get();
+    if (!eng.ok) return CG_ERR_DEVICE;
+    const int64_t count = int64_t(bt->systems.size());
+    std::vector<double> t(size_t(std::max<int64_t>(count, 1)));
+    std::vector<int32_t> st(t.size());
+    auto apply = [&](const Tuning& tu) {
+        std::lock_guard<std::mutex> lock(eng.mu);
+        set_tuning_override(&tu);
+        for (auto& u : bt->units) {
+            u.structure = structure_of(eng, bt->systems[u.points[0]]);
+            u.graph = SweepGraph{};
+        }
+        for (int64_t p = 0; p < count; ++p) structure_of(eng, bt->systems[p]);
+    };
+    int32_t rc = CG_OK;
+    auto measure = [&](const Tuning& tu) {
+        apply(tu);
+        bt->eager = true;
+        rc = run_batch(bt, t.data(), st.data(), nullptr);
+        double sec = INFINITY;
+        if (rc == CG_OK) {
+            auto t0 = std::chrono::steady_clock::now();
+            rc = run_batch(bt, t.data(), st.data(), nullptr);
+            sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        }
+        bt->eager = false;
+        return sec;
+    };
+    Tuning best = current_tuning();
+    double best_s = measure(best);
+    auto consider = [&](const Tuning& c) {
+        if (rc != CG_OK || (c.same_plan(best) && c.small_fronts == best.small_fronts)) return;
+        const double sec = measure(c);
+        if (rc == CG_OK && sec < 0.98 * best_s) {
+            best = c;
+            best_s = sec;
+        }
+    };
+    for (int sf : {0, 1, 2}) {
+        Tuning c = best;
+        c.small_fronts = sf;
+        consider(c);
+    }
+    if (!best.unblocked) {
+        for (int sp : {1, 0}) {
+            Tuning c = best;
+            c.split = sp;
+            consider(c);
+        }
+        for (int bm : {64, 128, 256}) {
+            Tuning c = best;
+            c.blocked_min_ne = bm;
+            consider(c);
+        }
+    }
+    apply(best);  // the winner stays in force, the batch on its plans (graphs captured anew)
+    if (rc != CG_OK) return rc;
+    if (out) *out = to_c(best);
+    if (out_seconds) *out_seconds = best_s;
+    return CG_OK;
+}
 
 int32_t front_schur(const double* f, int64_t n, int64_t ne, int64_t ne1, int32_t precision, double* out_s,
                     int32_t* out_status) {

@@ -16,6 +16,9 @@ int32_t create_batch(const cg_system* const* systems, const cg_sigma* const* sig
                      cg_batch** out);
 int32_t run_batch(cg_batch* b, double* out_t, int32_t* out_status, cg_batch_stats* stats);
 void destroy_batch(cg_batch* b);
+int32_t get_tuning(cg_tuning* out);
+int32_t set_tuning(const cg_tuning* t);
+int32_t autotune_batch(cg_batch* b, cg_tuning* out, double* out_seconds);
 int32_t kernel_timing(int32_t enable);
 int32_t front_schur(const double* f, int64_t n, int64_t ne, int64_t ne1, int32_t precision, double* out_s,
                     int32_t* out_status);

@@ -2077,22 +2077,54 @@ void launch_assemble_root(const NodeDesc* nodes, const int32_t* level, int32_t c
                            level, ent, diag, kids, maps, pv);
 }
 
-bool unblocked_fronts() {
-    const char* v = std::getenv("CAROLI_FRONT_KERNEL");
-    return v && std::strcmp(v, "unblocked") == 0;
+namespace {
+std::mutex tuning_mu;
+bool tuning_set = false;
+Tuning tuning_override;
+
+/// The defaults and the CAROLI_* variables: CAROLI_SMALL_FRONTS=lds|reg, CAROLI_FRONT_KERNEL=
+/// blocked (every large front and the root blocked) | border (no f11 / trsm split, none blocked) |
+/// unblocked, CAROLI_BLOCKED_MIN_NE=<separator size above which fronts go blocked>.
+Tuning env_tuning() {
+    Tuning t;
+    t.blocked_min_ne = kBlockedMinNe;
+    if (const char* v = std::getenv("CAROLI_SMALL_FRONTS")) {
+        if (std::strcmp(v, "lds") == 0) t.small_fronts = 1;
+        if (std::strcmp(v, "reg") == 0) t.small_fronts = 2;
+    }
+    if (const char* v = std::getenv("CAROLI_FRONT_KERNEL")) {
+        if (std::strcmp(v, "blocked") == 0) t.blocked_min_ne = 0;
+        if (std::strcmp(v, "border") == 0) {
+            t.split = 0;
+            t.blocked_min_ne = 1 << 30;
+        }
+        if (std::strcmp(v, "unblocked") == 0) {
+            t.unblocked = 1;
+            t.blocked_min_ne = 1 << 30;
+        }
+    }
+    if (const char* v = std::getenv("CAROLI_BLOCKED_MIN_NE"))
+        if (*v) t.blocked_min_ne = std::max(0, std::atoi(v));
+    return t;
+}
+}  // namespace
+
+Tuning current_tuning() {
+    std::lock_guard<std::mutex> g(tuning_mu);
+    return tuning_set ? tuning_override : env_tuning();
 }
 
-bool border_only() {
-    const char* v = std::getenv("CAROLI_FRONT_KERNEL");
-    return v && std::strcmp(v, "border") == 0;
+void set_tuning_override(const Tuning* t) {
+    std::lock_guard<std::mutex> g(tuning_mu);
+    tuning_set = t != nullptr;
+    if (t) tuning_override = *t;
 }
 
-int blocked_min_ne() {
-    const char* v = std::getenv("CAROLI_FRONT_KERNEL");
-    if (v && std::strcmp(v, "blocked") == 0) return 0;
-    if (v && (std::strcmp(v, "border") == 0 || std::strcmp(v, "unblocked") == 0)) return 1 << 30;
-    return kBlockedMinNe;
-}
+bool unblocked_fronts() { return current_tuning().unblocked != 0; }
+
+bool border_only() { return current_tuning().split == 0; }
+
+int blocked_min_ne() { return current_tuning().blocked_min_ne; }
 
 void launch_blocked_panel(const NodeDesc* nodes, const int32_t* list, int32_t count, const Entry* ent,
                           const int32_t* diag, const PointsView& pv, int32_t points, double tol, bool single, bool root,
@@ -2190,12 +2222,7 @@ void launch_front_lds(const NodeDesc* nodes, const int32_t* level, int32_t count
         assemble ? pick(double2{}, std::true_type{}) : pick(double2{}, std::false_type{});
 }
 
-int small_front_mode() {
-    const char* v = std::getenv("CAROLI_SMALL_FRONTS");
-    if (v && std::strcmp(v, "lds") == 0) return 1;
-    if (v && std::strcmp(v, "reg") == 0) return 2;
-    return 0;
-}
+int small_front_mode() { return current_tuning().small_fronts; }
 
 
 int32_t border_lds_rows(int32_t max_ne, bool single) {

@@ -135,6 +135,20 @@ void launch_f11(const NodeDesc* nodes, const int32_t* level, int32_t count, int3
                 const int32_t* diag, const PointsView& pv, int32_t points, double tol, bool single, hipStream_t st);
 void launch_trsm(const NodeDesc* nodes, const Tile* tiles, int32_t count, int32_t max_ne, const PointsView& pv,
                  int32_t points, bool single, hipStream_t st);
+/// Kernel choices per front class, process-wide. The CAROLI_* variables give the defaults;
+/// cg_set_tuning and cg_batch_autotune override them (set_tuning_override(nullptr) goes back).
+struct Tuning {
+    int32_t small_fronts = 0;      // 0 front_border_kernel, 1 front_lds_kernel, 2 front_reg_kernel (n <= 64)
+    int32_t split = 1;             // separators up to kF11Lds by f11_kernel + trsm_kernel, else border_kernel
+    int32_t blocked_min_ne = 128;  // separators above this many variables by the blocked kernels
+    int32_t unblocked = 0;         // every large front by factor_global_kernel
+    /// The fields that decide a plan's kernel lists (small_fronts is read at each launch).
+    bool same_plan(const Tuning& o) const {
+        return split == o.split && blocked_min_ne == o.blocked_min_ne && unblocked == o.unblocked;
+    }
+};
+Tuning current_tuning();
+void set_tuning_override(const Tuning* t);
 /// Small-front kernel: 0 border-only LDS (front_border_kernel, the default), 1 the whole front in
 /// LDS (CAROLI_SMALL_FRONTS=lds), 2 in registers for fronts up to 64 (CAROLI_SMALL_FRONTS=reg).
 int small_front_mode();

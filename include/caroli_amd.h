@@ -149,6 +149,28 @@ int32_t cg_batch_create(const cg_system* const* systems, const cg_sigma* const* 
 int32_t cg_batch_run(cg_batch* batch, double* out_t, int32_t* out_status, cg_batch_stats* stats);
 void cg_batch_destroy(cg_batch* batch);
 
+/* Kernel choices per front class (no counterpart in the reference, which has one CPU path per
+ * front; the choices mirror its split between small_schur, nested.rs:1167, and Field::schur,
+ * field.rs:79, at GPU granularity). Defaults: 0, 1, 128, 0; the environment variables
+ * CAROLI_SMALL_FRONTS, CAROLI_FRONT_KERNEL and CAROLI_BLOCKED_MIN_NE change them. */
+typedef struct cg_tuning {
+  int32_t small_fronts;   /* fronts up to 64 (c128) / 90 (c64) variables: 0 the border in LDS,
+                             1 the whole front in LDS, 2 in registers (n <= 64) */
+  int32_t split;          /* 1: separators up to 64 by an LDS LU and register triangular solves
+                             spread over many workgroups; 0: one workgroup per front */
+  int32_t blocked_min_ne; /* separators above this many variables by the blocked kernels */
+  int32_t unblocked;      /* 1: every large front by rank-1 updates in HBM */
+} cg_tuning;
+/* The tuning in force, or set one (NULL: back to the defaults and the variables). Plans built
+ * afterwards follow it; a system planned under another is re-planned at its next use. */
+int32_t cg_get_tuning(cg_tuning* out);
+int32_t cg_set_tuning(const cg_tuning* t);
+/* Autotuner (no counterpart in the reference): times whole eager sweeps of the batch's points
+ * under candidate tunings, by coordinate descent over the three choices from the one in force (a
+ * candidate must be 2 % faster to win), and leaves the fastest in force with the batch re-planned
+ * under it. out: the choice; out_seconds: its sweep time (either may be NULL). */
+int32_t cg_batch_autotune(cg_batch* batch, cg_tuning* out, double* out_seconds);
+
 /* Per-kernel timing with HIP events on the engine's streams (enable = 1 resets the counters).
  * Each record: kernel family name, launches, total seconds, algorithmic flops and bytes. While
  * timing is on, the launch units of a batch run one after another instead of two at a time, so

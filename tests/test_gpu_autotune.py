@@ -1,0 +1,71 @@
+"""The autotuner and the kernel choices, through the C ABI (cg_batch_autotune, cg_set_tuning).
+
+cg_batch_autotune times eager sweeps of a resident batch under candidate kernel choices and leaves
+the fastest in force; whatever it picks, the batch's results stay those of the oracle to 1e-10,
+also on the graph replays after it. A choice set by hand that classifies fronts differently
+re-plans systems that were planned before, and their results stay the oracle's too.
+"""
+import numpy as np
+import pytest
+
+import caroli_amd as ca
+from helpers import TOL, oracle_of, say
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture
+def restore_tuning():
+    yield
+    ca.set_tuning(None)
+
+
+def oracle_ts(systems, sigmas):
+    ref = []
+    for s, g in zip(systems, sigmas):
+        t, rc = oracle_of(s).transmission_with(g.energy(), [g.block(b)[:2] for b in range(s.block_count())])
+        assert rc == 0
+        ref.append(t)
+    return np.array(ref)
+
+
+def test_autotuned_batch_keeps_parity(restore_tuning):
+    # BHZ 60 x 24 (separators up to 96 orbitals: the f11 / trsm, border and blocked choices all
+    # apply) in two configurations, and a cubic 16 x 8 x 8 bar: an uneven batch.
+    bhz = [ca.NestedTwoTerminal.preset(ca.Model.BHZ, [60, 24, 0], disorder=2, strength=0.4, concentration=0.2,
+                                       seed=5 + c) for c in range(2)]
+    cubic = ca.NestedTwoTerminal.preset(ca.Model.Cubic, [16, 8, 8], disorder=1, strength=16.5, seed=1001)
+    pts_s, pts_g = [], []
+    for e in (-0.1, 0.03, 0.11):
+        g = bhz[0].self_energies(e)
+        for s in bhz:
+            pts_s.append(s)
+            pts_g.append(g)
+    for e in (-1.0, 1.0 / 3.0):
+        pts_s.append(cubic)
+        pts_g.append(cubic.self_energies(e))
+    ref = oracle_ts(pts_s, pts_g)
+    batch = ca.ResidentBatch(pts_s, pts_g, ca.Precision.Double)
+    say("[gpu] autotune: BHZ 60x24 x 6 points + cubic 16x8x8 x 2 points, before tuning")
+    t, st = batch.run()
+    assert (st == 0).all(), st
+    assert np.abs(t - ref).max() <= TOL * (1 + np.abs(ref).max())
+    say("[gpu] autotune: tuning")
+    choice, sec = batch.autotune()
+    say(f"    choice {choice}, {sec * 1e3:.2f} ms per sweep")
+    assert sec > 0 and ca.get_tuning() == choice
+    for run in range(3):  # eager, captured, replayed
+        say(f"[gpu] autotune: tuned batch, run {run}")
+        t, st = batch.run()
+        assert (st == 0).all(), st
+        assert np.abs(t - ref).max() <= TOL * (1 + np.abs(ref).max()), (run, np.abs(t - ref).max())
+    # A hand-set choice that classifies the separators differently: the systems planned above are
+    # re-planned at their next use.
+    forced = {"small_fronts": 1, "split": 0, "blocked_min_ne": 64, "unblocked": 0}
+    if forced["split"] == choice["split"] and forced["blocked_min_ne"] == choice["blocked_min_ne"]:
+        forced["blocked_min_ne"] = 1 << 30
+    ca.set_tuning(forced)
+    say(f"[gpu] autotune: forced {forced}")
+    t2, st = ca.batch_transmission(pts_s, pts_g, ca.Precision.Double)
+    assert (st == 0).all(), st
+    assert np.abs(t2 - ref).max() <= TOL * (1 + np.abs(ref).max()), np.abs(t2 - ref).max()
