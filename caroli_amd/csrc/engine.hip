@@ -1326,7 +1326,9 @@ int32_t create_batch(const cg_system* const* systems, const cg_sigma* const* sig
             const size_t per_point = size_t(hp.arena_stride) * es + size_t(hp.root_stride) * sizeof(double2) +
                                      size_t(hp.piv_stride) * 4 + size_t(hp.slots) * 4 + 64;
             const size_t budget = size_t(96) << 30;
-            const size_t even = std::max<size_t>(kMinUnitPoints, (idx.size() + splits - 1) / splits);
+            // As many parts as keep kMinUnitPoints each (33 points stay one unit, not 32 + 1).
+            const size_t parts = std::max<size_t>(1, std::min(splits, idx.size() / kMinUnitPoints));
+            const size_t even = (idx.size() + parts - 1) / parts;
             size_t chunk = std::max<size_t>(1, std::min<size_t>({budget / per_point, 65535, idx.size(), even}));
             for (size_t c0 = 0; c0 < idx.size(); c0 += chunk) {
                 size_t P = std::min(chunk, idx.size() - c0);
