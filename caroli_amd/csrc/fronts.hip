@@ -1154,8 +1154,13 @@ __global__ void __launch_bounds__(256)
     if (pv.status[p * pv.status_stride + nd.slot] != kFrontOk) return;
     T* F = reinterpret_cast<T*>(pv.arena) + p * pv.arena_stride + nd.off;
     const int32_t* piv = pv.piv + p * pv.piv_stride + nd.piv_off;
-    const int n = nd.n, ne = nd.ne, nb = n - ne, tid = threadIdx.x, la = ne + 1;
-    T* A = reinterpret_cast<T*>(lds_raw);  // L11 (kind 0) or U11 (kind 1), leading dimension la
+    const int n = nd.n, ne = nd.ne, nb = n - ne, tid = threadIdx.x;
+    // The triangle the solve reads, packed (ne (ne + 1) / 2 words, half the square): kind 0 the
+    // strict lower L11 by columns, L[i][k] at lo(k) + i - k - 1 (i > k); kind 1 U11 by rows,
+    // U[k][j] at up(k) + j - k (j >= k).
+    T* A = reinterpret_cast<T*>(lds_raw);
+    auto lo = [&](int k) { return k * ne - k * (k + 1) / 2; };
+    auto up = [&](int k) { return k * ne - k * (k - 1) / 2; };
     const bool upper = tl.pad != 0;
     if (!upper && tid < 64) {  // the rows of P F12: row i is the front's row src[i] (ne <= 64)
         // Lane i holds src[i]; LAPACK's interchanges applied in order, two readlanes each.
@@ -1168,10 +1173,9 @@ __global__ void __launch_bounds__(256)
         }
         if (tid < ne) src[tid] = v;
     }
-    // Only the triangle the solve reads: the strict lower L11 (kind 0) or U11 with its diagonal.
     for (int idx = tid; idx < ne * ne; idx += 256) {
         const int i = idx % ne, j = idx / ne;
-        if (upper ? i <= j : i > j) A[i + j * la] = F[i + int64_t(j) * n];
+        if (upper ? i <= j : i > j) A[upper ? up(i) + j - i : lo(j) + i - j - 1] = F[i + int64_t(j) * n];
     }
     __syncthreads();
     // Four lanes per column of U12 (kind 0) or row of L21 (kind 1), the workgroup's 64 of them in
@@ -1196,10 +1200,11 @@ __global__ void __launch_bounds__(256)
                     T xk;
                     xk.x = __shfl(x[kb].x, kk, 4);
                     xk.y = __shfl(x[kb].y, kk, 4);
+                    const int ck = lo(k) - k - 1;
 #pragma unroll
                     for (int s = kb; s < NS; ++s) {
                         const int i = g + 4 * s;
-                        if (i > k && i < ne) x[s] = csub(x[s], cmul(A[i + k * la], xk));
+                        if (i > k && i < ne) x[s] = csub(x[s], cmul(A[ck + i], xk));
                     }
                 }
             }
@@ -1210,7 +1215,8 @@ __global__ void __launch_bounds__(256)
             for (int kk = 0; kk < 4; ++kk) {
                 const int k = 4 * kb + kk;
                 if (k < ne) {
-                    const T cand = cmul(x[kb], crecip(A[k + k * la]));
+                    const int rk = up(k) - k;
+                    const T cand = cmul(x[kb], crecip(A[rk + k]));
                     T xk;
                     xk.x = __shfl(cand.x, kk, 4);
                     xk.y = __shfl(cand.y, kk, 4);
@@ -1218,7 +1224,7 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
                     for (int s = kb; s < NS; ++s) {
                         const int j = g + 4 * s;
-                        if (j > k && j < ne) x[s] = csub(x[s], cmul(xk, A[k + j * la]));
+                        if (j > k && j < ne) x[s] = csub(x[s], cmul(xk, A[rk + j]));
                     }
                 }
             }
@@ -2277,7 +2283,7 @@ void launch_trsm(const NodeDesc* nodes, const Tile* tiles, int32_t count, int32_
                  int32_t points, bool single, hipStream_t st) {
     if (count <= 0 || points <= 0) return;
     if (max_ne > kF11Lds) throw std::invalid_argument("separator too large for trsm_kernel");
-    const size_t lds = size_t(max_ne) * size_t(max_ne + 1) * (single ? sizeof(float2) : sizeof(double2));
+    const size_t lds = size_t(max_ne) * size_t(max_ne + 1) / 2 * (single ? sizeof(float2) : sizeof(double2));
     auto go = [&](auto kernel) {
         allow_lds(reinterpret_cast<const void*>(kernel));
         hipLaunchKernelGGL(kernel, grid2(count, points), dim3(256), lds, st, nodes, tiles, pv);
