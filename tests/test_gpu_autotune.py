@@ -69,3 +69,26 @@ def test_autotuned_batch_keeps_parity(restore_tuning):
     t2, st = ca.batch_transmission(pts_s, pts_g, ca.Precision.Double)
     assert (st == 0).all(), st
     assert np.abs(t2 - ref).max() <= TOL * (1 + np.abs(ref).max()), np.abs(t2 - ref).max()
+
+
+def test_autotune_mixed_precision_batch(restore_tuning):
+    """Tuning a refined mixed-precision batch: every candidate run repeats the refinement; the
+    tuned batch still meets 1e-8 against the oracle on every point."""
+    systems = [ca.NestedTwoTerminal.preset(ca.Model.BHZ, [40, 20, 0], disorder=2, strength=0.4, concentration=0.2,
+                                           seed=11 + c) for c in range(2)]
+    pts_s, pts_g = [], []
+    for e in (-0.08, 0.05):
+        g = systems[0].self_energies(e)
+        for s in systems:
+            pts_s.append(s)
+            pts_g.append(g)
+    ref = oracle_ts(pts_s, pts_g)
+    batch = ca.ResidentBatch(pts_s, pts_g, ca.Precision.Mixed)
+    say("[gpu] autotune: mixed BHZ 40x20 x 4 points, tuning")
+    choice, sec = batch.autotune()
+    say(f"    choice {choice}, {sec * 1e3:.2f} ms per refined run")
+    stats = ca.capi.BatchStats()
+    t, st = batch.run(stats)
+    assert (st == 0).all(), st
+    assert np.abs(t - ref).max() <= 1e-8, np.abs(t - ref).max()
+    assert stats.refinement_iters > 0
