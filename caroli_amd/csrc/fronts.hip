@@ -1764,20 +1764,39 @@ __global__ void __launch_bounds__(256)
             cr[x][y] = typename M::acc{0, 0, 0, 0};
             ci[x][y] = typename M::acc{0, 0, 0, 0};
         }
-    for (int kc = 0; kc < w; kc += kChunk) {
-        for (int e = tid; e < kTile * kChunk; e += 256) {
-            const int r = e % kTile, c = e / kTile, i = tl.i0 + r, k = kc + c;
-            T v = F[min(i, n - 1) + int64_t(k0 + min(k, w - 1)) * n];
-            if (!(i < n && k < w)) v = zero;
-            As[c * kTileLd + r] = v;
+    // The K loop in chunks of kChunk, as in schur_kernel: the next chunk's L / U entries are loaded
+    // into registers (kStage per thread and operand) while the MFMAs run on the current one from
+    // LDS. Addresses are clamped into the front; entries outside it are replaced by zero.
+    constexpr int kStage = kTile * kChunk / 256;
+    T sa[kStage], sb[kStage];
+    auto fetch = [&](int kc) {
+#pragma unroll
+        for (int u = 0; u < kStage; ++u) {
+            const int e = tid + u * 256;
+            {
+                const int r = e % kTile, c = e / kTile, i = tl.i0 + r, k = kc + c;
+                T v = F[min(i, n - 1) + int64_t(k0 + min(k, w - 1)) * n];
+                if (!(i < n && k < w)) v = zero;
+                sa[u] = v;
+            }
+            {
+                const int c = e % kChunk, r = e / kChunk, j = tl.j0 + r, k = kc + c;
+                T v = F[(k0 + min(k, w - 1)) + int64_t(min(j, n - 1)) * n];
+                if (!(j < n && k < w)) v = zero;
+                sb[u] = v;
+            }
         }
-        for (int e = tid; e < kTile * kChunk; e += 256) {
-            const int c = e % kChunk, r = e / kChunk, j = tl.j0 + r, k = kc + c;
-            T v = F[(k0 + min(k, w - 1)) + int64_t(min(j, n - 1)) * n];
-            if (!(j < n && k < w)) v = zero;
-            Bs[c * kTileLd + r] = v;
+    };
+    fetch(0);
+    for (int kc = 0; kc < w; kc += kChunk) {
+#pragma unroll
+        for (int u = 0; u < kStage; ++u) {
+            const int e = tid + u * 256;
+            As[(e / kTile) * kTileLd + e % kTile] = sa[u];
+            Bs[(e % kChunk) * kTileLd + e / kChunk] = sb[u];
         }
         __syncthreads();
+        if (kc + kChunk < w) fetch(kc + kChunk);
 #pragma unroll
         for (int kk = 0; kk < kChunk; kk += 4) {
             const int k = kk + lane / 16;
