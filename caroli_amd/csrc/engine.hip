@@ -248,6 +248,8 @@ LevelSet classify(const std::vector<NodeDesc>& nodes, const std::vector<KidDesc>
         if (!root) {
             add_gather_tiles(L.gtiles, q, 0, d.n, 0, d.ne);     // separator columns, all rows
             add_gather_tiles(L.gtiles, q, 0, d.ne, d.ne, d.n);  // separator rows, boundary columns
+        } else {
+            add_gather_tiles(L.gtiles, q, 0, d.n, 0, d.n);  // the whole root (root_gather_kernel)
         }
     }
     L.plan_blocked(nodes);
@@ -450,6 +452,7 @@ HostPlan build_plan(const Block& blk, const SparseMatrix& h) {
         hp.kids.push_back(kd);
     }
     r.kid_end = int32_t(hp.kids.size());
+    add_inverse_maps(hp.kids, r.kid_beg, r.kid_end, hp.maps, r.n);
     r.slot = nn;
     r.piv_off = piv;
     r.var_off = int64_t(hp.vars.size());
@@ -769,7 +772,8 @@ void run_level(Engine& eng, const LevelSet& L, const DevLevel& D, const NodeDesc
     if (root || unblocked) {  // the whole front assembled in HBM
         eng.timed(root ? kRootAssemble : kAssemble, 0.0, P * fes * L.asm_words, [&] {
             if (root)
-                launch_assemble_root(nodes, D.large, count, ent, diag, kids, maps, pv, P, single ? 1 : 0, eng.lane->stream);
+                launch_assemble_root(nodes, D.large, count, ent, diag, kids, maps, D.gtiles, int32_t(L.gtiles.size()), pv,
+                                     P, single ? 1 : 0, eng.lane->stream);
             else if (single)
                 launch_assemble_c64(nodes, D.large, count, ent, diag, kids, maps, pv, P, eng.lane->stream);
             else
@@ -905,7 +909,9 @@ int32_t replanned_point(Engine& eng, const cg_system* s, const cg_sigma* g, int 
                 kids.push_back(kd);
             }
             d.kid_end = int32_t(kids.size());
-            add_inverse_maps(kids, d.kid_beg, d.kid_end, maps, int(pos.v.size()));
+            // n entries each: for the root that is past the delayed and kept variables of `pos`
+            // (its border rows and columns, which no child reaches).
+            add_inverse_maps(kids, d.kid_beg, d.kid_end, maps, d.n);
         };
         if (!root_level) {
             for (int k = 0; k < nn; ++k) {

@@ -184,7 +184,9 @@ __device__ __forceinline__ void extend_add_global(TF* F, const TK* S, int ld, co
 }
 
 // Assembly of one front (see the file comment). TF: type of the front, TK: type of the children.
-template <class TF, class TK, bool kRoot>
+// kFill = false: the front was written whole by root_gather_kernel (children, -Sigma, unit
+// borders), and only -H and z are added here.
+template <class TF, class TK, bool kRoot, bool kFill = true>
 __global__ void __launch_bounds__(kThreads)
     assemble_kernel(const NodeDesc* __restrict__ nodes, const int32_t* __restrict__ level,
                     const Entry* __restrict__ ent, const int32_t* __restrict__ diag, const KidDesc* __restrict__ kids,
@@ -198,8 +200,10 @@ __global__ void __launch_bounds__(kThreads)
     const int n = nd.n, tid = threadIdx.x;
     const int64_t n2 = int64_t(n) * n;
     const TF zero = cx<TF>(0.0, 0.0);
-    for (int64_t i = tid; i < n2; i += kThreads) F[i] = zero;
-    __syncthreads();
+    if (kFill) {
+        for (int64_t i = tid; i < n2; i += kThreads) F[i] = zero;
+        __syncthreads();
+    }
     const double2* vals = pv.vals[p];
     for (int e = nd.ent_beg + tid; e < nd.ent_end; e += kThreads) {
         Entry en = ent[e];
@@ -215,7 +219,7 @@ __global__ void __launch_bounds__(kThreads)
         F[q + q * n] = cadd(F[q + q * n], z);
     }
     __syncthreads();
-    if (kRoot) {
+    if (kRoot && kFill) {
         const double2* sg = pv.sigma + p * pv.sigma_stride;
         const int nl = nd.nl, nr = nd.nr, b0 = nd.nd, r0 = nd.nd + nd.right_at, k0 = nd.ne;
         for (int idx = tid; idx < nl * nl; idx += kThreads) {
@@ -234,10 +238,73 @@ __global__ void __launch_bounds__(kThreads)
         for (int i = tid; i < nl; i += kThreads) F[(k0 + i) + int64_t(b0 + i) * n] = one;
         __syncthreads();
     }
-    for (int c = nd.kid_beg; c < nd.kid_end; ++c) {
-        const KidDesc kd = kids[c];
-        extend_add_global<kThreads>(F, kidbase + kd.src, kd.ld, maps + kd.map_beg, kd.size, n);
-        __syncthreads();
+    if (kFill)
+        for (int c = nd.kid_beg; c < nd.kid_end; ++c) {
+            const KidDesc kd = kids[c];
+            extend_add_global<kThreads>(F, kidbase + kd.src, kd.ld, maps + kd.map_beg, kd.size, n);
+            __syncthreads();
+        }
+}
+
+// The root front written whole, one workgroup per tile (Tile::pad: rows << 16 | cols) and point,
+// every entry once: the children's Schur complements through their inverse maps, minus Sigma_L on
+// the left slice block and Sigma_R on the right one, and the unit borders (rows r0 + t of columns
+// k0 + t, rows k0 + t of columns b0 + t), which no child reaches (children map into the first
+// nd + k positions). -H and z follow in assemble_kernel<kFill = false>. A root of 4,800 variables
+// (cubic L = 40) spans 5,625 tiles instead of one workgroup per point.
+template <class TK>
+__global__ void __launch_bounds__(256)
+    root_gather_kernel(const NodeDesc* __restrict__ nodes, const Tile* __restrict__ tiles,
+                       const KidDesc* __restrict__ kids, const int32_t* __restrict__ maps, PointsView pv) {
+    const TilePoint tp = xcd_tile();
+    const Tile tl = tiles[tp.tile];
+    const NodeDesc nd = nodes[tl.node];
+    const int64_t p = tp.point;
+    const TK* kidbase = reinterpret_cast<const TK*>(pv.arena) + p * pv.arena_stride;
+    double2* F = reinterpret_cast<double2*>(pv.root_arena) + p * pv.root_stride + nd.off;
+    const double2* sg = pv.sigma + p * pv.sigma_stride;
+    const int n = nd.n, nl = nd.nl, nr = nd.nr, b0 = nd.nd, r0 = nd.nd + nd.right_at, k0 = nd.ne;
+    const int rows = min(tl.pad >> 16, n - tl.i0), cols = min(tl.pad & 0xffff, n - tl.j0), cnt = rows * cols;
+    for (int base = int(threadIdx.x); base < cnt; base += 256 * kBatch) {
+        int ii[kBatch], jj[kBatch];
+#pragma unroll
+        for (int u = 0; u < kBatch; ++u) {
+            const int idx = min(base + u * 256, cnt - 1);
+            ii[u] = tl.i0 + idx % rows;
+            jj[u] = tl.j0 + idx / rows;
+        }
+        double2 acc[kBatch];
+#pragma unroll
+        for (int u = 0; u < kBatch; ++u) acc[u] = make_double2(0.0, 0.0);
+        for (int c = nd.kid_beg; c < nd.kid_end; ++c) {  // per child: all map loads, then all values
+            const KidDesc kd = kids[c];
+            const int32_t* inv = maps + kd.inv_beg;
+            const TK* S = kidbase + kd.src;
+            int a[kBatch], b[kBatch];
+#pragma unroll
+            for (int u = 0; u < kBatch; ++u) {
+                a[u] = inv[ii[u]];
+                b[u] = inv[jj[u]];
+            }
+#pragma unroll
+            for (int u = 0; u < kBatch; ++u) {
+                double2 v = convert<double2>(S[max(a[u], 0) + int64_t(max(b[u], 0)) * kd.ld]);
+                if (a[u] < 0 || b[u] < 0) v = make_double2(0.0, 0.0);
+                acc[u] = cadd(acc[u], v);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kBatch; ++u) {
+            const int i = ii[u], j = jj[u];
+            double2 f = acc[u];
+            if (unsigned(i - b0) < unsigned(nl) && unsigned(j - b0) < unsigned(nl))
+                f = csub(f, sg[(i - b0) + int64_t(j - b0) * nl]);
+            if (unsigned(i - r0) < unsigned(nr) && unsigned(j - r0) < unsigned(nr))
+                f = csub(f, sg[int64_t(nl) * nl + (i - r0) + int64_t(j - r0) * nr]);
+            if (unsigned(j - k0) < unsigned(nr) && i == r0 + (j - k0)) f = make_double2(1.0, 0.0);
+            if (unsigned(i - k0) < unsigned(nl) && j == b0 + (i - k0)) f = make_double2(1.0, 0.0);
+            if (base + u * 256 < cnt) F[i + int64_t(j) * n] = f;
+        }
     }
 }
 
@@ -2091,9 +2158,23 @@ void launch_assemble_c64(const NodeDesc* nodes, const int32_t* level, int32_t co
 }
 
 void launch_assemble_root(const NodeDesc* nodes, const int32_t* level, int32_t count, const Entry* ent,
-                          const int32_t* diag, const KidDesc* kids, const int32_t* maps, const PointsView& pv,
-                          int32_t points, int single, hipStream_t st) {
+                          const int32_t* diag, const KidDesc* kids, const int32_t* maps, const Tile* tiles,
+                          int32_t tile_count, const PointsView& pv, int32_t points, int single, hipStream_t st) {
     if (count <= 0 || points <= 0) return;
+    if (tile_count > 0) {  // the whole front by tiles, then -H and z
+        if (single) {
+            hipLaunchKernelGGL(root_gather_kernel<float2>, grid2(tile_count, points), dim3(256), 0, st, nodes, tiles, kids,
+                               maps, pv);
+            hipLaunchKernelGGL((assemble_kernel<double2, float2, true, false>), grid2(count, points), dim3(kThreads), 0,
+                               st, nodes, level, ent, diag, kids, maps, pv);
+        } else {
+            hipLaunchKernelGGL(root_gather_kernel<double2>, grid2(tile_count, points), dim3(256), 0, st, nodes, tiles,
+                               kids, maps, pv);
+            hipLaunchKernelGGL((assemble_kernel<double2, double2, true, false>), grid2(count, points), dim3(kThreads), 0,
+                               st, nodes, level, ent, diag, kids, maps, pv);
+        }
+        return;
+    }
     if (single)
         hipLaunchKernelGGL((assemble_kernel<double2, float2, true>), grid2(count, points), dim3(kThreads), 0, st, nodes,
                            level, ent, diag, kids, maps, pv);

@@ -105,10 +105,12 @@ void launch_assemble_c128(const NodeDesc* nodes, const int32_t* level, int32_t c
 void launch_assemble_c64(const NodeDesc* nodes, const int32_t* level, int32_t count, const Entry* ent,
                          const int32_t* diag, const KidDesc* kids, const int32_t* maps, const PointsView& pv,
                          int32_t points, hipStream_t st);
-/// Root assembly (complex f64 root, children in the front precision: c64 when single != 0).
+/// Root assembly (complex f64 root, children in the front precision: c64 when single != 0). With
+/// tile_count > 0 the root is written by tiles (root_gather_kernel: children, Sigma, unit borders)
+/// and then gets -H and z; else one workgroup per root and point does all of it.
 void launch_assemble_root(const NodeDesc* nodes, const int32_t* level, int32_t count, const Entry* ent,
-                          const int32_t* diag, const KidDesc* kids, const int32_t* maps, const PointsView& pv,
-                          int32_t points, int single, hipStream_t st);
+                          const int32_t* diag, const KidDesc* kids, const int32_t* maps, const Tile* tiles,
+                          int32_t tile_count, const PointsView& pv, int32_t points, int single, hipStream_t st);
 /// Small fronts (n <= nmax <= kLdsMaxN): assembly (or a copy of the given front when !assemble)
 /// and elimination in LDS. border_words: the largest (n + 1) ne + (ne + 1) (n - ne) of the fronts
 /// (LDS words of the border-only kernel, the default).
