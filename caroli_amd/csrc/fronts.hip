@@ -1367,6 +1367,13 @@ __global__ void __launch_bounds__(256)
             }
         }
     };
+    // The wave's 16 x 16 blocks that hold entries of S (a block past nb in rows or columns is
+    // padding of a partial tile and is skipped; wave-uniform).
+    bool live[2][2];
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y) live[x][y] = tl.j0 + wj + 16 * x < nb && tl.i0 + wi + 16 * y < nb;
     fetch(0);
     for (int kc = 0; kc < ne; kc += kChunk) {
 #pragma unroll
@@ -1379,6 +1386,7 @@ __global__ void __launch_bounds__(256)
         if (kc + kChunk < ne) fetch(kc + kChunk);
 #pragma unroll
         for (int kk = 0; kk < kChunk; kk += 4) {
+            if (kc + kk >= ne) break;  // the rest of the chunk is padding (zeros)
             const int k = kk + lane / 16;
             T a[2], b[2];
 #pragma unroll
@@ -1389,12 +1397,13 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
             for (int x = 0; x < 2; ++x)
 #pragma unroll
-                for (int y = 0; y < 2; ++y) {
-                    cr[x][y] = M::mac(a[x].x, b[y].x, cr[x][y]);
-                    cr[x][y] = M::mac(-a[x].y, b[y].y, cr[x][y]);
-                    ci[x][y] = M::mac(a[x].x, b[y].y, ci[x][y]);
-                    ci[x][y] = M::mac(a[x].y, b[y].x, ci[x][y]);
-                }
+                for (int y = 0; y < 2; ++y)
+                    if (live[x][y]) {
+                        cr[x][y] = M::mac(a[x].x, b[y].x, cr[x][y]);
+                        cr[x][y] = M::mac(-a[x].y, b[y].y, cr[x][y]);
+                        ci[x][y] = M::mac(a[x].x, b[y].y, ci[x][y]);
+                        ci[x][y] = M::mac(a[x].y, b[y].x, ci[x][y]);
+                    }
         }
         __syncthreads();
     }
