@@ -1863,6 +1863,16 @@ __global__ void __launch_bounds__(256)
             }
         }
     };
+    // The wave's 16 x 16 blocks with entries to update: inside the front and not wholly in F22
+    // (rows and columns >= ne, left to schur_kernel); the others are skipped (wave-uniform).
+    bool live[2][2];
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y) {
+            const int ib = tl.i0 + wi + 16 * y, jb = tl.j0 + wj + 16 * x;
+            live[x][y] = ib < n && jb < n && (ib < ne || jb < ne);
+        }
     fetch(0);
     for (int kc = 0; kc < w; kc += kChunk) {
 #pragma unroll
@@ -1875,6 +1885,7 @@ __global__ void __launch_bounds__(256)
         if (kc + kChunk < w) fetch(kc + kChunk);
 #pragma unroll
         for (int kk = 0; kk < kChunk; kk += 4) {
+            if (kc + kk >= w) break;  // the rest of the chunk is padding (zeros)
             const int k = kk + lane / 16;
             T a[2], b[2];
 #pragma unroll
@@ -1885,12 +1896,13 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
             for (int x = 0; x < 2; ++x)
 #pragma unroll
-                for (int y = 0; y < 2; ++y) {
-                    cr[x][y] = M::mac(a[x].x, b[y].x, cr[x][y]);
-                    cr[x][y] = M::mac(-a[x].y, b[y].y, cr[x][y]);
-                    ci[x][y] = M::mac(a[x].x, b[y].y, ci[x][y]);
-                    ci[x][y] = M::mac(a[x].y, b[y].x, ci[x][y]);
-                }
+                for (int y = 0; y < 2; ++y)
+                    if (live[x][y]) {
+                        cr[x][y] = M::mac(a[x].x, b[y].x, cr[x][y]);
+                        cr[x][y] = M::mac(-a[x].y, b[y].y, cr[x][y]);
+                        ci[x][y] = M::mac(a[x].x, b[y].y, ci[x][y]);
+                        ci[x][y] = M::mac(a[x].y, b[y].x, ci[x][y]);
+                    }
         }
         __syncthreads();
     }
