@@ -864,8 +864,8 @@ __device__ __forceinline__ void mfma_gemm_tile(T* F, int n, int i0, int j0, int 
     typename M::acc cr = {0, 0, 0, 0}, ci = {0, 0, 0, 0};
     const int li = lane % 16, lk = lane / 16;
     const int64_t ii = min(i0 + li, n - 1), jj0 = int64_t(min(j0 + li, n - 1)) * n;
-    for (int k = 0; k < kend; k += 16) {
-        T a[4], b[4];
+    // Two register sets: the next 16 values of K load while the MFMAs run on the current ones.
+    auto load = [&](int k, T (&a)[4], T (&b)[4]) {
 #pragma unroll
         for (int s = 0; s < 4; ++s) {
             const int kk = k + 4 * s + lk, kc = min(kk, kend - 1);
@@ -876,6 +876,8 @@ __device__ __forceinline__ void mfma_gemm_tile(T* F, int n, int i0, int j0, int 
                 b[s] = cx<T>(0.0, 0.0);
             }
         }
+    };
+    auto mma = [&](const T (&a)[4], const T (&b)[4]) {
 #pragma unroll
         for (int s = 0; s < 4; ++s) {
             cr = M::mac(a[s].x, b[s].x, cr);
@@ -883,6 +885,15 @@ __device__ __forceinline__ void mfma_gemm_tile(T* F, int n, int i0, int j0, int 
             ci = M::mac(a[s].x, b[s].y, ci);
             ci = M::mac(a[s].y, b[s].x, ci);
         }
+    };
+    T a0[4], b0[4], a1[4], b1[4];
+    if (kend > 0) load(0, a0, b0);
+    for (int k = 0; k < kend; k += 32) {
+        if (k + 16 < kend) load(k + 16, a1, b1);
+        mma(a0, b0);
+        if (k + 16 >= kend) break;
+        if (k + 32 < kend) load(k + 32, a0, b0);
+        mma(a1, b1);
     }
     const int i = i0 + li;
 #pragma unroll
