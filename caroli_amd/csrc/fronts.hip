@@ -2,15 +2,19 @@
 //
 // Every launch covers the fronts of one tree level (or one size class of them) times the points
 // of the batch, so thousands of fronts fill the 256 CUs.
-//  - front_lds_kernel: small fronts (n <= 64 in complex f64, <= 90 in f32) are assembled straight
-//    into LDS (F = z - H on the separator plus the children's Schur complements) and eliminated
-//    there, one wave per front: partial LU with row pivoting restricted to the separator rows and
-//    the pivot test of crates/caroli-dense/src/field.rs:61 / nested.rs:1167-1223.
-//  - large fronts: assemble_kernel writes F to HBM; border_kernel factors the separator block and
-//    the border strips (F11 = P^T L11 U11, L21, U12) panel by panel; schur_kernel forms
-//    S = F22 - L21 U12 on MFMA over 64 x 64 tiles, many workgroups per front.
-//  - the root is a large front bordered by the lead slices (assemble_kernel<root>), whose Schur
-//    block is -G_LR; trace_kernel evaluates Tr[Gamma_L G Gamma_R G^H] from it.
+//  - small fronts (n <= 64 in complex f64, <= 90 in f32): front_border_kernel assembles the border
+//    (F = z - H on the separator plus the children's border entries) into LDS, eliminates it there
+//    with row pivoting restricted to the separator rows and the pivot test of
+//    crates/caroli-dense/src/field.rs:61 / nested.rs:1167-1223, and forms S = F22 - L21 U12 with
+//    F22 gathered from the children; front_lds_kernel and front_reg_kernel are the whole-front
+//    variants the tuning can select.
+//  - large fronts: gather_kernel assembles the border strips in HBM; f11_kernel + trsm_kernel,
+//    border_kernel or the blocked kernels (by separator size and the tuning) factor F11 and form
+//    L21, U12; schur_kernel forms S = F22 - L21 U12 on MFMA over 64 x 64 tiles, many workgroups per
+//    front, with F22 gathered from the children.
+//  - the root is a large front bordered by the lead slices (root_gather_kernel, then
+//    assemble_kernel<root> for -H and z), whose Schur block is -G_LR; the trace kernels evaluate
+//    Tr[Gamma_L G Gamma_R G^H] from it.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
