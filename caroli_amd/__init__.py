@@ -282,6 +282,62 @@ class NestedTwoTerminal:
         _check(capi.cg_backward(self._h, sigma._h, C.byref(t), _dptr(g)))
         return t.value, g
 
+    def scattering_states(self, sigma: LeadSelfEnergies):
+        """psi_p = G W_p on every orbital for the left and right lead (nested.rs:851-895):
+        ScatteringStates with `.leads = [psi_left, psi_right]`, complex n x channels."""
+        n = self.sizes()[0]
+        cols = (C.c_int64 * 2)()
+        _check(capi.cg_scattering_states(self._h, sigma._h, cols, None, None))
+        left = np.zeros((int(cols[0]), n), complex)
+        right = np.zeros((int(cols[1]), n), complex)
+        _check(capi.cg_scattering_states(self._h, sigma._h, cols, _dptr(left.view(np.float64)),
+                                         _dptr(right.view(np.float64))), "scattering states")
+        return ScatteringStates(sigma.energy(), [left.T, right.T])
+
+    def site_densities(self, states, f, op=None):
+        """sum_p f_p psi_p^H O psi_p / 2 pi per site (nested.rs:994-1025), O = 1 without `op`."""
+        left, right, cols = states._packed()
+        out = np.zeros(len(self.arrays()["site_orbitals"]))
+        fa = np.asarray(f, float)
+        if op is None:
+            rc = capi.cg_site_densities(self._h, _dptr(left), _dptr(right), cols, _dptr(fa), None, 0, _dptr(out))
+        else:
+            o = np.asfortranarray(np.asarray(op, complex))
+            rc = capi.cg_site_densities(self._h, _dptr(left), _dptr(right), cols, _dptr(fa),
+                                        _dptr(o.ravel(order="F").view(np.float64)), o.shape[0], _dptr(out))
+        _check(rc, "site densities")
+        return out
+
+    def bond_currents(self, states, f):
+        """2 Re Tr[H_ij G^<_ji] for every coupled pair of sites i < j (nested.rs:930-990):
+        arrays (from, to, current), ordered by (from, to)."""
+        left, right, cols = states._packed()
+        fa = np.asarray(f, float)
+        count = C.c_int64()
+        _check(capi.cg_bond_currents(self._h, _dptr(left), _dptr(right), cols, _dptr(fa), C.byref(count), None, None,
+                                     None))
+        fr = np.zeros(count.value, np.int64)
+        to = np.zeros(count.value, np.int64)
+        cur = np.zeros(count.value)
+        _check(capi.cg_bond_currents(self._h, _dptr(left), _dptr(right), cols, _dptr(fa), C.byref(count),
+                                     fr.ctypes.data_as(C.POINTER(C.c_int64)), to.ctypes.data_as(C.POINTER(C.c_int64)),
+                                     _dptr(cur)), "bond currents")
+        return fr, to, cur
+
+
+class ScatteringStates:
+    """ScatteringStates of nested.rs: the energy and, per lead, the states psi_p (n x channels)."""
+
+    def __init__(self, energy, leads):
+        self.energy = energy
+        self.leads = leads
+
+    def _packed(self):
+        """Both leads as contiguous column-major (re, im) buffers and their channel counts."""
+        bufs = [np.ascontiguousarray(np.asarray(m, complex).T).view(np.float64) for m in self.leads]
+        cols = (C.c_int64 * 2)(*[m.shape[1] for m in self.leads])
+        return bufs[0], bufs[1], cols
+
 
 TwoTerminal = NestedTwoTerminal
 

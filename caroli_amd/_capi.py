@@ -118,3 +118,11 @@ cg_get_tuning = _sig("cg_get_tuning", C.c_int32, P(Tuning))
 cg_set_tuning = _sig("cg_set_tuning", C.c_int32, P(Tuning))
 cg_batch_autotune = _sig("cg_batch_autotune", C.c_int32, C.c_void_p, P(Tuning), P(C.c_double))
 EXPORTS += ["cg_get_tuning", "cg_set_tuning", "cg_batch_autotune"]
+
+cg_scattering_states = _sig("cg_scattering_states", C.c_int32, c_sys, c_sig, P(C.c_int64), P(C.c_double),
+                            P(C.c_double))
+cg_site_densities = _sig("cg_site_densities", C.c_int32, c_sys, P(C.c_double), P(C.c_double), P(C.c_int64),
+                         P(C.c_double), P(C.c_double), C.c_int32, P(C.c_double))
+cg_bond_currents = _sig("cg_bond_currents", C.c_int32, c_sys, P(C.c_double), P(C.c_double), P(C.c_int64),
+                        P(C.c_double), P(C.c_int64), P(C.c_int64), P(C.c_int64), P(C.c_double))
+EXPORTS += ["cg_scattering_states", "cg_site_densities", "cg_bond_currents"]

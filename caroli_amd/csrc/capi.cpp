@@ -3,6 +3,7 @@
 #include <cmath>
 #include <cstring>
 #include <exception>
+#include <map>
 
 #include "../../include/caroli_amd.h"
 #include "engine.hpp"
@@ -530,6 +531,106 @@ int32_t cg_backward(const cg_system* s, const cg_sigma* g, double* out_t, double
     } catch (const std::exception&) {
         return CG_ERR_DEVICE;
     }
+}
+
+int32_t cg_scattering_states(const cg_system* s, const cg_sigma* g, int64_t* cols, double* psi_left,
+                             double* psi_right) {
+    if (!s || !g || !cols || !sigma_matches(*s, *g)) return CG_ERR_INVALID;
+    try {
+        return gpu::scattering_states(s, g, cols, psi_left, psi_right);
+    } catch (const std::exception&) {
+        return CG_ERR_DEVICE;
+    }
+}
+
+namespace {
+/// (psi_p)_{a k}: states of lead p, n orbitals x cols[p] channels, interleaved complex.
+struct States {
+    const double* m[2];
+    const int64_t* cols;
+    int64_t n;
+    cplx at(int p, int64_t a, int64_t k) const {
+        const double* v = m[p] + 2 * (a + k * n);
+        return cplx(v[0], v[1]);
+    }
+    /// G^<_{ba} = i sum_p f_p sum_k psi_bk conj(psi_ak) (nested.rs:936-943).
+    cplx lesser(int64_t b, int64_t a, const double* f) const {
+        cplx acc = 0.0;
+        for (int p = 0; p < 2; ++p) {
+            cplx sp = 0.0;
+            for (int64_t k = 0; k < cols[p]; ++k) sp += at(p, b, k) * std::conj(at(p, a, k));
+            acc += sp * f[p];
+        }
+        return cplx(0.0, 1.0) * acc;
+    }
+};
+}  // namespace
+
+int32_t cg_site_densities(const cg_system* s, const double* psi_left, const double* psi_right, const int64_t* cols,
+                          const double* f, const double* op, int32_t op_dim, double* out) {
+    if (!s || !psi_left || !psi_right || !cols || !f || !out || cols[0] < 0 || cols[1] < 0) return CG_ERR_INVALID;
+    const auto& sp = s->spec;
+    const size_t sites = sp.site_orbitals.size();
+    if (op)
+        for (size_t i = 0; i < sites; ++i)
+            if (sp.site_orbitals[i] != op_dim) return CG_ERR_INVALID;  // the operator acts on a site's orbitals
+    States st{{psi_left, psi_right}, cols, int64_t(sp.h.n)};
+    for (size_t i = 0; i < sites; ++i) {
+        const int64_t o0 = int64_t(sp.site_offsets[i]), ni = sp.site_orbitals[i];
+        cplx acc = 0.0;
+        for (int p = 0; p < 2; ++p)
+            for (int64_t k = 0; k < cols[p]; ++k)
+                for (int64_t a = 0; a < ni; ++a) {
+                    const cplx left = std::conj(st.at(p, o0 + a, k));
+                    if (!op) {
+                        acc += left * st.at(p, o0 + a, k) * f[p];
+                    } else {
+                        cplx ob = 0.0;
+                        for (int64_t b = 0; b < ni; ++b)
+                            ob += cplx(op[2 * (a + b * ni)], op[2 * (a + b * ni) + 1]) * st.at(p, o0 + b, k);
+                        acc += left * ob * f[p];
+                    }
+                }
+        out[i] = acc.real() / (2.0 * M_PI);
+    }
+    return CG_OK;
+}
+
+int32_t cg_bond_currents(const cg_system* s, const double* psi_left, const double* psi_right, const int64_t* cols,
+                         const double* f, int64_t* count, int64_t* from, int64_t* to, double* current) {
+    if (!s || !psi_left || !psi_right || !cols || !f || !count || cols[0] < 0 || cols[1] < 0) return CG_ERR_INVALID;
+    const auto& sp = s->spec;
+    const auto& h = sp.h;
+    std::vector<int64_t> owner(h.n, 0);
+    for (size_t i = 0; i < sp.site_orbitals.size(); ++i)
+        for (int o = 0; o < sp.site_orbitals[i]; ++o) owner[sp.site_offsets[i] + size_t(o)] = int64_t(i);
+    // Each bond (i < j) once, ordered by (i, j): 2 sum_{a in i, b in j} H_ab G^<_ba (nested.rs:949-956).
+    std::map<std::pair<int64_t, int64_t>, cplx> bonds;
+    States st{{psi_left, psi_right}, cols, int64_t(h.n)};
+    const bool fill = from && to && current;
+    for (size_t a = 0; a < h.n; ++a)
+        for (int64_t e = h.row_ptr[a]; e < h.row_ptr[a + 1]; ++e) {
+            const size_t b = h.cols[e];
+            const int64_t i = owner[a], j = owner[b];
+            if (i < j) {
+                cplx& acc = bonds[{i, j}];
+                if (fill) acc += h.vals[e] * st.lesser(int64_t(b), int64_t(a), f) * 2.0;
+            }
+        }
+    if (!fill) {
+        *count = int64_t(bonds.size());
+        return CG_OK;
+    }
+    if (*count < int64_t(bonds.size())) return CG_ERR_INVALID;
+    int64_t q = 0;
+    for (const auto& [ij, acc] : bonds) {
+        from[q] = ij.first;
+        to[q] = ij.second;
+        current[q] = acc.real();
+        ++q;
+    }
+    *count = q;
+    return CG_OK;
 }
 
 int32_t cg_device_available(void) { return gpu::device_available() ? 1 : 0; }

@@ -1815,6 +1815,65 @@ int32_t kernel_stats(cg_kernel_stat* out, int32_t max_records) {
 // slice, by a solve with A^T), t = W_L^H psi_L:  dT/deps_a = 2 Re sum_cd psi_ac conj(t_dc) phi_ad.
 // Both solves reuse the complex f64 factors of one forward sweep: the root solve, then the
 // backward substitution through every front (the right-hand sides live on the lead slices).
+namespace {
+
+/// Block b of one point (s, g) factored on lane 0 with its factors kept, for the solves of the
+/// adjoint and the scattering states. CG_ERR_INVALID when a separator was delayed (the solves
+/// follow the static plan), CG_ERR_SINGULAR for a singular root.
+int32_t factor_point(Engine& eng, const cg_system* s, const cg_sigma* g, size_t b, const HostPlan& hp,
+                     const DevPlan& dp, PointsView& pv) {
+    const Block& blk = s->sym.blocks[b];
+    stage_points(eng, {s}, {g}, int(b), int(blk.left_orbitals.size()), int(blk.right_orbitals.size()), eng.scratch_in, pv);
+    eng.lane->arena.ensure(size_t(hp.arena_stride) * sizeof(double2));
+    eng.lane->root_arena.ensure(size_t(hp.root_stride) * sizeof(double2));
+    eng.lane->piv.ensure(size_t(hp.piv_stride) * 4);
+    eng.lane->status.ensure(size_t(hp.slots) * 4);
+    eng.lane->tout.ensure(sizeof(double));
+    pv.arena = eng.lane->arena.p;
+    pv.arena_stride = hp.arena_stride;
+    pv.root_arena = eng.lane->root_arena.p;
+    pv.root_stride = hp.root_stride;
+    pv.piv = eng.lane->piv.as<int32_t>();
+    pv.piv_stride = hp.piv_stride;
+    pv.status = eng.lane->status.as<int32_t>();
+    pv.status_stride = hp.slots;
+    pv.t_out = eng.lane->tout.as<double>();
+    pv.keep = 1;
+    sweep(eng, hp, dp, pv, 1, false, eng.scratch_in);
+    std::vector<int32_t> st(hp.slots);
+    HIP_CHECK(hipMemcpyAsync(st.data(), eng.lane->status.p, st.size() * 4, hipMemcpyDeviceToHost, eng.lane->stream));
+    HIP_CHECK(hipStreamSynchronize(eng.lane->stream));
+    eng.collect();
+    for (int32_t k = 0; k < hp.root; ++k)
+        if (st[k] != kFrontOk) return CG_ERR_INVALID;
+    if (st[hp.root] != kFrontOk) return CG_ERR_SINGULAR;
+    return CG_OK;
+}
+
+/// x = G E_p W (or G^T E_p W with transpose) with the stored factors of factor_point: W (m x c) on
+/// the slice orbitals `orbs` of the block, x (n_orb x c, complex, column-major) to the host.
+void solve_columns(Engine& eng, const HostPlan& hp, const DevPlan& dp, const PointsView& pv, int64_t n_orb,
+                   const CMat& w, int m, const DevBuf& orbs, bool transpose, std::vector<double2>& host) {
+    const int c = w.cols;
+    std::vector<double2> wh(size_t(m) * c);
+    for (int col = 0; col < c; ++col)
+        for (int a = 0; a < m; ++a) wh[a + size_t(col) * m] = double2{w(a, col).real(), w(a, col).imag()};
+    DevBuf dw, xb;
+    upload(dw, wh, eng.lane->stream);
+    const size_t bytes = size_t(n_orb) * c * sizeof(double2);
+    xb.ensure(bytes);
+    HIP_CHECK(hipMemsetAsync(xb.p, 0, bytes, eng.lane->stream));
+    SolveView xv{xb.as<double2>(), n_orb * c, int32_t(n_orb), c, dp.vars.as<int32_t>()};
+    launch_scatter(xv, orbs.as<int32_t>(), m, dw.as<double2>(), int64_t(m) * c, transpose, 1, eng.lane->stream);
+    solve(eng, hp, dp, pv, xv, 1, false, transpose, true);
+    host.resize(size_t(n_orb) * c);
+    HIP_CHECK(hipMemcpyAsync(host.data(), xb.p, bytes, hipMemcpyDeviceToHost, eng.lane->stream));
+    HIP_CHECK(hipStreamSynchronize(eng.lane->stream));
+    eng.collect();
+}
+
+}  // namespace
+
 int32_t backward(const cg_system* s, const cg_sigma* g, double* out_t, double* out_grad) {
     Engine& eng = Engine::get();
     if (!eng.ok) return CG_ERR_DEVICE;
@@ -1826,59 +1885,18 @@ int32_t backward(const cg_system* s, const cg_sigma* g, double* out_t, double* o
     for (size_t b = 0; b < sp->host.size(); ++b) {
         const HostPlan& hp = sp->host[b];
         const DevPlan& dp = sp->dev[b];
-        const Block& blk = s->sym.blocks[b];
         const auto& sb = g->blocks[b];
         PointsView pv{};
-        stage_points(eng, {s}, {g}, int(b), int(blk.left_orbitals.size()), int(blk.right_orbitals.size()), eng.scratch_in, pv);
-        eng.lane->arena.ensure(size_t(hp.arena_stride) * sizeof(double2));
-        eng.lane->root_arena.ensure(size_t(hp.root_stride) * sizeof(double2));
-        eng.lane->piv.ensure(size_t(hp.piv_stride) * 4);
-        eng.lane->status.ensure(size_t(hp.slots) * 4);
-        eng.lane->tout.ensure(sizeof(double));
-        pv.arena = eng.lane->arena.p;
-        pv.arena_stride = hp.arena_stride;
-        pv.root_arena = eng.lane->root_arena.p;
-        pv.root_stride = hp.root_stride;
-        pv.piv = eng.lane->piv.as<int32_t>();
-        pv.piv_stride = hp.piv_stride;
-        pv.status = eng.lane->status.as<int32_t>();
-        pv.status_stride = hp.slots;
-        pv.t_out = eng.lane->tout.as<double>();
-        pv.keep = 1;  // the adjoint solves with the factors
-        sweep(eng, hp, dp, pv, 1, false, eng.scratch_in);
-        std::vector<int32_t> st(hp.slots);
-        HIP_CHECK(hipMemcpyAsync(st.data(), eng.lane->status.p, st.size() * 4, hipMemcpyDeviceToHost, eng.lane->stream));
-        HIP_CHECK(hipStreamSynchronize(eng.lane->stream));
-        eng.collect();
-        for (int32_t k = 0; k < hp.root; ++k)
-            if (st[k] != kFrontOk) return CG_ERR_INVALID;  // delayed separators: not supported by the adjoint
-        if (st[hp.root] != kFrontOk) return CG_ERR_SINGULAR;
+        const int32_t rc = factor_point(eng, s, g, b, hp, dp, pv);
+        if (rc != CG_OK) return rc;  // delayed separators are not supported by the adjoint
         const int cr = sb.wr.cols, cl = sb.wl.cols;
         const int nl = int(hp.left_orbs.size()), nr = int(hp.right_orbs.size());
         if (cr == 0 || cl == 0) continue;
         if (int64_t(hp.max_ne) * cl > kSolveLds || !solve_fits(hp.max_n) || !solve_fits(hp.nodes[hp.root].ne))
             return CG_ERR_INVALID;  // the transposed solves hold all cl columns of a front in LDS
-        auto columns = [&](const CMat& w, int m, const DevBuf& orbs, bool transpose, std::vector<double2>& host) {
-            const int c = w.cols;
-            std::vector<double2> wh(size_t(m) * c);
-            for (int col = 0; col < c; ++col)
-                for (int a = 0; a < m; ++a) wh[a + size_t(col) * m] = double2{w(a, col).real(), w(a, col).imag()};
-            DevBuf dw, xb;
-            upload(dw, wh, eng.lane->stream);
-            const size_t bytes = size_t(n_orb) * c * sizeof(double2);
-            xb.ensure(bytes);
-            HIP_CHECK(hipMemsetAsync(xb.p, 0, bytes, eng.lane->stream));
-            SolveView xv{xb.as<double2>(), n_orb * c, int32_t(n_orb), c, dp.vars.as<int32_t>()};
-            launch_scatter(xv, orbs.as<int32_t>(), m, dw.as<double2>(), int64_t(m) * c, transpose, 1, eng.lane->stream);
-            solve(eng, hp, dp, pv, xv, 1, false, transpose, true);
-            host.resize(size_t(n_orb) * c);
-            HIP_CHECK(hipMemcpyAsync(host.data(), xb.p, bytes, hipMemcpyDeviceToHost, eng.lane->stream));
-            HIP_CHECK(hipStreamSynchronize(eng.lane->stream));
-            eng.collect();
-        };
         std::vector<double2> psi, phi;
-        columns(sb.wr, nr, dp.right_orbs, false, psi);  // psi = G E_R W_R
-        columns(sb.wl, nl, dp.left_orbs, true, phi);    // phi = G^T E_L conj(W_L)
+        solve_columns(eng, hp, dp, pv, n_orb, sb.wr, nr, dp.right_orbs, false, psi);  // psi = G E_R W_R
+        solve_columns(eng, hp, dp, pv, n_orb, sb.wl, nl, dp.left_orbs, true, phi);    // phi = G^T E_L conj(W_L)
         auto at = [&](const std::vector<double2>& m, int64_t row, int col) {
             double2 v = m[size_t(row) + size_t(col) * size_t(n_orb)];
             return cplx(v.x, v.y);
@@ -1899,6 +1917,46 @@ int32_t backward(const cg_system* s, const cg_sigma* g, double* out_t, double* o
         }
     }
     *out_t = total;
+    return CG_OK;
+}
+
+int32_t scattering_states(const cg_system* s, const cg_sigma* g, int64_t* cols, double* out_l, double* out_r) {
+    int64_t cl_total = 0, cr_total = 0;
+    for (const auto& sb : g->blocks) {
+        cl_total += sb.wl.cols;
+        cr_total += sb.wr.cols;
+    }
+    cols[0] = cl_total;
+    cols[1] = cr_total;
+    if (!out_l && !out_r) return CG_OK;
+    Engine& eng = Engine::get();
+    if (!eng.ok) return CG_ERR_DEVICE;
+    std::lock_guard<std::mutex> lock(eng.mu);
+    auto sp = structure_of(eng, s);
+    const int64_t n_orb = int64_t(s->spec.h.n);
+    if (out_l) std::fill(out_l, out_l + 2 * n_orb * cl_total, 0.0);
+    if (out_r) std::fill(out_r, out_r + 2 * n_orb * cr_total, 0.0);
+    int64_t c0l = 0, c0r = 0;  // first column of the block's states
+    for (size_t b = 0; b < sp->host.size(); ++b) {
+        const HostPlan& hp = sp->host[b];
+        const DevPlan& dp = sp->dev[b];
+        const auto& sb = g->blocks[b];
+        PointsView pv{};
+        const int32_t rc = factor_point(eng, s, g, b, hp, dp, pv);
+        if (rc != CG_OK) return rc;
+        if (!solve_fits(hp.max_n) || !solve_fits(hp.nodes[hp.root].ne)) return CG_ERR_INVALID;
+        std::vector<double2> x;
+        if (out_l && sb.wl.cols > 0) {
+            solve_columns(eng, hp, dp, pv, n_orb, sb.wl, int(hp.left_orbs.size()), dp.left_orbs, false, x);
+            std::memcpy(out_l + 2 * n_orb * c0l, x.data(), x.size() * sizeof(double2));
+        }
+        if (out_r && sb.wr.cols > 0) {
+            solve_columns(eng, hp, dp, pv, n_orb, sb.wr, int(hp.right_orbs.size()), dp.right_orbs, false, x);
+            std::memcpy(out_r + 2 * n_orb * c0r, x.data(), x.size() * sizeof(double2));
+        }
+        c0l += sb.wl.cols;
+        c0r += sb.wr.cols;
+    }
     return CG_OK;
 }
 

@@ -27,6 +27,10 @@ int32_t kernel_stats(cg_kernel_stat* out, int32_t max_records);
 /// T and dT/d(eps_a) for every orbital, from the stored fronts of one point.
 int32_t backward(const cg_system* s, const cg_sigma* g, double* out_t, double* out_grad);
 
+/// psi_L = G E_L W_L and psi_R = G E_R W_R on every orbital (cols[0], cols[1] channels, the
+/// blocks' columns one after another); with both outputs NULL only the channel counts.
+int32_t scattering_states(const cg_system* s, const cg_sigma* g, int64_t* cols, double* out_l, double* out_r);
+
 bool device_available();
 
 /// Frees the device copies cached for a system.

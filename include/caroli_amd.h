@@ -189,6 +189,26 @@ int32_t cg_kernel_stats(cg_kernel_stat* out, int32_t max_records);
  * from one backward sweep over the stored fronts (no counterpart in the reference). */
 int32_t cg_backward(const cg_system* s, const cg_sigma* g, double* out_t, double* out_grad);
 
+/* Scattering states (NestedTwoTerminal::scattering_states, crates/caroli-green/src/nested.rs:851):
+ * psi_p = G W_p on every device orbital for each lead p (W_p W_p^H = Gamma_p, one column per
+ * channel), the blocks' columns one after another. cols[0], cols[1]: channels of the left and
+ * right lead (always written). psi_left (n x cols[0]) and psi_right (n x cols[1]): interleaved
+ * complex, column-major; with both NULL only the counts are returned. CG_ERR_INVALID for a point
+ * whose separators delayed (the solves follow the static plan). */
+int32_t cg_scattering_states(const cg_system* s, const cg_sigma* g, int64_t* cols, double* psi_left,
+                             double* psi_right);
+/* Site densities (site_densities, nested.rs:994-1025): sum_p f[p] psi_p^H O psi_p / 2 pi on the
+ * orbitals of every site (sites in the order the system was created with); op: NULL for O = 1, else
+ * an op_dim x op_dim complex matrix (interleaved, column-major) for sites of op_dim orbitals. */
+int32_t cg_site_densities(const cg_system* s, const double* psi_left, const double* psi_right, const int64_t* cols,
+                          const double* f, const double* op, int32_t op_dim, double* out);
+/* Bond currents (bond_currents, nested.rs:930-990): 2 Re sum_{a in i, b in j} H_ab G^<_ba with
+ * G^< = i sum_p f[p] psi_p psi_p^H, every coupled pair of sites i < j once, ordered by (i, j). With
+ * from / to / current NULL, *count receives the number of bonds; else *count is the capacity on entry
+ * and the number written on return. */
+int32_t cg_bond_currents(const cg_system* s, const double* psi_left, const double* psi_right, const int64_t* cols,
+                         const double* f, int64_t* count, int64_t* from, int64_t* to, double* current);
+
 /* Device present and usable (1) or not (0). */
 int32_t cg_device_available(void);
 

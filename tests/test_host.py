@@ -347,6 +347,62 @@ def test_tuning_defaults_variables_and_override(monkeypatch):
     assert ca.get_tuning() == {"small_fronts": 1, "split": 1, "blocked_min_ne": 0, "unblocked": 0}
 
 
+def _oracle_states(s, e):
+    """Scattering states psi_p = G[:, p] W_p from the oracle's G columns (W W^H = Gamma by an eigen-
+    factorisation, a gauge of its own), with the oracle's T and the left-slice G columns."""
+    arr = s.arrays()
+    o = OracleSystem(arr)
+    _, sig = sigmas_of(s, e)
+    t, rc = o.transmission_with(e, sig)
+    assert rc == 0
+
+    def root(gam):
+        w, v = np.linalg.eigh((gam + gam.conj().T) / 2)
+        keep = w > 1e-12 * w.max()
+        return v[:, keep] * np.sqrt(w[keep])
+
+    left, right, gcols = [], [], []
+    for b in range(s.block_count()):
+        sl, sr = sig[b]
+        gl_cols = o.green_columns(b, e, sl, sr, 0)
+        left.append(gl_cols @ root(1j * (sl - sl.conj().T)))
+        right.append(o.green_columns(b, e, sl, sr, 1) @ root(1j * (sr - sr.conj().T)))
+        gcols.append((o.block(b)[0].astype(int), gl_cols))
+    return ca.ScatteringStates(e, [np.hstack(left), np.hstack(right)]), t, gcols
+
+
+def test_state_observables_conserve_current_and_sum_rule():
+    """cg_bond_currents and cg_site_densities (host code) on states built from the oracle's G
+    columns: with f = (1, 0) the current through every transverse cut is T (Landauer, and current
+    conservation); with f = (1, 1) a site's density is its spectral density -Im G_aa / pi summed over
+    its orbitals (G Gamma_L G^H + G Gamma_R G^H = i (G - G^H)), checked on the left-slice sites."""
+    s = ca.NestedTwoTerminal.preset(ca.Model.Square, [14, 7, 0], disorder=1, strength=1.0, seed=3)
+    states, t, gcols = _oracle_states(s, 0.4)
+    arr = s.arrays()
+    x = np.asarray(arr["cells"]).reshape(-1, 3)[:, 0]
+    fr, to, cur = s.bond_currents(states, [1.0, 0.0])
+    assert np.all(fr < to) and len(set(zip(fr, to))) == len(fr)
+    for c in range(int(x.min()), int(x.max())):
+        across = np.where((x[fr] <= c) & (c < x[to]), cur, 0.0) - np.where((x[to] <= c) & (c < x[fr]), cur, 0.0)
+        assert abs(across.sum() - t) <= 1e-10 * t, (c, across.sum(), t)
+    dens = s.site_densities(states, [1.0, 1.0])
+    offs, norb = np.asarray(arr["site_offsets"]), np.asarray(arr["site_orbitals"])
+    left, g = gcols[0]
+    pos = {int(o): k for k, o in enumerate(left)}
+    checked = 0
+    for i in range(len(offs)):
+        orbs = range(int(offs[i]), int(offs[i]) + int(norb[i]))
+        if all(o in pos for o in orbs):
+            ldos = sum(-g[o, pos[o]].imag / np.pi for o in orbs)
+            assert abs(dens[i] - ldos) <= 1e-10 * abs(ldos), (i, dens[i], ldos)
+            checked += 1
+    assert checked > 0
+    one = np.eye(int(norb[0]))
+    assert np.allclose(s.site_densities(states, [0.3, 0.7], op=one), s.site_densities(states, [0.3, 0.7]), rtol=1e-13)
+    with pytest.raises(ca.GreenError):
+        s.site_densities(states, [1.0, 0.0], op=np.eye(int(norb[0]) + 1))
+
+
 def test_oracle_gradient_matches_finite_differences():
     # The FP64 gradient built from G columns (2 Re (G Gamma_R G^H Gamma_L G)_aa, the checker of the
     # GPU adjoint) against central differences of the oracle's own transmission.
