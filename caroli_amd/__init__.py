@@ -324,6 +324,24 @@ class NestedTwoTerminal:
                                      _dptr(cur)), "bond currents")
         return fr, to, cur
 
+    def operator_currents(self, states, f, op):
+        """Re Tr[(O H_ij + H_ij O) G^<_ji] for every coupled pair of sites i < j (nested.rs:925-985;
+        a spin current for O = s_z): arrays (from, to, current)."""
+        left, right, cols = states._packed()
+        fa = np.asarray(f, float)
+        o = np.asarray(op, complex)
+        ob = np.ascontiguousarray(o.T).view(np.float64)  # column-major (re, im)
+        count = C.c_int64()
+        _check(capi.cg_operator_currents(self._h, _dptr(left), _dptr(right), cols, _dptr(fa), _dptr(ob), o.shape[0],
+                                         C.byref(count), None, None, None), "operator currents")
+        fr = np.zeros(count.value, np.int64)
+        to = np.zeros(count.value, np.int64)
+        cur = np.zeros(count.value)
+        _check(capi.cg_operator_currents(self._h, _dptr(left), _dptr(right), cols, _dptr(fa), _dptr(ob), o.shape[0],
+                                         C.byref(count), fr.ctypes.data_as(C.POINTER(C.c_int64)),
+                                         to.ctypes.data_as(C.POINTER(C.c_int64)), _dptr(cur)), "operator currents")
+        return fr, to, cur
+
 
 class ScatteringStates:
     """ScatteringStates of nested.rs: the energy and, per lead, the states psi_p (n x channels)."""

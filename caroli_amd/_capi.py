@@ -125,4 +125,7 @@ cg_site_densities = _sig("cg_site_densities", C.c_int32, c_sys, P(C.c_double), P
                          P(C.c_double), P(C.c_double), C.c_int32, P(C.c_double))
 cg_bond_currents = _sig("cg_bond_currents", C.c_int32, c_sys, P(C.c_double), P(C.c_double), P(C.c_int64),
                         P(C.c_double), P(C.c_int64), P(C.c_int64), P(C.c_int64), P(C.c_double))
-EXPORTS += ["cg_scattering_states", "cg_site_densities", "cg_bond_currents"]
+cg_operator_currents = _sig("cg_operator_currents", C.c_int32, c_sys, P(C.c_double), P(C.c_double), P(C.c_int64),
+                            P(C.c_double), P(C.c_double), C.c_int32, P(C.c_int64), P(C.c_int64), P(C.c_int64),
+                            P(C.c_double))
+EXPORTS += ["cg_scattering_states", "cg_site_densities", "cg_bond_currents", "cg_operator_currents"]

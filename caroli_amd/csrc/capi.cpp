@@ -633,6 +633,57 @@ int32_t cg_bond_currents(const cg_system* s, const double* psi_left, const doubl
     return CG_OK;
 }
 
+int32_t cg_operator_currents(const cg_system* s, const double* psi_left, const double* psi_right, const int64_t* cols,
+                             const double* f, const double* op, int32_t op_dim, int64_t* count, int64_t* from,
+                             int64_t* to, double* current) {
+    if (!s || !psi_left || !psi_right || !cols || !f || !op || op_dim <= 0 || !count || cols[0] < 0 || cols[1] < 0)
+        return CG_ERR_INVALID;
+    const auto& sp = s->spec;
+    const auto& h = sp.h;
+    std::vector<int64_t> owner(h.n, 0);
+    for (size_t i = 0; i < sp.site_orbitals.size(); ++i)
+        for (int o = 0; o < sp.site_orbitals[i]; ++o) owner[sp.site_offsets[i] + size_t(o)] = int64_t(i);
+    // The blocks H_ij of every bond i < j (nested.rs:958-975); the operator acts on a site's orbitals.
+    const int m = op_dim;
+    std::map<std::pair<int64_t, int64_t>, std::vector<cplx>> blocks;
+    for (size_t a = 0; a < h.n; ++a)
+        for (int64_t e = h.row_ptr[a]; e < h.row_ptr[a + 1]; ++e) {
+            const size_t b = h.cols[e];
+            const int64_t i = owner[a], j = owner[b];
+            if (i < j) {
+                if (sp.site_orbitals[size_t(i)] != m || sp.site_orbitals[size_t(j)] != m) return CG_ERR_INVALID;
+                auto& blk = blocks[{i, j}];
+                if (blk.empty()) blk.assign(size_t(m) * m, cplx(0.0, 0.0));
+                blk[(a - sp.site_offsets[size_t(i)]) + size_t(m) * (b - sp.site_offsets[size_t(j)])] += h.vals[e];
+            }
+        }
+    if (!(from && to && current)) {
+        *count = int64_t(blocks.size());
+        return CG_OK;
+    }
+    if (*count < int64_t(blocks.size())) return CG_ERR_INVALID;
+    auto O = [&](int a, int b) { return cplx(op[2 * (a + b * m)], op[2 * (a + b * m) + 1]); };
+    States st{{psi_left, psi_right}, cols, int64_t(h.n)};
+    int64_t q = 0;
+    for (const auto& [ij, hij] : blocks) {
+        const int64_t oi = int64_t(sp.site_offsets[size_t(ij.first)]), oj = int64_t(sp.site_offsets[size_t(ij.second)]);
+        // Tr[(O H_ij + H_ij O) G^<_ji] = sum_{a, b} (O H + H O)_ab G^<_{oj + b, oi + a} (nested.rs:976-985)
+        cplx acc = 0.0;
+        for (int a = 0; a < m; ++a)
+            for (int b = 0; b < m; ++b) {
+                cplx w = 0.0;
+                for (int c = 0; c < m; ++c) w += O(a, c) * hij[c + size_t(m) * b] + hij[a + size_t(m) * c] * O(c, b);
+                if (w != cplx(0.0, 0.0)) acc += w * st.lesser(oj + b, oi + a, f);
+            }
+        from[q] = ij.first;
+        to[q] = ij.second;
+        current[q] = acc.real();
+        ++q;
+    }
+    *count = q;
+    return CG_OK;
+}
+
 int32_t cg_device_available(void) { return gpu::device_available() ? 1 : 0; }
 
 }  // extern "C"

@@ -208,6 +208,13 @@ int32_t cg_site_densities(const cg_system* s, const double* psi_left, const doub
  * and the number written on return. */
 int32_t cg_bond_currents(const cg_system* s, const double* psi_left, const double* psi_right, const int64_t* cols,
                          const double* f, int64_t* count, int64_t* from, int64_t* to, double* current);
+/* Currents of an on-site operator O (operator_currents, nested.rs:925-928, :958-985; a spin current
+ * for O = s_z): Re Tr[(O H_ij + H_ij O) G^<_ji] for every coupled pair of sites i < j, with every
+ * site of op_dim orbitals (else CG_ERR_INVALID); op: op_dim x op_dim complex, interleaved,
+ * column-major. count / from / to / current as for cg_bond_currents. */
+int32_t cg_operator_currents(const cg_system* s, const double* psi_left, const double* psi_right, const int64_t* cols,
+                             const double* f, const double* op, int32_t op_dim, int64_t* count, int64_t* from,
+                             int64_t* to, double* current);
 
 /* Device present and usable (1) or not (0). */
 int32_t cg_device_available(void);

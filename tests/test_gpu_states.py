@@ -3,8 +3,8 @@ both leads, from the stored factors (nested.rs:851-895).
 
 The states are compared with the oracle's G columns through what does not depend on the choice of
 W_p (W W^H = Gamma fixes W only up to a unitary): psi_p psi_p^H = G Gamma_p G^H, seen through the
-site densities and the bond currents (host code, itself pinned in tests/test_host.py), both to 1e-9
-relative. The currents of the GPU states also carry T through every cut.
+site densities, the bond currents and (sites of several orbitals) the currents of an on-site
+operator (host code, itself pinned in tests/test_host.py), all to 1e-9 relative. The currents of the GPU states also carry T through every cut.
 """
 import numpy as np
 import pytest
@@ -38,6 +38,12 @@ def test_scattering_states_match_oracle(label, make, energy):
         _, _, c = s.bond_currents(states, f)
         _, _, cr = s.bond_currents(ref, f)
         assert np.abs(c - cr).max() <= 1e-9 * max(np.abs(cr).max(), 1e-300), (f, np.abs(c - cr).max())
+    norb = np.asarray(s.arrays()["site_orbitals"])
+    if norb.min() == norb.max() > 1:  # an on-site operator (s_z-like) on every site
+        op = np.diag([1.0 if k < norb[0] // 2 else -1.0 for k in range(int(norb[0]))])
+        _, _, co = s.operator_currents(states, [1.0, 0.0], op)
+        _, _, cor = s.operator_currents(ref, [1.0, 0.0], op)
+        assert np.abs(co - cor).max() <= 1e-9 * max(np.abs(cor).max(), 1e-12), np.abs(co - cor).max()
     # Conservation: the GPU states carry T through every cut.
     arr = s.arrays()
     x = np.asarray(arr["cells"]).reshape(-1, 3)[:, 0]

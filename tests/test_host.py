@@ -403,6 +403,40 @@ def test_state_observables_conserve_current_and_sum_rule():
         s.site_densities(states, [1.0, 0.0], op=np.eye(int(norb[0]) + 1))
 
 
+def test_operator_currents_resolve_the_decoupled_blocks():
+    """cg_operator_currents (host code) on the clean BHZ ribbon, whose two blocks are decoupled
+    sectors of each site's four orbitals: O = 1 gives the bond currents, and the projector on one
+    block's orbitals carries that block's transmission through every cut (f = (1, 0))."""
+    s = ca.NestedTwoTerminal.preset(ca.Model.BHZ, [20, 10, 0])
+    e = 0.05
+    states, t, _ = _oracle_states(s, e)
+    arr = s.arrays()
+    o = OracleSystem(arr)
+    _, sig = sigmas_of(s, e)
+    x = np.asarray(arr["cells"]).reshape(-1, 3)[:, 0]
+    fr, to, cb = s.bond_currents(states, [1.0, 0.0])
+    fr1, to1, c1 = s.operator_currents(states, [1.0, 0.0], np.eye(4))
+    assert np.array_equal(fr, fr1) and np.array_equal(to, to1)
+    assert np.abs(c1 - cb).max() <= 1e-13 * np.abs(cb).max()
+    total = 0.0
+    for b in range(s.block_count()):
+        sl, sr = sig[b]
+        left = o.block(b)[0].astype(int)
+        g_r = o.green_columns(b, e, sl, sr, 1)
+        local = sorted(set((np.nonzero(np.abs(g_r).max(axis=1) > 0)[0] % 4).tolist()))
+        g_lr = g_r[left, :]
+        t_b = np.trace(1j * (sl - sl.conj().T) @ g_lr @ (1j * (sr - sr.conj().T)) @ g_lr.conj().T).real
+        total += t_b
+        proj = np.diag([1.0 if k in local else 0.0 for k in range(4)])
+        f3, t3, cp = s.operator_currents(states, [1.0, 0.0], proj)
+        for c in range(int(x.min()), int(x.max())):
+            across = np.where((x[f3] <= c) & (c < x[t3]), cp, 0.0) - np.where((x[t3] <= c) & (c < x[f3]), cp, 0.0)
+            assert abs(across.sum() - t_b) <= 1e-10, (b, c, across.sum(), t_b)
+    assert abs(total - t) <= 1e-12 * t
+    with pytest.raises(ca.GreenError):
+        s.operator_currents(states, [1.0, 0.0], np.eye(3))
+
+
 def test_oracle_gradient_matches_finite_differences():
     # The FP64 gradient built from G columns (2 Re (G Gamma_R G^H Gamma_L G)_aa, the checker of the
     # GPU adjoint) against central differences of the oracle's own transmission.
