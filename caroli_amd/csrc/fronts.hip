@@ -1132,6 +1132,7 @@ template <class T>
 __global__ void __launch_bounds__(256)
     gather_kernel(const NodeDesc* __restrict__ nodes, const Tile* __restrict__ tiles, const KidDesc* __restrict__ kids,
                   const int32_t* __restrict__ maps, PointsView pv) {
+    constexpr int kGBatch = 4;  // entries per lane per batch of loads
     const TilePoint tp = xcd_tile();
     const Tile tl = tiles[tp.tile];
     const NodeDesc nd = nodes[tl.node];
@@ -1140,36 +1141,36 @@ __global__ void __launch_bounds__(256)
     T* F = reinterpret_cast<T*>(pv.arena) + p * pv.arena_stride + nd.off;
     const int n = nd.n, ne = nd.ne;
     const int rows = min(tl.pad >> 16, n - tl.i0), cols = min(tl.pad & 0xffff, n - tl.j0), cnt = rows * cols;
-    for (int base = int(threadIdx.x); base < cnt; base += 256 * kBatch) {
-        int ii[kBatch], jj[kBatch];
+    for (int base = int(threadIdx.x); base < cnt; base += 256 * kGBatch) {
+        int ii[kGBatch], jj[kGBatch];
 #pragma unroll
-        for (int u = 0; u < kBatch; ++u) {
+        for (int u = 0; u < kGBatch; ++u) {
             const int idx = min(base + u * 256, cnt - 1);
             ii[u] = tl.i0 + idx % rows;
             jj[u] = tl.j0 + idx / rows;
         }
-        T acc[kBatch];
+        T acc[kGBatch];
 #pragma unroll
-        for (int u = 0; u < kBatch; ++u) acc[u] = cx<T>(0.0, 0.0);
+        for (int u = 0; u < kGBatch; ++u) acc[u] = cx<T>(0.0, 0.0);
         for (int c = nd.kid_beg; c < nd.kid_end; ++c) {  // per child: all map loads, then all values
             const KidDesc kd = kids[c];
             const int32_t* inv = maps + kd.inv_beg;
             const T* S = kidbase + kd.src;
-            int a[kBatch], b[kBatch];
+            int a[kGBatch], b[kGBatch];
 #pragma unroll
-            for (int u = 0; u < kBatch; ++u) {
+            for (int u = 0; u < kGBatch; ++u) {
                 a[u] = inv[ii[u]];
                 b[u] = inv[jj[u]];
             }
 #pragma unroll
-            for (int u = 0; u < kBatch; ++u) {
+            for (int u = 0; u < kGBatch; ++u) {
                 T v = S[max(a[u], 0) + int64_t(max(b[u], 0)) * kd.ld];
                 if (a[u] < 0 || b[u] < 0) v = cx<T>(0.0, 0.0);
                 acc[u] = cadd(acc[u], v);
             }
         }
 #pragma unroll
-        for (int u = 0; u < kBatch; ++u)
+        for (int u = 0; u < kGBatch; ++u)
             if (base + u * 256 < cnt && (ii[u] < ne || jj[u] < ne)) F[ii[u] + int64_t(jj[u]) * n] = acc[u];
     }
 }
