@@ -24,8 +24,8 @@ sweep from a HIP graph after the warmups. The JSON line carries the roofline of 
 family: HIP events on the engine's streams during `--roofline-steps` steps that follow the timed
 ones and run the units one after the other (with several sweeps in flight, a launch's duration would
 include the other streams' work); the HBM traffic per launch comes from the rocprofv3 counter
-summary in profiles/ when one matches. For C2 and C3 the same points also run in the other
-precision (mixed vs plain). The CPU baseline is the oracle (FP64 C++ restatement of the reference
+summary in profiles/ when one matches. Unless `--no-compare`, the same points also run in the other
+precision (complex128 against refined mixed). The CPU baseline is the oracle (FP64 C++ restatement of the reference
 path) on the host cores for a bounded sample of the same points.
 """
 import argparse
