@@ -25,8 +25,8 @@ family: HIP events on the engine's streams during `--roofline-steps` steps that 
 ones and run the units one after the other (with several sweeps in flight, a launch's duration would
 include the other streams' work); the HBM traffic per launch comes from the rocprofv3 counter
 summary in profiles/ when one matches. Unless `--no-compare`, the same points also run in the other
-precision (complex128 against refined mixed). The CPU baseline is the oracle (FP64 C++ restatement of the reference
-path) on the host cores for a bounded sample of the same points.
+precision (complex128 against refined mixed). The CPU baseline is the oracle (FP64 C++ restatement
+of the reference path) on the host cores for a bounded sample of the same points.
 """
 import argparse
 import glob
