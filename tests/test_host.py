@@ -315,6 +315,18 @@ def test_invalid_arguments_are_refused():
     ext = (C.c_uint64 * 3)(4, 4, 0)
     assert capi.cg_preset_create(9, ext, 0, 0, 0.0, 0.0, 1, None, C.byref(h)) == 6
     assert capi.cg_transmission_with(None, None, 0, None) == 6
+    cols = (C.c_int64 * 2)()
+    assert capi.cg_scattering_states(None, None, cols, None, None) == 6
+    assert capi.cg_site_densities(None, None, None, None, None, None, 0, None) == 6
+    assert capi.cg_bond_currents(None, None, None, None, None, None, None, None, None) == 6
+    assert capi.cg_operator_currents(None, None, None, None, None, None, 0, None, None, None, None) == 6
+    # A system's states checked against its self-energies: channel counts need no device.
+    s = ca.NestedTwoTerminal.preset(ca.Model.Square, [8, 5, 0])
+    sig = s.self_energies(0.3)
+    assert capi.cg_scattering_states(s._h, sig._h, cols, None, None) == 0
+    assert cols[0] == cols[1] and cols[0] > 0
+    other = ca.NestedTwoTerminal.preset(ca.Model.Square, [8, 6, 0])
+    assert capi.cg_scattering_states(other._h, sig._h, cols, None, None) == 6  # foreign self-energies
 
 
 def test_tuning_defaults_variables_and_override(monkeypatch):
