@@ -4,7 +4,8 @@ both leads, from the stored factors (nested.rs:851-895).
 The states are compared with the oracle's G columns through what does not depend on the choice of
 W_p (W W^H = Gamma fixes W only up to a unitary): psi_p psi_p^H = G Gamma_p G^H, seen through the
 site densities, the bond currents and (sites of several orbitals) the currents of an on-site
-operator (host code, itself pinned in tests/test_host.py), all to 1e-9 relative. The currents of the GPU states also carry T through every cut.
+operator (host code, itself pinned in tests/test_host.py), all to 1e-9 relative, on square,
+two-block BHZ, graphene and cubic samples. The currents of the GPU states also carry T through every cut.
 """
 import numpy as np
 import pytest
@@ -22,6 +23,8 @@ pytestmark = pytest.mark.gpu
     ("BHZ 20x10 clean (two blocks)", lambda: ca.NestedTwoTerminal.preset(ca.Model.BHZ, [20, 10, 0]), 0.05),
     ("graphene 30x8 Anderson", lambda: ca.NestedTwoTerminal.preset(ca.Model.Graphene, [30, 8, 0], disorder=1,
                                                                      strength=0.5, seed=2), 0.3),
+    ("cubic 8x4x4 Anderson", lambda: ca.NestedTwoTerminal.preset(ca.Model.Cubic, [8, 4, 4], disorder=1,
+                                                                   strength=3.0, seed=7), 0.5),
 ])
 def test_scattering_states_match_oracle(label, make, energy):
     s = make()
