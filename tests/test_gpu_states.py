@@ -5,7 +5,8 @@ The states are compared with the oracle's G columns through what does not depend
 W_p (W W^H = Gamma fixes W only up to a unitary): psi_p psi_p^H = G Gamma_p G^H, seen through the
 site densities, the bond currents and (sites of several orbitals) the currents of an on-site
 operator (host code, itself pinned in tests/test_host.py), all to 1e-9 relative, on square,
-two-block BHZ, graphene and cubic samples. The currents of the GPU states also carry T through every cut.
+two-block BHZ, graphene and cubic samples. The currents of the GPU states also carry T through
+every cut.
 """
 import numpy as np
 import pytest
