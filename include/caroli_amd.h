@@ -166,7 +166,7 @@ typedef struct cg_tuning {
 int32_t cg_get_tuning(cg_tuning* out);
 int32_t cg_set_tuning(const cg_tuning* t);
 /* Autotuner (no counterpart in the reference): times whole eager sweeps of the batch's points
- * under candidate tunings, by coordinate descent over the three choices from the one in force (a
+ * (the faster of two per candidate, after one that settles) under candidate tunings, by coordinate descent over the three choices from the one in force (a
  * candidate must be 2 % faster to win), and leaves the fastest in force with the batch re-planned
  * under it. out: the choice; out_seconds: its sweep time (either may be NULL). */
 int32_t cg_batch_autotune(cg_batch* batch, cg_tuning* out, double* out_seconds);
