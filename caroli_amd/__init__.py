@@ -19,7 +19,7 @@ from . import _capi as capi
 
 __all__ = [
     "GreenError", "Precision", "PrecisionOptions", "PrecisionReport", "LeadSelfEnergies", "NestedTwoTerminal",
-    "TwoTerminal", "Model", "device_available", "batch_transmission",
+    "TwoTerminal", "Model", "device_available", "batch_transmission", "batch_transmission_checked",
 ]
 
 
@@ -49,22 +49,39 @@ class Precision(enum.IntEnum):
 
 @dataclass
 class PrecisionOptions:
-    """precision.rs:36-61. The engine reports |dT| for every point (`every` = 1)."""
+    """precision.rs:36-61: one point in `every` is recomputed in double precision and compared;
+    the default checks every point (the engine's |dT| report for every point)."""
 
     precision: Precision = Precision.Mixed
     tolerance: float = 1e-8
+    every: int = 1
+
+    @classmethod
+    def double(cls):
+        """precision.rs:48-50: nothing to check."""
+        return cls(Precision.Double, 0.0, 0)
+
+    @classmethod
+    def single(cls):
+        """precision.rs:54-56: one point in fifty checked, differences up to 1e-2 accepted."""
+        return cls(Precision.Single, 1e-2, 50)
 
 
 @dataclass
 class PrecisionReport:
-    """precision.rs:65-78, with the per-point differences."""
+    """precision.rs:65-78, with the per-point differences (NaN where a point was not compared)."""
 
     precision: Precision
     checked: int
     median: float
     max: float
-    fell_back: bool
+    failed: int
+    fallback: str
     differences: np.ndarray
+
+    @property
+    def fell_back(self) -> bool:
+        return bool(self.fallback)
 
 
 def device_available() -> bool:
@@ -263,16 +280,11 @@ class NestedTwoTerminal:
         return t.value
 
     def transmissions_checked(self, energies, options: PrecisionOptions):
-        """nested.rs:630-641 with every point checked against complex f64 (precision.rs:91-134):
-        returns (values, report); above the tolerance everything is recomputed in Double."""
+        """nested.rs:630-641: (values, report) of `checked` (precision.rs:91-134) over the energies,
+        through cg_batch_transmission_checked; NaN where a point failed."""
         sig = [self.self_energies(e) for e in energies]
-        fast, _ = batch_transmission([self] * len(sig), sig, options.precision)
-        exact, _ = batch_transmission([self] * len(sig), sig, Precision.Double)
-        d = np.abs(fast - exact)
-        worst = float(np.nanmax(d)) if len(d) else 0.0
-        fell = not (worst <= options.tolerance)
-        rep = PrecisionReport(options.precision, len(d), float(np.nanmedian(d)) if len(d) else 0.0, worst, fell, d)
-        return (exact if fell else fast), rep
+        t, st, rep = batch_transmission_checked([self] * len(sig), sig, options)
+        return t, rep
 
     def backward(self, sigma: LeadSelfEnergies):
         """T and dT/d(eps_a) for every orbital a (adjoint sweep over the stored fronts)."""
@@ -372,6 +384,26 @@ def batch_transmission(systems, sigmas, precision=Precision.Double, stats=None):
     _check(capi.cg_batch_transmission(sh, gh, n, int(precision), _dptr(t), st.ctypes.data_as(C.POINTER(C.c_int32)),
                                       C.byref(bs)))
     return t, st
+
+
+def batch_transmission_checked(systems, sigmas, options: PrecisionOptions, stats=None):
+    """checked (precision.rs:91-134) over one batched launch (cg_batch_transmission_checked):
+    returns (T, status, PrecisionReport) with the per-point |T - T_double| in report.differences."""
+    n = len(systems)
+    sh = (C.c_void_p * n)(*[s._h.value for s in systems])
+    gh = (C.c_void_p * n)(*[g._h.value for g in sigmas])
+    t = np.zeros(n)
+    st = np.zeros(n, np.int32)
+    dt = np.zeros(n)
+    opt = capi.PrecisionOptionsC(int(options.precision), float(options.tolerance), int(options.every))
+    rep = capi.PrecisionReportC()
+    bs = stats if stats is not None else capi.BatchStats()
+    _check(capi.cg_batch_transmission_checked(sh, gh, n, C.byref(opt), _dptr(t),
+                                              st.ctypes.data_as(C.POINTER(C.c_int32)), _dptr(dt), C.byref(rep),
+                                              C.byref(bs)))
+    report = PrecisionReport(Precision(rep.precision), rep.checked, rep.median, rep.max, rep.failed,
+                             rep.fallback.decode(), dt)
+    return np.where(st == 0, t, np.nan), st, report
 
 
 class ResidentBatch:

@@ -67,6 +67,22 @@ cg_transmissions = _sig("cg_transmissions", C.c_int32, c_sys, P(C.c_double), C.c
 cg_batch_transmission = _sig("cg_batch_transmission", C.c_int32, P(c_sys), P(c_sig), C.c_int64, C.c_int32,
                              P(C.c_double), P(C.c_int32), P(BatchStats))
 cg_backward = _sig("cg_backward", C.c_int32, c_sys, c_sig, P(C.c_double), P(C.c_double))
+
+
+class PrecisionOptionsC(C.Structure):
+    """cg_precision_options."""
+    _fields_ = [("precision", C.c_int32), ("tolerance", C.c_double), ("every", C.c_int64)]
+
+
+class PrecisionReportC(C.Structure):
+    """cg_precision_report."""
+    _fields_ = [("precision", C.c_int32), ("checked", C.c_int64), ("median", C.c_double), ("max", C.c_double),
+                ("failed", C.c_int64), ("fell_back", C.c_int32), ("fallback", C.c_char * 160)]
+
+
+cg_batch_transmission_checked = _sig("cg_batch_transmission_checked", C.c_int32, P(c_sys), P(c_sig), C.c_int64,
+                                     P(PrecisionOptionsC), P(C.c_double), P(C.c_int32), P(C.c_double),
+                                     P(PrecisionReportC), P(BatchStats))
 cg_device_available = _sig("cg_device_available", C.c_int32)
 
 # Every function the header declares, for the export check.
@@ -75,6 +91,7 @@ EXPORTS = [
     "cg_block_sizes", "cg_system_arrays", "cg_tree_node", "cg_work", "cg_self_energies", "cg_sigma_from_arrays",
     "cg_sigma_block", "cg_sigma_energy", "cg_sigma_destroy", "cg_transmission", "cg_transmission_with",
     "cg_transmissions", "cg_batch_transmission", "cg_backward", "cg_device_available",
+    "cg_batch_transmission_checked",
 ]
 
 

@@ -1,12 +1,15 @@
 // The C ABI (include/caroli_amd.h): system construction, host self-energies, and the calls into
 // the GPU engine.
 #include <cmath>
+#include <cstdio>
 #include <cstring>
+#include <memory>
 #include <exception>
 #include <map>
 
 #include "../../include/caroli_amd.h"
 #include "engine.hpp"
+#include "checked.hpp"
 #include "system.hpp"
 
 using namespace caroli;
@@ -124,6 +127,7 @@ void write_mat(const CMat& m, double* p) {
 }
 
 int32_t finish(cg_system* s, cg_system** out) {
+    std::unique_ptr<cg_system> hold(s);  // freed if the symbolic phase refuses the system
     s->sym = symbolic_phase(s->spec);
     uint64_t h = 1469598103934665603ULL;
     h = fnv(h, &s->spec.h.n, sizeof(s->spec.h.n));
@@ -136,7 +140,7 @@ int32_t finish(cg_system* s, cg_system** out) {
     h = fnv(h, &s->spec.axis, sizeof(int));
     for (auto* m : {&s->spec.left_h00, &s->spec.left_h01, &s->spec.right_h00, &s->spec.right_h01}) h = pattern_hash(h, *m);
     s->key = h;
-    *out = s;
+    *out = hold.release();
     return CG_OK;
 }
 
@@ -149,19 +153,36 @@ int32_t cg_system_create(uint64_t n, const int64_t* row_ptr, const uint32_t* col
                          const int64_t* range, int32_t axis, const double* left_h00, const double* left_h01, uint64_t nl,
                          const double* right_h00, const double* right_h01, uint64_t nr, cg_system** out) {
     if (!out || !row_ptr || !cols || !vals || !cells || !site_orbitals || !site_offsets || !range || !left_h00 ||
-        !left_h01 || !right_h00 || !right_h01 || dim < 1 || dim > 3 || axis < 0 || axis >= dim || nsites == 0)
+        !left_h01 || !right_h00 || !right_h01 || dim < 1 || dim > 3 || axis < 0 || axis >= dim || nsites == 0 ||
+        n == 0 || nl == 0 || nr == 0)
         return CG_ERR_INVALID;
+    // The CSR must start at 0 and be non-decreasing, its columns in range; every site must have a
+    // positive number of orbitals and the sites must cover the n orbitals exactly once (the
+    // reference builds these arrays itself and never sees others: crates/caroli-model/src/lib.rs:427-470).
+    if (row_ptr[0] != 0) return CG_ERR_INVALID;
+    for (uint64_t r = 0; r < n; ++r)
+        if (row_ptr[r] > row_ptr[r + 1]) return CG_ERR_INVALID;
+    for (int d = 0; d < dim; ++d)
+        if (range[d] < 0) return CG_ERR_INVALID;
+    {
+        std::vector<char> covered(n, 0);
+        for (uint64_t i = 0; i < nsites; ++i) {
+            if (site_orbitals[i] <= 0 || site_offsets[i] >= n || uint64_t(site_orbitals[i]) > n - site_offsets[i])
+                return CG_ERR_INVALID;
+            for (int32_t o = 0; o < site_orbitals[i]; ++o) {
+                if (covered[site_offsets[i] + uint64_t(o)]) return CG_ERR_INVALID;  // overlapping sites
+                covered[site_offsets[i] + uint64_t(o)] = 1;
+            }
+        }
+        for (char c : covered)
+            if (!c) return CG_ERR_INVALID;
+    }
     try {
         auto* s = new cg_system;
         auto& sp = s->spec;
         sp.h.n = n;
         sp.h.row_ptr.assign(row_ptr, row_ptr + n + 1);
         size_t nnz = size_t(row_ptr[n]);
-        for (size_t r = 0; r < n; ++r)
-            if (row_ptr[r] > row_ptr[r + 1]) {
-                delete s;
-                return CG_ERR_INVALID;
-            }
         sp.h.cols.assign(cols, cols + nnz);
         for (auto c : sp.h.cols)
             if (c >= n) {
@@ -170,20 +191,10 @@ int32_t cg_system_create(uint64_t n, const int64_t* row_ptr, const uint32_t* col
             }
         sp.h.vals.resize(nnz);
         for (size_t e = 0; e < nnz; ++e) sp.h.vals[e] = cplx(vals[2 * e], vals[2 * e + 1]);
-        size_t total = 0;
         for (size_t i = 0; i < nsites; ++i) {
             sp.cells.push_back({cells[3 * i], cells[3 * i + 1], cells[3 * i + 2]});
             sp.site_orbitals.push_back(site_orbitals[i]);
             sp.site_offsets.push_back(site_offsets[i]);
-            total += size_t(site_orbitals[i]);
-            if (site_offsets[i] + size_t(site_orbitals[i]) > n) {
-                delete s;
-                return CG_ERR_INVALID;
-            }
-        }
-        if (total != n) {
-            delete s;
-            return CG_ERR_INVALID;
         }
         sp.dim = dim;
         sp.range = {range[0], range[1], range[2]};
@@ -477,6 +488,41 @@ int32_t cg_kernel_timing(int32_t enable) {
 int32_t cg_kernel_stats(cg_kernel_stat* out, int32_t max_records) {
     if (!out || max_records < 0) return CG_ERR_INVALID;
     return gpu::kernel_stats(out, max_records);
+}
+
+int32_t cg_batch_transmission_checked(const cg_system* const* systems, const cg_sigma* const* sigmas, int64_t count,
+                                      const cg_precision_options* options, double* out_t, int32_t* out_status,
+                                      double* out_dt, cg_precision_report* report, cg_batch_stats* stats) {
+    if (!options || !report || count < 0 || (count > 0 && (!systems || !sigmas || !out_t || !out_status || !out_dt)))
+        return CG_ERR_INVALID;
+    if (options->precision < CG_DOUBLE || options->precision > CG_MIXED || !(options->tolerance >= 0.0))
+        return CG_ERR_INVALID;
+    bool first = true;
+    auto run = [&](const std::vector<int64_t>& idx, int32_t precision, double* t, int32_t* st) {
+        std::vector<const cg_system*> ss;
+        std::vector<const cg_sigma*> gg;
+        for (int64_t p : idx) {
+            ss.push_back(systems[p]);
+            gg.push_back(sigmas[p]);
+        }
+        cg_batch_stats local{};
+        const int32_t rc = cg_batch_transmission(ss.data(), gg.data(), int64_t(idx.size()), precision, t, st, &local);
+        if (first && stats) *stats = local;
+        first = false;
+        return rc;
+    };
+    CheckedReport rep;
+    const int32_t rc = checked_batch(count, options->precision, options->tolerance, options->every, run, out_t,
+                                     out_status, out_dt, rep);
+    *report = cg_precision_report{};
+    report->precision = rep.precision;
+    report->checked = rep.checked;
+    report->median = rep.median;
+    report->max = rep.max;
+    report->failed = rep.failed;
+    report->fell_back = rep.fell_back ? 1 : 0;
+    std::snprintf(report->fallback, sizeof report->fallback, "%s", rep.fallback.c_str());
+    return rc;
 }
 
 int32_t cg_transmission_with(const cg_system* s, const cg_sigma* g, int32_t precision, double* out_t) {

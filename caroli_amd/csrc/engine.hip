@@ -101,6 +101,9 @@ struct LevelSet {
     std::vector<int32_t> blk, blk_count, rt_off, ut_off;
     std::vector<Tile> rtiles, utiles;
     std::vector<double> panel_flops, panel_words, row_flops, row_words, upd_flops, upd_words;
+    // Classified for the unblocked fallback (every large front assembled whole and eliminated by
+    // rank-1 updates): its lists are the only ones the launches may follow.
+    bool unblocked = false;
     // Algorithmic work (complex words, flops) for the kernel timers.
     double lds_flops[kClasses] = {}, lds_words[kClasses] = {}, lds_s_words[kClasses] = {}, lds_f_words[kClasses] = {};
     double asm_words = 0, gather_words = 0, border_flops = 0, border_words = 0, schur_flops = 0, schur_words = 0;
@@ -192,6 +195,7 @@ LevelSet classify(const std::vector<NodeDesc>& nodes, const std::vector<KidDesc>
     LevelSet L;
     const int limit = single ? kLdsFrontC64 : kLdsFrontC128;
     const bool unblocked = unblocked_fronts(), split = !border_only();
+    L.unblocked = unblocked;
     const int blocked_above = blocked_min_ne();
     for (int32_t q : list) {
         const NodeDesc& d = nodes[q];
@@ -768,7 +772,9 @@ void run_level(Engine& eng, const LevelSet& L, const DevLevel& D, const NodeDesc
     }
     const int32_t count = int32_t(L.large.size());
     if (count == 0) return;
-    const bool fsingle = single && !root, unblocked = unblocked_fronts();
+    // The plan's own classification, not the tuning in force now: a plan classified for the
+    // unblocked path has no tile lists, and one classified for the tiled path no whole-front list.
+    const bool fsingle = single && !root, unblocked = L.unblocked;
     if (root || unblocked) {  // the whole front assembled in HBM
         eng.timed(root ? kRootAssemble : kAssemble, 0.0, P * fes * L.asm_words, [&] {
             if (root)
@@ -1235,11 +1241,11 @@ void release(const cg_system* s) {
 
 /// A resident batch: launch units (structure, block, chunk of points) with their inputs in HBM.
 /// A sweep captured as a HIP graph, with the device buffers it was captured on (the lane's
-/// arenas, pivots, statuses, results and trace scratch) and its point count: a replay is valid
-/// while they are the same.
+/// arenas, pivots, statuses, results and trace scratch), its point count and the small-front
+/// kernel in force: a replay is valid while they are the same.
 struct SweepGraph {
     hipGraphExec_t exec = nullptr;
-    std::array<const void*, 8> key{};
+    std::array<const void*, 9> key{};
     int eager_runs = 0;
     SweepGraph() = default;
     SweepGraph(SweepGraph&& o) noexcept : exec(o.exec), key(o.key), eager_runs(o.eager_runs) { o.exec = nullptr; }
@@ -1390,8 +1396,12 @@ bool graphs_enabled() {
 void sweep_graph(Engine& eng, cg_batch::Unit& u, const HostPlan& hp, const DevPlan& dp, const PointsView& pv,
                  int32_t P, bool single) {
     Lane& ln = *eng.lane;
-    const std::array<const void*, 8> key{ln.arena.p,   ln.root_arena.p, ln.piv.p,     ln.status.p,
-                                         ln.tout.p,    ln.trace_x.p,    ln.trace_p.p, reinterpret_cast<const void*>(intptr_t(P))};
+    // The small-front kernel is chosen at launch (the tuning's small_fronts): a graph captured
+    // under another choice is captured anew.
+    const std::array<const void*, 9> key{ln.arena.p,   ln.root_arena.p, ln.piv.p,
+                                         ln.status.p,  ln.tout.p,       ln.trace_x.p,
+                                         ln.trace_p.p, reinterpret_cast<const void*>(intptr_t(P)),
+                                         reinterpret_cast<const void*>(intptr_t(small_front_mode()))};
     SweepGraph& g = u.graph;
     if (g.exec && g.key == key) {
         HIP_CHECK(hipGraphLaunch(g.exec, ln.stream));
@@ -1428,6 +1438,15 @@ int32_t run_batch(cg_batch* bt, double* out_t, int32_t* out_status, cg_batch_sta
     {
         std::lock_guard<std::mutex> lock(eng.mu);
         eng.tol_single = mixed ? kTolMixed : kTolSingle;
+        // A unit planned under a tuning that classifies fronts differently from the one in force
+        // (cg_set_tuning after cg_batch_create) is re-planned here, and its captured graph dropped.
+        for (auto& u : bt->units) {
+            auto sp = structure_of(eng, bt->systems[u.points[0]]);
+            if (sp != u.structure) {
+                u.structure = sp;
+                u.graph = SweepGraph{};
+            }
+        }
         // Units go out in groups of bt->lanes, one lane (stream and buffers) each, so that their
         // sweeps overlap; results are read back once the group is done.
         // Per-kernel timing runs the units one at a time, so that each timed launch has the GPU to
@@ -1616,13 +1635,15 @@ int32_t set_tuning(const cg_tuning* t) {
 /// Coordinate descent over the kernel choices on the batch's own points: every candidate puts the
 /// batch's units on plans classified under it (their captured graphs dropped), runs one eager
 /// sweep to settle and keeps the faster of the next two; a candidate replaces the best so far when
-/// 2 % faster.
+/// 2 % faster and when every point ends with the status it had under the tuning in force at the
+/// start (a candidate that fails points does not win on speed). If a device error interrupts the
+/// search, the tuning in force before the call is restored before the error is returned.
 int32_t autotune_batch(cg_batch* bt, cg_tuning* out, double* out_seconds) {
     Engine& eng = Engine::get();
     if (!eng.ok) return CG_ERR_DEVICE;
     const int64_t count = int64_t(bt->systems.size());
     std::vector<double> t(size_t(std::max<int64_t>(count, 1)));
-    std::vector<int32_t> st(t.size());
+    std::vector<int32_t> st(t.size()), st0;
     auto apply = [&](const Tuning& tu) {
         std::lock_guard<std::mutex> lock(eng.mu);
         set_tuning_override(&tu);
@@ -1646,34 +1667,46 @@ int32_t autotune_batch(cg_batch* bt, cg_tuning* out, double* out_seconds) {
         bt->eager = false;
         return sec;
     };
-    Tuning best = current_tuning();
-    double best_s = measure(best);
-    auto consider = [&](const Tuning& c) {
-        if (rc != CG_OK || (c.same_plan(best) && c.small_fronts == best.small_fronts)) return;
-        const double sec = measure(c);
-        if (rc == CG_OK && sec < 0.98 * best_s) {
-            best = c;
-            best_s = sec;
-        }
-    };
-    for (int sf : {0, 1, 2}) {
-        Tuning c = best;
-        c.small_fronts = sf;
-        consider(c);
-    }
-    if (!best.unblocked) {
-        for (int sp : {1, 0}) {
+    const Tuning start = current_tuning();
+    Tuning best = start;
+    double best_s = INFINITY;
+    try {
+        best_s = measure(best);
+        st0 = st;
+        auto consider = [&](const Tuning& c) {
+            if (rc != CG_OK || (c.same_plan(best) && c.small_fronts == best.small_fronts)) return;
+            const double sec = measure(c);
+            if (rc == CG_OK && st == st0 && sec < 0.98 * best_s) {
+                best = c;
+                best_s = sec;
+            }
+        };
+        for (int sf : {0, 1, 2}) {
             Tuning c = best;
-            c.split = sp;
+            c.small_fronts = sf;
             consider(c);
         }
-        for (int bm : {64, 128, 256}) {
-            Tuning c = best;
-            c.blocked_min_ne = bm;
-            consider(c);
+        if (!best.unblocked) {
+            for (int sp : {1, 0}) {
+                Tuning c = best;
+                c.split = sp;
+                consider(c);
+            }
+            for (int bm : {64, 128, 256}) {
+                Tuning c = best;
+                c.blocked_min_ne = bm;
+                consider(c);
+            }
         }
+        apply(best);  // the winner stays in force, the batch on its plans (graphs captured anew)
+    } catch (...) {
+        bt->eager = false;
+        try {
+            apply(start);
+        } catch (...) {
+        }
+        throw;
     }
-    apply(best);  // the winner stays in force, the batch on its plans (graphs captured anew)
     if (rc != CG_OK) return rc;
     if (out) *out = to_c(best);
     if (out_seconds) *out_seconds = best_s;

@@ -9,7 +9,7 @@
  *   NestedTwoTerminal::self_energies       nested.rs:652   -> cg_self_energies
  *   NestedTwoTerminal::transmission_with   nested.rs:682   -> cg_transmission_with
  *   ::transmission_with_precision          nested.rs:688   -> cg_transmission_with (precision)
- *   ::transmissions_checked                nested.rs:630   -> cg_batch_transmission (per-point dT)
+ *   ::transmissions_checked                nested.rs:630   -> cg_batch_transmission_checked (per-point dT)
  *   TwoTerminal::transmission              two_terminal.rs:186 -> cg_transmission (same engine)
  * and adds the batched launch over systems x energies (the callers' rayon fan-out,
  * crates/caroli-cli/src/run.rs:113-127) and the adjoint dT/dV (no counterpart in the reference).
@@ -131,6 +131,31 @@ typedef struct cg_batch_stats {
  * out_t[p] = T or NaN; out_status[p] = status code. */
 int32_t cg_batch_transmission(const cg_system* const* systems, const cg_sigma* const* sigmas, int64_t count,
                               int32_t precision, double* out_t, int32_t* out_status, cg_batch_stats* stats);
+
+/* Checked reduced precision (PrecisionOptions / PrecisionReport / checked,
+ * crates/caroli-green/src/precision.rs:36-134; NestedTwoTerminal::transmissions_checked,
+ * nested.rs:630-641): every point runs in options->precision; the points that failed there and
+ * one point in `every` (at least 1; 1 checks every point) are recomputed in complex f64, their
+ * |T - T_double| written to out_dt (NaN for points not compared) and their double values returned;
+ * if the largest difference exceeds the tolerance (or a point failed in double only), the whole
+ * batch is recomputed in complex f64 and report->fell_back is set. stats: the first run's. */
+typedef struct cg_precision_options {
+  int32_t precision; /* CG_DOUBLE, CG_SINGLE or CG_MIXED */
+  double tolerance;  /* largest accepted |T - T_double| */
+  int64_t every;     /* one point in `every` is compared */
+} cg_precision_options;
+typedef struct cg_precision_report {
+  int32_t precision; /* precision of the values returned: CG_DOUBLE after a fallback */
+  int64_t checked;   /* points recomputed in complex f64 and compared */
+  double median;     /* median and largest difference among them */
+  double max;
+  int64_t failed;    /* points that failed in the reduced precision, computed in complex f64 */
+  int32_t fell_back; /* 1: the whole batch was recomputed in complex f64 */
+  char fallback[160]; /* why (empty when it was not) */
+} cg_precision_report;
+int32_t cg_batch_transmission_checked(const cg_system* const* systems, const cg_sigma* const* sigmas, int64_t count,
+                                      const cg_precision_options* options, double* out_t, int32_t* out_status,
+                                      double* out_dt, cg_precision_report* report, cg_batch_stats* stats);
 
 /* The front update on the GPU, as caroli_dense::Field::schur (crates/caroli-dense/src/field.rs:44):
  * S = F22 - F21 F11^-1 F12 of the leading ne x ne block of the n x n front f (column-major), with

@@ -17,9 +17,13 @@
 //   caroli_trace/broadening  crates/caroli-green/src/lib.rs:68-71, :108-113
 //   Factor::back_substitute  nested.rs:200-227 (G columns on lead slices, scattering_states)
 // The lead self-energies are inputs (as `transmission_with`, nested.rs:682-713).
-// Parity pins: tests/test_oracle.py checks it against dense inversion of z - H - Sigma on small
+// Parity pins: tests/test_host.py checks it against dense inversion of z - H - Sigma on small
 // systems (nested.rs:1-7 defines the same quantity), clean-ribbon channel counts, the chain
 // closed forms and the reference's golden data.
+//
+// The scalar type is ORACLE_REAL (double by default). Built with ORACLE_REAL=long double
+// (oracle/_ref/liboracle_ld.so) it computes the same elimination with a 64-bit mantissa: the
+// tests use it to measure the rounding floor of the double-precision results on large systems.
 #include <algorithm>
 #include <atomic>
 #include <climits>
@@ -32,7 +36,11 @@
 
 namespace {
 
-using cx = std::complex<double>;
+#ifndef ORACLE_REAL
+#define ORACLE_REAL double
+#endif
+using real = ORACLE_REAL;
+using cx = std::complex<real>;
 
 struct Mat {
     int r = 0, c = 0;
@@ -187,15 +195,15 @@ struct Front {
 // nested.rs:1167-1223; returns false when a pivot fails the test.
 bool small_schur(const Mat& f, int ne, bool want_x, Mat& s, Mat& x) {
     const int n = f.r, nb = n - ne;
-    double scale = 0.0;
+    real scale = 0.0;
     for (int j = 0; j < ne; ++j)
         for (int i = 0; i < ne; ++i) scale = std::max(scale, std::abs(f(i, j)));
     Mat w = f;
     for (int k = 0; k < ne; ++k) {
         int p = k;
-        double big = -1.0;
+        real big = -1.0;
         for (int i = k; i < ne; ++i) {
-            double m = std::abs(w(i, k));
+            real m = std::abs(w(i, k));
             if (m > big) {
                 big = m;
                 p = i;
@@ -204,7 +212,7 @@ bool small_schur(const Mat& f, int ne, bool want_x, Mat& s, Mat& x) {
         if (!(big > kTol * scale)) return false;
         if (p != k)
             for (int j = 0; j < n; ++j) std::swap(w(p, j), w(k, j));
-        cx inv = 1.0 / w(k, k);
+        cx inv = real(1) / w(k, k);
         for (int i = k + 1; i < n; ++i) w(i, k) *= inv;
         for (int j = k + 1; j < n; ++j) {
             cx u = w(k, j);
@@ -235,7 +243,7 @@ void lu(Mat& a, std::vector<int>& piv) {
     piv.resize(n);
     for (int k = 0; k < n; ++k) {
         int p = k;
-        double big = -1.0;
+        real big = -1.0;
         for (int i = k; i < n; ++i)
             if (std::abs(a(i, k)) > big) {
                 big = std::abs(a(i, k));
@@ -245,7 +253,7 @@ void lu(Mat& a, std::vector<int>& piv) {
         if (p != k)
             for (int j = 0; j < n; ++j) std::swap(a(p, j), a(k, j));
         if (a(k, k) == cx(0.0)) continue;
-        cx inv = 1.0 / a(k, k);
+        cx inv = real(1) / a(k, k);
         for (int i = k + 1; i < n; ++i) a(i, k) *= inv;
         for (int j = k + 1; j < n; ++j) {
             cx u = a(k, j);
@@ -273,7 +281,7 @@ void lu_solve(const Mat& a, const std::vector<int>& piv, Mat& b) {
 bool schur(const Mat& f, int ne, Mat& s, Mat& x) {
     const int n = f.r, nb = n - ne;
     Mat f11(ne, ne);
-    double scale = 0.0;
+    real scale = 0.0;
     for (int j = 0; j < ne; ++j)
         for (int i = 0; i < ne; ++i) {
             f11(i, j) = f(i, j);
@@ -281,7 +289,7 @@ bool schur(const Mat& f, int ne, Mat& s, Mat& x) {
         }
     std::vector<int> piv;
     lu(f11, piv);
-    double small = INFINITY;
+    real small = INFINITY;
     for (int i = 0; i < ne; ++i) small = std::min(small, std::abs(f11(i, i)));
     if (!(small > kTol * scale)) return false;
     x = Mat(ne, nb);
@@ -450,9 +458,9 @@ Mat mul(const Mat& a, const Mat& b) {
 }
 
 // lib.rs:108-113
-double caroli_trace(const Mat& gp, const Mat& g, const Mat& gq) {
+real caroli_trace(const Mat& gp, const Mat& g, const Mat& gq) {
     Mat a = mul(gp, g), b = mul(g, gq);
-    double t = 0.0;
+    real t = 0.0;
     for (size_t e = 0; e < a.a.size(); ++e) t += (a.a[e] * std::conj(b.a[e])).real();
     return t;
 }
@@ -496,6 +504,13 @@ void symbolic(const Csr& h, std::vector<ONode>& nodes) {
         keep.erase(std::unique(keep.begin(), keep.end()), keep.end());
         nodes[k].bnd = keep;
     }
+}
+
+// Interleaved (re, im) doubles of the C interface -> the oracle's complex scalars.
+std::vector<cx> to_cx(const double* v, size_t words) {
+    std::vector<cx> out(words);
+    for (size_t e = 0; e < words; ++e) out[e] = cx(v[2 * e], v[2 * e + 1]);
+    return out;
 }
 
 }  // namespace
@@ -618,9 +633,12 @@ int32_t oracle_node(void* h, int32_t b, int32_t k, uint64_t* ne, uint64_t* nb, u
 // concatenated (Sigma_L then Sigma_R, column-major complex). 0 = ok, 1 = Singular.
 int32_t oracle_transmission_with(void* h, double energy, const double* sigmas, double* out_t) {
     auto* sys = static_cast<OSystem*>(h);
-    const cx* s = reinterpret_cast<const cx*>(sigmas);
+    size_t words = 0;
+    for (auto& blk : sys->blocks) words += blk.left.size() * blk.left.size() + blk.right.size() * blk.right.size();
+    const std::vector<cx> sig = to_cx(sigmas, words);
+    const cx* s = sig.data();
     cx z(energy, 0.0);
-    double total = 0.0;
+    real total = 0.0;
     for (auto& blk : sys->blocks) {
         int nl = int(blk.left.size()), nr = int(blk.right.size());
         const cx* sl = s;
@@ -633,7 +651,7 @@ int32_t oracle_transmission_with(void* h, double energy, const double* sigmas, d
         }
         total += caroli_trace(broadening(sl, nl), g, broadening(sr, nr));
     }
-    *out_t = total;
+    *out_t = double(total);
     return 0;
 }
 
@@ -644,8 +662,9 @@ int32_t oracle_green_columns(void* h, int32_t b, double energy, const double* si
     auto* sys = static_cast<OSystem*>(h);
     auto& blk = sys->blocks.at(b);
     RootOut root;
-    if (root_with_leads(*sys, blk, cx(energy, 0.0), reinterpret_cast<const cx*>(sigma_l), reinterpret_cast<const cx*>(sigma_r), true, root))
-        return 1;
+    const std::vector<cx> sl = to_cx(sigma_l, blk.left.size() * blk.left.size());
+    const std::vector<cx> sr = to_cx(sigma_r, blk.right.size() * blk.right.size());
+    if (root_with_leads(*sys, blk, cx(energy, 0.0), sl.data(), sr.data(), true, root)) return 1;
     const int k = root.s.r;
     const int m = which ? int(blk.right.size()) : int(blk.left.size());
     const int at = which ? blk.right_at : 0;
@@ -658,7 +677,10 @@ int32_t oracle_green_columns(void* h, int32_t b, double energy, const double* si
     for (size_t a = 0; a < blk.kept.size(); ++a)
         for (int c = 0; c < m; ++c) psi(int(blk.kept[a]), c) = x(int(a), c);
     back_substitute(root.factor, psi);
-    std::memcpy(out, psi.a.data(), psi.a.size() * sizeof(cx));
+    for (size_t e = 0; e < psi.a.size(); ++e) {
+        out[2 * e] = double(psi.a[e].real());
+        out[2 * e + 1] = double(psi.a[e].imag());
+    }
     return 0;
 }
 

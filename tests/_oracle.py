@@ -12,15 +12,16 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB = os.path.join(ROOT, "oracle", "_ref", "liboracle.so")
+LIB_LD = os.path.join(ROOT, "oracle", "_ref", "liboracle_ld.so")
 
 
-def _load():
-    if not os.path.exists(LIB):
+def _load(path=LIB):
+    if not os.path.exists(path):
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import build as oracle_build  # noqa: E402
 
         oracle_build.build()
-    lib = C.CDLL(LIB)
+    lib = C.CDLL(path)
     P = C.POINTER
     lib.oracle_create.restype = C.c_void_p
     lib.oracle_create.argtypes = [C.c_uint64, P(C.c_int64), P(C.c_uint32), P(C.c_double), C.c_uint64, P(C.c_int64),
@@ -41,6 +42,15 @@ def _load():
 
 
 lib = _load()
+_lib_ld = None
+
+
+def _ld():
+    """The long-double build of the oracle (loaded on first use)."""
+    global _lib_ld
+    if _lib_ld is None:
+        _lib_ld = _load(LIB_LD)
+    return _lib_ld
 
 
 def _d(a):
@@ -50,7 +60,9 @@ def _d(a):
 class OracleSystem:
     """The oracle's copy of a system, built from the arrays of the engine's system (or any arrays)."""
 
-    def __init__(self, arrays, transpose=False):
+    def __init__(self, arrays, transpose=False, extended=False):
+        """extended: the long-double build (64-bit mantissa) instead of the double one."""
+        self.lib = _ld() if extended else lib
         a = dict(arrays)
         vals = np.ascontiguousarray(a["vals"], np.complex128)
         mats = [np.asfortranarray(a[k], np.complex128) for k in ("left_h00", "left_h01", "right_h00", "right_h01")]
@@ -65,7 +77,7 @@ class OracleSystem:
                       np.ascontiguousarray(a["site_offsets"], np.uint64), np.ascontiguousarray(a["range"], np.int64),
                       *mats]
         k = self._keep
-        self.h = lib.oracle_create(self.n, k[0].ctypes.data_as(C.POINTER(C.c_int64)),
+        self.h = self.lib.oracle_create(self.n, k[0].ctypes.data_as(C.POINTER(C.c_int64)),
                                    k[1].ctypes.data_as(C.POINTER(C.c_uint32)), _d(k[2]), len(k[4]),
                                    k[3].ctypes.data_as(C.POINTER(C.c_int64)), k[4].ctypes.data_as(C.POINTER(C.c_int32)),
                                    k[5].ctypes.data_as(C.POINTER(C.c_uint64)), int(a["dim"]),
@@ -76,27 +88,27 @@ class OracleSystem:
 
     def __del__(self):
         if getattr(self, "h", None):
-            lib.oracle_destroy(self.h)
+            self.lib.oracle_destroy(self.h)
             self.h = None
 
     def block_count(self):
-        return lib.oracle_block_count(self.h)
+        return self.lib.oracle_block_count(self.h)
 
     def block(self, b):
         nl, nr, nn = C.c_uint64(), C.c_uint64(), C.c_uint64()
-        lib.oracle_block_info(self.h, b, C.byref(nl), C.byref(nr), C.byref(nn), None, None)
+        self.lib.oracle_block_info(self.h, b, C.byref(nl), C.byref(nr), C.byref(nn), None, None)
         left = np.zeros(nl.value, np.uint64)
         right = np.zeros(nr.value, np.uint64)
-        lib.oracle_block_info(self.h, b, C.byref(nl), C.byref(nr), C.byref(nn),
+        self.lib.oracle_block_info(self.h, b, C.byref(nl), C.byref(nr), C.byref(nn),
                               left.ctypes.data_as(C.POINTER(C.c_uint64)), right.ctypes.data_as(C.POINTER(C.c_uint64)))
         return left, right, nn.value
 
     def node(self, b, k):
         ne, nb = C.c_uint64(), C.c_uint64()
-        lib.oracle_node(self.h, b, k, C.byref(ne), C.byref(nb), None, None)
+        self.lib.oracle_node(self.h, b, k, C.byref(ne), C.byref(nb), None, None)
         e = np.zeros(ne.value, np.uint64)
         d = np.zeros(nb.value, np.uint64)
-        lib.oracle_node(self.h, b, k, C.byref(ne), C.byref(nb), e.ctypes.data_as(C.POINTER(C.c_uint64)),
+        self.lib.oracle_node(self.h, b, k, C.byref(ne), C.byref(nb), e.ctypes.data_as(C.POINTER(C.c_uint64)),
                         d.ctypes.data_as(C.POINTER(C.c_uint64)))
         return e, d
 
@@ -109,7 +121,7 @@ class OracleSystem:
     def transmission_with(self, energy, sigmas):
         flat = self.pack(sigmas)
         t = C.c_double()
-        rc = lib.oracle_transmission_with(self.h, float(energy), _d(flat), C.byref(t))
+        rc = self.lib.oracle_transmission_with(self.h, float(energy), _d(flat), C.byref(t))
         return t.value, rc
 
     def green_columns(self, b, energy, sigma_l, sigma_r, which):
@@ -119,7 +131,7 @@ class OracleSystem:
         out = np.zeros((self.n, m), np.complex128, order="F")
         sl = np.asfortranarray(sigma_l, np.complex128)
         sr = np.asfortranarray(sigma_r, np.complex128)
-        rc = lib.oracle_green_columns(self.h, b, float(energy), _d(sl), _d(sr), which, _d(out))
+        rc = self.lib.oracle_green_columns(self.h, b, float(energy), _d(sl), _d(sr), which, _d(out))
         if rc:
             raise RuntimeError("oracle: singular root")
         return out

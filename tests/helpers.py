@@ -22,6 +22,16 @@ def oracle_of(sys_):
     return OracleSystem(sys_.arrays())
 
 
+def rebuilt(arrays, **changes):
+    """A system created through cg_system_create (NestedTwoTerminal.from_arrays) from the arrays of
+    another system (`arrays()`), with some of them replaced."""
+    a = dict(arrays)
+    a.update(changes)
+    return ca.NestedTwoTerminal.from_arrays(a["n"], a["row_ptr"], a["cols"], a["vals"], a["cells"],
+                                            a["site_orbitals"], a["site_offsets"], a["dim"], a["range"], a["axis"],
+                                            a["left_h00"], a["left_h01"], a["right_h00"], a["right_h01"])
+
+
 def compare(sys_, energies, label, precision=ca.Precision.Double, tol=TOL):
     """One batched launch over `energies` against the oracle's T, to tol (1 + max |T|)."""
     o = oracle_of(sys_)

@@ -14,7 +14,7 @@ import pytest
 import caroli_amd as ca
 from caroli_amd import _capi as capi
 from _oracle import OracleSystem, dense_transmission
-from helpers import hashed, sigmas_of
+from helpers import hashed, rebuilt, sigmas_of
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
@@ -329,6 +329,74 @@ def test_invalid_arguments_are_refused():
     assert capi.cg_scattering_states(other._h, sig._h, cols, None, None) == 6  # foreign self-energies
 
 
+@pytest.mark.parametrize("model,ext,kw", [
+    (ca.Model.BHZ, [20, 8, 0], dict(disorder=2, strength=0.4, concentration=0.2, seed=3)),
+    (ca.Model.Graphene, [12, 6, 0], dict(disorder=1, strength=1.0, seed=2)),
+    (ca.Model.Cubic, [6, 4, 4], dict(disorder=1, strength=16.5, seed=1000)),
+])
+def test_system_create_round_trip(model, ext, kw):
+    """preset -> cg_system_arrays -> cg_system_create gives the same system: the same arrays, the
+    same elimination trees node by node and bit-identical self-energies."""
+    s = ca.NestedTwoTerminal.preset(model, ext, **kw)
+    arr = s.arrays()
+    r = rebuilt(arr)
+    arr2 = r.arrays()
+    for k, v in arr.items():
+        assert np.array_equal(np.asarray(v), np.asarray(arr2[k])), k
+    assert r.block_count() == s.block_count()
+    for b in range(s.block_count()):
+        assert r.block_sizes(b) == s.block_sizes(b)
+        for k in range(s.block_sizes(b)[2]):
+            e1, d1 = s.tree_node(b, k)
+            e2, d2 = r.tree_node(b, k)
+            assert np.array_equal(e1, e2) and np.array_equal(d1, d2), (b, k)
+    g1, g2 = s.self_energies(0.07), r.self_energies(0.07)
+    for b in range(s.block_count()):
+        for m1, m2 in zip(g1.block(b), g2.block(b)):
+            assert np.array_equal(np.asarray(m1), np.asarray(m2))
+    # Self-energies of one are accepted by the other (same leads and blocks; the channel counts of
+    # the states need no device).
+    cols = (C.c_int64 * 2)()
+    assert capi.cg_scattering_states(r._h, g1._h, cols, None, None) == 0
+
+
+def test_system_create_refuses_malformed_arrays():
+    """cg_system_create refuses malformed CSR and geometry with CG_ERR_INVALID (6)."""
+    s = ca.NestedTwoTerminal.preset(ca.Model.Square, [6, 4, 0])
+    arr = s.arrays()
+    n = arr["n"]
+
+    def refused(**changes):
+        with pytest.raises(ca.GreenError) as err:
+            rebuilt(arr, **changes)
+        assert err.value.code == 6, changes
+
+    rp = arr["row_ptr"].copy()
+    rp[0] = 1
+    refused(row_ptr=rp)                                  # CSR not starting at 0
+    rp = arr["row_ptr"].copy()
+    rp[3], rp[4] = rp[4], rp[3]
+    refused(row_ptr=rp)                                  # decreasing row pointers
+    cl = arr["cols"].copy()
+    cl[0] = n
+    refused(cols=cl)                                     # column out of range
+    so = arr["site_orbitals"].copy()
+    so[2] = -1
+    refused(site_orbitals=so)                            # negative orbital count
+    so = arr["site_orbitals"].copy()
+    so[2] = 0
+    refused(site_orbitals=so)                            # empty site
+    sf = arr["site_offsets"].copy()
+    sf[3] = sf[2]
+    refused(site_offsets=sf)                             # overlapping sites (one orbital uncovered)
+    sf = arr["site_offsets"].copy()
+    sf[-1] = n
+    refused(site_offsets=sf)                             # a site past the last orbital
+    refused(left_h00=arr["left_h00"][:-1, :-1], left_h01=arr["left_h01"][:-1, :-1])  # lead cell != end slice
+    refused(axis=2)                                      # axis outside the lattice dimension
+    assert rebuilt(arr).sizes() == s.sizes()             # the untouched arrays are accepted
+
+
 def test_tuning_defaults_variables_and_override(monkeypatch):
     """cg_get_tuning / cg_set_tuning: the defaults, the CAROLI_* variables, an override over them,
     refused values, and the way back (no device needed)."""
@@ -468,3 +536,36 @@ def test_oracle_gradient_matches_finite_differences():
                                                       site_potential=-pot).arrays()).transmission_with(e, sig_arr)[0]
         fd = (tp - tm) / (2 * h)
         assert abs(fd - grad[4 * i:4 * i + 4].sum()) < 1e-6 * max(1.0, abs(fd)), (i, fd)
+
+
+# -- checked reduced precision (host logic of cg_batch_transmission_checked) ----------------------
+
+def test_checked_batch_sampling_failures_and_fallback(tmp_path):
+    """caroli_amd/csrc/checked.hpp with the reference's fake runner (precision.rs:140-168): index 7
+    off by 1e-4 in single precision, index 3 failing there."""
+    import subprocess
+    exe = str(tmp_path / "checked_driver")
+    subprocess.run(["g++", "-std=c++17", "-O1", os.path.join(ROOT, "tests", "native", "checked_driver.cpp"), "-o",
+                    exe], check=True)
+    lines = subprocess.run([exe], check=True, capture_output=True, text=True).stdout.split("\n")
+    rows = [[float(x) for x in ln.split()] for ln in lines if ln.strip()]
+    exact = list(range(10))
+    # every 5: index 7 not sampled (kept), index 3 failed (redone in double), 0 and 5 compared.
+    rc, checked, failed, fell, prec, *rest = rows[0]
+    t, dt = rest[:10], rest[10:20]
+    assert (rc, checked, failed, fell, prec) == (0, 2, 1, 0, 1)
+    assert t[3] == 3.0 and t[7] == 7.0 + 1e-4
+    assert dt[0] == 0.0 and dt[5] == 0.0 and np.isnan(dt[7]) and np.isnan(dt[3])
+    # every 7: index 7 sampled, 1e-4 above 1e-6: everything redone in double.
+    rc, checked, failed, fell, prec, *rest = rows[1]
+    assert (rc, fell, prec) == (0, 1, 0) and rest[:10] == exact
+    assert rest[10 + 7] == pytest.approx(1e-4) and rest[21] == pytest.approx(1e-4)  # dt[7], max
+    # double precision: nothing checked.
+    rc, checked, failed, fell, prec, *rest = rows[2]
+    assert (rc, checked, failed, fell, prec) == (0, 0, 0, 0, 0) and rest[:10] == exact
+    # every point checked (the engine's report), tolerance above the error: no fallback; the
+    # checked points return their double values, the differences are reported per point.
+    rc, checked, failed, fell, prec, *rest = rows[3]
+    assert (rc, checked, failed, fell, prec) == (0, 9, 1, 0, 1) and rest[:10] == exact
+    dt = np.array(rest[10:20])
+    assert dt[7] == pytest.approx(1e-4) and np.isnan(dt[3]) and np.nansum(dt) == pytest.approx(1e-4)
