@@ -26,7 +26,8 @@ __all__ = [
 class GreenError(RuntimeError):
     """GreenError of crates/caroli-green/src/lib.rs:35-47, by status code."""
 
-    KINDS = {1: "Singular", 2: "Eigen", 3: "ModeCount", 4: "NoConvergence", 6: "Invalid", 7: "Device"}
+    KINDS = {1: "Singular", 2: "Eigen", 3: "ModeCount", 4: "NoConvergence", 6: "Invalid", 7: "Device",
+             8: "Capacity"}
 
     def __init__(self, code, what=""):
         self.code = int(code)

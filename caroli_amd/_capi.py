@@ -112,7 +112,9 @@ cg_kernel_timing = _sig("cg_kernel_timing", C.c_int32, C.c_int32)
 cg_kernel_stats = _sig("cg_kernel_stats", C.c_int32, P(KernelStat), C.c_int32)
 cg_front_schur = _sig("cg_front_schur", C.c_int32, P(C.c_double), C.c_int64, C.c_int64, C.c_int64, C.c_int32,
                       P(C.c_double), P(C.c_int32))
-EXPORTS += ["cg_front_schur"]
+cg_front_schur_noalloc = _sig("cg_front_schur_noalloc", C.c_int32, P(C.c_double), C.c_int64, C.c_int64, C.c_int64,
+                              C.c_int32, P(C.c_double), P(C.c_int32))
+EXPORTS += ["cg_front_schur", "cg_front_schur_noalloc"]
 cg_uniforms = _sig("cg_uniforms", C.c_int32, C.c_int32, C.c_uint64, P(C.c_int32), C.c_int64, P(C.c_double))
 EXPORTS += ["cg_batch_create", "cg_batch_run", "cg_batch_destroy", "cg_kernel_timing", "cg_kernel_stats",
             "cg_uniforms"]

@@ -477,6 +477,19 @@ int32_t cg_front_schur(const double* f, int64_t n, int64_t ne, int64_t ne1, int3
     }
 }
 
+int32_t cg_front_schur_noalloc(const double* f, int64_t n, int64_t ne, int64_t ne1, int32_t precision, double* out_s,
+                               int32_t* out_status) {
+    if (!f || !out_status || n <= 0 || ne < 0 || ne > n || ne1 < 0 || ne1 > ne || (ne < n && !out_s) ||
+        (precision != CG_DOUBLE && precision != CG_SINGLE))
+        return CG_ERR_INVALID;
+    if (n > CG_FRONT_NOALLOC_MAX_N) return CG_ERR_CAPACITY;
+    try {
+        return gpu::front_schur_noalloc(f, n, ne, ne1, precision, out_s, out_status);
+    } catch (const std::exception&) {
+        return CG_ERR_DEVICE;
+    }
+}
+
 int32_t cg_kernel_timing(int32_t enable) {
     try {
         return gpu::kernel_timing(enable);

@@ -290,19 +290,22 @@ SetOffsets append_set(const LevelSet& L, std::vector<int32_t>& lv, std::vector<T
     tl.insert(tl.end(), L.utiles.begin(), L.utiles.end());
     return o;
 }
-DevLevel resolve(const SetOffsets& o, const DevBuf& lv, const DevBuf& tl) {
+DevLevel resolve(const SetOffsets& o, const int32_t* lv, const Tile* tl) {
     DevLevel d;
-    for (int c = 0; c < LevelSet::kClasses; ++c) d.lds[c] = lv.as<int32_t>() + o.lds[c];
-    d.large = lv.as<int32_t>() + o.large;
-    d.tiles = tl.as<Tile>() + o.tiles;
-    d.gtiles = tl.as<Tile>() + o.gtiles;
-    d.f11 = lv.as<int32_t>() + o.f11;
-    d.big = lv.as<int32_t>() + o.big;
-    d.ttiles = tl.as<Tile>() + o.ttiles;
-    d.blk = lv.as<int32_t>() + o.blk;
-    d.rtiles = tl.as<Tile>() + o.rtiles;
-    d.utiles = tl.as<Tile>() + o.utiles;
+    for (int c = 0; c < LevelSet::kClasses; ++c) d.lds[c] = lv + o.lds[c];
+    d.large = lv + o.large;
+    d.tiles = tl + o.tiles;
+    d.gtiles = tl + o.gtiles;
+    d.f11 = lv + o.f11;
+    d.big = lv + o.big;
+    d.ttiles = tl + o.ttiles;
+    d.blk = lv + o.blk;
+    d.rtiles = tl + o.rtiles;
+    d.utiles = tl + o.utiles;
     return d;
+}
+DevLevel resolve(const SetOffsets& o, const DevBuf& lv, const DevBuf& tl) {
+    return resolve(o, lv.as<int32_t>(), tl.as<Tile>());
 }
 
 /// Descriptors of one block's fronts (tree nodes, then the root), in host memory.
@@ -1713,20 +1716,97 @@ int32_t autotune_batch(cg_batch* bt, cg_tuning* out, double* out_seconds) {
     return CG_OK;
 }
 
+namespace {
+
+/// One front update of cg_front_schur: the front's descriptor, the kernel path its size takes
+/// under the tuning in force, and the host copies of its lists. `lv` holds the front's index (0)
+/// first; `tl` the Schur tiles first, then the triangular-solve or blocked tiles.
+struct FrontJob {
+    enum Path { kSmall, kUnblocked, kBlocked, kSplit, kBorder } path = kSmall;
+    NodeDesc d{};
+    std::vector<int32_t> lv{0};
+    std::vector<Tile> tl;
+    int32_t schur_tiles = 0, trsm_tiles = 0;
+    LevelSet blocked;
+    SetOffsets offsets{};
+};
+
+FrontJob plan_front(int64_t n, int64_t ne, int64_t ne1, bool single) {
+    FrontJob j;
+    j.d.off = 0;
+    j.d.n = int32_t(n);
+    j.d.ne = int32_t(ne);
+    j.d.ne1 = int32_t(ne1);
+    j.d.slot = 0;
+    j.d.piv_off = 0;
+    if (n <= (single ? kLdsFrontC64 : kLdsFrontC128)) {
+        j.path = FrontJob::kSmall;
+        return j;
+    }
+    if (unblocked_fronts()) {
+        j.path = FrontJob::kUnblocked;
+        return j;
+    }
+    for (int i0 = 0; i0 < n - ne; i0 += kTile)
+        for (int j0 = 0; j0 < n - ne; j0 += kTile) j.tl.push_back({0, i0, j0, 0});
+    j.schur_tiles = int32_t(j.tl.size());
+    if (ne > blocked_min_ne()) {
+        j.path = FrontJob::kBlocked;
+        j.blocked.blk = {0};
+        j.blocked.plan_blocked({j.d});
+        j.offsets = append_set(j.blocked, j.lv, j.tl);
+    } else if (ne <= kF11Lds && !border_only()) {
+        j.path = FrontJob::kSplit;
+        for (int o = 0; o < n - ne; o += kTrsmSpan) {
+            j.tl.push_back({0, 0, o, 0});
+            j.tl.push_back({0, o, 0, 1});
+        }
+        j.trsm_tiles = int32_t(j.tl.size()) - j.schur_tiles;
+    } else {
+        j.path = FrontJob::kBorder;
+    }
+    return j;
+}
+
+/// The kernels of a planned front update on the front in pv.arena, with the device copies of its
+/// descriptor (`nodes`), lists (`lv`) and tiles (`tl`).
+void launch_front_job(Engine& eng, const FrontJob& j, const NodeDesc* nodes, const int32_t* lv, const Tile* tl,
+                      const PointsView& pv, bool single) {
+    const double tol = single ? kTolSingle : kTolDouble;
+    const int32_t n = j.d.n, ne = j.d.ne;
+    hipStream_t st = eng.lane->stream;
+    switch (j.path) {
+        case FrontJob::kSmall:
+            launch_front_lds(nodes, lv, 1, n, border_lds_words(n, ne), nullptr, nullptr, nullptr, nullptr, pv, 1, tol,
+                             single, false, st);
+            return;
+        case FrontJob::kUnblocked:
+            launch_factor_unblocked(nodes, lv, 1, pv, 1, tol, single, false, st);
+            return;
+        case FrontJob::kBlocked:
+            run_blocked(eng, j.blocked, resolve(j.offsets, lv, tl), nodes, nullptr, nullptr, pv, 1, tol, single, false);
+            break;
+        case FrontJob::kSplit:
+            launch_f11(nodes, lv, 1, ne, nullptr, nullptr, pv, 1, tol, single, st);
+            launch_trsm(nodes, tl + j.schur_tiles, j.trsm_tiles, ne, pv, 1, single, st);
+            break;
+        case FrontJob::kBorder:
+            launch_border(nodes, lv, 1, ne, nullptr, nullptr, pv, 1, tol, single, false, st);
+            break;
+    }
+    launch_schur(nodes, tl, j.schur_tiles, nullptr, nullptr, pv, 1, single, false, false, st);
+}
+
+}  // namespace
+
 int32_t front_schur(const double* f, int64_t n, int64_t ne, int64_t ne1, int32_t precision, double* out_s,
                     int32_t* out_status) {
     Engine& eng = Engine::get();
     if (!eng.ok) return CG_ERR_DEVICE;
     std::lock_guard<std::mutex> lock(eng.mu);
     const bool single = precision == CG_SINGLE;
-    NodeDesc d{};
-    d.off = 0;
-    d.n = int32_t(n);
-    d.ne = int32_t(ne);
-    d.ne1 = int32_t(ne1);
-    d.slot = 0;
-    d.piv_off = 0;
-    DevBuf fb, nodes, lv, pivb, stb, tiles, ttiles, blists, btiles;
+    const FrontJob job = plan_front(n, ne, ne1, single);
+    DevBuf fb, nodes, lv, tl, pivb, stb;
     const size_t nn = size_t(n) * size_t(n);
     if (single) {
         std::vector<float2> h(nn);
@@ -1737,8 +1817,9 @@ int32_t front_schur(const double* f, int64_t n, int64_t ne, int64_t ne1, int32_t
         for (size_t e = 0; e < nn; ++e) h[e] = double2{f[2 * e], f[2 * e + 1]};
         upload(fb, h, eng.lane->stream);
     }
-    upload(nodes, std::vector<NodeDesc>{d}, eng.lane->stream);
-    upload(lv, std::vector<int32_t>{0}, eng.lane->stream);
+    upload(nodes, std::vector<NodeDesc>{job.d}, eng.lane->stream);
+    upload(lv, job.lv, eng.lane->stream);
+    upload(tl, job.tl, eng.lane->stream);
     pivb.ensure(size_t(ne + kPivExtra) * 4);
     stb.ensure(4);
     PointsView pv{};
@@ -1747,46 +1828,7 @@ int32_t front_schur(const double* f, int64_t n, int64_t ne, int64_t ne1, int32_t
     pv.status = stb.as<int32_t>();
     pv.status_stride = 1;
     pv.keep = 1;
-    const bool small = n <= (single ? kLdsFrontC64 : kLdsFrontC128);
-    const double tol = single ? kTolSingle : kTolDouble;
-    if (small) {
-        launch_front_lds(nodes.as<NodeDesc>(), lv.as<int32_t>(), 1, int32_t(n), border_lds_words(int32_t(n), int32_t(ne)),
-                         nullptr, nullptr, nullptr, nullptr, pv, 1, tol, single, false, eng.lane->stream);
-    } else if (unblocked_fronts()) {
-        launch_factor_unblocked(nodes.as<NodeDesc>(), lv.as<int32_t>(), 1, pv, 1, tol, single, false, eng.lane->stream);
-    } else {
-        std::vector<Tile> tl, tt;
-        for (int i0 = 0; i0 < n - ne; i0 += kTile)
-            for (int j0 = 0; j0 < n - ne; j0 += kTile) tl.push_back({0, i0, j0, 0});
-        upload(tiles, tl, eng.lane->stream);
-        if (ne > blocked_min_ne()) {
-            LevelSet L;
-            L.blk = {0};
-            L.plan_blocked({d});
-            std::vector<int32_t> lvb;
-            std::vector<Tile> tlb;
-            const SetOffsets o = append_set(L, lvb, tlb);
-            upload(blists, lvb, eng.lane->stream);
-            upload(btiles, tlb, eng.lane->stream);
-            run_blocked(eng, L, resolve(o, blists, btiles), nodes.as<NodeDesc>(), nullptr, nullptr, pv, 1, tol, single,
-                        false);
-        } else if (ne <= kF11Lds && !border_only()) {
-            for (int o = 0; o < n - ne; o += kTrsmSpan) {
-                tt.push_back({0, 0, o, 0});
-                tt.push_back({0, o, 0, 1});
-            }
-            upload(ttiles, tt, eng.lane->stream);
-            launch_f11(nodes.as<NodeDesc>(), lv.as<int32_t>(), 1, int32_t(ne), nullptr, nullptr, pv, 1, tol, single,
-                       eng.lane->stream);
-            launch_trsm(nodes.as<NodeDesc>(), ttiles.as<Tile>(), int32_t(tt.size()), int32_t(ne), pv, 1, single,
-                        eng.lane->stream);
-        } else {
-            launch_border(nodes.as<NodeDesc>(), lv.as<int32_t>(), 1, int32_t(ne), nullptr, nullptr, pv, 1, tol, single,
-                          false, eng.lane->stream);
-        }
-        launch_schur(nodes.as<NodeDesc>(), tiles.as<Tile>(), int32_t(tl.size()), nullptr, nullptr, pv, 1, single, false,
-                     false, eng.lane->stream);
-    }
+    launch_front_job(eng, job, nodes.as<NodeDesc>(), lv.as<int32_t>(), tl.as<Tile>(), pv, single);
     HIP_CHECK(hipGetLastError());
     int32_t st = 0;
     HIP_CHECK(hipMemcpyAsync(&st, stb.p, 4, hipMemcpyDeviceToHost, eng.lane->stream));
@@ -1813,6 +1855,134 @@ int32_t front_schur(const double* f, int64_t n, int64_t ne, int64_t ne1, int32_t
             }
     }
     *out_status = st;
+    return CG_OK;
+}
+
+// ---- the allocation-free front update -----------------------------------------------------------
+//
+// cg_front_schur_noalloc works in one file-scope device array, `scratch`, that the GPU holds from
+// the library's load to the end of the process (one copy per GPU, at its own address):
+//   [0, kNoAllocFront)                 the front in its working precision (n <= kNoAllocMaxN)
+//   [kNoAllocFront, 2 kNoAllocFront)   complex f64 staging (complex f32 fronts: input and S)
+//   [2 kNoAllocFront, + kNoAllocDesc)  descriptor, pivots, status, front list, tiles
+// Every call on a GPU uses the whole array: calls that overlap on one GPU race.
+
+constexpr size_t kNoAllocFront = size_t(kNoAllocMaxN) * size_t(kNoAllocMaxN) * sizeof(double2);
+constexpr size_t kNoAllocDesc = size_t(1) << 20;
+constexpr size_t kNoAllocBytes = 2 * kNoAllocFront + kNoAllocDesc;
+
+static_assert(kNoAllocBytes == CG_FRONT_NOALLOC_SCRATCH_BYTES, "the header states the work area's size");
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+namespace {
+
+/// Device address of `scratch` on the current GPU, looked up once per GPU (the array's name in
+/// host code is a stand-in, not a device address).
+unsigned char* scratch_on_device() {
+    static std::mutex mu;
+    static std::map<int, unsigned char*> at;
+    int dev = 0;
+    HIP_CHECK(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> g(mu);
+    auto it = at.find(dev);
+    if (it != at.end()) return it->second;
+    void* ptr = nullptr;
+    HIP_CHECK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    if (!ptr) throw DeviceError("hipGetSymbolAddress returned no address for the scratch array");
+    at[dev] = static_cast<unsigned char*>(ptr);
+    return at[dev];
+}
+
+/// Clears the front's status word and, for a complex f32 front, converts the complex f64 input.
+__global__ void noalloc_in_kernel(const double2* __restrict__ src, float2* __restrict__ dst, int64_t count,
+                                  int32_t* __restrict__ status) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i == 0) *status = kFrontOk;
+    for (int64_t e = i; e < count; e += int64_t(gridDim.x) * blockDim.x) dst[e] = float2{float(src[e].x), float(src[e].y)};
+}
+
+/// The trailing nb x nb block of a complex f32 front of order n, as complex f64, column-major.
+__global__ void noalloc_out_kernel(const float2* __restrict__ front, int64_t n, int64_t ne, double2* __restrict__ dst) {
+    const int64_t nb = n - ne, count = nb * nb;
+    for (int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; e < count; e += int64_t(gridDim.x) * blockDim.x) {
+        const int64_t i = e % nb, j = e / nb;
+        const float2 v = front[(ne + i) + (ne + j) * n];
+        dst[e] = double2{double(v.x), double(v.y)};
+    }
+}
+
+int32_t grid_for(int64_t count) { return int32_t(std::max<int64_t>(1, std::min<int64_t>((count + 255) / 256, 2048))); }
+
+size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
+
+}  // namespace
+
+int32_t front_schur_noalloc(const double* f, int64_t n, int64_t ne, int64_t ne1, int32_t precision, double* out_s,
+                            int32_t* out_status) {
+    Engine& eng = Engine::get();
+    if (!eng.ok) return CG_ERR_DEVICE;
+    if (n > kNoAllocMaxN) return CG_ERR_CAPACITY;
+    std::lock_guard<std::mutex> lock(eng.mu);
+    unsigned char* base = scratch_on_device();
+    const bool single = precision == CG_SINGLE;
+    const FrontJob job = plan_front(n, ne, ne1, single);
+    // Descriptor region: node, pivots, status, front list, tiles.
+    const size_t node_at = 0, piv_at = align256(sizeof(NodeDesc)), status_at = piv_at + align256(size_t(ne + kPivExtra) * 4),
+                 lv_at = status_at + 256, tl_at = lv_at + align256(job.lv.size() * sizeof(int32_t)),
+                 desc_end = tl_at + job.tl.size() * sizeof(Tile);
+    if (desc_end > kNoAllocDesc) return CG_ERR_CAPACITY;
+    unsigned char* front = base;
+    double2* stage = reinterpret_cast<double2*>(base + kNoAllocFront);
+    unsigned char* desc = base + 2 * kNoAllocFront;
+    NodeDesc* d_node = reinterpret_cast<NodeDesc*>(desc + node_at);
+    int32_t* d_piv = reinterpret_cast<int32_t*>(desc + piv_at);
+    int32_t* d_status = reinterpret_cast<int32_t*>(desc + status_at);
+    int32_t* d_lv = reinterpret_cast<int32_t*>(desc + lv_at);
+    Tile* d_tl = reinterpret_cast<Tile*>(desc + tl_at);
+    hipStream_t st = eng.lane->stream;
+    // Timing events are not created here (the call allocates nothing): the timers pause.
+    const bool timing = eng.timing;
+    eng.timing = false;
+    struct Restore {
+        Engine& e;
+        bool t;
+        ~Restore() { e.timing = t; }
+    } restore{eng, timing};
+    const int64_t nn = n * n, nb = n - ne;
+    if (single) {
+        HIP_CHECK(hipMemcpyAsync(stage, f, size_t(nn) * sizeof(double2), hipMemcpyHostToDevice, st));
+        noalloc_in_kernel<<<grid_for(nn), 256, 0, st>>>(stage, reinterpret_cast<float2*>(front), nn, d_status);
+    } else {
+        HIP_CHECK(hipMemcpyAsync(front, f, size_t(nn) * sizeof(double2), hipMemcpyHostToDevice, st));
+        noalloc_in_kernel<<<1, 256, 0, st>>>(nullptr, nullptr, 0, d_status);
+    }
+    HIP_CHECK(hipMemcpyAsync(d_node, &job.d, sizeof(NodeDesc), hipMemcpyHostToDevice, st));
+    HIP_CHECK(hipMemcpyAsync(d_lv, job.lv.data(), job.lv.size() * sizeof(int32_t), hipMemcpyHostToDevice, st));
+    if (!job.tl.empty())
+        HIP_CHECK(hipMemcpyAsync(d_tl, job.tl.data(), job.tl.size() * sizeof(Tile), hipMemcpyHostToDevice, st));
+    PointsView pv{};
+    pv.arena = front;
+    pv.piv = d_piv;
+    pv.status = d_status;
+    pv.status_stride = 1;
+    pv.keep = 1;
+    launch_front_job(eng, job, d_node, d_lv, d_tl, pv, single);
+    if (single) {
+        noalloc_out_kernel<<<grid_for(nb * nb), 256, 0, st>>>(reinterpret_cast<const float2*>(front), n, ne, stage);
+        HIP_CHECK(hipGetLastError());
+        HIP_CHECK(hipMemcpyAsync(out_s, stage, size_t(nb * nb) * sizeof(double2), hipMemcpyDeviceToHost, st));
+    } else {
+        HIP_CHECK(hipGetLastError());
+        if (nb > 0)
+            HIP_CHECK(hipMemcpy2DAsync(out_s, size_t(nb) * sizeof(double2),
+                                       front + size_t(ne + ne * n) * sizeof(double2), size_t(n) * sizeof(double2),
+                                       size_t(nb) * sizeof(double2), size_t(nb), hipMemcpyDeviceToHost, st));
+    }
+    int32_t status = 0;
+    HIP_CHECK(hipMemcpyAsync(&status, d_status, 4, hipMemcpyDeviceToHost, st));
+    HIP_CHECK(hipStreamSynchronize(st));
+    *out_status = status;
     return CG_OK;
 }
 

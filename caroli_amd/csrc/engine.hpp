@@ -23,6 +23,10 @@ int32_t kernel_timing(int32_t enable);
 int32_t front_schur(const double* f, int64_t n, int64_t ne, int64_t ne1, int32_t precision, double* out_s,
                     int32_t* out_status);
 int32_t kernel_stats(cg_kernel_stat* out, int32_t max_records);
+/// front_schur with its work area in a static device array (no device allocation).
+constexpr int64_t kNoAllocMaxN = CG_FRONT_NOALLOC_MAX_N;
+int32_t front_schur_noalloc(const double* f, int64_t n, int64_t ne, int64_t ne1, int32_t precision, double* out_s,
+                            int32_t* out_status);
 
 /// T and dT/d(eps_a) for every orbital, from the stored fronts of one point.
 int32_t backward(const cg_system* s, const cg_sigma* g, double* out_t, double* out_grad);

@@ -34,7 +34,8 @@ enum {
   CG_ERR_MODE_COUNT = 3,     /* GreenError::ModeCount */
   CG_ERR_NO_CONVERGENCE = 4, /* refinement did not converge (mixed precision) */
   CG_ERR_INVALID = 6,        /* malformed arguments (the reference panics, nested.rs:531-534, :694) */
-  CG_ERR_DEVICE = 7          /* HIP error, or no device */
+  CG_ERR_DEVICE = 7,         /* HIP error, or no device */
+  CG_ERR_CAPACITY = 8        /* an input larger than a fixed work area (cg_front_schur_noalloc) */
 };
 
 /* Front precision (crates/caroli-green/src/precision.rs:24-32, plus the refined mixed mode). */
@@ -164,6 +165,20 @@ int32_t cg_batch_transmission_checked(const cg_system* const* systems, const cg_
  * the block is refused (numerically singular: the reference returns None). */
 int32_t cg_front_schur(const double* f, int64_t n, int64_t ne, int64_t ne1, int32_t precision, double* out_s,
                        int32_t* out_status);
+
+/* The front update of cg_front_schur without any device allocation. No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc). Same arguments, results and kernels as
+ * cg_front_schur; its work area (the front, staging, pivots, descriptors and tiles) is one
+ * static device array of CG_FRONT_NOALLOC_SCRATCH_BYTES that each GPU the library runs on holds
+ * until the process ends. It calls no hipMalloc / hipFree. Fronts of more than
+ * CG_FRONT_NOALLOC_MAX_N variables are refused with CG_ERR_CAPACITY. All calls on one GPU share
+ * that GPU's array (each call clears what it needs and keeps the last call's contents): calls
+ * that overlap on one GPU race (calls through this library are serialised by its lock). */
+#define CG_FRONT_NOALLOC_MAX_N 1024
+#define CG_FRONT_NOALLOC_SCRATCH_BYTES (2u * 1024u * 1024u * 16u + (1u << 20))
+int32_t cg_front_schur_noalloc(const double* f, int64_t n, int64_t ne, int64_t ne1, int32_t precision, double* out_s,
+                               int32_t* out_status);
 
 /* A resident batch: the inputs of the points uploaded once to HBM, for repeated launches
  * (serving loops, benchmarks). cg_batch_run computes T for every point from the resident
