@@ -95,6 +95,25 @@ def roofline_of(ks, prec_name, tag):
     return roof
 
 
+def host_threads():
+    """Threads of the CPU baseline: the cores of this process's affinity mask, up to 16 (a one-GPU
+    box's CPU share, whatever nproc reports there)."""
+    try:
+        mask = len(os.sched_getaffinity(0))
+    except AttributeError:
+        mask = os.cpu_count() or 1
+    return max(1, min(mask, 16))
+
+
+def host_cores():
+    """(nproc, cores in the affinity mask) of this host."""
+    try:
+        mask = len(os.sched_getaffinity(0))
+    except AttributeError:
+        mask = None
+    return os.cpu_count(), mask
+
+
 def progress(*a):
     """A progress line on stderr (the JSON line alone goes to stdout)."""
     print("[bench]", *a, file=sys.stderr, flush=True)
@@ -191,7 +210,9 @@ def cpu_baseline(pts_sys, pts_sig, pts_e, t_gpu, args, threads):
     cpu_s = time.perf_counter() - t0
     progress(f"cpu baseline: {cpu_s:.1f} s")
     sample_rate = float(np.sum(flops[idx])) / cpu_s  # flops per second on all threads
+    nproc, mask = host_cores()
     cpu = {"value": sample / cpu_s, "unit": "points/s", "cores": threads, "kind": "port",
+           "host_nproc": nproc, "host_affinity_cores": mask,
            "sample": (f"{sample} of the {npts} points ({'evenly spaced' if fits.size == npts else 'evenly spaced among the ' + str(fits.size) + ' whose estimated single-thread time fits ' + str(args.cpu_seconds) + ' s'}), "
                       "FP64 oracle, one point per thread"),
            "single_thread_s_per_point": lat,
@@ -202,7 +223,7 @@ def cpu_baseline(pts_sys, pts_sig, pts_e, t_gpu, args, threads):
 
 def run_transmission(args, ca):
     prec = {"double": ca.Precision.Double, "single": ca.Precision.Single, "mixed": ca.Precision.Mixed}[args.precision]
-    threads = max(1, min(os.cpu_count() or 1, 16))
+    threads = host_threads()
     t0 = time.time()
     pts_sys, pts_sig, pts_e, desc, sigma_s = build_points(args, ca, threads)
     build_s = time.time() - t0 - sigma_s
@@ -240,6 +261,8 @@ def run_transmission(args, ca):
     other = None
     if args.compare and args.precision in ("double", "mixed"):
         other = compare_precision(ca, pts_sys, pts_sig, t, npts / step, args)
+    per_point = plain_per_point(ca, pts_sys, pts_sig, args) if args.per_point > 0 else None
+    fixture = fixture_check(args, pts_e, t)
     cpu, max_dt, max_rel = (None, None, None) if args.no_cpu else cpu_baseline(pts_sys, pts_sig, pts_e, t, args, threads)
     if cpu is not None and other is not None:
         # the other precision against the same CPU sample, through its difference to this run
@@ -275,7 +298,55 @@ def run_transmission(args, ca):
         "kernels": kernels_of(ks, args.roofline_steps),
         "roofline_steps": args.roofline_steps,
         ("mixed" if args.precision == "double" else "plain"): other,
+        "plain_per_point": per_point,
+        "long_double_fixture": fixture,
     }
+
+
+def plain_per_point(ca, pts_sys, pts_sig, args):
+    """The plain complex128 solver one point at a time, on `--per-point` points spread over the
+    batch: (1) through the per-point API, transmission_with (nested.rs:682), as a caller looping
+    over points would (inputs uploaded per call); (2) as a resident one-point batch replayed from
+    its HIP graph (the per-point latency floor). Rates in points/s."""
+    npts = len(pts_sys)
+    idx = np.linspace(0, npts - 1, min(args.per_point, npts)).astype(int)
+    for i in idx[:2]:
+        pts_sys[i].transmission_with(pts_sig[i])  # warm
+    t0 = time.perf_counter()
+    for i in idx:
+        pts_sys[i].transmission_with(pts_sig[i])
+    api = len(idx) / (time.perf_counter() - t0)
+    batches = [ca.ResidentBatch([pts_sys[i]], [pts_sig[i]], ca.Precision.Double) for i in idx]
+    for b in batches:
+        b.run()
+        b.run()  # eager, then captured
+    t0 = time.perf_counter()
+    for b in batches:
+        b.run()
+    resident = len(idx) / (time.perf_counter() - t0)
+    progress(f"plain per point: {api:.1f} points/s through transmission_with, {resident:.1f} resident")
+    return {"points": int(len(idx)), "api_points_per_s": api, "resident_points_per_s": resident,
+            "unit": "points/s", "precision": "double"}
+
+
+def fixture_check(args, pts_e, t):
+    """|T - T_long| against the long-double oracle values committed for the full equal-size batch
+    (tests/golden/bhz200x64_magnetic.json), when this run is that workload."""
+    path = os.path.join(ROOT, "tests", "golden", "bhz200x64_magnetic.json")
+    if args.config != "c2" or (args.length, args.width, args.configs, args.energies) != (200, 64, 8, 33):
+        return None
+    if not os.path.exists(path) or args.precision != "double":
+        return None
+    pts = np.array(json.load(open(path))["points"])
+    if not np.array_equal(pts[:, 1], np.asarray(pts_e)):
+        return None
+    err = np.abs(t - pts[:, 3])
+    floor = np.abs(pts[:, 2] - pts[:, 3])
+    return {"max_abs_dT_vs_long_double": float(err.max()), "median": float(np.median(err)),
+            "points_above_1e-10": int((err > 1e-10).sum()),
+            "double_oracle_floor_max": float(floor.max()),
+            "points_where_floor_above_1e-10": int((floor > 1e-10).sum()),
+            "within_max_1e-10_or_floor": bool(np.all(err <= np.maximum(1e-10, floor)))}
 
 
 def compare_precision(ca, pts_sys, pts_sig, t_this, rate_this, args):
@@ -377,6 +448,8 @@ def main():
                     help="time the kernel choices on the batch first (auto: complex128 and complex64 runs; "
                          "a mixed run re-runs its refinement per candidate)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--per-point", type=int, default=16,
+                    help="points timed one at a time by the plain complex128 solver (0: skip)")
     ap.add_argument("--no-compare", dest="compare", action="store_false",
                     help="skip the run of the same points in the other precision (mixed vs plain)")
     ap.add_argument("--copy-calibration", action="store_true", help="run a 2 GiB device copy and exit")
@@ -401,9 +474,37 @@ def main():
     line = run_adjoint(args, ca) if args.config == "c4" else run_transmission(args, ca)
     line.update({"n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "scaling": "weak", "vs_baseline": None,
                  "data": "synthetic (seeded disorder, host-built Hamiltonians)"})
+    invalid = validity(line, args)
+    if invalid:
+        line["invalid"] = invalid
     keys = ["metric", "value", "unit", "n_gpus", "steps", "warmup", "ms_per_step", "higher_is_better", "scaling",
             "vs_baseline", "dtype", "data", "config"]
     print(json.dumps({**{k: line[k] for k in keys}, **{k: v for k, v in line.items() if k not in keys}}), flush=True)
+    if invalid:
+        progress("INVALID: " + invalid)
+        sys.exit(1)
+
+
+def validity(line, args):
+    """Why the value may not be published, or None: the metric counts points at |dT| <= 1e-8
+    against the FP64 CPU oracle, so every point must have succeeded and the sampled points must
+    meet that bound."""
+    if args.config == "c4":
+        err = line.get("max_rel_grad_vs_cpu")
+        return None if err is None or err <= 1e-6 else f"gradient off by {err:.1e} relative"
+    npts = line["config"]["points_per_step"]
+    if line.get("status_ok") != npts:
+        return f"{npts - line.get('status_ok', 0)} of {npts} points failed"
+    d = line.get("max_abs_dT_vs_cpu")
+    if d is not None and not d <= 1e-8:
+        return f"max |dT| against the CPU oracle {d:.1e} above 1e-8"
+    fx = line.get("long_double_fixture")
+    if fx is not None and not fx["max_abs_dT_vs_long_double"] <= 1e-8:
+        return f"max |dT| against the long-double oracle {fx['max_abs_dT_vs_long_double']:.1e} above 1e-8"
+    plain = line.get("plain")  # a mixed headline: its difference to complex128 on every point
+    if plain is not None and not plain["max_abs_dT_vs_this"] <= 1e-8:
+        return f"max |dT| against complex128 {plain['max_abs_dT_vs_this']:.1e} above 1e-8"
+    return None
 
 
 if __name__ == "__main__":

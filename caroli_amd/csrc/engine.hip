@@ -564,9 +564,41 @@ const char* const kFamilyNames[kFamilies] = {"front_lds_kernel",    "gather_kern
 /// and the trace scratch. Several lanes let the sweeps of several launch units (spin blocks, halves
 /// of a block's points, systems of other sizes) run at once: their top tree levels hold few
 /// fronts each and leave CUs idle alone.
+/// Pinned host memory (hipHostMalloc), for the small copies a refinement iteration makes while
+/// other lanes' work is in flight.
+template <class T>
+struct Pinned {
+    T* p = nullptr;
+    size_t count = 0;
+    Pinned() = default;
+    Pinned(const Pinned&) = delete;
+    Pinned& operator=(const Pinned&) = delete;
+    ~Pinned() {
+        if (p) (void)hipHostFree(p);
+    }
+    void ensure(size_t n) {
+        if (n <= count) return;
+        if (p) HIP_CHECK(hipHostFree(p));
+        p = nullptr;
+        count = 0;
+        HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&p), std::max<size_t>(n, 1) * sizeof(T), hipHostMallocDefault));
+        count = std::max<size_t>(n, 1);
+    }
+};
+
+/// The refinement's vectors of one launch unit, kept in its lane between runs: W_R, the solution
+/// x, the right-hand side b, the correction d, the per-point maxima, x on the left slice, the list
+/// of iterating points; and their pinned host ends.
+struct RefineBufs {
+    DevBuf w, x, b, d, maxes, xl, act;
+    Pinned<int32_t> h_act;
+    Pinned<double> h_maxes;
+};
+
 struct Lane {
     hipStream_t stream = nullptr;
     DevBuf arena, root_arena, piv, status, tout, trace_x, trace_p;
+    RefineBufs rf;
 };
 constexpr int kLanes = 4;
 constexpr size_t kMinUnitPoints = 32;
@@ -1083,6 +1115,12 @@ constexpr int kRefineMaxIter = 12;
 /// iteration is left to the complex f64 recomputation (f32 factors of a well-conditioned point
 /// contract by 1e-3 or better; a slow one would hold the whole batch in the loop).
 constexpr double kRefineRate = 0.25;
+/// Once some point of the batch goes to the complex f64 recomputation anyway (a batch of a few
+/// points costs one latency-bound sweep, whatever their number), a point still iterating after
+/// kRefineJoinFrom iterations joins it when, at its observed contraction, it would need more than
+/// one further iteration to reach kRefineTol: the slowest points would otherwise hold every unit
+/// of the group in the loop for several more solves.
+constexpr int kRefineJoinFrom = 3;
 
 struct Refined {
     std::vector<double> t;
@@ -1098,136 +1136,194 @@ bool refine_trace() {
     return v && std::strcmp(v, "1") == 0;
 }
 
-/// Transmission of block b for the P points of `pv` from the factors of the last sweep (complex
-/// f32 fronts when `single`): x = A^-1 E_R W_R, refined with FP64 residuals b - A x of the full
-/// operator z - H - Sigma until the correction is below kRefineTol; then T_b = |W_L^H x_L|^2.
-Refined refine(Engine& eng, const HostPlan& hp, const DevPlan& dp, const PointsView& pv,
-               const std::vector<const cg_sigma*>& sig, int b, int64_t n_orb, bool single,
-               const std::vector<char>& skip) {
-    const int32_t P = int32_t(sig.size());
-    Refined out;
-    out.t.assign(P, 0.0);
-    out.ok.assign(P, 1);
-    int c = 0;
-    for (auto* g : sig) c = std::max(c, g->blocks[b].wr.cols);
-    if (c == 0) return out;
-    if (!solve_fits(hp.max_n) || !solve_fits(hp.nodes[hp.root].ne)) {  // left to the complex f64 recomputation
-        out.ok.assign(P, 0);
-        return out;
-    }
-    const int nl = int(hp.left_orbs.size()), nr = int(hp.right_orbs.size()), nrows = int(hp.rows.size());
-    std::vector<double2> w(size_t(P) * nr * c, double2{0.0, 0.0});
-    for (int32_t p = 0; p < P; ++p) {
-        const CMat& wr = sig[p]->blocks[b].wr;
-        for (int col = 0; col < wr.cols; ++col)
-            for (int a = 0; a < nr; ++a)
-                w[size_t(p) * nr * c + a + size_t(col) * nr] = double2{wr(a, col).real(), wr(a, col).imag()};
-    }
-    DevBuf dw, xb, bb, db, rb, gb;
-    upload(dw, w, eng.lane->stream);
-    const size_t bytes = size_t(P) * size_t(n_orb) * c * sizeof(double2);
-    xb.ensure(bytes);
-    bb.ensure(bytes);
-    db.ensure(bytes);
-    rb.ensure(size_t(P) * 2 * sizeof(double));
-    gb.ensure(size_t(P) * nl * c * sizeof(double2));
-    HIP_CHECK(hipMemsetAsync(bb.p, 0, bytes, eng.lane->stream));
-    SolveView xv{xb.as<double2>(), n_orb * c, int32_t(n_orb), c, dp.vars.as<int32_t>()};
-    SolveView bv = xv, dv = xv;
-    bv.x = bb.as<double2>();
-    dv.x = db.as<double2>();
-    launch_scatter(bv, dp.right_orbs.as<int32_t>(), nr, dw.as<double2>(), int64_t(nr) * c, false, P, eng.lane->stream);
-    HIP_CHECK(hipMemcpyAsync(xb.p, bb.p, bytes, hipMemcpyDeviceToDevice, eng.lane->stream));
-    solve(eng, hp, dp, pv, xv, P, single, false, true);
-    // Per point: 0 iterating, 1 converged (later corrections only add rounding noise), 2 given up
-    // (unusable factors, a correction that is not finite or shrinks too slowly): left to the
-    // complex f64 recomputation.
-    std::vector<double> ratio(P, INFINITY), last(P, INFINITY), maxes(size_t(P) * 2);
-    std::vector<char> state(P, 0);
-    for (int32_t p = 0; p < P; ++p)
-        if (skip[p]) state[p] = 2;
-    // Each iteration runs on the points still iterating only (SolveView::pts).
-    std::vector<int32_t> act;
-    DevBuf ab;
-    ab.ensure(size_t(P) * sizeof(int32_t));
-    for (int it = 0; it < kRefineMaxIter; ++it) {
-        act.clear();
-        for (int32_t p = 0; p < P; ++p)
-            if (state[p] == 0) act.push_back(p);
-        if (act.empty()) break;
-        const int32_t A = int32_t(act.size());
-        HIP_CHECK(hipMemcpyAsync(ab.p, act.data(), act.size() * sizeof(int32_t), hipMemcpyHostToDevice,
-                                 eng.lane->stream));
-        SolveView xa = xv, da = dv;
-        xa.pts = ab.as<int32_t>();
-        da.pts = ab.as<int32_t>();
-        launch_residual(pv, xa, bb.as<double2>(), db.as<double2>(), dp.rows.as<int32_t>(), nrows,
-                        dp.row_ptr.as<int64_t>(), dp.cols.as<uint32_t>(), dp.left_pos.as<int32_t>(),
-                        dp.right_pos.as<int32_t>(), dp.left_orbs.as<int32_t>(), nl, dp.right_orbs.as<int32_t>(), nr, A,
-                        eng.lane->stream);
-        solve(eng, hp, dp, pv, da, A, single, false, false);
-        launch_update(xa, db.as<double2>(), dp.rows.as<int32_t>(), nrows, rb.as<double>(), A, P, eng.lane->stream);
-        HIP_CHECK(hipGetLastError());
-        HIP_CHECK(hipMemcpyAsync(maxes.data(), rb.p, maxes.size() * sizeof(double), hipMemcpyDeviceToHost,
-                                 eng.lane->stream));
-        HIP_CHECK(hipStreamSynchronize(eng.lane->stream));
-        for (int32_t p = 0; p < P; ++p) {
-            const double md = maxes[2 * p], mx = maxes[2 * p + 1];
-            ratio[p] = mx > 0.0 ? md / mx : (md > 0.0 ? INFINITY : 0.0);
+/// The refinement of one launch unit (block b of P points) from the factors its sweep left in
+/// its lane (complex f32 fronts): x = A^-1 E_R W_R, refined with FP64 residuals b - A x of the
+/// full operator z - H - Sigma until the correction is below kRefineTol; then T_b = |W_L^H x_L|^2.
+/// Its work is issued on the lane's stream, so that the refinements of the units of a group (one
+/// lane each) run at once: start(), then issue() / collect() per iteration (with the lane's stream
+/// synchronised in between) until issue() returns false, then finish_issue() / finish().
+class Refinement {
+  public:
+    /// recompute: shared by the batch, set once some point goes to the complex f64 recomputation.
+    Refinement(Engine& eng, Lane& lane, const HostPlan& hp, const DevPlan& dp, const PointsView& pv,
+               std::vector<const cg_sigma*> sig, int b, int64_t n_orb, const std::vector<char>& skip, bool& recompute)
+        : eng_(eng), lane_(lane), hp_(hp), dp_(dp), pv_(pv), sig_(std::move(sig)), b_(b), n_orb_(n_orb),
+          recompute_(recompute) {
+        P_ = int32_t(sig_.size());
+        out_.t.assign(P_, 0.0);
+        out_.ok.assign(P_, 1);
+        for (auto* g : sig_) c_ = std::max(c_, g->blocks[b].wr.cols);
+        state_.assign(P_, 0);
+        for (int32_t p = 0; p < P_; ++p)
+            if (skip[p]) {
+                state_[p] = 2;
+                recompute_ = true;
+            }
+        ratio_.assign(P_, INFINITY);
+        last_.assign(P_, INFINITY);
+        if (c_ > 0 && (!solve_fits(hp.max_n) || !solve_fits(hp.nodes[hp.root].ne))) {
+            out_.ok.assign(P_, 0);  // left to the complex f64 recomputation
+            c_ = 0;
+            unusable_ = true;
+            recompute_ = true;
         }
-        ++out.iters;
+        nl_ = int(hp.left_orbs.size());
+        nr_ = int(hp.right_orbs.size());
+        nrows_ = int(hp.rows.size());
+    }
+
+    void start() {
+        if (c_ == 0) return;
+        eng_.lane = &lane_;
+        RefineBufs& r = lane_.rf;
+        hipStream_t st = lane_.stream;
+        w_host_.assign(size_t(P_) * nr_ * c_, double2{0.0, 0.0});
+        for (int32_t p = 0; p < P_; ++p) {
+            const CMat& wr = sig_[p]->blocks[b_].wr;
+            for (int col = 0; col < wr.cols; ++col)
+                for (int a = 0; a < nr_; ++a)
+                    w_host_[size_t(p) * nr_ * c_ + a + size_t(col) * nr_] = double2{wr(a, col).real(), wr(a, col).imag()};
+        }
+        bytes_ = size_t(P_) * size_t(n_orb_) * c_ * sizeof(double2);
+        r.w.ensure(w_host_.size() * sizeof(double2));
+        r.x.ensure(bytes_);
+        r.b.ensure(bytes_);
+        r.d.ensure(bytes_);
+        r.maxes.ensure(size_t(P_) * 2 * sizeof(double));
+        r.xl.ensure(size_t(P_) * nl_ * c_ * sizeof(double2));
+        r.act.ensure(size_t(P_) * sizeof(int32_t));
+        r.h_act.ensure(size_t(P_));
+        r.h_maxes.ensure(size_t(P_) * 2);
+        HIP_CHECK(hipMemcpyAsync(r.w.p, w_host_.data(), w_host_.size() * sizeof(double2), hipMemcpyHostToDevice, st));
+        HIP_CHECK(hipMemsetAsync(r.b.p, 0, bytes_, st));
+        xv_ = SolveView{r.x.as<double2>(), n_orb_ * c_, int32_t(n_orb_), c_, dp_.vars.as<int32_t>()};
+        SolveView bv = xv_;
+        bv.x = r.b.as<double2>();
+        launch_scatter(bv, dp_.right_orbs.as<int32_t>(), nr_, r.w.as<double2>(), int64_t(nr_) * c_, false, P_, st);
+        HIP_CHECK(hipMemcpyAsync(r.x.p, r.b.p, bytes_, hipMemcpyDeviceToDevice, st));
+        solve(eng_, hp_, dp_, pv_, xv_, P_, true, false, true);
+    }
+
+    /// Issues one iteration on the points still iterating; false when none is.
+    bool issue() {
+        if (c_ == 0 || it_ >= kRefineMaxIter) return false;
+        RefineBufs& r = lane_.rf;
+        A_ = 0;
+        for (int32_t p = 0; p < P_; ++p)
+            if (state_[p] == 0) r.h_act.p[A_++] = p;
+        if (A_ == 0) return false;
+        eng_.lane = &lane_;
+        hipStream_t st = lane_.stream;
+        HIP_CHECK(hipMemcpyAsync(r.act.p, r.h_act.p, size_t(A_) * sizeof(int32_t), hipMemcpyHostToDevice, st));
+        SolveView xa = xv_, da = xv_;
+        xa.pts = r.act.as<int32_t>();
+        da.pts = r.act.as<int32_t>();
+        da.x = r.d.as<double2>();
+        launch_residual(pv_, xa, r.b.as<double2>(), r.d.as<double2>(), dp_.rows.as<int32_t>(), nrows_,
+                        dp_.row_ptr.as<int64_t>(), dp_.cols.as<uint32_t>(), dp_.left_pos.as<int32_t>(),
+                        dp_.right_pos.as<int32_t>(), dp_.left_orbs.as<int32_t>(), nl_, dp_.right_orbs.as<int32_t>(), nr_,
+                        A_, st);
+        solve(eng_, hp_, dp_, pv_, da, A_, true, false, false);
+        launch_update(xa, r.d.as<double2>(), dp_.rows.as<int32_t>(), nrows_, r.maxes.as<double>(), A_, P_, st);
+        HIP_CHECK(hipGetLastError());
+        HIP_CHECK(hipMemcpyAsync(r.h_maxes.p, r.maxes.p, size_t(P_) * 2 * sizeof(double), hipMemcpyDeviceToHost, st));
+        return true;
+    }
+
+    /// After the lane's stream is synchronised: the convergence test of the issued iteration.
+    void collect() {
+        const double* maxes = lane_.rf.h_maxes.p;
+        for (int32_t p = 0; p < P_; ++p) {
+            const double md = maxes[2 * p], mx = maxes[2 * p + 1];
+            ratio_[p] = mx > 0.0 ? md / mx : (md > 0.0 ? INFINITY : 0.0);
+        }
+        ++out_.iters;
         if (refine_trace()) {
             int n_active = 0, n_done = 0, n_gone = 0;
             double worst = 0.0, best = INFINITY;
-            for (int32_t p = 0; p < P; ++p) {
-                n_active += state[p] == 0;
-                n_done += state[p] == 1;
-                n_gone += state[p] == 2;
-                if (state[p] == 0) {
-                    worst = std::max(worst, ratio[p]);
-                    best = std::min(best, ratio[p]);
+            for (int32_t p = 0; p < P_; ++p) {
+                n_active += state_[p] == 0;
+                n_done += state_[p] == 1;
+                n_gone += state_[p] == 2;
+                if (state_[p] == 0) {
+                    worst = std::max(worst, ratio_[p]);
+                    best = std::min(best, ratio_[p]);
                 }
             }
             std::fprintf(stderr, "[refine] block %d iteration %d: %d iterating (correction %.1e .. %.1e), %d done, %d given up\n",
-                         b, it, n_active, best, worst, n_done, n_gone);
+                         b_, it_, n_active, best, worst, n_done, n_gone);
         }
-        bool active = false;
-        for (int32_t p = 0; p < P; ++p) {
-            if (state[p] != 0) continue;
-            const double r = ratio[p];
-            if (r <= kRefineTol || (last[p] < INFINITY && r <= kRefineEstFrom && r * r <= kRefineEst * last[p])) {
-                state[p] = 1;
-                out.worst = std::max(out.worst, r);
-            } else if (!(r < INFINITY) || (it >= 2 && r > kRefineRate * last[p])) {
-                state[p] = 2;
+        for (int32_t p = 0; p < P_; ++p) {
+            if (state_[p] != 0) continue;
+            const double rr = ratio_[p];
+            if (rr <= kRefineTol || (last_[p] < INFINITY && rr <= kRefineEstFrom && rr * rr <= kRefineEst * last_[p])) {
+                state_[p] = 1;
+                out_.worst = std::max(out_.worst, rr);
+            } else if (!(rr < INFINITY) || (it_ >= 2 && rr > kRefineRate * last_[p]) ||
+                       (recompute_ && it_ >= kRefineJoinFrom && last_[p] < INFINITY &&
+                        rr * (rr / last_[p]) > kRefineTol)) {
+                state_[p] = 2;
+                recompute_ = true;
             } else {
-                last[p] = r;
-                active = true;
+                last_[p] = rr;
             }
         }
-        if (!active) break;
+        ++it_;
     }
-    launch_gather(xv, dp.left_orbs.as<int32_t>(), nl, gb.as<double2>(), P, eng.lane->stream);
-    std::vector<double2> xl(size_t(P) * nl * c);
-    HIP_CHECK(hipMemcpyAsync(xl.data(), gb.p, xl.size() * sizeof(double2), hipMemcpyDeviceToHost, eng.lane->stream));
-    HIP_CHECK(hipStreamSynchronize(eng.lane->stream));
-    for (int32_t p = 0; p < P; ++p) {
-        out.ok[p] = state[p] == 1;
-        const CMat& wl = sig[p]->blocks[b].wl;
-        double t = 0.0;
-        for (int col = 0; col < c; ++col)
-            for (int d = 0; d < wl.cols; ++d) {
-                cplx acc = 0.0;
-                for (int a = 0; a < nl; ++a) {
-                    double2 x = xl[size_t(p) * nl * c + a + size_t(col) * nl];
-                    acc += std::conj(wl(a, d)) * cplx(x.x, x.y);
+
+    void finish_issue() {
+        if (c_ == 0) return;
+        eng_.lane = &lane_;
+        RefineBufs& r = lane_.rf;
+        launch_gather(xv_, dp_.left_orbs.as<int32_t>(), nl_, r.xl.as<double2>(), P_, lane_.stream);
+        xl_.resize(size_t(P_) * nl_ * c_);
+        HIP_CHECK(hipMemcpyAsync(xl_.data(), r.xl.p, xl_.size() * sizeof(double2), hipMemcpyDeviceToHost, lane_.stream));
+    }
+
+    /// After the lane's stream is synchronised: T per point (points not converged: ok = 0).
+    Refined finish() {
+        if (c_ == 0) return std::move(out_);
+        for (int32_t p = 0; p < P_; ++p) {
+            out_.ok[p] = state_[p] == 1;
+            const CMat& wl = sig_[p]->blocks[b_].wl;
+            double t = 0.0;
+            for (int col = 0; col < c_; ++col)
+                for (int d = 0; d < wl.cols; ++d) {
+                    cplx acc = 0.0;
+                    for (int a = 0; a < nl_; ++a) {
+                        double2 x = xl_[size_t(p) * nl_ * c_ + a + size_t(col) * nl_];
+                        acc += std::conj(wl(a, d)) * cplx(x.x, x.y);
+                    }
+                    t += std::norm(acc);
                 }
-                t += std::norm(acc);
-            }
-        out.t[p] = t;
+            out_.t[p] = t;
+        }
+        return std::move(out_);
     }
-    return out;
-}
+
+  private:
+    Engine& eng_;
+    Lane& lane_;
+    const HostPlan& hp_;
+    const DevPlan& dp_;
+    PointsView pv_;
+    std::vector<const cg_sigma*> sig_;
+    int b_;
+    int64_t n_orb_;
+    int32_t P_ = 0, A_ = 0;
+    int c_ = 0, nl_ = 0, nr_ = 0, nrows_ = 0, it_ = 0;
+    bool unusable_ = false;
+    bool& recompute_;
+    size_t bytes_ = 0;
+    SolveView xv_{};
+    Refined out_;
+    // Per point: 0 iterating, 1 converged (later corrections only add rounding noise), 2 given up
+    // (unusable factors, a correction that is not finite or shrinks too slowly): left to the
+    // complex f64 recomputation.
+    std::vector<char> state_;
+    std::vector<double> ratio_, last_;
+    std::vector<double2> w_host_, xl_;
+};
 
 }  // namespace
 
@@ -1324,10 +1420,11 @@ int32_t create_batch(const cg_system* const* systems, const cg_sigma* const* sig
     std::vector<Need> need;  // device bytes of each unit
     // The points of a (structure, block) are split so that at least kLanes units can be in flight
     // (the top tree levels hold few fronts per point and leave CUs idle with one or two units),
-    // down to kMinUnitPoints points a unit; the refinement of the mixed mode works unit by unit.
+    // down to kMinUnitPoints points a unit; the refinements of the mixed mode run on the units'
+    // lanes as well.
     size_t blocks_total = 0;
     for (auto& [key, idx] : groups) blocks_total += structure_of(eng, systems[idx[0]])->host.size();
-    const size_t splits = precision == CG_MIXED ? 1 : (kLanes + blocks_total - 1) / std::max<size_t>(blocks_total, 1);
+    const size_t splits = (kLanes + blocks_total - 1) / std::max<size_t>(blocks_total, 1);
     for (auto& [key, idx] : groups) {
         auto sp = structure_of(eng, systems[idx[0]]);
         for (auto p : idx) structure_of(eng, systems[p]);
@@ -1366,12 +1463,12 @@ int32_t create_batch(const cg_system* const* systems, const cg_sigma* const* sig
         }
     }
     // Lanes: as many as there are units, up to kLanes, while the largest units fit the concurrent
-    // budget together (one when the mode runs the refinement, which works unit by unit).
+    // budget together.
     std::vector<size_t> sizes;
     for (auto& nd : need) sizes.push_back(nd.arena + nd.root + nd.piv + nd.status + nd.tout);
     std::sort(sizes.rbegin(), sizes.rend());
     const size_t concurrent_budget = size_t(160) << 30;
-    int lanes = precision == CG_MIXED ? 1 : int(std::min<size_t>(kLanes, sizes.size()));
+    int lanes = int(std::min<size_t>(kLanes, sizes.size()));
     auto fits = [&](int L) {
         size_t t = 0;
         for (int q = 0; q < L; ++q) t += sizes[q];
@@ -1437,6 +1534,7 @@ int32_t run_batch(cg_batch* bt, double* out_t, int32_t* out_status, cg_batch_sta
     local.front_bytes = bt->bytes;
     std::vector<int32_t> status(count, CG_OK);
     std::vector<char> redo(count, 0);  // mixed: recompute in complex f64
+    bool recompute = false;            // mixed: some point goes to the recomputation
     for (int64_t p = 0; p < count; ++p) out_t[p] = 0.0;
     {
         std::lock_guard<std::mutex> lock(eng.mu);
@@ -1454,7 +1552,7 @@ int32_t run_batch(cg_batch* bt, double* out_t, int32_t* out_status, cg_batch_sta
         // sweeps overlap; results are read back once the group is done.
         // Per-kernel timing runs the units one at a time, so that each timed launch has the GPU to
         // itself and its duration is its own.
-        const size_t L = (mixed || eng.timing) ? 1 : size_t(bt->lanes);
+        const size_t L = eng.timing ? 1 : size_t(bt->lanes);
         for (size_t u0 = 0; u0 < bt->units.size(); u0 += L) {
             const size_t u1 = std::min(u0 + L, bt->units.size());
             std::vector<PointsView> pvs(u1 - u0);
@@ -1482,7 +1580,7 @@ int32_t run_batch(cg_batch* bt, double* out_t, int32_t* out_status, cg_batch_sta
                 pv.status_stride = hp.slots;
                 pv.t_out = eng.lane->tout.as<double>();
                 pv.keep = mixed ? 1 : 0;  // the refinement solves with the factors
-                if (!mixed && !eng.timing && !bt->eager && graphs_enabled())
+                if (!eng.timing && !bt->eager && graphs_enabled())
                     sweep_graph(eng, u, hp, dp, pv, int32_t(u.points.size()), single);
                 else
                     sweep(eng, hp, dp, pv, int32_t(u.points.size()), single, u.in);
@@ -1503,23 +1601,52 @@ int32_t run_batch(cg_batch* bt, double* out_t, int32_t* out_status, cg_batch_sta
             eng.lane = &eng.lanes[0];
             eng.collect();
             local.seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - w0).count();
-            for (size_t ui = u0; ui < u1; ++ui) {
-                auto& u = bt->units[ui];
-                const HostPlan& hp = u.structure->host[u.block];
-                const DevPlan& dp = u.structure->dev[u.block];
-                const size_t P = u.points.size();
-                const std::vector<double>& t = ts[ui - u0];
-                const std::vector<int32_t>& st = sts[ui - u0];
-                if (mixed) {
-                    auto t0 = std::chrono::steady_clock::now();
+            if (mixed) {
+                // The refinements of the group's units at once, one lane each: every iteration is
+                // issued on all lanes, then the lanes are synchronised and each unit tests its
+                // points; a unit whose points are all done or given up stops issuing.
+                auto t0 = std::chrono::steady_clock::now();
+                std::vector<std::unique_ptr<Refinement>> refs;
+                for (int64_t p = 0; p < count; ++p) recompute |= redo[p] != 0;
+                for (size_t ui = u0; ui < u1; ++ui) {
+                    auto& u = bt->units[ui];
+                    const HostPlan& hp = u.structure->host[u.block];
+                    const size_t P = u.points.size();
+                    const std::vector<int32_t>& st = sts[ui - u0];
                     std::vector<const cg_sigma*> sig;
                     for (auto p : u.points) sig.push_back(bt->sigmas[p]);
                     std::vector<char> skip(P, 0);
                     for (size_t q = 0; q < P; ++q)
                         for (int32_t k = 0; k <= hp.root; ++k) skip[q] |= st[q * hp.slots + k] != kFrontOk;
-                    Refined rf = refine(eng, hp, dp, pvs[ui - u0], sig, u.block,
-                                        int64_t(bt->systems[u.points[0]]->spec.h.n), true, skip);
-                    local.seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+                    refs.push_back(std::make_unique<Refinement>(eng, eng.lanes[ui - u0], hp, u.structure->dev[u.block],
+                                                                pvs[ui - u0], std::move(sig), u.block,
+                                                                int64_t(bt->systems[u.points[0]]->spec.h.n), skip,
+                                                                recompute));
+                    refs.back()->start();
+                }
+                auto sync_group = [&] {
+                    for (size_t ui = u0; ui < u1; ++ui) HIP_CHECK(hipStreamSynchronize(eng.lanes[ui - u0].stream));
+                };
+                for (;;) {
+                    std::vector<char> issued(refs.size(), 0);
+                    bool any = false;
+                    for (size_t k = 0; k < refs.size(); ++k) any |= (issued[k] = refs[k]->issue()) != 0;
+                    if (!any) break;
+                    sync_group();
+                    for (size_t k = 0; k < refs.size(); ++k)
+                        if (issued[k]) refs[k]->collect();
+                }
+                for (auto& r : refs) r->finish_issue();
+                sync_group();
+                eng.lane = &eng.lanes[0];
+                eng.collect();
+                local.seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+                for (size_t ui = u0; ui < u1; ++ui) {
+                    auto& u = bt->units[ui];
+                    const HostPlan& hp = u.structure->host[u.block];
+                    const size_t P = u.points.size();
+                    const std::vector<int32_t>& st = sts[ui - u0];
+                    Refined rf = refs[ui - u0]->finish();
                     local.refinement_iters += rf.iters;
                     for (size_t q = 0; q < P; ++q) {
                         int64_t p = u.points[q];
@@ -1533,8 +1660,15 @@ int32_t run_batch(cg_batch* bt, double* out_t, int32_t* out_status, cg_batch_sta
                             out_t[p] += rf.t[q];
                         }
                     }
-                    continue;
                 }
+                continue;
+            }
+            for (size_t ui = u0; ui < u1; ++ui) {
+                auto& u = bt->units[ui];
+                const HostPlan& hp = u.structure->host[u.block];
+                const size_t P = u.points.size();
+                const std::vector<double>& t = ts[ui - u0];
+                const std::vector<int32_t>& st = sts[ui - u0];
                 for (size_t q = 0; q < P; ++q) {
                     int64_t p = u.points[q];
                     bool delayed = false;
