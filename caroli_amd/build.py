@@ -13,7 +13,7 @@ OUT = os.path.join(HERE, "libcaroli_amd.so")
 OBJ = os.path.join(HERE, "_build")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 HOST = ["model.cpp", "symbolic.cpp", "dense.cpp", "modes.cpp", "capi.cpp"]
-DEVICE = ["fronts.hip", "solve.hip", "engine.hip"]
+DEVICE = ["fronts.hip", "solve.hip", "engine.hip", "leads.hip"]
 HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-Wall",
              # packed FP32 ops off (compiler-generated v_pk_* FP32 forms are avoided)
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]

@@ -2,6 +2,7 @@
 // the GPU engine.
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <exception>
@@ -10,6 +11,7 @@
 #include "../../include/caroli_amd.h"
 #include "engine.hpp"
 #include "checked.hpp"
+#include "leads.hpp"
 #include "system.hpp"
 
 using namespace caroli;
@@ -325,6 +327,62 @@ void ensure_modes(const cg_system* s) {
     }
 }
 
+// The scalar-coupling leads go to the GPU (leads.hip) when a device is present, unless
+// CAROLI_SIGMA_HOST=1 keeps them on the host.
+bool cell_on_gpu(const ModeSolver& m) {
+    if (!m.cell()) return false;
+    const char* v = std::getenv("CAROLI_SIGMA_HOST");
+    if (v && std::strcmp(v, "1") == 0) return false;
+    return gpu::lead_device_available();
+}
+
+// Gamma = W W^H of a cell self-energy Sigma = Y diag(d) Y^H: Gamma = Y diag(-2 Im d) Y^H, so W is
+// the propagating eigenvectors scaled by sqrt(-2 Im d) (the evanescent modes have real d).
+CMat cell_gamma_factor(const CellModes& m, const std::vector<cplx>& d) {
+    const CMat& y = *m.y;
+    CMat w(y.rows, int(m.propagating.size()));
+    for (int q = 0; q < w.cols; ++q) {
+        const int k = m.propagating[q];
+        const double g = std::sqrt(std::max(0.0, -2.0 * d[k].imag()));
+        for (int i = 0; i < y.rows; ++i) w(i, q) = y(i, k) * g;
+    }
+    return w;
+}
+
+// Block b's self-energies from cell modes on the GPU: Sigma_L from the left lead's modes, Sigma_R
+// from the right lead's (one launch when the leads are equal).
+Status cell_block_sigmas(const ModeSolver& left, const ModeSolver& right, bool same, double energy, cg_sigma::Block& sb) {
+    CellModes cl, cr;
+    if (Status st = left.cell_modes_at(energy, cl); st != kOk) return st;
+    if (same) {
+        cr = cl;
+    } else if (Status st = right.cell_modes_at(energy, cr); st != kOk) {
+        return st;
+    }
+    const std::vector<cplx> dl = cl.sigma_left_diag(), dr = cr.sigma_right_diag();
+    const int nl = cl.y->rows, nr = cr.y->rows;
+    sb.sl = CMat(nl, nl);
+    sb.sr = CMat(nr, nr);
+    if (same) {
+        std::vector<cplx> d(dl);
+        d.insert(d.end(), dr.begin(), dr.end());
+        std::vector<cplx> both(size_t(2) * nl * nl);
+        gpu::cell_sigmas(cl.lead_id, cl.y->a.data(), nl, d.data(), 2, both.data());
+        std::copy(both.begin(), both.begin() + sb.sl.a.size(), sb.sl.a.begin());
+        std::copy(both.begin() + sb.sl.a.size(), both.end(), sb.sr.a.begin());
+    } else {
+        gpu::cell_sigmas(cl.lead_id, cl.y->a.data(), nl, dl.data(), 1, sb.sl.a.data());
+        gpu::cell_sigmas(cr.lead_id, cr.y->a.data(), nr, dr.data(), 1, sb.sr.a.data());
+    }
+    sb.gl = broadening(sb.sl);
+    sb.gr = broadening(sb.sr);
+    sb.wl = cell_gamma_factor(cl, dl);
+    sb.wr = cell_gamma_factor(cr, dr);
+    sb.channels_l = int(cl.propagating.size());
+    sb.channels_r = int(cr.propagating.size());
+    return kOk;
+}
+
 int32_t cg_self_energies(const cg_system* s, double energy, cg_sigma** out) {
     if (!s || !out) return CG_ERR_INVALID;
     try {
@@ -334,6 +392,22 @@ int32_t cg_self_energies(const cg_system* s, double energy, cg_sigma** out) {
         for (size_t b = 0; b < s->sym.blocks.size(); ++b) {
             auto& blk = s->sym.blocks[b];
             cg_sigma::Block sb;
+            const ModeSolver& left = *s->left_modes[b];
+            const ModeSolver& right = blk.same_lead ? left : *s->right_modes[b];
+            if (cell_on_gpu(left) && cell_on_gpu(right)) {
+                if (Status st = cell_block_sigmas(left, right, blk.same_lead, energy, sb); st != kOk) {
+                    delete g;
+                    return st;
+                }
+                sb.left_h00 = blk.left_h00;
+                sb.left_h01 = blk.left_h01;
+                sb.right_h00 = blk.right_h00;
+                sb.right_h01 = blk.right_h01;
+                sb.left_orbitals = blk.left_orbitals;
+                sb.right_orbitals = blk.right_orbitals;
+                g->blocks.push_back(std::move(sb));
+                continue;
+            }
             LeadModes ml, mr;
             Status st = s->left_modes[b]->modes_at(energy, ml);
             if (st == kOk) {

@@ -2,6 +2,7 @@
 #include "modes.hpp"
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <limits>
 
@@ -394,8 +395,24 @@ CMat LeadModes::sigma_left() const {
     return sandwich(v, t);
 }
 
+std::vector<cplx> CellModes::sigma_right_diag() const {
+    std::vector<cplx> d(right.size());
+    for (size_t k = 0; k < d.size(); ++k) d[k] = c * s * right[k];
+    return d;
+}
+
+std::vector<cplx> CellModes::sigma_left_diag() const {
+    std::vector<cplx> d(left.size());
+    for (size_t k = 0; k < d.size(); ++k) d[k] = c * s / left[k];
+    return d;
+}
+
+namespace {
+std::atomic<uint64_t> next_solver_id{1};
+}
+
 // DefiniteCoupling::new (modes.rs:575-618)
-ModeSolver::ModeSolver(const CMat& h0, const CMat& h1) : h0_(h0), h1_(h1) {
+ModeSolver::ModeSolver(const CMat& h0, const CMat& h1) : id_(next_solver_id++), h0_(h0), h1_(h1) {
     const int n = h0.rows;
     auto hermitian = [&](const CMat& m) {
         double asym = 0.0;
@@ -444,6 +461,34 @@ ModeSolver::ModeSolver(const CMat& h0, const CMat& h1) : h0_(h0), h1_(h1) {
 }
 
 namespace {
+// The per-mode part of DefiniteCoupling::modes (modes.rs:660-700): for nu_k (the generalized
+// eigenvalues of (H0 - e) against P, here -s nu_k = mu_k), the right- and left-going lambda, and
+// which modes propagate. kSingular at a subband edge (a degenerate pair) or a unit-modulus
+// evanescent root.
+Status mode_lambdas(const std::vector<double>& nu, double s, std::vector<cplx>& right, std::vector<cplx>& leftl,
+                    std::vector<int>& propagating) {
+    const int n = int(nu.size());
+    right.assign(n, 0.0);
+    leftl.assign(n, 0.0);
+    propagating.clear();
+    for (int k = 0; k < n; ++k) {
+        double mu = -s * nu[k];
+        if (std::abs(mu) < 2.0) {
+            cplx root(mu / 2.0, std::sqrt(4.0 - mu * mu) / 2.0);
+            if (2.0 * root.imag() < kDegenerateTol) return kSingular;
+            right[k] = s < 0 ? root : std::conj(root);
+            leftl[k] = s < 0 ? std::conj(root) : root;
+            propagating.push_back(k);
+        } else {
+            double big = (mu + (mu > 0 ? 1.0 : mu < 0 ? -1.0 : 0.0) * std::sqrt(mu * mu - 4.0)) / 2.0;
+            if (std::abs(big) <= 1.0 + kUnitTol) return kSingular;
+            right[k] = 1.0 / big;
+            leftl[k] = big;
+        }
+    }
+    return kOk;
+}
+
 // DefiniteCoupling::modes (modes.rs:621-710): one Hermitian eigenproblem of size n.
 Status definite_modes(const CMat& h0, double s, const CMat& l, bool cell, double c, const std::vector<double>& theta,
                       const CMat& ycell, double e, double scale, LeadModes& out) {
@@ -466,23 +511,9 @@ Status definite_modes(const CMat& h0, double s, const CMat& l, bool cell, double
         if (!dense::hermitian_eigen(mh, nu, computed)) return kEigen;
         y = &computed;
     }
-    std::vector<cplx> right(n), leftl(n);
+    std::vector<cplx> right, leftl;
     std::vector<int> propagating;
-    for (int k = 0; k < n; ++k) {
-        double mu = -s * nu[k];
-        if (std::abs(mu) < 2.0) {
-            cplx root(mu / 2.0, std::sqrt(4.0 - mu * mu) / 2.0);
-            if (2.0 * root.imag() < kDegenerateTol) return kSingular;
-            right[k] = s < 0 ? root : std::conj(root);
-            leftl[k] = s < 0 ? std::conj(root) : root;
-            propagating.push_back(k);
-        } else {
-            double big = (mu + (mu > 0 ? 1.0 : mu < 0 ? -1.0 : 0.0) * std::sqrt(mu * mu - 4.0)) / 2.0;
-            if (std::abs(big) <= 1.0 + kUnitTol) return kSingular;
-            right[k] = 1.0 / big;
-            leftl[k] = big;
-        }
-    }
+    if (Status st = mode_lambdas(nu, s, right, leftl, propagating); st != kOk) return st;
     // Velocities of the propagating modes, psi = L^-H y (L = sqrt(c) I in the cell case).
     CMat lhinv;
     if (!cell) lhinv = dense::inverse(l.adjoint());
@@ -540,6 +571,44 @@ Status ModeSolver::modes_at(double e, LeadModes& out) const {
                                                definite_->theta, definite_->y, es, scale, out)
                               : solve_general(h0_, h1_, cplx(es, 0.0), scale, out);
         if (st == kOk) return kOk;
+        last = st;
+    }
+    return last;
+}
+
+// The cell case of modes_at (modes.rs:130-145, :621-710) without the bases: per energy shift, the
+// lambdas of every cell mode and the velocity test of the propagating ones (the cell eigenvectors
+// are orthonormal, so |L^-H y_k|^2 = 1 / c).
+Status ModeSolver::cell_modes_at(double e, CellModes& out) const {
+    if (!cell()) return kInvalid;
+    const Definite& d = *definite_;
+    double scale = std::max(row_norm(h0_) + 2.0 * row_norm(h1_), std::numeric_limits<double>::min());
+    Status last = kEigen;
+    for (double shift : kShifts) {
+        const double es = e + shift * scale;
+        std::vector<double> nu;
+        for (double t : d.theta) nu.push_back((t - es) / d.c);
+        CellModes m;
+        Status st = mode_lambdas(nu, d.s, m.right, m.left, m.propagating);
+        if (st == kOk)
+            for (int k : m.propagating) {
+                double nn = 0.0;
+                for (int i = 0; i < d.y.rows; ++i) nn += std::norm(d.y(i, k) / std::sqrt(d.c));
+                const double v = -2.0 * d.s * m.right[k].imag() / nn;
+                if (!(std::abs(v) >= kVelocityTol * scale)) {
+                    st = kSingular;
+                    break;
+                }
+            }
+        if (st == kOk) {
+            m.y = &d.y;
+            m.lead_id = id_;
+            m.c = d.c;
+            m.s = d.s;
+            m.energy = es;
+            out = std::move(m);
+            return kOk;
+        }
         last = st;
     }
     return last;

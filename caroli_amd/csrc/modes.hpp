@@ -5,6 +5,7 @@
 // (DefiniteCoupling, :558-711), the energy shifts off band edges (LeadModes::shifted, :130-145)
 // and `Sigma_R = u X_g X_a^-1 u^H`, `Sigma_L = v Y_a Y_g^-1 v^H` (:494-511).
 #pragma once
+#include <cstdint>
 #include <optional>
 #include <vector>
 
@@ -38,14 +39,33 @@ struct LeadModes {
     CMat sigma_left() const;
 };
 
+/// The modes of a lead with a scalar coupling (H1 + H1^H = 2 s c I) at one energy, per eigenvector
+/// of the cell (modes.rs:621-710 with P = c 1): lambda of its right- and left-going mode, and which
+/// ones propagate. Sigma_R = Y diag(c s lambda_R) Y^H and Sigma_L = Y diag(c s / lambda_L) Y^H.
+struct CellModes {
+    const CMat* y = nullptr;  // unitary eigenvectors of the cell (owned by the solver)
+    uint64_t lead_id = 0;     // the solver's id (names y)
+    double c = 0.0, s = 1.0, energy = 0.0;
+    std::vector<cplx> right, left;
+    std::vector<int> propagating;
+    std::vector<cplx> sigma_right_diag() const;
+    std::vector<cplx> sigma_left_diag() const;
+};
+
 /// The energy-independent part of a lead's mode problem (modes.rs:519-538).
 class ModeSolver {
    public:
     ModeSolver(const CMat& h0, const CMat& h1);
     /// Modes at the real energy `e`, shifted off band edges when needed.
     Status modes_at(double e, LeadModes& out) const;
+    /// The scalar-coupling case with the cell's eigenvectors cached (cell_modes_at applies).
+    bool cell() const { return definite_ && definite_->cell; }
+    /// The cell modes at `e`, shifted off band edges as modes_at does (no bases built).
+    Status cell_modes_at(double e, CellModes& out) const;
+    uint64_t id() const { return id_; }
 
    private:
+    uint64_t id_;
     CMat h0_, h1_;
     struct Definite {
         double s;
