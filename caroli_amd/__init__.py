@@ -153,6 +153,21 @@ class NestedTwoTerminal:
         return cls(h.value)
 
     @classmethod
+    def bhz_magnetic(cls, extent, seed, x0, nx_imp, concentration, exchange, moments="cone", moment_param=0.0,
+                     coupling="opposite", seeding="gfortran"):
+        """BHZ ribbon with classical magnetic impurities (cg_bhz_magnetic_create): `moments`
+        "isotropic", "cone" (half-angle moment_param, radians) or "polarized" (m_z = moment_param);
+        `coupling` "opposite" or "same" g; `seeding` "gfortran" (Xoshiro256::gfortran) or "u64"
+        (Xoshiro256::seed_from_u64)."""
+        ext = (C.c_uint64 * 2)(int(extent[0]), int(extent[1]))
+        h = C.c_void_p()
+        _check(capi.cg_bhz_magnetic_create(
+            ext, {"u64": 0, "gfortran": 1}[seeding], int(seed), int(x0), int(nx_imp), float(concentration),
+            {"isotropic": 0, "cone": 1, "polarized": 2}[moments], float(moment_param),
+            {"opposite": 0, "same": 1}[coupling], float(exchange), C.byref(h)), "bhz_magnetic")
+        return cls(h.value)
+
+    @classmethod
     def from_arrays(cls, n, row_ptr, cols, vals, cells, site_orbitals, site_offsets, dim, hop_range, axis,
                     left_h00, left_h01, right_h00, right_h01):
         """`from_model` from the arrays the models produce (cg_system_create)."""
@@ -240,10 +255,15 @@ class NestedTwoTerminal:
         return f.value, b.value
 
     # -- self-energies -------------------------------------------------------------------------
-    def self_energies(self, energy) -> LeadSelfEnergies:
-        """nested.rs:652-676 (FP64 mode matching on the host)."""
+    def self_energies(self, energy, eta=0.0) -> LeadSelfEnergies:
+        """nested.rs:652-676 (FP64 mode matching; the scalar-coupling leads on the GPU when one is
+        present). eta > 0: SelfEnergyMethod::ModesBroadened (the leads at E + i eta, the device
+        at E; two_terminal.rs:157-172)."""
         g = C.c_void_p()
-        _check(capi.cg_self_energies(self._h, float(energy), C.byref(g)), f"E = {energy}")
+        if eta:
+            _check(capi.cg_self_energies_broadened(self._h, float(energy), float(eta), C.byref(g)), f"E = {energy}")
+        else:
+            _check(capi.cg_self_energies(self._h, float(energy), C.byref(g)), f"E = {energy}")
         return LeadSelfEnergies(g.value, self)
 
     def sigma_from_arrays(self, energy, blocks) -> LeadSelfEnergies:

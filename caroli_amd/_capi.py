@@ -41,6 +41,9 @@ cg_system_create = _sig("cg_system_create", C.c_int32, C.c_uint64, P(C.c_int64),
                         C.c_uint64, P(C.c_int64), P(C.c_int32), P(C.c_uint64), C.c_int32, P(C.c_int64), C.c_int32,
                         P(C.c_double), P(C.c_double), C.c_uint64, P(C.c_double), P(C.c_double), C.c_uint64,
                         P(c_sys))
+cg_bhz_magnetic_create = _sig("cg_bhz_magnetic_create", C.c_int32, P(C.c_uint64), C.c_int32, C.c_int64, C.c_int64,
+                              C.c_int64, C.c_double, C.c_int32, C.c_double, C.c_int32, C.c_double, P(c_sys))
+cg_self_energies_broadened = _sig("cg_self_energies_broadened", C.c_int32, c_sys, C.c_double, C.c_double, P(c_sig))
 cg_preset_create = _sig("cg_preset_create", C.c_int32, C.c_int32, P(C.c_uint64), C.c_int32, C.c_int32, C.c_double,
                         C.c_double, C.c_uint64, P(C.c_double), P(c_sys))
 cg_system_destroy = _sig("cg_system_destroy", None, c_sys)
@@ -91,7 +94,7 @@ EXPORTS = [
     "cg_block_sizes", "cg_system_arrays", "cg_tree_node", "cg_work", "cg_self_energies", "cg_sigma_from_arrays",
     "cg_sigma_block", "cg_sigma_energy", "cg_sigma_destroy", "cg_transmission", "cg_transmission_with",
     "cg_transmissions", "cg_batch_transmission", "cg_backward", "cg_device_available",
-    "cg_batch_transmission_checked",
+    "cg_batch_transmission_checked", "cg_bhz_magnetic_create", "cg_self_energies_broadened",
 ]
 
 

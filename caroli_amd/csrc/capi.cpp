@@ -211,6 +211,26 @@ int32_t cg_system_create(uint64_t n, const int64_t* row_ptr, const uint32_t* col
     }
 }
 
+int32_t cg_bhz_magnetic_create(const uint64_t* extent, int32_t seed_mode, int64_t seed, int64_t x0, int64_t nx_imp,
+                               double concentration, int32_t moments, double moment_param, int32_t coupling,
+                               double exchange, cg_system** out) {
+    if (!out || !extent || extent[0] < 3 || extent[1] < 1 || !(concentration >= 0.0 && concentration <= 1.0))
+        return CG_ERR_INVALID;
+    try {
+        MagneticArgs m;
+        m.moments = moments;
+        m.moment_param = moment_param;
+        m.coupling = coupling;
+        m.exchange = exchange;
+        const size_t ext[2] = {size_t(extent[0]), size_t(extent[1])};
+        auto* s = new cg_system;
+        s->spec = bhz_magnetic_system(ext, seed_mode, seed, x0, nx_imp, concentration, m);
+        return finish(s, out);
+    } catch (const std::exception&) {
+        return CG_ERR_INVALID;
+    }
+}
+
 int32_t cg_preset_create(int32_t model, const uint64_t* extent, int32_t axis, int32_t disorder, double strength,
                          double concentration, uint64_t seed, const double* site_potential, cg_system** out) {
     if (!out || !extent) return CG_ERR_INVALID;
@@ -329,7 +349,7 @@ void ensure_modes(const cg_system* s) {
 
 // The scalar-coupling leads go to the GPU (leads.hip) when a device is present, unless
 // CAROLI_SIGMA_HOST=1 keeps them on the host.
-bool cell_on_gpu(const ModeSolver& m) {
+static bool cell_on_gpu(const ModeSolver& m) {
     if (!m.cell()) return false;
     const char* v = std::getenv("CAROLI_SIGMA_HOST");
     if (v && std::strcmp(v, "1") == 0) return false;
@@ -338,7 +358,7 @@ bool cell_on_gpu(const ModeSolver& m) {
 
 // Gamma = W W^H of a cell self-energy Sigma = Y diag(d) Y^H: Gamma = Y diag(-2 Im d) Y^H, so W is
 // the propagating eigenvectors scaled by sqrt(-2 Im d) (the evanescent modes have real d).
-CMat cell_gamma_factor(const CellModes& m, const std::vector<cplx>& d) {
+static CMat cell_gamma_factor(const CellModes& m, const std::vector<cplx>& d) {
     const CMat& y = *m.y;
     CMat w(y.rows, int(m.propagating.size()));
     for (int q = 0; q < w.cols; ++q) {
@@ -351,7 +371,7 @@ CMat cell_gamma_factor(const CellModes& m, const std::vector<cplx>& d) {
 
 // Block b's self-energies from cell modes on the GPU: Sigma_L from the left lead's modes, Sigma_R
 // from the right lead's (one launch when the leads are equal).
-Status cell_block_sigmas(const ModeSolver& left, const ModeSolver& right, bool same, double energy, cg_sigma::Block& sb) {
+static Status cell_block_sigmas(const ModeSolver& left, const ModeSolver& right, bool same, double energy, cg_sigma::Block& sb) {
     CellModes cl, cr;
     if (Status st = left.cell_modes_at(energy, cl); st != kOk) return st;
     if (same) {
@@ -383,8 +403,8 @@ Status cell_block_sigmas(const ModeSolver& left, const ModeSolver& right, bool s
     return kOk;
 }
 
-int32_t cg_self_energies(const cg_system* s, double energy, cg_sigma** out) {
-    if (!s || !out) return CG_ERR_INVALID;
+// Self-energies of every block at E + i eta (eta = 0: the real axis, nested.rs:652-676).
+static int32_t self_energies_at(const cg_system* s, double energy, double eta, cg_sigma** out) {
     try {
         ensure_modes(s);
         auto* g = new cg_sigma;
@@ -394,7 +414,7 @@ int32_t cg_self_energies(const cg_system* s, double energy, cg_sigma** out) {
             cg_sigma::Block sb;
             const ModeSolver& left = *s->left_modes[b];
             const ModeSolver& right = blk.same_lead ? left : *s->right_modes[b];
-            if (cell_on_gpu(left) && cell_on_gpu(right)) {
+            if (eta == 0.0 && cell_on_gpu(left) && cell_on_gpu(right)) {
                 if (Status st = cell_block_sigmas(left, right, blk.same_lead, energy, sb); st != kOk) {
                     delete g;
                     return st;
@@ -409,12 +429,14 @@ int32_t cg_self_energies(const cg_system* s, double energy, cg_sigma** out) {
                 continue;
             }
             LeadModes ml, mr;
-            Status st = s->left_modes[b]->modes_at(energy, ml);
+            Status st = eta == 0.0 ? s->left_modes[b]->modes_at(energy, ml)
+                                   : s->left_modes[b]->modes_at(cplx(energy, eta), ml);
             if (st == kOk) {
                 if (blk.same_lead)
                     mr = ml;
                 else
-                    st = s->right_modes[b]->modes_at(energy, mr);
+                    st = eta == 0.0 ? s->right_modes[b]->modes_at(energy, mr)
+                                    : s->right_modes[b]->modes_at(cplx(energy, eta), mr);
             }
             if (st != kOk) {
                 delete g;
@@ -441,6 +463,16 @@ int32_t cg_self_energies(const cg_system* s, double energy, cg_sigma** out) {
     } catch (const std::exception&) {
         return CG_ERR_EIGEN;
     }
+}
+
+int32_t cg_self_energies(const cg_system* s, double energy, cg_sigma** out) {
+    if (!s || !out) return CG_ERR_INVALID;
+    return self_energies_at(s, energy, 0.0, out);
+}
+
+int32_t cg_self_energies_broadened(const cg_system* s, double energy, double eta, cg_sigma** out) {
+    if (!s || !out || !(eta >= 0.0) || !std::isfinite(eta)) return CG_ERR_INVALID;
+    return self_energies_at(s, energy, eta, out);
 }
 
 int32_t cg_sigma_from_arrays(const cg_system* s, double energy, const double* sigmas, cg_sigma** out) {

@@ -377,6 +377,70 @@ SystemSpec system_from_models(const Model& device, const Model& lead, const Regi
     return s;
 }
 
+// crates/caroli-disorder/src/lib.rs:18-100 (choose_sites, Moments::draw, exchange)
+std::map<Site, CMat> magnetic_exchange(Xoshiro256& rng, const std::vector<Site>& candidates, size_t n,
+                                       const MagneticArgs& m) {
+    auto sites = choose_sites(rng, candidates, n);
+    // Moments::nz (lib.rs:44-52): all n_z draws first, then all azimuths.
+    auto nz_of = [&](double u) {
+        switch (m.moments) {
+            case 0:
+                return 2.0 * u - 1.0;
+            case 1:
+                return -(1.0 - std::cos(m.moment_param)) * u + 1.0;
+            case 2:
+                if (!(m.moment_param >= 0.0 && m.moment_param <= 1.0))
+                    throw std::invalid_argument("polarisation outside [0, 1]");
+                return 2.0 * (1.0 - m.moment_param) * u + 2.0 * m.moment_param - 1.0;
+            default:
+                throw std::invalid_argument("unknown moment distribution");
+        }
+    };
+    std::vector<double> nz(n), phi(n);
+    for (auto& z : nz) z = nz_of(rng.uniform());
+    for (auto& p : phi) p = 2.0 * M_PI * rng.uniform();
+    // bhz_spin (presets.rs:69-74): (s_x, s_y, s_z) for same g, times tau_z for opposite g.
+    const bool same = m.coupling == 1;
+    if (m.coupling != 0 && m.coupling != 1) throw std::invalid_argument("unknown g-factor coupling");
+    CMat sx = gamma_matrix(same ? "23" : "23p"), sy = gamma_matrix(same ? "31" : "31p"),
+         sz = gamma_matrix(same ? "12" : "12p");
+    std::map<Site, CMat> map;
+    for (size_t k = 0; k < n; ++k) {
+        double perp = std::sqrt(1.0 - nz[k] * nz[k]);
+        double v3[3] = {perp * std::cos(phi[k]), perp * std::sin(phi[k]), nz[k]};
+        // MagneticImpurities::exchange (lib.rs:90-100): J (n_x S_x + n_y S_y + n_z S_z)
+        CMat v = combine({{cplx(m.exchange * v3[0]), sx}, {cplx(m.exchange * v3[1]), sy}, {cplx(m.exchange * v3[2]), sz}});
+        auto it = map.find(sites[k]);
+        if (it == map.end())
+            map.emplace(sites[k], v);
+        else
+            for (size_t e = 0; e < v.a.size(); ++e) it->second.a[e] += v.a[e];
+    }
+    return map;
+}
+
+SystemSpec bhz_magnetic_system(const size_t extent[2], int seed_mode, int64_t seed, int64_t x0, int64_t nx_imp,
+                               double concentration, const MagneticArgs& m) {
+    Model lead = bhz(BhzParams{});
+    Region region = Region::cuboid(lead.lattice, {extent[0], extent[1], 0});
+    if (x0 < 0 || nx_imp < 0 || size_t(x0 + nx_imp) > extent[0]) throw std::invalid_argument("impurity region outside");
+    std::vector<Site> cand;
+    for (size_t y = 0; y < extent[1]; ++y)
+        for (int64_t x = 0; x < nx_imp; ++x) cand.push_back(Site{{x0 + x, int64_t(y), 0}, 0});
+    const size_t nimp = size_t(concentration * double(nx_imp) * double(extent[1]));
+    if (seed_mode != 0 && seed_mode != 1) throw std::invalid_argument("unknown seeding");
+    auto rng = seed_mode == 0 ? Xoshiro256::seed_from_u64(uint64_t(seed)) : Xoshiro256::gfortran(int32_t(seed));
+    auto map = std::make_shared<std::map<Site, CMat>>(magnetic_exchange(rng, cand, nimp, m));
+    Model device = lead;
+    device.extra.push_back([map](const Site& s, CMat& v) {
+        auto it = map->find(s);
+        if (it == map->end()) return false;
+        v = it->second;
+        return true;
+    });
+    return system_from_models(device, lead, region, 0);
+}
+
 SystemSpec preset_system(const PresetArgs& a) {
     Model lead = a.model == 0 ? bhz(BhzParams{}) : a.model == 1 ? square_model() : a.model == 2 ? graphene_model() : cubic_model();
     if (a.model < 0 || a.model > 3) throw std::invalid_argument("unknown preset model");
@@ -396,26 +460,15 @@ SystemSpec preset_system(const PresetArgs& a) {
         });
     } else if (a.disorder == 2) {
         if (a.model != 0) throw std::invalid_argument("magnetic impurities need the BHZ model");
+        // crates/caroli-cli/src/build.rs:296-306: floor(c #sites) impurities over the region
+        // sites in region order, Polarized { mz: 1 }, opposite g.
         size_t nimp = size_t(a.concentration * double(region.sites.size()));
-        auto sites = choose_sites(rng, region.sites, nimp);
-        // crates/caroli-disorder/src/lib.rs:54-64 with Polarized { mz: 1 }
-        std::vector<double> nz(nimp), phi(nimp);
-        const double mz = 1.0;
-        for (auto& z : nz) z = 2.0 * (1.0 - mz) * rng.uniform() + 2.0 * mz - 1.0;
-        for (auto& p : phi) p = 2.0 * M_PI * rng.uniform();
-        CMat sx = gamma_matrix("23p"), sy = gamma_matrix("31p"), sz = gamma_matrix("12p");
-        auto map = std::make_shared<std::map<Site, CMat>>();
-        for (size_t k = 0; k < nimp; ++k) {
-            double perp = std::sqrt(1.0 - nz[k] * nz[k]);
-            double n[3] = {perp * std::cos(phi[k]), perp * std::sin(phi[k]), nz[k]};
-            // crates/caroli-cli/src/build.rs:299-301: combine(j n_x S_x + j n_y S_y + j n_z S_z)
-            CMat v = combine({{cplx(a.strength * n[0]), sx}, {cplx(a.strength * n[1]), sy}, {cplx(a.strength * n[2]), sz}});
-            auto it = map->find(sites[k]);
-            if (it == map->end())
-                map->emplace(sites[k], v);
-            else
-                for (size_t e = 0; e < v.a.size(); ++e) it->second.a[e] += v.a[e];
-        }
+        MagneticArgs m;
+        m.moments = 2;
+        m.moment_param = 1.0;
+        m.coupling = 0;
+        m.exchange = a.strength;
+        auto map = std::make_shared<std::map<Site, CMat>>(magnetic_exchange(rng, region.sites, nimp, m));
         device.extra.push_back([map](const Site& s, CMat& v) {
             auto it = map->find(s);
             if (it == map->end()) return false;

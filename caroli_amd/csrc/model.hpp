@@ -165,4 +165,26 @@ struct PresetArgs {
 };
 SystemSpec preset_system(const PresetArgs& a);
 
+/// Classical magnetic impurities and their exchange (crates/caroli-disorder/src/lib.rs:18-100,
+/// caroli-model presets.rs:60-80): moments 0 isotropic (n_z uniform in [-1, 1]), 1 in a cone of
+/// half-angle moment_param (radians) around z, 2 polarized with mean m_z = moment_param; coupling
+/// 0 opposite g (s tau_z), 1 same g (s); exchange J (n . S) per impurity site.
+struct MagneticArgs {
+    int moments = 2;
+    double moment_param = 1.0;
+    int coupling = 0;
+    double exchange = 0.0;
+};
+/// n distinct sites among `candidates` (partial Fisher-Yates), then all n_z draws and all
+/// azimuths (MagneticImpurities::generate), and each site's 4 x 4 exchange matrix.
+std::map<Site, CMat> magnetic_exchange(Xoshiro256& rng, const std::vector<Site>& candidates, size_t n,
+                                       const MagneticArgs& m);
+
+/// BHZ ribbon extent[0] x extent[1] (axis 0, clean BHZ leads) with int(conc nx_imp ny) magnetic
+/// impurities among the sites x0 <= x < x0 + nx_imp, all y, listed with x fastest (the thesis
+/// impurity region, crates/caroli-validation/src/thesis.rs:11-16, :66-78), drawn from
+/// Xoshiro256::seed_from_u64(seed) (seed_mode 0) or Xoshiro256::gfortran(seed) (1).
+SystemSpec bhz_magnetic_system(const size_t extent[2], int seed_mode, int64_t seed, int64_t x0, int64_t nx_imp,
+                               double concentration, const MagneticArgs& m);
+
 }  // namespace caroli

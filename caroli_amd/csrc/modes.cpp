@@ -576,6 +576,18 @@ Status ModeSolver::modes_at(double e, LeadModes& out) const {
     return last;
 }
 
+Status ModeSolver::modes_at(cplx z, LeadModes& out) const {
+    if (z.imag() == 0.0) return modes_at(z.real(), out);
+    double scale = std::max(row_norm(h0_) + 2.0 * row_norm(h1_), std::numeric_limits<double>::min());
+    Status last = kEigen;
+    for (double shift : kShifts) {
+        Status st = solve_general(h0_, h1_, z + shift * scale, scale, out);
+        if (st == kOk) return kOk;
+        last = st;
+    }
+    return last;
+}
+
 // The cell case of modes_at (modes.rs:130-145, :621-710) without the bases: per energy shift, the
 // lambdas of every cell mode and the velocity test of the propagating ones (the cell eigenvectors
 // are orthonormal, so |L^-H y_k|^2 = 1 / c).

@@ -58,6 +58,9 @@ class ModeSolver {
     ModeSolver(const CMat& h0, const CMat& h1);
     /// Modes at the real energy `e`, shifted off band edges when needed.
     Status modes_at(double e, LeadModes& out) const;
+    /// Modes at a complex energy z (modes.rs:531-537): the definite fast path only on the real
+    /// axis, else the general pencil, with the same shifts.
+    Status modes_at(cplx z, LeadModes& out) const;
     /// The scalar-coupling case with the cell's eigenvectors cached (cell_modes_at applies).
     bool cell() const { return definite_ && definite_->cell; }
     /// The cell modes at `e`, shifted off band edges as modes_at does (no bases built).

@@ -69,6 +69,18 @@ int32_t cg_system_create(uint64_t n, const int64_t* row_ptr, const uint32_t* col
 int32_t cg_preset_create(int32_t model, const uint64_t* extent, int32_t axis, int32_t disorder, double strength,
                          double concentration, uint64_t seed, const double* site_potential, cg_system** out);
 
+/* BHZ ribbon (BhzParams::default(), axis 0, clean BHZ leads) of extent[0] x extent[1] sites with
+ * classical magnetic impurities, as MagneticImpurities::generate (crates/caroli-disorder/src/lib.rs:
+ * 18-100) and the thesis setups (crates/caroli-validation/src/thesis.rs:11-16, :66-78) build them:
+ * int(concentration * nx_imp * extent[1]) distinct sites among x0 <= x < x0 + nx_imp, all y,
+ * listed with x fastest, from Xoshiro256::seed_from_u64(seed) (seed_mode 0) or
+ * Xoshiro256::gfortran(seed) (seed_mode 1); moments 0 isotropic, 1 in a cone of half-angle
+ * moment_param (radians) around z, 2 polarized with mean m_z = moment_param; coupling 0 opposite g,
+ * 1 same g (bhz_spin, crates/caroli-model/src/presets.rs:60-74); exchange J (n . S) per site. */
+int32_t cg_bhz_magnetic_create(const uint64_t* extent, int32_t seed_mode, int64_t seed, int64_t x0, int64_t nx_imp,
+                               double concentration, int32_t moments, double moment_param, int32_t coupling,
+                               double exchange, cg_system** out);
+
 void cg_system_destroy(cg_system* s);
 
 /* The random stream of the builders (crates/caroli-disorder/src/rng.rs): `count` uniforms in
@@ -98,6 +110,9 @@ int32_t cg_work(const cg_system* s, double* flops, double* bytes);
 
 /* NestedTwoTerminal::self_energies (nested.rs:652-676): FP64 mode matching on the host. */
 int32_t cg_self_energies(const cg_system* s, double energy, cg_sigma** out);
+/* SelfEnergyMethod::ModesBroadened { eta } (crates/caroli-green/src/two_terminal.rs:51-53, :157-172):
+ * the leads' modes at E + i eta, the device staying at the real energy E (the thesis codes). */
+int32_t cg_self_energies_broadened(const cg_system* s, double energy, double eta, cg_sigma** out);
 /* Self-energies given by the caller (per block: Sigma_L nl x nl then Sigma_R nr x nr). */
 int32_t cg_sigma_from_arrays(const cg_system* s, double energy, const double* sigmas, cg_sigma** out);
 /* Copies Sigma_L and Sigma_R of block b (either may be null); channels = propagating modes. */

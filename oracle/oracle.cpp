@@ -506,6 +506,13 @@ void symbolic(const Csr& h, std::vector<ONode>& nodes) {
     }
 }
 
+// The thesis transmission (crates/caroli-validation/src/thesis.rs:89-95): Tr[Re(Gamma_L) G
+// Re(Gamma_R) G^H], the broadenings truncated to their real part element by element.
+Mat real_part(Mat m) {
+    for (auto& v : m.a) v = cx(v.real(), 0.0);
+    return m;
+}
+
 // Interleaved (re, im) doubles of the C interface -> the oracle's complex scalars.
 std::vector<cx> to_cx(const double* v, size_t words) {
     std::vector<cx> out(words);
@@ -629,9 +636,21 @@ int32_t oracle_node(void* h, int32_t b, int32_t k, uint64_t* ne, uint64_t* nb, u
     return 0;
 }
 
+static int32_t transmission_of(void* h, double energy, const double* sigmas, bool thesis, double* out_t);
+
 // transmission_with (nested.rs:682-713) at `energy`, with the self-energies of each block
 // concatenated (Sigma_L then Sigma_R, column-major complex). 0 = ok, 1 = Singular.
 int32_t oracle_transmission_with(void* h, double energy, const double* sigmas, double* out_t) {
+    return transmission_of(h, energy, sigmas, false, out_t);
+}
+
+// The same with the thesis formula (real parts of the broadenings), for the reference's
+// ti-cond2t golden data (crates/caroli-validation/tests/ti_conductance_disorder.rs:25-58).
+int32_t oracle_transmission_thesis(void* h, double energy, const double* sigmas, double* out_t) {
+    return transmission_of(h, energy, sigmas, true, out_t);
+}
+
+static int32_t transmission_of(void* h, double energy, const double* sigmas, bool thesis, double* out_t) {
     auto* sys = static_cast<OSystem*>(h);
     size_t words = 0;
     for (auto& blk : sys->blocks) words += blk.left.size() * blk.left.size() + blk.right.size() * blk.right.size();
@@ -649,7 +668,12 @@ int32_t oracle_transmission_with(void* h, double energy, const double* sigmas, d
             *out_t = NAN;
             return 1;
         }
-        total += caroli_trace(broadening(sl, nl), g, broadening(sr, nr));
+        Mat gl = broadening(sl, nl), gr = broadening(sr, nr);
+        if (thesis) {
+            gl = real_part(gl);
+            gr = real_part(gr);
+        }
+        total += caroli_trace(gl, g, gr);
     }
     *out_t = double(total);
     return 0;

@@ -34,6 +34,7 @@ def _load(path=LIB):
     lib.oracle_node.argtypes = [C.c_void_p, C.c_int32, C.c_int32, P(C.c_uint64), P(C.c_uint64), P(C.c_uint64),
                                 P(C.c_uint64)]
     lib.oracle_transmission_with.argtypes = [C.c_void_p, C.c_double, P(C.c_double), P(C.c_double)]
+    lib.oracle_transmission_thesis.argtypes = [C.c_void_p, C.c_double, P(C.c_double), P(C.c_double)]
     lib.oracle_green_columns.argtypes = [C.c_void_p, C.c_int32, C.c_double, P(C.c_double), P(C.c_double), C.c_int32,
                                          P(C.c_double)]
     lib.oracle_transmissions_parallel.argtypes = [P(C.c_void_p), P(C.c_double), P(P(C.c_double)), C.c_int64,
@@ -122,6 +123,13 @@ class OracleSystem:
         flat = self.pack(sigmas)
         t = C.c_double()
         rc = self.lib.oracle_transmission_with(self.h, float(energy), _d(flat), C.byref(t))
+        return t.value, rc
+
+    def transmission_thesis(self, energy, sigmas):
+        """Tr[Re(Gamma_L) G Re(Gamma_R) G^H] (the thesis formula, thesis.rs:89-95)."""
+        flat = self.pack(sigmas)
+        t = C.c_double()
+        rc = self.lib.oracle_transmission_thesis(self.h, float(energy), _d(flat), C.byref(t))
         return t.value, rc
 
     def green_columns(self, b, energy, sigma_l, sigma_r, which):

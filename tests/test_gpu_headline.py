@@ -145,3 +145,29 @@ def test_two_terminal_readme_example():
     say(f"[gpu] TwoTerminal BHZ 16x10 at E = 0: T = {t!r}")
     assert abs(t - 2.0) < 1e-8
     assert dev.transmission(0.0, ca.Precision.Mixed) == pytest.approx(t, abs=1e-8)
+
+
+def test_thesis_magnetic_ribbons_on_the_gpu():
+    """The three ti-cond2t magnetic setups (the ones tests/test_host.py pins against the reference's
+    golden data) on the GPU, rebuilt through cg_system_create from their arrays: the exact Caroli
+    transmission with the leads at E + 1e-5 i (full-rank broadenings: every lead orbital a channel)
+    equals the oracle's to 1e-10."""
+    fx = json.load(open(os.path.join(HERE, "golden", "ti_cond2t_magnetic.json")))
+    worst = 0.0
+    for name, case in fx["cases"].items():
+        for j in case["exchange"]:
+            s0 = ca.NestedTwoTerminal.bhz_magnetic([18, 10], seed=12345, x0=5, nx_imp=8, concentration=0.3,
+                                                   exchange=j, moments=case["moments"], moment_param=0.0,
+                                                   coupling=case["coupling"], seeding="gfortran")
+            s = rebuilt(s0.arrays())
+            o = oracle_of(s)
+            pg, ref = [], []
+            for e in case["energies"]:
+                g = s.self_energies(e, eta=1e-5)
+                pg.append(g)
+                ref.append(o.transmission_with(e, [g.block(b)[:2] for b in range(s.block_count())])[0])
+            t, st = ca.batch_transmission([s] * len(pg), pg, ca.Precision.Double)
+            assert (st == 0).all(), st
+            worst = max(worst, np.abs(t - np.array(ref)).max())
+    say(f"[gpu] thesis magnetic ribbons (3 cases x 3 J x 5 E): max |T - oracle| {worst:.1e}")
+    assert worst <= TOL

@@ -68,9 +68,13 @@ def test_gpu_cell_gamma_factor():
 
 def test_cubic_large_sizes_against_fixture():
     fx = json.load(open(os.path.join(HERE, "golden", "cubic_large.json")))
+    systems = {}
     for s_idx, L, e, t_ref in fx["points"]:
-        s = ca.NestedTwoTerminal.preset(ca.Model.Cubic, [2 * L, L, L], disorder=1, strength=16.5,
-                                        seed=1000 + int(s_idx))
+        key = (int(s_idx), int(L))
+        if key not in systems:  # one system (and one cell eigendecomposition) for its energies
+            systems[key] = ca.NestedTwoTerminal.preset(ca.Model.Cubic, [2 * key[1], key[1], key[1]], disorder=1,
+                                                       strength=16.5, seed=1000 + key[0])
+        s = systems[key]
         t0 = time.perf_counter()
         g = s.self_energies(e)
         ts = time.perf_counter() - t0

@@ -569,3 +569,54 @@ def test_checked_batch_sampling_failures_and_fallback(tmp_path):
     assert (rc, checked, failed, fell, prec) == (0, 9, 1, 0, 1) and rest[:10] == exact
     dt = np.array(rest[10:20])
     assert dt[7] == pytest.approx(1e-4) and np.isnan(dt[3]) and np.nansum(dt) == pytest.approx(1e-4)
+
+
+# -- magnetic impurities against the thesis golden data ---------------------------------------------
+
+def thesis_magnetic(case, j):
+    """The ti-cond2t magnetic setups (reference/baseline/cases/ti-cond2t-*/ti.ini) through the
+    engine's builder (cg_bhz_magnetic_create): 18 x 10 BHZ ribbon, impurity region x in [5, 13)."""
+    return ca.NestedTwoTerminal.bhz_magnetic([18, 10], seed=12345, x0=5, nx_imp=8, concentration=0.3, exchange=j,
+                                             moments=case["moments"], moment_param=0.0, coupling=case["coupling"],
+                                             seeding="gfortran")
+
+
+@pytest.mark.parametrize("name", ["same-g", "isotropic", "opposite-g"])
+def test_magnetic_impurities_reproduce_thesis_golden(name):
+    """crates/caroli-validation/tests/ti_conductance_disorder.rs:25-58: with the thesis' own
+    random configurations (Xoshiro256::gfortran(12345), the engine's choose_sites and moment draws),
+    the leads' self-energies at E + 1e-5 i and the thesis formula Tr[Re(G_L) G Re(G_R) G^H], the
+    45 golden conductances are reproduced to 1e-9. The exact Caroli transmission differs from them
+    by up to ~0.2 (the thesis' real-part truncation), which the second bound records."""
+    fx = json.load(open(os.path.join(ROOT, "tests", "golden", "ti_cond2t_magnetic.json")))
+    case = fx["cases"][name]
+    worst_thesis = worst_exact = 0.0
+    for iu, j in enumerate(case["exchange"]):
+        s = thesis_magnetic(case, j)
+        assert s.block_count() == (1 if name == "isotropic" else 2)  # spin mixing only for isotropic moments
+        o = OracleSystem(s.arrays())
+        for ie, e in enumerate(case["energies"]):
+            g = s.self_energies(e, eta=1e-5)
+            blocks = [g.block(b)[:2] for b in range(s.block_count())]
+            t, rc = o.transmission_thesis(e, blocks)
+            assert rc == 0
+            worst_thesis = max(worst_thesis, abs(t - case["conductance"][iu][ie]))
+            t_exact, rc = o.transmission_with(e, blocks)
+            worst_exact = max(worst_exact, abs(t_exact - case["conductance"][iu][ie]))
+    print(f"{name}: |thesis formula - golden| <= {worst_thesis:.1e}, |exact - golden| <= {worst_exact:.2f}")
+    assert worst_thesis <= fx["tolerance"]
+    assert 1e-3 < worst_exact < 0.3
+
+
+def test_magnetic_builder_matches_preset_for_the_cli_seeding():
+    """The general builder with CLI seeding over the whole region in region order is the preset of
+    the equal-size batch: cg_preset_create(BHZ, disorder 2) draws from seed_from_u64 over all
+    sites (x slowest), the builder's candidate list lists x fastest; a one-column-wide ribbon has
+    both orders equal, so the two must give the same Hamiltonian bit for bit."""
+    a = ca.NestedTwoTerminal.preset(ca.Model.BHZ, [12, 1, 0], disorder=2, strength=0.4, concentration=0.25, seed=3)
+    b = ca.NestedTwoTerminal.bhz_magnetic([12, 1], seed=3, x0=0, nx_imp=12, concentration=0.25, exchange=0.4,
+                                          moments="polarized", moment_param=1.0, coupling="opposite", seeding="u64")
+    aa, bb = a.arrays(), b.arrays()
+    for k in ("row_ptr", "cols", "vals"):
+        assert np.array_equal(aa[k], bb[k]), k
+    assert not np.array_equal(aa["vals"], ca.NestedTwoTerminal.preset(ca.Model.BHZ, [12, 1, 0]).arrays()["vals"])
