@@ -44,6 +44,16 @@ cg_system_create = _sig("cg_system_create", C.c_int32, C.c_uint64, P(C.c_int64),
 cg_bhz_magnetic_create = _sig("cg_bhz_magnetic_create", C.c_int32, P(C.c_uint64), C.c_int32, C.c_int64, C.c_int64,
                               C.c_int64, C.c_double, C.c_int32, C.c_double, C.c_int32, C.c_double, P(c_sys))
 cg_self_energies_broadened = _sig("cg_self_energies_broadened", C.c_int32, c_sys, C.c_double, C.c_double, P(c_sig))
+class Disorder(C.Structure):
+    """cg_disorder: one [[disorder]] table of the CLI input."""
+    _fields_ = [("kind", C.c_int32), ("strength", C.c_double), ("concentration", C.c_double),
+                ("exchange", C.c_double), ("moments", C.c_int32), ("moment_param", C.c_double),
+                ("spin", C.c_int32), ("has_cells", C.c_int32 * 3), ("cells", (C.c_int64 * 2) * 3),
+                ("sublattices", C.c_uint64), ("seed", C.c_uint64)]
+
+
+cg_disordered_create = _sig("cg_disordered_create", C.c_int32, C.c_int32, P(C.c_uint64), C.c_int32, C.c_int64,
+                            P(Disorder), C.c_uint64, P(c_sys))
 cg_preset_create = _sig("cg_preset_create", C.c_int32, C.c_int32, P(C.c_uint64), C.c_int32, C.c_int32, C.c_double,
                         C.c_double, C.c_uint64, P(C.c_double), P(c_sys))
 cg_system_destroy = _sig("cg_system_destroy", None, c_sys)
@@ -95,6 +105,7 @@ EXPORTS = [
     "cg_sigma_block", "cg_sigma_energy", "cg_sigma_destroy", "cg_transmission", "cg_transmission_with",
     "cg_transmissions", "cg_batch_transmission", "cg_backward", "cg_device_available",
     "cg_batch_transmission_checked", "cg_bhz_magnetic_create", "cg_self_energies_broadened",
+    "cg_disordered_create",
 ]
 
 

@@ -400,10 +400,24 @@ std::map<Site, CMat> magnetic_exchange(Xoshiro256& rng, const std::vector<Site>&
     for (auto& z : nz) z = nz_of(rng.uniform());
     for (auto& p : phi) p = 2.0 * M_PI * rng.uniform();
     // bhz_spin (presets.rs:69-74): (s_x, s_y, s_z) for same g, times tau_z for opposite g.
+    // Pauli matrices for two-orbital sites (crates/caroli-cli/src/build.rs:322-333).
     const bool same = m.coupling == 1;
-    if (m.coupling != 0 && m.coupling != 1) throw std::invalid_argument("unknown g-factor coupling");
-    CMat sx = gamma_matrix(same ? "23" : "23p"), sy = gamma_matrix(same ? "31" : "31p"),
-         sz = gamma_matrix(same ? "12" : "12p");
+    if (m.coupling < 0 || m.coupling > 2) throw std::invalid_argument("unknown spin matrices");
+    CMat sx, sy, sz;
+    if (m.coupling == 2) {
+        sx = CMat(2, 2);
+        sy = CMat(2, 2);
+        sz = CMat(2, 2);
+        sx(0, 1) = sx(1, 0) = 1.0;
+        sy(0, 1) = cplx(0.0, -1.0);
+        sy(1, 0) = cplx(0.0, 1.0);
+        sz(0, 0) = 1.0;
+        sz(1, 1) = -1.0;
+    } else {
+        sx = gamma_matrix(same ? "23" : "23p");
+        sy = gamma_matrix(same ? "31" : "31p");
+        sz = gamma_matrix(same ? "12" : "12p");
+    }
     std::map<Site, CMat> map;
     for (size_t k = 0; k < n; ++k) {
         double perp = std::sqrt(1.0 - nz[k] * nz[k]);
@@ -439,6 +453,50 @@ SystemSpec bhz_magnetic_system(const size_t extent[2], int seed_mode, int64_t se
         return true;
     });
     return system_from_models(device, lead, region, 0);
+}
+
+SystemSpec disordered_system(int model, const size_t extent[3], int axis, const std::vector<DisorderSpec>& disorders) {
+    if (model < 0 || model > 3) throw std::invalid_argument("unknown preset model");
+    Model lead = model == 0 ? bhz(BhzParams{}) : model == 1 ? square_model() : model == 2 ? graphene_model() : cubic_model();
+    Region region = Region::cuboid(lead.lattice, {extent[0], extent[1], extent[2]});
+    Model device = lead;
+    for (const DisorderSpec& d : disorders) {
+        std::vector<Site> sites;
+        for (const Site& s : region.sites) {
+            bool in = true;
+            for (int k = 0; k < 3; ++k)
+                if (d.has_cells[k]) in = in && s.cell[k] >= d.cells[k][0] && s.cell[k] < d.cells[k][1];
+            if (d.sublattices) in = in && s.sub < 64 && ((d.sublattices >> s.sub) & 1u);
+            if (in) sites.push_back(s);
+        }
+        auto rng = Xoshiro256::seed_from_u64(d.seed);
+        if (d.kind == 0) {
+            // Anderson::generate (crates/caroli-disorder/src/lib.rs:112-115): w (u - 1/2) per site.
+            auto map = std::make_shared<std::map<Site, double>>();
+            for (auto& s : sites) (*map)[s] = d.strength * (rng.uniform() - 0.5);
+            device.extra.push_back([map](const Site& s, CMat& v) {
+                auto it = map->find(s);
+                if (it == map->end()) return false;
+                for (int i = 0; i < v.rows; ++i) v(i, i) = it->second;
+                return true;
+            });
+        } else if (d.kind == 1) {
+            const int orbitals = lead.lattice.subs.empty() ? 0 : lead.lattice.subs[0].orbitals;
+            if ((d.magnetic.coupling == 2) != (orbitals == 2) || (d.magnetic.coupling != 2 && orbitals != 4))
+                throw std::invalid_argument("spin matrices of another size than the sites' orbitals");
+            const size_t n = size_t(d.concentration * double(sites.size()));
+            auto map = std::make_shared<std::map<Site, CMat>>(magnetic_exchange(rng, sites, n, d.magnetic));
+            device.extra.push_back([map](const Site& s, CMat& v) {
+                auto it = map->find(s);
+                if (it == map->end()) return false;
+                v = it->second;
+                return true;
+            });
+        } else {
+            throw std::invalid_argument("unknown disorder");
+        }
+    }
+    return system_from_models(device, lead, region, axis);
 }
 
 SystemSpec preset_system(const PresetArgs& a) {

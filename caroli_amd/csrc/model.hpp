@@ -180,6 +180,24 @@ struct MagneticArgs {
 std::map<Site, CMat> magnetic_exchange(Xoshiro256& rng, const std::vector<Site>& candidates, size_t n,
                                        const MagneticArgs& m);
 
+/// One disorder of the CLI input (crates/caroli-cli/src/input.rs:131-170): on the region's sites
+/// inside `cells` (per direction [first, end) when has_cells[d]) and of the sublattices in the mask
+/// (0: all), in region order, drawn from Xoshiro256::seed_from_u64(seed).
+struct DisorderSpec {
+    int kind = 0;  // 0 Anderson (strength W), 1 magnetic
+    double strength = 0.0;
+    double concentration = 0.0;
+    MagneticArgs magnetic;  // coupling 2: the Pauli matrices (two-orbital sites)
+    int64_t cells[3][2] = {{0, 0}, {0, 0}, {0, 0}};
+    bool has_cells[3] = {false, false, false};
+    uint64_t sublattices = 0;
+    uint64_t seed = 1;
+};
+/// The CLI's disordered model (crates/caroli-cli/src/build.rs:251-310): the preset `model` (0 BHZ,
+/// 1 square, 2 zigzag graphene, 3 cubic) on Region::cuboid(extent) with every disorder added, as a
+/// system along `axis` with the clean preset as leads.
+SystemSpec disordered_system(int model, const size_t extent[3], int axis, const std::vector<DisorderSpec>& disorders);
+
 /// BHZ ribbon extent[0] x extent[1] (axis 0, clean BHZ leads) with int(conc nx_imp ny) magnetic
 /// impurities among the sites x0 <= x < x0 + nx_imp, all y, listed with x fastest (the thesis
 /// impurity region, crates/caroli-validation/src/thesis.rs:11-16, :66-78), drawn from

@@ -81,6 +81,27 @@ int32_t cg_bhz_magnetic_create(const uint64_t* extent, int32_t seed_mode, int64_
                                double concentration, int32_t moments, double moment_param, int32_t coupling,
                                double exchange, cg_system** out);
 
+/* One [[disorder]] table of the CLI input (crates/caroli-cli/src/input.rs:131-170). */
+typedef struct cg_disorder {
+  int32_t kind;          /* 0 anderson, 1 magnetic */
+  double strength;       /* anderson: on-site energies uniform in [-W/2, W/2] */
+  double concentration;  /* magnetic: fraction of the candidate sites carrying a moment */
+  double exchange;       /* magnetic: J of J n . S */
+  int32_t moments;       /* 0 isotropic, 1 cone (moment_param = theta0, radians), 2 polarized (moment_param = mz) */
+  double moment_param;
+  int32_t spin;          /* S: 0 opposite-g, 1 same-g (four-orbital BHZ sites), 2 Pauli (two-orbital sites) */
+  int32_t has_cells[3];  /* direction d restricted to cells[d][0] <= cell < cells[d][1] */
+  int64_t cells[3][2];
+  uint64_t sublattices;  /* bit mask of the sublattices carrying it (0: all) */
+  uint64_t seed;         /* configuration c draws from Xoshiro256::seed_from_u64(seed + c) */
+} cg_disorder;
+
+/* The CLI's system of configuration `config` (crates/caroli-cli/src/build.rs:251-310, run.rs:63-68):
+ * preset `model` (as cg_preset_create) on Region::cuboid(extent), every disorder added in order,
+ * leads the clean preset along `axis`. */
+int32_t cg_disordered_create(int32_t model, const uint64_t* extent, int32_t axis, int64_t count,
+                             const cg_disorder* disorders, uint64_t config, cg_system** out);
+
 void cg_system_destroy(cg_system* s);
 
 /* The random stream of the builders (crates/caroli-disorder/src/rng.rs): `count` uniforms in
