@@ -20,6 +20,7 @@ from . import _capi as capi
 __all__ = [
     "GreenError", "Precision", "PrecisionOptions", "PrecisionReport", "LeadSelfEnergies", "NestedTwoTerminal",
     "TwoTerminal", "Model", "device_available", "batch_transmission", "batch_transmission_checked",
+    "backward_batch",
 ]
 
 
@@ -425,6 +426,21 @@ def batch_transmission_checked(systems, sigmas, options: PrecisionOptions, stats
     report = PrecisionReport(Precision(rep.precision), rep.checked, rep.median, rep.max, rep.failed,
                              rep.fallback.decode(), dt)
     return np.where(st == 0, t, np.nan), st, report
+
+
+def backward_batch(systems, sigmas):
+    """T and dT/d(eps_a) of every orbital for many points of one structure in one sweep
+    (cg_backward_batch): returns (T[count], grad[count, n], status[count])."""
+    n = len(systems)
+    nn = systems[0].sizes()[0] if n else 0
+    sh = (C.c_void_p * n)(*[s._h.value for s in systems])
+    gh = (C.c_void_p * n)(*[g._h.value for g in sigmas])
+    t = np.zeros(n)
+    grad = np.zeros((n, nn))
+    st = np.zeros(n, np.int32)
+    _check(capi.cg_backward_batch(sh, gh, n, _dptr(t), _dptr(grad), st.ctypes.data_as(C.POINTER(C.c_int32))),
+           "backward batch")
+    return t, grad, st
 
 
 class ResidentBatch:

@@ -80,6 +80,8 @@ cg_transmissions = _sig("cg_transmissions", C.c_int32, c_sys, P(C.c_double), C.c
 cg_batch_transmission = _sig("cg_batch_transmission", C.c_int32, P(c_sys), P(c_sig), C.c_int64, C.c_int32,
                              P(C.c_double), P(C.c_int32), P(BatchStats))
 cg_backward = _sig("cg_backward", C.c_int32, c_sys, c_sig, P(C.c_double), P(C.c_double))
+cg_backward_batch = _sig("cg_backward_batch", C.c_int32, P(c_sys), P(c_sig), C.c_int64, P(C.c_double),
+                         P(C.c_double), P(C.c_int32))
 
 
 class PrecisionOptionsC(C.Structure):
@@ -105,7 +107,7 @@ EXPORTS = [
     "cg_sigma_block", "cg_sigma_energy", "cg_sigma_destroy", "cg_transmission", "cg_transmission_with",
     "cg_transmissions", "cg_batch_transmission", "cg_backward", "cg_device_available",
     "cg_batch_transmission_checked", "cg_bhz_magnetic_create", "cg_self_energies_broadened",
-    "cg_disordered_create",
+    "cg_disordered_create", "cg_backward_batch",
 ]
 
 

@@ -732,6 +732,18 @@ int32_t cg_backward(const cg_system* s, const cg_sigma* g, double* out_t, double
     }
 }
 
+int32_t cg_backward_batch(const cg_system* const* systems, const cg_sigma* const* sigmas, int64_t count, double* out_t,
+                          double* out_grad, int32_t* out_status) {
+    if (count < 0 || (count > 0 && (!systems || !sigmas || !out_t || !out_grad))) return CG_ERR_INVALID;
+    for (int64_t p = 0; p < count; ++p)
+        if (!systems[p] || !sigmas[p] || !sigma_matches(*systems[p], *sigmas[p])) return CG_ERR_INVALID;
+    try {
+        return gpu::backward_batch(systems, sigmas, count, out_t, out_grad, out_status);
+    } catch (const std::exception&) {
+        return CG_ERR_DEVICE;
+    }
+}
+
 int32_t cg_scattering_states(const cg_system* s, const cg_sigma* g, int64_t* cols, double* psi_left,
                              double* psi_right) {
     if (!s || !g || !cols || !sigma_matches(*s, *g)) return CG_ERR_INVALID;

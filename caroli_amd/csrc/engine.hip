@@ -603,8 +603,14 @@ struct Lane {
 constexpr int kLanes = 4;
 constexpr size_t kMinUnitPoints = 32;
 
+/// Device vectors of the batched adjoint, kept between calls: W_R, W_L, psi, phi, t, gradients.
+struct AdjointBufs {
+    DevBuf wr, wl, psi, phi, t, grad;
+};
+
 struct Engine {
     std::mutex mu;
+    AdjointBufs adjoint;
     double tol_single = kTolSingle;  // pivot test of complex f32 fronts in the current run
     bool ok = false;
     std::map<uint64_t, std::weak_ptr<Structure>> registry;
@@ -2212,50 +2218,139 @@ void solve_columns(Engine& eng, const HostPlan& hp, const DevPlan& dp, const Poi
 
 }  // namespace
 
-int32_t backward(const cg_system* s, const cg_sigma* g, double* out_t, double* out_grad) {
+int32_t backward_batch(const cg_system* const* systems, const cg_sigma* const* sigmas, int64_t count, double* out_t,
+                       double* out_grad, int32_t* out_status) {
     Engine& eng = Engine::get();
     if (!eng.ok) return CG_ERR_DEVICE;
+    if (count <= 0) return CG_OK;
+    for (int64_t p = 1; p < count; ++p)
+        if (systems[p]->key != systems[0]->key) return CG_ERR_INVALID;  // one structure per call
     std::lock_guard<std::mutex> lock(eng.mu);
-    auto sp = structure_of(eng, s);
-    const int64_t n_orb = int64_t(s->spec.h.n);
-    std::fill(out_grad, out_grad + n_orb, 0.0);
-    double total = 0.0;
+    auto sp = structure_of(eng, systems[0]);
+    for (int64_t p = 0; p < count; ++p) structure_of(eng, systems[p]);
+    const int64_t n_orb = int64_t(systems[0]->spec.h.n);
+    std::fill(out_grad, out_grad + count * n_orb, 0.0);
+    std::fill(out_t, out_t + count, 0.0);
+    std::vector<int32_t> status(count, CG_OK);
+    eng.lane = &eng.lanes[0];
+    Lane& ln = *eng.lane;
+    hipStream_t st = ln.stream;
     for (size_t b = 0; b < sp->host.size(); ++b) {
         const HostPlan& hp = sp->host[b];
         const DevPlan& dp = sp->dev[b];
-        const auto& sb = g->blocks[b];
-        PointsView pv{};
-        const int32_t rc = factor_point(eng, s, g, b, hp, dp, pv);
-        if (rc != CG_OK) return rc;  // delayed separators are not supported by the adjoint
-        const int cr = sb.wr.cols, cl = sb.wl.cols;
-        const int nl = int(hp.left_orbs.size()), nr = int(hp.right_orbs.size());
-        if (cr == 0 || cl == 0) continue;
+        const Block& blk = systems[0]->sym.blocks[b];
+        const int nl = int(hp.left_orbs.size()), nr = int(hp.right_orbs.size()), nrows = int(hp.rows.size());
+        int cl = 0, cr = 0;
+        for (int64_t p = 0; p < count; ++p) {
+            cl = std::max(cl, sigmas[p]->blocks[b].wl.cols);
+            cr = std::max(cr, sigmas[p]->blocks[b].wr.cols);
+        }
+        if (cl == 0 || cr == 0) continue;  // no channel: T and the gradient of the block are 0
         if (int64_t(hp.max_ne) * cl > kSolveLds || !solve_fits(hp.max_n) || !solve_fits(hp.nodes[hp.root].ne))
             return CG_ERR_INVALID;  // the transposed solves hold all cl columns of a front in LDS
-        std::vector<double2> psi, phi;
-        solve_columns(eng, hp, dp, pv, n_orb, sb.wr, nr, dp.right_orbs, false, psi);  // psi = G E_R W_R
-        solve_columns(eng, hp, dp, pv, n_orb, sb.wl, nl, dp.left_orbs, true, phi);    // phi = G^T E_L conj(W_L)
-        auto at = [&](const std::vector<double2>& m, int64_t row, int col) {
-            double2 v = m[size_t(row) + size_t(col) * size_t(n_orb)];
-            return cplx(v.x, v.y);
-        };
-        CMat t(cl, cr);  // t = W_L^H psi_L
-        for (int c = 0; c < cr; ++c)
-            for (int d = 0; d < cl; ++d) {
-                cplx acc = 0.0;
-                for (int a = 0; a < nl; ++a) acc += std::conj(sb.wl(a, d)) * at(psi, hp.left_orbs[a], c);
-                t(d, c) = acc;
-                total += std::norm(acc);
+        // Points per chunk: the stored fronts, the root and the two sets of columns within 64 GB.
+        const size_t per_point = size_t(hp.arena_stride) * sizeof(double2) + size_t(hp.root_stride) * sizeof(double2) +
+                                 size_t(hp.piv_stride) * 4 + size_t(hp.slots) * 4 +
+                                 size_t(n_orb) * (size_t(cl) + size_t(cr) + 1) * sizeof(double2);
+        const int64_t chunk = std::max<int64_t>(1, std::min<int64_t>({count, int64_t((size_t(64) << 30) / per_point), 65535}));
+        for (int64_t c0 = 0; c0 < count; c0 += chunk) {
+            const int32_t P = int32_t(std::min(chunk, count - c0));
+            std::vector<const cg_system*> ss(systems + c0, systems + c0 + P);
+            std::vector<const cg_sigma*> gg(sigmas + c0, sigmas + c0 + P);
+            // 1. The fronts of every point, factors kept (complex f64).
+            PointsView pv{};
+            stage_points(eng, ss, gg, int(b), int(blk.left_orbitals.size()), int(blk.right_orbitals.size()),
+                         eng.scratch_in, pv);
+            ensure_lane(ln, lane_need(hp, size_t(P), false));
+            pv.arena = ln.arena.p;
+            pv.arena_stride = hp.arena_stride;
+            pv.root_arena = ln.root_arena.p;
+            pv.root_stride = hp.root_stride;
+            pv.piv = ln.piv.as<int32_t>();
+            pv.piv_stride = hp.piv_stride;
+            pv.status = ln.status.as<int32_t>();
+            pv.status_stride = hp.slots;
+            pv.t_out = ln.tout.as<double>();
+            pv.keep = 1;
+            sweep(eng, hp, dp, pv, P, false, eng.scratch_in);
+            std::vector<int32_t> fs(size_t(P) * hp.slots);
+            HIP_CHECK(hipMemcpyAsync(fs.data(), ln.status.p, fs.size() * 4, hipMemcpyDeviceToHost, st));
+            // 2. psi = G E_R W_R and phi = G^T E_L conj(W_L) for all points (W zero-padded to the
+            //    largest channel counts), the root and the backward substitution only.
+            std::vector<double2> wr(size_t(P) * nr * cr, double2{0.0, 0.0}), wl(size_t(P) * nl * cl, double2{0.0, 0.0});
+            for (int32_t q = 0; q < P; ++q) {
+                const auto& sb = gg[q]->blocks[b];
+                for (int col = 0; col < sb.wr.cols; ++col)
+                    for (int a = 0; a < nr; ++a)
+                        wr[size_t(q) * nr * cr + a + size_t(col) * nr] = double2{sb.wr(a, col).real(), sb.wr(a, col).imag()};
+                for (int col = 0; col < sb.wl.cols; ++col)
+                    for (int a = 0; a < nl; ++a)
+                        wl[size_t(q) * nl * cl + a + size_t(col) * nl] = double2{sb.wl(a, col).real(), sb.wl(a, col).imag()};
             }
-        for (int32_t row : hp.rows) {
-            cplx acc = 0.0;
-            for (int c = 0; c < cr; ++c)
-                for (int d = 0; d < cl; ++d) acc += at(psi, row, c) * std::conj(t(d, c)) * at(phi, row, d);
-            out_grad[row] += 2.0 * acc.real();
+            AdjointBufs& ab = eng.adjoint;
+            upload(ab.wr, wr, st);
+            upload(ab.wl, wl, st);
+            const size_t psi_bytes = size_t(P) * size_t(n_orb) * cr * sizeof(double2);
+            const size_t phi_bytes = size_t(P) * size_t(n_orb) * cl * sizeof(double2);
+            ab.psi.ensure(psi_bytes);
+            ab.phi.ensure(phi_bytes);
+            ab.t.ensure(size_t(P) * cl * cr * sizeof(double2));
+            ab.grad.ensure(size_t(P) * size_t(n_orb) * sizeof(double));
+            HIP_CHECK(hipMemsetAsync(ab.psi.p, 0, psi_bytes, st));
+            HIP_CHECK(hipMemsetAsync(ab.phi.p, 0, phi_bytes, st));
+            HIP_CHECK(hipMemsetAsync(ab.grad.p, 0, size_t(P) * size_t(n_orb) * sizeof(double), st));
+            SolveView psi{ab.psi.as<double2>(), n_orb * cr, int32_t(n_orb), cr, dp.vars.as<int32_t>()};
+            SolveView phi{ab.phi.as<double2>(), n_orb * cl, int32_t(n_orb), cl, dp.vars.as<int32_t>()};
+            launch_scatter(psi, dp.right_orbs.as<int32_t>(), nr, ab.wr.as<double2>(), int64_t(nr) * cr, false, P, st);
+            launch_scatter(phi, dp.left_orbs.as<int32_t>(), nl, ab.wl.as<double2>(), int64_t(nl) * cl, true, P, st);
+            solve(eng, hp, dp, pv, psi, P, false, false, true);
+            solve(eng, hp, dp, pv, phi, P, false, true, true);
+            // 3. The contraction on the GPU: t = W_L^H psi_L and the gradient rows of the block.
+            launch_adjoint(psi, phi, dp.left_orbs.as<int32_t>(), nl, ab.wl.as<double2>(), ab.t.as<double2>(),
+                           dp.rows.as<int32_t>(), nrows, ab.grad.as<double>(), n_orb, P, st);
+            HIP_CHECK(hipGetLastError());
+            std::vector<double2> t(size_t(P) * cl * cr);
+            std::vector<double> grad(size_t(P) * size_t(n_orb));
+            HIP_CHECK(hipMemcpyAsync(t.data(), ab.t.p, t.size() * sizeof(double2), hipMemcpyDeviceToHost, st));
+            HIP_CHECK(hipMemcpyAsync(grad.data(), ab.grad.p, grad.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+            HIP_CHECK(hipStreamSynchronize(st));
+            eng.collect();
+            for (int32_t q = 0; q < P; ++q) {
+                const int64_t p = c0 + q;
+                bool delayed = false;
+                for (int32_t k = 0; k < hp.root; ++k) delayed |= fs[size_t(q) * hp.slots + k] != kFrontOk;
+                if (delayed) {
+                    status[p] = CG_ERR_INVALID;  // delayed separators: the solves follow the static plan
+                    continue;
+                }
+                if (fs[size_t(q) * hp.slots + hp.root] != kFrontOk) {
+                    status[p] = CG_ERR_SINGULAR;
+                    continue;
+                }
+                double tb = 0.0;
+                for (size_t e = 0; e < size_t(cl) * cr; ++e) {
+                    const double2 v = t[size_t(q) * cl * cr + e];
+                    tb += v.x * v.x + v.y * v.y;
+                }
+                out_t[p] += tb;
+                for (int32_t row : hp.rows) out_grad[p * n_orb + row] += grad[size_t(q) * n_orb + row];
+            }
         }
     }
-    *out_t = total;
+    for (int64_t p = 0; p < count; ++p) {
+        if (status[p] != CG_OK) {
+            out_t[p] = NAN;
+            std::fill(out_grad + p * n_orb, out_grad + (p + 1) * n_orb, NAN);
+        }
+        if (out_status) out_status[p] = status[p];
+    }
     return CG_OK;
+}
+
+int32_t backward(const cg_system* s, const cg_sigma* g, double* out_t, double* out_grad) {
+    int32_t st = CG_OK;
+    const int32_t rc = backward_batch(&s, &g, 1, out_t, out_grad, &st);
+    return rc != CG_OK ? rc : st;
 }
 
 int32_t scattering_states(const cg_system* s, const cg_sigma* g, int64_t* cols, double* out_l, double* out_r) {

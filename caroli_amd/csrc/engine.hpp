@@ -30,6 +30,9 @@ int32_t front_schur_noalloc(const double* f, int64_t n, int64_t ne, int64_t ne1,
 
 /// T and dT/d(eps_a) for every orbital, from the stored fronts of one point.
 int32_t backward(const cg_system* s, const cg_sigma* g, double* out_t, double* out_grad);
+/// The same for `count` points of one structure in one sweep (out_grad: count x n).
+int32_t backward_batch(const cg_system* const* systems, const cg_sigma* const* sigmas, int64_t count, double* out_t,
+                       double* out_grad, int32_t* out_status);
 
 /// psi_L = G E_L W_L and psi_R = G E_R W_R on every orbital (cols[0], cols[1] channels, the
 /// blocks' columns one after another); with both outputs NULL only the channel counts.

@@ -216,5 +216,11 @@ void launch_residual(const PointsView& pv, const SolveView& xv, const double2* b
 void launch_update(const SolveView& xv, const double2* d, const int32_t* rows, int32_t nrows, double* maxes,
                    int32_t points, int32_t all_points, hipStream_t st);
 void launch_gather(const SolveView& xv, const int32_t* orbs, int32_t m, double2* out, int32_t points, hipStream_t st);
+/// The adjoint's contraction per point: t = W_L^H psi_L (cl x cr, cl = phi.c, cr = psi.c; W_L is
+/// nl x cl per point) and grad[p n_orb + a] = 2 Re sum_{c,d} psi[a, c] conj(t[d, c]) phi[a, d] on
+/// the `rows` of the block (psi = G E_R W_R, phi = G^T E_L conj(W_L)).
+void launch_adjoint(const SolveView& psi, const SolveView& phi, const int32_t* left_orbs, int32_t nl,
+                    const double2* wl, double2* t, const int32_t* rows, int32_t nrows, double* grad, int64_t n_orb,
+                    int32_t points, hipStream_t st);
 
 }  // namespace caroli::gpu

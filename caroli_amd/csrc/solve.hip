@@ -545,7 +545,65 @@ __global__ void __launch_bounds__(kThreads)
         O[idx] = X[orbs[idx % m] + int64_t(idx / m) * xv.ldx];
 }
 
+// The adjoint's small product t = W_L^H psi_L per point (cl x cr, column-major, stride cl * cr):
+// one thread per entry, the dot product over the nl left-slice orbitals in a fixed order.
+__global__ void __launch_bounds__(kThreads)
+    adjoint_t_kernel(SolveView psi, const int32_t* __restrict__ left_orbs, int32_t nl, const double2* __restrict__ wl,
+                     int32_t cl, double2* __restrict__ t) {
+    const int64_t p = blockIdx.y;
+    const int e = blockIdx.x * kThreads + threadIdx.x;
+    if (e >= cl * psi.c) return;
+    const int d = e % cl, c = e / cl;
+    const double2* W = wl + p * int64_t(nl) * cl + int64_t(d) * nl;
+    const double2* X = psi.x + p * psi.x_stride + int64_t(c) * psi.ldx;
+    double2 acc{0.0, 0.0};
+    for (int a = 0; a < nl; ++a) {
+        const double2 w = W[a], x = X[left_orbs[a]];
+        acc.x += w.x * x.x + w.y * x.y;  // conj(w) x
+        acc.y += w.x * x.y - w.y * x.x;
+    }
+    t[p * int64_t(cl) * psi.c + e] = acc;
+}
+
+// dT/d(eps_a) = 2 Re sum_{c,d} psi[a, c] conj(t[d, c]) phi[a, d] on the rows of a block, one
+// thread per (row, point): no accumulation across threads, so the result does not depend on the
+// schedule. grad has n_orb entries per point.
+__global__ void __launch_bounds__(kThreads)
+    adjoint_grad_kernel(SolveView psi, SolveView phi, const double2* __restrict__ t, const int32_t* __restrict__ rows,
+                        int32_t nrows, double* __restrict__ grad, int64_t n_orb) {
+    const int64_t p = blockIdx.y;
+    const int i = blockIdx.x * kThreads + threadIdx.x;
+    if (i >= nrows) return;
+    const int r = rows[i], cr = psi.c, cl = phi.c;
+    const double2* X = psi.x + p * psi.x_stride;
+    const double2* Y = phi.x + p * phi.x_stride;
+    const double2* T = t + p * int64_t(cl) * cr;
+    double acc = 0.0;
+    for (int c = 0; c < cr; ++c) {
+        double2 u{0.0, 0.0};  // u = sum_d conj(t[d, c]) phi[r, d]
+        for (int d = 0; d < cl; ++d) {
+            const double2 tv = T[d + int64_t(c) * cl], y = Y[r + int64_t(d) * phi.ldx];
+            u.x += tv.x * y.x + tv.y * y.y;
+            u.y += tv.x * y.y - tv.y * y.x;
+        }
+        const double2 x = X[r + int64_t(c) * psi.ldx];
+        acc += x.x * u.x - x.y * u.y;
+    }
+    grad[p * n_orb + r] = 2.0 * acc;
+}
+
 }  // namespace
+
+void launch_adjoint(const SolveView& psi, const SolveView& phi, const int32_t* left_orbs, int32_t nl,
+                    const double2* wl, double2* t, const int32_t* rows, int32_t nrows, double* grad, int64_t n_orb,
+                    int32_t points, hipStream_t st) {
+    if (points <= 0 || psi.c <= 0 || phi.c <= 0) return;
+    const int entries = phi.c * psi.c;
+    hipLaunchKernelGGL(adjoint_t_kernel, dim3(unsigned((entries + kThreads - 1) / kThreads), unsigned(points)),
+                       dim3(kThreads), 0, st, psi, left_orbs, nl, wl, phi.c, t);
+    hipLaunchKernelGGL(adjoint_grad_kernel, dim3(unsigned((nrows + kThreads - 1) / kThreads), unsigned(points)),
+                       dim3(kThreads), 0, st, psi, phi, t, rows, nrows, grad, n_orb);
+}
 
 inline dim3 rows_grid(int64_t work, int32_t points) {
     return dim3(unsigned(std::min<int64_t>((work + kThreads - 1) / kThreads, 4096)), unsigned(points));

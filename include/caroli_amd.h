@@ -262,8 +262,17 @@ int32_t cg_kernel_timing(int32_t enable);
 int32_t cg_kernel_stats(cg_kernel_stat* out, int32_t max_records);
 
 /* Adjoint: T and dT/d(eps_a) for every orbital a of the device (n values, on-site energies),
- * from one backward sweep over the stored fronts (no counterpart in the reference). */
+ * from one backward sweep over the stored fronts (no counterpart in the reference): psi = G E_R W_R
+ * and phi = G^T E_L conj(W_L) by the root and backward solves, then t = W_L^H psi_L and
+ * dT/d(eps_a) = 2 Re sum_cd psi_ac conj(t_dc) phi_ad on the GPU (one thread per orbital). */
 int32_t cg_backward(const cg_system* s, const cg_sigma* g, double* out_t, double* out_grad);
+/* The adjoint of `count` points (systems[p], sigmas[p]) of one structure (the same lattice, sizes
+ * and leads; energies and disorder free) in one sweep: the factors of every point kept, psi and
+ * phi solved for all of them, the contraction on the GPU. out_t[p]; out_grad: count x n, point
+ * after point; out_status[p] as cg_backward's status for that point (its T and gradient NaN when
+ * not CG_OK). Points are processed in chunks that keep the stored factors within 64 GB. */
+int32_t cg_backward_batch(const cg_system* const* systems, const cg_sigma* const* sigmas, int64_t count, double* out_t,
+                          double* out_grad, int32_t* out_status);
 
 /* Scattering states (NestedTwoTerminal::scattering_states, crates/caroli-green/src/nested.rs:851):
  * psi_p = G W_p on every device orbital for each lead p (W_p W_p^H = Gamma_p, one column per

@@ -68,22 +68,57 @@ def test_adjoint_bhz_100x32():
     scale = np.abs(ref).max()
     say(f"    T = {t:.6f}, max |grad - FP64 gradient| {np.abs(grad - ref).max():.2e} (scale {scale:.2e})")
     assert np.abs(grad - ref).max() <= 1e-8 * scale
+    # The adjoint is deterministic: one thread per orbital, no atomics.
+    t2, grad2 = s.backward(sig)
+    assert t2 == t and np.array_equal(grad2, grad)
+    # Central differences (h = 1e-6) on 64 sites drawn from seed 2, the 128 shifted systems in one
+    # batched launch with the same self-energies.
     nsites = ext[0] * ext[1]
-    sites = np.random.default_rng(2).choice(nsites, 16, replace=False)
+    sites = np.random.default_rng(2).choice(nsites, 64, replace=False)
     h = 1e-6
+    shifted = []
     for i in sites:
         pot = np.zeros(nsites)
         pot[i] = h
-        tp = ca.NestedTwoTerminal.preset(ca.Model.BHZ, ext, disorder=1, strength=w, seed=1,
-                                         site_potential=pot).transmission_with(sig)
-        tm = ca.NestedTwoTerminal.preset(ca.Model.BHZ, ext, disorder=1, strength=w, seed=1,
-                                         site_potential=-pot).transmission_with(sig)
-        fd = (tp - tm) / (2 * h)
+        for sign in (1.0, -1.0):
+            shifted.append(ca.NestedTwoTerminal.preset(ca.Model.BHZ, ext, disorder=1, strength=w, seed=1,
+                                                       site_potential=sign * pot))
+    tt, st = ca.batch_transmission(shifted, [sig] * len(shifted), ca.Precision.Double)
+    assert (st == 0).all()
+    worst = 0.0
+    for k, i in enumerate(sites):
+        fd = (tt[2 * k] - tt[2 * k + 1]) / (2 * h)
         ad = grad[4 * i:4 * i + 4].sum()
         # Relative 1e-6, above the rounding floor of a central difference of T ~ 2 in double
         # precision (32 ulps of T over 2h): the gradient itself is pinned to 1e-8 above.
         floor = 32 * np.finfo(float).eps * abs(t) / (2 * h)
+        worst = max(worst, abs(fd - ad) / (1e-6 * abs(ad) + floor))
         assert abs(fd - ad) <= 1e-6 * abs(ad) + floor, (i, fd, ad, floor)
+    say(f"    64 sites: worst |FD - adjoint| / bound {worst:.2f}")
+
+
+def test_adjoint_batch_over_energies_and_configurations():
+    """cg_backward_batch over 3 configurations x 3 energies of BHZ 40 x 16 in one sweep equals the
+    point-by-point adjoint bit for bit and the oracle's FP64 gradient to 1e-8."""
+    ext = [40, 16, 0]
+    systems = [ca.NestedTwoTerminal.preset(ca.Model.BHZ, ext, disorder=1, strength=0.2, seed=1 + c) for c in range(3)]
+    energies = [-0.14, 0.01, 0.05]
+    sig = [systems[0].self_energies(e) for e in energies]
+    ps = [s for s in systems for _ in energies]
+    pg = [g for _ in systems for g in sig]
+    say("[gpu] batched adjoint BHZ 40x16, 9 points")
+    t, grad, st = ca.backward_batch(ps, pg)
+    assert (st == 0).all(), st
+    for k in range(len(ps)):
+        t1, g1 = ps[k].backward(pg[k])
+        assert t1 == t[k] and np.array_equal(g1, grad[k]), k
+    say(f"    T = {np.round(t, 4)}")
+    for k in (0, 4, 8):
+        assert t[k] > 0.1  # conducting points: the gradient is of order T
+        ref = oracle_gradient(ps[k], energies[k % 3])
+        assert np.abs(grad[k] - ref).max() <= 1e-8 * np.abs(ref).max()
+    tt, _ = ca.batch_transmission(ps, pg, ca.Precision.Double)
+    assert np.abs(t - tt).max() <= 1e-10
 
 
 def test_cubic_uneven_batch():
