@@ -327,6 +327,7 @@ struct HostPlan {
     std::vector<int32_t> vars;
     /// The block's orbitals, and each orbital's position in the left / right slice (-1 if none).
     std::vector<int32_t> rows, left_pos, right_pos, left_orbs, right_orbs;
+    int64_t upd_total = 0;  // sum of the fronts' n - ne (a point's update vectors, per column)
     int32_t max_ne = 0;  // largest ne of a front or the root
     int32_t max_n = 0;   // largest n of a front (the root excluded)
 };
@@ -409,6 +410,8 @@ HostPlan build_plan(const Block& blk, const SparseMatrix& h) {
         d.n = int32_t(pl.vars.size());
         d.ne = pl.ne;
         d.ne1 = pl.ne;
+        d.upd_off = hp.upd_total;
+        hp.upd_total += d.n - d.ne;
         d.ent_beg = int32_t(hp.ent.size());
         for (auto& e : pl.entries) hp.ent.push_back({e.a, e.b, int32_t(e.k), e.conj ? 1 : 0});
         d.ent_end = int32_t(hp.ent.size());
@@ -422,6 +425,7 @@ HostPlan build_plan(const Block& blk, const SparseMatrix& h) {
             kd.src = c.off + c.ne + int64_t(c.ne) * c.n;
             kd.ld = c.n;
             kd.size = c.n - c.ne;
+            kd.upd = c.upd_off;
             kd.map_beg = int32_t(hp.maps.size());
             hp.maps.insert(hp.maps.end(), pl.child_pos[q].begin(), pl.child_pos[q].end());
             hp.kids.push_back(kd);
@@ -450,6 +454,7 @@ HostPlan build_plan(const Block& blk, const SparseMatrix& h) {
         kd.src = c.off + c.ne + int64_t(c.ne) * c.n;
         kd.ld = c.n;
         kd.size = c.n - c.ne;
+        kd.upd = c.upd_off;
         kd.map_beg = int32_t(hp.maps.size());
         for (size_t o : t.nodes[top].boundary) {
             int q = kept.get(o);
@@ -590,7 +595,7 @@ struct Pinned {
 /// x, the right-hand side b, the correction d, the per-point maxima, x on the left slice, the list
 /// of iterating points; and their pinned host ends.
 struct RefineBufs {
-    DevBuf w, x, b, d, maxes, xl, act;
+    DevBuf w, x, b, d, maxes, xl, act, upd;
     Pinned<int32_t> h_act;
     Pinned<double> h_maxes;
 };
@@ -1091,11 +1096,17 @@ int32_t replanned_point(Engine& eng, const cg_system* s, const cg_sigma* g, int 
 
 /// Multifrontal solve of the columns of `sv` with the factors the last sweep left in the arenas:
 /// forward elimination (skipped when the right-hand sides live on the kept orbitals only, where
-/// it does nothing), the dense root, the backward substitution.
-void solve(Engine& eng, const HostPlan& hp, const DevPlan& dp, const PointsView& pv, const SolveView& sv, int32_t P,
+/// it does nothing; each front's boundary update goes to its slot of sv.upd and its parent gathers
+/// it), the dense root, the backward substitution.
+void solve(Engine& eng, const HostPlan& hp, const DevPlan& dp, const PointsView& pv, const SolveView& sv_in, int32_t P,
            bool single, bool transpose, bool kept_only) {
     const NodeDesc* nodes = dp.nodes.as<NodeDesc>();
     const int32_t* lv = dp.levels.as<int32_t>();
+    SolveView sv = sv_in;
+    sv.kids = dp.kids.as<KidDesc>();
+    sv.maps = dp.maps.as<int32_t>();
+    sv.gather = kept_only ? 0 : 1;
+    if (!kept_only && !sv.upd) throw DeviceError("a forward solve needs its update vectors");
     eng.timed(kSolve, 0.0, 0.0, [&] {
         if (!kept_only)
             for (size_t l = 0; l < dp.all.size(); ++l)
@@ -1198,11 +1209,14 @@ class Refinement {
         r.maxes.ensure(size_t(P_) * 2 * sizeof(double));
         r.xl.ensure(size_t(P_) * nl_ * c_ * sizeof(double2));
         r.act.ensure(size_t(P_) * sizeof(int32_t));
+        r.upd.ensure(size_t(P_) * size_t(hp_.upd_total) * c_ * sizeof(double2));
         r.h_act.ensure(size_t(P_));
         r.h_maxes.ensure(size_t(P_) * 2);
         HIP_CHECK(hipMemcpyAsync(r.w.p, w_host_.data(), w_host_.size() * sizeof(double2), hipMemcpyHostToDevice, st));
         HIP_CHECK(hipMemsetAsync(r.b.p, 0, bytes_, st));
         xv_ = SolveView{r.x.as<double2>(), n_orb_ * c_, int32_t(n_orb_), c_, dp_.vars.as<int32_t>()};
+        xv_.upd = r.upd.as<double2>();
+        xv_.upd_stride = hp_.upd_total * c_;
         SolveView bv = xv_;
         bv.x = r.b.as<double2>();
         launch_scatter(bv, dp_.right_orbs.as<int32_t>(), nr_, r.w.as<double2>(), int64_t(nr_) * c_, false, P_, st);

@@ -22,7 +22,11 @@ struct NodeDesc {
     int32_t slot;                // status / pivot slot of the front within a point
     int64_t piv_off;             // first pivot index of the front within a point
     int64_t var_off;             // first of its n orbital indices in the plan's variable list
+    int64_t upd_off;             // its boundary's first entry in a point's update vectors (sum of the
+                                 // earlier fronts' n - ne)
 };
+
+struct KidDesc;
 
 /// Right-hand sides of the multifrontal solves: per point, c columns of n_orb complex f64
 /// (column-major, leading dimension ldx), overwritten by the solution.
@@ -32,6 +36,14 @@ struct SolveView {
     int32_t ldx, c;
     const int32_t* vars;  // orbital index of every front variable (NodeDesc::var_off)
     const int32_t* pts;   // points of the launch (grid index q -> point pts[q]); null: all, in order
+    // The forward solve's update vectors: per point (stride upd_stride) and front, its boundary's
+    // c columns (n - ne rows each) at c * NodeDesc::upd_off, written by the front and gathered by
+    // its parent through the children's maps (kids, maps), in the order of the children.
+    double2* upd;
+    int64_t upd_stride;
+    const KidDesc* kids;
+    const int32_t* maps;
+    int32_t gather;  // 1: the forward solve ran, the root gathers its children's updates
 };
 
 /// An H entry: F[a, b] -= conj ? conj(vals[k]) : vals[k].
@@ -45,6 +57,7 @@ struct Entry {
 struct KidDesc {
     int64_t src;
     int32_t ld, size, map_beg, inv_beg;
+    int64_t upd;  // the child's NodeDesc::upd_off (its update vector in the forward solve)
 };
 
 /// Per-point inputs and outputs of one launch.

@@ -33,9 +33,37 @@ __device__ __forceinline__ double2 widen(TF a) {
     return double2{double(a.x), double(a.y)};
 }
 
-__device__ __forceinline__ void atomic_sub(double2* p, double2 v) {
-    atomicAdd(&p->x, -v.x);
-    atomicAdd(&p->y, -v.y);
+// Adds the children's update vectors (the boundary columns c0 .. c0 + cw of each, from the
+// forward solve) into y at their positions in this front (the children's maps), one child after
+// the other: a child's positions are distinct, so within a child the lanes never collide, and the
+// order of the sums does not depend on the schedule. Ends with a barrier.
+template <int NT>
+__device__ __forceinline__ void gather_updates(const NodeDesc& nd, const SolveView& sv, int64_t p, int c0, int cw,
+                                               double2* y, int ldy) {
+    const double2* U = sv.upd + p * sv.upd_stride;
+    for (int k = nd.kid_beg; k < nd.kid_end; ++k) {
+        const KidDesc kd = sv.kids[k];
+        const double2* Uc = U + int64_t(sv.c) * kd.upd + int64_t(c0) * kd.size;
+        const int32_t* mp = sv.maps + kd.map_beg;
+        for (int idx = threadIdx.x; idx < kd.size * cw; idx += NT) {
+            const int a = idx % kd.size, col = idx / kd.size;
+            double2& t = y[mp[a] + col * ldy];
+            t = cadd(t, Uc[a + int64_t(col) * kd.size]);
+        }
+        __syncthreads();
+    }
+}
+
+// This front's update vector: rows ne .. n - 1 of y (columns c0 .. c0 + cw) to its slot.
+template <int NT>
+__device__ __forceinline__ void store_update(const NodeDesc& nd, const SolveView& sv, int64_t p, int c0, int cw,
+                                             const double2* y, int ldy) {
+    const int nb = nd.n - nd.ne;
+    double2* U = sv.upd + p * sv.upd_stride + int64_t(sv.c) * nd.upd_off + int64_t(c0) * nb;
+    for (int idx = threadIdx.x; idx < nb * cw; idx += NT) {
+        const int b = idx % nb, col = idx / nb;
+        U[b + int64_t(col) * nb] = y[nd.ne + b + col * ldy];
+    }
 }
 
 // Transposed solves (A^T x = b, the adjoint's second set of columns), one workgroup per front and
@@ -44,7 +72,7 @@ template <class TF>
 __global__ void __launch_bounds__(kThreads)
     solve_forward_t_kernel(const NodeDesc* __restrict__ nodes, const int32_t* __restrict__ level, PointsView pv,
                            SolveView sv) {
-    extern __shared__ double2 y[];  // ne x c, column-major
+    extern __shared__ double2 y[];  // n x c, column-major
     const NodeDesc nd = nodes[level[blockIdx.x]];
     const int64_t p = point_of(sv, blockIdx.y);
     const TF* F = reinterpret_cast<const TF*>(pv.arena) + p * pv.arena_stride + nd.off;
@@ -52,24 +80,31 @@ __global__ void __launch_bounds__(kThreads)
     double2* X = sv.x + p * sv.x_stride;
     const int n = nd.n, ne = nd.ne, nb = n - ne, c = sv.c, tid = threadIdx.x;
     auto f = [&](int i, int j) { return widen(F[i + int64_t(j) * n]); };
-    for (int idx = tid; idx < ne * c; idx += kThreads) y[idx] = X[vars[idx % ne] + int64_t(idx / ne) * sv.ldx];
+    // y over all n rows: b_e on the separator, zero on the boundary, plus the children's updates.
+    for (int idx = tid; idx < n * c; idx += kThreads) {
+        const int r = idx % n, col = idx / n;
+        y[idx] = r < ne ? X[vars[r] + int64_t(col) * sv.ldx] : double2{0.0, 0.0};
+    }
     __syncthreads();
+    gather_updates<kThreads>(nd, sv, p, 0, c, y, n);
     for (int k = 0; k < ne; ++k) {  // U11^T g = b_e (lower, non-unit)
-        for (int col = tid; col < c; col += kThreads) y[k + col * ne] = cmul(y[k + col * ne], crecip(f(k, k)));
+        for (int col = tid; col < c; col += kThreads) y[k + col * n] = cmul(y[k + col * n], crecip(f(k, k)));
         __syncthreads();
         for (int idx = tid; idx < (ne - k - 1) * c; idx += kThreads) {
             int i = k + 1 + idx % (ne - k - 1), col = idx / (ne - k - 1);
-            y[i + col * ne] = csub(y[i + col * ne], cmul(f(k, i), y[k + col * ne]));
+            y[i + col * n] = csub(y[i + col * n], cmul(f(k, i), y[k + col * n]));
         }
         __syncthreads();
     }
-    for (int idx = tid; idx < nb * c; idx += kThreads) {  // b_b -= U12^T g
+    for (int idx = tid; idx < nb * c; idx += kThreads) {  // u_b = (updates) - U12^T g
         int b = idx % nb, col = idx / nb;
         double2 acc{0.0, 0.0};
-        for (int k = 0; k < ne; ++k) acc = cadd(acc, cmul(f(k, ne + b), y[k + col * ne]));
-        atomic_sub(&X[vars[ne + b] + int64_t(col) * sv.ldx], acc);
+        for (int k = 0; k < ne; ++k) acc = cadd(acc, cmul(f(k, ne + b), y[k + col * n]));
+        y[ne + b + col * n] = csub(y[ne + b + col * n], acc);
     }
-    for (int idx = tid; idx < ne * c; idx += kThreads) X[vars[idx % ne] + int64_t(idx / ne) * sv.ldx] = y[idx];
+    __syncthreads();
+    store_update<kThreads>(nd, sv, p, 0, c, y, n);
+    for (int idx = tid; idx < ne * c; idx += kThreads) X[vars[idx % ne] + int64_t(idx / ne) * sv.ldx] = y[idx % ne + (idx / ne) * n];
 }
 
 template <class TF>
@@ -308,8 +343,9 @@ __device__ __forceinline__ void apply_pivots(const int32_t* __restrict__ piv, in
     __syncthreads();
 }
 
-// Forward step of a front: y = L11^-1 P b_e on the separator, b_b -= L21 y on the boundary
-// (accumulated with FP64 atomics: sibling fronts share boundary variables).
+// Forward step of a front: with the children's update vectors added (in the order of the
+// children), y = L11^-1 P b_e on the separator, and the front's own update vector u_b - L21 y
+// (u_b: what the children passed on to its boundary) stored for its parent.
 template <class TF, int NT, int CC>
 __global__ void __launch_bounds__(NT, kSolveWaves)
     fwd_blocked_kernel(const NodeDesc* __restrict__ nodes, const int32_t* __restrict__ level, PointsView pv, SolveView sv) {
@@ -330,19 +366,14 @@ __global__ void __launch_bounds__(NT, kSolveWaves)
         ys[idx] = (r < ne && col < cw) ? X[vars[r] + int64_t(col) * sv.ldx] : double2{0.0, 0.0};
     }
     __syncthreads();
+    gather_updates<NT>(nd, sv, p, c0, cw, ys, n);
     apply_pivots(piv, ne, ys, n, cw);
     lower_blocked<TF, NT, CC>(F, n, ne, n, ys, n, red);
-    for (int idx = tid; idx < n * cw; idx += NT) {
-        const int r = idx % n, col = idx / n;
-        double2* at = &X[vars[r] + int64_t(col) * sv.ldx];
-        const double2 v = ys[r + col * n];
-        if (r < ne) {
-            *at = v;
-        } else {
-            atomicAdd(&at->x, v.x);
-            atomicAdd(&at->y, v.y);
-        }
+    for (int idx = tid; idx < ne * cw; idx += NT) {
+        const int r = idx % ne, col = idx / ne;
+        X[vars[r] + int64_t(col) * sv.ldx] = ys[r + col * n];
     }
+    store_update<NT>(nd, sv, p, c0, cw, ys, n);
 }
 
 // Backward step of a front: x_e = U11^-1 (y - U12 x_b), the boundary values final.
@@ -394,6 +425,7 @@ __global__ void __launch_bounds__(NT, kSolveWaves)
         ys[idx] = col < cw ? X[vars[r] + int64_t(col) * sv.ldx] : double2{0.0, 0.0};
     }
     __syncthreads();
+    if (sv.gather) gather_updates<NT>(nd, sv, p, c0, cw, ys, ne);
     apply_pivots(piv, ne, ys, ne, cw);
     lower_blocked<double2, NT, CC>(F, n, ne, ne, ys, ne, red);
     upper_blocked<double2, NT, CC>(F, n, ne, ys, ne, red);
@@ -417,6 +449,7 @@ __global__ void __launch_bounds__(kThreads)
     auto f = [&](int i, int j) { return F[i + int64_t(j) * n]; };
     for (int idx = tid; idx < ne * c; idx += kThreads) y[idx] = X[vars[idx % ne] + int64_t(idx / ne) * sv.ldx];
     __syncthreads();
+    if (sv.gather) gather_updates<kThreads>(nd, sv, p, 0, c, y, ne);
     auto swaps = [&](bool reverse) {
         for (int col = tid; col < c; col += kThreads)
             for (int s = 0; s < ne; ++s) {
@@ -695,6 +728,8 @@ void launch_solve_forward(const NodeDesc* nodes, const int32_t* level, int32_t c
                           hipStream_t st) {
     if (count <= 0 || points <= 0 || sv.c <= 0) return;
     if (transpose) {
+        // all n rows of a front's columns in LDS (the separator, and the boundary's update vector)
+        if (int64_t(max_n) * sv.c > kSolveLds) throw std::runtime_error("transposed forward solve beyond its LDS");
         SolveKernel k = single ? solve_forward_t_kernel<float2> : solve_forward_t_kernel<double2>;
         hipLaunchKernelGGL(k, dim3(unsigned(count), unsigned(points)), dim3(kThreads),
                            size_t(kSolveLds) * sizeof(double2), st, nodes, level, pv, sv);

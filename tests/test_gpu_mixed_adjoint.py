@@ -91,3 +91,26 @@ def test_adjoint_matches_fp64_gradient_and_finite_differences():
         fd = (tp - tm) / (2 * h)
         ad = grad[4 * i:4 * i + 4].sum()
         assert abs(fd - ad) <= 1e-6 * max(abs(ad), 1e-3 * scale), (i, fd, ad)
+
+
+def test_mixed_precision_is_bit_reproducible():
+    """The forward solve of the refinement passes each front's boundary update to its parent in
+    the order of the children (no atomics): repeated mixed runs give the same T bit for bit,
+    through a fresh batch and a resident one (HIP graphs)."""
+    systems = [ca.NestedTwoTerminal.preset(ca.Model.BHZ, [60, 24, 0], disorder=2, strength=0.4, concentration=0.2,
+                                           seed=1 + c) for c in range(2)]
+    energies = [-0.12, -0.03, 0.02, 0.1]
+    sig = [systems[0].self_energies(e) for e in energies]
+    ps = [s for s in systems for _ in energies]
+    pg = [g for _ in systems for g in sig]
+    stats = ca.capi.BatchStats()
+    t1, st1 = ca.batch_transmission(ps, pg, ca.Precision.Mixed, stats)
+    t2, st2 = ca.batch_transmission(ps, pg, ca.Precision.Mixed)
+    batch = ca.ResidentBatch(ps, pg, ca.Precision.Mixed)
+    runs = [batch.run()[0] for _ in range(3)]
+    say(f"[gpu] mixed reproducibility: refinement iterations {stats.refinement_iters}, T = {np.round(t1, 6)}")
+    assert stats.refinement_iters > 0
+    assert (st1 == 0).all() and (st2 == 0).all()
+    assert np.array_equal(t1, t2)
+    for r in runs:
+        assert np.array_equal(r, t1)
