@@ -262,6 +262,15 @@ def run_transmission(args, ca):
     if args.compare and args.precision in ("double", "mixed"):
         other = compare_precision(ca, pts_sys, pts_sig, t, npts / step, args)
     per_point = plain_per_point(ca, pts_sys, pts_sig, args) if args.per_point > 0 else None
+    if per_point is not None:
+        # Batched rates over the plain complex128 solver called one point at a time (resident
+        # one-point batches replayed from their graphs, the faster of its two forms).
+        mixed_rate = npts / step if args.precision == "mixed" else (other or {}).get("value")
+        plain_rate = npts / step if args.precision == "double" else (other or {}).get("value")
+        per_point["batched_plain_over_per_point"] = plain_rate / per_point["resident_points_per_s"] \
+            if plain_rate else None
+        per_point["batched_mixed_over_per_point"] = mixed_rate / per_point["resident_points_per_s"] \
+            if mixed_rate else None
     fixture = fixture_check(args, pts_e, t)
     cpu, max_dt, max_rel = (None, None, None) if args.no_cpu else cpu_baseline(pts_sys, pts_sig, pts_e, t, args, threads)
     if cpu is not None and other is not None:
