@@ -506,3 +506,224 @@ def kernel_stats():
     buf = (capi.KernelStat * 16)()
     n = capi.cg_kernel_stats(buf, 16)
     return {buf[i].name.decode(): (buf[i].launches, buf[i].seconds, buf[i].flops, buf[i].bytes) for i in range(n)}
+
+
+class MultiSelfEnergies:
+    """Self-energies of every lead of a multi-terminal device at one energy, with the labels of the
+    rows and columns of T: (lead, sector) pairs (nested_multi.rs:196-313)."""
+
+    def __init__(self, handle, owner):
+        self._h = C.c_void_p(handle)
+        self._owner = owner
+
+    def __del__(self):
+        if getattr(self, "_h", None) and self._h.value:
+            capi.cg_sigma_destroy(self._h)
+            self._h = C.c_void_p(None)
+
+    def energy(self) -> float:
+        return capi.cg_sigma_energy(self._h)
+
+    def sigma(self, p):
+        """Sigma_p of lead p on its coupled orbitals (NestedMultiTerminal.lead(p)["orbitals"])."""
+        k = len(self._owner.lead(p)["orbitals"])
+        out = np.zeros((k, k), np.complex128, order="F")
+        _check(capi.cg_multi_sigma(self._owner._h, self._h, p, _dptr(out)))
+        return out
+
+    def labels(self):
+        n = C.c_int32()
+        _check(capi.cg_multi_labels(self._h, C.byref(n), None, None))
+        lead = np.zeros(n.value, np.int32)
+        sector = np.zeros(n.value, np.int32)
+        _check(capi.cg_multi_labels(self._h, C.byref(n), lead.ctypes.data_as(C.POINTER(C.c_int32)),
+                                    sector.ctypes.data_as(C.POINTER(C.c_int32))))
+        return [(int(a), int(b)) for a, b in zip(lead, sector)]
+
+
+@dataclass
+class SectorTransmissions:
+    """nested_multi.rs:36-43: the (lead, sector) of each row and column, t[k][l] from labels[l]
+    into labels[k]."""
+
+    labels: list
+    t: np.ndarray
+
+
+class NestedMultiTerminal:
+    """A device with any number of leads solved by nested dissection (nested_multi.rs:45-56): the
+    sites the leads couple to stay at the root, whose border block gives G on all of their
+    orbitals, and T[p][q] = Tr[Gamma_p G_pq Gamma_q G_pq^H] from lead q into lead p."""
+
+    def __init__(self, handle):
+        self._h = C.c_void_p(handle)
+
+    def __del__(self):
+        if getattr(self, "_h", None) and self._h.value:
+            capi.cg_system_destroy(self._h)
+
+    @classmethod
+    def from_model(cls, model, sites, interfaces, site_potential=None):
+        """NestedMultiTerminal::from_model (nested_multi.rs:60) with a preset model as the clean lead:
+        `sites` the region's (x, y, z, sublattice) in order, `interfaces` a list of (slice sites,
+        outward step), site_potential an optional on-site potential per site."""
+        st = np.ascontiguousarray(sites, dtype=np.int64).reshape(-1, 4)
+        cells = np.ascontiguousarray(st[:, :3])
+        subs = np.ascontiguousarray(st[:, 3], dtype=np.int32)
+        keep = [cells, subs]
+        itf = (capi.Interface * len(interfaces))()
+        for p, (slice_sites, outward) in enumerate(interfaces):
+            ss = np.ascontiguousarray(slice_sites, dtype=np.int64).reshape(-1, 4)
+            sc = np.ascontiguousarray(ss[:, :3])
+            sb = np.ascontiguousarray(ss[:, 3], dtype=np.int32)
+            keep += [sc, sb]
+            itf[p].sites = len(ss)
+            itf[p].cells = sc.ctypes.data_as(C.POINTER(C.c_int64))
+            itf[p].subs = sb.ctypes.data_as(C.POINTER(C.c_int32))
+            for d in range(3):
+                itf[p].outward[d] = int(outward[d])
+        pot = None
+        if site_potential is not None:
+            pot = np.ascontiguousarray(site_potential, dtype=np.float64)
+            if len(pot) != len(st):
+                raise ValueError("one potential per site")
+        h = C.c_void_p()
+        _check(capi.cg_multi_model_create(int(model), len(st), cells.ctypes.data_as(C.POINTER(C.c_int64)),
+                                          subs.ctypes.data_as(C.POINTER(C.c_int32)),
+                                          _dptr(pot) if pot is not None else None, len(interfaces), itf,
+                                          C.byref(h)), "multi-terminal system")
+        return cls(h.value)
+
+    @classmethod
+    def from_arrays(cls, n, row_ptr, cols, vals, cells, site_orbitals, site_offsets, dim, hop_range, leads):
+        """The device arrays as NestedTwoTerminal.from_arrays, and `leads` a list of dicts with h00,
+        h01 (cell x cell), orbitals (the coupled device orbitals, increasing), coupling (coupled x
+        cell) and optionally kinds (the index within its site of each cell orbital)."""
+        rp = np.ascontiguousarray(row_ptr, dtype=np.int64)
+        cl = np.ascontiguousarray(cols, dtype=np.uint32)
+        vv = np.ascontiguousarray(vals, dtype=np.complex128)
+        ce = np.ascontiguousarray(cells, dtype=np.int64).reshape(-1, 3)
+        so = np.ascontiguousarray(site_orbitals, dtype=np.int32)
+        sf = np.ascontiguousarray(site_offsets, dtype=np.uint64)
+        rg = np.ascontiguousarray(hop_range, dtype=np.int64)
+        keep = []
+        arr = (capi.Lead * len(leads))()
+        for p, l in enumerate(leads):
+            h00 = np.asfortranarray(l["h00"], dtype=np.complex128)
+            h01 = np.asfortranarray(l["h01"], dtype=np.complex128)
+            orb = np.ascontiguousarray(l["orbitals"], dtype=np.uint64)
+            cp = np.asfortranarray(l["coupling"], dtype=np.complex128)
+            kd = None if l.get("kinds") is None else np.ascontiguousarray(l["kinds"], dtype=np.int32)
+            keep += [h00, h01, orb, cp, kd]
+            arr[p].cell = h00.shape[0]
+            arr[p].h00 = _dptr(h00)
+            arr[p].h01 = _dptr(h01)
+            arr[p].coupled = len(orb)
+            arr[p].orbitals = orb.ctypes.data_as(C.POINTER(C.c_uint64))
+            arr[p].coupling = _dptr(cp)
+            arr[p].kinds = kd.ctypes.data_as(C.POINTER(C.c_int32)) if kd is not None else None
+        h = C.c_void_p()
+        _check(capi.cg_multi_create(int(n), rp.ctypes.data_as(C.POINTER(C.c_int64)),
+                                    cl.ctypes.data_as(C.POINTER(C.c_uint32)), _dptr(vv), len(so),
+                                    ce.ctypes.data_as(C.POINTER(C.c_int64)), so.ctypes.data_as(C.POINTER(C.c_int32)),
+                                    sf.ctypes.data_as(C.POINTER(C.c_uint64)), int(dim),
+                                    rg.ctypes.data_as(C.POINTER(C.c_int64)), len(leads), arr, C.byref(h)),
+               "multi-terminal system")
+        return cls(h.value)
+
+    def lead_count(self) -> int:
+        return capi.cg_multi_lead_count(self._h)
+
+    def block_count(self) -> int:
+        return capi.cg_block_count(self._h)
+
+    def sizes(self):
+        n, nnz, ns = C.c_uint64(), C.c_uint64(), C.c_uint64()
+        _check(capi.cg_system_sizes(self._h, C.byref(n), C.byref(nnz), C.byref(ns)))
+        return n.value, nnz.value, ns.value
+
+    def arrays(self):
+        """The device arrays (CSR and geometry)."""
+        n, nnz, ns = self.sizes()
+        rp = np.zeros(n + 1, np.int64)
+        cl = np.zeros(nnz, np.uint32)
+        vv = np.zeros(nnz, np.complex128)
+        ce = np.zeros((ns, 3), np.int64)
+        so = np.zeros(ns, np.int32)
+        sf = np.zeros(ns, np.uint64)
+        dim = C.c_int32()
+        rg = np.zeros(3, np.int64)
+        _check(capi.cg_system_arrays(
+            self._h, rp.ctypes.data_as(C.POINTER(C.c_int64)), cl.ctypes.data_as(C.POINTER(C.c_uint32)), _dptr(vv),
+            ce.ctypes.data_as(C.POINTER(C.c_int64)), so.ctypes.data_as(C.POINTER(C.c_int32)),
+            sf.ctypes.data_as(C.POINTER(C.c_uint64)), C.byref(dim), rg.ctypes.data_as(C.POINTER(C.c_int64)),
+            None, None, None, None, None))
+        return dict(n=n, row_ptr=rp, cols=cl, vals=vv, cells=ce, site_orbitals=so, site_offsets=sf, dim=dim.value,
+                    range=rg)
+
+    def lead(self, p):
+        """Lead p as attached: dict of h00, h01, orbitals, coupling (multi.rs:15-25)."""
+        nc, nk = C.c_uint64(), C.c_uint64()
+        _check(capi.cg_multi_lead(self._h, p, C.byref(nc), C.byref(nk), None, None, None, None))
+        h00 = np.zeros((nc.value, nc.value), np.complex128, order="F")
+        h01 = np.zeros_like(h00)
+        orb = np.zeros(nk.value, np.uint64)
+        cp = np.zeros((nk.value, nc.value), np.complex128, order="F")
+        _check(capi.cg_multi_lead(self._h, p, C.byref(nc), C.byref(nk), _dptr(h00), _dptr(h01),
+                                  orb.ctypes.data_as(C.POINTER(C.c_uint64)), _dptr(cp)))
+        return dict(h00=h00, h01=h01, orbitals=orb.astype(np.int64), coupling=cp)
+
+    def self_energies(self, energy, eta=0.0, sectors=None) -> MultiSelfEnergies:
+        """Self-energies of the leads at E + i eta (nested_multi.rs:196-232); `sectors` as in
+        sector_transmissions splits the labels."""
+        g = C.c_void_p()
+        if sectors is None:
+            _check(capi.cg_multi_self_energies(self._h, float(energy), float(eta), None, None, C.byref(g)),
+                   f"E = {energy}")
+        else:
+            if len(sectors) != self.lead_count():
+                raise ValueError("one sector labelling per lead")
+            ln = np.array([0 if s is None else len(s) for s in sectors], np.int32)
+            tags = np.array([t for s in sectors if s is not None for t in s] or [0], np.int32)
+            _check(capi.cg_multi_self_energies(self._h, float(energy), float(eta),
+                                               ln.ctypes.data_as(C.POINTER(C.c_int32)),
+                                               tags.ctypes.data_as(C.POINTER(C.c_int32)), C.byref(g)),
+                   f"E = {energy}")
+        return MultiSelfEnergies(g.value, self)
+
+    def transmissions(self, energy, precision=Precision.Double):
+        """nested_multi.rs:174-176: T[p][q] from lead q into lead p."""
+        n = self.lead_count()
+        t = np.zeros((n, n))
+        _check(capi.cg_multi_transmissions(self._h, float(energy), int(precision), _dptr(t)), f"E = {energy}")
+        return t
+
+    def transmissions_with(self, sigma: MultiSelfEnergies, precision=Precision.Double):
+        """transmissions_with_precision (nested_multi.rs:180-194) with given self-energies."""
+        t, st = multi_batch_transmissions([self], [sigma], precision)
+        if st[0] != 0:
+            raise GreenError(int(st[0]), f"E = {sigma.energy()}")
+        return t[0]
+
+    def sector_transmissions(self, energy, sectors, precision=Precision.Double) -> SectorTransmissions:
+        """nested_multi.rs:246-313: `sectors[p]` tags the orbitals of each site of lead p (e.g.
+        [0, 0, 1, 1] for the spin blocks of BHZ), or None for the lead as a whole."""
+        g = self.self_energies(energy, sectors=sectors)
+        return SectorTransmissions(g.labels(), self.transmissions_with(g, precision))
+
+
+def multi_batch_transmissions(systems, sigmas, precision=Precision.Double, stats=None):
+    """Every point (systems[p], sigmas[p]) of multi-terminal devices in one batched launch
+    (cg_multi_batch_transmissions): T of shape (points, labels, labels) and the statuses."""
+    n = len(systems)
+    if n == 0:
+        return np.zeros((0, 0, 0)), np.zeros(0, np.int32)
+    L = len(sigmas[0].labels())
+    sh = (C.c_void_p * n)(*[s._h.value for s in systems])
+    gh = (C.c_void_p * n)(*[g._h.value for g in sigmas])
+    t = np.zeros((n, L, L))
+    st = np.zeros(n, np.int32)
+    bs = stats if stats is not None else capi.BatchStats()
+    _check(capi.cg_multi_batch_transmissions(sh, gh, n, int(precision), _dptr(t),
+                                             st.ctypes.data_as(C.POINTER(C.c_int32)), C.byref(bs)), "multi batch")
+    return t, st

@@ -164,3 +164,45 @@ cg_operator_currents = _sig("cg_operator_currents", C.c_int32, c_sys, P(C.c_doub
                             P(C.c_double), P(C.c_double), C.c_int32, P(C.c_int64), P(C.c_int64), P(C.c_int64),
                             P(C.c_double))
 EXPORTS += ["cg_scattering_states", "cg_site_densities", "cg_bond_currents", "cg_operator_currents"]
+
+
+class Lead(C.Structure):
+    """cg_lead: one lead as attached (cell, h00, h01, coupled orbitals, coupling, kinds)."""
+    _fields_ = [
+        ("cell", C.c_uint64),
+        ("h00", P(C.c_double)),
+        ("h01", P(C.c_double)),
+        ("coupled", C.c_uint64),
+        ("orbitals", P(C.c_uint64)),
+        ("coupling", P(C.c_double)),
+        ("kinds", P(C.c_int32)),
+    ]
+
+
+class Interface(C.Structure):
+    """cg_interface: the device sites of one interface slice and its outward step."""
+    _fields_ = [
+        ("sites", C.c_uint64),
+        ("cells", P(C.c_int64)),
+        ("subs", P(C.c_int32)),
+        ("outward", C.c_int64 * 3),
+    ]
+
+
+cg_multi_create = _sig("cg_multi_create", C.c_int32, C.c_uint64, P(C.c_int64), P(C.c_uint32), P(C.c_double),
+                       C.c_uint64, P(C.c_int64), P(C.c_int32), P(C.c_uint64), C.c_int32, P(C.c_int64), C.c_int32,
+                       P(Lead), P(c_sys))
+cg_multi_model_create = _sig("cg_multi_model_create", C.c_int32, C.c_int32, C.c_uint64, P(C.c_int64), P(C.c_int32),
+                             P(C.c_double), C.c_int32, P(Interface), P(c_sys))
+cg_multi_lead_count = _sig("cg_multi_lead_count", C.c_int32, c_sys)
+cg_multi_lead = _sig("cg_multi_lead", C.c_int32, c_sys, C.c_int32, P(C.c_uint64), P(C.c_uint64), P(C.c_double),
+                     P(C.c_double), P(C.c_uint64), P(C.c_double))
+cg_multi_self_energies = _sig("cg_multi_self_energies", C.c_int32, c_sys, C.c_double, C.c_double, P(C.c_int32),
+                              P(C.c_int32), P(c_sig))
+cg_multi_sigma = _sig("cg_multi_sigma", C.c_int32, c_sys, c_sig, C.c_int32, P(C.c_double))
+cg_multi_labels = _sig("cg_multi_labels", C.c_int32, c_sig, P(C.c_int32), P(C.c_int32), P(C.c_int32))
+cg_multi_batch_transmissions = _sig("cg_multi_batch_transmissions", C.c_int32, P(c_sys), P(c_sig), C.c_int64,
+                                    C.c_int32, P(C.c_double), P(C.c_int32), P(BatchStats))
+cg_multi_transmissions = _sig("cg_multi_transmissions", C.c_int32, c_sys, C.c_double, C.c_int32, P(C.c_double))
+EXPORTS += ["cg_multi_create", "cg_multi_model_create", "cg_multi_lead_count", "cg_multi_lead",
+            "cg_multi_self_energies", "cg_multi_sigma", "cg_multi_labels", "cg_multi_batch_transmissions", "cg_multi_transmissions"]

@@ -1,5 +1,6 @@
 // The C ABI (include/caroli_amd.h): system construction, host self-energies, and the calls into
 // the GPU engine.
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -11,6 +12,7 @@
 #include "../../include/caroli_amd.h"
 #include "engine.hpp"
 #include "checked.hpp"
+#include "dense.hpp"
 #include "leads.hpp"
 #include "system.hpp"
 
@@ -57,8 +59,25 @@ CMat gamma_factor(const CMat& g) {
     return w;
 }
 
+bool same_leads(const std::vector<AttachedLead>& a, const std::vector<AttachedLead>& b) {
+    if (a.size() != b.size()) return false;
+    for (size_t p = 0; p < a.size(); ++p)
+        if (!(a[p].h00 == b[p].h00 && a[p].h01 == b[p].h01 && a[p].orbitals == b[p].orbitals &&
+              a[p].coupling == b[p].coupling))
+            return false;
+    return true;
+}
+
 bool sigma_matches(const cg_system& s, const cg_sigma& g) {
-    if (g.blocks.size() != s.sym.blocks.size()) return false;
+    if (g.blocks.size() != s.sym.blocks.size() || g.multi != s.multi) return false;
+    if (s.multi) {
+        if (g.mblocks.size() != g.blocks.size()) return false;
+        for (size_t b = 0; b < g.blocks.size(); ++b)
+            if (!same_leads(s.sym.blocks[b].leads, g.mblocks[b].leads) ||
+                g.blocks[b].left_orbitals != s.sym.blocks[b].kept)
+                return false;
+        return true;
+    }
     for (size_t b = 0; b < g.blocks.size(); ++b) {
         auto& x = s.sym.blocks[b];
         auto& y = g.blocks[b];
@@ -146,6 +165,74 @@ int32_t finish(cg_system* s, cg_system** out) {
     return CG_OK;
 }
 
+// The device arrays of cg_system_create and cg_multi_create. The CSR must start at 0 and be
+// non-decreasing, its columns in range; every site must have a positive number of orbitals and
+// the sites must cover the n orbitals exactly once (the reference builds these arrays itself and
+// never sees others: crates/caroli-model/src/lib.rs:427-470). False when they are malformed.
+bool read_device(uint64_t n, const int64_t* row_ptr, const uint32_t* cols, const double* vals, uint64_t nsites,
+                 const int64_t* cells, const int32_t* site_orbitals, const uint64_t* site_offsets, int32_t dim,
+                 const int64_t* range, SystemSpec& sp) {
+    if (!row_ptr || !cols || !vals || !cells || !site_orbitals || !site_offsets || !range || dim < 1 || dim > 3 ||
+        nsites == 0 || n == 0)
+        return false;
+    if (row_ptr[0] != 0) return false;
+    for (uint64_t r = 0; r < n; ++r)
+        if (row_ptr[r] > row_ptr[r + 1]) return false;
+    for (int d = 0; d < dim; ++d)
+        if (range[d] < 0) return false;
+    std::vector<char> covered(n, 0);
+    for (uint64_t i = 0; i < nsites; ++i) {
+        if (site_orbitals[i] <= 0 || site_offsets[i] >= n || uint64_t(site_orbitals[i]) > n - site_offsets[i])
+            return false;
+        for (int32_t o = 0; o < site_orbitals[i]; ++o) {
+            if (covered[site_offsets[i] + uint64_t(o)]) return false;  // overlapping sites
+            covered[site_offsets[i] + uint64_t(o)] = 1;
+        }
+    }
+    for (char c : covered)
+        if (!c) return false;
+    sp.h.n = n;
+    sp.h.row_ptr.assign(row_ptr, row_ptr + n + 1);
+    size_t nnz = size_t(row_ptr[n]);
+    sp.h.cols.assign(cols, cols + nnz);
+    for (auto c : sp.h.cols)
+        if (c >= n) return false;
+    sp.h.vals.resize(nnz);
+    for (size_t e = 0; e < nnz; ++e) sp.h.vals[e] = cplx(vals[2 * e], vals[2 * e + 1]);
+    for (size_t i = 0; i < nsites; ++i) {
+        sp.cells.push_back({cells[3 * i], cells[3 * i + 1], cells[3 * i + 2]});
+        sp.site_orbitals.push_back(site_orbitals[i]);
+        sp.site_offsets.push_back(site_offsets[i]);
+    }
+    sp.dim = dim;
+    sp.range = {range[0], range[1], range[2]};
+    return true;
+}
+
+// A multi-terminal system from its device and leads: the symbolic phase of nested_multi.rs:60-161
+// and the structure key (the device pattern and the leads' patterns and orbitals).
+int32_t finish_multi(cg_system* s, std::vector<AttachedLead> leads, std::vector<std::vector<int>> kinds, cg_system** out) {
+    std::unique_ptr<cg_system> hold(s);
+    s->multi = true;
+    s->sym = multi_symbolic(s->spec, leads);
+    s->multi_leads = std::move(leads);
+    s->orbital_kinds = std::move(kinds);
+    uint64_t h = 7809847782465536322ULL;  // differs from every two-terminal key's start
+    h = fnv(h, &s->spec.h.n, sizeof(s->spec.h.n));
+    h = fnv_vec(h, s->spec.h.row_ptr);
+    h = fnv_vec(h, s->spec.h.cols);
+    h = fnv_vec(h, s->spec.cells);
+    h = fnv_vec(h, s->spec.site_orbitals);
+    h = fnv(h, &s->spec.dim, sizeof(int));
+    h = fnv(h, s->spec.range.data(), sizeof(int64_t) * 3);
+    for (auto& l : s->multi_leads) {
+        h = fnv_vec(h, l.orbitals);
+        for (auto* m : {&l.h00, &l.h01, &l.coupling}) h = pattern_hash(h, *m);
+    }
+    s->key = h;
+    *out = hold.release();
+    return CG_OK;
+}
 }  // namespace
 
 extern "C" {
@@ -154,58 +241,19 @@ int32_t cg_system_create(uint64_t n, const int64_t* row_ptr, const uint32_t* col
                          const int64_t* cells, const int32_t* site_orbitals, const uint64_t* site_offsets, int32_t dim,
                          const int64_t* range, int32_t axis, const double* left_h00, const double* left_h01, uint64_t nl,
                          const double* right_h00, const double* right_h01, uint64_t nr, cg_system** out) {
-    if (!out || !row_ptr || !cols || !vals || !cells || !site_orbitals || !site_offsets || !range || !left_h00 ||
-        !left_h01 || !right_h00 || !right_h01 || dim < 1 || dim > 3 || axis < 0 || axis >= dim || nsites == 0 ||
-        n == 0 || nl == 0 || nr == 0)
+    if (!out || !left_h00 || !left_h01 || !right_h00 || !right_h01 || axis < 0 || axis >= dim || nl == 0 || nr == 0)
         return CG_ERR_INVALID;
-    // The CSR must start at 0 and be non-decreasing, its columns in range; every site must have a
-    // positive number of orbitals and the sites must cover the n orbitals exactly once (the
-    // reference builds these arrays itself and never sees others: crates/caroli-model/src/lib.rs:427-470).
-    if (row_ptr[0] != 0) return CG_ERR_INVALID;
-    for (uint64_t r = 0; r < n; ++r)
-        if (row_ptr[r] > row_ptr[r + 1]) return CG_ERR_INVALID;
-    for (int d = 0; d < dim; ++d)
-        if (range[d] < 0) return CG_ERR_INVALID;
-    {
-        std::vector<char> covered(n, 0);
-        for (uint64_t i = 0; i < nsites; ++i) {
-            if (site_orbitals[i] <= 0 || site_offsets[i] >= n || uint64_t(site_orbitals[i]) > n - site_offsets[i])
-                return CG_ERR_INVALID;
-            for (int32_t o = 0; o < site_orbitals[i]; ++o) {
-                if (covered[site_offsets[i] + uint64_t(o)]) return CG_ERR_INVALID;  // overlapping sites
-                covered[site_offsets[i] + uint64_t(o)] = 1;
-            }
-        }
-        for (char c : covered)
-            if (!c) return CG_ERR_INVALID;
-    }
     try {
-        auto* s = new cg_system;
+        auto s = std::make_unique<cg_system>();
         auto& sp = s->spec;
-        sp.h.n = n;
-        sp.h.row_ptr.assign(row_ptr, row_ptr + n + 1);
-        size_t nnz = size_t(row_ptr[n]);
-        sp.h.cols.assign(cols, cols + nnz);
-        for (auto c : sp.h.cols)
-            if (c >= n) {
-                delete s;
-                return CG_ERR_INVALID;
-            }
-        sp.h.vals.resize(nnz);
-        for (size_t e = 0; e < nnz; ++e) sp.h.vals[e] = cplx(vals[2 * e], vals[2 * e + 1]);
-        for (size_t i = 0; i < nsites; ++i) {
-            sp.cells.push_back({cells[3 * i], cells[3 * i + 1], cells[3 * i + 2]});
-            sp.site_orbitals.push_back(site_orbitals[i]);
-            sp.site_offsets.push_back(site_offsets[i]);
-        }
-        sp.dim = dim;
-        sp.range = {range[0], range[1], range[2]};
+        if (!read_device(n, row_ptr, cols, vals, nsites, cells, site_orbitals, site_offsets, dim, range, sp))
+            return CG_ERR_INVALID;
         sp.axis = axis;
         sp.left_h00 = read_mat(left_h00, nl);
         sp.left_h01 = read_mat(left_h01, nl);
         sp.right_h00 = read_mat(right_h00, nr);
         sp.right_h01 = read_mat(right_h01, nr);
-        return finish(s, out);
+        return finish(s.release(), out);
     } catch (const std::exception&) {
         return CG_ERR_INVALID;
     }
@@ -439,6 +487,7 @@ static Status cell_block_sigmas(const ModeSolver& left, const ModeSolver& right,
 
 // Self-energies of every block at E + i eta (eta = 0: the real axis, nested.rs:652-676).
 static int32_t self_energies_at(const cg_system* s, double energy, double eta, cg_sigma** out) {
+    if (s->multi) return CG_ERR_INVALID;  // cg_multi_self_energies
     try {
         ensure_modes(s);
         auto* g = new cg_sigma;
@@ -510,7 +559,7 @@ int32_t cg_self_energies_broadened(const cg_system* s, double energy, double eta
 }
 
 int32_t cg_sigma_from_arrays(const cg_system* s, double energy, const double* sigmas, cg_sigma** out) {
-    if (!s || !sigmas || !out) return CG_ERR_INVALID;
+    if (!s || !sigmas || !out || s->multi) return CG_ERR_INVALID;
     auto* g = new cg_sigma;
     g->energy = energy;
     const double* p = sigmas;
@@ -724,7 +773,7 @@ int32_t cg_transmissions(const cg_system* s, const double* energies, int64_t cou
 }
 
 int32_t cg_backward(const cg_system* s, const cg_sigma* g, double* out_t, double* out_grad) {
-    if (!s || !g || !out_t || !out_grad || !sigma_matches(*s, *g)) return CG_ERR_INVALID;
+    if (!s || !g || !out_t || !out_grad || s->multi || !sigma_matches(*s, *g)) return CG_ERR_INVALID;
     try {
         return gpu::backward(s, g, out_t, out_grad);
     } catch (const std::exception&) {
@@ -736,7 +785,8 @@ int32_t cg_backward_batch(const cg_system* const* systems, const cg_sigma* const
                           double* out_grad, int32_t* out_status) {
     if (count < 0 || (count > 0 && (!systems || !sigmas || !out_t || !out_grad))) return CG_ERR_INVALID;
     for (int64_t p = 0; p < count; ++p)
-        if (!systems[p] || !sigmas[p] || !sigma_matches(*systems[p], *sigmas[p])) return CG_ERR_INVALID;
+        if (!systems[p] || !sigmas[p] || systems[p]->multi || !sigma_matches(*systems[p], *sigmas[p]))
+            return CG_ERR_INVALID;
     try {
         return gpu::backward_batch(systems, sigmas, count, out_t, out_grad, out_status);
     } catch (const std::exception&) {
@@ -746,7 +796,7 @@ int32_t cg_backward_batch(const cg_system* const* systems, const cg_sigma* const
 
 int32_t cg_scattering_states(const cg_system* s, const cg_sigma* g, int64_t* cols, double* psi_left,
                              double* psi_right) {
-    if (!s || !g || !cols || !sigma_matches(*s, *g)) return CG_ERR_INVALID;
+    if (!s || !g || !cols || s->multi || !sigma_matches(*s, *g)) return CG_ERR_INVALID;
     try {
         return gpu::scattering_states(s, g, cols, psi_left, psi_right);
     } catch (const std::exception&) {
@@ -896,5 +946,305 @@ int32_t cg_operator_currents(const cg_system* s, const double* psi_left, const d
 }
 
 int32_t cg_device_available(void) { return gpu::device_available() ? 1 : 0; }
+
+// ---- NestedMultiTerminal (crates/caroli-green/src/nested_multi.rs) ----
+
+int32_t cg_multi_create(uint64_t n, const int64_t* row_ptr, const uint32_t* cols, const double* vals, uint64_t nsites,
+                        const int64_t* cells, const int32_t* site_orbitals, const uint64_t* site_offsets, int32_t dim,
+                        const int64_t* range, int32_t leads, const cg_lead* lead, cg_system** out) {
+    if (!out || leads < 1 || !lead) return CG_ERR_INVALID;
+    try {
+        auto s = std::make_unique<cg_system>();
+        if (!read_device(n, row_ptr, cols, vals, nsites, cells, site_orbitals, site_offsets, dim, range, s->spec))
+            return CG_ERR_INVALID;
+        std::vector<AttachedLead> ls;
+        std::vector<std::vector<int>> kinds;
+        for (int32_t p = 0; p < leads; ++p) {
+            const cg_lead& l = lead[p];
+            if (l.cell == 0 || l.coupled == 0 || !l.h00 || !l.h01 || !l.orbitals || !l.coupling) return CG_ERR_INVALID;
+            AttachedLead a;
+            a.h00 = read_mat(l.h00, l.cell);
+            a.h01 = read_mat(l.h01, l.cell);
+            a.orbitals.assign(l.orbitals, l.orbitals + l.coupled);
+            for (size_t k = 0; k < a.orbitals.size(); ++k)
+                if (a.orbitals[k] >= n || (k > 0 && a.orbitals[k] <= a.orbitals[k - 1])) return CG_ERR_INVALID;
+            a.coupling = CMat(int(l.coupled), int(l.cell));
+            for (size_t e = 0; e < a.coupling.a.size(); ++e) a.coupling.a[e] = cplx(l.coupling[2 * e], l.coupling[2 * e + 1]);
+            // Sector labels by cell orbital: without the site layout of the cell, orbital k is kind k.
+            std::vector<int> kind(l.cell);
+            for (size_t k = 0; k < kind.size(); ++k) kind[k] = int(k);
+            if (l.kinds)
+                for (size_t k = 0; k < kind.size(); ++k) {
+                    if (l.kinds[k] < 0) return CG_ERR_INVALID;
+                    kind[k] = l.kinds[k];
+                }
+            ls.push_back(std::move(a));
+            kinds.push_back(std::move(kind));
+        }
+        return finish_multi(s.release(), std::move(ls), std::move(kinds), out);
+    } catch (const std::exception&) {
+        return CG_ERR_INVALID;
+    }
+}
+
+int32_t cg_multi_model_create(int32_t model, uint64_t sites, const int64_t* cells, const int32_t* subs,
+                              const double* site_potential, int32_t interfaces, const cg_interface* itf, cg_system** out) {
+    if (!out || sites == 0 || !cells || !subs || interfaces < 1 || !itf || model < 0 || model > 3) return CG_ERR_INVALID;
+    try {
+        Model lead = preset_model(model);
+        auto to_sites = [](uint64_t count, const int64_t* c, const int32_t* sb) {
+            std::vector<Site> v;
+            for (uint64_t i = 0; i < count; ++i) v.push_back(Site{{c[3 * i], c[3 * i + 1], c[3 * i + 2]}, sb[i]});
+            return v;
+        };
+        Region region = Region::from_sites(lead.lattice, to_sites(sites, cells, subs));
+        Model device = site_potential ? with_site_potential(lead, region, site_potential) : lead;
+        std::vector<Region> slices;
+        std::vector<Cell> outward;
+        for (int32_t p = 0; p < interfaces; ++p) {
+            if (itf[p].sites == 0 || !itf[p].cells || !itf[p].subs) return CG_ERR_INVALID;
+            Region slice = Region::from_sites(lead.lattice, to_sites(itf[p].sites, itf[p].cells, itf[p].subs));
+            for (auto& site : slice.sites)
+                if (region.find(site) < 0) return CG_ERR_INVALID;  // an interface of device sites
+            slices.push_back(std::move(slice));
+            outward.push_back(Cell{itf[p].outward[0], itf[p].outward[1], itf[p].outward[2]});
+        }
+        MultiSpec m = multi_from_models(device, lead, region, slices, outward);
+        auto s = std::make_unique<cg_system>();
+        s->spec = std::move(m.base);
+        return finish_multi(s.release(), std::move(m.leads), std::move(m.orbital_kinds), out);
+    } catch (const std::exception&) {
+        return CG_ERR_INVALID;
+    }
+}
+
+int32_t cg_multi_lead_count(const cg_system* s) { return s && s->multi ? int32_t(s->multi_leads.size()) : -1; }
+
+int32_t cg_multi_lead(const cg_system* s, int32_t p, uint64_t* cell, uint64_t* coupled, double* h00, double* h01,
+                      uint64_t* orbitals, double* coupling) {
+    if (!s || !s->multi || p < 0 || p >= int32_t(s->multi_leads.size())) return CG_ERR_INVALID;
+    const AttachedLead& l = s->multi_leads[size_t(p)];
+    if (cell) *cell = uint64_t(l.h00.rows);
+    if (coupled) *coupled = l.orbitals.size();
+    if (h00) write_mat(l.h00, h00);
+    if (h01) write_mat(l.h01, h01);
+    if (orbitals) std::copy(l.orbitals.begin(), l.orbitals.end(), orbitals);
+    if (coupling) write_mat(l.coupling, coupling);
+    return CG_OK;
+}
+
+}  // extern "C"
+
+namespace {
+// Self-energies and broadenings of a multi-terminal system at E + i eta (nested_multi.rs:196-232,
+// :246-313), per block:
+//  - each lead's surface Green's function g = (z - h00 - Sigma_rest)^-1 with Sigma_rest the
+//    right-going self-energy of its own modes (AttachedLead::surface_green_with, multi.rs:43-50),
+//    once per distinct lead;
+//  - Sigma_p = V g V^H of every lead on the block's kept orbitals, summed into one kept x kept matrix;
+//  - per label (p, a) the channels W = V R with R R^H = P_a i(g - g^H) P_a, the lead's broadening
+//    projected on sector a (the whole cell without sectors): W W^H = Gamma_{p,a}.
+int32_t multi_sigmas(const cg_system* s, double energy, double eta, const std::vector<std::vector<int32_t>>& sectors,
+                     cg_sigma** out) {
+    const size_t np = s->multi_leads.size();
+    auto tag = [&](size_t p, int cell_orbital) -> int32_t {
+        if (sectors[p].empty()) return 0;
+        return sectors[p][size_t(s->orbital_kinds[p][size_t(cell_orbital)])];
+    };
+    // Labels (lead, sector) in increasing order (nested_multi.rs:263-270); a lead's Hamiltonian
+    // must not couple different sectors (nested_multi.rs:281-287).
+    auto g = std::make_unique<cg_sigma>();
+    g->energy = energy;
+    g->multi = true;
+    for (size_t p = 0; p < np; ++p) {
+        const AttachedLead& l = s->multi_leads[p];
+        for (int k = 0; k < l.h00.rows; ++k)
+            if (!sectors[p].empty() && size_t(s->orbital_kinds[p][size_t(k)]) >= sectors[p].size()) return CG_ERR_INVALID;
+        for (const CMat* m : {&l.h00, &l.h01})
+            for (int b = 0; b < m->cols; ++b)
+                for (int a = 0; a < m->rows; ++a)
+                    if (tag(p, a) != tag(p, b) && (*m)(a, b) != cplx(0.0)) return CG_ERR_INVALID;
+        std::vector<int32_t> own;
+        for (int k = 0; k < l.h00.rows; ++k) own.push_back(tag(p, k));
+        std::sort(own.begin(), own.end());
+        own.erase(std::unique(own.begin(), own.end()), own.end());
+        for (int32_t a : own) g->labels.push_back({int32_t(p), a});
+    }
+    const size_t L = g->labels.size();
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        if (s->lead_modes.empty()) {
+            s->lead_modes.resize(s->sym.blocks.size());
+            for (size_t b = 0; b < s->sym.blocks.size(); ++b) {
+                const Block& blk = s->sym.blocks[b];
+                auto& v = s->lead_modes[b];
+                v.resize(np);
+                for (size_t p = 0; p < np; ++p) {
+                    const AttachedLead& l = blk.leads[p];
+                    if (l.orbitals.empty()) continue;
+                    for (size_t q = 0; q < p && !v[p]; ++q)
+                        if (v[q] && blk.leads[q].h00 == l.h00 && blk.leads[q].h01 == l.h01) v[p] = v[q];
+                    if (!v[p]) v[p] = std::make_shared<ModeSolver>(l.h00, l.h01);
+                }
+            }
+        }
+    }
+    const cplx z(energy, eta);
+    for (size_t b = 0; b < s->sym.blocks.size(); ++b) {
+        const Block& blk = s->sym.blocks[b];
+        const int k = int(blk.kept.size());
+        Positions kept(blk.kept);
+        cg_sigma::Block sb;
+        sb.sl = CMat(k, k);
+        sb.sr = CMat(k, k);
+        sb.gl = CMat(k, k);
+        sb.gr = CMat(k, k);
+        sb.wl = CMat(k, 0);
+        sb.wr = CMat(k, 0);
+        sb.left_orbitals = blk.kept;
+        sb.right_orbitals = blk.kept;
+        cg_sigma::MultiBlock mb;
+        mb.leads = blk.leads;
+        mb.sigma.resize(np);
+        std::vector<std::vector<cplx>> cols;  // W columns by label
+        std::vector<int32_t> count(L, 0);
+        std::map<const ModeSolver*, CMat> surfaces;  // one g per distinct lead
+        for (size_t p = 0; p < np; ++p) {
+            const AttachedLead& l = blk.leads[p];
+            if (l.orbitals.empty()) continue;
+            const ModeSolver* solver = s->lead_modes[b][p].get();
+            auto it = surfaces.find(solver);
+            if (it == surfaces.end()) {
+                LeadModes md;
+                Status st = eta == 0.0 ? solver->modes_at(energy, md) : solver->modes_at(z, md);
+                if (st != kOk) return st;
+                const CMat rest = md.sigma_right();
+                const int nc = l.h00.rows;
+                CMat a(nc, nc);
+                for (int j = 0; j < nc; ++j)
+                    for (int i = 0; i < nc; ++i) a(i, j) = (i == j ? z : cplx(0.0)) - l.h00(i, j) - rest(i, j);
+                it = surfaces.emplace(solver, dense::inverse(a)).first;
+            }
+            const CMat& gs = it->second;
+            const CMat sig = matmul(matmul(l.coupling, gs), l.coupling.adjoint());
+            mb.sigma[p] = sig;
+            std::vector<int> at;
+            for (size_t o : l.orbitals) at.push_back(kept.get(o));
+            for (size_t y = 0; y < at.size(); ++y)
+                for (size_t x = 0; x < at.size(); ++x) sb.sl(at[x], at[y]) += sig(int(x), int(y));
+            // Broadening of the cell, projected on each sector of the lead.
+            const int nc = gs.rows;
+            const std::vector<size_t>& cell = blk.lead_cells[p];
+            CMat gam(nc, nc);
+            for (int j = 0; j < nc; ++j)
+                for (int i = 0; i < nc; ++i) gam(i, j) = cplx(0.0, 1.0) * (gs(i, j) - std::conj(gs(j, i)));
+            for (size_t lab = 0; lab < L; ++lab) {
+                if (g->labels[lab].lead != int32_t(p)) continue;
+                CMat proj(nc, nc);
+                for (int j = 0; j < nc; ++j)
+                    for (int i = 0; i < nc; ++i)
+                        if (tag(p, int(cell[size_t(i)])) == g->labels[lab].sector &&
+                            tag(p, int(cell[size_t(j)])) == g->labels[lab].sector)
+                            proj(i, j) = gam(i, j);
+                const CMat w = matmul(l.coupling, gamma_factor(proj));
+                for (int c = 0; c < w.cols; ++c) {
+                    std::vector<cplx> col(size_t(k), cplx(0.0));
+                    for (size_t x = 0; x < at.size(); ++x) col[size_t(at[x])] = w(int(x), c);
+                    cols.push_back(std::move(col));
+                }
+                count[lab] += w.cols;
+            }
+        }
+        // The columns went in lead by lead and, within a lead, label by label: the label order.
+        mb.lab_off.assign(L + 1, 0);
+        for (size_t lab = 0; lab < L; ++lab) mb.lab_off[lab + 1] = mb.lab_off[lab] + count[lab];
+        mb.wab = CMat(k, int(cols.size()));
+        for (size_t c = 0; c < cols.size(); ++c) std::copy(cols[c].begin(), cols[c].end(), mb.wab.col(int(c)));
+        g->blocks.push_back(std::move(sb));
+        g->mblocks.push_back(std::move(mb));
+    }
+    *out = g.release();
+    return CG_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int32_t cg_multi_self_energies(const cg_system* s, double energy, double eta, const int32_t* sector_len,
+                               const int32_t* sector_tags, cg_sigma** out) {
+    if (!s || !s->multi || !out || !(eta >= 0.0) || !std::isfinite(eta) || !std::isfinite(energy)) return CG_ERR_INVALID;
+    std::vector<std::vector<int32_t>> sectors(s->multi_leads.size());
+    if (sector_len) {
+        const int32_t* t = sector_tags;
+        for (size_t p = 0; p < sectors.size(); ++p) {
+            if (sector_len[p] < 0 || (sector_len[p] > 0 && !t)) return CG_ERR_INVALID;
+            sectors[p].assign(t, t + sector_len[p]);
+            t += sector_len[p];
+        }
+    }
+    try {
+        return multi_sigmas(s, energy, eta, sectors, out);
+    } catch (const std::exception&) {
+        return CG_ERR_EIGEN;
+    }
+}
+
+int32_t cg_multi_sigma(const cg_system* s, const cg_sigma* g, int32_t p, double* out) {
+    if (!s || !g || !out || !s->multi || !sigma_matches(*s, *g) || p < 0 || p >= int32_t(s->multi_leads.size()))
+        return CG_ERR_INVALID;
+    const std::vector<size_t>& all = s->multi_leads[size_t(p)].orbitals;
+    CMat full(int(all.size()), int(all.size()));
+    for (size_t b = 0; b < g->mblocks.size(); ++b) {
+        const auto& orbs = g->mblocks[b].leads[size_t(p)].orbitals;
+        const CMat& sig = g->mblocks[b].sigma[size_t(p)];
+        std::vector<int> at;
+        for (size_t o : orbs) at.push_back(int(std::lower_bound(all.begin(), all.end(), o) - all.begin()));
+        for (size_t y = 0; y < at.size(); ++y)
+            for (size_t x = 0; x < at.size(); ++x) full(at[x], at[y]) = sig(int(x), int(y));
+    }
+    write_mat(full, out);
+    return CG_OK;
+}
+
+int32_t cg_multi_labels(const cg_sigma* g, int32_t* count, int32_t* lead, int32_t* sector) {
+    if (!g || !g->multi || !count) return CG_ERR_INVALID;
+    *count = int32_t(g->labels.size());
+    for (size_t q = 0; q < g->labels.size(); ++q) {
+        if (lead) lead[q] = g->labels[q].lead;
+        if (sector) sector[q] = g->labels[q].sector;
+    }
+    return CG_OK;
+}
+
+int32_t cg_multi_batch_transmissions(const cg_system* const* systems, const cg_sigma* const* sigmas, int64_t count,
+                                     int32_t precision, double* out, int32_t* out_status, cg_batch_stats* stats) {
+    if (count < 0 || (count > 0 && (!systems || !sigmas || !out))) return CG_ERR_INVALID;
+    if (precision != CG_DOUBLE && precision != CG_SINGLE) return CG_ERR_INVALID;
+    for (int64_t p = 0; p < count; ++p)
+        if (!systems[p] || !sigmas[p] || !systems[p]->multi || !sigma_matches(*systems[p], *sigmas[p]) ||
+            sigmas[p]->labels.size() != sigmas[0]->labels.size())
+            return CG_ERR_INVALID;
+    if (count == 0) return CG_OK;
+    try {
+        cg_batch* bt = nullptr;
+        int32_t rc = gpu::create_batch(systems, sigmas, count, precision, &bt);
+        if (rc != CG_OK) return rc;
+        std::unique_ptr<cg_batch, void (*)(cg_batch*)> hold(bt, gpu::destroy_batch);
+        return gpu::run_batch_labels(bt, out, out_status, stats);
+    } catch (const std::exception&) {
+        return CG_ERR_DEVICE;
+    }
+}
+
+int32_t cg_multi_transmissions(const cg_system* s, double energy, int32_t precision, double* out) {
+    if (!s || !s->multi || !out) return CG_ERR_INVALID;
+    cg_sigma* g = nullptr;
+    int32_t rc = cg_multi_self_energies(s, energy, 0.0, nullptr, nullptr, &g);
+    if (rc != CG_OK) return rc;
+    std::unique_ptr<cg_sigma, void (*)(cg_sigma*)> hold(g, cg_sigma_destroy);
+    int32_t st = CG_OK;
+    rc = cg_multi_batch_transmissions(&s, const_cast<const cg_sigma* const*>(&g), 1, precision, out, &st, nullptr);
+    return rc != CG_OK ? rc : st;
+}
 
 }  // extern "C"

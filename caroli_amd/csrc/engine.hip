@@ -536,6 +536,11 @@ struct Inputs {
     DevBuf vals_ptrs, z, sigma, gamma, w, wcols;
     int64_t stride = 0;
     int32_t cl_max = 0, cr_max = 0;  // largest channel counts of the points
+    // Multi-terminal points: the channels of every label (kept x lab_cols each) and the labels'
+    // column ranges (nlab + 1 each).
+    DevBuf wlab, lab_off;
+    int64_t wlab_stride = 0;
+    int32_t lab_cols = 0, nlab = 0;
 };
 
 /// Kernel families timed by cg_kernel_timing (border_kernel stands for factor_global_kernel when
@@ -602,7 +607,7 @@ struct RefineBufs {
 
 struct Lane {
     hipStream_t stream = nullptr;
-    DevBuf arena, root_arena, piv, status, tout, trace_x, trace_p;
+    DevBuf arena, root_arena, piv, status, tout, trace_x, trace_p, tlab;
     RefineBufs rf;
 };
 constexpr int kLanes = 4;
@@ -732,6 +737,11 @@ void bind_inputs(const Inputs& in, PointsView& pv) {
     pv.w = in.w.as<const double2>();
     pv.wcols = in.wcols.as<const int32_t>();
     pv.sigma_stride = in.stride;
+    pv.wlab = in.wlab.as<const double2>();
+    pv.lab_off = in.lab_off.as<const int32_t>();
+    pv.wlab_stride = in.wlab_stride;
+    pv.lab_cols = in.lab_cols;
+    pv.nlab = in.nlab;
 }
 
 /// Per-point inputs of one block: energies, self-energies and broadenings, Hamiltonian values.
@@ -772,6 +782,28 @@ void stage_points(Engine& eng, const std::vector<const cg_system*>& sys, const s
     upload(in.w, wv, eng.lane->stream);
     upload(in.wcols, wc, eng.lane->stream);
     in.stride = stride;
+    in.nlab = 0;
+    in.lab_cols = 0;
+    in.wlab_stride = 0;
+    if (P > 0 && sig[0]->multi) {
+        // W of every label on the kept orbitals (nl of them), padded to the widest point.
+        const int32_t L = int32_t(sig[0]->labels.size());
+        int32_t cols = 0;
+        for (size_t p = 0; p < P; ++p) cols = std::max(cols, sig[p]->mblocks[b].wab.cols);
+        std::vector<double2> wl(P * size_t(nl) * size_t(cols), double2{0.0, 0.0});
+        std::vector<int32_t> off(P * size_t(L + 1));
+        for (size_t p = 0; p < P; ++p) {
+            const auto& mb = sig[p]->mblocks[b];
+            double2* o = wl.data() + p * size_t(nl) * size_t(cols);
+            for (size_t e = 0; e < mb.wab.a.size(); ++e) o[e] = double2{mb.wab.a[e].real(), mb.wab.a[e].imag()};
+            std::copy(mb.lab_off.begin(), mb.lab_off.end(), off.begin() + p * size_t(L + 1));
+        }
+        upload(in.wlab, wl, eng.lane->stream);
+        upload(in.lab_off, off, eng.lane->stream);
+        in.nlab = L;
+        in.lab_cols = cols;
+        in.wlab_stride = int64_t(nl) * cols;
+    }
     bind_inputs(in, pv);
 }
 
@@ -865,6 +897,14 @@ void run_level(Engine& eng, const LevelSet& L, const DevLevel& D, const NodeDesc
 /// the engine).
 void launch_trace_for(Engine& eng, const NodeDesc* root, int32_t nl, const Inputs& in, const PointsView& pv,
                       int32_t P) {
+    if (in.nlab > 0) {  // multi-terminal: T between every pair of labels
+        const size_t cols = size_t(std::max(in.lab_cols, 1));
+        eng.lane->trace_x.ensure(size_t(P) * size_t(nl) * cols * sizeof(double2));
+        eng.lane->trace_p.ensure(size_t(P) * cols * cols * sizeof(double));
+        launch_trace_labels(root, nl, in.lab_cols, eng.lane->trace_x.as<double2>(), eng.lane->trace_p.as<double>(), pv,
+                            P, eng.lane->stream);
+        return;
+    }
     double2* xbuf = nullptr;
     double* partial = nullptr;
     if (std::max(in.cl_max, in.cr_max) > kTraceFewChannels) {
@@ -906,8 +946,9 @@ struct Handed {
 /// Re-planned sweep of one point of one block whose separators delayed: level by level, the
 /// fronts above delayed children are assembled from the variables actually handed over
 /// (nested.rs:331-344), exactly as the reference's unplanned path. Returns the block's T or NaN
-/// and the status.
-int32_t replanned_point(Engine& eng, const cg_system* s, const cg_sigma* g, int b, bool single, double& t_out) {
+/// and the status; for a multi-terminal point, T between its labels in lab_out (nlab^2).
+int32_t replanned_point(Engine& eng, const cg_system* s, const cg_sigma* g, int b, bool single, double& t_out,
+                        double* lab_out = nullptr) {
     const Block& blk = s->sym.blocks[b];
     const Tree& t = blk.tree;
     const int nn = int(t.nodes.size());
@@ -916,8 +957,14 @@ int32_t replanned_point(Engine& eng, const cg_system* s, const cg_sigma* g, int 
     std::vector<const cg_sigma*> one_g{g};
     PointsView pv{};
     stage_points(eng, one_s, one_g, b, int(blk.left_orbitals.size()), int(blk.right_orbitals.size()), eng.scratch_in, pv);
-    DevBuf arena, piv, status, tout, root, dn, dent, ddiag, dkids, dmaps;
+    DevBuf arena, piv, status, tout, root, dn, dent, ddiag, dkids, dmaps, tlab;
     int64_t used = 0, piv_used = 0;
+    const int32_t nlab = eng.scratch_in.nlab;
+    if (nlab > 0) {
+        if (!lab_out) throw std::logic_error("a multi-terminal point without its label output");
+        tlab.ensure(sizeof(double) * size_t(nlab) * size_t(nlab));
+        pv.t_lab = tlab.as<double>();
+    }
     auto grow = [&](DevBuf& buf, int64_t need_bytes, int64_t keep_bytes) {
         if (size_t(need_bytes) <= buf.bytes) return;
         DevBuf nb;
@@ -1082,8 +1129,16 @@ int32_t replanned_point(Engine& eng, const cg_system* s, const cg_sigma* g, int 
             HIP_CHECK(hipGetLastError());
             int32_t rs = 0;
             HIP_CHECK(hipMemcpyAsync(&rs, static_cast<int32_t*>(status.p) + nn, sizeof(int32_t), hipMemcpyDeviceToHost, eng.lane->stream));
-            HIP_CHECK(hipMemcpyAsync(&t_out, tout.p, sizeof(double), hipMemcpyDeviceToHost, eng.lane->stream));
+            if (nlab > 0) {
+                HIP_CHECK(hipMemcpyAsync(lab_out, tlab.p, sizeof(double) * size_t(nlab) * size_t(nlab),
+                                         hipMemcpyDeviceToHost, eng.lane->stream));
+                t_out = 0.0;
+            } else {
+                HIP_CHECK(hipMemcpyAsync(&t_out, tout.p, sizeof(double), hipMemcpyDeviceToHost, eng.lane->stream));
+            }
             HIP_CHECK(hipStreamSynchronize(eng.lane->stream));
+            if (nlab > 0)
+                for (int32_t q = 0; q < nlab * nlab; ++q) t_out += lab_out[q];
             if (rs != kFrontOk) {
                 t_out = NAN;
                 return CG_ERR_SINGULAR;
@@ -1364,7 +1419,7 @@ void release(const cg_system* s) {
 /// kernel in force: a replay is valid while they are the same.
 struct SweepGraph {
     hipGraphExec_t exec = nullptr;
-    std::array<const void*, 9> key{};
+    std::array<const void*, 10> key{};
     int eager_runs = 0;
     SweepGraph() = default;
     SweepGraph(SweepGraph&& o) noexcept : exec(o.exec), key(o.key), eager_runs(o.eager_runs) { o.exec = nullptr; }
@@ -1399,6 +1454,7 @@ struct cg_batch {
     double flops = 0, bytes = 0;
     int lanes = 1;       // units in flight at once (caroli::gpu::kLanes when their buffers fit)
     bool eager = false;  // sweeps without HIP graphs (the autotuner's timed runs)
+    int32_t nlab = 0;    // multi-terminal points: T is nlab x nlab per point
 };
 
 namespace caroli::gpu {
@@ -1430,6 +1486,7 @@ int32_t create_batch(const cg_system* const* systems, const cg_sigma* const* sig
     bt->systems.assign(systems, systems + count);
     bt->sigmas.assign(sigmas, sigmas + count);
     bt->precision = precision;
+    bt->nlab = count > 0 && sigmas[0]->multi ? int32_t(sigmas[0]->labels.size()) : 0;
     const bool single = precision != CG_DOUBLE;
     const size_t es = single ? sizeof(float2) : sizeof(double2);
     std::map<uint64_t, std::vector<int64_t>> groups;
@@ -1518,10 +1575,10 @@ void sweep_graph(Engine& eng, cg_batch::Unit& u, const HostPlan& hp, const DevPl
     Lane& ln = *eng.lane;
     // The small-front kernel is chosen at launch (the tuning's small_fronts): a graph captured
     // under another choice is captured anew.
-    const std::array<const void*, 9> key{ln.arena.p,   ln.root_arena.p, ln.piv.p,
-                                         ln.status.p,  ln.tout.p,       ln.trace_x.p,
-                                         ln.trace_p.p, reinterpret_cast<const void*>(intptr_t(P)),
-                                         reinterpret_cast<const void*>(intptr_t(small_front_mode()))};
+    const std::array<const void*, 10> key{ln.arena.p,   ln.root_arena.p, ln.piv.p,
+                                          ln.status.p,  ln.tout.p,       ln.trace_x.p,
+                                          ln.trace_p.p, ln.tlab.p,       reinterpret_cast<const void*>(intptr_t(P)),
+                                          reinterpret_cast<const void*>(intptr_t(small_front_mode()))};
     SweepGraph& g = u.graph;
     if (g.exec && g.key == key) {
         HIP_CHECK(hipGraphLaunch(g.exec, ln.stream));
@@ -1543,7 +1600,9 @@ void sweep_graph(Engine& eng, cg_batch::Unit& u, const HostPlan& hp, const DevPl
     HIP_CHECK(hipGraphLaunch(g.exec, ln.stream));
 }
 
-int32_t run_batch(cg_batch* bt, double* out_t, int32_t* out_status, cg_batch_stats* stats) {
+/// The batch's points; out_lab: for multi-terminal points, T between labels (nlab^2 per point,
+/// summed over blocks), with out_t the sum of each point's entries.
+int32_t run_batch_impl(cg_batch* bt, double* out_t, double* out_lab, int32_t* out_status, cg_batch_stats* stats) {
     Engine& eng = Engine::get();
     if (!eng.ok) return CG_ERR_DEVICE;
     const bool mixed = bt->precision == CG_MIXED;
@@ -1556,6 +1615,12 @@ int32_t run_batch(cg_batch* bt, double* out_t, int32_t* out_status, cg_batch_sta
     std::vector<char> redo(count, 0);  // mixed: recompute in complex f64
     bool recompute = false;            // mixed: some point goes to the recomputation
     for (int64_t p = 0; p < count; ++p) out_t[p] = 0.0;
+    const int32_t nlab = bt->nlab;
+    const size_t lab2 = size_t(nlab) * size_t(nlab);
+    if (nlab > 0) {
+        if (!out_lab || mixed) return CG_ERR_INVALID;
+        std::fill(out_lab, out_lab + size_t(count) * lab2, 0.0);
+    }
     {
         std::lock_guard<std::mutex> lock(eng.mu);
         eng.tol_single = mixed ? kTolMixed : kTolSingle;
@@ -1578,6 +1643,7 @@ int32_t run_batch(cg_batch* bt, double* out_t, int32_t* out_status, cg_batch_sta
             std::vector<PointsView> pvs(u1 - u0);
             std::vector<std::vector<double>> ts(u1 - u0);
             std::vector<std::vector<int32_t>> sts(u1 - u0);
+            std::vector<std::vector<double>> labs(u1 - u0);
             auto w0 = std::chrono::steady_clock::now();
             for (size_t ui = u0; ui < u1; ++ui) {
                 auto& u = bt->units[ui];
@@ -1599,6 +1665,10 @@ int32_t run_batch(cg_batch* bt, double* out_t, int32_t* out_status, cg_batch_sta
                 pv.status = eng.lane->status.as<int32_t>();
                 pv.status_stride = hp.slots;
                 pv.t_out = eng.lane->tout.as<double>();
+                if (nlab > 0) {
+                    eng.lane->tlab.ensure(u.points.size() * lab2 * sizeof(double));
+                    pv.t_lab = eng.lane->tlab.as<double>();
+                }
                 pv.keep = mixed ? 1 : 0;  // the refinement solves with the factors
                 if (!eng.timing && !bt->eager && graphs_enabled())
                     sweep_graph(eng, u, hp, dp, pv, int32_t(u.points.size()), single);
@@ -1616,6 +1686,11 @@ int32_t run_batch(cg_batch* bt, double* out_t, int32_t* out_status, cg_batch_sta
                                          ln.stream));
                 HIP_CHECK(hipMemcpyAsync(sts[ui - u0].data(), ln.status.p, sts[ui - u0].size() * 4,
                                          hipMemcpyDeviceToHost, ln.stream));
+                if (nlab > 0) {
+                    labs[ui - u0].resize(P * lab2);
+                    HIP_CHECK(hipMemcpyAsync(labs[ui - u0].data(), ln.tlab.p, P * lab2 * sizeof(double),
+                                             hipMemcpyDeviceToHost, ln.stream));
+                }
             }
             for (size_t ui = u0; ui < u1; ++ui) HIP_CHECK(hipStreamSynchronize(eng.lanes[ui - u0].stream));
             eng.lane = &eng.lanes[0];
@@ -1687,8 +1762,14 @@ int32_t run_batch(cg_batch* bt, double* out_t, int32_t* out_status, cg_batch_sta
                 auto& u = bt->units[ui];
                 const HostPlan& hp = u.structure->host[u.block];
                 const size_t P = u.points.size();
-                const std::vector<double>& t = ts[ui - u0];
+                std::vector<double>& t = ts[ui - u0];
                 const std::vector<int32_t>& st = sts[ui - u0];
+                if (nlab > 0)  // the point's T is the sum of its label matrix (NaN when any entry is)
+                    for (size_t q = 0; q < P; ++q) {
+                        t[q] = 0.0;
+                        for (size_t e = 0; e < lab2; ++e) t[q] += labs[ui - u0][q * lab2 + e];
+                    }
+                std::vector<double> lab_tmp(lab2);
                 for (size_t q = 0; q < P; ++q) {
                     int64_t p = u.points[q];
                     bool delayed = false;
@@ -1697,15 +1778,18 @@ int32_t run_batch(cg_batch* bt, double* out_t, int32_t* out_status, cg_batch_sta
                     if (delayed) {
                         double tb = NAN;
                         auto t0 = std::chrono::steady_clock::now();
-                        int32_t rc = replanned_point(eng, bt->systems[p], bt->sigmas[p], u.block, single, tb);
+                        int32_t rc = replanned_point(eng, bt->systems[p], bt->sigmas[p], u.block, single, tb,
+                                                     nlab > 0 ? lab_tmp.data() : nullptr);
                         local.seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
                         ++local.delayed_points;
                         if (rc != CG_OK) status[p] = rc;
                         out_t[p] += tb;
+                        for (size_t e = 0; e < lab2; ++e) out_lab[size_t(p) * lab2 + e] += lab_tmp[e];
                     } else if (root_status != kFrontOk || !std::isfinite(t[q])) {
                         status[p] = CG_ERR_SINGULAR;
                     } else {
                         out_t[p] += t[q];
+                        for (size_t e = 0; e < lab2; ++e) out_lab[size_t(p) * lab2 + e] += labs[ui - u0][q * lab2 + e];
                     }
                 }
             }
@@ -1741,11 +1825,25 @@ int32_t run_batch(cg_batch* bt, double* out_t, int32_t* out_status, cg_batch_sta
         eng.collect();  // timers of the refinement and of re-planned points
     }
     for (int64_t p = 0; p < count; ++p) {
-        if (status[p] != CG_OK) out_t[p] = NAN;
+        if (status[p] != CG_OK) {
+            out_t[p] = NAN;
+            if (nlab > 0) std::fill(out_lab + size_t(p) * lab2, out_lab + size_t(p + 1) * lab2, NAN);
+        }
         if (out_status) out_status[p] = status[p];
     }
     if (stats) *stats = local;
     return CG_OK;
+}
+
+int32_t run_batch(cg_batch* bt, double* out_t, int32_t* out_status, cg_batch_stats* stats) {
+    if (bt->nlab > 0) return CG_ERR_INVALID;  // multi-terminal batches give a matrix per point
+    return run_batch_impl(bt, out_t, nullptr, out_status, stats);
+}
+
+int32_t run_batch_labels(cg_batch* bt, double* out_lab, int32_t* out_status, cg_batch_stats* stats) {
+    if (bt->nlab <= 0) return CG_ERR_INVALID;
+    std::vector<double> t(bt->systems.size());
+    return run_batch_impl(bt, t.data(), out_lab, out_status, stats);
 }
 
 int32_t batch_transmission(const cg_system* const* systems, const cg_sigma* const* sigmas, int64_t count,
@@ -1811,14 +1909,18 @@ int32_t autotune_batch(cg_batch* bt, cg_tuning* out, double* out_seconds) {
         for (int64_t p = 0; p < count; ++p) structure_of(eng, bt->systems[p]);
     };
     int32_t rc = CG_OK;
+    std::vector<double> lab(t.size() * size_t(bt->nlab) * size_t(bt->nlab));
+    auto run = [&] {
+        return bt->nlab > 0 ? run_batch_labels(bt, lab.data(), st.data(), nullptr) : run_batch(bt, t.data(), st.data(), nullptr);
+    };
     auto measure = [&](const Tuning& tu) {
         apply(tu);
         bt->eager = true;
-        rc = run_batch(bt, t.data(), st.data(), nullptr);
+        rc = run();
         double sec = INFINITY;
         for (int rep = 0; rep < 2 && rc == CG_OK; ++rep) {  // the faster of two timed sweeps
             auto t0 = std::chrono::steady_clock::now();
-            rc = run_batch(bt, t.data(), st.data(), nullptr);
+            rc = run();
             sec = std::min(sec, std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count());
         }
         bt->eager = false;

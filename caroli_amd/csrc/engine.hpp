@@ -15,6 +15,8 @@ int32_t batch_transmission(const cg_system* const* systems, const cg_sigma* cons
 int32_t create_batch(const cg_system* const* systems, const cg_sigma* const* sigmas, int64_t count, int32_t precision,
                      cg_batch** out);
 int32_t run_batch(cg_batch* b, double* out_t, int32_t* out_status, cg_batch_stats* stats);
+/// A batch of multi-terminal points: T between labels, nlab x nlab per point (row: into).
+int32_t run_batch_labels(cg_batch* b, double* out_lab, int32_t* out_status, cg_batch_stats* stats);
 void destroy_batch(cg_batch* b);
 int32_t get_tuning(cg_tuning* out);
 int32_t set_tuning(const cg_tuning* t);

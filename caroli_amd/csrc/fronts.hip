@@ -2165,6 +2165,83 @@ __global__ void __launch_bounds__(64)
     }
 }
 
+// Multi-terminal roots: T[a][b] = Tr[Gamma_a G Gamma_b G^H] = |W_a^H G W_b|_F^2 between every
+// pair of labels (leads, or sectors of leads). X = G W over all labels' channels at once (the
+// tiles of trace_x_kernel), then Y = W^H X with |Y|^2 stored entry by entry, then per pair of
+// labels the sum over its block of Y, one thread per pair in a fixed order (no atomics).
+__global__ void __launch_bounds__(256)
+    label_x_kernel(const NodeDesc* __restrict__ rootp, PointsView pv, double2* __restrict__ X, int cols, int tiles_j) {
+    __shared__ double2 As[kChunk * kTileLd];
+    __shared__ double2 Bs[kChunk * kTileLd];
+    const NodeDesc nd = *rootp;
+    const int64_t p = blockIdx.y;
+    const int k = nd.nl, n = nd.n, k0 = nd.ne;
+    const int i0 = kTile * (blockIdx.x / tiles_j), j0 = kTile * (blockIdx.x % tiles_j);
+    const double2* G = reinterpret_cast<const double2*>(pv.root_arena) + p * pv.root_stride + nd.off + k0 +
+                       int64_t(k0) * n;  // the border block: -G on the kept orbitals
+    const double2* W = pv.wlab + p * pv.wlab_stride;
+    f64x4 cr[2][2], ci[2][2];
+    tile_zgemm<false>(G, n, W, k, k, cols, k, i0, j0, As, Bs, cr, ci);
+    double2* Xp = X + p * int64_t(k) * cols;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, wi = (wave & 1) * 32, wj = (wave >> 1) * 32;
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y) {
+            const int i = i0 + wi + 16 * y + lane % 16;
+#pragma unroll
+            for (int jj = 0; jj < 4; ++jj) {
+                const int j = j0 + wj + 16 * x + Mfma<double2>::row(lane, jj);
+                if (i < k && j < cols) Xp[i + int64_t(j) * k] = double2{cr[x][y][jj], ci[x][y][jj]};
+            }
+        }
+}
+
+__global__ void __launch_bounds__(256)
+    label_y_kernel(const NodeDesc* __restrict__ rootp, PointsView pv, const double2* __restrict__ X, int cols,
+                   int tiles_j, double* __restrict__ ysq) {
+    __shared__ double2 As[kChunk * kTileLd];
+    __shared__ double2 Bs[kChunk * kTileLd];
+    const NodeDesc nd = *rootp;
+    const int64_t p = blockIdx.y;
+    const int k = nd.nl;
+    const int i0 = kTile * (blockIdx.x / tiles_j), j0 = kTile * (blockIdx.x % tiles_j);
+    const double2* W = pv.wlab + p * pv.wlab_stride;
+    f64x4 cr[2][2], ci[2][2];
+    tile_zgemm<true>(W, k, X + p * int64_t(k) * cols, k, cols, cols, k, i0, j0, As, Bs, cr, ci);
+    double* Yp = ysq + p * int64_t(cols) * cols;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, wi = (wave & 1) * 32, wj = (wave >> 1) * 32;
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y) {
+            const int i = i0 + wi + 16 * y + lane % 16;
+#pragma unroll
+            for (int jj = 0; jj < 4; ++jj) {
+                const int j = j0 + wj + 16 * x + Mfma<double2>::row(lane, jj);
+                if (i < cols && j < cols)
+                    Yp[i + int64_t(j) * cols] = cr[x][y][jj] * cr[x][y][jj] + ci[x][y][jj] * ci[x][y][jj];
+            }
+        }
+}
+
+__global__ void __launch_bounds__(64)
+    label_sum_kernel(const NodeDesc* __restrict__ rootp, PointsView pv, const double* __restrict__ ysq, int cols) {
+    const NodeDesc nd = *rootp;
+    const int64_t p = blockIdx.x;
+    const int L = pv.nlab;
+    const int32_t* off = pv.lab_off + p * (L + 1);
+    const double* Yp = ysq + p * int64_t(cols) * cols;
+    const bool ok = pv.status[p * pv.status_stride + nd.slot] == kFrontOk;
+    for (int pair = threadIdx.x; pair < L * L; pair += 64) {
+        const int a = pair / L, b = pair % L;
+        double acc = 0.0;
+        for (int j = off[b]; j < off[b + 1]; ++j)
+            for (int i = off[a]; i < off[a + 1]; ++i) acc += Yp[i + int64_t(j) * cols];
+        pv.t_lab[p * int64_t(L) * L + pair] = ok ? acc : NAN;
+    }
+}
+
 inline dim3 grid2(int32_t count, int32_t points) { return dim3(unsigned(count), unsigned(points)); }
 
 // Dynamic LDS above the 64 KB default (gfx950 has 160 KB per workgroup), once per kernel.
@@ -2466,6 +2543,19 @@ void launch_factor_unblocked(const NodeDesc* nodes, const int32_t* level, int32_
     else
         hipLaunchKernelGGL((factor_global_kernel<double2, false>), grid2(count, points), dim3(kThreads), 0, st, nodes,
                            level, pv, tol);
+}
+
+void launch_trace_labels(const NodeDesc* root, int32_t k, int32_t lab_cols, double2* xbuf, double* ysq,
+                         const PointsView& pv, int32_t points, hipStream_t st) {
+    if (points <= 0) return;
+    if (lab_cols > 0) {
+        const int ti = (k + kTile - 1) / kTile, tj = (lab_cols + kTile - 1) / kTile;
+        hipLaunchKernelGGL(label_x_kernel, dim3(unsigned(ti * tj), unsigned(points)), dim3(256), 0, st, root, pv, xbuf,
+                           lab_cols, tj);
+        hipLaunchKernelGGL(label_y_kernel, dim3(unsigned(tj * tj), unsigned(points)), dim3(256), 0, st, root, pv, xbuf,
+                           lab_cols, tj, ysq);
+    }
+    hipLaunchKernelGGL(label_sum_kernel, dim3(unsigned(points)), dim3(64), 0, st, root, pv, ysq, lab_cols);
 }
 
 void launch_trace(const NodeDesc* root, int32_t nl, int32_t cl_max, int32_t cr_max, double2* xbuf, double* partial,

@@ -79,6 +79,15 @@ struct PointsView {
     int32_t status_stride;
     double* t_out;    // per point
     int32_t keep;     // 1: small fronts write their factors back (for the solves), 0: only S
+    // Multi-terminal roots (nlab > 0): the root's border block holds -G on all kept orbitals;
+    // wlab holds the channels of every label (kept x lab_cols per point, wlab_stride apart, zero
+    // past a point's columns), label l in columns lab_off[l] .. lab_off[l + 1] (nlab + 1 per
+    // point), and t_lab receives T between labels (nlab x nlab per point, row: into, column: from).
+    const double2* wlab;
+    const int32_t* lab_off;
+    int64_t wlab_stride;
+    int32_t lab_cols, nlab;
+    double* t_lab;
 };
 
 /// A kTile x kTile tile of the Schur block of front `node` (rows i0.., columns j0.. of S).
@@ -191,6 +200,11 @@ constexpr int kTraceRows = 4096;
 constexpr int kTraceFewChannels = 16;
 void launch_trace(const NodeDesc* root, int32_t nl, int32_t cl_max, int32_t cr_max, double2* xbuf, double* partial,
                   const PointsView& pv, int32_t points, hipStream_t st);
+/// T between labelled groups of lead channels of a multi-terminal root (PointsView::wlab): X = G W
+/// (scratch xbuf of points x k x lab_cols), |W^H X|^2 entry by entry (ysq, points x lab_cols^2),
+/// then the sums over each pair of labels into t_lab. k: the kept orbitals.
+void launch_trace_labels(const NodeDesc* root, int32_t k, int32_t lab_cols, double2* xbuf, double* ysq,
+                         const PointsView& pv, int32_t points, hipStream_t st);
 /// Sizes of the scratch of launch_trace's GEMM path.
 inline size_t trace_x_words(int32_t nl, int32_t cr_max) { return size_t(nl) * size_t(cr_max); }
 inline size_t trace_partials(int32_t cl_max, int32_t cr_max) {

@@ -499,9 +499,76 @@ SystemSpec disordered_system(int model, const size_t extent[3], int axis, const 
     return system_from_models(device, lead, region, axis);
 }
 
+Model preset_model(int model) {
+    if (model < 0 || model > 3) throw std::invalid_argument("unknown preset model");
+    return model == 0 ? bhz(BhzParams{}) : model == 1 ? square_model() : model == 2 ? graphene_model() : cubic_model();
+}
+
+Model with_site_potential(const Model& base, const Region& region, const double* potential) {
+    Model m = base;
+    auto pot = std::make_shared<std::map<Site, double>>();
+    for (size_t i = 0; i < region.sites.size(); ++i) (*pot)[region.sites[i]] = potential[i];
+    m.extra.push_back([pot](const Site& s, CMat& v) {
+        auto it = pot->find(s);
+        if (it == pot->end()) return false;
+        for (int i = 0; i < v.rows; ++i) v(i, i) = it->second;
+        return true;
+    });
+    return m;
+}
+
+// attach_leads (crates/caroli-green/src/multi.rs:112-140) and the device part of
+// NestedMultiTerminal::from_model (nested_multi.rs:60-64, :155-160).
+MultiSpec multi_from_models(const Model& device, const Model& lead, const Region& region,
+                            const std::vector<Region>& slices, const std::vector<Cell>& outward) {
+    if (slices.size() != outward.size()) throw std::invalid_argument("one outward step per interface");
+    MultiSpec m;
+    SystemSpec& s = m.base;
+    s.h = device.sparse_hamiltonian(region);
+    s.dim = region.lattice.dim();
+    s.axis = 0;
+    s.range = {1, 1, 1};
+    for (auto& hop : device.hoppings)
+        for (int d = 0; d < 3; ++d) s.range[d] = std::max<int64_t>(s.range[d], std::llabs(hop.offset[d]));
+    for (size_t i = 0; i < region.sites.size(); ++i) {
+        s.cells.push_back(region.sites[i].cell);
+        s.site_orbitals.push_back(region.site_orbitals(i));
+        s.site_offsets.push_back(region.offsets[i]);
+    }
+    for (size_t p = 0; p < slices.size(); ++p) {
+        const Region& slice = slices[p];
+        const Cell d = outward[p];
+        if (slice.sites.empty()) throw std::invalid_argument("empty interface slice");
+        if (d == Cell{0, 0, 0}) throw std::invalid_argument("interface without an outward step");
+        const Region first = slice.translated(d);
+        for (auto& site : first.sites)
+            if (region.find(site) >= 0) throw std::invalid_argument("the first lead cell overlaps the device");
+        const CMat skip = lead.coupling(slice, slice.translated(Cell{2 * d[0], 2 * d[1], 2 * d[2]}));
+        if (skip.norm_max() != 0.0) throw std::invalid_argument("a lead hopping reaches beyond the next cell");
+        AttachedLead l;
+        l.h00 = lead.hamiltonian(slice);
+        l.h01 = lead.coupling(slice, first);
+        const CMat v = lead.coupling(region, first);
+        for (int i = 0; i < v.rows; ++i)
+            for (int j = 0; j < v.cols; ++j)
+                if (v(i, j) != cplx(0.0)) {
+                    l.orbitals.push_back(size_t(i));
+                    break;
+                }
+        l.coupling = CMat(int(l.orbitals.size()), v.cols);
+        for (int j = 0; j < v.cols; ++j)
+            for (size_t a = 0; a < l.orbitals.size(); ++a) l.coupling(int(a), j) = v(int(l.orbitals[a]), j);
+        std::vector<int> kinds;
+        for (size_t i = 0; i < slice.sites.size(); ++i)
+            for (int o = 0; o < slice.site_orbitals(i); ++o) kinds.push_back(o);
+        m.leads.push_back(std::move(l));
+        m.orbital_kinds.push_back(std::move(kinds));
+    }
+    return m;
+}
+
 SystemSpec preset_system(const PresetArgs& a) {
-    Model lead = a.model == 0 ? bhz(BhzParams{}) : a.model == 1 ? square_model() : a.model == 2 ? graphene_model() : cubic_model();
-    if (a.model < 0 || a.model > 3) throw std::invalid_argument("unknown preset model");
+    Model lead = preset_model(a.model);
     Region region = Region::cuboid(lead.lattice, {a.extent[0], a.extent[1], a.extent[2]});
     Model device = lead;
     // crates/caroli-cli/src/build.rs:260-306: one draw stream over the region sites.
@@ -534,16 +601,7 @@ SystemSpec preset_system(const PresetArgs& a) {
             return true;
         });
     }
-    if (a.site_potential) {
-        auto pot = std::make_shared<std::map<Site, double>>();
-        for (size_t i = 0; i < region.sites.size(); ++i) (*pot)[region.sites[i]] = a.site_potential[i];
-        device.extra.push_back([pot](const Site& s, CMat& v) {
-            auto it = pot->find(s);
-            if (it == pot->end()) return false;
-            for (int i = 0; i < v.rows; ++i) v(i, i) = it->second;
-            return true;
-        });
-    }
+    if (a.site_potential) device = with_site_potential(device, region, a.site_potential);
     return system_from_models(device, lead, region, a.axis);
 }
 

@@ -145,6 +145,35 @@ struct SystemSpec {
 /// (crates/caroli-green/src/nested.rs:530-557, gauge.rs:81-98 without a field).
 SystemSpec system_from_models(const Model& device, const Model& lead, const Region& region, int axis);
 
+/// A semi-infinite lead attached to a device (crates/caroli-green/src/multi.rs:15-25): its cell,
+/// the coupling from a cell to the next one further from the device, the device orbitals coupled
+/// to its first cell (increasing) and that coupling (orbitals x cell orbitals).
+struct AttachedLead {
+    CMat h00, h01;
+    std::vector<size_t> orbitals;
+    CMat coupling;
+};
+
+/// A multi-terminal device (NestedMultiTerminal::from_model, nested_multi.rs:60-161): the device
+/// Hamiltonian and geometry in `base` (its two-terminal lead fields unused), the leads, and for
+/// each lead the index within its site of every orbital of its cell (sector labels).
+struct MultiSpec {
+    SystemSpec base;
+    std::vector<AttachedLead> leads;
+    std::vector<std::vector<int>> orbital_kinds;
+};
+
+/// The leads of `lead` continuing each interface slice outward (attach_leads, multi.rs:112-140,
+/// without a magnetic field): cell = the slice, h01 = its coupling to the slice moved one step
+/// outward, coupling = every device orbital the first lead cell reaches.
+MultiSpec multi_from_models(const Model& device, const Model& lead, const Region& region,
+                            const std::vector<Region>& slices, const std::vector<Cell>& outward);
+
+/// Preset model 0 BHZ, 1 square, 2 zigzag graphene, 3 cubic (the clean model of preset_system).
+Model preset_model(int model);
+/// `base` with an extra on-site potential on every orbital of each listed site.
+Model with_site_potential(const Model& base, const Region& region, const double* potential);
+
 /// Named workloads: the model, region and seeded disorder of a preset system.
 ///  model 0: BHZ (BhzParams::default());  1: square lattice, t = 1;
 ///        2: zigzag graphene, t = 1 (crates/caroli-bench/src/main.rs:86-91);  3: cubic, t = 1.

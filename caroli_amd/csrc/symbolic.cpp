@@ -277,4 +277,104 @@ Symbolic symbolic_phase(const SystemSpec& s) {
     return sym;
 }
 
+// nested_multi.rs:60-161. One difference from the reference, deliberate: an orbital of a site
+// that a lead couples to, but which no lead reaches itself, is kept at the root with its z - H
+// entries (the reference leaves it out of both the tree and the root); results agree whenever
+// every orbital of such sites is coupled, as in the BHZ, square, graphene and cubic leads.
+Symbolic multi_symbolic(const SystemSpec& s, const std::vector<AttachedLead>& leads) {
+    const size_t n = s.h.n, nsites = s.cells.size();
+    if (nsites == 0) throw std::invalid_argument("empty region");
+    if (leads.empty()) throw std::invalid_argument("no leads");
+    std::vector<char> coupled(n, 0);
+    for (auto& l : leads) {
+        if (l.h00.rows == 0 || l.h00.rows != l.h00.cols || l.h01.rows != l.h00.rows || l.h01.cols != l.h00.rows ||
+            l.coupling.rows != int(l.orbitals.size()) || l.coupling.cols != l.h00.rows || l.orbitals.empty())
+            throw std::invalid_argument("malformed lead");
+        for (size_t o : l.orbitals) {
+            if (o >= n) throw std::invalid_argument("lead orbital outside the device");
+            coupled[o] = 1;
+        }
+    }
+    std::vector<size_t> interior;
+    std::vector<char> at_root(n, 0);
+    for (size_t i = 0; i < nsites; ++i) {
+        bool any = false;
+        for (int o = 0; o < s.site_orbitals[i]; ++o) any |= coupled[s.site_offsets[i] + o] != 0;
+        if (!any) {
+            interior.push_back(i);
+        } else {
+            for (int o = 0; o < s.site_orbitals[i]; ++o) at_root[s.site_offsets[i] + o] = 1;
+        }
+    }
+    SiteTree st = site_tree(s, interior);
+
+    // Components over the device orbitals, then the orbitals of each lead cell (nested_multi.rs:79-97).
+    std::vector<size_t> offsets;
+    size_t total = n;
+    for (auto& l : leads) {
+        offsets.push_back(total);
+        total += size_t(l.h00.rows);
+    }
+    Components comp(total);
+    for (size_t r = 0; r < n; ++r)
+        for (int64_t e = s.h.row_ptr[r]; e < s.h.row_ptr[r + 1]; ++e)
+            if (s.h.vals[e] != cplx(0.0)) comp.join(r, s.h.cols[e]);
+    for (size_t p = 0; p < leads.size(); ++p) {
+        std::vector<size_t> cell(size_t(leads[p].h00.rows));
+        for (size_t k = 0; k < cell.size(); ++k) cell[k] = offsets[p] + k;
+        comp.join_matrix(leads[p].h00, cell, cell);
+        comp.join_matrix(leads[p].h01, cell, cell);
+        comp.join_matrix(leads[p].coupling, leads[p].orbitals, cell);
+    }
+    std::vector<size_t> roots(total);
+    for (size_t i = 0; i < total; ++i) roots[i] = comp.find(i);
+    std::vector<size_t> labels = roots;
+    std::sort(labels.begin(), labels.end());
+    labels.erase(std::unique(labels.begin(), labels.end()), labels.end());
+
+    Symbolic sym;
+    for (size_t label : labels) {
+        auto mine = [&](size_t o) { return roots[o] == label; };
+        Block b;
+        b.multi = true;
+        bool reached = false;
+        for (size_t p = 0; p < leads.size(); ++p) {
+            const AttachedLead& l = leads[p];
+            std::vector<size_t> kc, kd;
+            for (int k = 0; k < l.h00.rows; ++k)
+                if (mine(offsets[p] + size_t(k))) kc.push_back(size_t(k));
+            for (size_t k = 0; k < l.orbitals.size(); ++k)
+                if (mine(l.orbitals[k])) kd.push_back(k);
+            AttachedLead r;
+            if (!kc.empty() && !kd.empty()) {
+                reached = true;
+                r.h00 = restrict_to(l.h00, kc);
+                r.h01 = restrict_to(l.h01, kc);
+                for (size_t k : kd) r.orbitals.push_back(l.orbitals[k]);
+                r.coupling = CMat(int(kd.size()), int(kc.size()));
+                for (size_t y = 0; y < kc.size(); ++y)
+                    for (size_t x = 0; x < kd.size(); ++x) r.coupling(int(x), int(y)) = l.coupling(int(kd[x]), int(kc[y]));
+            } else {
+                kc.clear();
+            }
+            b.leads.push_back(std::move(r));
+            b.lead_cells.push_back(std::move(kc));
+        }
+        if (!reached) continue;
+        std::vector<char> in_block(n);
+        for (size_t o = 0; o < n; ++o) {
+            in_block[o] = mine(o);
+            if (in_block[o] && at_root[o]) b.kept.push_back(o);
+        }
+        b.left_orbitals = b.kept;
+        b.right_orbitals = b.kept;
+        b.right = 0;
+        b.same_lead = true;
+        b.tree = build_tree(st, s, in_block);
+        sym.blocks.push_back(std::move(b));
+    }
+    if (sym.blocks.empty()) throw std::invalid_argument("no lead reaches the device");
+    return sym;
+}
+
 }  // namespace caroli

@@ -79,6 +79,12 @@ struct Block {
     std::vector<size_t> kept;                           // left, then right unless the same
     int right = 0;                                      // position of the right slice in kept
     bool same_lead = false;                             // right lead equal to the left one
+    /// Multi-terminal blocks (nested_multi.rs:25-34): each lead restricted to the block (no
+    /// orbitals where it does not reach it) and its cell orbitals in the block. Such a block has
+    /// left = right = kept, so the root's border block holds -G on all kept orbitals.
+    bool multi = false;
+    std::vector<AttachedLead> leads;
+    std::vector<std::vector<size_t>> lead_cells;
 };
 
 struct Symbolic {
@@ -86,5 +92,9 @@ struct Symbolic {
 };
 
 Symbolic symbolic_phase(const SystemSpec& s);
+/// NestedMultiTerminal::from_model (nested_multi.rs:60-161): the tree over the sites that no lead
+/// couples to; every orbital of the other sites stays at the root; the independent blocks of
+/// orbitals (device, lead cells and couplings joined) that some lead reaches.
+Symbolic multi_symbolic(const SystemSpec& s, const std::vector<AttachedLead>& leads);
 
 }  // namespace caroli

@@ -36,6 +36,13 @@ struct cg_system {
     mutable std::shared_ptr<caroli::gpu::Structure> structure;
     /// Device copy of the Hamiltonian values (complex f64), uploaded on first use.
     mutable void* d_vals = nullptr;
+    /// Multi-terminal device (NestedMultiTerminal): its leads as attached, the orbital kind of each
+    /// cell orbital (sector labels), and per block and lead the mode solver, built on the first
+    /// self-energy request (leads of a block with equal cells share one, nested_multi.rs:130-145).
+    bool multi = false;
+    std::vector<caroli::AttachedLead> multi_leads;
+    std::vector<std::vector<int>> orbital_kinds;
+    mutable std::vector<std::vector<std::shared_ptr<caroli::ModeSolver>>> lead_modes;
 };
 
 struct cg_sigma {
@@ -48,6 +55,22 @@ struct cg_sigma {
     };
     double energy = 0;
     std::vector<Block> blocks;
+    /// Multi-terminal devices: T is a matrix over labels, one per lead or per (lead, sector).
+    /// Each block holds Sigma of all its leads on its kept orbitals in sl (sr is zero, both
+    /// kept x kept), and in wab the channels of every label (W W^H = Gamma of that label on the
+    /// kept orbitals), label l in columns lab_off[l] .. lab_off[l + 1].
+    struct Label {
+        int32_t lead, sector;
+    };
+    bool multi = false;
+    std::vector<Label> labels;
+    struct MultiBlock {
+        caroli::CMat wab;
+        std::vector<int32_t> lab_off;
+        std::vector<caroli::AttachedLead> leads;  // the block's leads they were computed for
+        std::vector<caroli::CMat> sigma;          // Sigma_p on each lead's orbitals in the block
+    };
+    std::vector<MultiBlock> mblocks;
 };
 
 namespace caroli {

@@ -304,6 +304,89 @@ int32_t cg_operator_currents(const cg_system* s, const double* psi_left, const d
 /* Device present and usable (1) or not (0). */
 int32_t cg_device_available(void);
 
+/* ---- NestedMultiTerminal (crates/caroli-green/src/nested_multi.rs) ----
+ * A device with any number of leads. The sites a lead couples to stay at the root of the
+ * elimination tree, whose bordered root front leaves -G on all of their orbitals; T[p][q] =
+ * Tr[Gamma_p G_pq Gamma_q G_pq^H] from lead q into lead p. A multi-terminal system is a cg_system
+ * handle (cg_block_count, cg_system_sizes, cg_system_arrays, cg_work, cg_tree_node and
+ * cg_system_destroy apply); the two-terminal calls refuse it and its self-energies
+ * (CG_ERR_INVALID). Precision: CG_DOUBLE or CG_SINGLE (the fronts in complex f32, the root in f64).
+ *
+ *   NestedMultiTerminal::from_model         nested_multi.rs:60   -> cg_multi_create / cg_multi_model_create
+ *   NestedMultiTerminal::lead_count         nested_multi.rs:164  -> cg_multi_lead_count
+ *   NestedMultiTerminal::transmissions      nested_multi.rs:174  -> cg_multi_transmissions
+ *   ::transmissions_with_precision          nested_multi.rs:180  -> cg_multi_batch_transmissions
+ *   ::sector_transmissions                  nested_multi.rs:246  -> cg_multi_self_energies (sectors)
+ *                                                                   + cg_multi_batch_transmissions */
+
+/* One lead as attached (AttachedLead, crates/caroli-green/src/multi.rs:15-25): a cell of `cell`
+ * orbitals with h00 and h01 (the coupling from a cell to the next one further from the device),
+ * the `coupled` device orbitals its first cell reaches (increasing) and that coupling (coupled x
+ * cell). kinds: optional, the index within its site of each cell orbital (sector labels; NULL:
+ * orbital k is kind k). */
+typedef struct {
+  uint64_t cell;
+  const double* h00;
+  const double* h01;
+  uint64_t coupled;
+  const uint64_t* orbitals;
+  const double* coupling;
+  const int32_t* kinds;
+} cg_lead;
+
+/* NestedMultiTerminal::from_model with the device and its leads built by the caller: the device
+ * arrays as for cg_system_create (no axis), then the leads. */
+int32_t cg_multi_create(uint64_t n, const int64_t* row_ptr, const uint32_t* cols, const double* vals,
+                        uint64_t nsites, const int64_t* cells, const int32_t* site_orbitals,
+                        const uint64_t* site_offsets, int32_t dim, const int64_t* range, int32_t leads,
+                        const cg_lead* lead, cg_system** out);
+
+/* One interface (Interface, multi.rs:66-72): the device sites of the slice (cells, 3 per site, and
+ * sublattices) and the lattice step pointing away from the device. */
+typedef struct {
+  uint64_t sites;
+  const int64_t* cells;
+  const int32_t* subs;
+  int64_t outward[3];
+} cg_interface;
+
+/* NestedMultiTerminal::from_model(device, lead, region, interfaces) with a preset model (0 BHZ,
+ * 1 square, 2 zigzag graphene, 3 cubic) as the clean lead: region = the listed sites in that order;
+ * the device adds site_potential[i] (optional) on every orbital of site i; the leads continue each
+ * interface outward (attach_leads, multi.rs:112-140). */
+int32_t cg_multi_model_create(int32_t model, uint64_t sites, const int64_t* cells, const int32_t* subs,
+                              const double* site_potential, int32_t interfaces, const cg_interface* itf,
+                              cg_system** out);
+
+/* Number of leads of a multi-terminal system (-1 for another handle). */
+int32_t cg_multi_lead_count(const cg_system* s);
+/* Lead p as attached: its cell size and coupled orbitals, and (each optional) h00, h01 (cell x cell),
+ * the orbitals and the coupling (coupled x cell), complex interleaved, column-major. */
+int32_t cg_multi_lead(const cg_system* s, int32_t p, uint64_t* cell, uint64_t* coupled, double* h00, double* h01,
+                      uint64_t* orbitals, double* coupling);
+
+/* Self-energies and broadenings of every lead at E + i eta (eta = 0: mode matching on the real
+ * axis; nested_multi.rs:196-232, SelfEnergyMethod::ModesBroadened otherwise). The rows and columns
+ * of T are labels (lead, sector): without sectors (sector_len NULL) one label per lead; else lead p
+ * with sector_len[p] > 0 is split by the tag sector_tags[...][kind] of each cell orbital (tags of
+ * all leads concatenated; sector_transmissions, nested_multi.rs:246-313), and a lead Hamiltonian that
+ * couples two sectors is refused. */
+int32_t cg_multi_self_energies(const cg_system* s, double energy, double eta, const int32_t* sector_len,
+                               const int32_t* sector_tags, cg_sigma** out);
+/* Sigma_p = V g V^H of lead p on its coupled orbitals (coupled x coupled, complex interleaved,
+ * column-major; zero between orbitals of different blocks). */
+int32_t cg_multi_sigma(const cg_system* s, const cg_sigma* g, int32_t p, double* out);
+/* The labels of T: *count, then (each optional) the lead and sector of each, in increasing order. */
+int32_t cg_multi_labels(const cg_sigma* g, int32_t* count, int32_t* lead, int32_t* sector);
+/* transmissions_with_precision for `count` points (systems[p], sigmas[p], every sigma with the same
+ * number of labels L) in one batched launch: out receives count x L x L values, point by point,
+ * row-major: out[p][a][b] = T from label b into label a (the diagonal is not a reflection,
+ * nested_multi.rs:246-252). */
+int32_t cg_multi_batch_transmissions(const cg_system* const* systems, const cg_sigma* const* sigmas, int64_t count,
+                                     int32_t precision, double* out, int32_t* out_status, cg_batch_stats* stats);
+/* NestedMultiTerminal::transmissions at one energy: out is lead_count x lead_count, row-major. */
+int32_t cg_multi_transmissions(const cg_system* s, double energy, int32_t precision, double* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -98,3 +98,77 @@ def oracle_gradient(s, energy):
         m = gr @ g_lr.conj().T @ gl                               # Gamma_R G_LR^H Gamma_L
         grad += 2 * np.real(np.einsum("ac,cd,ad->a", g_r, m, gt_l))
     return grad
+
+
+# -- multi-terminal devices ---------------------------------------------------------------------
+
+SUBS = {ca.Model.BHZ: 1, ca.Model.Square: 1, ca.Model.Graphene: 2, ca.Model.Cubic: 1}
+ORBITALS = {ca.Model.BHZ: 4, ca.Model.Square: 1, ca.Model.Graphene: 1, ca.Model.Cubic: 1}
+
+
+def cross(model, nx, ny, arm):
+    """An nx x ny block with arms of length `arm` on its four edges, as wide as the edges, and a lead
+    at the end of each arm (the geometry of crates/caroli-green/tests/nested_multi.rs:45-62):
+    (sites as (x, y, z, sublattice), interfaces as (slice sites, outward step))."""
+    ns = SUBS[model]
+    cells = [(x, y) for x in range(-arm, nx + arm) for y in range(-arm, ny + arm) if 0 <= y < ny or 0 <= x < nx]
+    sites = [(x, y, 0, s) for (x, y) in cells for s in range(ns)]
+    column = lambda x: [(x, y, 0, s) for y in range(ny) for s in range(ns)]
+    row = lambda y: [(x, y, 0, s) for x in range(nx) for s in range(ns)]
+    interfaces = [(column(-arm), (-1, 0, 0)), (row(ny + arm - 1), (0, 1, 0)), (column(nx + arm - 1), (1, 0, 0)),
+                  (row(-arm), (0, -1, 0))]
+    return sites, interfaces
+
+
+def multi_system(model, sites, interfaces, w=0.0):
+    """NestedMultiTerminal of a preset model on `sites` with the hashed disorder of strength w."""
+    pot = hash_disorder([s[:3] for s in sites], [s[3] for s in sites], w) if w else None
+    return ca.NestedMultiTerminal.from_model(model, sites, interfaces, site_potential=pot)
+
+
+def dense_hamiltonian(arr):
+    n = arr["n"]
+    h = np.zeros((n, n), np.complex128)
+    rp, cl, vv = arr["row_ptr"], arr["cols"], arr["vals"]
+    for r in range(n):
+        h[r, cl[rp[r]:rp[r + 1]]] = vv[rp[r]:rp[r + 1]]
+    return h
+
+
+def dense_multi(sys_, sigma, energy):
+    """T[p][q] = Tr[Gamma_p G_pq Gamma_q G_pq^H] from the whole device's Green's function
+    G = (E - H - sum_p Sigma_p)^-1 by a dense inverse (MultiTerminal::solve, multi.rs:89-110), with
+    the self-energies of `sigma`."""
+    arr = sys_.arrays()
+    n = arr["n"]
+    a = energy * np.eye(n) - dense_hamiltonian(arr)
+    leads = [sys_.lead(p)["orbitals"] for p in range(sys_.lead_count())]
+    sig = [sigma.sigma(p) for p in range(len(leads))]
+    for o, s in zip(leads, sig):
+        a[np.ix_(o, o)] -= s
+    g = np.linalg.inv(a)
+    gam = [1j * (s - s.conj().T) for s in sig]
+    P = len(leads)
+    t = np.zeros((P, P))
+    for p in range(P):
+        for q in range(P):
+            gpq = g[np.ix_(leads[p], leads[q])]
+            t[p, q] = np.trace(gam[p] @ gpq @ gam[q] @ gpq.conj().T).real
+    return t
+
+
+def decimation_surface(h00, h01, z, tol=1e-15, iters=400):
+    """Surface Green's function of semi-infinite cells h00 coupled outward by h01 at complex z, by
+    the decimation of Lopez Sancho, Lopez Sancho and Rubio (J. Phys. F 15, 851 (1985))."""
+    n = h00.shape[0]
+    eye = np.eye(n)
+    es, e, a, b = h00.copy(), h00.copy(), h01.copy(), h01.conj().T.copy()
+    for _ in range(iters):
+        g = np.linalg.inv(z * eye - e)
+        agb, bga = a @ g @ b, b @ g @ a
+        es = es + agb
+        e = e + agb + bga
+        a, b = a @ g @ a, b @ g @ b
+        if max(np.abs(a).max(), np.abs(b).max()) < tol:
+            break
+    return np.linalg.inv(z * eye - es)
