@@ -550,6 +550,24 @@ class SectorTransmissions:
     t: np.ndarray
 
 
+@dataclass
+class SMatrix:
+    """smatrix.rs:22-34: blocks[p][q] the amplitudes into the outgoing modes of lead p from the
+    incoming modes of lead q (unit current), and the Bloch factors of the modes."""
+
+    energy: float
+    blocks: list
+    incoming: list
+    outgoing: list
+
+    def transmission(self, p, q) -> float:
+        """sum |S_pq|^2 (a reflection for p = q)."""
+        return float(np.sum(np.abs(self.blocks[p][q]) ** 2))
+
+    def full(self):
+        return np.block(self.blocks) if self.blocks else np.zeros((0, 0), complex)
+
+
 class NestedMultiTerminal:
     """A device with any number of leads solved by nested dissection (nested_multi.rs:45-56): the
     sites the leads couple to stay at the root, whose border block gives G on all of their
@@ -704,6 +722,75 @@ class NestedMultiTerminal:
         if st[0] != 0:
             raise GreenError(int(st[0]), f"E = {sigma.energy()}")
         return t[0]
+
+    def smatrix(self, energy) -> "SMatrix":
+        """nested_multi.rs:440-470: the scattering matrix in the basis of the propagating lead modes
+        normalised to unit current (cg_multi_smatrix)."""
+        P = self.lead_count()
+        modes = np.zeros(2 * P, np.int32)
+        mp = modes.ctypes.data_as(C.POINTER(C.c_int32))
+        _check(capi.cg_multi_smatrix(self._h, float(energy), mp, None, None, None), f"E = {energy}")
+        nout, nin = modes[:P].copy(), modes[P:].copy()
+        full = np.zeros((int(nin.sum()), int(nout.sum())), complex)  # column-major = transposed C order
+        lo = np.zeros(int(nout.sum()), complex)
+        li = np.zeros(int(nin.sum()), complex)
+        _check(capi.cg_multi_smatrix(self._h, float(energy), mp, _dptr(full.view(np.float64)),
+                                     _dptr(lo.view(np.float64)), _dptr(li.view(np.float64))), f"E = {energy}")
+        full = full.T
+        ro = np.concatenate([[0], np.cumsum(nout)])
+        co = np.concatenate([[0], np.cumsum(nin)])
+        blocks = [[full[ro[p]:ro[p + 1], co[q]:co[q + 1]] for q in range(P)] for p in range(P)]
+        return SMatrix(float(energy), blocks, [li[co[q]:co[q + 1]] for q in range(P)],
+                       [lo[ro[p]:ro[p + 1]] for p in range(P)])
+
+    def scattering_states(self, sigma: MultiSelfEnergies) -> ScatteringStates:
+        """nested_multi.rs:354-413: psi = G W on every orbital for the channels of each label of
+        `sigma` (each lead, or each sector of a lead): ScatteringStates with one n x channels array
+        per label."""
+        n = self.sizes()[0]
+        L = len(sigma.labels())
+        cols = (C.c_int64 * L)()
+        _check(capi.cg_multi_scattering_states(self._h, sigma._h, cols, None))
+        bufs = [np.zeros((int(cols[l]), n), complex) for l in range(L)]
+        ptrs = (C.POINTER(C.c_double) * L)(*[_dptr(b.view(np.float64)) for b in bufs])
+        _check(capi.cg_multi_scattering_states(self._h, sigma._h, cols, ptrs), "scattering states")
+        return ScatteringStates(sigma.energy(), [b.T for b in bufs])
+
+    @staticmethod
+    def _states(states):
+        bufs = [np.ascontiguousarray(np.asarray(m, complex).T).view(np.float64) for m in states.leads]
+        L = len(bufs)
+        ptrs = (C.POINTER(C.c_double) * L)(*[_dptr(b) for b in bufs])
+        cols = (C.c_int64 * L)(*[m.shape[1] for m in states.leads])
+        return bufs, ptrs, cols, L
+
+    def site_densities(self, states, f, op=None):
+        """sum_p f_p psi_p^H O psi_p / 2 pi per site (nested_multi.rs:510-513), O = 1 without `op`."""
+        bufs, ptrs, cols, L = self._states(states)
+        fa = np.asarray(f, float)
+        out = np.zeros(self.sizes()[2])
+        o = None if op is None else np.ascontiguousarray(np.asarray(op, complex).T).view(np.float64)
+        _check(capi.cg_multi_site_densities(self._h, L, ptrs, cols, _dptr(fa), None if o is None else _dptr(o),
+                                            0 if op is None else np.asarray(op).shape[0], _dptr(out)), "site densities")
+        return out
+
+    def bond_currents(self, states, f, op=None):
+        """Bond currents 2 Re Tr[H_ij G^<_ji] (nested_multi.rs:497-500), or those of an on-site
+        operator O (nested_multi.rs:503-505): arrays (from, to, current) ordered by (from, to)."""
+        bufs, ptrs, cols, L = self._states(states)
+        fa = np.asarray(f, float)
+        o = None if op is None else np.ascontiguousarray(np.asarray(op, complex).T).view(np.float64)
+        od = 0 if op is None else np.asarray(op).shape[0]
+        count = C.c_int64()
+        _check(capi.cg_multi_bond_currents(self._h, L, ptrs, cols, _dptr(fa), None if o is None else _dptr(o), od,
+                                           C.byref(count), None, None, None), "bond currents")
+        fr = np.zeros(count.value, np.int64)
+        to = np.zeros(count.value, np.int64)
+        cur = np.zeros(count.value)
+        _check(capi.cg_multi_bond_currents(self._h, L, ptrs, cols, _dptr(fa), None if o is None else _dptr(o), od,
+                                           C.byref(count), fr.ctypes.data_as(C.POINTER(C.c_int64)),
+                                           to.ctypes.data_as(C.POINTER(C.c_int64)), _dptr(cur)), "bond currents")
+        return fr, to, cur
 
     def sector_transmissions(self, energy, sectors, precision=Precision.Double) -> SectorTransmissions:
         """nested_multi.rs:246-313: `sectors[p]` tags the orbitals of each site of lead p (e.g.

@@ -206,3 +206,13 @@ cg_multi_batch_transmissions = _sig("cg_multi_batch_transmissions", C.c_int32, P
 cg_multi_transmissions = _sig("cg_multi_transmissions", C.c_int32, c_sys, C.c_double, C.c_int32, P(C.c_double))
 EXPORTS += ["cg_multi_create", "cg_multi_model_create", "cg_multi_lead_count", "cg_multi_lead",
             "cg_multi_self_energies", "cg_multi_sigma", "cg_multi_labels", "cg_multi_batch_transmissions", "cg_multi_transmissions"]
+cg_multi_smatrix = _sig("cg_multi_smatrix", C.c_int32, c_sys, C.c_double, P(C.c_int32), P(C.c_double), P(C.c_double),
+                        P(C.c_double))
+cg_multi_scattering_states = _sig("cg_multi_scattering_states", C.c_int32, c_sys, c_sig, P(C.c_int64),
+                                  P(P(C.c_double)))
+cg_multi_site_densities = _sig("cg_multi_site_densities", C.c_int32, c_sys, C.c_int32, P(P(C.c_double)),
+                               P(C.c_int64), P(C.c_double), P(C.c_double), C.c_int32, P(C.c_double))
+cg_multi_bond_currents = _sig("cg_multi_bond_currents", C.c_int32, c_sys, C.c_int32, P(P(C.c_double)), P(C.c_int64),
+                              P(C.c_double), P(C.c_double), C.c_int32, P(C.c_int64), P(C.c_int64), P(C.c_int64),
+                              P(C.c_double))
+EXPORTS += ["cg_multi_smatrix", "cg_multi_scattering_states", "cg_multi_site_densities", "cg_multi_bond_currents"]

@@ -807,17 +807,17 @@ int32_t cg_scattering_states(const cg_system* s, const cg_sigma* g, int64_t* col
 namespace {
 /// (psi_p)_{a k}: states of lead p, n orbitals x cols[p] channels, interleaved complex.
 struct States {
-    const double* m[2];
+    std::vector<const double*> m;  // one per lead (or label)
     const int64_t* cols;
     int64_t n;
-    cplx at(int p, int64_t a, int64_t k) const {
+    cplx at(size_t p, int64_t a, int64_t k) const {
         const double* v = m[p] + 2 * (a + k * n);
         return cplx(v[0], v[1]);
     }
     /// G^<_{ba} = i sum_p f_p sum_k psi_bk conj(psi_ak) (nested.rs:936-943).
     cplx lesser(int64_t b, int64_t a, const double* f) const {
         cplx acc = 0.0;
-        for (int p = 0; p < 2; ++p) {
+        for (size_t p = 0; p < m.size(); ++p) {
             cplx sp = 0.0;
             for (int64_t k = 0; k < cols[p]; ++k) sp += at(p, b, k) * std::conj(at(p, a, k));
             acc += sp * f[p];
@@ -827,19 +827,26 @@ struct States {
 };
 }  // namespace
 
-int32_t cg_site_densities(const cg_system* s, const double* psi_left, const double* psi_right, const int64_t* cols,
-                          const double* f, const double* op, int32_t op_dim, double* out) {
-    if (!s || !psi_left || !psi_right || !cols || !f || !out || cols[0] < 0 || cols[1] < 0) return CG_ERR_INVALID;
+namespace {
+bool states_ok(int32_t leads, const double* const* psi, const int64_t* cols) {
+    if (leads < 1 || !psi || !cols) return false;
+    for (int32_t p = 0; p < leads; ++p)
+        if (cols[p] < 0 || (cols[p] > 0 && !psi[p])) return false;
+    return true;
+}
+
+int32_t site_densities(const cg_system* s, const States& st, const double* f, const double* op, int32_t op_dim,
+                       double* out) {
     const auto& sp = s->spec;
     const size_t sites = sp.site_orbitals.size();
     if (op)
         for (size_t i = 0; i < sites; ++i)
             if (sp.site_orbitals[i] != op_dim) return CG_ERR_INVALID;  // the operator acts on a site's orbitals
-    States st{{psi_left, psi_right}, cols, int64_t(sp.h.n)};
+    const int64_t* cols = st.cols;
     for (size_t i = 0; i < sites; ++i) {
         const int64_t o0 = int64_t(sp.site_offsets[i]), ni = sp.site_orbitals[i];
         cplx acc = 0.0;
-        for (int p = 0; p < 2; ++p)
+        for (size_t p = 0; p < st.m.size(); ++p)
             for (int64_t k = 0; k < cols[p]; ++k)
                 for (int64_t a = 0; a < ni; ++a) {
                     const cplx left = std::conj(st.at(p, o0 + a, k));
@@ -857,9 +864,8 @@ int32_t cg_site_densities(const cg_system* s, const double* psi_left, const doub
     return CG_OK;
 }
 
-int32_t cg_bond_currents(const cg_system* s, const double* psi_left, const double* psi_right, const int64_t* cols,
-                         const double* f, int64_t* count, int64_t* from, int64_t* to, double* current) {
-    if (!s || !psi_left || !psi_right || !cols || !f || !count || cols[0] < 0 || cols[1] < 0) return CG_ERR_INVALID;
+int32_t bond_currents(const cg_system* s, const States& st, const double* f, int64_t* count, int64_t* from,
+                      int64_t* to, double* current) {
     const auto& sp = s->spec;
     const auto& h = sp.h;
     std::vector<int64_t> owner(h.n, 0);
@@ -867,7 +873,6 @@ int32_t cg_bond_currents(const cg_system* s, const double* psi_left, const doubl
         for (int o = 0; o < sp.site_orbitals[i]; ++o) owner[sp.site_offsets[i] + size_t(o)] = int64_t(i);
     // Each bond (i < j) once, ordered by (i, j): 2 sum_{a in i, b in j} H_ab G^<_ba (nested.rs:949-956).
     std::map<std::pair<int64_t, int64_t>, cplx> bonds;
-    States st{{psi_left, psi_right}, cols, int64_t(h.n)};
     const bool fill = from && to && current;
     for (size_t a = 0; a < h.n; ++a)
         for (int64_t e = h.row_ptr[a]; e < h.row_ptr[a + 1]; ++e) {
@@ -894,11 +899,8 @@ int32_t cg_bond_currents(const cg_system* s, const double* psi_left, const doubl
     return CG_OK;
 }
 
-int32_t cg_operator_currents(const cg_system* s, const double* psi_left, const double* psi_right, const int64_t* cols,
-                             const double* f, const double* op, int32_t op_dim, int64_t* count, int64_t* from,
-                             int64_t* to, double* current) {
-    if (!s || !psi_left || !psi_right || !cols || !f || !op || op_dim <= 0 || !count || cols[0] < 0 || cols[1] < 0)
-        return CG_ERR_INVALID;
+int32_t operator_currents(const cg_system* s, const States& st, const double* f, const double* op, int32_t op_dim,
+                          int64_t* count, int64_t* from, int64_t* to, double* current) {
     const auto& sp = s->spec;
     const auto& h = sp.h;
     std::vector<int64_t> owner(h.n, 0);
@@ -924,7 +926,6 @@ int32_t cg_operator_currents(const cg_system* s, const double* psi_left, const d
     }
     if (*count < int64_t(blocks.size())) return CG_ERR_INVALID;
     auto O = [&](int a, int b) { return cplx(op[2 * (a + b * m)], op[2 * (a + b * m) + 1]); };
-    States st{{psi_left, psi_right}, cols, int64_t(h.n)};
     int64_t q = 0;
     for (const auto& [ij, hij] : blocks) {
         const int64_t oi = int64_t(sp.site_offsets[size_t(ij.first)]), oj = int64_t(sp.site_offsets[size_t(ij.second)]);
@@ -943,6 +944,46 @@ int32_t cg_operator_currents(const cg_system* s, const double* psi_left, const d
     }
     *count = q;
     return CG_OK;
+}
+}  // namespace
+
+int32_t cg_site_densities(const cg_system* s, const double* psi_left, const double* psi_right, const int64_t* cols,
+                          const double* f, const double* op, int32_t op_dim, double* out) {
+    const double* psi[2] = {psi_left, psi_right};
+    if (!s || !psi_left || !psi_right || !states_ok(2, psi, cols) || !f || !out) return CG_ERR_INVALID;
+    return site_densities(s, States{{psi_left, psi_right}, cols, int64_t(s->spec.h.n)}, f, op, op_dim, out);
+}
+
+int32_t cg_bond_currents(const cg_system* s, const double* psi_left, const double* psi_right, const int64_t* cols,
+                         const double* f, int64_t* count, int64_t* from, int64_t* to, double* current) {
+    const double* psi[2] = {psi_left, psi_right};
+    if (!s || !psi_left || !psi_right || !states_ok(2, psi, cols) || !f || !count) return CG_ERR_INVALID;
+    return bond_currents(s, States{{psi_left, psi_right}, cols, int64_t(s->spec.h.n)}, f, count, from, to, current);
+}
+
+int32_t cg_operator_currents(const cg_system* s, const double* psi_left, const double* psi_right, const int64_t* cols,
+                             const double* f, const double* op, int32_t op_dim, int64_t* count, int64_t* from,
+                             int64_t* to, double* current) {
+    const double* psi[2] = {psi_left, psi_right};
+    if (!s || !psi_left || !psi_right || !states_ok(2, psi, cols) || !f || !op || op_dim <= 0 || !count)
+        return CG_ERR_INVALID;
+    return operator_currents(s, States{{psi_left, psi_right}, cols, int64_t(s->spec.h.n)}, f, op, op_dim, count, from,
+                             to, current);
+}
+
+int32_t cg_multi_site_densities(const cg_system* s, int32_t leads, const double* const* psi, const int64_t* cols,
+                                const double* f, const double* op, int32_t op_dim, double* out) {
+    if (!s || !states_ok(leads, psi, cols) || !f || !out) return CG_ERR_INVALID;
+    return site_densities(s, States{{psi, psi + leads}, cols, int64_t(s->spec.h.n)}, f, op, op_dim, out);
+}
+
+int32_t cg_multi_bond_currents(const cg_system* s, int32_t leads, const double* const* psi, const int64_t* cols,
+                               const double* f, const double* op, int32_t op_dim, int64_t* count, int64_t* from,
+                               int64_t* to, double* current) {
+    if (!s || !states_ok(leads, psi, cols) || !f || !count || (op && op_dim <= 0)) return CG_ERR_INVALID;
+    const States st{{psi, psi + leads}, cols, int64_t(s->spec.h.n)};
+    return op ? operator_currents(s, st, f, op, op_dim, count, from, to, current)
+              : bond_currents(s, st, f, count, from, to, current);
 }
 
 int32_t cg_device_available(void) { return gpu::device_available() ? 1 : 0; }
@@ -1231,6 +1272,233 @@ int32_t cg_multi_batch_transmissions(const cg_system* const* systems, const cg_s
         if (rc != CG_OK) return rc;
         std::unique_ptr<cg_batch, void (*)(cg_batch*)> hold(bt, gpu::destroy_batch);
         return gpu::run_batch_labels(bt, out, out_status, stats);
+    } catch (const std::exception&) {
+        return CG_ERR_DEVICE;
+    }
+}
+
+int32_t cg_multi_scattering_states(const cg_system* s, const cg_sigma* g, int64_t* cols, double* const* psi) {
+    if (!s || !g || !cols || !s->multi || !sigma_matches(*s, *g)) return CG_ERR_INVALID;
+    const size_t L = g->labels.size();
+    for (size_t l = 0; l < L; ++l) {
+        cols[l] = 0;
+        for (auto& mb : g->mblocks) cols[l] += mb.lab_off[l + 1] - mb.lab_off[l];
+    }
+    if (!psi) return CG_OK;
+    const int64_t n = int64_t(s->spec.h.n);
+    for (size_t l = 0; l < L; ++l)
+        if (cols[l] > 0) {
+            if (!psi[l]) return CG_ERR_INVALID;
+            std::fill(psi[l], psi[l] + 2 * n * cols[l], 0.0);
+        }
+    try {
+        // nested_multi.rs:354-413: per block the columns of every label, the blocks' columns of a
+        // label one after another.
+        std::vector<int64_t> c0(L, 0);
+        for (size_t b = 0; b < g->mblocks.size(); ++b) {
+            const auto& off = g->mblocks[b].lab_off;
+            std::vector<cplx> x;
+            const int32_t rc = gpu::multi_block_solve(s, g, int(b), nullptr, &x);
+            if (rc != CG_OK) return rc;
+            for (size_t l = 0; l < L; ++l) {
+                const int64_t w = off[l + 1] - off[l];
+                if (w > 0)
+                    std::memcpy(psi[l] + 2 * n * c0[l], x.data() + size_t(n) * size_t(off[l]),
+                                size_t(n) * size_t(w) * sizeof(cplx));
+                c0[l] += w;
+            }
+        }
+        return CG_OK;
+    } catch (const std::exception&) {
+        return CG_ERR_DEVICE;
+    }
+}
+
+}  // extern "C"
+
+namespace {
+// The scattering matrix in the basis of the propagating lead modes (crates/caroli-green/src/
+// smatrix.rs, NestedMultiTerminal::smatrix, nested_multi.rs:440-470): per block, G on the kept
+// orbitals from the GPU root, and for every lead q and incoming mode phi (Bloch factor lambda) the
+// source V_q (phi lambda - g H1^H phi) on the device; its wave G V_q s reaches the first cell of
+// lead p as psi_1 = g V_p^H G_pq V_q s (minus g H1^H phi for p = q), whose amplitude on the
+// outgoing mode (lambda') is i conj(lambda') [phi'^H (H1 F) psi_1 - conj(lambda') phi'^H H1^H psi_1].
+struct SBlock {
+    std::vector<std::vector<cplx>> in, out;  // per lead: Bloch factors
+    std::vector<std::vector<CMat>> s;        // [p][q]
+};
+
+int32_t multi_smatrix(const cg_system* s, double energy, bool counts_only, std::vector<std::vector<cplx>>& in,
+                      std::vector<std::vector<cplx>>& out, std::vector<std::vector<CMat>>& full) {
+    cg_sigma* graw = nullptr;
+    const size_t P = s->multi_leads.size();
+    int32_t rc = multi_sigmas(s, energy, 0.0, std::vector<std::vector<int32_t>>(P), &graw);
+    if (rc != CG_OK) return rc;
+    std::unique_ptr<cg_sigma> g(graw);
+    const cplx z(energy, 0.0), iu(0.0, 1.0);
+    std::vector<SBlock> parts;
+    for (size_t b = 0; b < s->sym.blocks.size(); ++b) {
+        const Block& blk = s->sym.blocks[b];
+        struct Solution {
+            LeadModes modes;
+            CMat rest, g;
+        };
+        std::map<const ModeSolver*, std::shared_ptr<Solution>> solved;
+        std::vector<std::shared_ptr<Solution>> sol(P);
+        for (size_t p = 0; p < P; ++p) {
+            const AttachedLead& l = blk.leads[p];
+            if (l.orbitals.empty()) continue;
+            const ModeSolver* solver = s->lead_modes[b][p].get();
+            auto it = solved.find(solver);
+            if (it == solved.end()) {
+                auto so = std::make_shared<Solution>();
+                if (Status st = solver->modes_at(energy, so->modes); st != kOk) return st;
+                so->rest = so->modes.sigma_right();
+                const int nc = l.h00.rows;
+                CMat a(nc, nc);
+                for (int j = 0; j < nc; ++j)
+                    for (int i = 0; i < nc; ++i) a(i, j) = (i == j ? z : cplx(0.0)) - l.h00(i, j) - so->rest(i, j);
+                so->g = dense::inverse(a);
+                it = solved.emplace(solver, so).first;
+            }
+            sol[p] = it->second;
+        }
+        SBlock part;
+        part.in.resize(P);
+        part.out.resize(P);
+        part.s.assign(P, std::vector<CMat>(P));
+        for (size_t p = 0; p < P; ++p)
+            if (sol[p]) {
+                part.in[p] = sol[p]->modes.left_lambdas;
+                part.out[p] = sol[p]->modes.right_lambdas;
+            }
+        if (counts_only) {  // the mode counts need the leads only
+            parts.push_back(std::move(part));
+            continue;
+        }
+        CMat green;
+        rc = gpu::multi_block_solve(s, g.get(), int(b), &green, nullptr);
+        if (rc != CG_OK) return rc;
+        if (!std::isfinite(green.norm_max())) return CG_ERR_SINGULAR;
+        Positions kept(blk.kept);
+        std::vector<std::vector<int>> at(P);
+        for (size_t p = 0; p < P; ++p)
+            for (size_t o : blk.leads[p].orbitals) at[p].push_back(kept.get(o));
+        for (size_t q = 0; q < P; ++q) {
+            for (size_t p = 0; p < P; ++p) part.s[p][q] = CMat(int(part.out[p].size()), int(part.in[q].size()));
+            if (!sol[q]) continue;
+            const AttachedLead& lq = blk.leads[q];
+            const CMat& phi_in = sol[q]->modes.left_modes;
+            const std::vector<cplx>& lam_in = sol[q]->modes.left_lambdas;
+            const CMat back = matmul(sol[q]->g, matmul(lq.h01.adjoint(), phi_in));
+            CMat source(phi_in.rows, phi_in.cols);
+            for (int c = 0; c < phi_in.cols; ++c)
+                for (int a = 0; a < phi_in.rows; ++a) source(a, c) = phi_in(a, c) * lam_in[size_t(c)] - back(a, c);
+            const CMat device_source = matmul(lq.coupling, source);
+            for (size_t p = 0; p < P; ++p) {
+                if (!sol[p]) continue;
+                const AttachedLead& lp = blk.leads[p];
+                CMat gpq(int(at[p].size()), int(at[q].size()));
+                for (size_t y = 0; y < at[q].size(); ++y)
+                    for (size_t x = 0; x < at[p].size(); ++x) gpq(int(x), int(y)) = green(at[p][x], at[q][y]);
+                CMat psi1 = matmul(sol[p]->g, matmul(lp.coupling.adjoint(), matmul(gpq, device_source)));
+                if (p == q)
+                    for (size_t e = 0; e < psi1.a.size(); ++e) psi1.a[e] -= back.a[e];
+                const CMat& phi_out = sol[p]->modes.right_modes;
+                const std::vector<cplx>& lam_out = sol[p]->modes.right_lambdas;
+                const CMat a = matmul(phi_out.adjoint(), matmul(sol[p]->rest, psi1));
+                const CMat bm = matmul(phi_out.adjoint(), matmul(lp.h01.adjoint(), psi1));
+                CMat& out_pq = part.s[p][q];
+                for (int c = 0; c < out_pq.cols; ++c)
+                    for (int m = 0; m < out_pq.rows; ++m) {
+                        const cplx cj = std::conj(lam_out[size_t(m)]);
+                        out_pq(m, c) = iu * cj * (a(m, c) - cj * bm(m, c));
+                    }
+            }
+        }
+        parts.push_back(std::move(part));
+    }
+    // SMatrix::assemble (smatrix.rs:62-90): a lead's modes are those of every block in turn, and
+    // the blocks do not mix.
+    in.assign(P, {});
+    out.assign(P, {});
+    for (auto& part : parts)
+        for (size_t p = 0; p < P; ++p) {
+            in[p].insert(in[p].end(), part.in[p].begin(), part.in[p].end());
+            out[p].insert(out[p].end(), part.out[p].begin(), part.out[p].end());
+        }
+    full.assign(P, std::vector<CMat>(P));
+    if (counts_only) return CG_OK;
+    for (size_t p = 0; p < P; ++p)
+        for (size_t q = 0; q < P; ++q) {
+            CMat m(int(out[p].size()), int(in[q].size()));
+            int r0 = 0, c0 = 0;
+            for (auto& part : parts) {
+                const CMat& bq = part.s[p][q];
+                for (int c = 0; c < bq.cols; ++c)
+                    for (int r = 0; r < bq.rows; ++r) m(r0 + r, c0 + c) = bq(r, c);
+                r0 += int(part.out[p].size());
+                c0 += int(part.in[q].size());
+            }
+            full[p][q] = std::move(m);
+        }
+    return CG_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int32_t cg_multi_smatrix(const cg_system* s, double energy, int32_t* modes, double* full, double* lambda_out,
+                         double* lambda_in) {
+    if (!s || !s->multi || !modes || !std::isfinite(energy)) return CG_ERR_INVALID;
+    try {
+        std::vector<std::vector<cplx>> in, out;
+        std::vector<std::vector<CMat>> blocks;
+        const int32_t rc = multi_smatrix(s, energy, !full && !lambda_out && !lambda_in, in, out, blocks);
+        if (rc != CG_OK) return rc;
+        const size_t P = in.size();
+        size_t rows = 0;
+        for (size_t p = 0; p < P; ++p) {
+            modes[p] = int32_t(out[p].size());
+            modes[P + p] = int32_t(in[p].size());
+            rows += out[p].size();
+        }
+        if (full) {
+            size_t r0 = 0;
+            for (size_t p = 0; p < P; ++p) {
+                size_t c0 = 0;
+                for (size_t q = 0; q < P; ++q) {
+                    const CMat& b = blocks[p][q];
+                    for (int c = 0; c < b.cols; ++c)
+                        for (int r = 0; r < b.rows; ++r) {
+                            const size_t e = (r0 + size_t(r)) + (c0 + size_t(c)) * rows;
+                            full[2 * e] = b(r, c).real();
+                            full[2 * e + 1] = b(r, c).imag();
+                        }
+                    c0 += in[q].size();
+                }
+                r0 += out[p].size();
+            }
+        }
+        size_t k = 0;
+        for (size_t p = 0; p < P; ++p)
+            for (cplx v : out[p]) {
+                if (lambda_out) {
+                    lambda_out[2 * k] = v.real();
+                    lambda_out[2 * k + 1] = v.imag();
+                }
+                ++k;
+            }
+        k = 0;
+        for (size_t p = 0; p < P; ++p)
+            for (cplx v : in[p]) {
+                if (lambda_in) {
+                    lambda_in[2 * k] = v.real();
+                    lambda_in[2 * k + 1] = v.imag();
+                }
+                ++k;
+            }
+        return CG_OK;
     } catch (const std::exception&) {
         return CG_ERR_DEVICE;
     }

@@ -898,6 +898,7 @@ void run_level(Engine& eng, const LevelSet& L, const DevLevel& D, const NodeDesc
 void launch_trace_for(Engine& eng, const NodeDesc* root, int32_t nl, const Inputs& in, const PointsView& pv,
                       int32_t P) {
     if (in.nlab > 0) {  // multi-terminal: T between every pair of labels
+        if (!pv.t_lab || !pv.lab_off) throw std::logic_error("a multi-terminal trace without its buffers");
         const size_t cols = size_t(std::max(in.lab_cols, 1));
         eng.lane->trace_x.ensure(size_t(P) * size_t(nl) * cols * sizeof(double2));
         eng.lane->trace_p.ensure(size_t(P) * cols * cols * sizeof(double));
@@ -2298,6 +2299,10 @@ int32_t factor_point(Engine& eng, const cg_system* s, const cg_sigma* g, size_t 
     pv.status = eng.lane->status.as<int32_t>();
     pv.status_stride = hp.slots;
     pv.t_out = eng.lane->tout.as<double>();
+    if (eng.scratch_in.nlab > 0) {  // a multi-terminal point: T between its labels
+        eng.lane->tlab.ensure(sizeof(double) * size_t(eng.scratch_in.nlab) * size_t(eng.scratch_in.nlab));
+        pv.t_lab = eng.lane->tlab.as<double>();
+    }
     pv.keep = 1;
     sweep(eng, hp, dp, pv, 1, false, eng.scratch_in);
     std::vector<int32_t> st(hp.slots);
@@ -2467,6 +2472,42 @@ int32_t backward(const cg_system* s, const cg_sigma* g, double* out_t, double* o
     int32_t st = CG_OK;
     const int32_t rc = backward_batch(&s, &g, 1, out_t, out_grad, &st);
     return rc != CG_OK ? rc : st;
+}
+
+int32_t multi_block_solve(const cg_system* s, const cg_sigma* g, int b, CMat* green, std::vector<cplx>* states) {
+    Engine& eng = Engine::get();
+    if (!eng.ok) return CG_ERR_DEVICE;
+    std::lock_guard<std::mutex> lock(eng.mu);
+    auto sp = structure_of(eng, s);
+    const HostPlan& hp = sp->host[size_t(b)];
+    const DevPlan& dp = sp->dev[size_t(b)];
+    PointsView pv{};
+    const int32_t rc = factor_point(eng, s, g, size_t(b), hp, dp, pv);
+    if (rc != CG_OK) return rc;
+    if (green) {
+        // The root's border block holds -G on the kept orbitals (k0 = the root's ne).
+        const NodeDesc& r = hp.nodes[hp.root];
+        const int k = r.nl, n = r.n, k0 = r.ne;
+        std::vector<double2> blk(size_t(k) * size_t(k));
+        const double2* src = static_cast<const double2*>(eng.lane->root_arena.p) + k0 + int64_t(k0) * n;
+        HIP_CHECK(hipMemcpy2DAsync(blk.data(), size_t(k) * sizeof(double2), src, size_t(n) * sizeof(double2),
+                                   size_t(k) * sizeof(double2), size_t(k), hipMemcpyDeviceToHost, eng.lane->stream));
+        HIP_CHECK(hipStreamSynchronize(eng.lane->stream));
+        *green = CMat(k, k);
+        for (size_t e = 0; e < blk.size(); ++e) green->a[e] = -cplx(blk[e].x, blk[e].y);
+    }
+    if (states) {
+        const CMat& w = g->mblocks[size_t(b)].wab;
+        if (!solve_fits(hp.max_n) || !solve_fits(hp.nodes[hp.root].ne)) return CG_ERR_INVALID;
+        states->clear();
+        if (w.cols > 0) {
+            std::vector<double2> x;
+            solve_columns(eng, hp, dp, pv, int64_t(s->spec.h.n), w, int(hp.left_orbs.size()), dp.left_orbs, false, x);
+            states->resize(x.size());
+            for (size_t e = 0; e < x.size(); ++e) (*states)[e] = cplx(x[e].x, x[e].y);
+        }
+    }
+    return CG_OK;
 }
 
 int32_t scattering_states(const cg_system* s, const cg_sigma* g, int64_t* cols, double* out_l, double* out_r) {

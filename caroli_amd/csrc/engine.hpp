@@ -40,6 +40,12 @@ int32_t backward_batch(const cg_system* const* systems, const cg_sigma* const* s
 /// blocks' columns one after another); with both outputs NULL only the channel counts.
 int32_t scattering_states(const cg_system* s, const cg_sigma* g, int64_t* cols, double* out_l, double* out_r);
 
+/// A multi-terminal point, block b, from one complex f64 sweep with the factors kept: G on the
+/// block's kept orbitals (green), and/or the states G W of every label's channels on all orbitals
+/// (n x the block's channel columns, complex, column-major). CG_ERR_INVALID for delayed separators.
+int32_t multi_block_solve(const cg_system* s, const cg_sigma* g, int b, caroli::CMat* green,
+                          std::vector<caroli::cplx>* states);
+
 bool device_available();
 
 /// Frees the device copies cached for a system.

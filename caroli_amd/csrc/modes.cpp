@@ -97,6 +97,9 @@ Status solve_with(const CMat& h0, const CMat& h1, cplx z, double scale, bool rea
     out.energy = z;
     if (r == 0) {
         out.right = out.left = CMat(0, 0);
+        out.right_modes = out.left_modes = CMat(n, 0);
+        out.right_lambdas.clear();
+        out.left_lambdas.clear();
         out.channels = 0;
         return kOk;
     }
@@ -293,6 +296,8 @@ Status solve_with(const CMat& h0, const CMat& h1, cplx z, double scale, bool rea
     for (size_t e = 0; e < phis.a.size(); ++e) phis.a[e] += t2.a[e];
     CMat h1phi = matmul(h1, phis), h1dphi = matmul(h1.adjoint(), phis);
     int nright = 0;
+    std::vector<std::vector<cplx>> right_phi, left_phi;
+    std::vector<cplx> right_lam, left_lam;
     for (auto& gr : groups) {
         int p0 = gr.g0 - n_re, m = gr.g1 - gr.g0;
         CMat xg = cols_of(x, p0, p0 + m), fg = cols_of(phis, p0, p0 + m);
@@ -315,16 +320,29 @@ Status solve_with(const CMat& h0, const CMat& h1, cplx z, double scale, bool rea
         if (!dense::hermitian_eigen(jh, vel, w)) return kEigen;
         CMat coeffs = matmul(linv.adjoint(), w);
         CMat phi_k = matmul(fg, coeffs), x_k = matmul(xg, coeffs);
+        // The Bloch factor of a group: the mean of its members (equal to kDegenerateTol).
+        cplx mean = 0.0;
+        for (int k = 0; k < m; ++k) mean += plam[p0 + k];
+        mean /= double(m);
+        const cplx lambda = mean / std::abs(mean);
         for (int k = 0; k < m; ++k) {
             double nn = 0.0;
             for (int i = 0; i < n; ++i) nn += std::norm(phi_k(i, k));
             double vk = vel[k] / nn;
             if (std::isnan(vk) || std::abs(vk) < kVelocityTol * scale) return kSingular;  // subband edge
+            // J(phi_k, phi_k) = vel_k: unit current after dividing by its root.
+            std::vector<cplx> flux(static_cast<size_t>(n));
+            const double root = std::sqrt(std::abs(vel[k]));
+            for (int i = 0; i < n; ++i) flux[size_t(i)] = phi_k(i, k) / root;
             if (vk > 0.0) {
                 right.emplace_back(x_k.col(k), x_k.col(k) + m2);
+                right_phi.push_back(std::move(flux));
+                right_lam.push_back(lambda);
                 ++nright;
             } else {
                 left.emplace_back(x_k.col(k), x_k.col(k) + m2);
+                left_phi.push_back(std::move(flux));
+                left_lam.push_back(lambda);
             }
         }
     }
@@ -352,6 +370,15 @@ Status solve_with(const CMat& h0, const CMat& h1, cplx z, double scale, bool rea
     auto invertible = [](const CMat& m) { return dense::Lu(m).inverse_condition_1(dense::norm_1(m)) >= kSingularTol; };
     if (!invertible(rows_of(out.right, 0, r)) || !invertible(rows_of(out.left, r, m2))) return kSingular;
     out.channels = nright;
+    auto columns = [&](const std::vector<std::vector<cplx>>& cols) {
+        CMat m(n, int(cols.size()));
+        for (size_t c = 0; c < cols.size(); ++c) std::copy(cols[c].begin(), cols[c].end(), m.col(int(c)));
+        return m;
+    };
+    out.right_modes = columns(right_phi);
+    out.left_modes = columns(left_phi);
+    out.right_lambdas = right_lam;
+    out.left_lambdas = left_lam;
     return kOk;
 }
 
@@ -514,22 +541,28 @@ Status definite_modes(const CMat& h0, double s, const CMat& l, bool cell, double
     std::vector<cplx> right, leftl;
     std::vector<int> propagating;
     if (Status st = mode_lambdas(nu, s, right, leftl, propagating); st != kOk) return st;
-    // Velocities of the propagating modes, psi = L^-H y (L = sqrt(c) I in the cell case).
+    // Velocities of the propagating modes, psi = L^-H y (L = sqrt(c) I in the cell case); psi is
+    // P-normalised, so J(psi, psi) = -2 s Im lambda: unit current after dividing by its root.
     CMat lhinv;
     if (!cell) lhinv = dense::inverse(l.adjoint());
-    for (int k : propagating) {
+    CMat flux(n, int(propagating.size()));
+    for (size_t q = 0; q < propagating.size(); ++q) {
+        const int k = propagating[q];
         double nn = 0.0;
-        if (cell) {
-            for (int i = 0; i < n; ++i) nn += std::norm((*y)(i, k) / std::sqrt(c));
-        } else {
-            for (int i = 0; i < n; ++i) {
-                cplx v = 0.0;
+        for (int i = 0; i < n; ++i) {
+            cplx v = 0.0;
+            if (cell) {
+                v = (*y)(i, k) / std::sqrt(c);
+            } else {
                 for (int j = 0; j < n; ++j) v += lhinv(i, j) * (*y)(j, k);
-                nn += std::norm(v);
             }
+            flux(i, int(q)) = v;
+            nn += std::norm(v);
         }
         double v = -2.0 * s * right[k].imag() / nn;
         if (!(std::abs(v) >= kVelocityTol * scale)) return kSingular;
+        const double root = std::sqrt(2.0 * std::abs(right[k].imag()));
+        for (int i = 0; i < n; ++i) flux(i, int(q)) /= root;
     }
     auto basis = [&](const std::vector<cplx>& lams) {
         CMat m(2 * n, n);
@@ -549,6 +582,14 @@ Status definite_modes(const CMat& h0, double s, const CMat& l, bool cell, double
     out.left = basis(leftl);
     out.channels = int(propagating.size());
     out.energy = e;
+    out.right_modes = flux;
+    out.left_modes = flux;
+    out.right_lambdas.clear();
+    out.left_lambdas.clear();
+    for (int k : propagating) {
+        out.right_lambdas.push_back(right[k]);
+        out.left_lambdas.push_back(leftl[k]);
+    }
     if (cell) {
         out.cell_y = y;
         out.cell_c = c;

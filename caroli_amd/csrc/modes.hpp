@@ -29,6 +29,11 @@ struct LeadModes {
     CMat right, left;     // orthonormal bases of right- and left-going modes (2r x r)
     int channels = 0;
     cplx energy;
+    /// The propagating modes psi_n = phi lambda^n on a cell (phi in columns), normalised to unit
+    /// current J(a, b) = i(a_n^H H1 b_{n+1} - a_{n+1}^H H1^H b_n), and their Bloch factors
+    /// (modes.rs:64-72): right-going (J = +1) and left-going (J = -1).
+    CMat right_modes, left_modes;
+    std::vector<cplx> right_lambdas, left_lambdas;
     // Scalar coupling (H1 + H1^H = 2 s c I, the Hermitian cell fast path): the bases are
     // Y diag(.) with Y the unitary eigenvectors of the cell, so Sigma_R = c Y diag(s lambda_R) Y^H
     // and Sigma_L = c Y diag(s / lambda_L) Y^H, one product each instead of a solve and two.

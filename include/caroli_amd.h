@@ -317,7 +317,11 @@ int32_t cg_device_available(void);
  *   NestedMultiTerminal::transmissions      nested_multi.rs:174  -> cg_multi_transmissions
  *   ::transmissions_with_precision          nested_multi.rs:180  -> cg_multi_batch_transmissions
  *   ::sector_transmissions                  nested_multi.rs:246  -> cg_multi_self_energies (sectors)
- *                                                                   + cg_multi_batch_transmissions */
+ *                                                                   + cg_multi_batch_transmissions
+ *   ::scattering_states                     nested_multi.rs:354  -> cg_multi_scattering_states
+ *   ::smatrix                               nested_multi.rs:440  -> cg_multi_smatrix
+ *   ::site_densities / bond_currents /      nested_multi.rs:497  -> cg_multi_site_densities /
+ *     operator_currents                                             cg_multi_bond_currents */
 
 /* One lead as attached (AttachedLead, crates/caroli-green/src/multi.rs:15-25): a cell of `cell`
  * orbitals with h00 and h01 (the coupling from a cell to the next one further from the device),
@@ -386,6 +390,30 @@ int32_t cg_multi_batch_transmissions(const cg_system* const* systems, const cg_s
                                      int32_t precision, double* out, int32_t* out_status, cg_batch_stats* stats);
 /* NestedMultiTerminal::transmissions at one energy: out is lead_count x lead_count, row-major. */
 int32_t cg_multi_transmissions(const cg_system* s, double energy, int32_t precision, double* out);
+
+/* NestedMultiTerminal::smatrix (nested_multi.rs:440-470, crates/caroli-green/src/smatrix.rs): the
+ * scattering matrix at `energy` in the basis of the propagating lead modes normalised to unit
+ * current, from one complex f64 sweep per block on the GPU (G on the kept orbitals) and the leads'
+ * modes on the host. modes (2 x lead_count) receives the outgoing then the incoming mode count of
+ * each lead; full (optional, sum(out) x sum(in), complex, column-major): rows the outgoing modes of
+ * lead 0, 1, ..., columns the incoming ones, so |block p,q|^2 sums to T[p][q] and full is unitary;
+ * lambda_out / lambda_in (optional): their Bloch factors. With all three NULL only the counts (no
+ * device work). CG_ERR_INVALID where separators delayed. */
+int32_t cg_multi_smatrix(const cg_system* s, double energy, int32_t* modes, double* full, double* lambda_out,
+                         double* lambda_in);
+/* NestedMultiTerminal::scattering_states (nested_multi.rs:354-413): psi = G W on every orbital for
+ * the channels of each label of g (W W^H = Gamma of the label), the blocks' columns one after
+ * another. cols (one per label) receives the channel counts; with psi non-NULL, psi[l] (n x cols[l],
+ * complex, column-major) the states. CG_ERR_INVALID where separators delayed. */
+int32_t cg_multi_scattering_states(const cg_system* s, const cg_sigma* g, int64_t* cols, double* const* psi);
+/* Site densities and bond currents of multi-terminal states (nested_multi.rs:497-513, as
+ * cg_site_densities / cg_bond_currents / cg_operator_currents with `leads` state arrays psi[p] of
+ * cols[p] columns and occupations f[p]); op NULL for the charge current. */
+int32_t cg_multi_site_densities(const cg_system* s, int32_t leads, const double* const* psi, const int64_t* cols,
+                                const double* f, const double* op, int32_t op_dim, double* out);
+int32_t cg_multi_bond_currents(const cg_system* s, int32_t leads, const double* const* psi, const int64_t* cols,
+                               const double* f, const double* op, int32_t op_dim, int64_t* count, int64_t* from,
+                               int64_t* to, double* current);
 
 #ifdef __cplusplus
 }

@@ -122,3 +122,88 @@ def test_batch_equals_points():
     assert (st == 0).all()
     for k, (s, e) in enumerate(pts):
         assert np.array_equal(t[k], s.transmissions_with(sig[k]))
+
+
+@pytest.mark.parametrize("name", ["square", "bhz", "graphene"])
+def test_smatrix_unitary_with_the_transmissions(name):
+    """The scattering matrix in the basis of the lead modes is unitary and its blocks give the
+    transmissions (crates/caroli-green/tests/nested_multi.rs:322-340)."""
+    if name == "square":
+        sites, itf = cross(ca.Model.Square, 9, 7, 3)
+        s, energies = multi_system(ca.Model.Square, sites, itf, w=1.5), [-2.7, -0.4, 1.3]
+    elif name == "bhz":
+        sites, itf = cross(ca.Model.BHZ, 8, 6, 2)
+        s, energies = multi_system(ca.Model.BHZ, sites, itf, w=0.5), [-0.2, 0.02, 0.3]
+    else:
+        sites, itf = graphene_narrow_lead()
+        s, energies = multi_system(ca.Model.Graphene, sites, itf, w=0.8), [-1.4, 0.45, 1.3]
+    for e in energies:
+        sm = s.smatrix(e)
+        full = sm.full()
+        n = full.shape[1]
+        assert full.shape[0] == n
+        unitarity = np.abs(full.conj().T @ full - np.eye(n)).max()
+        t = s.transmissions(e)
+        worst = max(abs(sm.transmission(p, q) - t[p, q]) for p in range(len(t)) for q in range(len(t)) if p != q)
+        say(f"[gpu] {name} S-matrix E = {e}: {n} modes, |S^H S - 1| {unitarity:.1e}, max |sum|S_pq|^2 - T_pq| {worst:.1e}")
+        assert unitarity < 1e-9 and worst < 1e-9
+        assert all(np.allclose(np.abs(l), 1.0) for l in sm.incoming + sm.outgoing)
+
+
+def test_smatrix_clean_ribbon_sends_each_mode_through():
+    """A clean ribbon: each column of S_RL has one entry of modulus 1, the others vanish, and
+    nothing is reflected (nested_multi.rs:387-415)."""
+    sites = [(x, y, 0, 0) for x in range(8) for y in range(9)]
+    itf = [([(0, y, 0, 0) for y in range(9)], (-1, 0, 0)), ([(7, y, 0, 0) for y in range(9)], (1, 0, 0))]
+    s = ca.NestedMultiTerminal.from_model(ca.Model.Square, sites, itf)
+    for e in (-2.9, -1.1, 0.3, 1.7):
+        sm = s.smatrix(e)
+        t = np.abs(sm.blocks[1][0])
+        assert ((np.abs(t - 1) < 1e-9).sum(axis=0) == 1).all() and ((t < 1e-9).sum(axis=0) == t.shape[0] - 1).all()
+        assert np.abs(sm.blocks[0][0]).max() < 1e-9
+
+
+def test_cross_bond_currents():
+    """With lead q alone injecting, the current out through a cut across arm p is T_pq, into the
+    device through arm q their sum, and Kirchhoff's law holds at every site no lead touches
+    (crates/caroli-green/tests/nested_multi.rs:166-228)."""
+    sites, itf = cross(ca.Model.Square, 9, 7, 4)
+    s = multi_system(ca.Model.Square, sites, itf, w=1.5)
+    lead_sites = {tuple(x) for sl, _ in itf for x in sl}
+    for e in (-2.4, -0.6, 0.8):
+        t = s.transmissions(e)
+        states = s.scattering_states(s.self_energies(e))
+        assert len(states.leads) == 4
+        for q in range(4):
+            f = np.zeros(4)
+            f[q] = 1.0
+            fr, to, cur = s.bond_currents(states, f)
+            for p, (sl, d) in enumerate(itf):
+                depth = lambda i: sites[i][0] * d[0] + sites[i][1] * d[1]
+                edge = sl[0][0] * d[0] + sl[0][1] * d[1] - 1
+                out = sum(c if (depth(a) < edge <= depth(b)) else -c if (depth(b) < edge <= depth(a)) else 0.0
+                          for a, b, c in zip(fr, to, cur))
+                expected = -sum(t[r, q] for r in range(4) if r != q) if p == q else t[p, q]
+                assert abs(out - expected) < 1e-9, (e, q, p, out, expected)
+            net = np.zeros(len(sites))
+            np.add.at(net, fr, cur)
+            np.add.at(net, to, -cur)
+            inner = [i for i, x in enumerate(sites) if tuple(x) not in lead_sites]
+            assert np.abs(net[inner]).max() < 1e-9
+
+
+def test_multi_states_as_two_terminal():
+    """Two leads at the ends of a ribbon: the multi-terminal states give the densities and bond
+    currents of the two-terminal ones (nested_multi.rs:134-163)."""
+    sites = [(x, y, 0, 0) for x in range(11) for y in range(6)]
+    pot = hash_disorder([s[:3] for s in sites], [0] * len(sites), 2.0)
+    itf = [([(0, y, 0, 0) for y in range(6)], (-1, 0, 0)), ([(10, y, 0, 0) for y in range(6)], (1, 0, 0))]
+    multi = ca.NestedMultiTerminal.from_model(ca.Model.Square, sites, itf, site_potential=pot)
+    two = ca.NestedTwoTerminal.preset(ca.Model.Square, [11, 6, 0], site_potential=pot)
+    f = [0.9, 0.2]
+    for e in (-2.2, 0.3, 1.6):
+        sm = multi.scattering_states(multi.self_energies(e))
+        st = two.scattering_states(two.self_energies(e))
+        assert np.abs(multi.site_densities(sm, f) - two.site_densities(st, f)).max() < 1e-10
+        a, b = multi.bond_currents(sm, f), two.bond_currents(st, f)
+        assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]) and np.abs(a[2] - b[2]).max() < 1e-10
