@@ -216,3 +216,6 @@ cg_multi_bond_currents = _sig("cg_multi_bond_currents", C.c_int32, c_sys, C.c_in
                               P(C.c_double), P(C.c_double), C.c_int32, P(C.c_int64), P(C.c_int64), P(C.c_int64),
                               P(C.c_double))
 EXPORTS += ["cg_multi_smatrix", "cg_multi_scattering_states", "cg_multi_site_densities", "cg_multi_bond_currents"]
+cg_multi_disordered_create = _sig("cg_multi_disordered_create", C.c_int32, C.c_int32, P(C.c_uint64), C.c_int32,
+                                  P(C.c_int32), P(C.c_int32), C.c_int64, P(Disorder), C.c_uint64, P(c_sys))
+EXPORTS += ["cg_multi_disordered_create"]

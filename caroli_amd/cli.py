@@ -6,18 +6,20 @@
 
 Reads the input format of the reference's command line (crates/caroli-cli/src/input.rs:36-170:
 `[parameters]`, `[model]`, `[[disorder]]`, `[system]`, `[[run]]`) and runs its transmission
-runs as crates/caroli-cli/src/run.rs:70-190 does for two leads along one axis: per scan point,
-the lead self-energies once per energy on configuration 0, then every configuration's system
-(disorder seeded `seed + c`, crates/caroli-cli/src/build.rs:251-310) and ONE batched launch over
-all (configuration, energy) points instead of a rayon task per point. Each run writes a CSV with
-the reference's columns (`<scan>,config,energy,T`) and number formats.
+runs as crates/caroli-cli/src/run.rs:70-190 does: per scan point, the lead self-energies once per
+energy on configuration 0, then every configuration's system (disorder seeded `seed + c`,
+crates/caroli-cli/src/build.rs:251-310) and ONE batched launch over all (configuration, energy)
+points instead of a rayon task per point. Two leads on opposite faces of one axis go to the
+two-terminal solver (`<scan>,config,energy,T`); leads on other faces, or `sectors`, to the
+multi-terminal one (`T_p_q` or `T_p.a_q.b` columns, run.rs:80-104, :130-185). Each run writes a CSV
+with the reference's columns and number formats.
 
 Covered: the engine's presets (`bhz` with the default parameters, `square`, `cubic`, `graphene`
 with t = 1 and spacing 1), Anderson and magnetic disorder (isotropic, polarized or cone moments;
 same-g or opposite-g spin matrices; `cells` and `sublattices` restrictions), parameters with
-expressions, scans, `configurations`. Everything else of the reference's input (other run
-types, continuum models, lattices of one's own, fields, more than two leads, sectors) is
-refused with an error naming it.
+expressions, scans, `configurations`, leads on any faces, `sectors`. Everything else of the
+reference's input (other run types, continuum models, lattices of one's own, fields) is refused with
+an error naming it.
 """
 import argparse
 import ast
@@ -30,7 +32,7 @@ from concurrent.futures import ThreadPoolExecutor
 
 import numpy as np
 
-from . import NestedTwoTerminal, Precision, _capi as capi, batch_transmission
+from . import NestedMultiTerminal, NestedTwoTerminal, Precision, _capi as capi, batch_transmission, multi_batch_transmissions
 
 try:
     import tomllib as _toml
@@ -214,8 +216,9 @@ def preset_of(model, params):
     return name
 
 
-def leads_axis(system, dim):
-    """build.rs:376-413: two leads on opposite faces of one axis (the two-terminal solver)."""
+def leads_of(system, dim):
+    """build.rs:371-413: ("axis", axis) for two leads on opposite faces of one axis (the two-terminal
+    solver), else ("faces", [(sign, axis), ...]) (the multi-terminal one)."""
     names = system.get("leads", ["-x", "+x"])
     axes = {"x": 0, "y": 1, "z": 2}
     parsed = []
@@ -228,8 +231,8 @@ def leads_axis(system, dim):
     if len(parsed) < 2:
         raise InputError("transport needs at least two leads")
     if len(parsed) == 2 and parsed[0] == (-1, parsed[0][1]) and parsed[1] == (1, parsed[0][1]):
-        return parsed[0][1]
-    raise InputError("the MI355X front end solves two leads on opposite faces of one axis (-x, +x)")
+        return "axis", parsed[0][1]
+    return "faces", parsed
 
 
 def disorders_of(data, params, name):
@@ -309,7 +312,9 @@ class Setup:
         if not size or len(size) != dim or any(int(x) <= 0 for x in size):
             raise InputError(f"`size` needs {dim} positive numbers of cells")
         self.extent = [int(x) for x in size] + [0] * (3 - dim)
-        self.axis = leads_axis(system, dim)
+        self.leads, lead_arg = leads_of(system, dim)
+        self.axis = lead_arg if self.leads == "axis" else None
+        self.faces = lead_arg if self.leads == "faces" else [(-1, lead_arg), (1, lead_arg)]  # run.rs:131-134
         self.disorders = disorders_of(data, params, self.name)
 
     def system(self, config):
@@ -324,26 +329,98 @@ class Setup:
             raise InputError(f"the engine refused the system (status {rc})")
         return NestedTwoTerminal(h.value)
 
+    def multi(self, config):
+        """The multi-terminal system of configuration `config` with a lead on every face
+        (run.rs:130-138), built by cg_multi_disordered_create."""
+        import ctypes as C
+        ext = (C.c_uint64 * 3)(*self.extent)
+        arr = (capi.Disorder * max(1, len(self.disorders)))(*self.disorders)
+        n = len(self.faces)
+        signs = (C.c_int32 * n)(*[f[0] for f in self.faces])
+        axes = (C.c_int32 * n)(*[f[1] for f in self.faces])
+        h = C.c_void_p()
+        rc = capi.cg_multi_disordered_create(PRESETS[self.name], ext, n, signs, axes, len(self.disorders), arr, config,
+                                             C.byref(h))
+        if rc != 0:
+            raise InputError(f"the engine refused the multi-terminal system (status {rc})")
+        return NestedMultiTerminal(h.value)
+
+
+def multi_rows(setup, energies, configs, sectors, prefix, precision, threads):
+    """run.rs:130-185: every configuration's multi-terminal system, the self-energies once per
+    energy (the leads are clean), one batched launch; per point the transmissions between leads (or
+    (lead, sector) labels) p != q in the order of the header. Mixed precision runs these points in
+    complex f64 (the refinement traces one lead pair)."""
+    if precision == Precision.Mixed:
+        precision = Precision.Double
+    systems = [setup.multi(c) for c in range(configs)]
+    count = len(setup.faces)
+    labs = sorted(set(sectors)) if sectors is not None else None
+
+    def sigma(e):
+        try:
+            return systems[0].self_energies(e, sectors=None if sectors is None else [list(sectors)] * count)
+        except Exception:
+            return None
+
+    with ThreadPoolExecutor(threads) as ex:
+        sig = list(ex.map(sigma, energies))
+    pts = [(c, k) for c in range(configs) for k in range(len(energies)) if sig[k] is not None]
+    width = count * (count - 1) if labs is None else (count * len(labs)) * (count * len(labs) - 1)
+    cells = np.full((configs, len(energies), width), np.nan)
+    if pts:
+        t, st = multi_batch_transmissions([systems[c] for c, _ in pts], [sig[k] for _, k in pts], precision)
+        for (c, k), tm, s in zip(pts, t, st):
+            if s != 0:
+                continue
+            if labs is None:
+                cells[c, k] = [tm[p, q] for p in range(count) for q in range(count) if q != p]
+            else:
+                at = {l: i for i, l in enumerate(sig[k].labels())}
+                keys = [(p, a) for p in range(count) for a in labs]
+                cells[c, k] = [tm[at[x], at[y]] if x in at and y in at else 0.0 for x in keys for y in keys if x != y]
+    rows = []
+    for c in range(configs):
+        for k, e in enumerate(energies):
+            rows.append(f"{prefix}{c},{fmt_display(e)}," + ",".join(fmt_exp(v) for v in cells[c, k]) + "\n")
+    return rows
+
 
 def transmission(data, run, base, precision, threads=16):
-    """run.rs:70-190 for two leads along one axis, the configurations x energies of a scan point
-    in one batched launch."""
+    """run.rs:70-190: the configurations x energies of a scan point in one batched launch, on the
+    two-terminal solver for two leads along one axis, else on the multi-terminal one."""
     allowed = {"type", "output", "energies", "scan", "configurations", "sectors"}
     unknown = set(run) - allowed
     if unknown:
         raise InputError(f"unknown key(s) {sorted(unknown)}")
-    if run.get("sectors") is not None:
-        raise InputError("sector transmissions need the multi-terminal solver (not in the MI355X front end)")
+    sectors = run.get("sectors")
+    if sectors is not None and (not isinstance(sectors, list) or not all(isinstance(x, int) and x >= 0 for x in sectors)):
+        raise InputError("`sectors`: a list of non-negative sector numbers, one per orbital of a site")
     if "energies" not in run:
         raise InputError("a transmission run needs `energies`")
     scan = run.get("scan", {})
     points = scan_points(data.get("parameters", {}), base, scan)
     configs = int(run.get("configurations", 1))
-    header = "".join(f"{k}," for k in sorted(scan)) + "config,energy,T\n"
-    rows = [header]
+    first = Setup(data, points[0][1])
+    two = first.leads == "axis" and sectors is None
+    count = len(first.faces)
+    header = "".join(f"{k}," for k in sorted(scan)) + "config,energy"
+    if two:
+        header += ",T"
+    elif sectors is not None:
+        labs = sorted(set(sectors))
+        keys = [(p, a) for p in range(count) for a in labs]
+        header += "".join(f",T_{p}.{a}_{q}.{b}" for (p, a) in keys for (q, b) in keys if (p, a) != (q, b))
+    else:
+        header += "".join(f",T_{p}_{q}" for p in range(count) for q in range(count) if p != q)
+    rows = [header + "\n"]
     for vals, params in points:
         energies = values(run["energies"], params)
         setup = Setup(data, params)
+        prefix = "".join(f"{v}," for v in map(fmt_display, vals))
+        if not two:
+            rows += multi_rows(setup, energies, configs, sectors, prefix, precision, threads)
+            continue
         systems = [setup.system(c) for c in range(configs)]
         reference = systems[0]  # the leads are clean: one set of self-energies serves every configuration
 
@@ -361,11 +438,10 @@ def transmission(data, run, base, precision, threads=16):
             tv, st = batch_transmission([systems[c] for c, _ in pts], [sig[k] for _, k in pts], precision)
             for (c, k), x, s in zip(pts, tv, st):
                 t[c, k] = x if s == 0 else np.nan
-        prefix = "".join(f"{v}," for v in map(fmt_display, vals))
         for c in range(configs):
             for k, e in enumerate(energies):
                 rows.append(f"{prefix}{c},{fmt_display(e)},{fmt_exp(t[c, k])}\n")
-    summary = f"transmission: {len(points)} scan points × {configs} configurations, 2 leads"
+    summary = f"transmission: {len(points)} scan points × {configs} configurations, {count} leads"
     return "".join(rows), summary
 
 

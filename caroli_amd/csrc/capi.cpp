@@ -279,31 +279,62 @@ int32_t cg_bhz_magnetic_create(const uint64_t* extent, int32_t seed_mode, int64_
     }
 }
 
+}  // extern "C"
+
+namespace {
+bool disorder_specs(int64_t count, const cg_disorder* disorders, uint64_t config, std::vector<DisorderSpec>& ds) {
+    for (int64_t k = 0; k < count; ++k) {
+        const cg_disorder& c = disorders[k];
+        DisorderSpec d;
+        d.kind = c.kind;
+        d.strength = c.strength;
+        d.concentration = c.concentration;
+        d.magnetic.moments = c.moments;
+        d.magnetic.moment_param = c.moment_param;
+        d.magnetic.coupling = c.spin;
+        d.magnetic.exchange = c.exchange;
+        for (int a = 0; a < 3; ++a) {
+            d.has_cells[a] = c.has_cells[a] != 0;
+            d.cells[a][0] = c.cells[a][0];
+            d.cells[a][1] = c.cells[a][1];
+        }
+        d.sublattices = c.sublattices;
+        d.seed = c.seed + config;
+        if (!(d.concentration >= 0.0 && d.concentration <= 1.0)) return false;
+        ds.push_back(d);
+    }
+    return true;
+}
+}  // namespace
+
+extern "C" {
+
+int32_t cg_multi_disordered_create(int32_t model, const uint64_t* extent, int32_t faces, const int32_t* face_sign,
+                                   const int32_t* face_axis, int64_t count, const cg_disorder* disorders,
+                                   uint64_t config, cg_system** out) {
+    if (!out || !extent || faces < 2 || !face_sign || !face_axis || count < 0 || (count > 0 && !disorders))
+        return CG_ERR_INVALID;
+    try {
+        std::vector<DisorderSpec> ds;
+        if (!disorder_specs(count, disorders, config, ds)) return CG_ERR_INVALID;
+        std::vector<std::pair<int, int>> fs;
+        for (int32_t k = 0; k < faces; ++k) fs.emplace_back(face_sign[k], face_axis[k]);
+        const size_t ext[3] = {size_t(extent[0]), size_t(extent[1]), size_t(extent[2])};
+        MultiSpec m = disordered_multi(model, ext, fs, ds);
+        auto s = std::make_unique<cg_system>();
+        s->spec = std::move(m.base);
+        return finish_multi(s.release(), std::move(m.leads), std::move(m.orbital_kinds), out);
+    } catch (const std::exception&) {
+        return CG_ERR_INVALID;
+    }
+}
+
 int32_t cg_disordered_create(int32_t model, const uint64_t* extent, int32_t axis, int64_t count,
                              const cg_disorder* disorders, uint64_t config, cg_system** out) {
     if (!out || !extent || count < 0 || (count > 0 && !disorders)) return CG_ERR_INVALID;
     try {
         std::vector<DisorderSpec> ds;
-        for (int64_t k = 0; k < count; ++k) {
-            const cg_disorder& c = disorders[k];
-            DisorderSpec d;
-            d.kind = c.kind;
-            d.strength = c.strength;
-            d.concentration = c.concentration;
-            d.magnetic.moments = c.moments;
-            d.magnetic.moment_param = c.moment_param;
-            d.magnetic.coupling = c.spin;
-            d.magnetic.exchange = c.exchange;
-            for (int a = 0; a < 3; ++a) {
-                d.has_cells[a] = c.has_cells[a] != 0;
-                d.cells[a][0] = c.cells[a][0];
-                d.cells[a][1] = c.cells[a][1];
-            }
-            d.sublattices = c.sublattices;
-            d.seed = c.seed + config;
-            if (!(d.concentration >= 0.0 && d.concentration <= 1.0)) return CG_ERR_INVALID;
-            ds.push_back(d);
-        }
+        if (!disorder_specs(count, disorders, config, ds)) return CG_ERR_INVALID;
         const size_t ext[3] = {size_t(extent[0]), size_t(extent[1]), size_t(extent[2])};
         auto* s = new cg_system;
         s->spec = disordered_system(model, ext, axis, ds);

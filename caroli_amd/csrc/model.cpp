@@ -455,10 +455,10 @@ SystemSpec bhz_magnetic_system(const size_t extent[2], int seed_mode, int64_t se
     return system_from_models(device, lead, region, 0);
 }
 
-SystemSpec disordered_system(int model, const size_t extent[3], int axis, const std::vector<DisorderSpec>& disorders) {
-    if (model < 0 || model > 3) throw std::invalid_argument("unknown preset model");
-    Model lead = model == 0 ? bhz(BhzParams{}) : model == 1 ? square_model() : model == 2 ? graphene_model() : cubic_model();
-    Region region = Region::cuboid(lead.lattice, {extent[0], extent[1], extent[2]});
+namespace {
+// The CLI's device model on its cuboid region (crates/caroli-cli/src/build.rs:251-310): every
+// disorder added in order.
+Model disordered_model(const Model& lead, const Region& region, const std::vector<DisorderSpec>& disorders) {
     Model device = lead;
     for (const DisorderSpec& d : disorders) {
         std::vector<Site> sites;
@@ -496,7 +496,37 @@ SystemSpec disordered_system(int model, const size_t extent[3], int axis, const 
             throw std::invalid_argument("unknown disorder");
         }
     }
-    return system_from_models(device, lead, region, axis);
+    return device;
+}
+}  // namespace
+
+SystemSpec disordered_system(int model, const size_t extent[3], int axis, const std::vector<DisorderSpec>& disorders) {
+    Model lead = preset_model(model);
+    Region region = Region::cuboid(lead.lattice, {extent[0], extent[1], extent[2]});
+    return system_from_models(disordered_model(lead, region, disorders), lead, region, axis);
+}
+
+// build.rs:371-413 (Leads::Faces): one interface per face, the region's sites on it, stepping out.
+MultiSpec disordered_multi(int model, const size_t extent[3], const std::vector<std::pair<int, int>>& faces,
+                           const std::vector<DisorderSpec>& disorders) {
+    Model lead = preset_model(model);
+    Region region = Region::cuboid(lead.lattice, {extent[0], extent[1], extent[2]});
+    std::vector<Region> slices;
+    std::vector<Cell> outward;
+    for (auto [sign, axis] : faces) {
+        if ((sign != 1 && sign != -1) || axis < 0 || axis >= region.lattice.dim())
+            throw std::invalid_argument("a face such as -x or +y");
+        int64_t edge = sign < 0 ? INT64_MAX : INT64_MIN;
+        for (auto& st : region.sites) edge = sign < 0 ? std::min(edge, st.cell[axis]) : std::max(edge, st.cell[axis]);
+        std::vector<Site> on;
+        for (auto& st : region.sites)
+            if (st.cell[axis] == edge) on.push_back(st);
+        slices.push_back(Region::from_sites(region.lattice, on));
+        Cell d{0, 0, 0};
+        d[axis] = sign;
+        outward.push_back(d);
+    }
+    return multi_from_models(disordered_model(lead, region, disorders), lead, region, slices, outward);
 }
 
 Model preset_model(int model) {

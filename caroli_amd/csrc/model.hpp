@@ -226,6 +226,10 @@ struct DisorderSpec {
 /// 1 square, 2 zigzag graphene, 3 cubic) on Region::cuboid(extent) with every disorder added, as a
 /// system along `axis` with the clean preset as leads.
 SystemSpec disordered_system(int model, const size_t extent[3], int axis, const std::vector<DisorderSpec>& disorders);
+/// The same model with leads on the given faces (sign, axis) of the cuboid, a multi-terminal device
+/// (build.rs:371-413, Leads::Faces).
+MultiSpec disordered_multi(int model, const size_t extent[3], const std::vector<std::pair<int, int>>& faces,
+                           const std::vector<DisorderSpec>& disorders);
 
 /// BHZ ribbon extent[0] x extent[1] (axis 0, clean BHZ leads) with int(conc nx_imp ny) magnetic
 /// impurities among the sites x0 <= x < x0 + nx_imp, all y, listed with x fastest (the thesis

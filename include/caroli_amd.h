@@ -361,6 +361,13 @@ typedef struct {
 int32_t cg_multi_model_create(int32_t model, uint64_t sites, const int64_t* cells, const int32_t* subs,
                               const double* site_potential, int32_t interfaces, const cg_interface* itf,
                               cg_system** out);
+/* The CLI's multi-terminal system of configuration `config` (crates/caroli-cli/src/build.rs:251-310,
+ * :371-413, run.rs:130-138): the preset model on Region::cuboid(extent) with every disorder added
+ * (as cg_disordered_create), and a clean lead on each face (face_sign[k] = -1 or +1 along face_axis[k];
+ * the interface is the region's sites on that face). */
+int32_t cg_multi_disordered_create(int32_t model, const uint64_t* extent, int32_t faces, const int32_t* face_sign,
+                                   const int32_t* face_axis, int64_t count, const cg_disorder* disorders,
+                                   uint64_t config, cg_system** out);
 
 /* Number of leads of a multi-terminal system (-1 for another handle). */
 int32_t cg_multi_lead_count(const cg_system* s);

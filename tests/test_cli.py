@@ -124,8 +124,10 @@ energies = [0.1]
 @pytest.mark.parametrize("text,message", [
     ("[model]\npreset = 'qwz'\n[system]\nsize=[4,4]\n[[run]]\ntype='transmission'\nenergies=[0]", "qwz"),
     ("[model]\npreset = 'square'\nt = 2.0\n[system]\nsize=[4,4]\n[[run]]\ntype='transmission'\nenergies=[0]", "t = 2"),
-    ("[model]\npreset = 'square'\n[system]\nsize=[4,4]\nleads=['-x','+y']\n[[run]]\ntype='transmission'\n"
-     "energies=[0]", "opposite faces"),
+    ("[model]\npreset = 'square'\n[system]\nsize=[4,4]\nleads=['-x']\n[[run]]\ntype='transmission'\n"
+     "energies=[0]", "at least two leads"),
+    ("[model]\npreset = 'square'\n[system]\nsize=[4,4]\nleads=['-x','+z']\n[[run]]\ntype='transmission'\n"
+     "energies=[0]", "dimension 2"),
     ("[model]\npreset = 'bhz'\n[[disorder]]\ntype='magnetic'\nconcentration=0.1\nexchange=1\n[system]\n"
      "size=[4,4]\n[[run]]\ntype='transmission'\nenergies=[0]", "spin"),
     ("[model]\npreset = 'square'\n[system]\nsize=[4,4]\n[[run]]\ntype='dos'\nenergies=[0]", "dos"),
@@ -149,3 +151,43 @@ def test_cli_check_describes_the_sample(tmp_path, capsys):
     assert "sample: [20, 8] cells, 160 sites, 640 orbitals" in out and "run 0: 00-transmission.csv" in out
     assert "'half': 0.2" in out
     assert cli.main(["check", str(path), "--set", "nope=1"]) == 1
+
+
+CROSS_LIKE = """
+[model]
+preset = "bhz"
+[[disorder]]
+type = "anderson"
+strength = 0.5
+[system]
+size = [10, 8]
+leads = ["-x", "+x", "-y", "+y"]
+[[run]]
+type = "transmission"
+energies = [0.02, 0.3]
+configurations = 2
+[[run]]
+type = "transmission"
+energies = [0.3]
+sectors = [0, 0, 1, 1]
+"""
+
+
+def test_cli_multi_terminal_systems():
+    """Leads on four faces (build.rs:371-413): the engine's multi-terminal system of each
+    configuration has the face slices as interfaces and the Hamiltonian of the two-terminal builder
+    with the same seeded disorder."""
+    data = cli.parse(CROSS_LIKE)
+    setup = cli.Setup(data, {})
+    assert setup.leads == "faces" and setup.faces == [(-1, 0), (1, 0), (-1, 1), (1, 1)]
+    two = cli.Setup(cli.parse(CROSS_LIKE.replace('leads = ["-x", "+x", "-y", "+y"]', "")), {})
+    assert two.leads == "axis"
+    for c in (0, 1):
+        m, s2 = setup.multi(c), two.system(c)
+        a, b = m.arrays(), s2.arrays()
+        assert np.array_equal(a["vals"], b["vals"]) and np.array_equal(a["cols"], b["cols"])
+        assert m.lead_count() == 4
+        for p, (sign, axis) in enumerate(setup.faces):
+            edge = a["cells"][:, axis].min() if sign < 0 else a["cells"][:, axis].max()
+            on = np.flatnonzero(a["cells"][:, axis] == edge)
+            assert list(m.lead(p)["orbitals"]) == sorted(int(a["site_offsets"][i]) + o for i in on for o in range(4))

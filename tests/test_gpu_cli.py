@@ -90,3 +90,56 @@ def test_cli_clean_bhz_ribbon_plateau(tmp_path):
     rows = read(tmp_path / "o" / "00-transmission.csv")
     assert rows[0]["config"] == "0" and rows[0]["energy"] == "0"
     assert abs(float(rows[0]["T"]) - 2.0) < 1e-8
+
+
+CROSS = """
+[model]
+preset = "bhz"
+[[disorder]]
+type = "anderson"
+strength = 0.5
+[system]
+size = [10, 8]
+leads = ["-x", "+x", "-y", "+y"]
+[[run]]
+type = "transmission"
+output = "t.csv"
+energies = [0.02, 0.3]
+configurations = 2
+[[run]]
+type = "transmission"
+output = "s.csv"
+energies = [0.3]
+sectors = [0, 0, 1, 1]
+"""
+
+
+def test_cli_multi_terminal_run(tmp_path):
+    """Leads on four faces go to the multi-terminal solver (run.rs:130-185): T_p_q columns for p != q
+    against the dense solve of the same systems; with `sectors` the T_p.a_q.b columns, whose spin
+    sectors add up to T_p_q."""
+    path = tmp_path / "in.toml"
+    path.write_text(CROSS)
+    out = tmp_path / "out"
+    assert cli.main(["run", str(path), "--out", str(out)]) == 0
+    rows = read(out / "t.csv")
+    pairs = [(p, q) for p in range(4) for q in range(4) if p != q]
+    assert list(rows[0].keys()) == ["config", "energy"] + [f"T_{p}_{q}" for p, q in pairs] and len(rows) == 4
+    setup = cli.Setup(cli.parse(CROSS), {})
+    from helpers import dense_multi
+    worst = 0.0
+    for r in rows:
+        m = setup.multi(int(r["config"]))
+        e = float(r["energy"])
+        d = dense_multi(m, m.self_energies(e), e)
+        for p, q in pairs:
+            worst = max(worst, abs(float(r[f"T_{p}_{q}"]) - d[p, q]))
+    say(f"[gpu] caroli run, four leads: max |T_cli - T_dense| {worst:.1e}")
+    assert worst < 1e-9
+    sec = read(out / "s.csv")[0]
+    keys = [(p, a) for p in range(4) for a in (0, 1)]
+    assert list(sec.keys())[2:] == [f"T_{p}.{a}_{q}.{b}" for (p, a) in keys for (q, b) in keys if (p, a) != (q, b)]
+    t0 = [r for r in rows if r["config"] == "0" and r["energy"] == "0.3"][0]
+    for p, q in pairs:
+        total = sum(float(sec[f"T_{p}.{a}_{q}.{b}"]) for a in (0, 1) for b in (0, 1))
+        assert abs(total - float(t0[f"T_{p}_{q}"])) < 1e-10
