@@ -38,7 +38,8 @@ class GreenError(RuntimeError):
 
 def _check(rc, what=""):
     if rc != 0:
-        raise GreenError(rc, what)
+        detail = (capi.cg_last_error() or b"").decode(errors="replace") if rc in (2, 6, 7) else ""
+        raise GreenError(rc, "; ".join(x for x in (what, detail) if x))
 
 
 class Precision(enum.IntEnum):

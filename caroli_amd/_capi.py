@@ -219,3 +219,5 @@ EXPORTS += ["cg_multi_smatrix", "cg_multi_scattering_states", "cg_multi_site_den
 cg_multi_disordered_create = _sig("cg_multi_disordered_create", C.c_int32, C.c_int32, P(C.c_uint64), C.c_int32,
                                   P(C.c_int32), P(C.c_int32), C.c_int64, P(Disorder), C.c_uint64, P(c_sys))
 EXPORTS += ["cg_multi_disordered_create"]
+cg_last_error = _sig("cg_last_error", C.c_char_p)
+EXPORTS += ["cg_last_error"]

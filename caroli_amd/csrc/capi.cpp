@@ -8,6 +8,7 @@
 #include <memory>
 #include <exception>
 #include <map>
+#include <string>
 
 #include "../../include/caroli_amd.h"
 #include "engine.hpp"
@@ -114,6 +115,10 @@ Work algorithmic_work(const Symbolic& sym, size_t nnz) {
 }  // namespace caroli
 
 namespace {
+
+// The message of the last exception a C ABI call of this thread turned into a status code.
+thread_local std::string last_error;
+void note_error(const std::exception& e) { last_error = e.what(); }
 
 uint64_t fnv(uint64_t h, const void* p, size_t n) {
     auto* c = static_cast<const unsigned char*>(p);
@@ -254,7 +259,8 @@ int32_t cg_system_create(uint64_t n, const int64_t* row_ptr, const uint32_t* col
         sp.right_h00 = read_mat(right_h00, nr);
         sp.right_h01 = read_mat(right_h01, nr);
         return finish(s.release(), out);
-    } catch (const std::exception&) {
+    } catch (const std::exception& e_) {
+        note_error(e_);
         return CG_ERR_INVALID;
     }
 }
@@ -274,7 +280,8 @@ int32_t cg_bhz_magnetic_create(const uint64_t* extent, int32_t seed_mode, int64_
         auto* s = new cg_system;
         s->spec = bhz_magnetic_system(ext, seed_mode, seed, x0, nx_imp, concentration, m);
         return finish(s, out);
-    } catch (const std::exception&) {
+    } catch (const std::exception& e_) {
+        note_error(e_);
         return CG_ERR_INVALID;
     }
 }
@@ -324,7 +331,8 @@ int32_t cg_multi_disordered_create(int32_t model, const uint64_t* extent, int32_
         auto s = std::make_unique<cg_system>();
         s->spec = std::move(m.base);
         return finish_multi(s.release(), std::move(m.leads), std::move(m.orbital_kinds), out);
-    } catch (const std::exception&) {
+    } catch (const std::exception& e_) {
+        note_error(e_);
         return CG_ERR_INVALID;
     }
 }
@@ -339,7 +347,8 @@ int32_t cg_disordered_create(int32_t model, const uint64_t* extent, int32_t axis
         auto* s = new cg_system;
         s->spec = disordered_system(model, ext, axis, ds);
         return finish(s, out);
-    } catch (const std::exception&) {
+    } catch (const std::exception& e_) {
+        note_error(e_);
         return CG_ERR_INVALID;
     }
 }
@@ -360,7 +369,8 @@ int32_t cg_preset_create(int32_t model, const uint64_t* extent, int32_t axis, in
         auto* s = new cg_system;
         s->spec = preset_system(a);
         return finish(s, out);
-    } catch (const std::exception&) {
+    } catch (const std::exception& e_) {
+        note_error(e_);
         return CG_ERR_INVALID;
     }
 }
@@ -574,7 +584,8 @@ static int32_t self_energies_at(const cg_system* s, double energy, double eta, c
         }
         *out = g;
         return CG_OK;
-    } catch (const std::exception&) {
+    } catch (const std::exception& e_) {
+        note_error(e_);
         return CG_ERR_EIGEN;
     }
 }
@@ -640,7 +651,8 @@ int32_t cg_batch_transmission(const cg_system* const* systems, const cg_sigma* c
         if (!systems[p] || !sigmas[p] || !sigma_matches(*systems[p], *sigmas[p])) return CG_ERR_INVALID;
     try {
         return gpu::batch_transmission(systems, sigmas, count, precision, out_t, out_status, stats);
-    } catch (const std::exception&) {
+    } catch (const std::exception& e_) {
+        note_error(e_);
         return CG_ERR_DEVICE;
     }
 }
@@ -653,7 +665,8 @@ int32_t cg_batch_create(const cg_system* const* systems, const cg_sigma* const* 
         if (!systems[p] || !sigmas[p] || !sigma_matches(*systems[p], *sigmas[p])) return CG_ERR_INVALID;
     try {
         return gpu::create_batch(systems, sigmas, count, precision, out);
-    } catch (const std::exception&) {
+    } catch (const std::exception& e_) {
+        note_error(e_);
         return CG_ERR_DEVICE;
     }
 }
@@ -662,7 +675,8 @@ int32_t cg_batch_run(cg_batch* batch, double* out_t, int32_t* out_status, cg_bat
     if (!batch || !out_t) return CG_ERR_INVALID;
     try {
         return gpu::run_batch(batch, out_t, out_status, stats);
-    } catch (const std::exception&) {
+    } catch (const std::exception& e_) {
+        note_error(e_);
         return CG_ERR_DEVICE;
     }
 }
@@ -680,7 +694,8 @@ int32_t cg_batch_autotune(cg_batch* batch, cg_tuning* out, double* out_seconds) 
     if (!batch) return CG_ERR_INVALID;
     try {
         return gpu::autotune_batch(batch, out, out_seconds);
-    } catch (const std::exception&) {
+    } catch (const std::exception& e_) {
+        note_error(e_);
         return CG_ERR_DEVICE;
     }
 }
@@ -692,7 +707,8 @@ int32_t cg_front_schur(const double* f, int64_t n, int64_t ne, int64_t ne1, int3
         return CG_ERR_INVALID;
     try {
         return gpu::front_schur(f, n, ne, ne1, precision, out_s, out_status);
-    } catch (const std::exception&) {
+    } catch (const std::exception& e_) {
+        note_error(e_);
         return CG_ERR_DEVICE;
     }
 }
@@ -705,7 +721,8 @@ int32_t cg_front_schur_noalloc(const double* f, int64_t n, int64_t ne, int64_t n
     if (n > CG_FRONT_NOALLOC_MAX_N) return CG_ERR_CAPACITY;
     try {
         return gpu::front_schur_noalloc(f, n, ne, ne1, precision, out_s, out_status);
-    } catch (const std::exception&) {
+    } catch (const std::exception& e_) {
+        note_error(e_);
         return CG_ERR_DEVICE;
     }
 }
@@ -713,7 +730,8 @@ int32_t cg_front_schur_noalloc(const double* f, int64_t n, int64_t ne, int64_t n
 int32_t cg_kernel_timing(int32_t enable) {
     try {
         return gpu::kernel_timing(enable);
-    } catch (const std::exception&) {
+    } catch (const std::exception& e_) {
+        note_error(e_);
         return CG_ERR_DEVICE;
     }
 }
@@ -807,7 +825,8 @@ int32_t cg_backward(const cg_system* s, const cg_sigma* g, double* out_t, double
     if (!s || !g || !out_t || !out_grad || s->multi || !sigma_matches(*s, *g)) return CG_ERR_INVALID;
     try {
         return gpu::backward(s, g, out_t, out_grad);
-    } catch (const std::exception&) {
+    } catch (const std::exception& e_) {
+        note_error(e_);
         return CG_ERR_DEVICE;
     }
 }
@@ -820,7 +839,8 @@ int32_t cg_backward_batch(const cg_system* const* systems, const cg_sigma* const
             return CG_ERR_INVALID;
     try {
         return gpu::backward_batch(systems, sigmas, count, out_t, out_grad, out_status);
-    } catch (const std::exception&) {
+    } catch (const std::exception& e_) {
+        note_error(e_);
         return CG_ERR_DEVICE;
     }
 }
@@ -830,7 +850,8 @@ int32_t cg_scattering_states(const cg_system* s, const cg_sigma* g, int64_t* col
     if (!s || !g || !cols || s->multi || !sigma_matches(*s, *g)) return CG_ERR_INVALID;
     try {
         return gpu::scattering_states(s, g, cols, psi_left, psi_right);
-    } catch (const std::exception&) {
+    } catch (const std::exception& e_) {
+        note_error(e_);
         return CG_ERR_DEVICE;
     }
 }
@@ -1019,6 +1040,8 @@ int32_t cg_multi_bond_currents(const cg_system* s, int32_t leads, const double* 
 
 int32_t cg_device_available(void) { return gpu::device_available() ? 1 : 0; }
 
+const char* cg_last_error(void) { return last_error.c_str(); }
+
 // ---- NestedMultiTerminal (crates/caroli-green/src/nested_multi.rs) ----
 
 int32_t cg_multi_create(uint64_t n, const int64_t* row_ptr, const uint32_t* cols, const double* vals, uint64_t nsites,
@@ -1054,7 +1077,8 @@ int32_t cg_multi_create(uint64_t n, const int64_t* row_ptr, const uint32_t* cols
             kinds.push_back(std::move(kind));
         }
         return finish_multi(s.release(), std::move(ls), std::move(kinds), out);
-    } catch (const std::exception&) {
+    } catch (const std::exception& e_) {
+        note_error(e_);
         return CG_ERR_INVALID;
     }
 }
@@ -1085,7 +1109,8 @@ int32_t cg_multi_model_create(int32_t model, uint64_t sites, const int64_t* cell
         auto s = std::make_unique<cg_system>();
         s->spec = std::move(m.base);
         return finish_multi(s.release(), std::move(m.leads), std::move(m.orbital_kinds), out);
-    } catch (const std::exception&) {
+    } catch (const std::exception& e_) {
+        note_error(e_);
         return CG_ERR_INVALID;
     }
 }
@@ -1256,7 +1281,8 @@ int32_t cg_multi_self_energies(const cg_system* s, double energy, double eta, co
     }
     try {
         return multi_sigmas(s, energy, eta, sectors, out);
-    } catch (const std::exception&) {
+    } catch (const std::exception& e_) {
+        note_error(e_);
         return CG_ERR_EIGEN;
     }
 }
@@ -1303,7 +1329,8 @@ int32_t cg_multi_batch_transmissions(const cg_system* const* systems, const cg_s
         if (rc != CG_OK) return rc;
         std::unique_ptr<cg_batch, void (*)(cg_batch*)> hold(bt, gpu::destroy_batch);
         return gpu::run_batch_labels(bt, out, out_status, stats);
-    } catch (const std::exception&) {
+    } catch (const std::exception& e_) {
+        note_error(e_);
         return CG_ERR_DEVICE;
     }
 }
@@ -1340,7 +1367,8 @@ int32_t cg_multi_scattering_states(const cg_system* s, const cg_sigma* g, int64_
             }
         }
         return CG_OK;
-    } catch (const std::exception&) {
+    } catch (const std::exception& e_) {
+        note_error(e_);
         return CG_ERR_DEVICE;
     }
 }
@@ -1530,7 +1558,8 @@ int32_t cg_multi_smatrix(const cg_system* s, double energy, int32_t* modes, doub
                 ++k;
             }
         return CG_OK;
-    } catch (const std::exception&) {
+    } catch (const std::exception& e_) {
+        note_error(e_);
         return CG_ERR_DEVICE;
     }
 }

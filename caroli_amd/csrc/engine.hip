@@ -1493,7 +1493,7 @@ int32_t create_batch(const cg_system* const* systems, const cg_sigma* const* sig
     std::map<uint64_t, std::vector<int64_t>> groups;
     for (int64_t p = 0; p < count; ++p) groups[systems[p]->key].push_back(p);
     struct Need {
-        size_t arena, root, piv, status, tout;
+        size_t arena, root, piv, status, tout, refine;
     };
     std::vector<Need> need;  // device bytes of each unit
     // The points of a (structure, block) are split so that at least kLanes units can be in flight
@@ -1512,9 +1512,20 @@ int32_t create_batch(const cg_system* const* systems, const cg_sigma* const* sig
         for (size_t b = 0; b < sp->host.size(); ++b) {
             const HostPlan& hp = sp->host[b];
             const Block& blk = systems[idx[0]]->sym.blocks[b];
-            // Points per launch: bounded by the scratch budget and the grid's y dimension.
-            const size_t per_point = size_t(hp.arena_stride) * es + size_t(hp.root_stride) * sizeof(double2) +
-                                     size_t(hp.piv_stride) * 4 + size_t(hp.slots) * 4 + 64;
+            // Points per launch: bounded by the scratch budget and the grid's y dimension. Mixed
+            // precision also holds, per point and channel, the refinement's solution, right-hand
+            // side, correction and update vectors (complex f64) beside the factors.
+            size_t per_point = size_t(hp.arena_stride) * es + size_t(hp.root_stride) * sizeof(double2) +
+                               size_t(hp.piv_stride) * 4 + size_t(hp.slots) * 4 + 64;
+            size_t refine_point = 0;
+            if (precision == CG_MIXED) {
+                int32_t c = 0;
+                for (auto p : idx) c = std::max(c, sigmas[p]->blocks[b].wr.cols);
+                const size_t n_orb = systems[idx[0]]->spec.h.n;
+                refine_point = (3 * n_orb + size_t(hp.upd_total) + blk.left_orbitals.size() +
+                                blk.right_orbitals.size()) * size_t(c) * sizeof(double2);
+                per_point += refine_point;
+            }
             const size_t budget = size_t(96) << 30;
             // As many parts as keep kMinUnitPoints each (33 points stay one unit, not 32 + 1).
             const size_t parts = std::max<size_t>(1, std::min(splits, idx.size() / kMinUnitPoints));
@@ -1535,7 +1546,7 @@ int32_t create_batch(const cg_system* const* systems, const cg_sigma* const* sig
                 PointsView pv{};
                 stage_points(eng, ss, gg, int(b), int(blk.left_orbitals.size()), int(blk.right_orbitals.size()), u.in, pv);
                 const LaneNeed ln = lane_need(hp, P, single);
-                need.push_back({ln.arena, ln.root, ln.piv, ln.status, ln.tout});
+                need.push_back({ln.arena, ln.root, ln.piv, ln.status, ln.tout, refine_point * P});
                 bt->units.push_back(std::move(u));
             }
         }
@@ -1543,7 +1554,7 @@ int32_t create_batch(const cg_system* const* systems, const cg_sigma* const* sig
     // Lanes: as many as there are units, up to kLanes, while the largest units fit the concurrent
     // budget together.
     std::vector<size_t> sizes;
-    for (auto& nd : need) sizes.push_back(nd.arena + nd.root + nd.piv + nd.status + nd.tout);
+    for (auto& nd : need) sizes.push_back(nd.arena + nd.root + nd.piv + nd.status + nd.tout + nd.refine);
     std::sort(sizes.rbegin(), sizes.rend());
     const size_t concurrent_budget = size_t(160) << 30;
     int lanes = int(std::min<size_t>(kLanes, sizes.size()));

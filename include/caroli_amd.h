@@ -303,6 +303,10 @@ int32_t cg_operator_currents(const cg_system* s, const double* psi_left, const d
 
 /* Device present and usable (1) or not (0). */
 int32_t cg_device_available(void);
+/* The message behind the last CG_ERR_DEVICE / CG_ERR_INVALID / CG_ERR_EIGEN that a call of this
+ * thread turned an exception into (the HIP call and its error string for a device error), or "".
+ * Valid until this thread's next call. */
+const char* cg_last_error(void);
 
 /* ---- NestedMultiTerminal (crates/caroli-green/src/nested_multi.rs) ----
  * A device with any number of leads. The sites a lead couples to stay at the root of the
