@@ -143,6 +143,7 @@ class Tuning(C.Structure):
         ("split", C.c_int32),
         ("blocked_min_ne", C.c_int32),
         ("unblocked", C.c_int32),
+        ("schur_tile", C.c_int32),
     ]
 
     def as_dict(self):

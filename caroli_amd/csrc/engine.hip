@@ -104,6 +104,7 @@ struct LevelSet {
     // Classified for the unblocked fallback (every large front assembled whole and eliminated by
     // rank-1 updates): its lists are the only ones the launches may follow.
     bool unblocked = false;
+    int32_t schur_tile = 64;  // the width of the Schur tiles listed in `tiles`
     // Algorithmic work (complex words, flops) for the kernel timers.
     double lds_flops[kClasses] = {}, lds_words[kClasses] = {}, lds_s_words[kClasses] = {}, lds_f_words[kClasses] = {};
     double asm_words = 0, gather_words = 0, border_flops = 0, border_words = 0, schur_flops = 0, schur_words = 0;
@@ -196,6 +197,16 @@ LevelSet classify(const std::vector<NodeDesc>& nodes, const std::vector<KidDesc>
     const int limit = single ? kLdsFrontC64 : kLdsFrontC128;
     const bool unblocked = unblocked_fronts(), split = !border_only();
     L.unblocked = unblocked;
+    // Schur tile width: the tuning's, or by level (0): 32 when every Schur block of the level's
+    // large fronts has at most kSchurTile32MaxNb rows, else 64.
+    int ts = current_tuning().schur_tile;
+    if (ts == 0) {
+        int nb_max = 0;
+        for (int32_t q : list)
+            if (root || nodes[q].n > limit) nb_max = std::max(nb_max, nodes[q].n - nodes[q].ne);
+        ts = nb_max <= kSchurTile32MaxNb ? 32 : 64;
+    }
+    L.schur_tile = ts;
     const int blocked_above = blocked_min_ne();
     for (int32_t q : list) {
         const NodeDesc& d = nodes[q];
@@ -247,8 +258,8 @@ LevelSet classify(const std::vector<NodeDesc>& nodes, const std::vector<KidDesc>
         }
         L.schur_flops += 8.0 * ne * nb * nb;
         L.schur_words += nb * nb + 2.0 * ne * nb + (root ? nb * nb : (1.0 - border_share) * kid_words);
-        for (int i0 = 0; i0 < d.n - d.ne; i0 += kTile)
-            for (int j0 = 0; j0 < d.n - d.ne; j0 += kTile) L.tiles.push_back({q, i0, j0, 0});
+        for (int i0 = 0; i0 < d.n - d.ne; i0 += ts)
+            for (int j0 = 0; j0 < d.n - d.ne; j0 += ts) L.tiles.push_back({q, i0, j0, 0});
         if (!root) {
             add_gather_tiles(L.gtiles, q, 0, d.n, 0, d.ne);     // separator columns, all rows
             add_gather_tiles(L.gtiles, q, 0, d.ne, d.ne, d.n);  // separator rows, boundary columns
@@ -889,7 +900,8 @@ void run_level(Engine& eng, const LevelSet& L, const DevLevel& D, const NodeDesc
                           eng.lane->stream);
         });
     eng.timed(root ? kRootSchur : kSchur, P * L.schur_flops, P * fes * L.schur_words, [&] {
-        launch_schur(nodes, D.tiles, int32_t(L.tiles.size()), kids, maps, pv, P, fsingle, root, !root, eng.lane->stream);
+        launch_schur(nodes, D.tiles, int32_t(L.tiles.size()), kids, maps, pv, P, fsingle, root, !root, L.schur_tile,
+                     eng.lane->stream);
     });
 }
 
@@ -1870,13 +1882,14 @@ int32_t batch_transmission(const cg_system* const* systems, const cg_sigma* cons
 void destroy_batch(cg_batch* b) { delete b; }
 
 namespace {
-cg_tuning to_c(const Tuning& t) { return {t.small_fronts, t.split, t.blocked_min_ne, t.unblocked}; }
+cg_tuning to_c(const Tuning& t) { return {t.small_fronts, t.split, t.blocked_min_ne, t.unblocked, t.schur_tile}; }
 Tuning from_c(const cg_tuning& c) {
     Tuning t;
     t.small_fronts = c.small_fronts;
     t.split = c.split;
     t.blocked_min_ne = c.blocked_min_ne;
     t.unblocked = c.unblocked;
+    t.schur_tile = c.schur_tile;
     return t;
 }
 }  // namespace
@@ -1892,7 +1905,7 @@ int32_t set_tuning(const cg_tuning* t) {
         return CG_OK;
     }
     if (t->small_fronts < 0 || t->small_fronts > 2 || t->split < 0 || t->split > 1 || t->blocked_min_ne < 0 ||
-        t->unblocked < 0 || t->unblocked > 1)
+        t->unblocked < 0 || t->unblocked > 1 || (t->schur_tile != 64 && t->schur_tile != 32 && t->schur_tile != 0))
         return CG_ERR_INVALID;
     const Tuning tu = from_c(*t);
     set_tuning_override(&tu);
@@ -1968,6 +1981,11 @@ int32_t autotune_batch(cg_batch* bt, cg_tuning* out, double* out_seconds) {
                 c.blocked_min_ne = bm;
                 consider(c);
             }
+            for (int ts : {0, 64, 32}) {
+                Tuning c = best;
+                c.schur_tile = ts;
+                consider(c);
+            }
         }
         apply(best);  // the winner stays in force, the batch on its plans (graphs captured anew)
     } catch (...) {
@@ -1994,7 +2012,7 @@ struct FrontJob {
     NodeDesc d{};
     std::vector<int32_t> lv{0};
     std::vector<Tile> tl;
-    int32_t schur_tiles = 0, trsm_tiles = 0;
+    int32_t schur_tiles = 0, trsm_tiles = 0, schur_tile = 64;
     LevelSet blocked;
     SetOffsets offsets{};
 };
@@ -2015,8 +2033,10 @@ FrontJob plan_front(int64_t n, int64_t ne, int64_t ne1, bool single) {
         j.path = FrontJob::kUnblocked;
         return j;
     }
-    for (int i0 = 0; i0 < n - ne; i0 += kTile)
-        for (int j0 = 0; j0 < n - ne; j0 += kTile) j.tl.push_back({0, i0, j0, 0});
+    j.schur_tile = current_tuning().schur_tile;
+    if (j.schur_tile == 0) j.schur_tile = n - ne <= kSchurTile32MaxNb ? 32 : 64;
+    for (int i0 = 0; i0 < n - ne; i0 += j.schur_tile)
+        for (int j0 = 0; j0 < n - ne; j0 += j.schur_tile) j.tl.push_back({0, i0, j0, 0});
     j.schur_tiles = int32_t(j.tl.size());
     if (ne > blocked_min_ne()) {
         j.path = FrontJob::kBlocked;
@@ -2062,7 +2082,7 @@ void launch_front_job(Engine& eng, const FrontJob& j, const NodeDesc* nodes, con
             launch_border(nodes, lv, 1, ne, nullptr, nullptr, pv, 1, tol, single, false, st);
             break;
     }
-    launch_schur(nodes, tl, j.schur_tiles, nullptr, nullptr, pv, 1, single, false, false, st);
+    launch_schur(nodes, tl, j.schur_tiles, nullptr, nullptr, pv, 1, single, false, false, j.schur_tile, st);
 }
 
 }  // namespace

@@ -1326,21 +1326,24 @@ __global__ void __launch_bounds__(256)
 }
 
 // ---------------------------------------------------------------------------------------------
-// Large fronts, step 2: S = F22 - L21 U12 (K = ne) over 64 x 64 tiles of the Schur block, one
-// workgroup per (tile, point), on MFMA. Each wave owns a 32 x 32 quarter (2 x 2 blocks of 16 x 16,
-// complex = 4 real MFMAs per block and K-step of 4); L21 and U12 are staged through LDS in
-// chunks of kChunk columns / rows. Fronts whose border failed are skipped. (The compiler keeps the
-// constant zero in 16-32 bytes of scratch here; the form without it, zero written as a literal,
-// schedules the K loop worse: 10 % slower on the BHZ 200 x 64 batch.)
+// Large fronts, step 2: S = F22 - L21 U12 (K = ne) over TS x TS tiles of the Schur block, one
+// workgroup per (tile, point), on MFMA: 64 x 64 tiles with four waves or 32 x 32 tiles with one
+// (the tuning's schur_tile). Each wave owns a 32 x 32 part (2 x 2 blocks of 16 x 16, complex = 4
+// real MFMAs per block and K-step of 4); L21 and U12 are staged through LDS in chunks of kChunk
+// columns / rows. Fronts whose border failed are skipped. (The compiler keeps the constant zero
+// in 16-32 bytes of scratch here; the form without it, zero written as a literal, schedules the K
+// loop worse: 10 % slower on the BHZ 200 x 64 batch.)
 constexpr int kChunk = 16, kTileLd = kTile + 1;
 
-template <class T, bool kRoot, bool kGather>
-__global__ void __launch_bounds__(256)
+template <class T, bool kRoot, bool kGather, int TS>
+__global__ void __launch_bounds__(TS == 64 ? 256 : 64)
     schur_kernel(const NodeDesc* __restrict__ nodes, const Tile* __restrict__ tiles, const KidDesc* __restrict__ kids,
                  const int32_t* __restrict__ maps, PointsView pv) {
+    static_assert(TS == 64 || TS == 32, "Schur tiles are 64 x 64 or 32 x 32");
     using M = Mfma<T>;
-    __shared__ T As[kChunk * kTileLd];  // As[k * kTileLd + i] = L21[i0 + i, kc + k]
-    __shared__ T Bs[kChunk * kTileLd];  // Bs[k * kTileLd + j] = U12[kc + k, j0 + j]
+    constexpr int NT = TS == 64 ? 256 : 64, kLd = TS + 1;
+    __shared__ T As[kChunk * kLd];  // As[k * kLd + i] = L21[i0 + i, kc + k]
+    __shared__ T Bs[kChunk * kLd];  // Bs[k * kLd + j] = U12[kc + k, j0 + j]
     const TilePoint tp = xcd_tile();
     const Tile tl = tiles[tp.tile];
     const NodeDesc nd = nodes[tl.node];
@@ -1363,14 +1366,14 @@ __global__ void __launch_bounds__(256)
     // The K loop in chunks of kChunk: the next chunk's L21 / U12 entries are loaded into registers
     // (kStage per thread and operand) while the MFMAs run on the current one from LDS. Clamped
     // loads and a select of the value (a select of addresses would go through flat).
-    constexpr int kStage = kTile * kChunk / 256;
+    constexpr int kStage = TS * kChunk / NT;
     T sa[kStage], sb[kStage];
     auto fetch = [&](int kc) {
 #pragma unroll
         for (int u = 0; u < kStage; ++u) {
-            const int e = tid + u * 256;
+            const int e = tid + u * NT;
             {
-                const int r = e % kTile, c = e / kTile, i = tl.i0 + r, k = kc + c;
+                const int r = e % TS, c = e / TS, i = tl.i0 + r, k = kc + c;
                 T v = F[(ne + min(i, nb - 1)) + int64_t(min(k, ne - 1)) * n];
                 if (!(i < nb && k < ne)) v = zero;
                 sa[u] = v;
@@ -1394,9 +1397,9 @@ __global__ void __launch_bounds__(256)
     for (int kc = 0; kc < ne; kc += kChunk) {
 #pragma unroll
         for (int u = 0; u < kStage; ++u) {
-            const int e = tid + u * 256;
-            As[(e / kTile) * kTileLd + e % kTile] = sa[u];
-            Bs[(e % kChunk) * kTileLd + e / kChunk] = sb[u];
+            const int e = tid + u * NT;
+            As[(e / TS) * kLd + e % TS] = sa[u];
+            Bs[(e % kChunk) * kLd + e / kChunk] = sb[u];
         }
         __syncthreads();
         if (kc + kChunk < ne) fetch(kc + kChunk);
@@ -1407,8 +1410,8 @@ __global__ void __launch_bounds__(256)
             T a[2], b[2];
 #pragma unroll
             for (int s = 0; s < 2; ++s) {
-                a[s] = Bs[k * kTileLd + wj + 16 * s + lane % 16];  // U12^T operand
-                b[s] = As[k * kTileLd + wi + 16 * s + lane % 16];  // L21^T operand
+                a[s] = Bs[k * kLd + wj + 16 * s + lane % 16];  // U12^T operand
+                b[s] = As[k * kLd + wi + 16 * s + lane % 16];  // L21^T operand
             }
 #pragma unroll
             for (int x = 0; x < 2; ++x)
@@ -2304,7 +2307,8 @@ Tuning tuning_override;
 
 /// The defaults and the CAROLI_* variables: CAROLI_SMALL_FRONTS=lds|reg, CAROLI_FRONT_KERNEL=
 /// blocked (every large front and the root blocked) | border (no f11 / trsm split, none blocked) |
-/// unblocked, CAROLI_BLOCKED_MIN_NE=<separator size above which fronts go blocked>.
+/// unblocked, CAROLI_BLOCKED_MIN_NE=<separator size above which fronts go blocked>,
+/// CAROLI_SCHUR_TILE=64|32 (unset: by level).
 Tuning env_tuning() {
     Tuning t;
     t.blocked_min_ne = kBlockedMinNe;
@@ -2325,6 +2329,10 @@ Tuning env_tuning() {
     }
     if (const char* v = std::getenv("CAROLI_BLOCKED_MIN_NE"))
         if (*v) t.blocked_min_ne = std::max(0, std::atoi(v));
+    if (const char* v = std::getenv("CAROLI_SCHUR_TILE")) {
+        if (std::atoi(v) == 32) t.schur_tile = 32;
+        if (std::atoi(v) == 64) t.schur_tile = 64;
+    }
     return t;
 }
 }  // namespace
@@ -2519,16 +2527,21 @@ void launch_trsm(const NodeDesc* nodes, const Tile* tiles, int32_t count, int32_
 }
 
 void launch_schur(const NodeDesc* nodes, const Tile* tiles, int32_t count, const KidDesc* kids, const int32_t* maps,
-                  const PointsView& pv, int32_t points, bool single, bool root, bool gather, hipStream_t st) {
+                  const PointsView& pv, int32_t points, bool single, bool root, bool gather, int tile, hipStream_t st) {
     if (count <= 0 || points <= 0) return;
-    const dim3 g = grid2(count, points), b(256);
+    if (tile != 64 && tile != 32) throw std::runtime_error("launch_schur: tiles are 64 or 32 wide");
+    const dim3 g = grid2(count, points), b(tile == 64 ? 256 : 64);
     auto go = [&](auto kernel) { hipLaunchKernelGGL(kernel, g, b, 0, st, nodes, tiles, kids, maps, pv); };
-    if (root)
-        go(schur_kernel<double2, true, false>);
-    else if (single)
-        gather ? go(schur_kernel<float2, false, true>) : go(schur_kernel<float2, false, false>);
-    else
-        gather ? go(schur_kernel<double2, false, true>) : go(schur_kernel<double2, false, false>);
+    auto pick = [&](auto ts) {
+        constexpr int TS = decltype(ts)::value;
+        if (root)
+            go(schur_kernel<double2, true, false, TS>);
+        else if (single)
+            gather ? go(schur_kernel<float2, false, true, TS>) : go(schur_kernel<float2, false, false, TS>);
+        else
+            gather ? go(schur_kernel<double2, false, true, TS>) : go(schur_kernel<double2, false, false, TS>);
+    };
+    tile == 64 ? pick(std::integral_constant<int, 64>{}) : pick(std::integral_constant<int, 32>{});
 }
 
 void launch_factor_unblocked(const NodeDesc* nodes, const int32_t* level, int32_t count, const PointsView& pv,

@@ -151,7 +151,7 @@ void launch_border(const NodeDesc* nodes, const int32_t* level, int32_t count, i
                    const int32_t* diag, const PointsView& pv, int32_t points, double tol, bool single, bool root,
                    hipStream_t st);
 void launch_schur(const NodeDesc* nodes, const Tile* tiles, int32_t count, const KidDesc* kids, const int32_t* maps,
-                  const PointsView& pv, int32_t points, bool single, bool root, bool gather, hipStream_t st);
+                  const PointsView& pv, int32_t points, bool single, bool root, bool gather, int tile, hipStream_t st);
 /// Large fronts with ne <= kF11Lds: the LU of F11 in LDS (after -H and z), then the triangular
 /// solves over `tiles` (pad 0: kTrsmSpan columns of U12 from j0; pad 1: kTrsmSpan rows of L21
 /// from i0).
@@ -161,14 +161,19 @@ void launch_trsm(const NodeDesc* nodes, const Tile* tiles, int32_t count, int32_
                  int32_t points, bool single, hipStream_t st);
 /// Kernel choices per front class, process-wide. The CAROLI_* variables give the defaults;
 /// cg_set_tuning and cg_batch_autotune override them (set_tuning_override(nullptr) goes back).
+/// With the tuning's schur_tile 0, a level whose Schur blocks have at most this many rows takes
+/// 32 x 32 tiles, others 64 x 64.
+constexpr int kSchurTile32MaxNb = 256;
 struct Tuning {
     int32_t small_fronts = 0;      // 0 front_border_kernel, 1 front_lds_kernel, 2 front_reg_kernel (n <= 64)
     int32_t split = 1;             // separators up to kF11Lds by f11_kernel + trsm_kernel, else border_kernel
     int32_t blocked_min_ne = 128;  // separators above this many variables by the blocked kernels
     int32_t unblocked = 0;         // every large front by factor_global_kernel
+    int32_t schur_tile = 0;        // Schur tiles of 64 x 64 (four waves), 32 x 32 (one wave) or by level (0)
     /// The fields that decide a plan's kernel lists (small_fronts is read at each launch).
     bool same_plan(const Tuning& o) const {
-        return split == o.split && blocked_min_ne == o.blocked_min_ne && unblocked == o.unblocked;
+        return split == o.split && blocked_min_ne == o.blocked_min_ne && unblocked == o.unblocked &&
+               schur_tile == o.schur_tile;
     }
 };
 Tuning current_tuning();

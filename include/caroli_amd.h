@@ -236,13 +236,16 @@ typedef struct cg_tuning {
                              spread over many workgroups; 0: one workgroup per front */
   int32_t blocked_min_ne; /* separators above this many variables by the blocked kernels */
   int32_t unblocked;      /* 1: every large front by rank-1 updates in HBM */
+  int32_t schur_tile;     /* Schur-complement MFMA tiles: 64 (64 x 64, four waves), 32 (32 x 32,
+                             one wave) or 0 (by level: 32 where its Schur blocks have at most
+                             256 rows) */
 } cg_tuning;
 /* The tuning in force, or set one (NULL: back to the defaults and the variables). Plans built
  * afterwards follow it; a system planned under another is re-planned at its next use. */
 int32_t cg_get_tuning(cg_tuning* out);
 int32_t cg_set_tuning(const cg_tuning* t);
 /* Autotuner (no counterpart in the reference): times whole eager sweeps of the batch's points
- * (the faster of two per candidate, after one that settles) under candidate tunings, by coordinate descent over the three choices from the one in force (a
+ * (the faster of two per candidate, after one that settles) under candidate tunings, by coordinate descent over the four choices from the one in force (a
  * candidate must be 2 % faster to win), and leaves the fastest in force with the batch re-planned
  * under it. out: the choice; out_seconds: its sweep time (either may be NULL). */
 int32_t cg_batch_autotune(cg_batch* batch, cg_tuning* out, double* out_seconds);

@@ -61,7 +61,8 @@ def test_autotuned_batch_keeps_parity(restore_tuning):
         assert np.abs(t - ref).max() <= TOL * (1 + np.abs(ref).max()), (run, np.abs(t - ref).max())
     # A hand-set choice that classifies the separators differently: the systems planned above are
     # re-planned at their next use.
-    forced = {"small_fronts": 1, "split": 0, "blocked_min_ne": 64, "unblocked": 0}
+    forced = {"small_fronts": 1, "split": 0, "blocked_min_ne": 64, "unblocked": 0,
+              "schur_tile": 32 if choice["schur_tile"] != 32 else 64}
     if forced["split"] == choice["split"] and forced["blocked_min_ne"] == choice["blocked_min_ne"]:
         forced["blocked_min_ne"] = 1 << 30
     ca.set_tuning(forced)
@@ -92,3 +93,31 @@ def test_autotune_mixed_precision_batch(restore_tuning):
     assert (st == 0).all(), st
     assert np.abs(t - ref).max() <= 1e-8, np.abs(t - ref).max()
     assert stats.refinement_iters > 0
+
+
+@pytest.mark.parametrize("precision", [ca.Precision.Double, ca.Precision.Mixed])
+def test_schur_tile_shapes_agree(precision, restore_tuning):
+    """32 x 32 Schur tiles (one wave) against 64 x 64 (four waves) on whole systems: BHZ 60 x 24
+    (Schur blocks of 40 to 300 rows, partial tiles of both widths, the bordered root) and graphene
+    zigzag 64 x 16 cells; complex128 to the oracle's bar, mixed to 1e-8, and the two tilings
+    within rounding of each other."""
+    bhz = ca.NestedTwoTerminal.preset(ca.Model.BHZ, [60, 24, 0], disorder=2, strength=0.4, concentration=0.2, seed=9)
+    gra = ca.NestedTwoTerminal.preset(ca.Model.Graphene, [64, 16, 0], disorder=1, strength=1.0, seed=4)
+    pts_s, pts_g = [], []
+    for s, es in ((bhz, (-0.1, 0.07)), (gra, (0.2, 0.45))):
+        for e in es:
+            pts_s.append(s)
+            pts_g.append(s.self_energies(e))
+    ref = oracle_ts(pts_s, pts_g)
+    out = {}
+    for tile in (64, 32, 0):
+        ca.set_tuning(schur_tile=tile)
+        say(f"[gpu] schur tiles {tile}: BHZ 60x24 + graphene 64x16, precision {precision}")
+        t, st = ca.batch_transmission(pts_s, pts_g, precision)
+        assert (st == 0).all(), st
+        bar = TOL * (1 + np.abs(ref).max()) if precision == ca.Precision.Double else 1e-8
+        assert np.abs(t - ref).max() <= bar, (tile, np.abs(t - ref).max())
+        out[tile] = t
+    if precision == ca.Precision.Double:
+        assert np.abs(out[32] - out[64]).max() <= 1e-12 * (1 + np.abs(ref).max())
+        assert np.abs(out[0] - out[64]).max() <= 1e-12 * (1 + np.abs(ref).max())

@@ -4,7 +4,7 @@ S = F22 - F21 F11^-1 F12 for random fronts of the sizes the trees produce, throu
 path: LDS-resident fronts (n <= 64), the separator LU in LDS with the triangular-solve tiles
 (ne <= 64), the panel-by-panel border kernel, the blocked path (panel, row and update launches per
 64 columns; the default above 128 separator variables) and the unblocked fallback; then the MFMA
-Schur tiles. Complex f64 to a relative 1e-11, complex f32 to 1e-4; a singular leading block is refused
+Schur tiles (32 x 32 or 64 x 64). Complex f64 to a relative 1e-11, complex f32 to 1e-4; a singular leading block is refused
 (field.rs:199-208).
 """
 import ctypes as C
@@ -55,6 +55,23 @@ def test_front_schur_double(n, ne, ne1, kernel, monkeypatch):
         return  # nothing left: only the pivot test ran
     ref = numpy_schur(f, ne)
     assert np.abs(s - ref).max() <= 1e-11 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("tile", ["32", "64"])
+@pytest.mark.parametrize("kernel", ["default", "blocked"])
+@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("n,ne", [(90, 40), (100, 37), (200, 64), (384, 128), (530, 200), (700, 300)])
+def test_front_schur_tiles(n, ne, precision, kernel, tile, monkeypatch):
+    """Both Schur tile shapes (32 x 32 with one wave, 64 x 64 with four) on Schur blocks of 50 to
+    400 rows, whole and partial tiles."""
+    monkeypatch.setenv("CAROLI_FRONT_KERNEL", kernel)
+    monkeypatch.setenv("CAROLI_SCHUR_TILE", tile)
+    f = front(n, 5 * n + ne)
+    print(f"[gpu] front tile {tile} n={n} ne={ne} precision {precision} {kernel}", flush=True)
+    s, st = gpu_schur(f, ne, ne, precision)
+    assert st == 0
+    ref = numpy_schur(f, ne)
+    assert np.abs(s - ref).max() <= (1e-11 if precision == 0 else 1e-4) * np.abs(ref).max()
 
 
 @pytest.mark.parametrize("n,ne", [(12, 5), (90, 40), (200, 64), (384, 128), (500, 200)])

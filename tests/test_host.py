@@ -400,22 +400,27 @@ def test_system_create_refuses_malformed_arrays():
 def test_tuning_defaults_variables_and_override(monkeypatch):
     """cg_get_tuning / cg_set_tuning: the defaults, the CAROLI_* variables, an override over them,
     refused values, and the way back (no device needed)."""
-    for v in ("CAROLI_SMALL_FRONTS", "CAROLI_FRONT_KERNEL", "CAROLI_BLOCKED_MIN_NE"):
+    for v in ("CAROLI_SMALL_FRONTS", "CAROLI_FRONT_KERNEL", "CAROLI_BLOCKED_MIN_NE", "CAROLI_SCHUR_TILE"):
         monkeypatch.delenv(v, raising=False)
     ca.set_tuning(None)
     try:
-        assert ca.get_tuning() == {"small_fronts": 0, "split": 1, "blocked_min_ne": 128, "unblocked": 0}
+        assert ca.get_tuning() == {"small_fronts": 0, "split": 1, "blocked_min_ne": 128, "unblocked": 0,
+                                   "schur_tile": 0}
         monkeypatch.setenv("CAROLI_FRONT_KERNEL", "border")
         monkeypatch.setenv("CAROLI_SMALL_FRONTS", "reg")
-        assert ca.get_tuning() == {"small_fronts": 2, "split": 0, "blocked_min_ne": 1 << 30, "unblocked": 0}
+        monkeypatch.setenv("CAROLI_SCHUR_TILE", "32")
+        assert ca.get_tuning() == {"small_fronts": 2, "split": 0, "blocked_min_ne": 1 << 30, "unblocked": 0,
+                                   "schur_tile": 32}
+        monkeypatch.delenv("CAROLI_SCHUR_TILE")
         monkeypatch.setenv("CAROLI_FRONT_KERNEL", "blocked")
         monkeypatch.setenv("CAROLI_BLOCKED_MIN_NE", "96")  # read after CAROLI_FRONT_KERNEL
         assert ca.get_tuning()["blocked_min_ne"] == 96
         ca.set_tuning(small_fronts=1, blocked_min_ne=256)
-        assert ca.get_tuning() == {"small_fronts": 1, "split": 1, "blocked_min_ne": 256, "unblocked": 0}
+        assert ca.get_tuning() == {"small_fronts": 1, "split": 1, "blocked_min_ne": 256, "unblocked": 0,
+                                   "schur_tile": 0}
         monkeypatch.setenv("CAROLI_SMALL_FRONTS", "lds")  # the override wins over the variables
         assert ca.get_tuning()["small_fronts"] == 1
-        for bad in ({"small_fronts": 3}, {"split": 2}, {"blocked_min_ne": -1}, {"unblocked": -1}):
+        for bad in ({"small_fronts": 3}, {"split": 2}, {"blocked_min_ne": -1}, {"unblocked": -1}, {"schur_tile": 48}):
             with pytest.raises(ca.GreenError):
                 ca.set_tuning(**bad)
         assert ca.get_tuning()["blocked_min_ne"] == 256
@@ -424,7 +429,7 @@ def test_tuning_defaults_variables_and_override(monkeypatch):
     finally:
         ca.set_tuning(None)
     monkeypatch.delenv("CAROLI_BLOCKED_MIN_NE")
-    assert ca.get_tuning() == {"small_fronts": 1, "split": 1, "blocked_min_ne": 0, "unblocked": 0}
+    assert ca.get_tuning() == {"small_fronts": 1, "split": 1, "blocked_min_ne": 0, "unblocked": 0, "schur_tile": 0}
 
 
 def _oracle_states(s, e):
