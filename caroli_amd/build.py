@@ -39,7 +39,8 @@ def build(verbose=False):
         list(ex.map(_run, [j for j in jobs]))
     objs = [os.path.join(OBJ, f + ".o") for f in HOST + DEVICE]
     tmp = OUT + ".tmp"
-    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs, "-lpthread"])
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs, "-L/opt/rocm/lib", "-lrocsolver",
+          "-lrocblas", "-Wl,-rpath,/opt/rocm/lib", "-lpthread"])
     os.replace(tmp, OUT)
     if verbose:
         print("built", OUT)

@@ -458,6 +458,27 @@ int32_t cg_work(const cg_system* s, double* flops, double* bytes) {
 
 // NestedTwoTerminal::self_energies (nested.rs:652-676): per block, the left lead's Sigma_L and
 // the right lead's Sigma_R (the left modes serve both when the leads are equal).
+// The scalar-coupling leads go to the GPU (leads.hip) when a device is present, unless
+// CAROLI_SIGMA_HOST=1 keeps them on the host.
+static bool sigma_host_forced() {
+    const char* v = std::getenv("CAROLI_SIGMA_HOST");
+    return v && std::strcmp(v, "1") == 0;
+}
+
+static bool cell_on_gpu(const ModeSolver& m) {
+    if (!m.cell()) return false;
+    return !sigma_host_forced() && gpu::lead_device_available();
+}
+
+// Cells of at least this many orbitals take the GPU eigensolver when the leads go to the GPU.
+constexpr int kGpuEigenMin = 128;
+
+// The cell eigensolver of a lead: rocSOLVER on the GPU for large cells, the host's otherwise.
+static HermitianEigen cell_eigen(const CMat& h0) {
+    return h0.rows >= kGpuEigenMin && !sigma_host_forced() && gpu::lead_device_available() ? gpu::hermitian_eigen
+                                                                                           : nullptr;
+}
+
 void ensure_modes(const cg_system* s) {
     std::lock_guard<std::mutex> g(s->mu);
     if (!s->left_modes.empty() || s->sym.blocks.empty()) return;
@@ -465,18 +486,9 @@ void ensure_modes(const cg_system* s) {
     s->right_modes.resize(s->sym.blocks.size());
     for (size_t b = 0; b < s->sym.blocks.size(); ++b) {
         const auto& blk = s->sym.blocks[b];
-        s->left_modes[b].emplace(blk.left_h00, blk.left_h01);
-        if (!blk.same_lead) s->right_modes[b].emplace(blk.right_h00, blk.right_h01);
+        s->left_modes[b].emplace(blk.left_h00, blk.left_h01, cell_eigen(blk.left_h00));
+        if (!blk.same_lead) s->right_modes[b].emplace(blk.right_h00, blk.right_h01, cell_eigen(blk.right_h00));
     }
-}
-
-// The scalar-coupling leads go to the GPU (leads.hip) when a device is present, unless
-// CAROLI_SIGMA_HOST=1 keeps them on the host.
-static bool cell_on_gpu(const ModeSolver& m) {
-    if (!m.cell()) return false;
-    const char* v = std::getenv("CAROLI_SIGMA_HOST");
-    if (v && std::strcmp(v, "1") == 0) return false;
-    return gpu::lead_device_available();
 }
 
 // Gamma = W W^H of a cell self-energy Sigma = Y diag(d) Y^H: Gamma = Y diag(-2 Im d) Y^H, so W is
@@ -1181,7 +1193,7 @@ int32_t multi_sigmas(const cg_system* s, double energy, double eta, const std::v
                     if (l.orbitals.empty()) continue;
                     for (size_t q = 0; q < p && !v[p]; ++q)
                         if (v[q] && blk.leads[q].h00 == l.h00 && blk.leads[q].h01 == l.h01) v[p] = v[q];
-                    if (!v[p]) v[p] = std::make_shared<ModeSolver>(l.h00, l.h01);
+                    if (!v[p]) v[p] = std::make_shared<ModeSolver>(l.h00, l.h01, cell_eigen(l.h00));
                 }
             }
         }

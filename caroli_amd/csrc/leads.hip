@@ -1,12 +1,14 @@
-// Lead self-energies on the GPU, FP64: the per-energy part of the definite-coupling "cell" case of
-// the mode solver (crates/caroli-green/src/modes.rs:621-710 with P = c 1, sigma_left / sigma_right
-// :494-511). With the cell Hamiltonian's eigenvectors Y (cached once per lead on the host), every
-// self-energy of the lead is Sigma = Y diag(d) Y^H, d the per-mode factors of that energy: a dense
-// complex GEMM of the cell size (L^2 = 2,304 for a cubic lead of cross-section 48), here on FP64
-// MFMA. The host keeps the O(n) per-mode work and the one-off eigendecomposition.
+// Lead self-energies on the GPU, FP64: the definite-coupling "cell" case of the mode solver
+// (crates/caroli-green/src/modes.rs:621-710 with P = c 1, sigma_left / sigma_right :494-511).
+// With the cell Hamiltonian's eigenvectors Y (one eigendecomposition per lead, on the GPU by
+// rocSOLVER's Jacobi zheevj), every self-energy of the lead is Sigma = Y diag(d) Y^H,
+// d the per-mode factors of that energy: a dense complex GEMM of the cell size (L^2 = 2,304 for a
+// cubic lead of cross-section 48), here on FP64 MFMA. The host keeps the O(n) per-mode work.
 #include <hip/hip_runtime.h>
+#include <rocsolver/rocsolver.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -14,6 +16,7 @@
 #include <string>
 #include <vector>
 
+#include "dense.hpp"
 #include "leads.hpp"
 
 namespace caroli::gpu {
@@ -31,6 +34,7 @@ typedef double f64x4 __attribute__((ext_vector_type(4)));
 constexpr int kT = 64;   // output tile edge
 constexpr int kK = 16;   // K chunk staged in LDS
 constexpr int kNT = 256; // four waves, 32 x 32 of the tile each
+constexpr int kJacobiSweeps = 100;  // limit of the cell eigensolver's Jacobi sweeps
 
 // out[z] = Y diag(d[z]) Y^H for z = blockIdx.z: n x n complex f64, column-major, leading dimension
 // n. Tile (i0, j0) = 64 * (blockIdx.x, blockIdx.y). Per K chunk the workgroup stages Z = Y diag(d)
@@ -119,6 +123,7 @@ struct LeadCache {
     size_t d_bytes = 0, out_bytes = 0;
     uint64_t clock = 0;
     bool ok = false;
+    rocblas_handle blas = nullptr;  // created at the first eigendecomposition
     LeadCache() {
         int count = 0;
         if (hipGetDeviceCount(&count) != hipSuccess || count == 0) return;
@@ -183,6 +188,60 @@ void cell_sigmas(uint64_t lead_id, const std::complex<double>* y, int n, const s
     LEAD_CHECK(hipGetLastError());
     LEAD_CHECK(hipMemcpyAsync(out, dout, size_t(count) * nn * sizeof(double2), hipMemcpyDeviceToHost, c.stream));
     LEAD_CHECK(hipStreamSynchronize(c.stream));
+}
+
+bool hermitian_eigen(const CMat& a, std::vector<double>& w, CMat& v) {
+    LeadCache& c = LeadCache::get();
+    if (!c.ok) throw std::runtime_error("no HIP device for the lead eigendecomposition");
+    const int n = a.rows;
+    w.assign(size_t(n), 0.0);
+    v = CMat(n, n);
+    if (n == 0) return true;
+    // The Hermitian part, as the host solver takes it.
+    for (int j = 0; j < n; ++j)
+        for (int i = 0; i < n; ++i) v(i, j) = (a(i, j) + std::conj(a(j, i))) * 0.5;
+    bool good = false;
+    {
+        std::lock_guard<std::mutex> g(c.mu);
+        if (!c.blas) {
+            if (rocblas_create_handle(&c.blas) != rocblas_status_success) throw std::runtime_error("rocblas_create_handle");
+            if (rocblas_set_stream(c.blas, c.stream) != rocblas_status_success) throw std::runtime_error("rocblas_set_stream");
+        }
+        const size_t bytes = size_t(n) * size_t(n) * sizeof(double2);
+        double2* da = nullptr;
+        double *dw = nullptr, *dres = nullptr;
+        rocblas_int *dinfo = nullptr, *dsweeps = nullptr;
+        LEAD_CHECK(hipMalloc(reinterpret_cast<void**>(&da), bytes));
+        LEAD_CHECK(hipMalloc(reinterpret_cast<void**>(&dw), size_t(n) * sizeof(double)));
+        LEAD_CHECK(hipMalloc(reinterpret_cast<void**>(&dres), sizeof(double)));
+        LEAD_CHECK(hipMalloc(reinterpret_cast<void**>(&dinfo), sizeof(rocblas_int)));
+        LEAD_CHECK(hipMalloc(reinterpret_cast<void**>(&dsweeps), sizeof(rocblas_int)));
+        rocblas_int info = -1;
+        LEAD_CHECK(hipMemcpyAsync(da, v.a.data(), bytes, hipMemcpyHostToDevice, c.stream));
+        // Jacobi sweeps to machine precision (abstol 0), eigenvalues ascending. (The
+        // divide-and-conquer zheevd gave non-finite eigenvectors for cells with degenerate
+        // eigenvalues, as every clean lattice cross-section has.)
+        const rocblas_status st =
+            rocsolver_zheevj(c.blas, rocblas_esort_ascending, rocblas_evect_original, rocblas_fill_lower, n,
+                             reinterpret_cast<rocblas_double_complex*>(da), n, 0.0, dres, kJacobiSweeps, dsweeps, dw,
+                             dinfo);
+        if (st == rocblas_status_success) {
+            LEAD_CHECK(hipMemcpyAsync(v.a.data(), da, bytes, hipMemcpyDeviceToHost, c.stream));
+            LEAD_CHECK(hipMemcpyAsync(w.data(), dw, size_t(n) * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+            LEAD_CHECK(hipMemcpyAsync(&info, dinfo, sizeof(rocblas_int), hipMemcpyDeviceToHost, c.stream));
+        }
+        LEAD_CHECK(hipStreamSynchronize(c.stream));
+        (void)hipFree(da);
+        (void)hipFree(dw);
+        (void)hipFree(dres);
+        (void)hipFree(dinfo);
+        (void)hipFree(dsweeps);
+        good = st == rocblas_status_success && info == 0;
+        for (size_t k = 0; good && k < w.size(); ++k) good = std::isfinite(w[k]);
+        for (size_t k = 0; good && k < v.a.size(); ++k) good = std::isfinite(v.a[k].real()) && std::isfinite(v.a[k].imag());
+    }
+    // Not converged or not finite: the host's eigensolver.
+    return good || dense::hermitian_eigen(a, w, v);
 }
 
 }  // namespace caroli::gpu

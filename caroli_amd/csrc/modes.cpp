@@ -439,7 +439,8 @@ std::atomic<uint64_t> next_solver_id{1};
 }
 
 // DefiniteCoupling::new (modes.rs:575-618)
-ModeSolver::ModeSolver(const CMat& h0, const CMat& h1) : id_(next_solver_id++), h0_(h0), h1_(h1) {
+ModeSolver::ModeSolver(const CMat& h0, const CMat& h1, HermitianEigen eigen)
+    : id_(next_solver_id++), h0_(h0), h1_(h1) {
     const int n = h0.rows;
     auto hermitian = [&](const CMat& m) {
         double asym = 0.0;
@@ -475,7 +476,7 @@ ModeSolver::ModeSolver(const CMat& h0, const CMat& h1) : id_(next_solver_id++), 
             CMat h(n, n);
             for (int j = 0; j < n; ++j)
                 for (int i = 0; i < n; ++i) h(i, j) = (h0(i, j) + std::conj(h0(j, i))) * 0.5;
-            if (dense::hermitian_eigen(h, d.theta, d.y)) {
+            if ((eigen ? eigen : dense::hermitian_eigen)(h, d.theta, d.y)) {
                 d.cell = true;
                 d.c = c;
                 d.l = CMat(n, n);

@@ -58,9 +58,13 @@ struct CellModes {
 };
 
 /// The energy-independent part of a lead's mode problem (modes.rs:519-538).
+/// A Hermitian eigensolver (eigenvalues ascending, unitary eigenvectors; false: no convergence).
+using HermitianEigen = bool (*)(const CMat& a, std::vector<double>& w, CMat& v);
+
 class ModeSolver {
    public:
-    ModeSolver(const CMat& h0, const CMat& h1);
+    /// eigen: the solver of the scalar-coupling cell's eigendecomposition (the host's when null).
+    ModeSolver(const CMat& h0, const CMat& h1, HermitianEigen eigen = nullptr);
     /// Modes at the real energy `e`, shifted off band edges when needed.
     Status modes_at(double e, LeadModes& out) const;
     /// Modes at a complex energy z (modes.rs:531-537): the definite fast path only on the real

@@ -4,6 +4,8 @@ host mode solver, and the uneven cubic batch at its large sizes against committe
 - Sigma_L, Sigma_R from the GPU (Y diag(d) Y^H on FP64 MFMA) equal the host's to 1e-12 of their
   size on cubic and square leads; Gamma = W W^H with W from the propagating eigenvectors; the
   channel counts agree; T through either self-energies agrees to 1e-12.
+- Cells of 128 orbitals or more take the eigendecomposition on the GPU too (rocSOLVER zheevj):
+  Sigma and T against a twin system whose leads stay on the host, to 1e-10.
 - Cubic Anderson model W = 16.5, 2L x L x L at L = 32, 40, 48 (separators up to 2,304 variables,
   roots of 2L^2 = 4,608 kept orbitals): GPU complex128 T against the double oracle values in
   tests/golden/cubic_large.json (written on the CPU by tests/golden/make_transmission_fixtures.py,
@@ -52,6 +54,31 @@ def test_gpu_cell_self_energies_match_host(model, ext, energies):
         t1 = s.transmission_with(g_gpu)
         t2 = s.transmission_with(g_host)
         assert abs(t1 - t2) <= 1e-12 * (1 + abs(t2)), (t1, t2)
+
+
+@pytest.mark.parametrize("ext,energies", [([24, 12, 12], [-1.7, 0.45]), ([32, 16, 16], [0.0, 2.5])])
+def test_gpu_cell_eigendecomposition_matches_host(ext, energies):
+    """Cells of 144 and 256 orbitals take the GPU eigensolver (rocSOLVER zheevj); a twin system
+    built under CAROLI_SIGMA_HOST=1 takes the host's (Householder + implicit QL) and computes Sigma
+    on the host. The clean cubic cell has degenerate eigenvalues, so the two eigenvector bases
+    differ; Sigma, Gamma, the channel counts and T do not."""
+    s_gpu = ca.NestedTwoTerminal.preset(ca.Model.Cubic, ext, disorder=1, strength=2.0, seed=3)
+    s_host = ca.NestedTwoTerminal.preset(ca.Model.Cubic, ext, disorder=1, strength=2.0, seed=3)
+    for e in energies:
+        t0 = time.perf_counter()
+        g_gpu = s_gpu.self_energies(e)
+        t_gpu = time.perf_counter() - t0
+        g_host = host_sigma(s_host, e)
+        sl1, sr1, cl1, cr1 = g_gpu.block(0)
+        sl2, sr2, cl2, cr2 = g_host.block(0)
+        scale = max(np.abs(sl2).max(), np.abs(sr2).max())
+        err = max(np.abs(sl1 - sl2).max(), np.abs(sr1 - sr2).max()) / scale
+        t1, t2 = s_gpu.transmission_with(g_gpu), s_host.transmission_with(g_host)
+        say(f"[gpu] cell eigensolver cubic {ext} E={e}: channels {cl1}/{cr1}, |dSigma| / |Sigma| {err:.1e}, "
+            f"|dT| {abs(t1 - t2):.1e}, GPU Sigma {t_gpu:.3f} s")
+        assert (cl1, cr1) == (cl2, cr2)
+        assert err <= 1e-10
+        assert abs(t1 - t2) <= 1e-10 * (1 + abs(t2)), (t1, t2)
 
 
 def test_gpu_cell_gamma_factor():
