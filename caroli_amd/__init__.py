@@ -504,8 +504,8 @@ def kernel_timing(enable=True):
 
 def kernel_stats():
     """Per kernel family: dict name -> (launches, seconds, algorithmic flops, algorithmic bytes)."""
-    buf = (capi.KernelStat * 16)()
-    n = capi.cg_kernel_stats(buf, 16)
+    buf = (capi.KernelStat * 32)()
+    n = capi.cg_kernel_stats(buf, 32)
     return {buf[i].name.decode(): (buf[i].launches, buf[i].seconds, buf[i].flops, buf[i].bytes) for i in range(n)}
 
 

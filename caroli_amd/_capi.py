@@ -27,6 +27,7 @@ class BatchStats(C.Structure):
         ("refinement_iters", C.c_int64),
         ("max_refine_resid", C.c_double),
         ("recomputed_points", C.c_int64),
+        ("goal_points", C.c_int64),
     ]
 
 

@@ -43,6 +43,11 @@ __device__ __forceinline__ T csub(T a, T b) {
     return a;
 }
 template <class T>
+__device__ __forceinline__ T cconj(T a) {
+    a.y = -a.y;
+    return a;
+}
+template <class T>
 __device__ __forceinline__ double cabs(T a) {
     return hypot(double(a.x), double(a.y));
 }

@@ -573,13 +573,17 @@ enum Family {
     kBlkPanel,
     kBlkRow,
     kBlkUpdate,
+    kSolveT,
+    kRefineVec,
+    kSolveKept,
     kFamilies
 };
 const char* const kFamilyNames[kFamilies] = {"front_lds_kernel",    "gather_kernel",         "assemble_kernel",
                                              "f11_kernel",          "trsm_kernel",           "border_kernel",
                                              "schur_kernel",        "assemble_kernel<root>", "border_kernel<root>",
                                              "schur_kernel<root>",  "trace_kernel",          "solve_kernels",
-                                             "blocked_panel_kernel", "blocked_row_kernel",   "blocked_update_kernel"};
+                                             "blocked_panel_kernel", "blocked_row_kernel",   "blocked_update_kernel",
+                                             "solve_t_kernels",      "refine_vector_kernels", "solve_kept_kernels"};
 
 /// A stream with the device buffers of one sweep in flight: the arenas, pivots, statuses, results
 /// and the trace scratch. Several lanes let the sweeps of several launch units (spin blocks, halves
@@ -607,13 +611,14 @@ struct Pinned {
     }
 };
 
-/// The refinement's vectors of one launch unit, kept in its lane between runs: W_R, the solution
-/// x, the right-hand side b, the correction d, the per-point maxima, x on the left slice, the list
-/// of iterating points; and their pinned host ends.
+/// The refinement's vectors of one launch unit, kept in its lane between runs: the solution x, the
+/// right-hand side b, the residual and correction (d, r, used in turn), the dual
+/// solution phi and its residual s, the per-point maxima and estimates (and their partial sums),
+/// the list of iterating points, the update vectors; and the pinned host ends.
 struct RefineBufs {
-    DevBuf w, x, b, d, maxes, xl, act, upd;
+    DevBuf x, b, d, r, phi, s, maxes, goal, part, act, upd;
     Pinned<int32_t> h_act;
-    Pinned<double> h_maxes;
+    Pinned<double> h_maxes, h_goal, h_goal0;
 };
 
 struct Lane {
@@ -1175,7 +1180,7 @@ void solve(Engine& eng, const HostPlan& hp, const DevPlan& dp, const PointsView&
     sv.maps = dp.maps.as<int32_t>();
     sv.gather = kept_only ? 0 : 1;
     if (!kept_only && !sv.upd) throw DeviceError("a forward solve needs its update vectors");
-    eng.timed(kSolve, 0.0, 0.0, [&] {
+    eng.timed(transpose ? kSolveT : kept_only ? kSolveKept : kSolve, 0.0, 0.0, [&] {
         if (!kept_only)
             for (size_t l = 0; l < dp.all.size(); ++l)
                 launch_solve_forward(nodes, lv + dp.all[l].first, dp.all[l].second, dp.all_max_n[l], pv, sv, P, single,
@@ -1188,13 +1193,29 @@ void solve(Engine& eng, const HostPlan& hp, const DevPlan& dp, const PointsView&
     HIP_CHECK(hipGetLastError());
 }
 
-/// Refinement: a point is done when its relative correction r_k = max|d| / max|x| is below
-/// kRefineTol, or when the error left after the update, estimated as r_k * (r_k / r_{k-1}) (the
-/// next correction at the observed contraction rate), is below kRefineEst with r_k below
-/// kRefineEstFrom.
+/// Refinement of x = G E_R W_R with the complex f32 factors M and FP64 residuals: x0 = M^-1 b,
+/// then x_{k+1} = x_k + d_k with d_k = M^-1 (b - A x_k). T = |t|_F^2 with the transmission
+/// amplitudes t = W_L^H x_L, formed on the device per iteration. A point is done when its
+/// relative correction r_k = max|d_k| / max|x_{k+1}| is below kRefineTol, or below kRefineEstFrom
+/// with the error estimated at the observed contraction, r_k^2 / r_{k-1}, below kRefineEst.
+///
+/// Optionally (refine_goal()) the dual solution phi0 = M^-T g, g = E_L conj(W_L) (A^T phi = g),
+/// is solved once and every iterate is evaluated through the dual-corrected amplitudes
+///     t'_k = W_L^H x_k,L + phi0^T (b - A x_k),
+/// whose error is t - t'_k = s0^T e_k exactly, with s0 = g - A^T phi0 the dual residual (FP64) and
+/// e_k = x - x_k. By Cauchy-Schwarz |t - t'_k|[d, c] <= |s0_d| |e_k,c|, and |e_{k+1}| is at most
+/// rho / (1 - rho) times the correction |d_k| just applied (rho the contraction per iteration);
+/// t'_{k+1} - t'_k = s0^T d_k, so the error of t'_{k+1} is also about rho / (1 - rho) times that
+/// change. A point is then also done when either, carried to T as sum 2 |t'| delta + delta^2, is
+/// below kGoalTol (the bound) or kGoalEst (the estimate), with rho taken as kGoalSafety times the
+/// larger observed contraction of the corrections and of the amplitudes' changes (r_0 after the
+/// first iteration).
 constexpr double kRefineTol = 1e-10;
 constexpr double kRefineEst = 1e-12;
 constexpr double kRefineEstFrom = 1e-7;
+constexpr double kGoalTol = 1e-11;
+constexpr double kGoalEst = 1e-12;
+constexpr double kGoalSafety = 10.0;
 constexpr int kRefineMaxIter = 12;
 /// From the third iteration on, a point whose correction shrinks by less than 1 / kRefineRate per
 /// iteration is left to the complex f64 recomputation (f32 factors of a well-conditioned point
@@ -1212,6 +1233,7 @@ struct Refined {
     std::vector<char> ok;
     int64_t iters = 0;
     double worst = 0.0;
+    int64_t goal_done = 0;  // points accepted by the goal-oriented bound
 };
 
 /// CAROLI_REFINE_TRACE=1 prints, per refinement iteration, how many points are still iterating
@@ -1221,12 +1243,19 @@ bool refine_trace() {
     return v && std::strcmp(v, "1") == 0;
 }
 
+/// CAROLI_REFINE_GOAL=1 adds the dual solution and its acceptance tests; by default the primal
+/// corrections alone decide (the dual solve costs about what it saves: BHZ 200x64, 264 points,
+/// 139 against 136 ms a step; graphene 512x64 with about 20 channels, 666 against 527 ms).
+bool refine_goal() {
+    const char* v = std::getenv("CAROLI_REFINE_GOAL");
+    return v && std::strcmp(v, "1") == 0;
+}
+
 /// The refinement of one launch unit (block b of P points) from the factors its sweep left in
-/// its lane (complex f32 fronts): x = A^-1 E_R W_R, refined with FP64 residuals b - A x of the
-/// full operator z - H - Sigma until the correction is below kRefineTol; then T_b = |W_L^H x_L|^2.
-/// Its work is issued on the lane's stream, so that the refinements of the units of a group (one
-/// lane each) run at once: start(), then issue() / collect() per iteration (with the lane's stream
-/// synchronised in between) until issue() returns false, then finish_issue() / finish().
+/// its lane (complex f32 fronts). Its work is issued on the lane's stream, so that the refinements
+/// of the units of a group (one lane each) run at once: start(), then issue() / collect() per
+/// iteration (with the lane's stream synchronised in between) until issue() returns false, then
+/// finish().
 class Refinement {
   public:
     /// recompute: shared by the batch, set once some point goes to the complex f64 recomputation.
@@ -1237,7 +1266,11 @@ class Refinement {
         P_ = int32_t(sig_.size());
         out_.t.assign(P_, 0.0);
         out_.ok.assign(P_, 1);
-        for (auto* g : sig_) c_ = std::max(c_, g->blocks[b].wr.cols);
+        for (auto* g : sig_) {
+            c_ = std::max(c_, g->blocks[b].wr.cols);
+            cl_ = std::max(cl_, g->blocks[b].wl.cols);
+        }
+        if (cl_ == 0) c_ = 0;  // no channel on one side: T = 0
         state_.assign(P_, 0);
         for (int32_t p = 0; p < P_; ++p)
             if (skip[p]) {
@@ -1249,12 +1282,14 @@ class Refinement {
         if (c_ > 0 && (!solve_fits(hp.max_n) || !solve_fits(hp.nodes[hp.root].ne))) {
             out_.ok.assign(P_, 0);  // left to the complex f64 recomputation
             c_ = 0;
-            unusable_ = true;
             recompute_ = true;
         }
+        goal_ = c_ > 0 && refine_goal();
         nl_ = int(hp.left_orbs.size());
         nr_ = int(hp.right_orbs.size());
         nrows_ = int(hp.rows.size());
+        gstride_ = 2 * int64_t(cl_) * c_ + c_ + cl_;
+        wr_ = pv.w ? pv.w + int64_t(nl_) * nl_ : nullptr;
     }
 
     void start() {
@@ -1262,34 +1297,66 @@ class Refinement {
         eng_.lane = &lane_;
         RefineBufs& r = lane_.rf;
         hipStream_t st = lane_.stream;
-        w_host_.assign(size_t(P_) * nr_ * c_, double2{0.0, 0.0});
-        for (int32_t p = 0; p < P_; ++p) {
-            const CMat& wr = sig_[p]->blocks[b_].wr;
-            for (int col = 0; col < wr.cols; ++col)
-                for (int a = 0; a < nr_; ++a)
-                    w_host_[size_t(p) * nr_ * c_ + a + size_t(col) * nr_] = double2{wr(a, col).real(), wr(a, col).imag()};
-        }
         bytes_ = size_t(P_) * size_t(n_orb_) * c_ * sizeof(double2);
-        r.w.ensure(w_host_.size() * sizeof(double2));
+        const size_t dual_bytes = size_t(P_) * size_t(n_orb_) * cl_ * sizeof(double2);
         r.x.ensure(bytes_);
         r.b.ensure(bytes_);
         r.d.ensure(bytes_);
+        r.r.ensure(bytes_);
+        if (goal_) {
+            r.phi.ensure(dual_bytes);
+            r.s.ensure(dual_bytes);
+        }
         r.maxes.ensure(size_t(P_) * 2 * sizeof(double));
-        r.xl.ensure(size_t(P_) * nl_ * c_ * sizeof(double2));
+        r.goal.ensure(size_t(P_) * gstride_ * sizeof(double));
+        r.part.ensure(goal_part_words(nrows_, P_, cl_ * c_, std::max(c_, cl_)) * sizeof(double2));
         r.act.ensure(size_t(P_) * sizeof(int32_t));
         r.upd.ensure(size_t(P_) * size_t(hp_.upd_total) * c_ * sizeof(double2));
         r.h_act.ensure(size_t(P_));
         r.h_maxes.ensure(size_t(P_) * 2);
-        HIP_CHECK(hipMemcpyAsync(r.w.p, w_host_.data(), w_host_.size() * sizeof(double2), hipMemcpyHostToDevice, st));
+        r.h_goal.ensure(size_t(P_) * gstride_);
         HIP_CHECK(hipMemsetAsync(r.b.p, 0, bytes_, st));
+        HIP_CHECK(hipMemsetAsync(r.goal.p, 0, size_t(P_) * gstride_ * sizeof(double), st));
         xv_ = SolveView{r.x.as<double2>(), n_orb_ * c_, int32_t(n_orb_), c_, dp_.vars.as<int32_t>()};
         xv_.upd = r.upd.as<double2>();
         xv_.upd_stride = hp_.upd_total * c_;
         SolveView bv = xv_;
         bv.x = r.b.as<double2>();
-        launch_scatter(bv, dp_.right_orbs.as<int32_t>(), nr_, r.w.as<double2>(), int64_t(nr_) * c_, false, P_, st);
+        // W_L (nl x cl) and W_R (nr x cr) of every point are resident with the inputs (zero past a
+        // point's own columns).
+        launch_scatter(bv, dp_.right_orbs.as<int32_t>(), nr_, wr_, pv_.sigma_stride, false, P_, st);
         HIP_CHECK(hipMemcpyAsync(r.x.p, r.b.p, bytes_, hipMemcpyDeviceToDevice, st));
-        solve(eng_, hp_, dp_, pv_, xv_, P_, true, false, true);
+        solve(eng_, hp_, dp_, pv_, xv_, P_, true, false, true);  // x0 = M^-1 b (root and backward only)
+        res_ = r.d.as<double2>();
+        spare_ = r.r.as<double2>();
+        if (goal_) {
+            // phi0 = M^-T g, g = E_L conj(W_L) (root and backward only); then s0 = g - A^T phi0
+            // in FP64.
+            HIP_CHECK(hipMemsetAsync(r.phi.p, 0, dual_bytes, st));
+            SolveView fv{r.phi.as<double2>(), n_orb_ * cl_, int32_t(n_orb_), cl_, dp_.vars.as<int32_t>()};
+            launch_scatter(fv, dp_.left_orbs.as<int32_t>(), nl_, pv_.w, pv_.sigma_stride, true, P_, st);
+            HIP_CHECK(hipMemcpyAsync(r.s.p, r.phi.p, dual_bytes, hipMemcpyDeviceToDevice, st));
+            solve(eng_, hp_, dp_, pv_, fv, P_, true, true, true);
+            launch_residual(pv_, fv, r.s.as<double2>(), r.s.as<double2>(), dp_.rows.as<int32_t>(), nrows_,
+                            dp_.row_ptr.as<int64_t>(), dp_.cols.as<uint32_t>(), dp_.left_pos.as<int32_t>(),
+                            dp_.right_pos.as<int32_t>(), dp_.left_orbs.as<int32_t>(), nl_,
+                            dp_.right_orbs.as<int32_t>(), nr_, P_, st, true);
+            // |s0_d|^2 per column d (kept in the estimate buffer for every later iteration).
+            SolveView sv = fv;
+            launch_goal(sv, nullptr, nullptr, 0, nullptr, 0, nl_, 0, dp_.left_orbs.as<int32_t>(), r.s.as<double2>(),
+                        int64_t(n_orb_) * cl_, cl_, int32_t(2 * int64_t(cl_) * c_ + c_), dp_.rows.as<int32_t>(),
+                        nrows_, r.part.as<double2>(), r.goal.as<double>(), gstride_, P_, st);
+        }
+        residual(P_, xv_);  // r0 = b - A x0, the first iteration's right-hand side
+        if (goal_) {  // t'_0, the first term of the sequence of dual-corrected amplitudes
+            launch_goal(xv_, res_, r.phi.as<double2>(), int64_t(n_orb_) * cl_, pv_.w, pv_.sigma_stride, nl_, cl_,
+                        dp_.left_orbs.as<int32_t>(), nullptr, 0, 0, 0, dp_.rows.as<int32_t>(), nrows_,
+                        r.part.as<double2>(), r.goal.as<double>(), gstride_, P_, st);
+            r.h_goal0.ensure(size_t(P_) * gstride_);
+            HIP_CHECK(hipMemcpyAsync(r.h_goal0.p, r.goal.p, size_t(P_) * gstride_ * sizeof(double),
+                                     hipMemcpyDeviceToHost, st));
+        }
+        HIP_CHECK(hipGetLastError());
     }
 
     /// Issues one iteration on the points still iterating; false when none is.
@@ -1306,22 +1373,30 @@ class Refinement {
         SolveView xa = xv_, da = xv_;
         xa.pts = r.act.as<int32_t>();
         da.pts = r.act.as<int32_t>();
-        da.x = r.d.as<double2>();
-        launch_residual(pv_, xa, r.b.as<double2>(), r.d.as<double2>(), dp_.rows.as<int32_t>(), nrows_,
-                        dp_.row_ptr.as<int64_t>(), dp_.cols.as<uint32_t>(), dp_.left_pos.as<int32_t>(),
-                        dp_.right_pos.as<int32_t>(), dp_.left_orbs.as<int32_t>(), nl_, dp_.right_orbs.as<int32_t>(), nr_,
-                        A_, st);
-        solve(eng_, hp_, dp_, pv_, da, A_, true, false, false);
-        launch_update(xa, r.d.as<double2>(), dp_.rows.as<int32_t>(), nrows_, r.maxes.as<double>(), A_, P_, st);
+        da.x = res_;
+        solve(eng_, hp_, dp_, pv_, da, A_, true, false, false);  // d_k = M^-1 r_k, in place
+        eng_.timed(kRefineVec, 0.0, 0.0, [&] {
+            launch_update(xa, res_, dp_.rows.as<int32_t>(), nrows_, r.maxes.as<double>(), A_, P_, st);
+            std::swap(res_, spare_);
+            residual(A_, xa);  // r_{k+1} = b - A x_{k+1}
+            // t'_{k+1} and |d_k|^2 per column.
+            launch_goal(xa, res_, goal_ ? r.phi.as<double2>() : nullptr, int64_t(n_orb_) * cl_, pv_.w,
+                        pv_.sigma_stride, nl_, cl_, dp_.left_orbs.as<int32_t>(), spare_, int64_t(n_orb_) * c_, c_,
+                        int32_t(2 * int64_t(cl_) * c_), dp_.rows.as<int32_t>(), nrows_, r.part.as<double2>(),
+                        r.goal.as<double>(), gstride_, A_, st);
+        });
         HIP_CHECK(hipGetLastError());
         HIP_CHECK(hipMemcpyAsync(r.h_maxes.p, r.maxes.p, size_t(P_) * 2 * sizeof(double), hipMemcpyDeviceToHost, st));
+        HIP_CHECK(hipMemcpyAsync(r.h_goal.p, r.goal.p, size_t(P_) * gstride_ * sizeof(double), hipMemcpyDeviceToHost, st));
         return true;
     }
 
     /// After the lane's stream is synchronised: the convergence test of the issued iteration.
     void collect() {
         const double* maxes = lane_.rf.h_maxes.p;
+        const double* gl = lane_.rf.h_goal.p;
         for (int32_t p = 0; p < P_; ++p) {
+            if (state_[p] != 0) continue;
             const double md = maxes[2 * p], mx = maxes[2 * p + 1];
             ratio_[p] = mx > 0.0 ? md / mx : (md > 0.0 ? INFINITY : 0.0);
         }
@@ -1341,11 +1416,58 @@ class Refinement {
             std::fprintf(stderr, "[refine] block %d iteration %d: %d iterating (correction %.1e .. %.1e), %d done, %d given up\n",
                          b_, it_, n_active, best, worst, n_done, n_gone);
         }
+        const int64_t npair = int64_t(cl_) * c_;
+        if (goal_ && it_ == 0) {  // t'_0 from start()
+            prev_t_.assign(lane_.rf.h_goal0.p, lane_.rf.h_goal0.p + size_t(P_) * gstride_);
+            prev_dt_.assign(P_, INFINITY);
+        }
+        std::vector<double> trace_bounds, trace_est;
         for (int32_t p = 0; p < P_; ++p) {
             if (state_[p] != 0) continue;
+            const double* g = gl + size_t(p) * gstride_;
+            double tsum = 0.0;
+            for (int64_t e = 0; e < npair; ++e) tsum += g[2 * e] * g[2 * e] + g[2 * e + 1] * g[2 * e + 1];
             const double rr = ratio_[p];
-            if (rr <= kRefineTol || (last_[p] < INFINITY && rr <= kRefineEstFrom && rr * rr <= kRefineEst * last_[p])) {
+            bool goal_ok = false;
+            if (goal_ && rr < INFINITY && std::isfinite(tsum)) {
+                double* pt = prev_t_.data() + size_t(p) * gstride_;
+                double dtn = 0.0;  // |t'_{k+1} - t'_k|_F = |s0^T d_k|_F
+                for (int64_t e = 0; e < 2 * npair; ++e) dtn += (g[e] - pt[e]) * (g[e] - pt[e]);
+                dtn = std::sqrt(dtn);
+                // Contraction per iteration: of the corrections, and of the amplitudes' differences.
+                const double rho_x = last_[p] < INFINITY ? (last_[p] > 0.0 ? rr / last_[p] : 0.0) : rr;
+                const double rho_t = prev_dt_[p] < INFINITY && prev_dt_[p] > 0.0 ? dtn / prev_dt_[p] : 0.0;
+                const double rho = kGoalSafety * std::max(rho_x, rho_t);
+                if (rho < 0.5) {
+                    // |e_{k+1}| <= rho |e_k| <= rho (|d_k| + |e_{k+1}|), so |e_{k+1}| <= rho / (1 - rho) |d_k|;
+                    // likewise for the error s0^T e_{k+1} of t'_{k+1} against t'_{k+1} - t'_k = s0^T d_k.
+                    const double grow = rho / (1.0 - rho);
+                    const double* dn = g + 2 * npair;
+                    const double* sn = dn + c_;
+                    double bound = 0.0, est = 0.0;
+                    for (int c = 0; c < c_; ++c)
+                        for (int d = 0; d < cl_; ++d) {
+                            const int64_t e = d + int64_t(c) * cl_;
+                            const double tv = std::hypot(g[2 * e], g[2 * e + 1]);
+                            const double del = std::sqrt(sn[d]) * grow * std::sqrt(dn[c]);
+                            bound += 2.0 * tv * del + del * del;
+                            const double dif = grow * std::hypot(g[2 * e] - pt[2 * e], g[2 * e + 1] - pt[2 * e + 1]);
+                            est += 2.0 * tv * dif + dif * dif;
+                        }
+                    goal_ok = bound <= kGoalTol || est <= kGoalEst;
+                    if (refine_trace()) {
+                        trace_bounds.push_back(bound);
+                        trace_est.push_back(est);
+                    }
+                }
+                std::copy(g, g + 2 * npair, pt);
+                prev_dt_[p] = dtn;
+            }
+            if (goal_ok || rr <= kRefineTol ||
+                (last_[p] < INFINITY && rr <= kRefineEstFrom && rr * rr <= kRefineEst * last_[p])) {
                 state_[p] = 1;
+                out_.t[p] = tsum;
+                out_.goal_done += goal_ok && rr > kRefineTol;
                 out_.worst = std::max(out_.worst, rr);
             } else if (!(rr < INFINITY) || (it_ >= 2 && rr > kRefineRate * last_[p]) ||
                        (recompute_ && it_ >= kRefineJoinFrom && last_[p] < INFINITY &&
@@ -1356,40 +1478,35 @@ class Refinement {
                 last_[p] = rr;
             }
         }
+        if (refine_trace() && !trace_bounds.empty()) {
+            std::sort(trace_bounds.begin(), trace_bounds.end());
+            std::sort(trace_est.begin(), trace_est.end());
+            const size_t m = trace_bounds.size();
+            std::fprintf(stderr,
+                         "[refine] block %d iteration %d, %zu points: |T' - T| bound %.1e .. %.1e (median %.1e), "
+                         "estimate %.1e .. %.1e (median %.1e)\n",
+                         b_, it_, m, trace_bounds[0], trace_bounds[m - 1], trace_bounds[m / 2], trace_est[0],
+                         trace_est[m - 1], trace_est[m / 2]);
+        }
         ++it_;
     }
 
-    void finish_issue() {
-        if (c_ == 0) return;
-        eng_.lane = &lane_;
-        RefineBufs& r = lane_.rf;
-        launch_gather(xv_, dp_.left_orbs.as<int32_t>(), nl_, r.xl.as<double2>(), P_, lane_.stream);
-        xl_.resize(size_t(P_) * nl_ * c_);
-        HIP_CHECK(hipMemcpyAsync(xl_.data(), r.xl.p, xl_.size() * sizeof(double2), hipMemcpyDeviceToHost, lane_.stream));
-    }
-
-    /// After the lane's stream is synchronised: T per point (points not converged: ok = 0).
+    /// T per point (points not converged: ok = 0).
     Refined finish() {
         if (c_ == 0) return std::move(out_);
-        for (int32_t p = 0; p < P_; ++p) {
-            out_.ok[p] = state_[p] == 1;
-            const CMat& wl = sig_[p]->blocks[b_].wl;
-            double t = 0.0;
-            for (int col = 0; col < c_; ++col)
-                for (int d = 0; d < wl.cols; ++d) {
-                    cplx acc = 0.0;
-                    for (int a = 0; a < nl_; ++a) {
-                        double2 x = xl_[size_t(p) * nl_ * c_ + a + size_t(col) * nl_];
-                        acc += std::conj(wl(a, d)) * cplx(x.x, x.y);
-                    }
-                    t += std::norm(acc);
-                }
-            out_.t[p] = t;
-        }
+        for (int32_t p = 0; p < P_; ++p) out_.ok[p] = state_[p] == 1;
         return std::move(out_);
     }
 
   private:
+    /// r = b - A x on the `points` of xv into res_.
+    void residual(int32_t points, const SolveView& xv) {
+        RefineBufs& r = lane_.rf;
+        launch_residual(pv_, xv, r.b.as<double2>(), res_, dp_.rows.as<int32_t>(), nrows_, dp_.row_ptr.as<int64_t>(),
+                        dp_.cols.as<uint32_t>(), dp_.left_pos.as<int32_t>(), dp_.right_pos.as<int32_t>(),
+                        dp_.left_orbs.as<int32_t>(), nl_, dp_.right_orbs.as<int32_t>(), nr_, points, lane_.stream);
+    }
+
     Engine& eng_;
     Lane& lane_;
     const HostPlan& hp_;
@@ -1399,18 +1516,24 @@ class Refinement {
     int b_;
     int64_t n_orb_;
     int32_t P_ = 0, A_ = 0;
-    int c_ = 0, nl_ = 0, nr_ = 0, nrows_ = 0, it_ = 0;
-    bool unusable_ = false;
+    int c_ = 0, cl_ = 0, nl_ = 0, nr_ = 0, nrows_ = 0, it_ = 0;
+    bool goal_ = false;
     bool& recompute_;
     size_t bytes_ = 0;
+    int64_t gstride_ = 0;  // doubles per point of the estimate buffer: t' (2 cl c), |d_c|^2 (c), |s0_d|^2 (cl)
     SolveView xv_{};
+    const double2* wr_ = nullptr;  // W_R of point 0 in the resident inputs (stride pv_.sigma_stride)
+    double2* res_ = nullptr;    // the residual of the current iterate (the next solve's right-hand side)
+    double2* spare_ = nullptr;  // the other residual buffer (holds the last correction)
     Refined out_;
     // Per point: 0 iterating, 1 converged (later corrections only add rounding noise), 2 given up
     // (unusable factors, a correction that is not finite or shrinks too slowly): left to the
     // complex f64 recomputation.
     std::vector<char> state_;
     std::vector<double> ratio_, last_;
-    std::vector<double2> w_host_, xl_;
+    // The last dual-corrected amplitudes per point (gstride_ doubles each, t' first) and the
+    // norm of their last change.
+    std::vector<double> prev_t_, prev_dt_;
 };
 
 }  // namespace
@@ -1531,11 +1654,17 @@ int32_t create_batch(const cg_system* const* systems, const cg_sigma* const* sig
                                size_t(hp.piv_stride) * 4 + size_t(hp.slots) * 4 + 64;
             size_t refine_point = 0;
             if (precision == CG_MIXED) {
-                int32_t c = 0;
-                for (auto p : idx) c = std::max(c, sigmas[p]->blocks[b].wr.cols);
+                int32_t c = 0, cl = 0;
+                for (auto p : idx) {
+                    c = std::max(c, sigmas[p]->blocks[b].wr.cols);
+                    cl = std::max(cl, sigmas[p]->blocks[b].wl.cols);
+                }
                 const size_t n_orb = systems[idx[0]]->spec.h.n;
-                refine_point = (3 * n_orb + size_t(hp.upd_total) + blk.left_orbitals.size() +
-                                blk.right_orbitals.size()) * size_t(c) * sizeof(double2);
+                // x, b and two residual / correction buffers, the update vectors; the dual solution
+                // and its residual; the estimates.
+                refine_point = (4 * n_orb + size_t(hp.upd_total)) * size_t(c) * sizeof(double2) +
+                               2 * n_orb * size_t(cl) * sizeof(double2) + (2 * size_t(cl) * c + c + cl) * sizeof(double) +
+                               goal_part_words(int32_t(n_orb), 1, cl * c, std::max(c, cl)) * sizeof(double2);
                 per_point += refine_point;
             }
             const size_t budget = size_t(96) << 30;
@@ -1746,7 +1875,16 @@ int32_t run_batch_impl(cg_batch* bt, double* out_t, double* out_lab, int32_t* ou
                 auto sync_group = [&] {
                     for (size_t ui = u0; ui < u1; ++ui) HIP_CHECK(hipStreamSynchronize(eng.lanes[ui - u0].stream));
                 };
+                auto since = [&](std::chrono::steady_clock::time_point a) {
+                    return std::chrono::duration<double>(std::chrono::steady_clock::now() - a).count() * 1e3;
+                };
+                if (refine_trace()) {
+                    sync_group();
+                    std::fprintf(stderr, "[refine] sweep %.2f ms, start %.2f ms\n",
+                                 std::chrono::duration<double>(t0 - w0).count() * 1e3, since(t0));
+                }
                 for (;;) {
+                    const auto ti = std::chrono::steady_clock::now();
                     std::vector<char> issued(refs.size(), 0);
                     bool any = false;
                     for (size_t k = 0; k < refs.size(); ++k) any |= (issued[k] = refs[k]->issue()) != 0;
@@ -1754,9 +1892,8 @@ int32_t run_batch_impl(cg_batch* bt, double* out_t, double* out_lab, int32_t* ou
                     sync_group();
                     for (size_t k = 0; k < refs.size(); ++k)
                         if (issued[k]) refs[k]->collect();
+                    if (refine_trace()) std::fprintf(stderr, "[refine] iteration %.2f ms\n", since(ti));
                 }
-                for (auto& r : refs) r->finish_issue();
-                sync_group();
                 eng.lane = &eng.lanes[0];
                 eng.collect();
                 local.seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -1767,6 +1904,7 @@ int32_t run_batch_impl(cg_batch* bt, double* out_t, double* out_lab, int32_t* ou
                     const std::vector<int32_t>& st = sts[ui - u0];
                     Refined rf = refs[ui - u0]->finish();
                     local.refinement_iters += rf.iters;
+                    local.goal_points += rf.goal_done;
                     for (size_t q = 0; q < P; ++q) {
                         int64_t p = u.points[q];
                         bool delayed = false;
@@ -1835,7 +1973,11 @@ int32_t run_batch_impl(cg_batch* bt, double* out_t, double* out_lab, int32_t* ou
         std::vector<double> t(again.size());
         std::vector<int32_t> st(again.size());
         cg_batch_stats s2{};
+        const auto tr = std::chrono::steady_clock::now();
         int32_t rc = batch_transmission(ss.data(), gg.data(), int64_t(again.size()), CG_DOUBLE, t.data(), st.data(), &s2);
+        if (refine_trace())
+            std::fprintf(stderr, "[refine] complex f64 recomputation of %zu points %.2f ms\n", again.size(),
+                         std::chrono::duration<double>(std::chrono::steady_clock::now() - tr).count() * 1e3);
         if (rc != CG_OK) return rc;
         local.seconds += s2.seconds;
         for (size_t q = 0; q < again.size(); ++q) {
@@ -2398,8 +2540,8 @@ int32_t backward_batch(const cg_system* const* systems, const cg_sigma* const* s
             cr = std::max(cr, sigmas[p]->blocks[b].wr.cols);
         }
         if (cl == 0 || cr == 0) continue;  // no channel: T and the gradient of the block are 0
-        if (int64_t(hp.max_ne) * cl > kSolveLds || !solve_fits(hp.max_n) || !solve_fits(hp.nodes[hp.root].ne))
-            return CG_ERR_INVALID;  // the transposed solves hold all cl columns of a front in LDS
+        if (!solve_fits(hp.max_n) || !solve_fits(hp.nodes[hp.root].ne))
+            return CG_ERR_INVALID;  // a front's rows (one column at a time) must fit the solves' LDS
         // Points per chunk: the stored fronts, the root and the two sets of columns within 64 GB.
         const size_t per_point = size_t(hp.arena_stride) * sizeof(double2) + size_t(hp.root_stride) * sizeof(double2) +
                                  size_t(hp.piv_stride) * 4 + size_t(hp.slots) * 4 +

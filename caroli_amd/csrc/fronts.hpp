@@ -242,7 +242,17 @@ void launch_scatter(const SolveView& sv, const int32_t* orbs, int32_t m, const d
 void launch_residual(const PointsView& pv, const SolveView& xv, const double2* b, double2* d, const int32_t* rows,
                      int32_t nrows, const int64_t* row_ptr, const uint32_t* cols, const int32_t* left_pos,
                      const int32_t* right_pos, const int32_t* left_orbs, int32_t nl, const int32_t* right_orbs,
-                     int32_t nr, int32_t points, hipStream_t st);
+                     int32_t nr, int32_t points, hipStream_t st, bool transpose = false);
+/// The refinement's goal-oriented estimate per point of xv.pts (fixed-order sums): with wl,
+/// t' = W_L^H x_L (+ phi^T res with phi) (cl x cr complex at out[0 .. 2 cl cr), W_L nl x cl per
+/// point at stride wl_stride, phi with the leading dimension of xv); with nrm, the squared 2-norms
+/// of its nc columns over the block rows at out[nrm_at ..]. out_stride doubles per point. part:
+/// scratch of goal_part_words(nrows, points, pairs, norms) complex words.
+void launch_goal(const SolveView& xv, const double2* res, const double2* phi, int64_t phi_stride, const double2* wl,
+                 int64_t wl_stride, int32_t nl, int32_t cl, const int32_t* left_orbs, const double2* nrm,
+                 int64_t nrm_stride, int32_t nc, int32_t nrm_at, const int32_t* rows, int32_t nrows, double2* part,
+                 double* out, int64_t out_stride, int32_t points, hipStream_t st);
+size_t goal_part_words(int32_t nrows, int32_t points, int32_t pairs, int32_t norms);
 /// X += D; maxes[2p], maxes[2p + 1] receive max |D| and max |X| of point p (2 doubles per point,
 /// all_points of them cleared first).
 void launch_update(const SolveView& xv, const double2* d, const int32_t* rows, int32_t nrows, double* maxes,

@@ -4,11 +4,12 @@
 // the separator rows), L21 = F21 U11^-1, U12 = L11^-1 P F12. For A x = b:
 //   forward   y = L11^-1 P b_e,        b_b -= L21 y          (children before parents)
 //   backward  x_e = U11^-1 (y - U12 x_b)                     (parents before children)
-// and for A^T x = b (used by the adjoint):
+// and for A^T x = b (the adjoint and the refinement's dual solve):
 //   forward   g = U11^-T b_e,           b_b -= U12^T g
 //   backward  x_e = P^T L11^-T (g - L21^T x_b).
-// The root is a dense LU of its leading nd + k block. Boundary updates of sibling fronts meet on
-// the parent's variables, so they are accumulated with FP64 atomics.
+// The root is a dense LU of its leading nd + k block. A forward step hands its boundary part to
+// its parent as an update vector, which the parent adds in the order of its children (no
+// atomics): every solve is bit-reproducible.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -66,8 +67,8 @@ __device__ __forceinline__ void store_update(const NodeDesc& nd, const SolveView
     }
 }
 
-// Transposed solves (A^T x = b, the adjoint's second set of columns), one workgroup per front and
-// point with all columns in LDS (ne * c <= kSolveLds).
+// Transposed forward step (A^T x = b with right-hand sides off the kept orbitals), one workgroup
+// per front and point with all columns in LDS (n * c <= kSolveLds).
 template <class TF>
 __global__ void __launch_bounds__(kThreads)
     solve_forward_t_kernel(const NodeDesc* __restrict__ nodes, const int32_t* __restrict__ level, PointsView pv,
@@ -107,50 +108,6 @@ __global__ void __launch_bounds__(kThreads)
     for (int idx = tid; idx < ne * c; idx += kThreads) X[vars[idx % ne] + int64_t(idx / ne) * sv.ldx] = y[idx % ne + (idx / ne) * n];
 }
 
-template <class TF>
-__global__ void __launch_bounds__(kThreads)
-    solve_backward_t_kernel(const NodeDesc* __restrict__ nodes, const int32_t* __restrict__ level, PointsView pv,
-                            SolveView sv) {
-    extern __shared__ double2 y[];  // ne x c
-    const NodeDesc nd = nodes[level[blockIdx.x]];
-    const int64_t p = point_of(sv, blockIdx.y);
-    const TF* F = reinterpret_cast<const TF*>(pv.arena) + p * pv.arena_stride + nd.off;
-    const int32_t* piv = pv.piv + p * pv.piv_stride + nd.piv_off;
-    const int32_t* vars = sv.vars + nd.var_off;
-    double2* X = sv.x + p * sv.x_stride;
-    const int n = nd.n, ne = nd.ne, nb = n - ne, c = sv.c, tid = threadIdx.x;
-    auto f = [&](int i, int j) { return widen(F[i + int64_t(j) * n]); };
-    // y = x_e - L21^T x_b
-    for (int idx = tid; idx < ne * c; idx += kThreads) {
-        int k = idx % ne, col = idx / ne;
-        double2 acc = X[vars[k] + int64_t(col) * sv.ldx];
-        for (int b = 0; b < nb; ++b) {
-            double2 xb = X[vars[ne + b] + int64_t(col) * sv.ldx];
-            acc = csub(acc, cmul(f(ne + b, k), xb));
-        }
-        y[idx] = acc;
-    }
-    __syncthreads();
-    for (int k = ne - 1; k >= 0; --k) {  // L11^T (unit upper)
-        for (int idx = tid; idx < k * c; idx += kThreads) {
-            int i = idx % k, col = idx / k;
-            y[i + col * ne] = csub(y[i + col * ne], cmul(f(k, i), y[k + col * ne]));
-        }
-        __syncthreads();
-    }
-    for (int col = tid; col < c; col += kThreads)  // P^T: the swaps in reverse order
-        for (int k = ne - 1; k >= 0; --k) {
-            int q = piv[k];
-            if (q != k) {
-                double2 t = y[k + col * ne];
-                y[k + col * ne] = y[q + col * ne];
-                y[q + col * ne] = t;
-            }
-        }
-    __syncthreads();
-    for (int idx = tid; idx < ne * c; idx += kThreads) X[vars[idx % ne] + int64_t(idx / ne) * sv.ldx] = y[idx];
-}
-
 // ---------------------------------------------------------------------------------------------
 // Blocked solves of A x = b (the refinement and the adjoint's G columns). A workgroup holds CC
 // right-hand-side columns of its front in LDS (grid.z covers the columns) and reads each factor
@@ -160,7 +117,18 @@ __global__ void __launch_bounds__(kThreads)
 // across groups of lanes when the rows are few).
 constexpr int kDiag = 64;
 constexpr int kPre = 8;  // factor columns loaded ahead in a diagonal block
-constexpr int kGemvBatch = 4;  // factor entries a lane loads at once in the matrix-vector products
+// Factor entries a lane loads at once in the matrix-vector products (the loads of a batch are in
+// flight together; the products are latency-bound otherwise).
+template <class TF>
+struct GemvBatch;
+template <>
+struct GemvBatch<float2> {
+    static constexpr int value = 16;
+};
+template <>
+struct GemvBatch<double2> {
+    static constexpr int value = 16;
+};
 // Waves per SIMD the blocked solve kernels are compiled for (at most 128 registers a lane, which
 // holds 4 columns without spilling): the solves are latency-bound and need the occupancy more
 // than wider column chunks.
@@ -197,7 +165,15 @@ __device__ __forceinline__ void gemv_sub(const TF* __restrict__ F, int ld, int r
                                          int ldy, double2* red) {
     const int R = r1 - r0, K = k1 - k0, tid = threadIdx.x;
     if (R <= 0 || K <= 0) return;  // uniform
-    const int RP = min(NT, (R + 63) / 64 * 64), G = NT / RP, rr = tid % RP, g = tid / RP;
+    // Lanes per row group: the rows rounded up to a power of two below 64 (a wave then covers
+    // several groups, with no lane reading a duplicate row), to whole waves above.
+    int RP = 64;
+    if (R < 64)
+        for (RP = 1; RP < R; RP <<= 1) {
+        }
+    else
+        RP = min(NT, (R + 63) / 64 * 64);
+    const int G = NT / RP, rr = tid % RP, g = tid / RP;
     const int kc = (K + G - 1) / G, ka = k0 + min(g, G - 1) * kc, kz = g < G ? min(k1, ka + kc) : ka;
     for (int base = 0; base < R; base += RP) {
         const int r = r0 + base + rr;
@@ -205,20 +181,19 @@ __device__ __forceinline__ void gemv_sub(const TF* __restrict__ F, int ld, int r
         double2 acc[CC];
 #pragma unroll
         for (int col = 0; col < CC; ++col) acc[col] = double2{0.0, 0.0};
-        int k = ka;
-        for (; k + kGemvBatch <= kz; k += kGemvBatch) {
-            double2 l[kGemvBatch];
+        // Batches of factor entries in their own type, the last one clamped and guarded.
+        constexpr int BT = GemvBatch<TF>::value;
+        for (int k = ka; k < kz; k += BT) {
+            TF l[BT];
 #pragma unroll
-            for (int u = 0; u < kGemvBatch; ++u) l[u] = widen(F[rc + int64_t(k + u) * ld]);
+            for (int u = 0; u < BT; ++u) l[u] = F[rc + int64_t(min(k + u, kz - 1)) * ld];
 #pragma unroll
-            for (int u = 0; u < kGemvBatch; ++u)
+            for (int u = 0; u < BT; ++u)
+                if (k + u < kz) {
+                    const double2 lv = widen(l[u]);
 #pragma unroll
-                for (int col = 0; col < CC; ++col) acc[col] = cadd(acc[col], cmul(l[u], y[k + u + col * ldy]));
-        }
-        for (; k < kz; ++k) {
-            const double2 l = widen(F[rc + int64_t(k) * ld]);
-#pragma unroll
-            for (int col = 0; col < CC; ++col) acc[col] = cadd(acc[col], cmul(l, y[k + col * ldy]));
+                    for (int col = 0; col < CC; ++col) acc[col] = cadd(acc[col], cmul(lv, y[k + u + col * ldy]));
+                }
         }
         if (G == 1) {
             if (g == 0 && r < r1)
@@ -435,53 +410,221 @@ __global__ void __launch_bounds__(NT, kSolveWaves)
     }
 }
 
-// Transposed dense solve with the root's LU (its leading ne x ne block, complex f64).
-__global__ void __launch_bounds__(kThreads)
-    solve_root_t_kernel(const NodeDesc* __restrict__ nodes, const int32_t* __restrict__ level, PointsView pv, SolveView sv) {
-    extern __shared__ double2 y[];
+// ---------------------------------------------------------------------------------------------
+// Blocked transposed solves (A^T x = b) for right-hand sides on the kept orbitals: the root's
+// (P^-1 L U)^T and, parents before children, each front's backward step
+// x_e = P^T L11^-T (x_e - L21^T x_b). The transposed products read the factors down their
+// columns: a wave takes one output and its lanes run down the column (coalesced loads), then a
+// wave reduction in a fixed lane order, so the result does not depend on the schedule.
+
+// y[o] -= sum_{r in [r0, r1)} F[r + o ld] y[r] for o in [o0, o1) (CC columns of y, leading
+// dimension ldy; the two row ranges are disjoint). Ends with a barrier.
+template <class TF, int NT, int CC>
+__device__ __forceinline__ void gemv_t_sub(const TF* __restrict__ F, int ld, int r0, int r1, int o0, int o1,
+                                           double2* y, int ldy) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (r1 > r0) {
+        for (int o = o0 + wave; o < o1; o += NT / 64) {
+            const TF* col = F + int64_t(o) * ld;
+            double2 acc[CC];
+#pragma unroll
+            for (int c = 0; c < CC; ++c) acc[c] = double2{0.0, 0.0};
+            int r = r0 + lane;
+            for (; r + 3 * 64 < r1; r += 4 * 64) {
+                double2 f[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) f[u] = widen(col[r + u * 64]);
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+#pragma unroll
+                    for (int c = 0; c < CC; ++c) acc[c] = cadd(acc[c], cmul(f[u], y[r + u * 64 + c * ldy]));
+            }
+            for (; r < r1; r += 64) {
+                const double2 f = widen(col[r]);
+#pragma unroll
+                for (int c = 0; c < CC; ++c) acc[c] = cadd(acc[c], cmul(f, y[r + c * ldy]));
+            }
+#pragma unroll
+            for (int c = 0; c < CC; ++c)
+                for (int off = 32; off > 0; off >>= 1) {
+                    acc[c].x += __shfl_xor(acc[c].x, off, 64);
+                    acc[c].y += __shfl_xor(acc[c].y, off, 64);
+                }
+            if (lane == 0)
+#pragma unroll
+                for (int c = 0; c < CC; ++c) y[o + c * ldy] = csub(y[o + c * ldy], acc[c]);
+        }
+    }
+    __syncthreads();
+}
+
+// Wave 0: L^T z = y on the diagonal block [kb, kb + w) (L unit lower, so L^T unit upper), back
+// substitution: once z_i is final (lane i), the lanes j < i subtract L[kb + i, kb + j] z_i; lane j
+// reads its own column of L.
+template <class TF, int CC>
+__device__ __forceinline__ void diag_lower_t(const TF* __restrict__ F, int ld, int kb, int w, double2* y, int ldy) {
+    const int lane = threadIdx.x;
+    const bool mine = lane < w;
+    const int64_t jc = int64_t(kb + min(lane, w - 1)) * ld;
+    double2 v[CC];
+#pragma unroll
+    for (int c = 0; c < CC; ++c) v[c] = mine ? y[kb + lane + c * ldy] : double2{0.0, 0.0};
+    for (int top = w - 1; top >= 0; top -= kPre) {
+        double2 l[kPre];
+#pragma unroll
+        for (int s = 0; s < kPre; ++s) l[s] = widen(F[kb + max(top - s, 0) + jc]);
+#pragma unroll
+        for (int s = 0; s < kPre; ++s) {
+            const int i = top - s;
+            if (i >= 0) {
+#pragma unroll
+                for (int c = 0; c < CC; ++c) {
+                    const double2 z = lane_value(v[c], i);
+                    if (lane < i) v[c] = csub(v[c], cmul(l[s], z));
+                }
+            }
+        }
+    }
+    if (mine)
+#pragma unroll
+        for (int c = 0; c < CC; ++c) y[kb + lane + c * ldy] = v[c];
+}
+
+// Wave 0: U^T a = y on the diagonal block [kb, kb + w) (U^T lower, non-unit), forward
+// substitution: a_k = y_k / U[kb + k, kb + k] (lane k), then the lanes j > k subtract
+// U[kb + k, kb + j] a_k; lane j reads its own column of U.
+template <class TF, int CC>
+__device__ __forceinline__ void diag_upper_t(const TF* __restrict__ F, int ld, int kb, int w, double2* y, int ldy) {
+    const int lane = threadIdx.x;
+    const bool mine = lane < w;
+    const int64_t jc = int64_t(kb + min(lane, w - 1)) * ld;
+    double2 v[CC];
+#pragma unroll
+    for (int c = 0; c < CC; ++c) v[c] = mine ? y[kb + lane + c * ldy] : double2{0.0, 0.0};
+    for (int k0 = 0; k0 < w; k0 += kPre) {
+        double2 u[kPre];
+#pragma unroll
+        for (int s = 0; s < kPre; ++s) u[s] = widen(F[kb + min(k0 + s, w - 1) + jc]);
+#pragma unroll
+        for (int s = 0; s < kPre; ++s) {
+            const int k = k0 + s;
+            if (k < w) {
+                const double2 inv = crecip(lane_value(u[s], k));  // 1 / U[kb + k, kb + k]
+#pragma unroll
+                for (int c = 0; c < CC; ++c) {
+                    const double2 a = cmul(lane_value(v[c], k), inv);
+                    if (lane == k)
+                        v[c] = a;
+                    else if (lane > k)
+                        v[c] = csub(v[c], cmul(u[s], a));
+                }
+            }
+        }
+    }
+    if (mine)
+#pragma unroll
+        for (int c = 0; c < CC; ++c) y[kb + lane + c * ldy] = v[c];
+}
+
+// L^T z = y with the unit-lower columns [0, ne) of F (rows [0, ne)), bottom block first.
+template <class TF, int NT, int CC>
+__device__ __forceinline__ void lower_t_blocked(const TF* F, int ld, int ne, double2* y, int ldy) {
+    for (int kb = (ne - 1) / kDiag * kDiag; kb >= 0; kb -= kDiag) {
+        const int w = min(kDiag, ne - kb);
+        gemv_t_sub<TF, NT, CC>(F, ld, kb + w, ne, kb, kb + w, y, ldy);
+        if (threadIdx.x < 64) diag_lower_t<TF, CC>(F, ld, kb, w, y, ldy);
+        __syncthreads();
+    }
+}
+
+// U^T a = y with the upper triangle of F[0:ne, 0:ne], top block first.
+template <class TF, int NT, int CC>
+__device__ __forceinline__ void upper_t_blocked(const TF* F, int ld, int ne, double2* y, int ldy) {
+    for (int kb = 0; kb < ne; kb += kDiag) {
+        const int w = min(kDiag, ne - kb);
+        gemv_t_sub<TF, NT, CC>(F, ld, 0, kb, kb, kb + w, y, ldy);
+        if (threadIdx.x < 64) diag_upper_t<TF, CC>(F, ld, kb, w, y, ldy);
+        __syncthreads();
+    }
+}
+
+// P^T y: LAPACK's interchanges in reverse order, one lane per column.
+__device__ __forceinline__ void apply_pivots_t(const int32_t* __restrict__ piv, int ne, double2* y, int ldy, int cw) {
+    if (int(threadIdx.x) < cw) {
+        double2* yc = y + threadIdx.x * ldy;
+        for (int k = ne - 1; k >= 0; --k) {
+            const int q = piv[k];
+            if (q > k && q < ne) {
+                const double2 t = yc[k];
+                yc[k] = yc[q];
+                yc[q] = t;
+            }
+        }
+    }
+    __syncthreads();
+}
+
+// Transposed backward step of a front: x_e = P^T L11^-T (x_e - L21^T x_b), the boundary values
+// final (set by the ancestors).
+template <class TF, int NT, int CC>
+__global__ void __launch_bounds__(NT, kSolveWaves)
+    bwd_t_blocked_kernel(const NodeDesc* __restrict__ nodes, const int32_t* __restrict__ level, PointsView pv,
+                         SolveView sv) {
+    extern __shared__ double2 ys[];  // n x CC
+    const SolveItem it = solve_item((sv.c + CC - 1) / CC);
+    const NodeDesc nd = nodes[level[it.front]];
+    const int64_t p = point_of(sv, it.point);
+    const int c0 = it.chunk * CC, cw = min(CC, sv.c - c0);
+    if (cw <= 0) return;
+    const TF* F = reinterpret_cast<const TF*>(pv.arena) + p * pv.arena_stride + nd.off;
+    const int32_t* piv = pv.piv + p * pv.piv_stride + nd.piv_off;
+    const int32_t* vars = sv.vars + nd.var_off;
+    double2* X = sv.x + p * sv.x_stride + int64_t(c0) * sv.ldx;
+    const int n = nd.n, ne = nd.ne, tid = threadIdx.x;
+    for (int idx = tid; idx < n * CC; idx += NT) {
+        const int r = idx % n, col = idx / n;
+        ys[idx] = col < cw ? X[vars[r] + int64_t(col) * sv.ldx] : double2{0.0, 0.0};
+    }
+    __syncthreads();
+    gemv_t_sub<TF, NT, CC>(F, n, ne, n, 0, ne, ys, n);
+    lower_t_blocked<TF, NT, CC>(F, n, ne, ys, n);
+    apply_pivots_t(piv, ne, ys, n, cw);
+    for (int idx = tid; idx < ne * cw; idx += NT) {
+        const int r = idx % ne, col = idx / ne;
+        X[vars[r] + int64_t(col) * sv.ldx] = ys[r + col * n];
+    }
+}
+
+// Transposed solve with the root's LU (its leading ne x ne block, complex f64): U^T a = b,
+// L^T z = a, x = P^T z; the children's update vectors first when the transposed forward solve ran.
+template <int NT, int CC>
+__global__ void __launch_bounds__(NT, kSolveWaves)
+    root_t_blocked_kernel(const NodeDesc* __restrict__ nodes, const int32_t* __restrict__ level, PointsView pv,
+                          SolveView sv) {
+    extern __shared__ double2 ys[];  // ne x CC
+    const SolveItem it = solve_item((sv.c + CC - 1) / CC);
     const NodeDesc nd = nodes[level[0]];
-    const int64_t p = point_of(sv, blockIdx.x);
+    const int64_t p = point_of(sv, it.point);
+    const int c0 = it.chunk * CC, cw = min(CC, sv.c - c0);
+    if (cw <= 0) return;
     const double2* F = reinterpret_cast<const double2*>(pv.root_arena) + p * pv.root_stride + nd.off;
     const int32_t* piv = pv.piv + p * pv.piv_stride + nd.piv_off;
     const int32_t* vars = sv.vars + nd.var_off;
-    double2* X = sv.x + p * sv.x_stride;
-    const int n = nd.n, ne = nd.ne, c = sv.c, tid = threadIdx.x;
-    auto f = [&](int i, int j) { return F[i + int64_t(j) * n]; };
-    for (int idx = tid; idx < ne * c; idx += kThreads) y[idx] = X[vars[idx % ne] + int64_t(idx / ne) * sv.ldx];
-    __syncthreads();
-    if (sv.gather) gather_updates<kThreads>(nd, sv, p, 0, c, y, ne);
-    auto swaps = [&](bool reverse) {
-        for (int col = tid; col < c; col += kThreads)
-            for (int s = 0; s < ne; ++s) {
-                int k = reverse ? ne - 1 - s : s, q = piv[k];
-                if (q != k) {
-                    double2 t = y[k + col * ne];
-                    y[k + col * ne] = y[q + col * ne];
-                    y[q + col * ne] = t;
-                }
-            }
-        __syncthreads();
-    };
-    {
-        for (int k = 0; k < ne; ++k) {  // U^T a = b
-            for (int col = tid; col < c; col += kThreads) y[k + col * ne] = cmul(y[k + col * ne], crecip(f(k, k)));
-            __syncthreads();
-            for (int idx = tid; idx < (ne - k - 1) * c; idx += kThreads) {
-                int i = k + 1 + idx % (ne - k - 1), col = idx / (ne - k - 1);
-                y[i + col * ne] = csub(y[i + col * ne], cmul(f(k, i), y[k + col * ne]));
-            }
-            __syncthreads();
-        }
-        for (int k = ne - 1; k >= 0; --k) {  // L^T (unit upper)
-            for (int idx = tid; idx < k * c; idx += kThreads) {
-                int i = idx % k, col = idx / k;
-                y[i + col * ne] = csub(y[i + col * ne], cmul(f(k, i), y[k + col * ne]));
-            }
-            __syncthreads();
-        }
-        swaps(true);
+    double2* X = sv.x + p * sv.x_stride + int64_t(c0) * sv.ldx;
+    const int n = nd.n, ne = nd.ne, tid = threadIdx.x;
+    for (int idx = tid; idx < ne * CC; idx += NT) {
+        const int r = idx % ne, col = idx / ne;
+        ys[idx] = col < cw ? X[vars[r] + int64_t(col) * sv.ldx] : double2{0.0, 0.0};
     }
-    for (int idx = tid; idx < ne * c; idx += kThreads) X[vars[idx % ne] + int64_t(idx / ne) * sv.ldx] = y[idx];
+    __syncthreads();
+    if (sv.gather) gather_updates<NT>(nd, sv, p, c0, cw, ys, ne);
+    upper_t_blocked<double2, NT, CC>(F, n, ne, ys, ne);
+    lower_t_blocked<double2, NT, CC>(F, n, ne, ys, ne);
+    apply_pivots_t(piv, ne, ys, ne, cw);
+    for (int idx = tid; idx < ne * cw; idx += NT) {
+        const int r = idx % ne, col = idx / ne;
+        X[vars[r] + int64_t(col) * sv.ldx] = ys[r + col * ne];
+    }
 }
 
 // B[orbs[a], col] = W[a, col] (or conj) for a < m, per point; B must be zero elsewhere.
@@ -499,14 +642,18 @@ __global__ void __launch_bounds__(kThreads)
     }
 }
 
-// D = B - (z - H - Sigma_L - Sigma_R) X on the rows of a block, in FP64 (the residual of the
-// refinement). Sigma_L acts on the left-slice rows (positions left_pos), Sigma_R on the right.
+constexpr int kResBatch = 8;
+
+// D = B - A X on the rows of a block in FP64, A = z - H - Sigma_L - Sigma_R (the residual of the
+// refinement), or D = B - A^T X with transpose (the residual of the dual solve). Sigma_L acts on
+// the left-slice rows (positions left_pos), Sigma_R on the right. H is Hermitian, so the rows of
+// H^T are the conjugated rows of H.
 __global__ void __launch_bounds__(kThreads)
     residual_kernel(PointsView pv, SolveView xv, const double2* __restrict__ b, double2* __restrict__ d,
                     const int32_t* __restrict__ rows, int32_t nrows, const int64_t* __restrict__ row_ptr,
                     const uint32_t* __restrict__ cols, const int32_t* __restrict__ left_pos,
                     const int32_t* __restrict__ right_pos, const int32_t* __restrict__ left_orbs, int32_t nl,
-                    const int32_t* __restrict__ right_orbs, int32_t nr) {
+                    const int32_t* __restrict__ right_orbs, int32_t nr, int32_t transpose) {
     const int64_t p = point_of(xv, blockIdx.y);
     const double2* X = xv.x + p * xv.x_stride;
     const double2* B = b + p * xv.x_stride;
@@ -515,18 +662,166 @@ __global__ void __launch_bounds__(kThreads)
     const double2 z = pv.z[p];
     const double2* sl = pv.sigma + p * pv.sigma_stride;
     const double2* sr = sl + int64_t(nl) * nl;
+    // Sigma (m x m, column-major) entry (row, a), or (a, row) transposed.
+    const int64_t sa = transpose ? 1 : 0;
     for (int idx = blockIdx.x * kThreads + threadIdx.x; idx < nrows * xv.c; idx += gridDim.x * kThreads) {
         const int r = rows[idx % nrows], col = idx / nrows;
         const double2* x = X + int64_t(col) * xv.ldx;
         double2 acc = cmul(z, x[r]);
-        for (int64_t e = row_ptr[r]; e < row_ptr[r + 1]; ++e) acc = csub(acc, cmul(vals[e], x[cols[e]]));
+        // Loads in batches of kResBatch (indexes clamped, only the sums guarded), so that a lane
+        // has a batch of loads in flight instead of one.
+        const int64_t e0 = row_ptr[r], e1 = row_ptr[r + 1];
+        for (int64_t e = e0; e < e1; e += kResBatch) {
+            uint32_t cc[kResBatch];
+            double2 v[kResBatch], xv[kResBatch];
+#pragma unroll
+            for (int u = 0; u < kResBatch; ++u) {
+                const int64_t q = min(e + u, e1 - 1);
+                cc[u] = cols[q];
+                v[u] = vals[q];
+            }
+#pragma unroll
+            for (int u = 0; u < kResBatch; ++u) xv[u] = x[cc[u]];
+#pragma unroll
+            for (int u = 0; u < kResBatch; ++u)
+                if (e + u < e1) acc = csub(acc, cmul(transpose ? cconj(v[u]) : v[u], xv[u]));
+        }
+        // Sigma (m x m, column-major) row `pos` (column `pos` transposed) against x on the slice.
+        auto slice = [&](const double2* sg, int pos, const int32_t* orbs, int m) {
+            for (int a0 = 0; a0 < m; a0 += kResBatch) {
+                int32_t o[kResBatch];
+                double2 sv[kResBatch], xs[kResBatch];
+#pragma unroll
+                for (int u = 0; u < kResBatch; ++u) {
+                    const int a = min(a0 + u, m - 1);
+                    o[u] = orbs[a];
+                    sv[u] = sg[sa ? a + int64_t(pos) * m : pos + int64_t(a) * m];
+                }
+#pragma unroll
+                for (int u = 0; u < kResBatch; ++u) xs[u] = x[o[u]];
+#pragma unroll
+                for (int u = 0; u < kResBatch; ++u)
+                    if (a0 + u < m) acc = csub(acc, cmul(sv[u], xs[u]));
+            }
+        };
         const int lp = left_pos[r], rp = right_pos[r];
-        if (lp >= 0)
-            for (int a = 0; a < nl; ++a) acc = csub(acc, cmul(sl[lp + int64_t(a) * nl], x[left_orbs[a]]));
-        if (rp >= 0)
-            for (int a = 0; a < nr; ++a) acc = csub(acc, cmul(sr[rp + int64_t(a) * nr], x[right_orbs[a]]));
+        if (lp >= 0) slice(sl, lp, left_orbs, nl);
+        if (rp >= 0) slice(sr, rp, right_orbs, nr);
         const int64_t at = r + int64_t(col) * xv.ldx;
         D[at] = csub(B[at], acc);
+    }
+}
+
+// The goal-oriented estimate of the refinement, per point of xv.pts, every sum in a fixed order
+// (the result does not depend on the schedule):
+//  - with wl: t'[d, c] = sum_a conj(W_L[a, d]) X[left_orbs[a], c] (+ sum_rows phi[r, d] R[r, c]
+//    with phi): the transmission amplitudes W_L^H x_L, corrected by the dual solution phi
+//    (A^T phi = E_L conj(W_L)) against the residual R = b - A x;
+//  - with nrm: sum_rows |nrm[r, c]|^2 per column c of nrm.
+// goal_part_kernel: grid (chunks, points), each workgroup over kGoalRows block rows (the left-slice
+// term in chunk 0), items (the cl cr pairs, then the nc columns) summed by G lanes over
+// interleaved rows and the G partial sums in lane order, to part[(q chunks + chunk) items + item].
+// goal_sum_kernel: per point, the chunks' partial sums in chunk order to out (t' at
+// out[0 .. 2 cl cr), the norms at out[nrm_at ..], out_stride doubles per point).
+constexpr int kGoalRows = 2048;
+struct GoalArgs {
+    SolveView xv;
+    const double2* res;
+    const double2* phi;
+    int64_t phi_stride;
+    const double2* wl;
+    int64_t wl_stride;
+    int32_t nl, cl;
+    const int32_t* left_orbs;
+    const double2* nrm;
+    int64_t nrm_stride;
+    int32_t nc, nrm_at;
+    const int32_t* rows;
+    int32_t nrows;
+    double2* part;
+    int32_t chunks;
+    double* out;
+    int64_t out_stride;
+};
+
+__global__ void __launch_bounds__(kThreads) goal_part_kernel(GoalArgs ga) {
+    __shared__ double2 red[kThreads];
+    const SolveView& xv = ga.xv;
+    const int64_t q = blockIdx.y, p = point_of(xv, q);
+    const int tid = threadIdx.x, cr = xv.c, ld = xv.ldx, chunk = blockIdx.x;
+    const int i0 = chunk * kGoalRows, i1 = min(ga.nrows, i0 + kGoalRows);
+    const int pairs = ga.wl ? ga.cl * cr : 0, norms = ga.nrm ? ga.nc : 0, items = pairs + norms;
+    const double2* X = xv.x + p * xv.x_stride;
+    const double2* R = ga.res ? ga.res + p * xv.x_stride : nullptr;
+    const double2* F = ga.phi ? ga.phi + p * ga.phi_stride : nullptr;
+    const double2* W = ga.wl ? ga.wl + p * ga.wl_stride : nullptr;
+    const double2* V = ga.nrm ? ga.nrm + p * ga.nrm_stride : nullptr;
+    double2* out = ga.part + (q * ga.chunks + chunk) * int64_t(items);
+    const int G = max(1, kThreads / max(items, 1)), slots = kThreads / G;
+    constexpr int B = 4;
+    for (int t0 = 0; t0 < items; t0 += slots) {
+        const int t = t0 + tid / G, g = tid % G;
+        double2 acc{0.0, 0.0};
+        if (tid / G < slots && t < items) {
+            if (t < pairs) {
+                const int dd = t % ga.cl, c = t / ga.cl;
+                if (chunk == 0)
+                    for (int a = g; a < ga.nl; a += G)
+                        acc = cadd(acc, cmul(cconj(W[a + int64_t(dd) * ga.nl]), X[ga.left_orbs[a] + int64_t(c) * ld]));
+                if (F)
+                    for (int i = i0 + g; i < i1; i += G * B) {
+                        int rr[B];
+#pragma unroll
+                        for (int u = 0; u < B; ++u) rr[u] = ga.rows[min(i + u * G, i1 - 1)];
+                        double2 f[B], v[B];
+#pragma unroll
+                        for (int u = 0; u < B; ++u) {
+                            f[u] = F[rr[u] + int64_t(dd) * ld];
+                            v[u] = R[rr[u] + int64_t(c) * ld];
+                        }
+#pragma unroll
+                        for (int u = 0; u < B; ++u)
+                            if (i + u * G < i1) acc = cadd(acc, cmul(f[u], v[u]));
+                    }
+            } else {
+                const int c = t - pairs;
+                for (int i = i0 + g; i < i1; i += G * B) {
+                    int rr[B];
+#pragma unroll
+                    for (int u = 0; u < B; ++u) rr[u] = ga.rows[min(i + u * G, i1 - 1)];
+                    double2 v[B];
+#pragma unroll
+                    for (int u = 0; u < B; ++u) v[u] = V[rr[u] + int64_t(c) * ld];
+#pragma unroll
+                    for (int u = 0; u < B; ++u)
+                        if (i + u * G < i1) acc.x += v[u].x * v[u].x + v[u].y * v[u].y;
+                }
+            }
+        }
+        red[tid] = acc;
+        __syncthreads();
+        if (g == 0 && tid / G < slots && t < items) {
+            double2 s = red[tid];
+            for (int k = 1; k < G; ++k) s = cadd(s, red[tid + k]);
+            out[t] = s;
+        }
+        __syncthreads();
+    }
+}
+
+__global__ void __launch_bounds__(kThreads) goal_sum_kernel(GoalArgs ga) {
+    const int64_t q = blockIdx.x, p = point_of(ga.xv, q);
+    const int pairs = ga.wl ? ga.cl * ga.xv.c : 0, norms = ga.nrm ? ga.nc : 0, items = pairs + norms;
+    double* O = ga.out + p * ga.out_stride;
+    for (int t = threadIdx.x; t < items; t += kThreads) {
+        double2 s{0.0, 0.0};
+        for (int k = 0; k < ga.chunks; ++k) s = cadd(s, ga.part[(q * ga.chunks + k) * int64_t(items) + t]);
+        if (t < pairs) {
+            O[2 * t] = s.x;
+            O[2 * t + 1] = s.y;
+        } else {
+            O[ga.nrm_at + (t - pairs)] = s.x;
+        }
     }
 }
 
@@ -652,10 +947,29 @@ void launch_scatter(const SolveView& sv, const int32_t* orbs, int32_t m, const d
 void launch_residual(const PointsView& pv, const SolveView& xv, const double2* b, double2* d, const int32_t* rows,
                      int32_t nrows, const int64_t* row_ptr, const uint32_t* cols, const int32_t* left_pos,
                      const int32_t* right_pos, const int32_t* left_orbs, int32_t nl, const int32_t* right_orbs,
-                     int32_t nr, int32_t points, hipStream_t st) {
+                     int32_t nr, int32_t points, hipStream_t st, bool transpose) {
     if (points <= 0 || xv.c <= 0) return;
     hipLaunchKernelGGL(residual_kernel, rows_grid(int64_t(nrows) * xv.c, points), dim3(kThreads), 0, st, pv, xv, b, d,
-                       rows, nrows, row_ptr, cols, left_pos, right_pos, left_orbs, nl, right_orbs, nr);
+                       rows, nrows, row_ptr, cols, left_pos, right_pos, left_orbs, nl, right_orbs, nr,
+                       transpose ? 1 : 0);
+}
+
+int32_t goal_chunks(int32_t nrows) { return std::max<int32_t>(1, (nrows + kGoalRows - 1) / kGoalRows); }
+
+size_t goal_part_words(int32_t nrows, int32_t points, int32_t pairs, int32_t norms) {
+    return size_t(goal_chunks(nrows)) * size_t(points) * size_t(pairs + norms);
+}
+
+void launch_goal(const SolveView& xv, const double2* res, const double2* phi, int64_t phi_stride, const double2* wl,
+                 int64_t wl_stride, int32_t nl, int32_t cl, const int32_t* left_orbs, const double2* nrm,
+                 int64_t nrm_stride, int32_t nc, int32_t nrm_at, const int32_t* rows, int32_t nrows, double2* part,
+                 double* out, int64_t out_stride, int32_t points, hipStream_t st) {
+    if (points <= 0) return;
+    GoalArgs ga{xv,   res,    phi,   phi_stride, wl,   wl_stride,           nl,  cl,  left_orbs, nrm,
+                nrm_stride, nc, nrm_at, rows, nrows, part, goal_chunks(nrows), out, out_stride};
+    if ((wl ? cl * xv.c : 0) + (nrm ? nc : 0) == 0) return;
+    hipLaunchKernelGGL(goal_part_kernel, dim3(unsigned(ga.chunks), unsigned(points)), dim3(kThreads), 0, st, ga);
+    hipLaunchKernelGGL(goal_sum_kernel, dim3(unsigned(points)), dim3(kThreads), 0, st, ga);
 }
 
 void launch_update(const SolveView& xv, const double2* d, const int32_t* rows, int32_t nrows, double* maxes,
@@ -705,6 +1019,14 @@ template <class, int NT, int CC>
 struct Root {
     static constexpr SolveKernel fn = root_blocked_kernel<NT, CC>;
 };
+template <class TF, int NT, int CC>
+struct BwdT {
+    static constexpr SolveKernel fn = bwd_t_blocked_kernel<TF, NT, CC>;
+};
+template <class, int NT, int CC>
+struct RootT {
+    static constexpr SolveKernel fn = root_t_blocked_kernel<NT, CC>;
+};
 template <template <class, int, int> class K, class TF>
 SolveKernel pick(const SolveShape& s) {
     return s.nt == 64 ? by_cc<K, TF, 64>(s.cc) : by_cc<K, TF, 256>(s.cc);
@@ -746,14 +1068,9 @@ void launch_solve_backward(const NodeDesc* nodes, const int32_t* level, int32_t 
                            const PointsView& pv, const SolveView& sv, int32_t points, bool single, bool transpose,
                            hipStream_t st) {
     if (count <= 0 || points <= 0 || sv.c <= 0) return;
-    if (transpose) {
-        SolveKernel k = single ? solve_backward_t_kernel<float2> : solve_backward_t_kernel<double2>;
-        hipLaunchKernelGGL(k, dim3(unsigned(count), unsigned(points)), dim3(kThreads),
-                           size_t(kSolveLds) * sizeof(double2), st, nodes, level, pv, sv);
-        return;
-    }
     const SolveShape s = solve_shape(max_n, sv.c);
-    SolveKernel k = single ? pick<Bwd, float2>(s) : pick<Bwd, double2>(s);
+    SolveKernel k = transpose ? (single ? pick<BwdT, float2>(s) : pick<BwdT, double2>(s))
+                              : (single ? pick<Bwd, float2>(s) : pick<Bwd, double2>(s));
     allow_solve_lds(k, s.lds);
     hipLaunchKernelGGL(k, dim3(unsigned(count * ((sv.c + s.cc - 1) / s.cc)), unsigned(points)), dim3(s.nt), s.lds, st,
                        nodes, level, pv, sv);
@@ -762,14 +1079,9 @@ void launch_solve_backward(const NodeDesc* nodes, const int32_t* level, int32_t 
 void launch_solve_root(const NodeDesc* nodes, const int32_t* level, int32_t root_ne, const PointsView& pv,
                        const SolveView& sv, int32_t points, bool transpose, hipStream_t st) {
     if (points <= 0 || sv.c <= 0) return;
-    if (transpose) {
-        hipLaunchKernelGGL(solve_root_t_kernel, dim3(unsigned(points)), dim3(kThreads),
-                           size_t(kSolveLds) * sizeof(double2), st, nodes, level, pv, sv);
-        return;
-    }
     const SolveShape s = solve_shape(root_ne, sv.c, 256);
     const dim3 grid(unsigned((sv.c + s.cc - 1) / s.cc), unsigned(points));
-    const SolveKernel k = by_cc<Root, double2, 256>(s.cc);
+    const SolveKernel k = transpose ? by_cc<RootT, double2, 256>(s.cc) : by_cc<Root, double2, 256>(s.cc);
     allow_solve_lds(k, s.lds);
     hipLaunchKernelGGL(k, grid, dim3(256), s.lds, st, nodes, level, pv, sv);
 }

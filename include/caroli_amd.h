@@ -161,6 +161,8 @@ typedef struct cg_batch_stats {
   double max_refine_resid;  /* mixed precision: largest final relative correction */
   int64_t recomputed_points; /* mixed precision: points recomputed in complex f64 (their f32
                                 fronts delayed or their refinement did not converge) */
+  int64_t goal_points;      /* mixed precision: points accepted by the dual-weighted error bound
+                               on T before their corrections fell below the relative tolerance */
 } cg_batch_stats;
 
 /* The batched launch: point p is (systems[p], sigmas[p]); all energies x configurations (x

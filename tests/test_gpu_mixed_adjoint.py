@@ -93,6 +93,37 @@ def test_adjoint_matches_fp64_gradient_and_finite_differences():
         assert abs(fd - ad) <= 1e-6 * max(abs(ad), 1e-3 * scale), (i, fd, ad)
 
 
+@pytest.mark.parametrize("model,extent,energies,kw", [
+    (ca.Model.BHZ, [60, 24, 0], [-0.16, -0.05, 0.02, 0.16], dict(disorder=2, strength=0.4, concentration=0.2)),
+    (ca.Model.Graphene, [64, 16, 0], [-0.5, -0.1, 0.3], dict(disorder=1, strength=1.0)),
+], ids=["BHZ 60x24 magnetic", "graphene 64x16 Anderson"])
+def test_mixed_dual_corrected_refinement(monkeypatch, model, extent, energies, kw):
+    """The dual-weighted acceptance (CAROLI_REFINE_GOAL=1: the transposed f32 solve, its FP64
+    residual and the corrected amplitudes W_L^H x_L + phi^T (b - A x)) must give the same
+    transmissions as the oracle to 1e-8, and accept points before their corrections fall below the
+    relative tolerance."""
+    monkeypatch.setenv("CAROLI_REFINE_GOAL", "1")
+    systems = [ca.NestedTwoTerminal.preset(model, extent, seed=1 + c, **kw) for c in range(2)]
+    sig = [systems[0].self_energies(e) for e in energies]
+    ps = [s for s in systems for _ in energies]
+    pg = [g for _ in systems for g in sig]
+    ref = []
+    for s in systems:
+        o = oracle_of(s)
+        for e, g in zip(energies, sig):
+            t, rc = o.transmission_with(e, [g.block(b)[:2] for b in range(s.block_count())])
+            assert rc == 0
+            ref.append(t)
+    stats = ca.capi.BatchStats()
+    t, st = ca.batch_transmission(ps, pg, ca.Precision.Mixed, stats)
+    d = np.abs(t - np.array(ref))
+    say(f"[gpu] dual-corrected refinement: max |dT| {d.max():.2e}, iterations {stats.refinement_iters}, "
+        f"accepted by the dual estimate {stats.goal_points}, recomputed {stats.recomputed_points}")
+    assert (st == 0).all(), st
+    assert d.max() <= 1e-8, d
+    assert stats.goal_points > 0
+
+
 def test_mixed_precision_is_bit_reproducible():
     """The forward solve of the refinement passes each front's boundary update to its parent in
     the order of the children (no atomics): repeated mixed runs give the same T bit for bit,
