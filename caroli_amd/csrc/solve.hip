@@ -15,6 +15,7 @@
 #include <algorithm>
 #include <mutex>
 #include <set>
+#include <type_traits>
 #include <stdexcept>
 
 #include "cx.hpp"
@@ -134,6 +135,7 @@ struct GemvBatch<double2> {
 // than wider column chunks.
 constexpr int kSolveWaves = 4;
 constexpr int kSolveCols = 4;
+constexpr int kSolveColsWide = 8;  // columns per workgroup past kSolveCols right-hand sides
 
 // Workgroup -> (front, point, column chunk) with the chunk fastest and each XCD (workgroups go
 // round-robin over the 8 XCDs) taking one contiguous range, so that the chunks of a front and
@@ -643,6 +645,7 @@ __global__ void __launch_bounds__(kThreads)
 }
 
 constexpr int kResBatch = 8;
+constexpr int kResCols = 4;  // columns of a row per thread in the residual
 
 // D = B - A X on the rows of a block in FP64, A = z - H - Sigma_L - Sigma_R (the residual of the
 // refinement), or D = B - A^T X with transpose (the residual of the dual solve). Sigma_L acts on
@@ -664,33 +667,44 @@ __global__ void __launch_bounds__(kThreads)
     const double2* sr = sl + int64_t(nl) * nl;
     // Sigma (m x m, column-major) entry (row, a), or (a, row) transposed.
     const int64_t sa = transpose ? 1 : 0;
-    for (int idx = blockIdx.x * kThreads + threadIdx.x; idx < nrows * xv.c; idx += gridDim.x * kThreads) {
-        const int r = rows[idx % nrows], col = idx / nrows;
-        const double2* x = X + int64_t(col) * xv.ldx;
-        double2 acc = cmul(z, x[r]);
+    // One thread per row and kResCols columns: the row's entries are loaded once for them.
+    const int chunks = (xv.c + kResCols - 1) / kResCols;
+    for (int idx = blockIdx.x * kThreads + threadIdx.x; idx < nrows * chunks; idx += gridDim.x * kThreads) {
+        const int r = rows[idx % nrows], c0 = (idx / nrows) * kResCols, cw = min(kResCols, xv.c - c0);
+        const double2* x = X + int64_t(c0) * xv.ldx;
+        const int64_t ldx = xv.ldx;
+        double2 acc[kResCols];
+#pragma unroll
+        for (int c = 0; c < kResCols; ++c) acc[c] = c < cw ? cmul(z, x[r + c * ldx]) : double2{0.0, 0.0};
         // Loads in batches of kResBatch (indexes clamped, only the sums guarded), so that a lane
         // has a batch of loads in flight instead of one.
         const int64_t e0 = row_ptr[r], e1 = row_ptr[r + 1];
         for (int64_t e = e0; e < e1; e += kResBatch) {
             uint32_t cc[kResBatch];
-            double2 v[kResBatch], xv[kResBatch];
+            double2 v[kResBatch];
 #pragma unroll
             for (int u = 0; u < kResBatch; ++u) {
                 const int64_t q = min(e + u, e1 - 1);
                 cc[u] = cols[q];
                 v[u] = vals[q];
+                if (transpose) v[u] = cconj(v[u]);
             }
 #pragma unroll
-            for (int u = 0; u < kResBatch; ++u) xv[u] = x[cc[u]];
+            for (int c = 0; c < kResCols; ++c)
+                if (c < cw) {
+                    double2 xv[kResBatch];
 #pragma unroll
-            for (int u = 0; u < kResBatch; ++u)
-                if (e + u < e1) acc = csub(acc, cmul(transpose ? cconj(v[u]) : v[u], xv[u]));
+                    for (int u = 0; u < kResBatch; ++u) xv[u] = x[cc[u] + c * ldx];
+#pragma unroll
+                    for (int u = 0; u < kResBatch; ++u)
+                        if (e + u < e1) acc[c] = csub(acc[c], cmul(v[u], xv[u]));
+                }
         }
         // Sigma (m x m, column-major) row `pos` (column `pos` transposed) against x on the slice.
         auto slice = [&](const double2* sg, int pos, const int32_t* orbs, int m) {
             for (int a0 = 0; a0 < m; a0 += kResBatch) {
                 int32_t o[kResBatch];
-                double2 sv[kResBatch], xs[kResBatch];
+                double2 sv[kResBatch];
 #pragma unroll
                 for (int u = 0; u < kResBatch; ++u) {
                     const int a = min(a0 + u, m - 1);
@@ -698,17 +712,26 @@ __global__ void __launch_bounds__(kThreads)
                     sv[u] = sg[sa ? a + int64_t(pos) * m : pos + int64_t(a) * m];
                 }
 #pragma unroll
-                for (int u = 0; u < kResBatch; ++u) xs[u] = x[o[u]];
+                for (int c = 0; c < kResCols; ++c)
+                    if (c < cw) {
+                        double2 xs[kResBatch];
 #pragma unroll
-                for (int u = 0; u < kResBatch; ++u)
-                    if (a0 + u < m) acc = csub(acc, cmul(sv[u], xs[u]));
+                        for (int u = 0; u < kResBatch; ++u) xs[u] = x[o[u] + c * ldx];
+#pragma unroll
+                        for (int u = 0; u < kResBatch; ++u)
+                            if (a0 + u < m) acc[c] = csub(acc[c], cmul(sv[u], xs[u]));
+                    }
             }
         };
         const int lp = left_pos[r], rp = right_pos[r];
         if (lp >= 0) slice(sl, lp, left_orbs, nl);
         if (rp >= 0) slice(sr, rp, right_orbs, nr);
-        const int64_t at = r + int64_t(col) * xv.ldx;
-        D[at] = csub(B[at], acc);
+#pragma unroll
+        for (int c = 0; c < kResCols; ++c)
+            if (c < cw) {
+                const int64_t at = r + (c0 + c) * ldx;
+                D[at] = csub(B[at], acc[c]);
+            }
     }
 }
 
@@ -949,7 +972,8 @@ void launch_residual(const PointsView& pv, const SolveView& xv, const double2* b
                      const int32_t* right_pos, const int32_t* left_orbs, int32_t nl, const int32_t* right_orbs,
                      int32_t nr, int32_t points, hipStream_t st, bool transpose) {
     if (points <= 0 || xv.c <= 0) return;
-    hipLaunchKernelGGL(residual_kernel, rows_grid(int64_t(nrows) * xv.c, points), dim3(kThreads), 0, st, pv, xv, b, d,
+    hipLaunchKernelGGL(residual_kernel, rows_grid(int64_t(nrows) * ((xv.c + kResCols - 1) / kResCols), points),
+                       dim3(kThreads), 0, st, pv, xv, b, d,
                        rows, nrows, row_ptr, cols, left_pos, right_pos, left_orbs, nl, right_orbs, nr,
                        transpose ? 1 : 0);
 }
@@ -989,23 +1013,28 @@ void launch_gather(const SolveView& xv, const int32_t* orbs, int32_t m, double2*
 using SolveKernel = void (*)(const NodeDesc*, const int32_t*, PointsView, SolveView);
 
 // Shape of a blocked solve launch over fronts of at most `rows` rows and c columns: threads,
-// columns per workgroup (kSolveCols, or one when c is one or the LDS would pass 64 KB; one column
-// fits up to kSolveMaxLds) and LDS bytes (the columns, then the partial sums).
+// columns per workgroup (c itself up to two, kSolveCols up to four, else kSolveColsWide; fewer when
+// the LDS would pass 64 KB; one column fits up to kSolveMaxLds) and LDS bytes (the columns, then
+// the partial sums).
 struct SolveShape {
     int nt, cc;
     size_t lds;
 };
-SolveShape solve_shape(int rows, int c, int nt) {
+SolveShape solve_shape(int rows, int c, int nt, bool single) {
     auto bytes = [&](int cc) { return size_t(rows + nt) * cc * sizeof(double2); };
-    int cc = c <= 1 ? 1 : kSolveCols;
+    // kSolveColsWide for complex f32 factors only (complex f64 ones would spill at that width).
+    int cc = c <= 2 ? std::max(c, 1) : (c <= kSolveCols || !single) ? kSolveCols : kSolveColsWide;
+    if (cc == kSolveColsWide && bytes(cc) > 64 * 1024) cc = kSolveCols;
     if (bytes(cc) > 64 * 1024) cc = 1;
     return {nt, cc, bytes(cc)};
 }
-SolveShape solve_shape(int rows, int c) { return solve_shape(rows, c, rows <= 128 ? 64 : 256); }
+SolveShape solve_shape(int rows, int c, bool single) { return solve_shape(rows, c, rows <= 128 ? 64 : 256, single); }
 
 template <template <class, int, int> class K, class TF, int NT>
 SolveKernel by_cc(int cc) {
-    return cc == kSolveCols ? K<TF, NT, kSolveCols>::fn : K<TF, NT, 1>::fn;
+    if constexpr (std::is_same<TF, float2>::value)
+        if (cc == kSolveColsWide) return K<TF, NT, kSolveColsWide>::fn;
+    return cc == kSolveCols ? K<TF, NT, kSolveCols>::fn : cc == 2 ? K<TF, NT, 2>::fn : K<TF, NT, 1>::fn;
 }
 template <class TF, int NT, int CC>
 struct Fwd {
@@ -1043,7 +1072,7 @@ void allow_solve_lds(SolveKernel k, size_t lds) {
                                   kSolveMaxLds);
 }
 
-bool solve_fits(int rows) { return solve_shape(rows, 1, 256).lds <= size_t(kSolveMaxLds); }
+bool solve_fits(int rows) { return solve_shape(rows, 1, 256, false).lds <= size_t(kSolveMaxLds); }
 
 void launch_solve_forward(const NodeDesc* nodes, const int32_t* level, int32_t count, int32_t max_n,
                           const PointsView& pv, const SolveView& sv, int32_t points, bool single, bool transpose,
@@ -1057,7 +1086,7 @@ void launch_solve_forward(const NodeDesc* nodes, const int32_t* level, int32_t c
                            size_t(kSolveLds) * sizeof(double2), st, nodes, level, pv, sv);
         return;
     }
-    const SolveShape s = solve_shape(max_n, sv.c);
+    const SolveShape s = solve_shape(max_n, sv.c, single);
     SolveKernel k = single ? pick<Fwd, float2>(s) : pick<Fwd, double2>(s);
     allow_solve_lds(k, s.lds);
     hipLaunchKernelGGL(k, dim3(unsigned(count * ((sv.c + s.cc - 1) / s.cc)), unsigned(points)), dim3(s.nt), s.lds, st,
@@ -1068,7 +1097,7 @@ void launch_solve_backward(const NodeDesc* nodes, const int32_t* level, int32_t 
                            const PointsView& pv, const SolveView& sv, int32_t points, bool single, bool transpose,
                            hipStream_t st) {
     if (count <= 0 || points <= 0 || sv.c <= 0) return;
-    const SolveShape s = solve_shape(max_n, sv.c);
+    const SolveShape s = solve_shape(max_n, sv.c, single);
     SolveKernel k = transpose ? (single ? pick<BwdT, float2>(s) : pick<BwdT, double2>(s))
                               : (single ? pick<Bwd, float2>(s) : pick<Bwd, double2>(s));
     allow_solve_lds(k, s.lds);
@@ -1079,7 +1108,7 @@ void launch_solve_backward(const NodeDesc* nodes, const int32_t* level, int32_t 
 void launch_solve_root(const NodeDesc* nodes, const int32_t* level, int32_t root_ne, const PointsView& pv,
                        const SolveView& sv, int32_t points, bool transpose, hipStream_t st) {
     if (points <= 0 || sv.c <= 0) return;
-    const SolveShape s = solve_shape(root_ne, sv.c, 256);
+    const SolveShape s = solve_shape(root_ne, sv.c, 256, false);
     const dim3 grid(unsigned((sv.c + s.cc - 1) / s.cc), unsigned(points));
     const SolveKernel k = transpose ? by_cc<RootT, double2, 256>(s.cc) : by_cc<Root, double2, 256>(s.cc);
     allow_solve_lds(k, s.lds);
