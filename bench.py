@@ -7,7 +7,8 @@ step, complex128. The lead self-energies are computed once per energy on the hos
 by the configurations (crates/caroli-cli/src/run.rs:113-127); they are inputs, resident in HBM with
 the Hamiltonians before the timed region.
 
-Other workloads: `--config c3` zigzag graphene 512 x 64 two-site cells, Anderson w = 1, 256
+Other workloads: `--config c1` the clean BHZ ribbon 16 x 10 at 33 energies (complex128); `--config
+c3` zigzag graphene 512 x 64 two-site cells, Anderson w = 1, 256
 energies x 32 configurations (8,192 points; refined mixed precision by default); `--config c4`
 the adjoint of BHZ 100 x 32 (Anderson w = 40/72.9, E = 0.01; a step is one backward sweep giving
 dT/d(eps) for all 12,800 orbitals); `--config c5` the uneven cubic batch (W = 16.5, lengths 2L,
@@ -50,7 +51,7 @@ PEAK_HBM = 8.0e12
 def workload_tag(args):
     if args.config == "c5":
         return f"cubic_uneven_l{args.max_l}_s{args.systems}"
-    return {"c2": "bhz200x64", "c3": "graphene512x64", "c4": "bhz100x32_adjoint"}[args.config]
+    return {"c1": "bhz16x10", "c2": "bhz200x64", "c3": "graphene512x64", "c4": "bhz100x32_adjoint"}[args.config]
 
 
 def traffic_from_profiles(family, tag, precision):
@@ -127,7 +128,12 @@ def kernels_of(ks, steps):
 
 def build_points(args, ca, threads):
     """Systems, per-point (system, self-energies, energy) lists and the description of a T workload."""
-    if args.config == "c2":
+    if args.config == "c1":
+        systems = [ca.NestedTwoTerminal.preset(ca.Model.BHZ, [16, 10, 0])]
+        energies = np.linspace(-0.16, 0.16, args.energies)
+        desc = f"clean BHZ 16x10, {args.energies} E"
+        groups = [(systems, energies)]
+    elif args.config == "c2":
         systems = [ca.NestedTwoTerminal.preset(ca.Model.BHZ, [args.length, args.width, 0], disorder=2, strength=0.4,
                                                concentration=0.2, seed=1 + c) for c in range(args.configs)]
         energies = np.linspace(-0.16, 0.16, args.energies)
@@ -459,7 +465,7 @@ def main():
                     help="untimed steps (the first runs eagerly, the second captures the sweeps into HIP graphs)")
     ap.add_argument("--roofline-steps", type=int, default=1,
                     help="steps after the timed ones with the per-kernel timers on (units one at a time)")
-    ap.add_argument("--config", default="c2", choices=["c2", "c3", "c4", "c5"])
+    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4", "c5"])
     ap.add_argument("--precision", default=None, choices=["double", "single", "mixed"])
     ap.add_argument("--configs", type=int, default=None, help="disorder configurations (c2: 8, c3: 32)")
     ap.add_argument("--energies", type=int, default=None, help="energies (c2: 33, c3: 256)")
@@ -485,7 +491,8 @@ def main():
     if args.copy_calibration:
         copy_calibration()
         return
-    defaults = {"c2": ("double", 8, 33), "c3": ("mixed", 32, 256), "c4": ("double", 1, 1), "c5": ("double", 1, 4)}
+    defaults = {"c1": ("double", 1, 33), "c2": ("double", 8, 33), "c3": ("mixed", 32, 256), "c4": ("double", 1, 1),
+                "c5": ("double", 1, 4)}
     p, c, e = defaults[args.config]
     args.precision = args.precision or p
     args.configs = args.configs or c

@@ -341,6 +341,9 @@ struct HostPlan {
     int64_t upd_total = 0;  // sum of the fronts' n - ne (a point's update vectors, per column)
     int32_t max_ne = 0;  // largest ne of a front or the root
     int32_t max_n = 0;   // largest n of a front (the root excluded)
+    /// Factor entries one triangular sweep of the solves reads (per front n ne - ne (ne - 1) / 2:
+    /// L11 and L21 forward, U11 and U12 backward; the root excluded).
+    double tri_words = 0.0;
 };
 
 struct DevPlan {
@@ -449,6 +452,7 @@ HostPlan build_plan(const Block& blk, const SparseMatrix& h) {
         for (size_t o : pl.vars) hp.vars.push_back(int32_t(o));
         hp.max_ne = std::max(hp.max_ne, d.ne);
         hp.max_n = std::max(hp.max_n, d.n);
+        hp.tri_words += double(d.n) * d.ne - 0.5 * double(d.ne) * (d.ne - 1);
         off += int64_t(d.n) * d.n;
         piv += d.ne + kPivExtra;
         hp.nodes.push_back(d);
@@ -1180,7 +1184,14 @@ void solve(Engine& eng, const HostPlan& hp, const DevPlan& dp, const PointsView&
     sv.maps = dp.maps.as<int32_t>();
     sv.gather = kept_only ? 0 : 1;
     if (!kept_only && !sv.upd) throw DeviceError("a forward solve needs its update vectors");
-    eng.timed(transpose ? kSolveT : kept_only ? kSolveKept : kSolve, 0.0, 0.0, [&] {
+    // Algorithmic work: every factor entry of the sweeps read once (the root's LU in complex f64),
+    // the right-hand sides read and written once per sweep, 8 flops per complex multiply-add.
+    const double es = single ? sizeof(float2) : sizeof(double2), rne = hp.nodes[hp.root].ne;
+    const double sweeps = kept_only ? 1.0 : 2.0, words = sweeps * hp.tri_words;
+    const double bytes = double(P) * (words * es + rne * rne * sizeof(double2) +
+                                      2.0 * sweeps * double(sv_in.ldx) * sv_in.c * sizeof(double2));
+    const double flops = double(P) * 8.0 * (words + rne * rne) * sv_in.c;
+    eng.timed(transpose ? kSolveT : kept_only ? kSolveKept : kSolve, flops, bytes, [&] {
         if (!kept_only)
             for (size_t l = 0; l < dp.all.size(); ++l)
                 launch_solve_forward(nodes, lv + dp.all[l].first, dp.all[l].second, dp.all_max_n[l], pv, sv, P, single,
