@@ -106,6 +106,9 @@ __device__ __forceinline__ void wave_argmax(double& v, int& idx) {
 // kBatch loads in flight instead of one (the loops are latency-bound otherwise). Indexes past the
 // end are clamped for the loads; only the updates are guarded.
 constexpr int kBatch = 8;
+// Schur entries a lane of the small-front kernels forms at once (their children's loads in flight
+// together).
+constexpr int kSBatch = 2;
 
 // Workgroups of a launch go round-robin to the 8 XCDs (each with its own L2): linear id l runs
 // on XCD l % 8. Tile kernels renumber their (tile, point) pairs so that each XCD receives one
@@ -663,23 +666,57 @@ __global__ void __launch_bounds__(NT)
     __syncthreads();
     // 3. S = F22 - L21 U12 (boundary rows are at their logical positions), written once.
     const T* kidbase = reinterpret_cast<const T*>(pv.arena) + p * pv.arena_stride;
+    // kSBatch entries of a lane at a time, children two at a time: their inverse-map loads, then
+    // their S loads, in flight together (indexes clamped, the sums guarded; each entry sums its
+    // children in child order).
     const int nb2 = nb * nb;
-    for (int idx = tid; idx < nb2; idx += NT) {
-        const int i = ne + idx % nb, j = ne + idx / nb;
-        T f = zero;
+    for (int base = tid; base < nb2; base += NT * kSBatch) {
+        int ii[kSBatch], jj[kSBatch];
+        T f[kSBatch];
+#pragma unroll
+        for (int u = 0; u < kSBatch; ++u) {
+            const int idx = min(base + u * NT, nb2 - 1);
+            ii[u] = ne + idx % nb;
+            jj[u] = ne + idx / nb;
+            f[u] = zero;
+        }
         if (kAssemble) {
-            for (int c = nd.kid_beg; c < nd.kid_end; ++c) {
-                const KidDesc kd = kids[c];
-                const int32_t* inv = maps + kd.inv_beg;
-                const int a = inv[i], b = inv[j];
-                if (a >= 0 && b >= 0) f = cadd(f, kidbase[kd.src + a + int64_t(b) * kd.ld]);
+            for (int c = nd.kid_beg; c < nd.kid_end; c += 2) {
+                const bool two = c + 1 < nd.kid_end;
+                const KidDesc k0 = kids[c], k1 = kids[two ? c + 1 : c];
+                const int32_t *inv0 = maps + k0.inv_beg, *inv1 = maps + k1.inv_beg;
+                int a0[kSBatch], b0[kSBatch], a1[kSBatch], b1[kSBatch];
+#pragma unroll
+                for (int u = 0; u < kSBatch; ++u) {
+                    a0[u] = inv0[ii[u]];
+                    b0[u] = inv0[jj[u]];
+                    a1[u] = inv1[ii[u]];
+                    b1[u] = inv1[jj[u]];
+                }
+                T v0[kSBatch], v1[kSBatch];
+#pragma unroll
+                for (int u = 0; u < kSBatch; ++u) {
+                    v0[u] = kidbase[k0.src + max(a0[u], 0) + int64_t(max(b0[u], 0)) * k0.ld];
+                    v1[u] = kidbase[k1.src + max(a1[u], 0) + int64_t(max(b1[u], 0)) * k1.ld];
+                }
+#pragma unroll
+                for (int u = 0; u < kSBatch; ++u) {
+                    if (a0[u] >= 0 && b0[u] >= 0) f[u] = cadd(f[u], v0[u]);
+                    if (two && a1[u] >= 0 && b1[u] >= 0) f[u] = cadd(f[u], v1[u]);
+                }
             }
         } else {
-            f = F[i + int64_t(j) * n];
+#pragma unroll
+            for (int u = 0; u < kSBatch; ++u) f[u] = F[ii[u] + int64_t(jj[u]) * n];
         }
-        T acc = zero;
-        for (int k = 0; k < ne; ++k) acc = cadd(acc, cmul(Wl[i + k * ldl], Wr[phys[k] + (j - ne) * ldr]));
-        F[i + int64_t(j) * n] = csub(f, acc);
+#pragma unroll
+        for (int u = 0; u < kSBatch; ++u)
+            if (base + u * NT < nb2) {
+                const int i = ii[u], j = jj[u];
+                T acc = zero;
+                for (int k = 0; k < ne; ++k) acc = cadd(acc, cmul(Wl[i + k * ldl], Wr[phys[k] + (j - ne) * ldr]));
+                F[i + int64_t(j) * n] = csub(f[u], acc);
+            }
     }
     // The factors (kept for the solves): the border in logical row order.
     if (pv.keep && r < n) {
@@ -1540,6 +1577,7 @@ __global__ void __launch_bounds__(TS == 64 ? 256 : 64)
 // The factors end as border_kernel leaves them (LAPACK's getrf layout with the interchanges
 // applied to whole separator rows), so the solves and the root's trace read them alike.
 constexpr int kPanelThreads = 512, kSub = 16;
+constexpr int kUpdCols = 2;  // panel columns a lane updates at once in the rank-kSub update
 // Separators above this many variables take the blocked path by default.
 constexpr int kBlockedMinNe = 128;
 
@@ -1686,19 +1724,21 @@ __global__ void __launch_bounds__(kPanelThreads)
             T l[kSub];
 #pragma unroll
             for (int t = 0; t < kSub; ++t) l[t] = t < ws ? at(i, c0 + t) : cx<T>(0.0, 0.0);
+            // kUpdCols columns of the row at a time, their loads in flight together.
 #pragma unroll 1
-            for (int j0 = 0; j0 < wr; j0 += 2) {
-                T a[2];
+            for (int j0 = 0; j0 < wr; j0 += kUpdCols) {
+                T a[kUpdCols];
 #pragma unroll
-                for (int u = 0; u < 2; ++u) a[u] = at(i, r0 + min(j0 + u, wr - 1));
+                for (int u = 0; u < kUpdCols; ++u) a[u] = at(i, r0 + min(j0 + u, wr - 1));
 #pragma unroll
                 for (int t = 0; t < kSub; ++t) {
 #pragma unroll
-                    for (int u = 0; u < 2; ++u) a[u] = csub(a[u], cmul(l[t], us[t * kBlk + min(j0 + u, wr - 1)]));
+                    for (int u = 0; u < kUpdCols; ++u)
+                        a[u] = csub(a[u], cmul(l[t], us[t * kBlk + min(j0 + u, wr - 1)]));
                     __builtin_amdgcn_sched_barrier(0);  // bounds the LDS loads in flight (registers)
                 }
 #pragma unroll
-                for (int u = 0; u < 2; ++u)
+                for (int u = 0; u < kUpdCols; ++u)
                     if (j0 + u < wr) at(i, r0 + j0 + u) = a[u];
                 if (j0 == 0 && i < limn) {
                     const double v = cabs(a[0]);
