@@ -84,23 +84,45 @@ __device__ double block_sum(double v, double* sv) {
     return r;
 }
 
-// Largest value and smallest index on ties over one wavefront, with a payload; every lane gets
-// the result.
-__device__ __forceinline__ void wave_argmax(double& v, int& idx, int& pay) {
-    for (int off = 32; off > 0; off >>= 1) {
-        const double ov = __shfl_xor(v, off, 64);
-        const int oi = __shfl_xor(idx, off, 64), op = __shfl_xor(pay, off, 64);
-        if (ov > v || (ov == v && oi < idx)) {
+// Lane exchanges inside rows of 16 lanes by DPP, without a trip through LDS: quad_perm [1,0,3,2]
+// and [2,3,0,1] swap neighbours and pairs, row_half_mirror reflects groups of 8 lanes and
+// row_mirror groups of 16.
+constexpr int kDppXor1 = 0xB1, kDppXor2 = 0x4E, kDppHalfMirror = 0x141, kDppMirror = 0x140;
+template <int kCtrl>
+__device__ __forceinline__ int dpp(int x) {
+    return __builtin_amdgcn_update_dpp(x, x, kCtrl, 0xF, 0xF, false);
+}
+template <int kCtrl>
+__device__ __forceinline__ double dpp(double x) {
+    return __hiloint2double(dpp<kCtrl>(__double2hiint(x)), dpp<kCtrl>(__double2loint(x)));
+}
+
+// Largest value and smallest key on ties over one wavefront (all lanes active); every lane gets
+// the result. Four rounds by DPP inside rows of 16 lanes, then two across rows; the partners of
+// each round see the same two pairs and keep the same one.
+__device__ __forceinline__ void wave_argmax_key(double& v, int& key) {
+    auto take = [&](double ov, int ok) {
+        if (ov > v || (ov == v && ok < key)) {
             v = ov;
-            idx = oi;
-            pay = op;
+            key = ok;
         }
-    }
+    };
+    take(dpp<kDppXor1>(v), dpp<kDppXor1>(key));
+    take(dpp<kDppXor2>(v), dpp<kDppXor2>(key));
+    take(dpp<kDppHalfMirror>(v), dpp<kDppHalfMirror>(key));
+    take(dpp<kDppMirror>(v), dpp<kDppMirror>(key));
+    take(__shfl_xor(v, 16, 64), __shfl_xor(key, 16, 64));
+    take(__shfl_xor(v, 32, 64), __shfl_xor(key, 32, 64));
 }
-__device__ __forceinline__ void wave_argmax(double& v, int& idx) {
-    int pay = 0;
-    wave_argmax(v, idx, pay);
+// With a payload: idx (below 2^15) and pay (below 2^16) travel as one key, idx in the high half,
+// so that ties still go to the smallest idx.
+__device__ __forceinline__ void wave_argmax(double& v, int& idx, int& pay) {
+    int key = (idx << 16) | pay;
+    wave_argmax_key(v, key);
+    idx = key >> 16;
+    pay = key & 0xFFFF;
 }
+__device__ __forceinline__ void wave_argmax(double& v, int& idx) { wave_argmax_key(v, idx); }
 
 // Assembly loops issue their global loads in batches of kBatch per lane, so that a lane has
 // kBatch loads in flight instead of one (the loops are latency-bound otherwise). Indexes past the
@@ -1277,7 +1299,8 @@ __global__ void __launch_bounds__(256)
     const int n = nd.n, ne = nd.ne, nb = n - ne, tid = threadIdx.x;
     // The triangle the solve reads, packed (ne (ne + 1) / 2 words, half the square): kind 0 the
     // strict lower L11 by columns, L[i][k] at lo(k) + i - k - 1 (i > k); kind 1 U11 by rows,
-    // U[k][j] at up(k) + j - k (j >= k).
+    // U[k][j] at up(k) + j - k (j >= k), with 1 / U[k][k] in place of the diagonal (the solve
+    // only divides by it).
     T* A = reinterpret_cast<T*>(lds_raw);
     auto lo = [&](int k) { return k * ne - k * (k + 1) / 2; };
     auto up = [&](int k) { return k * ne - k * (k - 1) / 2; };
@@ -1295,7 +1318,10 @@ __global__ void __launch_bounds__(256)
     }
     for (int idx = tid; idx < ne * ne; idx += 256) {
         const int i = idx % ne, j = idx / ne;
-        if (upper ? i <= j : i > j) A[upper ? up(i) + j - i : lo(j) + i - j - 1] = F[i + int64_t(j) * n];
+        if (upper ? i <= j : i > j) {
+            const T v = F[i + int64_t(j) * n];
+            A[upper ? up(i) + j - i : lo(j) + i - j - 1] = i == j ? crecip(v) : v;
+        }
     }
     __syncthreads();
     // Four lanes per column of U12 (kind 0) or row of L21 (kind 1), the workgroup's 64 of them in
@@ -1336,7 +1362,7 @@ __global__ void __launch_bounds__(256)
                 const int k = 4 * kb + kk;
                 if (k < ne) {
                     const int rk = up(k) - k;
-                    const T cand = cmul(x[kb], crecip(A[rk + k]));
+                    const T cand = cmul(x[kb], A[rk + k]);
                     T xk;
                     xk.x = __shfl(cand.x, kk, 4);
                     xk.y = __shfl(cand.y, kk, 4);
@@ -1785,9 +1811,13 @@ __global__ void __launch_bounds__(256)
     const int n = nd.n, ne = nd.ne, tid = threadIdx.x, w = min(kBlk, ne - k0), k1 = k0 + w;
     auto at = [&](int i, int j) -> T& { return F[i + int64_t(j) * n]; };
     constexpr int la = kBlk + 1;
-    if (tl.pad == 2) {  // rows j0.. of the L21 strip: l U_kk = f, eight lanes per row (U_kk in LDS)
+    if (tl.pad == 2) {  // rows j0.. of the L21 strip: l U_kk = f, eight lanes per row (U_kk in LDS,
+                        // 1 / U[k][k] in place of its diagonal)
         T* A = X;
-        for (int e = tid; e < w * w; e += 256) A[(e % w) + (e / w) * la] = at(k0 + e % w, k0 + e / w);
+        for (int e = tid; e < w * w; e += 256) {
+            const T v = at(k0 + e % w, k0 + e / w);
+            A[(e % w) + (e / w) * la] = e % w == e / w ? crecip(v) : v;
+        }
         __syncthreads();
         const int rows = min(kRowCols, n - tl.j0), grp = tid / 8, g = tid % 8, row = tl.j0 + min(grp, rows - 1);
         T x[8];
@@ -1799,7 +1829,7 @@ __global__ void __launch_bounds__(256)
             for (int kk = 0; kk < 8; ++kk) {
                 const int k = 8 * kb + kk;
                 if (k < w) {
-                    const T cand = cmul(x[kb], crecip(A[k + k * la]));
+                    const T cand = cmul(x[kb], A[k + k * la]);
                     T xk;
                     xk.x = __shfl(cand.x, kk, 8);
                     xk.y = __shfl(cand.y, kk, 8);

@@ -13,6 +13,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB = os.path.join(ROOT, "oracle", "_ref", "liboracle.so")
 LIB_LD = os.path.join(ROOT, "oracle", "_ref", "liboracle_ld.so")
+LIB_FMA = os.path.join(ROOT, "oracle", "_ref", "liboracle_fma.so")
 
 
 def _load(path=LIB):
@@ -44,6 +45,7 @@ def _load(path=LIB):
 
 lib = _load()
 _lib_ld = None
+_lib_fma = None
 
 
 def _ld():
@@ -54,6 +56,14 @@ def _ld():
     return _lib_ld
 
 
+def _fma():
+    """The double build with contracted multiply-adds (loaded on first use; needs an FMA host)."""
+    global _lib_fma
+    if _lib_fma is None:
+        _lib_fma = _load(LIB_FMA)
+    return _lib_fma
+
+
 def _d(a):
     return a.ctypes.data_as(C.POINTER(C.c_double))
 
@@ -61,9 +71,10 @@ def _d(a):
 class OracleSystem:
     """The oracle's copy of a system, built from the arrays of the engine's system (or any arrays)."""
 
-    def __init__(self, arrays, transpose=False, extended=False):
-        """extended: the long-double build (64-bit mantissa) instead of the double one."""
-        self.lib = _ld() if extended else lib
+    def __init__(self, arrays, transpose=False, extended=False, fma=False):
+        """extended: the long-double build (64-bit mantissa) instead of the double one; fma: the
+        double build with contracted multiply-adds (another rounding order of the same steps)."""
+        self.lib = _ld() if extended else (_fma() if fma else lib)
         a = dict(arrays)
         vals = np.ascontiguousarray(a["vals"], np.complex128)
         mats = [np.asfortranarray(a[k], np.complex128) for k in ("left_h00", "left_h01", "right_h00", "right_h01")]

@@ -6,9 +6,11 @@ oracle would take minutes to hours per point on the GPU box's host.
 bhz200x64_magnetic.json (equal-size batch): BHZ 200 x 64 sites, polarized magnetic impurities
 (concentration 0.2, J = 0.4, opposite g), configurations seeded 1 + c for c = 0..7 (the CLI's
 default seeding, crates/caroli-cli/src/build.rs:281), 33 energies linspace(-0.16, 0.16). For
-every point: T from the double oracle and T from the long-double oracle (the same elimination
-with a 64-bit mantissa), both with the engine's host self-energies. |T_double - T_long| is the
-rounding floor of a double-precision nested dissection on that point.
+every point: T from the double oracle, from the long-double oracle (the same elimination with a
+64-bit mantissa) and from the double oracle built with contracted multiply-adds (the same
+elimination in double precision, rounded in another order), all with the engine's host
+self-energies. The two double values, against the long-double one, sample the rounding error of a
+double-precision nested dissection on that point.
 
 cubic_large.json (uneven batch): cubic Anderson model W = 16.5, 2L x L x L, the systems s of the
 512-system list (L_s = [16, 24, 32, 40, 48][floor(5 u_s)], u from xoshiro256** seeded 5) with
@@ -62,16 +64,18 @@ def bhz_config(c):
 
     s = bhz_system(c)
     arr = s.arrays()
-    o, oe = OracleSystem(arr), OracleSystem(arr, extended=True)
+    o, oe, of = OracleSystem(arr), OracleSystem(arr, extended=True), OracleSystem(arr, fma=True)
     rows = []
     for e in BHZ_ENERGIES:
         sig = s.self_energies(float(e))
         blocks = [sig.block(b)[:2] for b in range(s.block_count())]
         td, rc = o.transmission_with(float(e), blocks)
         tl, rc2 = oe.transmission_with(float(e), blocks)
-        assert rc == 0 and rc2 == 0
-        rows.append((float(e), td, tl))
-        print(f"bhz c={c} E={e:+.4f} T={td:.15f} |T - T_long|={abs(td - tl):.2e}", flush=True)
+        tf, rc3 = of.transmission_with(float(e), blocks)
+        assert rc == 0 and rc2 == 0 and rc3 == 0
+        rows.append((float(e), td, tl, tf))
+        print(f"bhz c={c} E={e:+.4f} T={td:.15f} |T - T_long|={abs(td - tl):.2e} (fma {abs(tf - tl):.2e})",
+              flush=True)
     return c, rows
 
 
@@ -101,8 +105,9 @@ def main():
             out = dict(ex.map(bhz_config, range(8)))
         fixture = {"workload": "BHZ 200x64 magnetic c=0.2 J=0.4 opposite-g, configurations seeded 1+c, "
                                "33 energies linspace(-0.16, 0.16)",
-                   "columns": ["configuration", "energy", "T_double_oracle", "T_long_double_oracle"],
-                   "points": [[c, e, td, tl] for c in range(8) for (e, td, tl) in out[c]]}
+                   "columns": ["configuration", "energy", "T_double_oracle", "T_long_double_oracle",
+                               "T_double_fma_oracle"],
+                   "points": [[c, e, td, tl, tf] for c in range(8) for (e, td, tl, tf) in out[c]]}
         path = os.path.join(HERE, "bhz200x64_magnetic.json")
     else:
         sizes = cubic_sizes()

@@ -196,6 +196,40 @@ def test_oracle_delays_on_singular_blocks():
     assert rc == 0 and abs(t - round(t)) < 1e-9
 
 
+def has_fma():
+    try:
+        return " fma " in open("/proc/cpuinfo").read().replace("\n", " ")
+    except OSError:
+        return False
+
+
+@pytest.mark.skipif(not has_fma(), reason="the contracted build of the oracle needs FMA instructions")
+@pytest.mark.parametrize("config, k", [(0, 0), (5, 2)])
+def test_c2_fixture_points_recompute(config, k):
+    """Points of tests/golden/bhz200x64_magnetic.json recomputed on the CPU from the three oracle
+    builds (double, long double, double with contracted multiply-adds): each value is reproduced
+    exactly. On the second point (configuration 5, E = -0.14) the two double evaluations differ by
+    about 1.5e-9, a rounding error of double precision alone, against which the long-double value
+    is the reference."""
+    fx = np.array(json.load(open(os.path.join(ROOT, "tests", "golden", "bhz200x64_magnetic.json")))["points"])
+    row = fx[config * 33 + k]
+    e = float(np.linspace(-0.16, 0.16, 33)[k])
+    assert row[0] == config and row[1] == e
+    s = ca.NestedTwoTerminal.preset(ca.Model.BHZ, [200, 64, 0], disorder=2, strength=0.4, concentration=0.2,
+                                    seed=1 + config)
+    arr = s.arrays()
+    _, sig = sigmas_of(s, e)
+    got = []
+    for kw in (dict(), dict(extended=True), dict(fma=True)):
+        t, rc = OracleSystem(arr, **kw).transmission_with(e, sig)
+        assert rc == 0
+        got.append(t)
+    assert got == [row[2], row[3], row[4]]
+    spread = abs(got[0] - got[2])
+    assert (spread > 1e-9) == (config == 5), spread
+    assert abs(got[0] - got[1]) < 2e-10 and abs(got[2] - got[1]) < 2e-9
+
+
 def channel_count(w, e):
     """Open channels of a clean square ribbon of width w at energy e (t = 1): transverse modes
     eps_k = -2 cos(k pi / (w + 1)) with |e - eps_k| < 2."""
