@@ -51,6 +51,11 @@ template <class T>
 __device__ __forceinline__ double cabs(T a) {
     return hypot(double(a.x), double(a.y));
 }
+// Squared modulus: the pivot searches compare these (the order of the moduli, no square root).
+template <class T>
+__device__ __forceinline__ double cabs2(T a) {
+    return double(a.x) * double(a.x) + double(a.y) * double(a.y);
+}
 template <class T>
 __device__ __forceinline__ T crecip(T a) {
     // 1 / a with one division of the squared modulus, scaled against overflow.

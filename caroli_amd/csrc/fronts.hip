@@ -35,35 +35,6 @@ namespace {
 constexpr int kThreads = 256;
 
 
-// Block-wide max of (value, -index): the largest value, the smallest index on ties.
-__device__ void block_argmax(double& v, int& idx, double* sv, int* si) {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    for (int off = 32; off > 0; off >>= 1) {
-        double ov = __shfl_xor(v, off, 64);
-        int oi = __shfl_xor(idx, off, 64);
-        if (ov > v || (ov == v && oi < idx)) {
-            v = ov;
-            idx = oi;
-        }
-    }
-    if (lane == 0) {
-        sv[wave] = v;
-        si[wave] = idx;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        for (int w = 1; w < int(blockDim.x) / 64; ++w)
-            if (sv[w] > sv[0] || (sv[w] == sv[0] && si[w] < si[0])) {
-                sv[0] = sv[w];
-                si[0] = si[w];
-            }
-    }
-    __syncthreads();
-    v = sv[0];
-    idx = si[0];
-    __syncthreads();
-}
-
 __device__ double block_max(double v, double* sv) {
     for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
     if ((threadIdx.x & 63) == 0) sv[threadIdx.x >> 6] = v;
@@ -123,6 +94,33 @@ __device__ __forceinline__ void wave_argmax(double& v, int& idx, int& pay) {
     pay = key & 0xFFFF;
 }
 __device__ __forceinline__ void wave_argmax(double& v, int& idx) { wave_argmax_key(v, idx); }
+
+// Block-wide max of (value, -index): the largest value, the smallest index on ties, in every
+// lane. sv / si hold one entry per wave; the caller separates consecutive calls by a barrier.
+__device__ void block_argmax(double& v, int& idx, double* sv, int* si) {
+    wave_argmax_key(v, idx);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) {
+        sv[wave] = v;
+        si[wave] = idx;
+    }
+    __syncthreads();
+    for (int w = 0; w < int(blockDim.x) / 64; ++w) {
+        const double ov = sv[w];
+        const int oi = si[w];
+        if (ov > v || (ov == v && oi < idx)) {
+            v = ov;
+            idx = oi;
+        }
+    }
+}
+
+// The pivot test of field.rs:61 on the squared modulus big of the pivot: |p| > tol scale on the
+// tested columns (the first ne1), else p nonzero and finite.
+__device__ __forceinline__ bool pivot_ok(double big, bool tested, double tol, double scale) {
+    const double ts = tol * scale;
+    return tested ? big > ts * ts : (big > 0.0 && big < INFINITY);
+}
 
 // Assembly loops issue their global loads in batches of kBatch per lane, so that a lane has
 // kBatch loads in flight instead of one (the loops are latency-bound otherwise). Indexes past the
@@ -351,14 +349,14 @@ __device__ void eliminate(T* W, const NodeDesc& nd, int32_t* piv, int32_t* statu
         double big = -1.0;
         int at = k;
         for (int i = k + tid; i < lim; i += kThreads) {
-            double v = cabs(W[i + int64_t(k) * n]);
+            double v = cabs2(W[i + int64_t(k) * n]);
             if (v > big) {
                 big = v;
                 at = i;
             }
         }
         block_argmax(big, at, sv, si);
-        const bool ok = k < ne1 ? (big > tol * scale) : (big > 0.0 && big < INFINITY);
+        const bool ok = pivot_ok(big, k < ne1, tol, scale);
         if (!ok) {
             if (tid == 0) *status = (k < ne1 && !root) ? kFrontDelayed : kFrontSingular;
             return;
@@ -434,7 +432,7 @@ __device__ __forceinline__ bool lds_lu(T* W, int ld, int n, int ne, int ne1, dou
         double big = -1.0;
         int at = kLdsMaxN, ph = 0;  // pivot: value, logical row, physical row
         if (g == 0 && r < n && lg >= k && lg < lim) {
-            big = cabs(W[r + k * ld]);
+            big = cabs2(W[r + k * ld]);
             at = lg;
             ph = r;
         }
@@ -457,7 +455,7 @@ __device__ __forceinline__ bool lds_lu(T* W, int ld, int n, int ne, int ne1, dou
                 }
             }
         }
-        const bool ok = k < ne1 ? (big > tol * scale) : (big > 0.0 && big < INFINITY);
+        const bool ok = pivot_ok(big, k < ne1, tol, scale);
         if (!ok) {
             if (tid == 0) *status = k < ne1 ? kFrontDelayed : kFrontSingular;
             return false;
@@ -638,7 +636,7 @@ __global__ void __launch_bounds__(NT)
             double big = -1.0;
             int pat = kLdsMaxN, ph = 0;  // pivot: value, logical row, physical row
             if (g == 0 && r < n && lg >= k && lg < lim) {
-                big = cabs(Wl[r + k * ldl]);
+                big = cabs2(Wl[r + k * ldl]);
                 pat = lg;
                 ph = r;
             }
@@ -661,7 +659,7 @@ __global__ void __launch_bounds__(NT)
                     }
                 }
             }
-            const bool ok = k < ne1 ? (big > tol * scale) : (big > 0.0 && big < INFINITY);
+            const bool ok = pivot_ok(big, k < ne1, tol, scale);
             if (!ok) {
                 if (tid == 0) *status = k < ne1 ? kFrontDelayed : kFrontSingular;
                 return;
@@ -851,12 +849,12 @@ __global__ void __launch_bounds__(64 * NW)
             double big = -1.0;
             int at = kLdsMaxN, ph = 0;
             if (r < n && lg >= k && lg < lim) {
-                big = cabs(colk);
+                big = cabs2(colk);
                 at = lg;
                 ph = r;
             }
             wave_argmax(big, at, ph);
-            const bool ok = k < ne1 ? (big > tol * scale) : (big > 0.0 && big < INFINITY);
+            const bool ok = pivot_ok(big, k < ne1, tol, scale);
             const int phs = __builtin_amdgcn_readfirstlane(ph);
             const int lgn = r == phs ? k : (lg == k ? at : lg);
             T l = zero;
@@ -997,14 +995,14 @@ __device__ __forceinline__ void border_factor(T* F, T* P, T* buf, int32_t* spiv,
             double big = -1.0;
             int q = k;
             for (int i = k + lane; i < lim; i += 64) {
-                const double v = cabs(pp(i, c));
+                const double v = cabs2(pp(i, c));
                 if (v > big) {
                     big = v;
                     q = i;
                 }
             }
             wave_argmax(big, q);  // every wave finds the same pivot
-            const bool ok = k < ne1 ? (big > tol * scale) : (big > 0.0 && big < INFINITY);
+            const bool ok = pivot_ok(big, k < ne1, tol, scale);
             if (!ok) {
                 if (tid == 0) *status = (k < ne1 && !kRoot) ? kFrontDelayed : kFrontSingular;
                 return;
@@ -1666,7 +1664,7 @@ __global__ void __launch_bounds__(kPanelThreads)
     double big = -1.0;
     int q = k0;
     for (int i = k0 + tid; i < (k0 < ne1 ? ne1 : ne); i += NT) {
-        const double v = cabs(at(i, k0));
+        const double v = cabs2(at(i, k0));
         if (v > big) {
             big = v;
             q = i;
@@ -1677,7 +1675,7 @@ __global__ void __launch_bounds__(kPanelThreads)
         for (int c = 0; c < ws; ++c) {
             const int k = c0 + c;
             block_argmax(big, q, sv, si);
-            const bool ok = k < ne1 ? (big > tol * scale) : (big > 0.0 && big < INFINITY);
+            const bool ok = pivot_ok(big, k < ne1, tol, scale);
             if (!ok) {
                 if (tid == 0) *status = (k < ne1 && !kRoot) ? kFrontDelayed : kFrontSingular;
                 return;
@@ -1711,7 +1709,7 @@ __global__ void __launch_bounds__(kPanelThreads)
                 for (int t = 1; t < kSub; ++t)
                     if (t < ws - c) at(i, k + t) = v[t];
                 if (track && i < limn) {
-                    const double a = cabs(v[1]);
+                    const double a = cabs2(v[1]);
                     if (a > big) {
                         big = a;
                         q = i;
@@ -1767,7 +1765,7 @@ __global__ void __launch_bounds__(kPanelThreads)
                 for (int u = 0; u < kUpdCols; ++u)
                     if (j0 + u < wr) at(i, r0 + j0 + u) = a[u];
                 if (j0 == 0 && i < limn) {
-                    const double v = cabs(a[0]);
+                    const double v = cabs2(a[0]);
                     if (v > big) {
                         big = v;
                         q = i;
