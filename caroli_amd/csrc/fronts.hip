@@ -130,6 +130,21 @@ constexpr int kBatch = 8;
 // together).
 constexpr int kSBatch = 2;
 
+// dst(idx, src(idx)) for idx < count over the NT lanes, kBatch loads of a lane in flight together
+// (indexes clamped for the loads, the stores guarded): the copies of fronts and triangles into
+// LDS, which otherwise wait for each load before the next.
+template <int NT, class T, class Src, class Dst>
+__device__ __forceinline__ void copy_batched(int count, Src src, Dst dst) {
+    for (int base = int(threadIdx.x); base < count; base += NT * kBatch) {
+        T v[kBatch];
+#pragma unroll
+        for (int u = 0; u < kBatch; ++u) v[u] = src(min(base + u * NT, count - 1));
+#pragma unroll
+        for (int u = 0; u < kBatch; ++u)
+            if (base + u * NT < count) dst(base + u * NT, v[u]);
+    }
+}
+
 // Workgroups of a launch go round-robin to the 8 XCDs (each with its own L2): linear id l runs
 // on XCD l % 8. Tile kernels renumber their (tile, point) pairs so that each XCD receives one
 // contiguous range, and neighbouring tiles of one front and point share an L2.
@@ -524,7 +539,8 @@ __global__ void __launch_bounds__(NT)
             __syncthreads();
         }
     } else {
-        for (int idx = tid; idx < n * n; idx += NT) W[(idx % n) + (idx / n) * ld] = F[idx];
+        copy_batched<NT, T>(n * n, [&](int idx) { return F[idx]; },
+                            [&](int idx, T v) { W[(idx % n) + (idx / n) * ld] = v; });
         __syncthreads();
     }
     const int g = tid % G, r = tid / G;  // this lane's column class and physical row
@@ -550,7 +566,8 @@ __global__ void __launch_bounds__(NT)
 //     land in it (their F22 entries are not read here);
 //  2. the partial LU over the separator columns with pivots among the separator rows, as lds_lu:
 //     G lanes per physical row, logical positions swapped only, the pivot test of field.rs:61;
-//     boundary rows (never pivots) update their separator columns only;
+//     every row updates its separator columns only, and the separator rows' boundary columns
+//     (U12) follow in one triangular solve, a column per lane;
 //  3. S = F22 - L21 U12 entry by entry: F22 gathered from the children through their inverse maps
 //     (or read from the given front), the product from LDS, S written once.
 // The factors written back (when kept) are those of front_lds_kernel.
@@ -619,9 +636,10 @@ __global__ void __launch_bounds__(NT)
             __syncthreads();
         }
     } else {
-        for (int idx = tid; idx < n * ne; idx += NT) Wl[(idx % n) + (idx / n) * ldl] = F[idx];
-        for (int idx = tid; idx < ne * nb; idx += NT)
-            Wr[(idx % ne) + (idx / ne) * ldr] = F[(idx % ne) + int64_t(ne + idx / ne) * n];
+        copy_batched<NT, T>(n * ne, [&](int idx) { return F[idx]; },
+                            [&](int idx, T v) { Wl[(idx % n) + (idx / n) * ldl] = v; });
+        copy_batched<NT, T>(ne * nb, [&](int idx) { return F[(idx % ne) + int64_t(ne + idx / ne) * n]; },
+                            [&](int idx, T v) { Wr[(idx % ne) + (idx / ne) * ldr] = v; });
         __syncthreads();
     }
     // 2. Partial LU of the border (lds_lu on the split layout).
@@ -671,18 +689,31 @@ __global__ void __launch_bounds__(NT)
                 lg = pat;
             if (r < n && lg > k) {
                 // The G lanes of this row read W[r, k] before lane g = 0 overwrites it (one wavefront).
+                // The separator columns only: the boundary columns of the separator rows (U12)
+                // follow in one triangular solve after the loop.
                 const T l = cmul(Wl[r + k * ldl], crecip(Wl[ph + k * ldl]));
                 const int j0 = k + 1 + ((g - (k + 1)) % G + G) % G;
-                int j = j0;
-                for (; j < ne; j += G) Wl[r + j * ldl] = csub(Wl[r + j * ldl], cmul(l, Wl[ph + j * ldl]));
-                if (r < ne)  // separator rows: their boundary columns too
-                    for (; j < n; j += G) Wr[r + (j - ne) * ldr] = csub(Wr[r + (j - ne) * ldr], cmul(l, Wr[ph + (j - ne) * ldr]));
+                for (int j = j0; j < ne; j += G) Wl[r + j * ldl] = csub(Wl[r + j * ldl], cmul(l, Wl[ph + j * ldl]));
                 if (g == 0) Wl[r + k * ldl] = l;
             }
             __syncthreads();
         }
     }
     if (g == 0 && r < ne) phys[lg] = r;  // separator rows keep logical positions below ne
+    __syncthreads();
+    // U12 = L11^-1 P F12, a boundary column per lane: the separator rows in logical order k at
+    // physical rows phys[k], L11[k][m] = Wl[phys[k] + m ldl]. Each entry takes the updates of
+    // the pivots before it in pivot order, as the right-looking elimination would have applied
+    // them, without a barrier per pivot.
+    for (int jc = tid; jc < nb; jc += NT) {
+        T* col = Wr + jc * ldr;
+        for (int k = 1; k < ne; ++k) {
+            const int pk = phys[k];
+            T acc = col[pk];
+            for (int m = 0; m < k; ++m) acc = csub(acc, cmul(Wl[pk + m * ldl], col[phys[m]]));
+            col[pk] = acc;
+        }
+    }
     __syncthreads();
     // 3. S = F22 - L21 U12 (boundary rows are at their logical positions), written once.
     const T* kidbase = reinterpret_cast<const T*>(pv.arena) + p * pv.arena_stride;
@@ -817,7 +848,8 @@ __global__ void __launch_bounds__(64 * NW)
             __syncthreads();
         }
     } else {
-        for (int idx = tid; idx < n * n; idx += NT) W[(idx % n) + (idx / n) * ld] = F[idx];
+        copy_batched<NT, T>(n * n, [&](int idx) { return F[idx]; },
+                            [&](int idx, T v) { W[(idx % n) + (idx / n) * ld] = v; });
         __syncthreads();
     }
     const int r = tid & 63, wave = tid >> 6;
@@ -974,6 +1006,7 @@ __device__ __forceinline__ void mfma_gemm_tile(T* F, int n, int i0, int j0, int 
 template <class T, bool kRoot, bool kLds, int NT>
 __device__ __forceinline__ void border_factor(T* F, T* P, T* buf, int32_t* spiv, int32_t* mv, int n, int ne, int ne1,
                                               int32_t* piv, int32_t* status, double scale, double tol) {
+    __shared__ T dinv[kPanel];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     auto at = [&](int i, int j) -> T& { return F[i + int64_t(j) * n]; };
     for (int k0 = 0; k0 < ne; k0 += kPanel) {
@@ -986,7 +1019,7 @@ __device__ __forceinline__ void border_factor(T* F, T* P, T* buf, int32_t* spiv,
         // Panel element (row i in [k0, ne), panel column c).
         auto pp = [&](int i, int c) -> T& { return kLds ? P[(i - k0) + c * R] : at(i, k0 + c); };
         if (kLds) {
-            for (int e = tid; e < R * w; e += NT) P[e] = at(k0 + e % R, k0 + e / R);
+            copy_batched<NT, T>(R * w, [&](int e) { return at(k0 + e % R, k0 + e / R); }, [&](int e, T v) { P[e] = v; });
             __syncthreads();
         }
         // 2. Panel factorisation on the separator rows.
@@ -1025,6 +1058,8 @@ __device__ __forceinline__ void border_factor(T* F, T* P, T* buf, int32_t* spiv,
             }
             __syncthreads();
         }
+        // The reciprocals of the panel's pivots, for the L21 rows of step 3.
+        if (tid < w) dinv[tid] = crecip(pp(k0 + tid, tid));
         // The interchanges k0 + c <-> spiv[c], in order, as one permutation: row mv[1 + m] of the
         // front receives the old row mv[1 + 2 kPanel + m].
         if (tid == 0) {  // rows touched in mv[1 + 2 kPanel + s] (positions) and mv[1 + s] (sources)
@@ -1098,7 +1133,7 @@ __device__ __forceinline__ void border_factor(T* F, T* P, T* buf, int32_t* spiv,
                     T acc = l[c];
 #pragma unroll
                     for (int c2 = 0; c2 < c; ++c2) acc = csub(acc, cmul(l[c2], pp(k0 + c2, c)));
-                    l[c] = cmul(acc, crecip(pp(k0 + c, c)));
+                    l[c] = cmul(acc, dinv[c]);
                 }
             }
 #pragma unroll
@@ -1209,21 +1244,31 @@ __global__ void __launch_bounds__(256)
         T acc[kGBatch];
 #pragma unroll
         for (int u = 0; u < kGBatch; ++u) acc[u] = cx<T>(0.0, 0.0);
-        for (int c = nd.kid_beg; c < nd.kid_end; ++c) {  // per child: all map loads, then all values
-            const KidDesc kd = kids[c];
-            const int32_t* inv = maps + kd.inv_beg;
-            const T* S = kidbase + kd.src;
-            int a[kGBatch], b[kGBatch];
+        // Children two at a time: the map loads of both, then the values of both, in flight
+        // together; each entry sums its children in child order.
+        for (int c = nd.kid_beg; c < nd.kid_end; c += 2) {
+            const bool two = c + 1 < nd.kid_end;
+            const KidDesc k0 = kids[c], k1 = kids[two ? c + 1 : c];
+            const int32_t *inv0 = maps + k0.inv_beg, *inv1 = maps + k1.inv_beg;
+            const T *S0 = kidbase + k0.src, *S1 = kidbase + k1.src;
+            int a0[kGBatch], b0[kGBatch], a1[kGBatch], b1[kGBatch];
 #pragma unroll
             for (int u = 0; u < kGBatch; ++u) {
-                a[u] = inv[ii[u]];
-                b[u] = inv[jj[u]];
+                a0[u] = inv0[ii[u]];
+                b0[u] = inv0[jj[u]];
+                a1[u] = inv1[ii[u]];
+                b1[u] = inv1[jj[u]];
+            }
+            T v0[kGBatch], v1[kGBatch];
+#pragma unroll
+            for (int u = 0; u < kGBatch; ++u) {
+                v0[u] = S0[max(a0[u], 0) + int64_t(max(b0[u], 0)) * k0.ld];
+                v1[u] = S1[max(a1[u], 0) + int64_t(max(b1[u], 0)) * k1.ld];
             }
 #pragma unroll
             for (int u = 0; u < kGBatch; ++u) {
-                T v = S[max(a[u], 0) + int64_t(max(b[u], 0)) * kd.ld];
-                if (a[u] < 0 || b[u] < 0) v = cx<T>(0.0, 0.0);
-                acc[u] = cadd(acc[u], v);
+                acc[u] = cadd(acc[u], a0[u] < 0 || b0[u] < 0 ? cx<T>(0.0, 0.0) : v0[u]);
+                if (two) acc[u] = cadd(acc[u], a1[u] < 0 || b1[u] < 0 ? cx<T>(0.0, 0.0) : v1[u]);
             }
         }
 #pragma unroll
@@ -1253,7 +1298,8 @@ __global__ void __launch_bounds__(256)
     const int n = nd.n, ne = nd.ne, ne1 = nd.ne1, tid = threadIdx.x, ld = ne + 1;
     // F11 (the gathered children's part) into LDS, then -H and z added there; the entries in the
     // border strips F12 / F21 are added in place in HBM (trsm_kernel reads them next).
-    for (int idx = tid; idx < ne * ne; idx += 256) W[(idx % ne) + (idx / ne) * ld] = F[(idx % ne) + int64_t(idx / ne) * n];
+    copy_batched<256, T>(ne * ne, [&](int idx) { return F[(idx % ne) + int64_t(idx / ne) * n]; },
+                         [&](int idx, T v) { W[(idx % ne) + (idx / ne) * ld] = v; });
     __syncthreads();
     if (nd.ent_end > nd.ent_beg) {
         add_entries<256>(ent, nd.ent_beg, nd.ent_end, pv.vals[p], n, [&](int at, double2 v) {
@@ -1314,13 +1360,16 @@ __global__ void __launch_bounds__(256)
         }
         if (tid < ne) src[tid] = v;
     }
-    for (int idx = tid; idx < ne * ne; idx += 256) {
-        const int i = idx % ne, j = idx / ne;
-        if (upper ? i <= j : i > j) {
-            const T v = F[i + int64_t(j) * n];
-            A[upper ? up(i) + j - i : lo(j) + i - j - 1] = i == j ? crecip(v) : v;
-        }
-    }
+    auto wanted = [&](int i, int j) { return upper ? i <= j : i > j; };
+    copy_batched<256, T>(ne * ne,
+                         [&](int idx) {
+                             const int i = idx % ne, j = idx / ne;
+                             return wanted(i, j) ? F[i + int64_t(j) * n] : cx<T>(0.0, 0.0);
+                         },
+                         [&](int idx, T v) {
+                             const int i = idx % ne, j = idx / ne;
+                             if (wanted(i, j)) A[upper ? up(i) + j - i : lo(j) + i - j - 1] = i == j ? crecip(v) : v;
+                         });
     __syncthreads();
     // Four lanes per column of U12 (kind 0) or row of L21 (kind 1), the workgroup's 64 of them in
     // one pass: lane g holds entries g + 4 s (s < NS, ne <= 4 NS) in registers, and the solve
@@ -1334,10 +1383,12 @@ __global__ void __launch_bounds__(256)
         const int e = min(g + 4 * s, ne - 1);
         x[s] = upper ? F[(ne + col) + int64_t(e) * n] : F[src[e] + int64_t(ne + col) * n];
     }
+    // The steps k = 4 kb + kk: kb unrolled (x[kb] stays a register), kk a loop, which keeps the
+    // code a quarter of the fully unrolled size.
     if (!upper) {  // forward substitution with the unit-lower L11
 #pragma unroll
         for (int kb = 0; kb < NS; ++kb)
-#pragma unroll
+#pragma unroll 1
             for (int kk = 0; kk < 4; ++kk) {
                 const int k = 4 * kb + kk;
                 if (k < ne - 1) {
@@ -1355,7 +1406,7 @@ __global__ void __launch_bounds__(256)
     } else {  // x U11 = f, column by column
 #pragma unroll
         for (int kb = 0; kb < NS; ++kb)
-#pragma unroll
+#pragma unroll 1
             for (int kk = 0; kk < 4; ++kk) {
                 const int k = 4 * kb + kk;
                 if (k < ne) {
@@ -1812,10 +1863,8 @@ __global__ void __launch_bounds__(256)
     if (tl.pad == 2) {  // rows j0.. of the L21 strip: l U_kk = f, eight lanes per row (U_kk in LDS,
                         // 1 / U[k][k] in place of its diagonal)
         T* A = X;
-        for (int e = tid; e < w * w; e += 256) {
-            const T v = at(k0 + e % w, k0 + e / w);
-            A[(e % w) + (e / w) * la] = e % w == e / w ? crecip(v) : v;
-        }
+        copy_batched<256, T>(w * w, [&](int e) { return at(k0 + e % w, k0 + e / w); },
+                             [&](int e, T v) { A[(e % w) + (e / w) * la] = e % w == e / w ? crecip(v) : v; });
         __syncthreads();
         const int rows = min(kRowCols, n - tl.j0), grp = tid / 8, g = tid % 8, row = tl.j0 + min(grp, rows - 1);
         T x[8];
@@ -1823,8 +1872,8 @@ __global__ void __launch_bounds__(256)
         for (int s = 0; s < 8; ++s) x[s] = at(row, k0 + min(g + 8 * s, w - 1));
 #pragma unroll
         for (int kb = 0; kb < 8; ++kb)
-#pragma unroll
-            for (int kk = 0; kk < 8; ++kk) {
+#pragma unroll 1
+            for (int kk = 0; kk < 8; ++kk) {  // a loop (x[kb] stays a register): an eighth of the code
                 const int k = 8 * kb + kk;
                 if (k < w) {
                     const T cand = cmul(x[kb], A[k + k * la]);
@@ -1859,11 +1908,12 @@ __global__ void __launch_bounds__(256)
         osrc[t] = first ? traced_row(pos, sp, k0, w) : pos;
     }
     __syncthreads();
-    for (int e = tid; e < w * cols; e += 256) {
-        const int r = e % w, jc = e / w;
-        X[r + jc * kBlk] = at(psrc[r], j0 + jc);
-        if (opos[r] >= 0) O[r + jc * kBlk] = at(osrc[r], j0 + jc);
-    }
+    copy_batched<256, T>(w * cols, [&](int e) { return at(psrc[e % w], j0 + e / w); },
+                         [&](int e, T v) { X[e % w + (e / w) * kBlk] = v; });
+    copy_batched<256, T>(w * cols, [&](int e) { return opos[e % w] >= 0 ? at(osrc[e % w], j0 + e / w) : cx<T>(0.0, 0.0); },
+                         [&](int e, T v) {
+                             if (opos[e % w] >= 0) O[e % w + (e / w) * kBlk] = v;
+                         });
     __syncthreads();
     for (int e = tid; e < w * cols; e += 256) {
         const int r = e % w, jc = e / w;
@@ -1878,7 +1928,8 @@ __global__ void __launch_bounds__(256)
     for (int s = 0; s < 8; ++s) x[s] = X[min(g + 8 * s, w - 1) + col * kBlk];
     __syncthreads();
     T* A = X;  // L_kk, leading dimension la
-    for (int e = tid; e < w * w; e += 256) A[(e % w) + (e / w) * la] = at(k0 + e % w, k0 + e / w);
+    copy_batched<256, T>(w * w, [&](int e) { return at(k0 + e % w, k0 + e / w); },
+                         [&](int e, T v) { A[(e % w) + (e / w) * la] = v; });
     __syncthreads();
 #pragma unroll
     for (int kb = 0; kb < 8; ++kb)
