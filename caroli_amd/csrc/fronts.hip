@@ -18,6 +18,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <cstdlib>
 #include <cstring>
 #include <mutex>
@@ -426,52 +427,101 @@ struct Mfma<float2> {
 
 
 
-// Partial LU in LDS of the n x n matrix W (leading dimension ld) over its first ne columns, row
-// pivoting restricted to the separator rows and the pivot test of field.rs:61 on columns
-// [0, ne1) (later columns only for zero). G lanes share a row: lane t owns physical row t / G and
-// the columns j = t (mod G); lg is the row's logical position on entry (its physical row) and on
-// return. Interchanges only swap logical positions; LAPACK's pivots go to piv. Returns false (and
+// Readlane of a double, a float or a complex value (w uniform over the wave).
+__device__ __forceinline__ double readlane(double x, int w) {
+    return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(x), w),
+                            __builtin_amdgcn_readlane(__double2loint(x), w));
+}
+__device__ __forceinline__ float readlane(float x, int w) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), w));
+}
+template <class T>
+__device__ __forceinline__ T readlane(T x, int w) {
+    T r;
+    r.x = readlane(x.x, w);
+    r.y = readlane(x.y, w);
+    return r;
+}
+
+// Pivot search of the LDS eliminations at step k, by each wave on its own (no LDS exchange, no
+// barrier): lane t takes the separator rows t, t + 64, ... (physical) whose logical position
+// pos[t] lies in [k, lim); the largest squared modulus of colk[t] wins, the smallest logical
+// position on exact ties. Every lane receives the winner's squared modulus (big; -1 when no row
+// is left), logical position (at), physical row (ph) and the reciprocal of its entry (inv; each
+// candidate forms its own before the reduction, off the chain of dependent steps). The values
+// alone are reduced (DPP inside rows of 16 lanes, then across rows); a ballot finds the lane.
+template <class T>
+__device__ __forceinline__ void lds_pivot(const T* colk, const int32_t* pos, int ne, int k, int lim, double& big,
+                                          int& at, int& ph, T& inv) {
+    const int lane = threadIdx.x & 63;
+    double v = -1.0;
+    int lv = kLdsMaxN, pv = 0;
+    T c = cx<T>(1.0, 0.0);
+    for (int t = lane; t < ne; t += 64) {
+        const int lp = pos[t];
+        const T x = colk[t];
+        const double a = cabs2(x);
+        if (lp >= k && lp < lim && (a > v || (a == v && lp < lv))) {
+            v = a;
+            lv = lp;
+            pv = t;
+            c = x;
+        }
+    }
+    const T cinv = crecip(c);
+    double m = v;
+    m = fmax(m, dpp<kDppXor1>(m));
+    m = fmax(m, dpp<kDppXor2>(m));
+    m = fmax(m, dpp<kDppHalfMirror>(m));
+    m = fmax(m, dpp<kDppMirror>(m));
+    m = fmax(m, __shfl_xor(m, 16, 64));
+    m = fmax(m, __shfl_xor(m, 32, 64));
+    const bool top = v == m;
+    unsigned long long mask = __ballot(top);
+    if (__popcll(mask) > 1) {  // an exact tie (or no row left): the smallest logical position
+        double dummy = 0.0;
+        int key = top ? lv : INT_MAX;
+        wave_argmax_key(dummy, key);  // equal values: the smallest key
+        mask = __ballot(top && lv == key);
+    }
+    const int w = __ffsll(mask) - 1;
+    big = m;
+    at = __builtin_amdgcn_readlane(lv, w);
+    ph = __builtin_amdgcn_readlane(pv, w);
+    inv = readlane(cinv, w);
+}
+
+// Partial LU in LDS of the n rows of W (leading dimension ld; columns [0, ncol) updated) over its
+// first ne columns, row pivoting restricted to the separator rows (the first ne) and the pivot
+// test of field.rs:61 on columns [0, ne1) (later columns only for zero). G lanes share a row:
+// lane t owns physical row t / G and the columns j = t (mod G); lg is the row's logical position
+// on entry (its physical row) and on return. Interchanges only swap logical positions; LAPACK's pivots go to piv. Returns false (and
 // sets the status) when a pivot fails; the whole workgroup returns alike.
+// One barrier a step: every wave searches the pivot itself (lds_pivot) over the logical positions
+// of the previous step (pos, double-buffered by step parity), and a row's multiplier goes to
+// column k one step late, so that no wave's search of column k sees it change.
 template <class T, int NT, int G>
-__device__ __forceinline__ bool lds_lu(T* W, int ld, int n, int ne, int ne1, double tol, int32_t* piv,
+__device__ __forceinline__ bool lds_lu(T* W, int ld, int n, int ncol, int ne, int ne1, double tol, int32_t* piv,
                                        int32_t* status, int& lg) {
     __shared__ double sv[NT / 64];
-    __shared__ double rv[2][NT / 64];
-    __shared__ int32_t ri[2][NT / 64], rp[2][NT / 64];
-    const int tid = threadIdx.x, lane = tid & 63, g = tid % G, r = tid / G;
+    __shared__ int32_t pos[2][kLdsMaxN];
+    const int tid = threadIdx.x, g = tid % G, r = tid / G;
     double m = 0.0;
     for (int idx = tid; idx < ne1 * ne1; idx += NT) m = fmax(m, cabs(W[(idx % ne1) + (idx / ne1) * ld]));
-    const double scale = block_max(m, sv);
+    for (int t = tid; t < ne; t += NT) pos[0][t] = t;
+    const double scale = block_max(m, sv);  // its barriers publish pos[0] too
+    T lk = cx<T>(0.0, 0.0);                  // this row's multiplier of the previous step (lane g = 0)
+    bool has = false;
     for (int k = 0; k < ne; ++k) {
-        const int lim = k < ne1 ? ne1 : ne;
-        double big = -1.0;
-        int at = kLdsMaxN, ph = 0;  // pivot: value, logical row, physical row
-        if (g == 0 && r < n && lg >= k && lg < lim) {
-            big = cabs2(W[r + k * ld]);
-            at = lg;
-            ph = r;
-        }
-        wave_argmax(big, at, ph);
-        if constexpr (NT > 64) {  // combine the waves (double-buffered by step parity)
-            const int par = k & 1;
-            if (lane == 0) {
-                rv[par][tid >> 6] = big;
-                ri[par][tid >> 6] = at;
-                rp[par][tid >> 6] = ph;
-            }
-            __syncthreads();
-            for (int w = 0; w < NT / 64; ++w) {
-                const double ov = rv[par][w];
-                const int oi = ri[par][w];
-                if (ov > big || (ov == big && oi < at)) {
-                    big = ov;
-                    at = oi;
-                    ph = rp[par][w];
-                }
-            }
-        }
-        const bool ok = pivot_ok(big, k < ne1, tol, scale);
-        if (!ok) {
+        const int lim = k < ne1 ? ne1 : ne, par = k & 1;
+        if (has) W[r + (k - 1) * ld] = lk;
+        has = false;
+        const T own = W[min(r, n - 1) + k * ld];
+        double big;
+        int at, ph;
+        T inv;
+        lds_pivot(W + k * ld, pos[par], ne, k, lim, big, at, ph, inv);
+        if (!pivot_ok(big, k < ne1, tol, scale)) {
             if (tid == 0) *status = k < ne1 ? kFrontDelayed : kFrontSingular;
             return false;
         }
@@ -480,15 +530,20 @@ __device__ __forceinline__ bool lds_lu(T* W, int ld, int n, int ne, int ne1, dou
             lg = k;
         else if (lg == k)
             lg = at;
+        if (g == 0 && r < ne) pos[par ^ 1][r] = lg;
         if (r < n && lg > k) {
-            // The G lanes of this row read W[r, k] before lane g = 0 overwrites it (one wavefront).
-            const T l = cmul(W[r + k * ld], crecip(W[ph + k * ld]));
+            const T l = cmul(own, inv);
             const int j0 = k + 1 + ((g - (k + 1)) % G + G) % G;
-            for (int j = j0; j < n; j += G) W[r + j * ld] = csub(W[r + j * ld], cmul(l, W[ph + j * ld]));
-            if (g == 0) W[r + k * ld] = l;
+            for (int j = j0; j < ncol; j += G) W[r + j * ld] = csub(W[r + j * ld], cmul(l, W[ph + j * ld]));
+            if (g == 0) {
+                lk = l;
+                has = true;
+            }
         }
         __syncthreads();
     }
+    if (has) W[r + (ne - 1) * ld] = lk;
+    __syncthreads();
     if (tid == 0) *status = kFrontOk;
     return true;
 }
@@ -545,7 +600,7 @@ __global__ void __launch_bounds__(NT)
     }
     const int g = tid % G, r = tid / G;  // this lane's column class and physical row
     int lg = r;                           // the row's logical position
-    if (!lds_lu<T, NT, G>(W, ld, n, ne, ne1, tol, piv, status, lg)) return;
+    if (!lds_lu<T, NT, G>(W, ld, n, n, ne, ne1, tol, piv, status, lg)) return;
     // Row r goes to logical row lg: lanes of one wave cover G columns of 64 / G rows.
     if (r < n) {
         if (pv.keep) {
@@ -578,9 +633,6 @@ __global__ void __launch_bounds__(NT)
                         const KidDesc* __restrict__ kids, const int32_t* __restrict__ maps, PointsView pv,
                         double tol) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
-    __shared__ double sv[NT / 64];
-    __shared__ double rv[2][NT / 64];
-    __shared__ int32_t ri[2][NT / 64], rp[2][NT / 64];
     __shared__ int32_t phys[kLdsMaxN];  // physical row of logical separator row k
     const NodeDesc nd = nodes[level[blockIdx.x]];
     const int64_t p = blockIdx.y;
@@ -642,63 +694,10 @@ __global__ void __launch_bounds__(NT)
                             [&](int idx, T v) { Wr[(idx % ne) + (idx / ne) * ldr] = v; });
         __syncthreads();
     }
-    // 2. Partial LU of the border (lds_lu on the split layout).
+    // 2. Partial LU of the border: lds_lu on Wl (every row, the separator columns only).
     const int g = tid % G, r = tid / G;
     int lg = r;
-    {
-        double mx = 0.0;
-        for (int idx = tid; idx < ne1 * ne1; idx += NT) mx = fmax(mx, cabs(Wl[(idx % ne1) + (idx / ne1) * ldl]));
-        const double scale = block_max(mx, sv);
-        for (int k = 0; k < ne; ++k) {
-            const int lim = k < ne1 ? ne1 : ne;
-            double big = -1.0;
-            int pat = kLdsMaxN, ph = 0;  // pivot: value, logical row, physical row
-            if (g == 0 && r < n && lg >= k && lg < lim) {
-                big = cabs2(Wl[r + k * ldl]);
-                pat = lg;
-                ph = r;
-            }
-            wave_argmax(big, pat, ph);
-            if constexpr (NT > 64) {
-                const int par = k & 1;
-                if (lane == 0) {
-                    rv[par][tid >> 6] = big;
-                    ri[par][tid >> 6] = pat;
-                    rp[par][tid >> 6] = ph;
-                }
-                __syncthreads();
-                for (int w = 0; w < NT / 64; ++w) {
-                    const double ov = rv[par][w];
-                    const int oi = ri[par][w];
-                    if (ov > big || (ov == big && oi < pat)) {
-                        big = ov;
-                        pat = oi;
-                        ph = rp[par][w];
-                    }
-                }
-            }
-            const bool ok = pivot_ok(big, k < ne1, tol, scale);
-            if (!ok) {
-                if (tid == 0) *status = k < ne1 ? kFrontDelayed : kFrontSingular;
-                return;
-            }
-            if (tid == 0) piv[k] = pat;
-            if (r == ph)
-                lg = k;
-            else if (lg == k)
-                lg = pat;
-            if (r < n && lg > k) {
-                // The G lanes of this row read W[r, k] before lane g = 0 overwrites it (one wavefront).
-                // The separator columns only: the boundary columns of the separator rows (U12)
-                // follow in one triangular solve after the loop.
-                const T l = cmul(Wl[r + k * ldl], crecip(Wl[ph + k * ldl]));
-                const int j0 = k + 1 + ((g - (k + 1)) % G + G) % G;
-                for (int j = j0; j < ne; j += G) Wl[r + j * ldl] = csub(Wl[r + j * ldl], cmul(l, Wl[ph + j * ldl]));
-                if (g == 0) Wl[r + k * ldl] = l;
-            }
-            __syncthreads();
-        }
-    }
+    if (!lds_lu<T, NT, G>(Wl, ldl, n, ne, ne, ne1, tol, piv, status, lg)) return;
     if (g == 0 && r < ne) phys[lg] = r;  // separator rows keep logical positions below ne
     __syncthreads();
     // U12 = L11^-1 P F12, a boundary column per lane: the separator rows in logical order k at
@@ -710,7 +709,21 @@ __global__ void __launch_bounds__(NT)
         for (int k = 1; k < ne; ++k) {
             const int pk = phys[k];
             T acc = col[pk];
-            for (int m = 0; m < k; ++m) acc = csub(acc, cmul(Wl[pk + m * ldl], col[phys[m]]));
+            int m = 0;
+            for (; m + 4 <= k; m += 4) {  // four terms' loads in flight together, summed in order
+                int pm[4];
+                T lv[4], uv[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) pm[u] = phys[m + u];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    lv[u] = Wl[pk + (m + u) * ldl];
+                    uv[u] = col[pm[u]];
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u) acc = csub(acc, cmul(lv[u], uv[u]));
+            }
+            for (; m < k; ++m) acc = csub(acc, cmul(Wl[pk + m * ldl], col[phys[m]]));
             col[pk] = acc;
         }
     }
@@ -1321,7 +1334,7 @@ __global__ void __launch_bounds__(256)
     }
     const int g = tid % G, r = tid / G;
     int lg = r;
-    if (!lds_lu<T, 256, G>(W, ld, ne, ne, ne1, tol, piv, status, lg)) return;
+    if (!lds_lu<T, 256, G>(W, ld, ne, ne, ne, ne1, tol, piv, status, lg)) return;
     if (r < ne)
         for (int j = g; j < ne; j += G) F[lg + int64_t(j) * n] = W[r + j * ld];
 }
