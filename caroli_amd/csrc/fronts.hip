@@ -1459,6 +1459,7 @@ __global__ void __launch_bounds__(256)
 // in 16-32 bytes of scratch here; the form without it, zero written as a literal, schedules the K
 // loop worse: 10 % slower on the BHZ 200 x 64 batch.)
 constexpr int kChunk = 16, kTileLd = kTile + 1;
+constexpr int kChunk32 = 8, kSchurPad = 1;  // K chunk of the 32 x 32 Schur tiles (half the LDS of 16: more workgroups a CU); LDS row padding
 
 template <class T, bool kRoot, bool kGather, int TS>
 __global__ void __launch_bounds__(TS == 64 ? 256 : 64)
@@ -1466,9 +1467,9 @@ __global__ void __launch_bounds__(TS == 64 ? 256 : 64)
                  const int32_t* __restrict__ maps, PointsView pv) {
     static_assert(TS == 64 || TS == 32, "Schur tiles are 64 x 64 or 32 x 32");
     using M = Mfma<T>;
-    constexpr int NT = TS == 64 ? 256 : 64, kLd = TS + 1;
-    __shared__ T As[kChunk * kLd];  // As[k * kLd + i] = L21[i0 + i, kc + k]
-    __shared__ T Bs[kChunk * kLd];  // Bs[k * kLd + j] = U12[kc + k, j0 + j]
+    constexpr int NT = TS == 64 ? 256 : 64, kLd = TS + kSchurPad, kCh = TS == 64 ? kChunk : kChunk32;
+    __shared__ T As[kCh * kLd];  // As[k * kLd + i] = L21[i0 + i, kc + k]
+    __shared__ T Bs[kCh * kLd];  // Bs[k * kLd + j] = U12[kc + k, j0 + j]
     const TilePoint tp = xcd_tile();
     const Tile tl = tiles[tp.tile];
     const NodeDesc nd = nodes[tl.node];
@@ -1488,10 +1489,10 @@ __global__ void __launch_bounds__(TS == 64 ? 256 : 64)
             cr[x][y] = typename M::acc{0, 0, 0, 0};
             ci[x][y] = typename M::acc{0, 0, 0, 0};
         }
-    // The K loop in chunks of kChunk: the next chunk's L21 / U12 entries are loaded into registers
+    // The K loop in chunks of kCh: the next chunk's L21 / U12 entries are loaded into registers
     // (kStage per thread and operand) while the MFMAs run on the current one from LDS. Clamped
     // loads and a select of the value (a select of addresses would go through flat).
-    constexpr int kStage = TS * kChunk / NT;
+    constexpr int kStage = TS * kCh / NT;
     T sa[kStage], sb[kStage];
     auto fetch = [&](int kc) {
 #pragma unroll
@@ -1504,7 +1505,7 @@ __global__ void __launch_bounds__(TS == 64 ? 256 : 64)
                 sa[u] = v;
             }
             {
-                const int c = e % kChunk, r = e / kChunk, j = tl.j0 + r, k = kc + c;
+                const int c = e % kCh, r = e / kCh, j = tl.j0 + r, k = kc + c;
                 T v = F[min(k, ne - 1) + int64_t(ne + min(j, nb - 1)) * n];
                 if (!(j < nb && k < ne)) v = zero;
                 sb[u] = v;
@@ -1519,17 +1520,17 @@ __global__ void __launch_bounds__(TS == 64 ? 256 : 64)
 #pragma unroll
         for (int y = 0; y < 2; ++y) live[x][y] = tl.j0 + wj + 16 * x < nb && tl.i0 + wi + 16 * y < nb;
     fetch(0);
-    for (int kc = 0; kc < ne; kc += kChunk) {
+    for (int kc = 0; kc < ne; kc += kCh) {
 #pragma unroll
         for (int u = 0; u < kStage; ++u) {
             const int e = tid + u * NT;
             As[(e / TS) * kLd + e % TS] = sa[u];
-            Bs[(e % kChunk) * kLd + e / kChunk] = sb[u];
+            Bs[(e % kCh) * kLd + e / kCh] = sb[u];
         }
         __syncthreads();
-        if (kc + kChunk < ne) fetch(kc + kChunk);
+        if (kc + kCh < ne) fetch(kc + kCh);
 #pragma unroll
-        for (int kk = 0; kk < kChunk; kk += 4) {
+        for (int kk = 0; kk < kCh; kk += 4) {
             if (kc + kk >= ne) break;  // the rest of the chunk is padding (zeros)
             const int k = kk + lane / 16;
             T a[2], b[2];
