@@ -1013,6 +1013,19 @@ __device__ __forceinline__ void mfma_gemm_tile(T* F, int n, int i0, int j0, int 
     }
 }
 
+// The interchanges k0 + c <-> sp[c] (c < w, in order) as one permutation: new row r holds the old
+// row traced back through them.
+__device__ __forceinline__ int traced_row(int r, const int32_t* sp, int k0, int w) {
+    for (int c = w - 1; c >= 0; --c) {
+        const int a = k0 + c, b = sp[c];
+        if (r == a)
+            r = b;
+        else if (r == b)
+            r = a;
+    }
+    return r;
+}
+
 // The border factorisation of one front; kLds: the panel's separator rows are held in LDS (P).
 // mv: LDS table of the panel's row moves (count, then destination and source rows); buf: LDS
 // staging of the moved rows (2 kPanel x kMoveCols).
@@ -1074,33 +1087,29 @@ __device__ __forceinline__ void border_factor(T* F, T* P, T* buf, int32_t* spiv,
         // The reciprocals of the panel's pivots, for the L21 rows of step 3.
         if (tid < w) dinv[tid] = crecip(pp(k0 + tid, tid));
         // The interchanges k0 + c <-> spiv[c], in order, as one permutation: row mv[1 + m] of the
-        // front receives the old row mv[1 + 2 kPanel + m].
-        if (tid == 0) {  // rows touched in mv[1 + 2 kPanel + s] (positions) and mv[1 + s] (sources)
-            int32_t* pos = mv + 1 + 2 * kPanel;
-            int32_t* src = mv + 1;
-            int cnt = 0;
-            auto slot = [&](int r) {
-                for (int s = 0; s < cnt; ++s)
-                    if (pos[s] == r) return s;
-                pos[cnt] = r;
-                src[cnt] = r;
-                return cnt++;
-            };
-            for (int c = 0; c < w; ++c) {
-                const int a = slot(k0 + c), b = slot(spiv[c]);
-                const int x = src[a];
-                src[a] = src[b];
-                src[b] = x;
+        // front receives the old row mv[1 + 2 kPanel + m]. Wave 0 finds the rows that change, a
+        // lane each: lane t < w the panel row k0 + t, lane w + c the row spiv[c] below the panel
+        // (at its first occurrence), each traced back through the interchanges; a ballot numbers
+        // the moves in lane order.
+        if (wave == 0) {
+            int dst = -1;
+            if (lane < w) {
+                dst = k0 + lane;
+            } else if (lane < 2 * w) {
+                const int c = lane - w, b = spiv[c];
+                bool first = b >= k1;
+                for (int c2 = 0; c2 < c; ++c2) first = first && spiv[c2] != b;
+                if (first) dst = b;
             }
-            int m = 0;  // compact to the rows that change: mv[1 + m] <- mv[1 + 2 kPanel + m]
-            for (int s = 0; s < cnt; ++s)
-                if (pos[s] != src[s]) {
-                    const int ps = pos[s], sr = src[s];
-                    mv[1 + m] = ps;
-                    mv[1 + 2 * kPanel + m] = sr;
-                    ++m;
-                }
-            mv[0] = m;
+            const int src = dst >= 0 ? traced_row(dst, spiv, k0, w) : dst;
+            const bool move = src != dst;
+            const unsigned long long mask = __ballot(move);
+            if (move) {
+                const int m = __popcll(mask & ((1ull << lane) - 1));
+                mv[1 + m] = dst;
+                mv[1 + 2 * kPanel + m] = src;
+            }
+            if (lane == 0) mv[0] = __popcll(mask);
         }
         __syncthreads();
         // 3. The moves on the columns outside the panel (separator rows only), the panel back in
@@ -1111,19 +1120,8 @@ __device__ __forceinline__ void border_factor(T* F, T* P, T* buf, int32_t* spiv,
         for (int j0 = 0; moves > 0 && j0 < n - w; j0 += kMoveCols) {
             const int cols = min(kMoveCols, n - w - j0), pairs = moves * cols;
             auto col = [&](int jc) { return j0 + jc < k0 ? j0 + jc : j0 + jc + w; };
-            for (int e0 = tid; e0 < pairs; e0 += NT * 4) {
-                T v[4];
-#pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    const int e = min(e0 + u * NT, pairs - 1), m = e / cols, jc = e % cols;
-                    v[u] = at(mv[1 + 2 * kPanel + m], col(jc));
-                }
-#pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    const int e = e0 + u * NT;
-                    if (e < pairs) buf[(e / cols) * kMoveCols + e % cols] = v[u];
-                }
-            }
+            copy_batched<NT, T>(pairs, [&](int e) { return at(mv[1 + 2 * kPanel + e / cols], col(e % cols)); },
+                                [&](int e, T v) { buf[(e / cols) * kMoveCols + e % cols] = v; });
             __syncthreads();
             for (int e = tid; e < pairs; e += NT) {
                 const int m = e / cols, jc = e % cols;
@@ -1842,18 +1840,6 @@ __global__ void __launch_bounds__(kPanelThreads)
     }
 }
 
-// The interchanges k0 + c <-> sp[c] (c < w, in order) as one permutation: new row r holds the old
-// row traced back through them.
-__device__ __forceinline__ int traced_row(int r, const int32_t* sp, int k0, int w) {
-    for (int c = w - 1; c >= 0; --c) {
-        const int a = k0 + c, b = sp[c];
-        if (r == a)
-            r = b;
-        else if (r == b)
-            r = a;
-    }
-    return r;
-}
 
 template <class T, bool kRoot>
 __global__ void __launch_bounds__(256)
