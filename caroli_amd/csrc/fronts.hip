@@ -1027,10 +1027,9 @@ __device__ __forceinline__ int traced_row(int r, const int32_t* sp, int k0, int 
 }
 
 // The border factorisation of one front; kLds: the panel's separator rows are held in LDS (P).
-// mv: LDS table of the panel's row moves (count, then destination and source rows); buf: LDS
-// staging of the moved rows (2 kPanel x kMoveCols).
+// mv: LDS table of the panel's row moves (count, then destination and source rows).
 template <class T, bool kRoot, bool kLds, int NT>
-__device__ __forceinline__ void border_factor(T* F, T* P, T* buf, int32_t* spiv, int32_t* mv, int n, int ne, int ne1,
+__device__ __forceinline__ void border_factor(T* F, T* P, int32_t* spiv, int32_t* mv, int n, int ne, int ne1,
                                               int32_t* piv, int32_t* status, double scale, double tol) {
     __shared__ T dinv[kPanel];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -1114,21 +1113,29 @@ __device__ __forceinline__ void border_factor(T* F, T* P, T* buf, int32_t* spiv,
         __syncthreads();
         // 3. The moves on the columns outside the panel (separator rows only), the panel back in
         //    place, and the L21 rows: l U11 = f with U11 the panel's upper triangle.
-        //    The moved rows are staged through LDS (buf) kMoveCols columns at a time: all loads of a
-        //    chunk land before any store.
+        //    All moves of one column are done by the lanes of one wave in one load and one store
+        //    instruction (lane = column slot x move, the moves padded to a power of two), so every
+        //    source entry of the column is read before any of them is written; kBatch columns of a
+        //    lane are in flight together (different columns: no overlap).
         const int moves = mv[0];
-        for (int j0 = 0; moves > 0 && j0 < n - w; j0 += kMoveCols) {
-            const int cols = min(kMoveCols, n - w - j0), pairs = moves * cols;
-            auto col = [&](int jc) { return j0 + jc < k0 ? j0 + jc : j0 + jc + w; };
-            copy_batched<NT, T>(pairs, [&](int e) { return at(mv[1 + 2 * kPanel + e / cols], col(e % cols)); },
-                                [&](int e, T v) { buf[(e / cols) * kMoveCols + e % cols] = v; });
-            __syncthreads();
-            for (int e = tid; e < pairs; e += NT) {
-                const int m = e / cols, jc = e % cols;
-                at(mv[1 + m], col(jc)) = buf[m * kMoveCols + jc];
+        if (moves > 0) {
+            int pm = 1;
+            while (pm < moves) pm <<= 1;  // at most 2 kPanel = 32
+            const int per = 64 / pm, mi = lane % pm, slot = lane / pm, cols = n - w;
+            const bool on = mi < moves;
+            const int dst = on ? mv[1 + mi] : 0, src = on ? mv[1 + 2 * kPanel + mi] : 0;
+            auto col = [&](int jc) { return jc < k0 ? jc : jc + w; };
+            const int stride = (NT / 64) * per;
+            for (int base = wave * per + slot; base < cols; base += stride * kBatch) {
+                T v[kBatch];
+#pragma unroll
+                for (int u = 0; u < kBatch; ++u) v[u] = at(src, col(min(base + u * stride, cols - 1)));
+#pragma unroll
+                for (int u = 0; u < kBatch; ++u)
+                    if (on && base + u * stride < cols) at(dst, col(base + u * stride)) = v[u];
             }
-            __syncthreads();
         }
+        __syncthreads();
         if (kLds)
             for (int e = tid; e < R * w; e += NT) at(k0 + e % R, k0 + e / R) = P[e];
         for (int i = ne + tid; i < n; i += NT) {
@@ -1217,12 +1224,11 @@ __global__ void __launch_bounds__(NT)
     for (int idx = threadIdx.x; idx < ne1 * ne1; idx += NT)
         m = fmax(m, cabs(F[(idx % ne1) + int64_t(idx / ne1) * n]));
     const double scale = block_max(m, sv);
-    T* buf = reinterpret_cast<T*>(lds_raw);  // the move buffer first, then the panel
     if (ne <= lds_rows)
-        border_factor<T, kRoot, true, NT>(F, buf + 2 * kPanel * kMoveCols, buf, spiv, mv, n, ne, ne1, piv, status, scale,
-                                      tol);
+        border_factor<T, kRoot, true, NT>(F, reinterpret_cast<T*>(lds_raw), spiv, mv, n, ne, ne1, piv, status, scale,
+                                          tol);
     else
-        border_factor<T, kRoot, false, NT>(F, nullptr, buf, spiv, mv, n, ne, ne1, piv, status, scale, tol);
+        border_factor<T, kRoot, false, NT>(F, nullptr, spiv, mv, n, ne, ne1, piv, status, scale, tol);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -2593,7 +2599,7 @@ void launch_border(const NodeDesc* nodes, const int32_t* level, int32_t count, i
                    hipStream_t st) {
     if (count <= 0 || points <= 0) return;
     const int32_t rows = border_lds_rows(max_ne, single);
-    const size_t lds = size_t(rows * kPanel + 2 * kPanel * kMoveCols) * (single ? sizeof(float2) : sizeof(double2));
+    const size_t lds = size_t(rows * kPanel) * (single ? sizeof(float2) : sizeof(double2));
     const dim3 g = grid2(count, points);
     auto go = [&](auto kernel, int nt) {
         allow_lds(reinterpret_cast<const void*>(kernel));

@@ -106,7 +106,6 @@ constexpr int kLdsMaxN = 128;
 /// Panel width of the border factorisation, LDS budget of its panel, Schur tile size.
 constexpr int kPanel = 16;
 constexpr int kBorderLdsBytes = 64 * 1024;
-constexpr int kMoveCols = 64;  // columns per chunk of the border's row moves
 /// Separators up to kF11Lds variables are factored in LDS (f11_kernel) and their border strips
 /// solved kTrsmSpan columns / rows per workgroup (trsm_kernel).
 constexpr int kF11Lds = 64;
