@@ -443,6 +443,35 @@ __device__ __forceinline__ T readlane(T x, int w) {
     return r;
 }
 
+// The largest v over the wave and, among the lanes holding it, the smallest key, with that lane's
+// payload and value; every lane receives them (w uniform). The values alone are reduced (DPP
+// inside rows of 16 lanes, then across rows); a ballot finds the lane, a second reduction of the
+// keys runs only on an exact tie.
+template <class T>
+__device__ __forceinline__ void wave_pick(double v, int key, int pay, T val, double& big, int& kout, int& pout,
+                                          T& vout) {
+    double m = v;
+    m = fmax(m, dpp<kDppXor1>(m));
+    m = fmax(m, dpp<kDppXor2>(m));
+    m = fmax(m, dpp<kDppHalfMirror>(m));
+    m = fmax(m, dpp<kDppMirror>(m));
+    m = fmax(m, __shfl_xor(m, 16, 64));
+    m = fmax(m, __shfl_xor(m, 32, 64));
+    const bool top = v == m;
+    unsigned long long mask = __ballot(top);
+    if (__popcll(mask) > 1) {
+        double dummy = 0.0;
+        int k = top ? key : INT_MAX;
+        wave_argmax_key(dummy, k);  // equal values: the smallest key
+        mask = __ballot(top && key == k);
+    }
+    const int w = __ffsll(mask) - 1;
+    big = m;
+    kout = __builtin_amdgcn_readlane(key, w);
+    pout = __builtin_amdgcn_readlane(pay, w);
+    vout = readlane(val, w);
+}
+
 // Pivot search of the LDS eliminations at step k, by each wave on its own (no LDS exchange, no
 // barrier): lane t takes the separator rows t, t + 64, ... (physical) whose logical position
 // pos[t] lies in [k, lim); the largest squared modulus of colk[t] wins, the smallest logical
@@ -468,27 +497,7 @@ __device__ __forceinline__ void lds_pivot(const T* colk, const int32_t* pos, int
             c = x;
         }
     }
-    const T cinv = crecip(c);
-    double m = v;
-    m = fmax(m, dpp<kDppXor1>(m));
-    m = fmax(m, dpp<kDppXor2>(m));
-    m = fmax(m, dpp<kDppHalfMirror>(m));
-    m = fmax(m, dpp<kDppMirror>(m));
-    m = fmax(m, __shfl_xor(m, 16, 64));
-    m = fmax(m, __shfl_xor(m, 32, 64));
-    const bool top = v == m;
-    unsigned long long mask = __ballot(top);
-    if (__popcll(mask) > 1) {  // an exact tie (or no row left): the smallest logical position
-        double dummy = 0.0;
-        int key = top ? lv : INT_MAX;
-        wave_argmax_key(dummy, key);  // equal values: the smallest key
-        mask = __ballot(top && lv == key);
-    }
-    const int w = __ffsll(mask) - 1;
-    big = m;
-    at = __builtin_amdgcn_readlane(lv, w);
-    ph = __builtin_amdgcn_readlane(pv, w);
-    inv = readlane(cinv, w);
+    wave_pick(v, lv, pv, crecip(c), big, at, ph, inv);
 }
 
 // Partial LU in LDS of the n rows of W (leading dimension ld; columns [0, ncol) updated) over its
@@ -1050,16 +1059,21 @@ __device__ __forceinline__ void border_factor(T* F, T* P, int32_t* spiv, int32_t
         // 2. Panel factorisation on the separator rows.
         for (int c = 0; c < w; ++c) {
             const int k = k0 + c, lim = k < ne1 ? ne1 : ne;
-            double big = -1.0;
-            int q = k;
+            // Every wave finds the same pivot; the rows k and q are swapped only after every wave
+            // has read column c (a barrier), so that no wave sees them half swapped.
+            double v = -1.0;
+            int qi = k;
             for (int i = k + lane; i < lim; i += 64) {
-                const double v = cabs2(pp(i, c));
-                if (v > big) {
-                    big = v;
-                    q = i;
+                const double a = cabs2(pp(i, c));
+                if (a > v) {
+                    v = a;
+                    qi = i;
                 }
             }
-            wave_argmax(big, q);  // every wave finds the same pivot
+            double big;
+            int q, unused;
+            T unused_v;
+            wave_pick(v, qi, 0, cx<T>(0.0, 0.0), big, q, unused, unused_v);
             const bool ok = pivot_ok(big, k < ne1, tol, scale);
             if (!ok) {
                 if (tid == 0) *status = (k < ne1 && !kRoot) ? kFrontDelayed : kFrontSingular;
@@ -1069,6 +1083,7 @@ __device__ __forceinline__ void border_factor(T* F, T* P, int32_t* spiv, int32_t
                 piv[k] = q;
                 spiv[c] = q;
             }
+            __syncthreads();
             if (q != k && tid < w) {
                 T x = pp(k, tid);
                 pp(k, tid) = pp(q, tid);
