@@ -94,7 +94,6 @@ __device__ __forceinline__ void wave_argmax(double& v, int& idx, int& pay) {
     idx = key >> 16;
     pay = key & 0xFFFF;
 }
-__device__ __forceinline__ void wave_argmax(double& v, int& idx) { wave_argmax_key(v, idx); }
 
 // Block-wide max of (value, -index): the largest value, the smallest index on ties, in every
 // lane. sv / si hold one entry per wave; the caller separates consecutive calls by a barrier.
@@ -648,7 +647,7 @@ __global__ void __launch_bounds__(NT)
     T* F = reinterpret_cast<T*>(pv.arena) + p * pv.arena_stride + nd.off;
     int32_t* piv = pv.piv + p * pv.piv_stride + nd.piv_off;
     int32_t* status = pv.status + p * pv.status_stride + nd.slot;
-    const int n = nd.n, ne = nd.ne, ne1 = nd.ne1, nb = n - ne, tid = threadIdx.x, lane = tid & 63;
+    const int n = nd.n, ne = nd.ne, ne1 = nd.ne1, nb = n - ne, tid = threadIdx.x;
     const int ldl = n + 1, ldr = ne + 1;
     T* Wl = reinterpret_cast<T*>(lds_raw);  // Wl[r + j ldl], r < n, j < ne
     T* Wr = Wl + ldl * ne;                   // Wr[r + (j - ne) ldr], r < ne, ne <= j < n
