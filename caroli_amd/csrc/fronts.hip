@@ -1315,8 +1315,8 @@ __global__ void __launch_bounds__(256)
 //    F11 in LDS (lds_lu) written back with its pivots;
 //  - trsm_kernel, one workgroup per kTrsmSpan columns of F12 (U12 = L11^-1 P F12) or rows of F21
 //    (L21 = F21 U11^-1): the triangle in LDS, each column / row in the registers of 4 lanes.
-template <class T, int G>
-__global__ void __launch_bounds__(256)
+template <class T, int NT, int G>
+__global__ void __launch_bounds__(NT)
     f11_kernel(const NodeDesc* __restrict__ nodes, const int32_t* __restrict__ level, const Entry* __restrict__ ent,
                const int32_t* __restrict__ diag, PointsView pv, double tol) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
@@ -1329,11 +1329,11 @@ __global__ void __launch_bounds__(256)
     const int n = nd.n, ne = nd.ne, ne1 = nd.ne1, tid = threadIdx.x, ld = ne + 1;
     // F11 (the gathered children's part) into LDS, then -H and z added there; the entries in the
     // border strips F12 / F21 are added in place in HBM (trsm_kernel reads them next).
-    copy_batched<256, T>(ne * ne, [&](int idx) { return F[(idx % ne) + int64_t(idx / ne) * n]; },
+    copy_batched<NT, T>(ne * ne, [&](int idx) { return F[(idx % ne) + int64_t(idx / ne) * n]; },
                          [&](int idx, T v) { W[(idx % ne) + (idx / ne) * ld] = v; });
     __syncthreads();
     if (nd.ent_end > nd.ent_beg) {
-        add_entries<256>(ent, nd.ent_beg, nd.ent_end, pv.vals[p], n, [&](int at, double2 v) {
+        add_entries<NT>(ent, nd.ent_beg, nd.ent_end, pv.vals[p], n, [&](int at, double2 v) {
             const int a = at % n, b = at / n;
             if (a < ne && b < ne)
                 W[a + b * ld] = cadd(W[a + b * ld], convert<T>(v));
@@ -1344,7 +1344,7 @@ __global__ void __launch_bounds__(256)
     }
     if (nd.diag_end > nd.diag_beg) {  // the eliminated orbitals: all inside F11
         const T z = convert<T>(pv.z[p]);
-        for (int d = nd.diag_beg + tid; d < nd.diag_end; d += 256) {
+        for (int d = nd.diag_beg + tid; d < nd.diag_end; d += NT) {
             const int q = diag[d];
             W[q + q * ld] = cadd(W[q + q * ld], z);
         }
@@ -1352,7 +1352,7 @@ __global__ void __launch_bounds__(256)
     }
     const int g = tid % G, r = tid / G;
     int lg = r;
-    if (!lds_lu<T, 256, G>(W, ld, ne, ne, ne, ne1, tol, piv, status, lg)) return;
+    if (!lds_lu<T, NT, G>(W, ld, ne, ne, ne, ne1, tol, piv, status, lg)) return;
     if (r < ne)
         for (int j = g; j < ne; j += G) F[lg + int64_t(j) * n] = W[r + j * ld];
 }
@@ -2635,14 +2635,16 @@ void launch_f11(const NodeDesc* nodes, const int32_t* level, int32_t count, int3
     if (count <= 0 || points <= 0) return;
     if (max_ne > kLdsMaxN) throw std::invalid_argument("separator too large for f11_kernel");
     const size_t lds = size_t(max_ne) * size_t(max_ne + 1) * (single ? sizeof(float2) : sizeof(double2));
-    auto go = [&](auto kernel) {
+    auto go = [&](auto kernel, int nt) {
         allow_lds(reinterpret_cast<const void*>(kernel));
-        hipLaunchKernelGGL(kernel, grid2(count, points), dim3(256), lds, st, nodes, level, ent, diag, pv, tol);
+        hipLaunchKernelGGL(kernel, grid2(count, points), dim3(nt), lds, st, nodes, level, ent, diag, pv, tol);
     };
+    // Complex f64 separators up to 64: two waves, two lanes a row; fewer waves shorten each
+    // elimination step (6.7 -> 6.2 ms a BHZ 200 x 64 step against four waves and four lanes a row).
     if (single)
-        max_ne <= 64 ? go(f11_kernel<float2, 4>) : go(f11_kernel<float2, 2>);
+        max_ne <= 64 ? go(f11_kernel<float2, 256, 4>, 256) : go(f11_kernel<float2, 256, 2>, 256);
     else
-        max_ne <= 64 ? go(f11_kernel<double2, 4>) : go(f11_kernel<double2, 2>);
+        max_ne <= 64 ? go(f11_kernel<double2, 128, 2>, 128) : go(f11_kernel<double2, 256, 2>, 256);
 }
 
 void launch_trsm(const NodeDesc* nodes, const Tile* tiles, int32_t count, int32_t max_ne, const PointsView& pv,
