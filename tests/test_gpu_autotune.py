@@ -49,7 +49,7 @@ def test_autotuned_batch_keeps_parity(restore_tuning):
     say("[gpu] autotune: BHZ 60x24 x 6 points + cubic 16x8x8 x 2 points, before tuning")
     t, st = batch.run()
     assert (st == 0).all(), st
-    assert np.abs(t - ref).max() <= TOL * (1 + np.abs(ref).max())
+    assert np.abs(t - ref).max() <= TOL
     say("[gpu] autotune: tuning")
     choice, sec = batch.autotune()
     say(f"    choice {choice}, {sec * 1e3:.2f} ms per sweep")
@@ -58,7 +58,7 @@ def test_autotuned_batch_keeps_parity(restore_tuning):
         say(f"[gpu] autotune: tuned batch, run {run}")
         t, st = batch.run()
         assert (st == 0).all(), st
-        assert np.abs(t - ref).max() <= TOL * (1 + np.abs(ref).max()), (run, np.abs(t - ref).max())
+        assert np.abs(t - ref).max() <= TOL, (run, np.abs(t - ref).max())
     # A hand-set choice that classifies the separators differently: the systems planned above are
     # re-planned at their next use.
     forced = {"small_fronts": 1, "split": 0, "blocked_min_ne": 64, "unblocked": 0,
@@ -69,7 +69,7 @@ def test_autotuned_batch_keeps_parity(restore_tuning):
     say(f"[gpu] autotune: forced {forced}")
     t2, st = ca.batch_transmission(pts_s, pts_g, ca.Precision.Double)
     assert (st == 0).all(), st
-    assert np.abs(t2 - ref).max() <= TOL * (1 + np.abs(ref).max()), np.abs(t2 - ref).max()
+    assert np.abs(t2 - ref).max() <= TOL, np.abs(t2 - ref).max()
 
 
 def test_autotune_mixed_precision_batch(restore_tuning):
@@ -115,7 +115,7 @@ def test_schur_tile_shapes_agree(precision, restore_tuning):
         say(f"[gpu] schur tiles {tile}: BHZ 60x24 + graphene 64x16, precision {precision}")
         t, st = ca.batch_transmission(pts_s, pts_g, precision)
         assert (st == 0).all(), st
-        bar = TOL * (1 + np.abs(ref).max()) if precision == ca.Precision.Double else 1e-8
+        bar = TOL if precision == ca.Precision.Double else 1e-8
         assert np.abs(t - ref).max() <= bar, (tile, np.abs(t - ref).max())
         out[tile] = t
     if precision == ca.Precision.Double:

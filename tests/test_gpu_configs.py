@@ -45,7 +45,7 @@ def test_graphene_512x64_double_and_mixed():
     say(f"[gpu] graphene 512x64: oracle T = {np.round(ref, 6)}")
     t, st = ca.batch_transmission(pts_s, pts_g, ca.Precision.Double)
     assert (st == 0).all(), st
-    assert np.abs(t - ref).max() <= 1e-10 * (1 + ref.max()), np.abs(t - ref).max()
+    assert np.abs(t - ref).max() <= 1e-10, np.abs(t - ref).max()
     stats = ca.capi.BatchStats()
     tm, st = ca.batch_transmission(pts_s, pts_g, ca.Precision.Mixed, stats)
     assert (st == 0).all(), st
@@ -134,7 +134,7 @@ def test_cubic_uneven_batch():
     say(f"[gpu] cubic uneven batch (L = 16, 24, 16): oracle T = {np.round(ref, 6)}")
     t, st = ca.batch_transmission(pts_s, pts_g, ca.Precision.Double)
     assert (st == 0).all(), st
-    assert np.abs(t - ref).max() <= 1e-10 * (1 + np.abs(ref).max()), np.abs(t - ref).max()
+    assert np.abs(t - ref).max() <= 1e-10, np.abs(t - ref).max()
     # Mixed: many channels (several column chunks per solve), separators of 256 and 576 variables.
     stats = ca.capi.BatchStats()
     tm, st = ca.batch_transmission(pts_s, pts_g, ca.Precision.Mixed, stats)
@@ -166,8 +166,8 @@ def test_uneven_resident_batch_with_kernel_timing():
     finally:
         ca.kernel_timing(False)
     assert (st == 0).all(), st
-    assert np.abs(t - ref).max() <= 1e-10 * (1 + np.abs(ref).max()), np.abs(t - ref).max()
+    assert np.abs(t - ref).max() <= 1e-10, np.abs(t - ref).max()
     say("[gpu] the same batch, units concurrent")
     t2, st = batch.run()
     assert (st == 0).all(), st
-    assert np.abs(t2 - ref).max() <= 1e-10 * (1 + np.abs(ref).max()), np.abs(t2 - ref).max()
+    assert np.abs(t2 - ref).max() <= 1e-10, np.abs(t2 - ref).max()

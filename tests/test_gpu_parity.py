@@ -82,7 +82,7 @@ def test_large_fronts_bhz_magnetic():
     say("[gpu] BHZ 48x32 magnetic, 2 configs x 3 energies")
     t, st = ca.batch_transmission(pts_s, pts_g, ca.Precision.Double)
     assert (st == 0).all()
-    assert np.abs(t - np.array(tref)).max() <= TOL * (1 + max(tref))
+    assert np.abs(t - np.array(tref)).max() <= TOL
 
 
 def test_reused_self_energies():
